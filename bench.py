@@ -1,0 +1,182 @@
+"""Benchmark: forward + backward of one 800x600 view of 300,000 surfels.
+
+Workload: surfels sampled on 8 random superquadrics (SH degree 3), one camera
+at radius 3, tile 16, upstream gradients N(0, 1) on all 10 output channels;
+a step is GaussianRasterizer.forward + backward through the drop-in API with
+all inputs resident on the GPU.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fwd+bwd ms/view (300k surfels, 800x600), %HBM peak; iters-to-loss vs plain"
+
+
+def algorithmic_bytes(P_vis, M, L, N, T):
+    """Compulsory HBM bytes per stage (each datum read or written once)."""
+    return {
+        "preprocess": (40 + 12 * M) * P_vis + 87 * P_vis,
+        "tile_scan": 8 * T,
+        "scatter": 20 * P_vis + 8 * L,
+        "tile_sort": 8 * L + 4 * L,
+        "blend_fwd": 8 * T + 76 * L + 60 * N,
+        "grad_clear": 0,
+        "blend_bwd": 8 * T + 76 * L + 60 * N + 76 * P_vis,
+        "preprocess_bwd": (151 + 12 * M) * P_vis + (48 + 12 * M) * P_vis,
+    }
+
+
+def pipeline_bytes(P, M, L, N, T):
+    """B_alg(view) = (430 + 36 M) P + 196 L + 120 N + 24 T (fwd + bwd)."""
+    return (430 + 36 * M) * P + 196 * L + 120 * N + 24 * T
+
+
+def cpu_baseline(scene, gc, go, budget_s=20.0):
+    import numpy as np  # noqa: F401
+
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_threads(threads)
+    cam = scene.camera
+    kw = dict(means3D=scene.means3D, opacities=scene.opacities, viewmatrix=cam.world_view_transform,
+              projmatrix=cam.full_proj_transform, campos=cam.camera_center, bg=scene.bg, W=cam.image_width,
+              H=cam.image_height, tanfovx=math.tan(cam.FoVx * 0.5), tanfovy=math.tan(cam.FoVy * 0.5),
+              shs=scene.shs, sh_degree=scene.sh_degree, scales=scene.scales, rotations=scene.rotations)
+    gcn, gon = gc.numpy(), go.numpy()
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 5 and (time.perf_counter() - t_start) < budget_s:
+        o = O.Oracle()
+        t0 = time.perf_counter()
+        o.forward(**kw)
+        o.backward(gcn, gon)
+        times.append((time.perf_counter() - t0) * 1e3)
+        print(f"[bench] cpu oracle fwd+bwd run {len(times)}: {times[-1]:.1f} ms", file=sys.stderr, flush=True)
+    return {"value": statistics.median(times), "unit": "ms/view", "cores": O.get_threads(), "kind": "port",
+            "sample": f"{len(times)} full fwd+bwd views of the same 300k-surfel 800x600 workload (median)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only (one view per step on one device); "
+                                   "multi-GPU sharding by view is described in DESIGN.md, not launched",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from partgs_amd import _lib
+    from partgs_amd.rasterizer import GaussianRasterizationSettings, GaussianRasterizer
+    from partgs_amd.scenes import superquadric_surfels, upstream_grads
+
+    dev = torch.device("cuda:0")
+    scene_cpu = superquadric_surfels(0, P=300_000, width=800, height=600, sh_degree=3)
+    scene = scene_cpu.to(dev)
+    cam = scene.camera
+    H, W = cam.image_height, cam.image_width
+    gc_cpu, go_cpu = upstream_grads(0, H, W)
+    gc, go = gc_cpu.to(dev), go_cpu.to(dev)
+    rs = GaussianRasterizationSettings(image_height=H, image_width=W, tanfovx=math.tan(cam.FoVx * 0.5),
+                                       tanfovy=math.tan(cam.FoVy * 0.5), bg=scene.bg, scale_modifier=1.0,
+                                       viewmatrix=cam.world_view_transform, projmatrix=cam.full_proj_transform,
+                                       sh_degree=scene.sh_degree, campos=cam.camera_center, prefiltered=False,
+                                       debug=False, tile=args.tile)
+    rast = GaussianRasterizer(rs)
+    leaves = [t.clone().requires_grad_(True) for t in (scene.means3D, scene.shs, scene.opacities, scene.scales,
+                                                       scene.rotations)]
+    means2D = torch.zeros_like(scene.means3D, requires_grad=True)
+    info = {}
+
+    def step():
+        for t in leaves + [means2D]:
+            t.grad = None
+        color, radii, allmap = rast(means3D=leaves[0], means2D=means2D, shs=leaves[1], opacities=leaves[2],
+                                    scales=leaves[3], rotations=leaves[4])
+        torch.autograd.backward((color, allmap), (gc, go))
+        info["radii"] = radii
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.reset_timing()
+    _lib.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.set_timing(False)
+    stages = _lib.get_timing()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # sizes for the byte model, from one more forward
+    from partgs_amd.rasterizer import rasterize_gaussians_fwd
+    e = torch.empty(0, device=dev)
+    L = rasterize_gaussians_fwd(scene.bg, scene.means3D, e, scene.opacities, scene.scales, scene.rotations, 1.0, e,
+                                cam.world_view_transform, cam.full_proj_transform, rs.tanfovx, rs.tanfovy, H, W,
+                                scene.shs, scene.sh_degree, cam.camera_center, False, False, args.tile)[0]
+    P = scene.P
+    P_vis = int((info["radii"] > 0).sum())
+    M = scene.shs.shape[1]
+    N = W * H
+    T = ((W + args.tile - 1) // args.tile) * ((H + args.tile - 1) // args.tile)
+    per_stage_ms = {k: v[0] / max(v[1], 1) for k, v in stages.items() if v[1] > 0}
+    kern_ms = sum(v[0] for v in stages.values()) / args.steps
+    dominant = max(per_stage_ms, key=lambda k: per_stage_ms[k])
+    alg = algorithmic_bytes(P_vis, M, L, N, T)
+    dom_bytes = alg[dominant]
+    achieved = dom_bytes / (per_stage_ms[dominant] * 1e-3) / 1e9
+    b_pipe = pipeline_bytes(P, M, L, N, T)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene_cpu, gc_cpu, go_cpu)
+    out = {
+        "metric": METRIC,
+        "value": round(ms_per_step, 4),
+        "unit": "ms/view",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (8 random superquadrics, seeded)",
+        "config": {"workload": "300k superquadric surfels, SH degree 3, 800x600, 1 view fwd+bwd", "P": P,
+                   "width": W, "height": H, "sh_degree": scene.sh_degree, "tile": args.tile,
+                   "num_rendered": int(L), "visible": P_vis},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "alg_bytes": int(dom_bytes), "avg_ms": round(per_stage_ms[dominant], 5)},
+        "pipeline": {"alg_bytes": int(b_pipe), "kernel_ms": round(kern_ms, 5),
+                     "hbm_frac": round(b_pipe / (kern_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 5),
+                     "stages_ms": {k: round(v, 5) for k, v in per_stage_ms.items()}},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(cpu["value"] / ms_per_step, 2)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,151 @@
+/*
+ * surfel_rast.h -- C ABI of the MI355X (gfx950) differentiable surfel
+ * rasteriser (2D Gaussian splatting) used by PartGS.
+ *
+ * Every pointer marked "device" is HIP device memory on the current device;
+ * "host" pointers are ordinary host memory.  All work is enqueued on the
+ * hipStream_t passed in (NULL = the legacy default stream); the only host
+ * synchronisation is the 4-byte read of the tile-instance count inside
+ * sr_forward, as in the reference.  Functions return SR_OK (0) or an SR_ERR_*
+ * code; sr_last_error() gives a message for the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree,
+ * dsr = submodules/diff-surfel-rasterization):
+ *   sr_forward        <- CudaRasterizer::Rasterizer::forward,
+ *                        dsr/cuda_rasterizer/rasterizer.h:31-57 and its torch
+ *                        glue RasterizeGaussiansCUDA, dsr/rasterize_points.cu:40-134
+ *   sr_backward       <- CudaRasterizer::Rasterizer::backward,
+ *                        dsr/cuda_rasterizer/rasterizer.h:59-86 and
+ *                        RasterizeGaussiansBackwardCUDA, dsr/rasterize_points.cu:137-233
+ *   sr_mark_visible   <- Rasterizer::markVisible, rasterizer.h:23-29 and
+ *                        markVisible, dsr/rasterize_points.cu:235-254
+ *   sr_*_bytes        <- required<GeometryState/ImageState/BinningState>,
+ *                        dsr/cuda_rasterizer/rasterizer_impl.h:67-73
+ */
+#ifndef SURFEL_RAST_H
+#define SURFEL_RAST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* sr_stream_t; /* == hipStream_t */
+
+enum {
+  SR_OK = 0,
+  SR_ERR_INVALID = 1,     /* bad argument (shape, NULL, unsupported tile) */
+  SR_ERR_HIP = 2,         /* a HIP runtime call failed */
+  SR_ERR_ALLOC = 3,       /* the allocation callback returned NULL */
+  SR_ERR_PREFILTERED = 4, /* prefiltered set but a surfel failed the near test
+                             (the reference traps, dsr/cuda_rasterizer/auxiliary.h:202-206) */
+  SR_ERR_CAPACITY = 5     /* more tile instances than the caller's capacity */
+};
+
+/* GaussianRasterizationSettings, dsr/diff_surfel_rasterization/__init__.py:158-170.
+ * tile is the screen tile edge: 16 reproduces the reference (config.h:15-17);
+ * 8 and 32 are also accepted. */
+typedef struct {
+  int image_height;
+  int image_width;
+  float tanfovx;
+  float tanfovy;
+  const float* bg;         /* device, 3 */
+  float scale_modifier;
+  const float* viewmatrix; /* device, 16, as the reference's world_view_transform */
+  const float* projmatrix; /* device, 16, full_proj_transform */
+  int sh_degree;
+  const float* campos;     /* device, 3 */
+  int prefiltered;
+  int debug;               /* synchronise and check after every stage */
+  int tile;
+} sr_settings;
+
+/* Per-surfel inputs (device).  Exactly one of shs / colors_precomp and exactly
+ * one of (scales and rotations) / transMat_precomp must be non-NULL, as
+ * checked by GaussianRasterizer.forward, __init__.py:192-196. */
+typedef struct {
+  int P;                         /* number of surfels */
+  int M;                         /* SH coefficients per surfel (shs is P x M x 3) */
+  const float* means3D;          /* P x 3 */
+  const float* shs;              /* P x M x 3, or NULL */
+  const float* colors_precomp;   /* P x 3, or NULL */
+  const float* opacities;        /* P */
+  const float* scales;           /* P x 2, or NULL */
+  const float* rotations;        /* P x 4 (w, x, y, z), or NULL */
+  const float* transMat_precomp; /* P x 9, or NULL */
+} sr_surfels;
+
+/* Gradient outputs (device), the tuple of rasterize_points.cu:232.  Shapes:
+ * dL_dmeans2D P x 3, dL_dcolors P x 3, dL_dopacity P, dL_dmeans3D P x 3,
+ * dL_dtransMat P x 9, dL_dsh P x M x 3 (may be NULL when M == 0),
+ * dL_dscales P x 2, dL_drotations P x 4.  sr_backward overwrites them all. */
+typedef struct {
+  float* dL_dmeans2D;
+  float* dL_dcolors;
+  float* dL_dopacity;
+  float* dL_dmeans3D;
+  float* dL_dtransMat;
+  float* dL_dsh;
+  float* dL_dscales;
+  float* dL_drotations;
+} sr_grads;
+
+/* Allocation callback: returns device memory of at least nbytes that stays
+ * valid until the matching sr_backward has returned, or NULL on failure.
+ * which = 0 geometry buffer, 1 binning buffer, 2 image buffer.  Mirrors the
+ * std::function<char*(size_t)> resize callbacks, rasterize_points.cu:31-37. */
+typedef void* (*sr_alloc_fn)(void* ctx, size_t nbytes, int which);
+
+/* Forward: preprocess, tile binning + per-tile depth sort, blend.
+ * out_color 3 x H x W, out_others 7 x H x W (depth, alpha, normal xyz,
+ * median depth, distortion; forward.cu:422-440), radii P (int32).
+ * *num_rendered receives the number of (surfel, tile) instances.
+ * The three buffers obtained from alloc are the state sr_backward needs. */
+int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, void* alloc_ctx,
+               float* out_color, float* out_others, int* radii, int* num_rendered,
+               sr_stream_t stream);
+
+/* Backward.  geom_buf / bin_buf / img_buf are the buffers sr_forward
+ * obtained (geom_buf is also used as scratch); R is its num_rendered;
+ * radii its radii output.  dL_dcolor is
+ * 3 x H x W, dL_dothers 7 x H x W. */
+int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* radii,
+                void* geom_buf, const void* bin_buf, const void* img_buf,
+                const float* dL_dcolor, const float* dL_dothers, sr_grads* grads,
+                sr_stream_t stream);
+
+/* present[i] = 1 when surfel i passes the near test (view z > 0.2). */
+int sr_mark_visible(int P, const float* means3D, const float* viewmatrix,
+                    const float* projmatrix, uint8_t* present, sr_stream_t stream);
+
+/* Byte sizes of the three state buffers (what sr_forward asks alloc for). */
+size_t sr_geom_bytes(int P);
+size_t sr_image_bytes(int W, int H, int tile);
+size_t sr_binning_bytes(int num_rendered);
+
+/* Byte offsets of the named arrays inside the state buffers, so callers and
+ * tests can read internal state (e.g. "depths", "rec", "tiles_touched",
+ * "clamped", "ranges", "n_contrib", "final_T", "point_list", "keys").
+ * Returns (size_t)-1 for unknown names. */
+size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int num_rendered);
+
+/* Per-stage GPU timing with HIP events on the caller's stream.
+ * Stages: 0 preprocess, 1 tile scan, 2 instance scatter, 3 tile sort,
+ * 4 forward blend, 5 gradient clear, 6 backward blend, 7 backward preprocess. */
+#define SR_NUM_STAGES 8
+int sr_set_timing(int enabled);
+/* Synchronises pending events; ms[i] = total milliseconds, count[i] = launches. */
+int sr_get_timing(double* ms, int* count, int n);
+void sr_reset_timing(void);
+
+const char* sr_last_error(void);
+const char* sr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SURFEL_RAST_H */

@@ -1,0 +1,1068 @@
+// CPU oracle for the 2DGS surfel rasteriser.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, on the host CPU, the
+// reference's per-surfel and per-pixel algorithm so that the HIP library can
+// be checked against it.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it; the product path never does.
+//
+// Each stage cites the reference file:line it follows (paths relative to the
+// reference tree, submodules/diff-surfel-rasterization/ abbreviated "dsr/").
+//
+// Two precisions from one template:
+//   * R = float  : "reference mode".  FP32 with the evaluation order pinned
+//     (built with -ffp-contract=off, left-to-right sums, no fused ops except
+//     the explicit fmaf of sro_expf).  The HIP kernels evaluate the same
+//     expressions in the same order, so integer outputs (radii, tile counts,
+//     keys, point_list, ranges) and the forward images agree bit for bit.
+//     Backward per-pair terms are FP32; their sums are accumulated in FP64.
+//   * R = double : "truth mode" for FP64 central differences.
+//
+// How each stage is pinned (tests/test_oracle_*.py):
+//   expf            vs FP64 libm exp, max error in ulps, whole domain used.
+//   preprocess      closed forms: fronto-parallel surfel centre/extent,
+//                   geometric ray/plane intersection through T, normal =
+//                   R e_z seen from the camera; SH basis orthonormality and
+//                   golden vectors of the reference's own eval_sh.
+//   binning         brute force: numpy lexsort over (tile, depth bits,
+//                   emission order); prefix sums; range scan.
+//   render fwd      single-surfel closed form alpha*c + (1-alpha)*bg with
+//                   alpha from a geometric intersection; two-surfel
+//                   distortion w1 w2 (m1-m2)^2; invariants sum(w) = 1 - T.
+//   render bwd,     FP64 central differences of the FP64 forward (loss =
+//   preprocess bwd  <g, outputs>) for means3D, scales, rotations (tangent
+//                   part, see quat note), opacities, shs, colors, transMat.
+//   dL_dmean2D      the densification statistic written at dsr/cuda_rasterizer/
+//                   backward.cu:626-629 is not a gradient of anything: parity
+//                   unpinned (formula restated only).
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+
+// Scene description for the oracle: host arrays, float32 or float64 as
+// selected by sro_forward's f64 flag (NULL for absent optional inputs).
+typedef struct sro_scene_c {
+  int P, D, M, W, H, tile, prefiltered;
+  const void *means3D, *shs, *colors_precomp, *opacities, *scales, *rotations, *transMat_precomp;
+  const void *viewmatrix, *projmatrix, *campos, *bg;
+  double scale_modifier, tan_fovx, tan_fovy;
+} sro_scene_c;
+
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// exp: Cody-Waite range reduction x = k ln2 + r, |r| <= ln2/2, then the
+// degree-7 Taylor polynomial of e^r in Horner form with fused multiply-adds
+// and an exact scaling by 2^k.  ln2 is split as hi (12 significant bits, so
+// k*hi is exact for |k| < 2^12) + lo.  Coefficients are 1/n! rounded to
+// float.  Truncation error of the series on |r| <= 0.347 is < 8e-9 relative.
+// The reference calls exp() on a float (dsr/cuda_rasterizer/forward.cu:379,
+// backward.cu:311); this routine fixes the bits on both CPU and GPU.
+// ---------------------------------------------------------------------------
+inline float sro_expf_impl(float x) {
+  if (x < -87.0f) return 0.0f;
+  if (x > 88.0f) return INFINITY;
+  const float kf = rintf(x * 1.44269502f);
+  float r = fmaf(kf, -0.693115234375f, x);
+  r = fmaf(kf, -3.19461833e-05f, r);
+  float p = 1.98412701e-04f;
+  p = fmaf(p, r, 1.38888892e-03f);
+  p = fmaf(p, r, 8.33333377e-03f);
+  p = fmaf(p, r, 4.16666679e-02f);
+  p = fmaf(p, r, 1.66666672e-01f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  return ldexpf(p, (int)kf);
+}
+
+template <typename R> inline R rexp(R x);
+template <> inline float rexp<float>(float x) { return sro_expf_impl(x); }
+template <> inline double rexp<double>(double x) { return std::exp(x); }
+
+// Float -> int conversion with the GPU's semantics (NaN -> 0, saturate).
+template <typename R> inline int f2i(R x) {
+  if (x != x) return 0;
+  if (x >= (R)2147483647.0) return INT_MAX;
+  if (x <= (R)-2147483648.0) return INT_MIN;
+  return (int)x;
+}
+// Float -> uint32 with saturation (dsr/cuda_rasterizer/forward.cu:430 stores
+// a float median_contributor, -1 when unset, into a uint32 buffer).
+template <typename R> inline uint32_t f2u(R x) {
+  if (!(x > (R)0)) return 0u;
+  if (x >= (R)4294967295.0) return 0xFFFFFFFFu;
+  return (uint32_t)x;
+}
+
+template <typename R> inline R rmin(R a, R b) { return (b < a) ? b : a; }
+template <typename R> inline R rmax(R a, R b) { return (a < b) ? b : a; }
+
+// Real spherical-harmonics constants, dsr/cuda_rasterizer/auxiliary.h:43-60.
+const double kSH_C0 = 0.28209479177387814;
+const double kSH_C1 = 0.4886025119029199;
+const double kSH_C2[5] = {1.0925484305920792, -1.0925484305920792, 0.31539156525252005,
+                          -1.0925484305920792, 0.5462742152960396};
+const double kSH_C3[7] = {-0.5900435899266435, 2.890611442640554, -0.4570457994644658,
+                          0.3731763325901154, -0.4570457994644658, 1.445305721320277,
+                          -0.5900435899266435};
+
+template <typename R> struct V3 {
+  R x, y, z;
+};
+template <typename R> inline V3<R> v3(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <typename R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename R> inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename R> inline V3<R> cross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+struct Scene {
+  int P, D, M, W, H, B, prefiltered;
+  bool has_shs, has_colors, has_sr, has_tprec;
+  double mod, tanx, tany;
+};
+
+template <typename R> struct Ctx {
+  Scene sc{};
+  int gx = 0, gy = 0, T = 0, N = 0;
+  std::vector<R> means3D, shs, colors, opac, scales, rots, tprec, vm, pm, campos, bg;
+  // geometry state (dsr/cuda_rasterizer/rasterizer_impl.h:29-44)
+  std::vector<int> radii;
+  std::vector<R> depths, means2D, transMat, normal_opacity, rgb;
+  std::vector<uint8_t> clamped;
+  std::vector<uint32_t> tiles_touched, point_offsets;
+  int prefilter_violation = 0;
+  // binning state
+  int L = 0;
+  std::vector<uint64_t> keys;
+  std::vector<uint32_t> point_list, ranges;
+  // image state + outputs
+  std::vector<R> out_color, out_others, final_T;
+  std::vector<uint32_t> n_contrib;
+  // backward: render-stage sums in FP64, then final grads in R
+  std::vector<double> acc_transMat, acc_mean2D, acc_normal, acc_opacity, acc_colors;
+  std::vector<R> g_mean2D, g_colors, g_opacity, g_means3D, g_transMat, g_sh, g_scales, g_rots, g_normal;
+};
+
+template <typename R> void copy_in(std::vector<R>& dst, const void* src, size_t n, int f64) {
+  dst.assign(n, (R)0);
+  if (!src || n == 0) return;
+  if (f64) {
+    const double* s = (const double*)src;
+    for (size_t i = 0; i < n; ++i) dst[i] = (R)s[i];
+  } else {
+    const float* s = (const float*)src;
+    for (size_t i = 0; i < n; ++i) dst[i] = (R)s[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SH -> RGB, dsr/cuda_rasterizer/forward.cu:20-71.
+// ---------------------------------------------------------------------------
+template <typename R> V3<R> color_from_sh(const Ctx<R>& c, int idx, uint8_t* clamped) {
+  const int deg = c.sc.D, M = c.sc.M;
+  const R* m = &c.means3D[3 * idx];
+  V3<R> dir = v3(m[0] - c.campos[0], m[1] - c.campos[1], m[2] - c.campos[2]);
+  const R len = std::sqrt((dir.x * dir.x + dir.y * dir.y) + dir.z * dir.z);
+  dir = v3(dir.x / len, dir.y / len, dir.z / len);
+  const R* sh = &c.shs[(size_t)idx * M * 3];
+  auto S = [&](int k) { return v3(sh[3 * k], sh[3 * k + 1], sh[3 * k + 2]); };
+  const R C0 = (R)(float)kSH_C0, C1 = (R)(float)kSH_C1;
+  R C2[5], C3[7];
+  for (int i = 0; i < 5; ++i) C2[i] = (R)(float)kSH_C2[i];
+  for (int i = 0; i < 7; ++i) C3[i] = (R)(float)kSH_C3[i];
+  V3<R> res = C0 * S(0);
+  if (deg > 0) {
+    const R x = dir.x, y = dir.y, z = dir.z;
+    res = res - (C1 * y) * S(1);
+    res = res + (C1 * z) * S(2);
+    res = res - (C1 * x) * S(3);
+    if (deg > 1) {
+      const R xx = x * x, yy = y * y, zz = z * z;
+      const R xy = x * y, yz = y * z, xz = x * z;
+      res = res + (C2[0] * xy) * S(4);
+      res = res + (C2[1] * yz) * S(5);
+      res = res + (C2[2] * (((R)2 * zz - xx) - yy)) * S(6);
+      res = res + (C2[3] * xz) * S(7);
+      res = res + (C2[4] * (xx - yy)) * S(8);
+      if (deg > 2) {
+        res = res + ((C3[0] * y) * ((R)3 * xx - yy)) * S(9);
+        res = res + ((C3[1] * xy) * z) * S(10);
+        res = res + ((C3[2] * y) * (((R)4 * zz - xx) - yy)) * S(11);
+        res = res + ((C3[3] * z) * (((R)2 * zz - (R)3 * xx) - (R)3 * yy)) * S(12);
+        res = res + ((C3[4] * x) * (((R)4 * zz - xx) - yy)) * S(13);
+        res = res + ((C3[5] * z) * (xx - yy)) * S(14);
+        res = res + ((C3[6] * x) * (xx - (R)3 * yy)) * S(15);
+      }
+    }
+  }
+  res = v3(res.x + (R)0.5f, res.y + (R)0.5f, res.z + (R)0.5f);
+  clamped[0] = res.x < (R)0;
+  clamped[1] = res.y < (R)0;
+  clamped[2] = res.z < (R)0;
+  return v3(rmax(res.x, (R)0), rmax(res.y, (R)0), rmax(res.z, (R)0));
+}
+
+// Rotation matrix columns of a (w,x,y,z) quaternion, dsr/cuda_rasterizer/auxiliary.h:213-235.
+// The reference normalises with rsqrtf; here 1/sqrt (correctly rounded).
+template <typename R> void quat_cols(const R* q, R& w, R& x, R& y, R& z, V3<R> col[3]) {
+  const R n2 = ((q[3] * q[3] + q[0] * q[0]) + q[1] * q[1]) + q[2] * q[2];
+  const R s = (R)1 / std::sqrt(n2);
+  w = q[0] * s;
+  x = q[1] * s;
+  y = q[2] * s;
+  z = q[3] * s;
+  col[0] = v3((R)1 - (R)2 * (y * y + z * z), (R)2 * (x * y + w * z), (R)2 * (x * z - w * y));
+  col[1] = v3((R)2 * (x * y - w * z), (R)1 - (R)2 * (x * x + z * z), (R)2 * (y * z + w * x));
+  col[2] = v3((R)2 * (x * z + w * y), (R)2 * (y * z - w * x), (R)1 - (R)2 * (x * x + y * y));
+}
+
+// getRect, dsr/cuda_rasterizer/auxiliary.h:67-77, with tile size B.
+template <typename R>
+void get_rect(R px, R py, int r, int B, int gx, int gy, int& x0, int& y0, int& x1, int& y1) {
+  const R rr = (R)r, BB = (R)B;
+  x0 = std::min(gx, std::max(0, f2i((px - rr) / BB)));
+  y0 = std::min(gy, std::max(0, f2i((py - rr) / BB)));
+  x1 = std::min(gx, std::max(0, f2i((((px + rr) + BB) - (R)1) / BB)));
+  y1 = std::min(gy, std::max(0, f2i((((py + rr) + BB) - (R)1) / BB)));
+}
+
+// ---------------------------------------------------------------------------
+// Forward preprocess, dsr/cuda_rasterizer/forward.cu:75-145 (compute_transmat,
+// compute_aabb) and :148-251 (preprocessCUDA), in_frustum auxiliary.h:185-210.
+// ---------------------------------------------------------------------------
+template <typename R> void preprocess_one(Ctx<R>& c, int idx) {
+  const Scene& s = c.sc;
+  c.radii[idx] = 0;
+  c.tiles_touched[idx] = 0;
+  const R* m = &c.means3D[3 * idx];
+  const R* vm = c.vm.data();
+  const R* pm = c.pm.data();
+  const R vx = ((vm[0] * m[0] + vm[4] * m[1]) + vm[8] * m[2]) + vm[12];
+  const R vy = ((vm[1] * m[0] + vm[5] * m[1]) + vm[9] * m[2]) + vm[13];
+  const R vz = ((vm[2] * m[0] + vm[6] * m[1]) + vm[10] * m[2]) + vm[14];
+  if (vz <= (R)0.2f) {
+    if (s.prefiltered) c.prefilter_violation = 1;
+    return;
+  }
+  V3<R> Tu, Tv, Tw, nrm;
+  if (!s.has_tprec) {
+    R w, x, y, z;
+    V3<R> col[3];
+    quat_cols(&c.rots[4 * idx], w, x, y, z, col);
+    const R sx = (R)s.mod * c.scales[2 * idx], sy = (R)s.mod * c.scales[2 * idx + 1];
+    const V3<R> L0 = sx * col[0], L1 = sy * col[1], L2 = col[2];
+    // A(i,c) = sum_r M(r,i) F(r,c), F(r,c) = pm[4r+c]; only c = 0, 1, 3 are needed.
+    auto rowdot = [&](const V3<R>& a, int cc) { return (a.x * pm[cc] + a.y * pm[4 + cc]) + a.z * pm[8 + cc]; };
+    const R a00 = rowdot(L0, 0), a01 = rowdot(L0, 1), a03 = rowdot(L0, 3);
+    const R a10 = rowdot(L1, 0), a11 = rowdot(L1, 1), a13 = rowdot(L1, 3);
+    const V3<R> pc = v3(m[0], m[1], m[2]);
+    const R a20 = rowdot(pc, 0) + pm[12], a21 = rowdot(pc, 1) + pm[13], a23 = rowdot(pc, 3) + pm[15];
+    const R hw = (R)s.W / (R)2, hw1 = (R)(s.W - 1) / (R)2;
+    const R hh = (R)s.H / (R)2, hh1 = (R)(s.H - 1) / (R)2;
+    Tu = v3(a00 * hw + a03 * hw1, a10 * hw + a13 * hw1, a20 * hw + a23 * hw1);
+    Tv = v3(a01 * hh + a03 * hh1, a11 * hh + a13 * hh1, a21 * hh + a23 * hh1);
+    Tw = v3(a03, a13, a23);
+    nrm = v3((vm[0] * L2.x + vm[4] * L2.y) + vm[8] * L2.z, (vm[1] * L2.x + vm[5] * L2.y) + vm[9] * L2.z,
+             (vm[2] * L2.x + vm[6] * L2.y) + vm[10] * L2.z);
+  } else {
+    const R* t = &c.tprec[9 * idx];
+    Tu = v3(t[0], t[1], t[2]);
+    Tv = v3(t[3], t[4], t[5]);
+    Tw = v3(t[6], t[7], t[8]);
+    nrm = v3((R)0, (R)0, (R)1);
+  }
+  // dual-visible normal flip, forward.cu:209-214
+  const R cosv = -((vx * nrm.x + vy * nrm.y) + vz * nrm.z);
+  if (cosv == (R)0) return;
+  const R mult = cosv > (R)0 ? (R)1 : (R)-1;
+  nrm = mult * nrm;
+  // compute_aabb with cutoff 3, forward.cu:119-145
+  const R d = ((R)9 * (Tw.x * Tw.x) + (R)9 * (Tw.y * Tw.y)) + (R)-1 * (Tw.z * Tw.z);
+  if (d == (R)0) return;
+  const R inv = (R)1 / d;
+  const R fx = inv * (R)9, fy = inv * (R)9, fz = inv * (R)-1;
+  const R px = (fx * (Tu.x * Tw.x) + fy * (Tu.y * Tw.y)) + fz * (Tu.z * Tw.z);
+  const R py = (fx * (Tv.x * Tw.x) + fy * (Tv.y * Tw.y)) + fz * (Tv.z * Tw.z);
+  const R h0x = px * px - ((fx * (Tu.x * Tu.x) + fy * (Tu.y * Tu.y)) + fz * (Tu.z * Tu.z));
+  const R h0y = py * py - ((fx * (Tv.x * Tv.x) + fy * (Tv.y * Tv.y)) + fz * (Tv.z * Tv.z));
+  const R hx = std::sqrt(rmax((R)1e-4f, h0x));
+  const R hy = std::sqrt(rmax((R)1e-4f, h0y));
+  const R radius = std::ceil(rmax(rmax(hx, hy), (R)(3.0f * 0.707106f)));
+  const int ir = f2i(radius);
+  int x0, y0, x1, y1;
+  get_rect(px, py, ir, s.B, c.gx, c.gy, x0, y0, x1, y1);
+  const uint32_t area = (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
+  if (area == 0) return;
+  if (!s.has_colors) {
+    const V3<R> rgb = color_from_sh(c, idx, &c.clamped[3 * idx]);
+    c.rgb[3 * idx] = rgb.x;
+    c.rgb[3 * idx + 1] = rgb.y;
+    c.rgb[3 * idx + 2] = rgb.z;
+  } else {
+    for (int k = 0; k < 3; ++k) c.rgb[3 * idx + k] = c.colors[3 * idx + k];
+  }
+  c.depths[idx] = vz;
+  c.radii[idx] = ir;
+  c.means2D[2 * idx] = px;
+  c.means2D[2 * idx + 1] = py;
+  R* T9 = &c.transMat[9 * idx];
+  T9[0] = Tu.x; T9[1] = Tu.y; T9[2] = Tu.z;
+  T9[3] = Tv.x; T9[4] = Tv.y; T9[5] = Tv.z;
+  T9[6] = Tw.x; T9[7] = Tw.y; T9[8] = Tw.z;
+  R* no = &c.normal_opacity[4 * idx];
+  no[0] = nrm.x; no[1] = nrm.y; no[2] = nrm.z; no[3] = c.opac[idx];
+  c.tiles_touched[idx] = area;
+}
+
+// getHigherMsb, dsr/cuda_rasterizer/rasterizer_impl.cu:35-50.
+uint32_t higher_msb(uint32_t n) {
+  uint32_t msb = sizeof(n) * 4;
+  uint32_t step = msb;
+  while (step > 1) {
+    step /= 2;
+    if (n >> msb) msb += step; else msb -= step;
+  }
+  if (n >> msb) msb++;
+  return msb;
+}
+
+template <typename R> uint32_t depth_bits(R d) {
+  const float f = (float)d;
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+
+// ---------------------------------------------------------------------------
+// Binning: inclusive scan (rasterizer_impl.cu:278), duplicateWithKeys
+// (:70-111), stable radix sort over bits [0, 32+bit) (:301-309), memset +
+// identifyTileRanges (:311-318, :116-138).
+// ---------------------------------------------------------------------------
+template <typename R> void binning(Ctx<R>& c) {
+  const int P = c.sc.P;
+  c.point_offsets.assign(P, 0);
+  uint64_t acc = 0;
+  for (int i = 0; i < P; ++i) {
+    acc += c.tiles_touched[i];
+    c.point_offsets[i] = (uint32_t)acc;
+  }
+  c.L = P ? (int)c.point_offsets[P - 1] : 0;
+  std::vector<uint64_t> ku(c.L);
+  std::vector<uint32_t> vu(c.L);
+  for (int idx = 0; idx < P; ++idx) {
+    if (c.radii[idx] <= 0) continue;
+    uint32_t off = idx == 0 ? 0 : c.point_offsets[idx - 1];
+    int x0, y0, x1, y1;
+    get_rect(c.means2D[2 * idx], c.means2D[2 * idx + 1], c.radii[idx], c.sc.B, c.gx, c.gy, x0, y0, x1, y1);
+    for (int y = y0; y < y1; ++y)
+      for (int x = x0; x < x1; ++x) {
+        uint64_t key = (uint64_t)(uint32_t)(y * c.gx + x);
+        key <<= 32;
+        key |= depth_bits(c.depths[idx]);
+        ku[off] = key;
+        vu[off] = (uint32_t)idx;
+        ++off;
+      }
+  }
+  const uint32_t bit = higher_msb((uint32_t)(c.gx * c.gy));
+  const uint64_t mask = (32 + bit >= 64) ? ~0ull : ((1ull << (32 + bit)) - 1ull);
+  std::vector<uint32_t> order(c.L);
+  for (int i = 0; i < c.L; ++i) order[i] = (uint32_t)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return (ku[a] & mask) < (ku[b] & mask); });
+  c.keys.resize(c.L);
+  c.point_list.resize(c.L);
+  for (int i = 0; i < c.L; ++i) {
+    c.keys[i] = ku[order[i]];
+    c.point_list[i] = vu[order[i]];
+  }
+  c.ranges.assign(2 * (size_t)c.T, 0u);
+  for (int i = 0; i < c.L; ++i) {
+    const uint32_t cur = (uint32_t)(c.keys[i] >> 32);
+    if (i == 0) c.ranges[2 * cur] = 0;
+    else {
+      const uint32_t prev = (uint32_t)(c.keys[i - 1] >> 32);
+      if (cur != prev) {
+        c.ranges[2 * prev + 1] = (uint32_t)i;
+        c.ranges[2 * cur] = (uint32_t)i;
+      }
+    }
+    if (i == c.L - 1) c.ranges[2 * cur + 1] = (uint32_t)c.L;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-pixel forward blend, dsr/cuda_rasterizer/forward.cu:256-441.
+// ---------------------------------------------------------------------------
+template <typename R> void render_pixel(Ctx<R>& c, int tile, int pxi, int pyi) {
+  const int W = c.sc.W, H = c.sc.H;
+  const uint32_t rs = c.ranges[2 * tile], re = c.ranges[2 * tile + 1];
+  const R pixx = (R)pxi, pixy = (R)pyi;
+  R T = (R)1;
+  uint32_t contributor = 0, last_contributor = 0;
+  R C[3] = {0, 0, 0}, N[3] = {0, 0, 0};
+  R D = 0, M1 = 0, M2 = 0, distortion = 0, median_depth = 0, median_contributor = (R)-1;
+  const R near_n = (R)0.2f, far_n = (R)100.0f;
+  const R mcoef = far_n / (far_n - near_n);
+  for (uint32_t e = rs; e < re; ++e) {
+    contributor++;
+    const int id = (int)c.point_list[e];
+    const R* T9 = &c.transMat[9 * id];
+    const V3<R> Tu = v3(T9[0], T9[1], T9[2]), Tv = v3(T9[3], T9[4], T9[5]), Tw = v3(T9[6], T9[7], T9[8]);
+    const V3<R> k = pixx * Tw - Tu;
+    const V3<R> l = pixy * Tw - Tv;
+    const V3<R> p = cross(k, l);
+    if (p.z == (R)0) continue;
+    const R sx = p.x / p.z, sy = p.y / p.z;
+    const R rho3d = sx * sx + sy * sy;
+    const R dx = c.means2D[2 * id] - pixx, dy = c.means2D[2 * id + 1] - pixy;
+    const R rho2d = (R)2 * (dx * dx + dy * dy);
+    const R rho = rmin(rho3d, rho2d);
+    const R depth = (rho3d <= rho2d) ? (sx * Tw.x + sy * Tw.y) + Tw.z : Tw.z;
+    if (depth < near_n) continue;
+    const R* no = &c.normal_opacity[4 * id];
+    const R power = (R)-0.5f * rho;
+    if (power > (R)0) continue;
+    const R alpha = rmin((R)0.99f, no[3] * rexp<R>(power));
+    if (alpha < (R)(1.0f / 255.0f)) continue;
+    const R test_T = T * ((R)1 - alpha);
+    if (test_T < (R)0.0001f) break;
+    const R w = alpha * T;
+    const R A = (R)1 - T;
+    const R mm = mcoef * ((R)1 - near_n / depth);
+    distortion += (((mm * mm) * A + M2) - ((R)2 * mm) * M1) * w;
+    D += depth * w;
+    M1 += mm * w;
+    M2 += (mm * mm) * w;
+    if (T > (R)0.5) {
+      median_depth = depth;
+      median_contributor = (R)contributor;
+    }
+    for (int ch = 0; ch < 3; ++ch) N[ch] += no[ch] * w;
+    for (int ch = 0; ch < 3; ++ch) C[ch] += c.rgb[3 * id + ch] * w;
+    T = test_T;
+    last_contributor = contributor;
+  }
+  const size_t HW = (size_t)H * W, pid = (size_t)W * pyi + pxi;
+  c.final_T[pid] = T;
+  c.n_contrib[pid] = last_contributor;
+  for (int ch = 0; ch < 3; ++ch) c.out_color[ch * HW + pid] = C[ch] + T * c.bg[ch];
+  c.n_contrib[pid + HW] = f2u(median_contributor);
+  c.final_T[pid + HW] = M1;
+  c.final_T[pid + 2 * HW] = M2;
+  c.out_others[pid] = D;
+  c.out_others[pid + HW] = (R)1 - T;
+  for (int ch = 0; ch < 3; ++ch) c.out_others[pid + (2 + ch) * HW] = N[ch];
+  c.out_others[pid + 5 * HW] = median_depth;
+  c.out_others[pid + 6 * HW] = distortion;
+}
+
+template <typename R> void run_forward(Ctx<R>& c) {
+  const Scene& s = c.sc;
+  const int P = s.P;
+  c.gx = (s.W + s.B - 1) / s.B;
+  c.gy = (s.H + s.B - 1) / s.B;
+  c.T = c.gx * c.gy;
+  c.N = s.W * s.H;
+  c.radii.assign(P, 0);
+  c.tiles_touched.assign(P, 0);
+  c.depths.assign(P, 0);
+  c.means2D.assign(2 * (size_t)P, 0);
+  c.transMat.assign(9 * (size_t)P, 0);
+  c.normal_opacity.assign(4 * (size_t)P, 0);
+  c.rgb.assign(3 * (size_t)P, 0);
+  c.clamped.assign(3 * (size_t)P, 0);
+  c.prefilter_violation = 0;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < P; ++i) preprocess_one(c, i);
+  binning(c);
+  c.out_color.assign(3 * (size_t)c.N, 0);
+  c.out_others.assign(7 * (size_t)c.N, 0);
+  c.final_T.assign(3 * (size_t)c.N, 0);
+  c.n_contrib.assign(2 * (size_t)c.N, 0);
+  const int B = s.B;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int t = 0; t < c.T; ++t) {
+    const int tx = t % c.gx, ty = t / c.gx;
+    for (int j = 0; j < B; ++j)
+      for (int i = 0; i < B; ++i) {
+        const int x = tx * B + i, y = ty * B + j;
+        if (x < s.W && y < s.H) render_pixel(c, t, x, y);
+      }
+  }
+}
+
+inline void atomic_addd(double* p, double v) {
+#pragma omp atomic
+  *p += v;
+}
+
+// ---------------------------------------------------------------------------
+// Per-pixel backward blend, dsr/cuda_rasterizer/backward.cu:143-440.
+// ---------------------------------------------------------------------------
+template <typename R>
+void render_bwd_pixel(Ctx<R>& c, int tile, int pxi, int pyi, const R* dL_dpix, const R* dL_dothers) {
+  const int W = c.sc.W, H = c.sc.H;
+  const size_t HW = (size_t)H * W, pid = (size_t)W * pyi + pxi;
+  const uint32_t rs = c.ranges[2 * tile], re = c.ranges[2 * tile + 1];
+  const R pixx = (R)pxi, pixy = (R)pyi;
+  const R T_final = c.final_T[pid];
+  R T = T_final;
+  uint32_t contributor = re - rs;
+  const int last_contributor = (int)c.n_contrib[pid];
+  const int median_contributor = (int)c.n_contrib[pid + HW];
+  R accum_rec[3] = {0, 0, 0}, dL_dpixel[3], last_color[3] = {0, 0, 0};
+  const R dL_ddepth = dL_dothers[pid];
+  const R dL_daccum = dL_dothers[HW + pid];
+  const R dL_dreg = dL_dothers[6 * HW + pid];
+  R dL_dnormal2D[3];
+  for (int i = 0; i < 3; ++i) dL_dnormal2D[i] = dL_dothers[(2 + i) * HW + pid];
+  const R dL_dmedian_depth = dL_dothers[5 * HW + pid];
+  R last_depth = 0, last_normal[3] = {0, 0, 0}, accum_depth_rec = 0, accum_alpha_rec = 0;
+  R accum_normal_rec[3] = {0, 0, 0};
+  const R final_D = c.final_T[pid + HW], final_D2 = c.final_T[pid + 2 * HW];
+  const R final_A = (R)1 - T_final;
+  R last_dL_dT = 0;
+  for (int i = 0; i < 3; ++i) dL_dpixel[i] = dL_dpix[i * HW + pid];
+  R last_alpha = 0;
+  const R near_n = (R)0.2f, far_n = (R)100.0f;
+  const R mcoef = far_n / (far_n - near_n);
+  const R dmd_num = far_n * near_n, dmd_den0 = far_n - near_n;
+  R bg_dot_dpixel = 0;
+  for (int i = 0; i < 3; ++i) bg_dot_dpixel += c.bg[i] * dL_dpixel[i];
+  for (uint32_t e = re; e > rs; --e) {
+    contributor--;
+    if (contributor >= (uint32_t)last_contributor) continue;
+    const int id = (int)c.point_list[e - 1];
+    const R* T9 = &c.transMat[9 * id];
+    const V3<R> Tu = v3(T9[0], T9[1], T9[2]), Tv = v3(T9[3], T9[4], T9[5]), Tw = v3(T9[6], T9[7], T9[8]);
+    const V3<R> k = pixx * Tw - Tu;
+    const V3<R> l = pixy * Tw - Tv;
+    const V3<R> p = cross(k, l);
+    if (p.z == (R)0) continue;
+    const R sx = p.x / p.z, sy = p.y / p.z;
+    const R rho3d = sx * sx + sy * sy;
+    const R dx = c.means2D[2 * id] - pixx, dy = c.means2D[2 * id + 1] - pixy;
+    const R rho2d = (R)2 * (dx * dx + dy * dy);
+    const R rho = rmin(rho3d, rho2d);
+    const R c_d = (rho3d <= rho2d) ? (sx * Tw.x + sy * Tw.y) + Tw.z : Tw.z;
+    if (c_d < near_n) continue;
+    const R* no = &c.normal_opacity[4 * id];
+    const R power = (R)-0.5f * rho;
+    if (power > (R)0) continue;
+    const R G = rexp<R>(power);
+    const R alpha = rmin((R)0.99f, no[3] * G);
+    if (alpha < (R)(1.0f / 255.0f)) continue;
+    T = T / ((R)1 - alpha);
+    const R w = alpha * T;
+    R dL_dalpha = 0;
+    for (int ch = 0; ch < 3; ++ch) {
+      const R col = c.rgb[3 * id + ch];
+      accum_rec[ch] = last_alpha * last_color[ch] + ((R)1 - last_alpha) * accum_rec[ch];
+      last_color[ch] = col;
+      dL_dalpha += (col - accum_rec[ch]) * dL_dpixel[ch];
+      atomic_addd(&c.acc_colors[3 * id + ch], (double)(w * dL_dpixel[ch]));
+    }
+    R dL_dz = 0, dL_dweight = 0;
+    const R m_d = mcoef * ((R)1 - near_n / c_d);
+    const R dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
+    if ((int)contributor == median_contributor - 1) dL_dz += dL_dmedian_depth;
+    dL_dweight += ((final_D2 + (m_d * m_d) * final_A) - ((R)2 * m_d) * final_D) * dL_dreg;
+    dL_dalpha += dL_dweight - last_dL_dT;
+    last_dL_dT = dL_dweight * alpha + ((R)1 - alpha) * last_dL_dT;
+    const R dL_dmd = (((R)2 * (T * alpha)) * (m_d * final_A - final_D)) * dL_dreg;
+    dL_dz += dL_dmd * dmd_dd;
+    accum_depth_rec = last_alpha * last_depth + ((R)1 - last_alpha) * accum_depth_rec;
+    last_depth = c_d;
+    dL_dalpha += (c_d - accum_depth_rec) * dL_ddepth;
+    accum_alpha_rec = last_alpha + ((R)1 - last_alpha) * accum_alpha_rec;
+    dL_dalpha += ((R)1 - accum_alpha_rec) * dL_daccum;
+    for (int ch = 0; ch < 3; ++ch) {
+      accum_normal_rec[ch] = last_alpha * last_normal[ch] + ((R)1 - last_alpha) * accum_normal_rec[ch];
+      last_normal[ch] = no[ch];
+      dL_dalpha += (no[ch] - accum_normal_rec[ch]) * dL_dnormal2D[ch];
+      atomic_addd(&c.acc_normal[3 * id + ch], (double)((alpha * T) * dL_dnormal2D[ch]));
+    }
+    dL_dalpha *= T;
+    last_alpha = alpha;
+    dL_dalpha += (-T_final / ((R)1 - alpha)) * bg_dot_dpixel;
+    const R dL_dG = no[3] * dL_dalpha;
+    dL_dz += (alpha * T) * dL_ddepth;
+    if (rho3d <= rho2d) {
+      const R dL_dsx = (dL_dG * -G) * sx + dL_dz * Tw.x;
+      const R dL_dsy = (dL_dG * -G) * sy + dL_dz * Tw.y;
+      const R dsx_pz = dL_dsx / p.z, dsy_pz = dL_dsy / p.z;
+      const V3<R> dL_dp = v3(dsx_pz, dsy_pz, -(dsx_pz * sx + dsy_pz * sy));
+      const V3<R> dL_dk = cross(l, dL_dp);
+      const V3<R> dL_dl = cross(dL_dp, k);
+      double* g = &c.acc_transMat[9 * id];
+      atomic_addd(&g[0], (double)(-dL_dk.x));
+      atomic_addd(&g[1], (double)(-dL_dk.y));
+      atomic_addd(&g[2], (double)(-dL_dk.z));
+      atomic_addd(&g[3], (double)(-dL_dl.x));
+      atomic_addd(&g[4], (double)(-dL_dl.y));
+      atomic_addd(&g[5], (double)(-dL_dl.z));
+      atomic_addd(&g[6], (double)((pixx * dL_dk.x + pixy * dL_dl.x) + dL_dz * sx));
+      atomic_addd(&g[7], (double)((pixx * dL_dk.y + pixy * dL_dl.y) + dL_dz * sy));
+      atomic_addd(&g[8], (double)((pixx * dL_dk.z + pixy * dL_dl.z) + dL_dz * (R)1));
+    } else {
+      const R dG_ddelx = (-G * (R)2) * dx;
+      const R dG_ddely = (-G * (R)2) * dy;
+      atomic_addd(&c.acc_mean2D[3 * id], (double)(dL_dG * dG_ddelx));
+      atomic_addd(&c.acc_mean2D[3 * id + 1], (double)(dL_dG * dG_ddely));
+      atomic_addd(&c.acc_transMat[9 * id + 8], (double)dL_dz);
+    }
+    atomic_addd(&c.acc_opacity[id], (double)(G * dL_dalpha));
+  }
+}
+
+// dnormvdv, dsr/cuda_rasterizer/auxiliary.h:128-138.
+template <typename R> V3<R> dnormvdv(V3<R> v, V3<R> dv) {
+  const R sum2 = (v.x * v.x + v.y * v.y) + v.z * v.z;
+  const R invsum32 = (R)1 / std::sqrt((sum2 * sum2) * sum2);
+  V3<R> r;
+  r.x = ((((sum2 - v.x * v.x) * dv.x) - (v.y * v.x) * dv.y) - (v.z * v.x) * dv.z) * invsum32;
+  r.y = (((-v.x * v.y) * dv.x + (sum2 - v.y * v.y) * dv.y) - (v.z * v.y) * dv.z) * invsum32;
+  r.z = (((-v.x * v.z) * dv.x - (v.y * v.z) * dv.y) + (sum2 - v.z * v.z) * dv.z) * invsum32;
+  return r;
+}
+
+// SH backward, dsr/cuda_rasterizer/backward.cu:20-139.
+template <typename R> void sh_backward(Ctx<R>& c, int idx) {
+  const int deg = c.sc.D, M = c.sc.M;
+  const R* m = &c.means3D[3 * idx];
+  const V3<R> dir_orig = v3(m[0] - c.campos[0], m[1] - c.campos[1], m[2] - c.campos[2]);
+  const R len = std::sqrt((dir_orig.x * dir_orig.x + dir_orig.y * dir_orig.y) + dir_orig.z * dir_orig.z);
+  const V3<R> dir = v3(dir_orig.x / len, dir_orig.y / len, dir_orig.z / len);
+  const R* sh = &c.shs[(size_t)idx * M * 3];
+  auto S = [&](int k) { return v3(sh[3 * k], sh[3 * k + 1], sh[3 * k + 2]); };
+  V3<R> dL_dRGB = v3(c.g_colors[3 * idx], c.g_colors[3 * idx + 1], c.g_colors[3 * idx + 2]);
+  if (c.clamped[3 * idx + 0]) dL_dRGB.x = 0;
+  if (c.clamped[3 * idx + 1]) dL_dRGB.y = 0;
+  if (c.clamped[3 * idx + 2]) dL_dRGB.z = 0;
+  const R C0 = (R)(float)kSH_C0, C1 = (R)(float)kSH_C1;
+  R C2[5], C3[7];
+  for (int i = 0; i < 5; ++i) C2[i] = (R)(float)kSH_C2[i];
+  for (int i = 0; i < 7; ++i) C3[i] = (R)(float)kSH_C3[i];
+  V3<R> dx = v3<R>(0, 0, 0), dy = v3<R>(0, 0, 0), dz = v3<R>(0, 0, 0);
+  const R x = dir.x, y = dir.y, z = dir.z;
+  R* g = &c.g_sh[(size_t)idx * M * 3];
+  auto put = [&](int k, R coef) {
+    g[3 * k] = coef * dL_dRGB.x;
+    g[3 * k + 1] = coef * dL_dRGB.y;
+    g[3 * k + 2] = coef * dL_dRGB.z;
+  };
+  put(0, C0);
+  if (deg > 0) {
+    put(1, -C1 * y);
+    put(2, C1 * z);
+    put(3, -C1 * x);
+    dx = -C1 * S(3);
+    dy = -C1 * S(1);
+    dz = C1 * S(2);
+    if (deg > 1) {
+      const R xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+      put(4, C2[0] * xy);
+      put(5, C2[1] * yz);
+      put(6, C2[2] * (((R)2 * zz - xx) - yy));
+      put(7, C2[3] * xz);
+      put(8, C2[4] * (xx - yy));
+      dx = dx + ((((C2[0] * y) * S(4) + ((C2[2] * (R)2) * -x) * S(6)) + (C2[3] * z) * S(7)) + ((C2[4] * (R)2) * x) * S(8));
+      dy = dy + ((((C2[0] * x) * S(4) + (C2[1] * z) * S(5)) + ((C2[2] * (R)2) * -y) * S(6)) + ((C2[4] * (R)2) * -y) * S(8));
+      dz = dz + (((C2[1] * y) * S(5) + (((C2[2] * (R)2) * (R)2) * z) * S(6)) + (C2[3] * x) * S(7));
+      if (deg > 2) {
+        put(9, (C3[0] * y) * ((R)3 * xx - yy));
+        put(10, (C3[1] * xy) * z);
+        put(11, (C3[2] * y) * (((R)4 * zz - xx) - yy));
+        put(12, (C3[3] * z) * (((R)2 * zz - (R)3 * xx) - (R)3 * yy));
+        put(13, (C3[4] * x) * (((R)4 * zz - xx) - yy));
+        put(14, (C3[5] * z) * (xx - yy));
+        put(15, (C3[6] * x) * (xx - (R)3 * yy));
+        // term order and association follow backward.cu:99-122
+        auto mul = [](V3<R> v, R f) { return v3(v.x * f, v.y * f, v.z * f); };
+        V3<R> ax = mul(mul(mul(C3[0] * S(9), (R)3), (R)2), xy);
+        ax = ax + mul(C3[1] * S(10), yz);
+        ax = ax + mul(mul(C3[2] * S(11), (R)-2), xy);
+        ax = ax + mul(mul(mul(C3[3] * S(12), (R)-3), (R)2), xz);
+        ax = ax + mul(C3[4] * S(13), (((R)-3 * xx + (R)4 * zz) - yy));
+        ax = ax + mul(mul(C3[5] * S(14), (R)2), xz);
+        ax = ax + mul(mul(C3[6] * S(15), (R)3), (xx - yy));
+        V3<R> ay = mul(mul(C3[0] * S(9), (R)3), (xx - yy));
+        ay = ay + mul(C3[1] * S(10), xz);
+        ay = ay + mul(C3[2] * S(11), (((R)-3 * yy + (R)4 * zz) - xx));
+        ay = ay + mul(mul(mul(C3[3] * S(12), (R)-3), (R)2), yz);
+        ay = ay + mul(mul(C3[4] * S(13), (R)-2), xy);
+        ay = ay + mul(mul(C3[5] * S(14), (R)-2), yz);
+        ay = ay + mul(mul(mul(C3[6] * S(15), (R)-3), (R)2), xy);
+        V3<R> az = mul(C3[1] * S(10), xy);
+        az = az + mul(mul(mul(C3[2] * S(11), (R)4), (R)2), yz);
+        az = az + mul(mul(C3[3] * S(12), (R)3), (((R)2 * zz - xx) - yy));
+        az = az + mul(mul(mul(C3[4] * S(13), (R)4), (R)2), xz);
+        az = az + mul(C3[5] * S(14), (xx - yy));
+        dx = dx + ax;
+        dy = dy + ay;
+        dz = dz + az;
+      }
+    }
+  }
+  auto dot3 = [](V3<R> a, V3<R> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; };
+  const V3<R> dL_ddir = v3(dot3(dx, dL_dRGB), dot3(dy, dL_dRGB), dot3(dz, dL_dRGB));
+  const V3<R> dm = dnormvdv(dir_orig, dL_ddir);
+  c.g_means3D[3 * idx] += dm.x;
+  c.g_means3D[3 * idx + 1] += dm.y;
+  c.g_means3D[3 * idx + 2] += dm.z;
+}
+
+// ---------------------------------------------------------------------------
+// Backward preprocess, dsr/cuda_rasterizer/backward.cu:443-630 and the
+// quaternion VJP auxiliary.h:238-282.
+// ---------------------------------------------------------------------------
+template <typename R> void preprocess_bwd_one(Ctx<R>& c, int idx) {
+  const Scene& s = c.sc;
+  if (!(c.radii[idx] > 0)) return;
+  const R focal_x = (R)(float)s.W / ((R)2.0f * (R)(float)s.tanx);
+  const R focal_y = (R)(float)s.H / ((R)2.0f * (R)(float)s.tany);
+  const int W = f2i((focal_x * (R)(float)s.tanx) * (R)2);
+  const int H = f2i((focal_y * (R)(float)s.tany) * (R)2);
+  const R* pm = c.pm.data();
+  const R* vm = c.vm.data();
+  R* gT = &c.g_transMat[9 * idx];
+  V3<R> T0, T1, T2, nrm, L0c, L1c;
+  V3<R> Rc[3];
+  R qw = 0, qx = 0, qy = 0, qz = 0, scx = 0, scy = 0;
+  V3<R> Pc[3][2];  // P columns as (xyz, w) halves: P[k] = (a, b) with a = rows 0..2, b.x = row 3
+  R P4[3][4];
+  const bool precomp = s.has_tprec;
+  const R* m = &c.means3D[3 * idx];
+  if (precomp) {
+    const R* t = &c.tprec[9 * idx];
+    T0 = v3(t[0], t[1], t[2]);
+    T1 = v3(t[3], t[4], t[5]);
+    T2 = v3(t[6], t[7], t[8]);
+  } else {
+    quat_cols(&c.rots[4 * idx], qw, qx, qy, qz, Rc);
+    scx = c.scales[2 * idx];
+    scy = c.scales[2 * idx + 1];
+    // scale_to_mat(scale, 1.0f): the scale modifier is not applied here (backward.cu:481)
+    L0c = ((R)1 * scx) * Rc[0];
+    L1c = ((R)1 * scy) * Rc[1];
+    const R hw = (R)W / (R)2, hw1 = (R)(W - 1) / (R)2, hh = (R)H / (R)2, hh1 = (R)(H - 1) / (R)2;
+    // P = world2ndc * ndc2pix (backward.cu:503): P[k](i) for rows i = 0..3
+    for (int i = 0; i < 4; ++i) {
+      P4[0][i] = pm[4 * i] * hw + pm[4 * i + 3] * hw1;
+      P4[1][i] = pm[4 * i + 1] * hh + pm[4 * i + 3] * hh1;
+      P4[2][i] = pm[4 * i + 3];
+    }
+    // T = transpose(M) * P, T(i,k) = dot(M_i, P[k]) with M_0 = (L0,0), M_1 = (L1,0), M_2 = (p,1)
+    auto dotM = [&](const V3<R>& a, R a4, int k) {
+      return ((a.x * P4[k][0] + a.y * P4[k][1]) + a.z * P4[k][2]) + a4 * P4[k][3];
+    };
+    const V3<R> pc = v3(m[0], m[1], m[2]);
+    T0 = v3(dotM(L0c, 0, 0), dotM(L1c, 0, 0), dotM(pc, 1, 0));
+    T1 = v3(dotM(L0c, 0, 1), dotM(L1c, 0, 1), dotM(pc, 1, 1));
+    T2 = v3(dotM(L0c, 0, 2), dotM(L1c, 0, 2), dotM(pc, 1, 2));
+    const V3<R> L2 = Rc[2];
+    nrm = v3((vm[0] * L2.x + vm[4] * L2.y) + vm[8] * L2.z, (vm[1] * L2.x + vm[5] * L2.y) + vm[9] * L2.z,
+             (vm[2] * L2.x + vm[6] * L2.y) + vm[10] * L2.z);
+  }
+  (void)Pc;
+  V3<R> dT0 = v3(gT[0], gT[1], gT[2]), dT1 = v3(gT[3], gT[4], gT[5]), dT2 = v3(gT[6], gT[7], gT[8]);
+  const R gmx = c.g_mean2D[3 * idx], gmy = c.g_mean2D[3 * idx + 1];
+  if (gmx != (R)0 || gmy != (R)0) {
+    // adjoint of the AABB centre (backward.cu:515-530); mixed double steps mirror the reference
+    const V3<R> tv = v3<R>(9, 9, -1);
+    const R d = (tv.x * (T2.x * T2.x) + tv.y * (T2.y * T2.y)) + tv.z * (T2.z * T2.z);
+    const R invd = (R)1.0f / d;
+    const V3<R> f = v3(tv.x * invd, tv.y * invd, tv.z * invd);
+    auto had = [](V3<R> a, V3<R> b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); };
+    const V3<R> dL_dT0 = had(gmx * f, T2);
+    const V3<R> dL_dT1 = had(gmy * f, T2);
+    V3<R> dL_dT3 = had(gmx * f, T0) + had(gmy * f, T1);
+    const V3<R> dL_df = had(gmx * T0, T2) + had(gmy * T1, T2);
+    const R dotdf = (dL_df.x * f.x + dL_df.y * f.y) + dL_df.z * f.z;
+    const R dL_dd = (R)((double)dotdf * (-1.0 / (double)d));
+    const V3<R> dd_dT3 = v3((tv.x * T2.x) * (R)2.0f, (tv.y * T2.y) * (R)2.0f, (tv.z * T2.z) * (R)2.0f);
+    dL_dT3 = dL_dT3 + dL_dd * dd_dT3;
+    dT0 = dT0 + dL_dT0;
+    dT1 = dT1 + dL_dT1;
+    dT2 = dT2 + dL_dT3;
+    if (precomp) {
+      gT[0] = dT0.x; gT[1] = dT0.y; gT[2] = dT0.z;
+      gT[3] = dT1.x; gT[4] = dT1.y; gT[5] = dT1.z;
+      gT[6] = dT2.x; gT[7] = dT2.y; gT[8] = dT2.z;
+    }
+  }
+  if (!precomp) {
+    // dL_dM = P * transpose(dL_dT): column cc = P[0]*dT0[cc] + P[1]*dT1[cc] + P[2]*dT2[cc]
+    auto comp = [](V3<R> v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); };
+    R dM[3][3];
+    for (int cc = 0; cc < 3; ++cc)
+      for (int i = 0; i < 3; ++i)
+        dM[cc][i] = (P4[0][i] * comp(dT0, cc) + P4[1][i] * comp(dT1, cc)) + P4[2][i] * comp(dT2, cc);
+    const R* gn = &c.g_normal[3 * idx];
+    V3<R> dL_dtn = v3((vm[0] * gn[0] + vm[1] * gn[1]) + vm[2] * gn[2], (vm[4] * gn[0] + vm[5] * gn[1]) + vm[6] * gn[2],
+                      (vm[8] * gn[0] + vm[9] * gn[1]) + vm[10] * gn[2]);
+    const R pvx = ((vm[0] * m[0] + vm[4] * m[1]) + vm[8] * m[2]) + vm[12];
+    const R pvy = ((vm[1] * m[0] + vm[5] * m[1]) + vm[9] * m[2]) + vm[13];
+    const R pvz = ((vm[2] * m[0] + vm[6] * m[1]) + vm[10] * m[2]) + vm[14];
+    const R cosv = -((pvx * nrm.x + pvy * nrm.y) + pvz * nrm.z);
+    const R mult = cosv > (R)0 ? (R)1 : (R)-1;
+    dL_dtn = mult * dL_dtn;
+    const V3<R> dRS0 = v3(dM[0][0], dM[0][1], dM[0][2]);
+    const V3<R> dRS1 = v3(dM[1][0], dM[1][1], dM[1][2]);
+    const V3<R> dR0 = v3(dRS0.x * scx, dRS0.y * scx, dRS0.z * scx);
+    const V3<R> dR1 = v3(dRS1.x * scy, dRS1.y * scy, dRS1.z * scy);
+    const V3<R> dR2 = dL_dtn;
+    // quat_to_rotmat_vjp with v_R[col][row]
+    const R vR[3][3] = {{dR0.x, dR0.y, dR0.z}, {dR1.x, dR1.y, dR1.z}, {dR2.x, dR2.y, dR2.z}};
+    const R w = qw, x = qx, y = qy, z = qz;
+    R* gq = &c.g_rots[4 * idx];
+    gq[0] = (R)2 * ((x * (vR[1][2] - vR[2][1]) + y * (vR[2][0] - vR[0][2])) + z * (vR[0][1] - vR[1][0]));
+    gq[1] = (R)2 * ((((R)-2 * x) * (vR[1][1] + vR[2][2]) + y * (vR[0][1] + vR[1][0])) + z * (vR[0][2] + vR[2][0]) +
+                    w * (vR[1][2] - vR[2][1]));
+    gq[2] = (R)2 * (((x * (vR[0][1] + vR[1][0]) - ((R)2 * y) * (vR[0][0] + vR[2][2])) + z * (vR[1][2] + vR[2][1])) +
+                    w * (vR[2][0] - vR[0][2]));
+    gq[3] = (R)2 * (((x * (vR[0][2] + vR[2][0]) + y * (vR[1][2] + vR[2][1])) - ((R)2 * z) * (vR[0][0] + vR[1][1])) +
+                    w * (vR[0][1] - vR[1][0]));
+    c.g_scales[2 * idx] = (dRS0.x * Rc[0].x + dRS0.y * Rc[0].y) + dRS0.z * Rc[0].z;
+    c.g_scales[2 * idx + 1] = (dRS1.x * Rc[1].x + dRS1.y * Rc[1].y) + dRS1.z * Rc[1].z;
+    c.g_means3D[3 * idx] = dM[2][0];
+    c.g_means3D[3 * idx + 1] = dM[2][1];
+    c.g_means3D[3 * idx + 2] = dM[2][2];
+  }
+  if (s.has_shs) sh_backward(c, idx);
+  // densification statistic, backward.cu:626-629
+  const R depth = c.transMat[9 * idx + 8];
+  c.g_mean2D[3 * idx] = (R)(((double)(gT[2] * depth) * 0.5) * (double)(R)(float)W);
+  c.g_mean2D[3 * idx + 1] = (R)(((double)(gT[5] * depth) * 0.5) * (double)(R)(float)H);
+}
+
+template <typename R> void run_backward(Ctx<R>& c, const R* dL_dpix, const R* dL_dothers) {
+  const Scene& s = c.sc;
+  const int P = s.P;
+  c.acc_transMat.assign(9 * (size_t)P, 0.0);
+  c.acc_mean2D.assign(3 * (size_t)P, 0.0);
+  c.acc_normal.assign(3 * (size_t)P, 0.0);
+  c.acc_opacity.assign((size_t)P, 0.0);
+  c.acc_colors.assign(3 * (size_t)P, 0.0);
+  const int B = s.B;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int t = 0; t < c.T; ++t) {
+    const int tx = t % c.gx, ty = t / c.gx;
+    for (int j = 0; j < B; ++j)
+      for (int i = 0; i < B; ++i) {
+        const int x = tx * B + i, y = ty * B + j;
+        if (x < s.W && y < s.H) render_bwd_pixel(c, t, x, y, dL_dpix, dL_dothers);
+      }
+  }
+  auto round_into = [](std::vector<R>& dst, const std::vector<double>& src) {
+    dst.resize(src.size());
+    for (size_t i = 0; i < src.size(); ++i) dst[i] = (R)src[i];
+  };
+  round_into(c.g_transMat, c.acc_transMat);
+  round_into(c.g_mean2D, c.acc_mean2D);
+  round_into(c.g_normal, c.acc_normal);
+  round_into(c.g_opacity, c.acc_opacity);
+  round_into(c.g_colors, c.acc_colors);
+  c.g_means3D.assign(3 * (size_t)P, (R)0);
+  c.g_sh.assign((size_t)P * s.M * 3, (R)0);
+  c.g_scales.assign(2 * (size_t)P, (R)0);
+  c.g_rots.assign(4 * (size_t)P, (R)0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < P; ++i) preprocess_bwd_one(c, i);
+}
+
+struct Handle {
+  int f64 = 0;
+  Ctx<float> cf;
+  Ctx<double> cd;
+  std::string err;
+};
+
+
+}  // namespace
+
+namespace {
+template <typename R> void load_scene(Ctx<R>& c, const sro_scene_c* s, int f64) {
+  Scene& sc = c.sc;
+  sc.P = s->P; sc.D = s->D; sc.M = s->M; sc.W = s->W; sc.H = s->H; sc.B = s->tile;
+  sc.prefiltered = s->prefiltered;
+  sc.has_shs = s->shs != nullptr;
+  sc.has_colors = s->colors_precomp != nullptr;
+  sc.has_tprec = s->transMat_precomp != nullptr;
+  sc.has_sr = s->scales != nullptr;
+  // Python floats cross the reference's pybind boundary as C float (rasterize_points.h:18-68).
+  sc.mod = f64 ? s->scale_modifier : (double)(float)s->scale_modifier;
+  sc.tanx = f64 ? s->tan_fovx : (double)(float)s->tan_fovx;
+  sc.tany = f64 ? s->tan_fovy : (double)(float)s->tan_fovy;
+  const size_t P = (size_t)s->P;
+  copy_in(c.means3D, s->means3D, 3 * P, f64);
+  copy_in(c.shs, s->shs, P * (size_t)(s->M > 0 ? s->M : 0) * 3, f64);
+  copy_in(c.colors, s->colors_precomp, 3 * P, f64);
+  copy_in(c.opac, s->opacities, P, f64);
+  copy_in(c.scales, s->scales, 2 * P, f64);
+  copy_in(c.rots, s->rotations, 4 * P, f64);
+  copy_in(c.tprec, s->transMat_precomp, 9 * P, f64);
+  copy_in(c.vm, s->viewmatrix, 16, f64);
+  copy_in(c.pm, s->projmatrix, 16, f64);
+  copy_in(c.campos, s->campos, 3, f64);
+  copy_in(c.bg, s->bg, 3, f64);
+}
+
+template <typename T> int copy_out(const std::vector<T>& v, void* dst, size_t nbytes) {
+  if (nbytes != v.size() * sizeof(T)) return -(int)(v.size() * sizeof(T)) - 1;
+  if (nbytes) std::memcpy(dst, v.data(), nbytes);
+  return 0;
+}
+
+template <typename R> int get_named(Ctx<R>& c, const std::string& n, void* dst, size_t nbytes) {
+  if (n == "radii") return copy_out(c.radii, dst, nbytes);
+  if (n == "depths") return copy_out(c.depths, dst, nbytes);
+  if (n == "means2D") return copy_out(c.means2D, dst, nbytes);
+  if (n == "transMat") return copy_out(c.transMat, dst, nbytes);
+  if (n == "normal_opacity") return copy_out(c.normal_opacity, dst, nbytes);
+  if (n == "rgb") return copy_out(c.rgb, dst, nbytes);
+  if (n == "clamped") return copy_out(c.clamped, dst, nbytes);
+  if (n == "tiles_touched") return copy_out(c.tiles_touched, dst, nbytes);
+  if (n == "point_offsets") return copy_out(c.point_offsets, dst, nbytes);
+  if (n == "keys") return copy_out(c.keys, dst, nbytes);
+  if (n == "point_list") return copy_out(c.point_list, dst, nbytes);
+  if (n == "ranges") return copy_out(c.ranges, dst, nbytes);
+  if (n == "out_color") return copy_out(c.out_color, dst, nbytes);
+  if (n == "out_others") return copy_out(c.out_others, dst, nbytes);
+  if (n == "final_T") return copy_out(c.final_T, dst, nbytes);
+  if (n == "n_contrib") return copy_out(c.n_contrib, dst, nbytes);
+  if (n == "acc_dL_dtransMat") return copy_out(c.acc_transMat, dst, nbytes);
+  if (n == "acc_dL_dmean2D") return copy_out(c.acc_mean2D, dst, nbytes);
+  if (n == "acc_dL_dnormal") return copy_out(c.acc_normal, dst, nbytes);
+  if (n == "acc_dL_dopacity") return copy_out(c.acc_opacity, dst, nbytes);
+  if (n == "acc_dL_dcolors") return copy_out(c.acc_colors, dst, nbytes);
+  if (n == "dL_dmeans2D") return copy_out(c.g_mean2D, dst, nbytes);
+  if (n == "dL_dcolors") return copy_out(c.g_colors, dst, nbytes);
+  if (n == "dL_dopacity") return copy_out(c.g_opacity, dst, nbytes);
+  if (n == "dL_dmeans3D") return copy_out(c.g_means3D, dst, nbytes);
+  if (n == "dL_dtransMat") return copy_out(c.g_transMat, dst, nbytes);
+  if (n == "dL_dsh") return copy_out(c.g_sh, dst, nbytes);
+  if (n == "dL_dscales") return copy_out(c.g_scales, dst, nbytes);
+  if (n == "dL_drotations") return copy_out(c.g_rots, dst, nbytes);
+  if (n == "dL_dnormal") return copy_out(c.g_normal, dst, nbytes);
+  return -1;
+}
+}  // namespace
+
+extern "C" {
+
+void* sro_create(void) { return new Handle(); }
+void sro_destroy(void* h) { delete (Handle*)h; }
+
+void sro_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+int sro_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+float sro_expf(float x) { return sro_expf_impl(x); }
+
+void sro_expf_array(const float* x, float* y, int n) {
+  for (int i = 0; i < n; ++i) y[i] = sro_expf_impl(x[i]);
+}
+
+// SH -> RGB of n points (FP64 mode when f64 != 0), the colour step of the
+// forward preprocess on its own: sh is n x M x 3, means n x 3, campos 3.
+void sro_sh_rgb(int deg, int M, const void* sh, const void* means, const void* campos, int n, void* rgb,
+                uint8_t* clamped, int f64) {
+  auto run = [&](auto tag) {
+    using R = decltype(tag);
+    Ctx<R> c;
+    c.sc.D = deg;
+    c.sc.M = M;
+    copy_in(c.shs, sh, (size_t)n * M * 3, f64);
+    copy_in(c.means3D, means, (size_t)n * 3, f64);
+    copy_in(c.campos, campos, 3, f64);
+    for (int i = 0; i < n; ++i) {
+      const V3<R> v = color_from_sh(c, i, clamped + 3 * i);
+      R* o = (R*)rgb + 3 * i;
+      o[0] = v.x;
+      o[1] = v.y;
+      o[2] = v.z;
+    }
+  };
+  if (f64) run(0.0); else run(0.0f);
+}
+uint32_t sro_higher_msb(uint32_t n) { return higher_msb(n); }
+
+// Runs preprocess, binning and the forward blend.  Returns 0, 1 when a
+// surfel fails the near test although prefiltered is set (the reference
+// traps there, auxiliary.h:202-206), or -1 on bad arguments.
+int sro_forward(void* hv, const sro_scene_c* s, int f64) {
+  Handle* h = (Handle*)hv;
+  if (!h || !s || s->P < 0 || s->W <= 0 || s->H <= 0 || s->tile <= 0) return -1;
+  if ((s->shs == nullptr) == (s->colors_precomp == nullptr)) return -1;
+  if ((s->scales == nullptr || s->rotations == nullptr) == (s->transMat_precomp == nullptr)) return -1;
+  h->f64 = f64;
+  if (f64) {
+    load_scene(h->cd, s, 1);
+    run_forward(h->cd);
+    return h->cd.prefilter_violation;
+  }
+  load_scene(h->cf, s, 0);
+  run_forward(h->cf);
+  return h->cf.prefilter_violation;
+}
+
+int sro_num_rendered(void* hv) {
+  Handle* h = (Handle*)hv;
+  return h->f64 ? h->cd.L : h->cf.L;
+}
+
+// dL_dcolor (3,H,W) and dL_dothers (7,H,W) in the precision of the forward.
+int sro_backward(void* hv, const void* dL_dcolor, const void* dL_dothers) {
+  Handle* h = (Handle*)hv;
+  if (h->f64) run_backward(h->cd, (const double*)dL_dcolor, (const double*)dL_dothers);
+  else run_backward(h->cf, (const float*)dL_dcolor, (const float*)dL_dothers);
+  return 0;
+}
+
+// Copies a named internal array out; returns 0, or a negative value whose
+// magnitude minus one is the required byte count, or -1 for unknown names.
+int sro_get(void* hv, const char* name, void* dst, size_t nbytes) {
+  Handle* h = (Handle*)hv;
+  const std::string n(name);
+  return h->f64 ? get_named(h->cd, n, dst, nbytes) : get_named(h->cf, n, dst, nbytes);
+}
+
+// markVisible / checkFrustum, rasterizer_impl.cu:54-66,141-153.
+void sro_mark_visible(int P, const float* means3D, const float* viewmatrix, uint8_t* present) {
+  for (int i = 0; i < P; ++i) {
+    const float* m = means3D + 3 * i;
+    const float* vm = viewmatrix;
+    const float vz = ((vm[2] * m[0] + vm[6] * m[1]) + vm[10] * m[2]) + vm[14];
+    present[i] = !(vz <= 0.2f);
+  }
+}
+
+}  // extern "C"

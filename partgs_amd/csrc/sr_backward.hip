@@ -1,0 +1,485 @@
+// Backward kernels of the surfel rasteriser for gfx950 (CDNA4, wave64).
+//
+//   k_blend_bwd        one workgroup per tile, one lane per pixel, surfels
+//                      visited back to front (dsr/cuda_rasterizer/backward.cu:143-440).
+//                      Each lane forms its pixel's 18 per-surfel partial
+//                      gradients; the wave sums them across its 64 lanes and
+//                      lanes 0..17 issue one FP32 atomic each, instead of up
+//                      to 64 x 13 atomics per surfel.
+//   k_preprocess_bwd   per surfel: AABB-centre adjoint, dL/dT -> dL/d(mean,
+//                      scale, rotation), SH backward, and the densification
+//                      statistic written into dL_dmean2D (backward.cu:443-630).
+#include "sr_device.h"
+#include "sr_internal.h"
+
+namespace sr {
+
+// Partial-gradient slots: 0..8 dL_dtransMat, 9..10 dL_dmean2D.xy,
+// 11..13 dL_dnormal, 14 dL_dopacity, 15..17 dL_dcolors.
+constexpr int kNumG = 18;
+
+SR_DEV void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+SR_DEV float* grad_slot(const BlendBwdArgs& a, uint32_t id, int k) {
+  if (k < 9) return a.dL_dtransMat + 9 * (size_t)id + k;
+  if (k < 11) return a.dL_dmean2D + 3 * (size_t)id + (k - 9);
+  if (k < 14) return a.dL_dnormal + 3 * (size_t)id + (k - 11);
+  if (k == 14) return a.dL_dopacity + id;
+  return a.dL_dcolors + 3 * (size_t)id + (k - 15);
+}
+
+template <int B>
+__global__ __launch_bounds__(B * B) void k_blend_bwd(BlendBwdArgs a) {
+  constexpr int BS = B * B;
+  __shared__ float4 s_q0[BS], s_q1[BS], s_q2[BS], s_q3[BS], s_q4[BS];
+  __shared__ uint32_t s_id[BS];
+  const int tile = blockIdx.x;
+  const int tx = tile % a.gx, ty = tile / a.gx;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const uint32_t pxi = (uint32_t)(tx * B + tid % B), pyi = (uint32_t)(ty * B + tid / B);
+  const bool inside = pxi < (uint32_t)a.W && pyi < (uint32_t)a.H;
+  const size_t HW = (size_t)a.H * a.W, pid = (size_t)a.W * pyi + pxi;
+  const uint32_t rbeg = a.ranges[2 * tile], rend = a.ranges[2 * tile + 1];
+  const int rounds = (int)((rend - rbeg + BS - 1) / BS);
+  int toDo = (int)(rend - rbeg);
+  const float pixx = (float)pxi, pixy = (float)pyi;
+
+  const float T_final = inside ? a.final_T[pid] : 0.0f;
+  float T = T_final;
+  uint32_t contributor = (uint32_t)toDo;
+  const int last_contributor = inside ? (int)a.n_contrib[pid] : 0;
+  const int median_contributor = inside ? (int)a.n_contrib[pid + HW] : 0;
+  float dLp0 = 0.0f, dLp1 = 0.0f, dLp2 = 0.0f;
+  float dL_ddepth = 0.0f, dL_daccum = 0.0f, dL_dreg = 0.0f, dL_dmedian = 0.0f;
+  float dLn0 = 0.0f, dLn1 = 0.0f, dLn2 = 0.0f;
+  float final_D = 0.0f, final_D2 = 0.0f;
+  if (inside) {
+    dLp0 = a.dL_dpix[pid];
+    dLp1 = a.dL_dpix[HW + pid];
+    dLp2 = a.dL_dpix[2 * HW + pid];
+    dL_ddepth = a.dL_dothers[pid];
+    dL_daccum = a.dL_dothers[HW + pid];
+    dLn0 = a.dL_dothers[2 * HW + pid];
+    dLn1 = a.dL_dothers[3 * HW + pid];
+    dLn2 = a.dL_dothers[4 * HW + pid];
+    dL_dmedian = a.dL_dothers[5 * HW + pid];
+    dL_dreg = a.dL_dothers[6 * HW + pid];
+    final_D = a.final_T[pid + HW];
+    final_D2 = a.final_T[pid + 2 * HW];
+  }
+  const float final_A = 1.0f - T_final;
+  float acc_r0 = 0.0f, acc_r1 = 0.0f, acc_r2 = 0.0f;
+  float last_c0 = 0.0f, last_c1 = 0.0f, last_c2 = 0.0f;
+  float last_depth = 0.0f, accum_depth_rec = 0.0f, accum_alpha_rec = 0.0f;
+  float last_n0 = 0.0f, last_n1 = 0.0f, last_n2 = 0.0f;
+  float acc_n0 = 0.0f, acc_n1 = 0.0f, acc_n2 = 0.0f;
+  float last_dL_dT = 0.0f, last_alpha = 0.0f;
+  const float mcoef = kFar / (kFar - kNear);
+  const float dmd_num = kFar * kNear, dmd_den0 = kFar - kNear;
+  float bg_dot = 0.0f;
+  bg_dot += a.bg[0] * dLp0;
+  bg_dot += a.bg[1] * dLp1;
+  bg_dot += a.bg[2] * dLp2;
+
+  for (int i = 0; i < rounds; i++, toDo -= BS) {
+    __syncthreads();
+    const uint32_t progress = (uint32_t)(i * BS + tid);
+    if (rbeg + progress < rend) {
+      const uint32_t id = a.point_list[rend - progress - 1];
+      s_id[tid] = id;
+      const float4* r = a.rec + (size_t)kRecQuads * id;
+      s_q0[tid] = r[0];
+      s_q1[tid] = r[1];
+      s_q2[tid] = r[2];
+      s_q3[tid] = r[3];
+      s_q4[tid] = r[4];
+    }
+    __syncthreads();
+    const int n = min(BS, toDo);
+    for (int j = 0; j < n; j++) {
+      float g[kNumG];
+#pragma unroll
+      for (int k = 0; k < kNumG; ++k) g[k] = 0.0f;
+      bool has = false, brA = false, brB = false;
+      if (inside) {
+        contributor--;
+        do {
+          if (contributor >= (uint32_t)last_contributor) break;
+          const float4 q0 = s_q0[j], q1 = s_q1[j], q2 = s_q2[j];
+          const float Tux = q0.x, Tuy = q0.y, Tuz = q0.z, Tvx = q0.w, Tvy = q1.x, Tvz = q1.y;
+          const float Twx = q1.z, Twy = q1.w, Twz = q2.x;
+          const float kx = pixx * Twx - Tux, ky = pixx * Twy - Tuy, kz = pixx * Twz - Tuz;
+          const float lx = pixy * Twx - Tvx, ly = pixy * Twy - Tvy, lz = pixy * Twz - Tvz;
+          const float px = ky * lz - kz * ly, py = kz * lx - kx * lz, pz = kx * ly - ky * lx;
+          if (pz == 0.0f) break;
+          const float sx = px / pz, sy = py / pz;
+          const float rho3d = sx * sx + sy * sy;
+          const float dx = q2.y - pixx, dy = q2.z - pixy;
+          const float rho2d = 2.0f * (dx * dx + dy * dy);
+          const float rho = fmin_(rho3d, rho2d);
+          const float c_d = (rho3d <= rho2d) ? (sx * Twx + sy * Twy) + Twz : Twz;
+          if (c_d < kNear) break;
+          const float power = -0.5f * rho;
+          if (power > 0.0f) break;
+          const float opa = q2.w;
+          const float G = exp_pinned(power);
+          const float alpha = fmin_(0.99f, opa * G);
+          if (alpha < 1.0f / 255.0f) break;
+          has = true;
+          const float4 q3 = s_q3[j], q4 = s_q4[j];
+          T = T / (1.0f - alpha);
+          const float w = alpha * T;
+          float dL_dalpha = 0.0f;
+          acc_r0 = last_alpha * last_c0 + (1.0f - last_alpha) * acc_r0;
+          last_c0 = q3.w;
+          dL_dalpha += (q3.w - acc_r0) * dLp0;
+          g[15] = w * dLp0;
+          acc_r1 = last_alpha * last_c1 + (1.0f - last_alpha) * acc_r1;
+          last_c1 = q4.x;
+          dL_dalpha += (q4.x - acc_r1) * dLp1;
+          g[16] = w * dLp1;
+          acc_r2 = last_alpha * last_c2 + (1.0f - last_alpha) * acc_r2;
+          last_c2 = q4.y;
+          dL_dalpha += (q4.y - acc_r2) * dLp2;
+          g[17] = w * dLp2;
+          float dL_dz = 0.0f, dL_dweight = 0.0f;
+          const float m_d = mcoef * (1.0f - kNear / c_d);
+          const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
+          if ((int)contributor == median_contributor - 1) dL_dz += dL_dmedian;
+          dL_dweight += ((final_D2 + (m_d * m_d) * final_A) - (2.0f * m_d) * final_D) * dL_dreg;
+          dL_dalpha += dL_dweight - last_dL_dT;
+          last_dL_dT = dL_dweight * alpha + (1.0f - alpha) * last_dL_dT;
+          const float dL_dmd = ((2.0f * (T * alpha)) * (m_d * final_A - final_D)) * dL_dreg;
+          dL_dz += dL_dmd * dmd_dd;
+          accum_depth_rec = last_alpha * last_depth + (1.0f - last_alpha) * accum_depth_rec;
+          last_depth = c_d;
+          dL_dalpha += (c_d - accum_depth_rec) * dL_ddepth;
+          accum_alpha_rec = last_alpha + (1.0f - last_alpha) * accum_alpha_rec;
+          dL_dalpha += (1.0f - accum_alpha_rec) * dL_daccum;
+          acc_n0 = last_alpha * last_n0 + (1.0f - last_alpha) * acc_n0;
+          last_n0 = q3.x;
+          dL_dalpha += (q3.x - acc_n0) * dLn0;
+          g[11] = (alpha * T) * dLn0;
+          acc_n1 = last_alpha * last_n1 + (1.0f - last_alpha) * acc_n1;
+          last_n1 = q3.y;
+          dL_dalpha += (q3.y - acc_n1) * dLn1;
+          g[12] = (alpha * T) * dLn1;
+          acc_n2 = last_alpha * last_n2 + (1.0f - last_alpha) * acc_n2;
+          last_n2 = q3.z;
+          dL_dalpha += (q3.z - acc_n2) * dLn2;
+          g[13] = (alpha * T) * dLn2;
+          dL_dalpha *= T;
+          last_alpha = alpha;
+          dL_dalpha += (-T_final / (1.0f - alpha)) * bg_dot;
+          const float dL_dG = opa * dL_dalpha;
+          dL_dz += (alpha * T) * dL_ddepth;
+          if (rho3d <= rho2d) {
+            brA = true;
+            const float dL_dsx = (dL_dG * -G) * sx + dL_dz * Twx;
+            const float dL_dsy = (dL_dG * -G) * sy + dL_dz * Twy;
+            const float dsx_pz = dL_dsx / pz, dsy_pz = dL_dsy / pz;
+            const float dpx = dsx_pz, dpy = dsy_pz, dpz = -(dsx_pz * sx + dsy_pz * sy);
+            // dL_dk = cross(l, dL_dp), dL_dl = cross(dL_dp, k)
+            const float dkx = ly * dpz - lz * dpy, dky = lz * dpx - lx * dpz, dkz = lx * dpy - ly * dpx;
+            const float dlx = dpy * kz - dpz * ky, dly = dpz * kx - dpx * kz, dlz = dpx * ky - dpy * kx;
+            g[0] = -dkx;
+            g[1] = -dky;
+            g[2] = -dkz;
+            g[3] = -dlx;
+            g[4] = -dly;
+            g[5] = -dlz;
+            g[6] = (pixx * dkx + pixy * dlx) + dL_dz * sx;
+            g[7] = (pixx * dky + pixy * dly) + dL_dz * sy;
+            g[8] = (pixx * dkz + pixy * dlz) + dL_dz * 1.0f;
+          } else {
+            brB = true;
+            const float dG_ddelx = (-G * 2.0f) * dx;
+            const float dG_ddely = (-G * 2.0f) * dy;
+            g[9] = dL_dG * dG_ddelx;
+            g[10] = dL_dG * dG_ddely;
+            g[8] = dL_dz;
+          }
+          g[14] = G * dL_dalpha;
+        } while (false);
+      }
+      const uint64_t m = __ballot(has);
+      if (m == 0) continue;
+      const uint32_t id = s_id[j];
+      if (__popcll(m) == 1) {
+        if (has) {
+          if (brA) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) atomic_add_f32(grad_slot(a, id, k), g[k]);
+          } else {
+            atomic_add_f32(grad_slot(a, id, 8), g[8]);
+            atomic_add_f32(grad_slot(a, id, 9), g[9]);
+            atomic_add_f32(grad_slot(a, id, 10), g[10]);
+          }
+#pragma unroll
+          for (int k = 11; k < kNumG; ++k) atomic_add_f32(grad_slot(a, id, k), g[k]);
+        }
+        continue;
+      }
+      const bool anyA = __ballot(brA) != 0ull, anyB = __ballot(brB) != 0ull;
+      if (anyA) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) g[k] = wave_sum(g[k]);
+      }
+      g[8] = wave_sum(g[8]);
+      if (anyB) {
+        g[9] = wave_sum(g[9]);
+        g[10] = wave_sum(g[10]);
+      }
+#pragma unroll
+      for (int k = 11; k < kNumG; ++k) g[k] = wave_sum(g[k]);
+      // lane k adds slot k
+      float v = 0.0f;
+#pragma unroll
+      for (int k = 0; k < kNumG; ++k) v = (lane == k) ? g[k] : v;
+      const bool skip = (lane < 8 && !anyA) || ((lane == 9 || lane == 10) && !anyB) || lane >= kNumG;
+      if (!skip) atomic_add_f32(grad_slot(a, id, lane), v);
+    }
+  }
+}
+
+hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s) {
+  if (a.T == 0) return hipSuccess;
+  switch (a.B) {
+    case 8: hipLaunchKernelGGL(k_blend_bwd<8>, dim3(a.T), dim3(64), 0, s, a); break;
+    case 16: hipLaunchKernelGGL(k_blend_bwd<16>, dim3(a.T), dim3(256), 0, s, a); break;
+    case 32: hipLaunchKernelGGL(k_blend_bwd<32>, dim3(a.T), dim3(1024), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+struct F3 {
+  float x, y, z;
+};
+SR_DEV F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+SR_DEV F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+SR_DEV F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+SR_DEV F3 mul(F3 v, float s) { return f3(v.x * s, v.y * s, v.z * s); }
+SR_DEV F3 had(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+SR_DEV float dot3(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+SR_DEV float comp(F3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+// SH backward, dsr/cuda_rasterizer/backward.cu:20-139 (same association).
+SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, float* dsh) {
+  const float len = sqrtf((dir_orig.x * dir_orig.x + dir_orig.y * dir_orig.y) + dir_orig.z * dir_orig.z);
+  const F3 dir = f3(dir_orig.x / len, dir_orig.y / len, dir_orig.z / len);
+  auto S = [&](int k) { return f3(sh[3 * k], sh[3 * k + 1], sh[3 * k + 2]); };
+  auto put = [&](int k, float c) {
+    dsh[3 * k] = c * dL_dRGB.x;
+    dsh[3 * k + 1] = c * dL_dRGB.y;
+    dsh[3 * k + 2] = c * dL_dRGB.z;
+  };
+  F3 dx = f3(0.0f, 0.0f, 0.0f), dy = dx, dz = dx;
+  const float x = dir.x, y = dir.y, z = dir.z;
+  put(0, kSH_C0);
+  if (deg > 0) {
+    put(1, -kSH_C1 * y);
+    put(2, kSH_C1 * z);
+    put(3, -kSH_C1 * x);
+    dx = -kSH_C1 * S(3);
+    dy = -kSH_C1 * S(1);
+    dz = kSH_C1 * S(2);
+    if (deg > 1) {
+      const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+      put(4, kSH_C2_0 * xy);
+      put(5, kSH_C2_1 * yz);
+      put(6, kSH_C2_2 * ((2.0f * zz - xx) - yy));
+      put(7, kSH_C2_3 * xz);
+      put(8, kSH_C2_4 * (xx - yy));
+      dx = dx + ((((kSH_C2_0 * y) * S(4) + ((kSH_C2_2 * 2.0f) * -x) * S(6)) + (kSH_C2_3 * z) * S(7)) +
+                 ((kSH_C2_4 * 2.0f) * x) * S(8));
+      dy = dy + ((((kSH_C2_0 * x) * S(4) + (kSH_C2_1 * z) * S(5)) + ((kSH_C2_2 * 2.0f) * -y) * S(6)) +
+                 ((kSH_C2_4 * 2.0f) * -y) * S(8));
+      dz = dz + (((kSH_C2_1 * y) * S(5) + (((kSH_C2_2 * 2.0f) * 2.0f) * z) * S(6)) + (kSH_C2_3 * x) * S(7));
+      if (deg > 2) {
+        put(9, (kSH_C3_0 * y) * (3.0f * xx - yy));
+        put(10, (kSH_C3_1 * xy) * z);
+        put(11, (kSH_C3_2 * y) * ((4.0f * zz - xx) - yy));
+        put(12, (kSH_C3_3 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy));
+        put(13, (kSH_C3_4 * x) * ((4.0f * zz - xx) - yy));
+        put(14, (kSH_C3_5 * z) * (xx - yy));
+        put(15, (kSH_C3_6 * x) * (xx - 3.0f * yy));
+        F3 ax = mul(mul(mul(kSH_C3_0 * S(9), 3.0f), 2.0f), xy);
+        ax = ax + mul(kSH_C3_1 * S(10), yz);
+        ax = ax + mul(mul(kSH_C3_2 * S(11), -2.0f), xy);
+        ax = ax + mul(mul(mul(kSH_C3_3 * S(12), -3.0f), 2.0f), xz);
+        ax = ax + mul(kSH_C3_4 * S(13), ((-3.0f * xx + 4.0f * zz) - yy));
+        ax = ax + mul(mul(kSH_C3_5 * S(14), 2.0f), xz);
+        ax = ax + mul(mul(kSH_C3_6 * S(15), 3.0f), (xx - yy));
+        F3 ay = mul(mul(kSH_C3_0 * S(9), 3.0f), (xx - yy));
+        ay = ay + mul(kSH_C3_1 * S(10), xz);
+        ay = ay + mul(kSH_C3_2 * S(11), ((-3.0f * yy + 4.0f * zz) - xx));
+        ay = ay + mul(mul(mul(kSH_C3_3 * S(12), -3.0f), 2.0f), yz);
+        ay = ay + mul(mul(kSH_C3_4 * S(13), -2.0f), xy);
+        ay = ay + mul(mul(kSH_C3_5 * S(14), -2.0f), yz);
+        ay = ay + mul(mul(mul(kSH_C3_6 * S(15), -3.0f), 2.0f), xy);
+        F3 az = mul(kSH_C3_1 * S(10), xy);
+        az = az + mul(mul(mul(kSH_C3_2 * S(11), 4.0f), 2.0f), yz);
+        az = az + mul(mul(kSH_C3_3 * S(12), 3.0f), ((2.0f * zz - xx) - yy));
+        az = az + mul(mul(mul(kSH_C3_4 * S(13), 4.0f), 2.0f), xz);
+        az = az + mul(kSH_C3_5 * S(14), (xx - yy));
+        dx = dx + ax;
+        dy = dy + ay;
+        dz = dz + az;
+      }
+    }
+  }
+  const F3 dL_ddir = f3(dot3(dx, dL_dRGB), dot3(dy, dL_dRGB), dot3(dz, dL_dRGB));
+  // dnormvdv, auxiliary.h:128-138
+  const F3 v = dir_orig, dv = dL_ddir;
+  const float sum2 = (v.x * v.x + v.y * v.y) + v.z * v.z;
+  const float invsum32 = 1.0f / sqrtf((sum2 * sum2) * sum2);
+  F3 r;
+  r.x = ((((sum2 - v.x * v.x) * dv.x) - (v.y * v.x) * dv.y) - (v.z * v.x) * dv.z) * invsum32;
+  r.y = (((-v.x * v.y) * dv.x + (sum2 - v.y * v.y) * dv.y) - (v.z * v.y) * dv.z) * invsum32;
+  r.z = (((-v.x * v.z) * dv.x - (v.y * v.z) * dv.y) + (sum2 - v.z * v.z) * dv.z) * invsum32;
+  return r;
+}
+
+// Backward preprocess, dsr/cuda_rasterizer/backward.cu:443-630.
+__global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.P || !(a.radii[idx] > 0)) return;
+  const float focal_x = (float)a.W / (2.0f * a.tanx);
+  const float focal_y = (float)a.H / (2.0f * a.tany);
+  const int W = f2i((focal_x * a.tanx) * 2.0f);
+  const int H = f2i((focal_y * a.tany) * 2.0f);
+  const float* pm = a.pm;
+  const float* vm = a.vm;
+  float* gT = a.dL_dtransMat + 9 * (size_t)idx;
+  const float m0 = a.means3D[3 * idx], m1 = a.means3D[3 * idx + 1], m2 = a.means3D[3 * idx + 2];
+  const bool precomp = a.tprec != nullptr;
+  F3 T0, T1, T2, nrm = f3(0.0f, 0.0f, 0.0f), Rc0, Rc1;
+  float qw = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f, scx = 0.0f, scy = 0.0f;
+  float P4[3][4];
+  if (precomp) {
+    const float* t = a.tprec + 9 * (size_t)idx;
+    T0 = f3(t[0], t[1], t[2]);
+    T1 = f3(t[3], t[4], t[5]);
+    T2 = f3(t[6], t[7], t[8]);
+  } else {
+    const float* q = a.rots + 4 * idx;
+    const float q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    const float n2 = ((q3 * q3 + q0 * q0) + q1 * q1) + q2 * q2;
+    const float s = 1.0f / sqrtf(n2);
+    qw = q0 * s;
+    qx = q1 * s;
+    qy = q2 * s;
+    qz = q3 * s;
+    const float w = qw, x = qx, y = qy, z = qz;
+    Rc0 = f3(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + w * z), 2.0f * (x * z - w * y));
+    Rc1 = f3(2.0f * (x * y - w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + w * x));
+    const F3 Rc2 = f3(2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y));
+    scx = a.scales[2 * idx];
+    scy = a.scales[2 * idx + 1];
+    const F3 L0c = (1.0f * scx) * Rc0, L1c = (1.0f * scy) * Rc1;
+    const float hw = (float)W / 2.0f, hw1 = (float)(W - 1) / 2.0f;
+    const float hh = (float)H / 2.0f, hh1 = (float)(H - 1) / 2.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      P4[0][i] = pm[4 * i] * hw + pm[4 * i + 3] * hw1;
+      P4[1][i] = pm[4 * i + 1] * hh + pm[4 * i + 3] * hh1;
+      P4[2][i] = pm[4 * i + 3];
+    }
+    auto dotM = [&](const F3& v, float v4, int k) {
+      return ((v.x * P4[k][0] + v.y * P4[k][1]) + v.z * P4[k][2]) + v4 * P4[k][3];
+    };
+    const F3 pc = f3(m0, m1, m2);
+    T0 = f3(dotM(L0c, 0.0f, 0), dotM(L1c, 0.0f, 0), dotM(pc, 1.0f, 0));
+    T1 = f3(dotM(L0c, 0.0f, 1), dotM(L1c, 0.0f, 1), dotM(pc, 1.0f, 1));
+    T2 = f3(dotM(L0c, 0.0f, 2), dotM(L1c, 0.0f, 2), dotM(pc, 1.0f, 2));
+    nrm = f3((vm[0] * Rc2.x + vm[4] * Rc2.y) + vm[8] * Rc2.z, (vm[1] * Rc2.x + vm[5] * Rc2.y) + vm[9] * Rc2.z,
+             (vm[2] * Rc2.x + vm[6] * Rc2.y) + vm[10] * Rc2.z);
+  }
+  F3 dT0 = f3(gT[0], gT[1], gT[2]), dT1 = f3(gT[3], gT[4], gT[5]), dT2 = f3(gT[6], gT[7], gT[8]);
+  const float gmx = a.dL_dmean2D[3 * idx], gmy = a.dL_dmean2D[3 * idx + 1];
+  if (gmx != 0.0f || gmy != 0.0f) {
+    const F3 tv = f3(9.0f, 9.0f, -1.0f);
+    const float d = (tv.x * (T2.x * T2.x) + tv.y * (T2.y * T2.y)) + tv.z * (T2.z * T2.z);
+    const float invd = 1.0f / d;
+    const F3 f = f3(tv.x * invd, tv.y * invd, tv.z * invd);
+    const F3 dL_dT0 = had(gmx * f, T2);
+    const F3 dL_dT1 = had(gmy * f, T2);
+    F3 dL_dT3 = had(gmx * f, T0) + had(gmy * f, T1);
+    const F3 dL_df = had(gmx * T0, T2) + had(gmy * T1, T2);
+    const float dotdf = (dL_df.x * f.x + dL_df.y * f.y) + dL_df.z * f.z;
+    const float dL_dd = (float)((double)dotdf * (-1.0 / (double)d));
+    const F3 dd_dT3 = f3((tv.x * T2.x) * 2.0f, (tv.y * T2.y) * 2.0f, (tv.z * T2.z) * 2.0f);
+    dL_dT3 = dL_dT3 + dL_dd * dd_dT3;
+    dT0 = dT0 + dL_dT0;
+    dT1 = dT1 + dL_dT1;
+    dT2 = dT2 + dL_dT3;
+    if (precomp) {
+      gT[0] = dT0.x; gT[1] = dT0.y; gT[2] = dT0.z;
+      gT[3] = dT1.x; gT[4] = dT1.y; gT[5] = dT1.z;
+      gT[6] = dT2.x; gT[7] = dT2.y; gT[8] = dT2.z;
+    }
+  }
+  if (!precomp) {
+    float dM[3][3];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        dM[cc][i] = (P4[0][i] * comp(dT0, cc) + P4[1][i] * comp(dT1, cc)) + P4[2][i] * comp(dT2, cc);
+    const float* gn = a.dL_dnormal + 3 * (size_t)idx;
+    F3 dL_dtn = f3((vm[0] * gn[0] + vm[1] * gn[1]) + vm[2] * gn[2], (vm[4] * gn[0] + vm[5] * gn[1]) + vm[6] * gn[2],
+                   (vm[8] * gn[0] + vm[9] * gn[1]) + vm[10] * gn[2]);
+    const float pvx = ((vm[0] * m0 + vm[4] * m1) + vm[8] * m2) + vm[12];
+    const float pvy = ((vm[1] * m0 + vm[5] * m1) + vm[9] * m2) + vm[13];
+    const float pvz = ((vm[2] * m0 + vm[6] * m1) + vm[10] * m2) + vm[14];
+    const float cosv = -((pvx * nrm.x + pvy * nrm.y) + pvz * nrm.z);
+    const float mult = cosv > 0.0f ? 1.0f : -1.0f;
+    dL_dtn = mult * dL_dtn;
+    const F3 dRS0 = f3(dM[0][0], dM[0][1], dM[0][2]);
+    const F3 dRS1 = f3(dM[1][0], dM[1][1], dM[1][2]);
+    const F3 dR0 = f3(dRS0.x * scx, dRS0.y * scx, dRS0.z * scx);
+    const F3 dR1 = f3(dRS1.x * scy, dRS1.y * scy, dRS1.z * scy);
+    const F3 dR2 = dL_dtn;
+    const float vR[3][3] = {{dR0.x, dR0.y, dR0.z}, {dR1.x, dR1.y, dR1.z}, {dR2.x, dR2.y, dR2.z}};
+    const float w = qw, x = qx, y = qy, z = qz;
+    float* gq = a.dL_drots + 4 * (size_t)idx;
+    gq[0] = 2.0f * ((x * (vR[1][2] - vR[2][1]) + y * (vR[2][0] - vR[0][2])) + z * (vR[0][1] - vR[1][0]));
+    gq[1] = 2.0f * ((((-2.0f * x) * (vR[1][1] + vR[2][2]) + y * (vR[0][1] + vR[1][0])) + z * (vR[0][2] + vR[2][0])) +
+                    w * (vR[1][2] - vR[2][1]));
+    gq[2] = 2.0f * (((x * (vR[0][1] + vR[1][0]) - (2.0f * y) * (vR[0][0] + vR[2][2])) + z * (vR[1][2] + vR[2][1])) +
+                    w * (vR[2][0] - vR[0][2]));
+    gq[3] = 2.0f * (((x * (vR[0][2] + vR[2][0]) + y * (vR[1][2] + vR[2][1])) - (2.0f * z) * (vR[0][0] + vR[1][1])) +
+                    w * (vR[0][1] - vR[1][0]));
+    a.dL_dscales[2 * idx] = (dRS0.x * Rc0.x + dRS0.y * Rc0.y) + dRS0.z * Rc0.z;
+    a.dL_dscales[2 * idx + 1] = (dRS1.x * Rc1.x + dRS1.y * Rc1.y) + dRS1.z * Rc1.z;
+    a.dL_dmean3D[3 * idx] = dM[2][0];
+    a.dL_dmean3D[3 * idx + 1] = dM[2][1];
+    a.dL_dmean3D[3 * idx + 2] = dM[2][2];
+  }
+  if (a.shs != nullptr) {
+    F3 dRGB = f3(a.dL_dcolors[3 * idx], a.dL_dcolors[3 * idx + 1], a.dL_dcolors[3 * idx + 2]);
+    const uchar4 cl = reinterpret_cast<const uchar4*>(a.clamped)[idx];
+    if (cl.x) dRGB.x = 0.0f;
+    if (cl.y) dRGB.y = 0.0f;
+    if (cl.z) dRGB.z = 0.0f;
+    const F3 dir_orig = f3(m0 - a.campos[0], m1 - a.campos[1], m2 - a.campos[2]);
+    const F3 dm = sh_backward(a.D, a.M, a.shs + (size_t)idx * a.M * 3, dir_orig, dRGB,
+                              a.dL_dsh + (size_t)idx * a.M * 3);
+    a.dL_dmean3D[3 * idx] += dm.x;
+    a.dL_dmean3D[3 * idx + 1] += dm.y;
+    a.dL_dmean3D[3 * idx + 2] += dm.z;
+  }
+  const float depth = a.rec[(size_t)kRecQuads * idx + 2].x;  // Tw.z = transMat[8]
+  a.dL_dmean2D[3 * idx] = (float)(((double)(gT[2] * depth) * 0.5) * (double)(float)W);
+  a.dL_dmean2D[3 * idx + 1] = (float)(((double)(gT[5] * depth) * 0.5) * (double)(float)H);
+}
+
+hipError_t launch_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s) {
+  if (a.P == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_preprocess_bwd, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace sr

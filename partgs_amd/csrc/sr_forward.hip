@@ -1,0 +1,512 @@
+// Forward kernels of the surfel rasteriser for gfx950 (CDNA4, wave64).
+//
+//   k_preprocess   per surfel: near test, ray-splat homography T, camera-facing
+//                  normal, screen AABB and radius, SH -> RGB, and the per-tile
+//                  instance histogram (one atomic per covered tile).
+//   k_scan_tiles   one workgroup: exclusive scan of the tile histogram into
+//                  tile start offsets, tile ranges and the instance count.
+//   k_scatter      per surfel: writes a 64-bit key (depth bits << 32 | surfel
+//                  id) for every covered tile into that tile's bucket.
+//   k_sort_tiles   one workgroup per tile: sorts its bucket in LDS (bitonic,
+//                  up to 4096 keys; larger tiles sort 4096-key chunks and
+//                  merge them in global memory) and emits point_list.
+//   k_blend_fwd    one workgroup per tile, one lane per pixel: rounds of
+//                  cooperative fetches of packed surfel records into LDS, then
+//                  front-to-back alpha blending of colour, depth, alpha,
+//                  normal, median depth and depth distortion.
+//
+// Sorting each tile bucket by (depth bits, surfel id) gives exactly the order
+// of the reference's stable radix sort over (tile, depth) keys emitted in
+// surfel order (dsr/cuda_rasterizer/rasterizer_impl.cu:70-111, 301-309).
+#include "sr_device.h"
+#include "sr_internal.h"
+
+namespace sr {
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_f32(float* p, float v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+static unsigned fill_blocks(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  if (b == 0) b = 1;
+  return (unsigned)b;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks(n)), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(n)), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+struct F3 {
+  float x, y, z;
+};
+SR_DEV F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+SR_DEV F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+SR_DEV F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+SR_DEV F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+
+// SH -> RGB, dsr/cuda_rasterizer/forward.cu:20-71 (same association).
+SR_DEV F3 color_from_sh(int deg, int M, const float* sh, F3 dir, uint8_t* clamped) {
+  auto S = [&](int k) { return f3(sh[3 * k], sh[3 * k + 1], sh[3 * k + 2]); };
+  F3 res = kSH_C0 * S(0);
+  if (deg > 0) {
+    const float x = dir.x, y = dir.y, z = dir.z;
+    res = res - (kSH_C1 * y) * S(1);
+    res = res + (kSH_C1 * z) * S(2);
+    res = res - (kSH_C1 * x) * S(3);
+    if (deg > 1) {
+      const float xx = x * x, yy = y * y, zz = z * z;
+      const float xy = x * y, yz = y * z, xz = x * z;
+      res = res + (kSH_C2_0 * xy) * S(4);
+      res = res + (kSH_C2_1 * yz) * S(5);
+      res = res + (kSH_C2_2 * ((2.0f * zz - xx) - yy)) * S(6);
+      res = res + (kSH_C2_3 * xz) * S(7);
+      res = res + (kSH_C2_4 * (xx - yy)) * S(8);
+      if (deg > 2) {
+        res = res + ((kSH_C3_0 * y) * (3.0f * xx - yy)) * S(9);
+        res = res + ((kSH_C3_1 * xy) * z) * S(10);
+        res = res + ((kSH_C3_2 * y) * ((4.0f * zz - xx) - yy)) * S(11);
+        res = res + ((kSH_C3_3 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy)) * S(12);
+        res = res + ((kSH_C3_4 * x) * ((4.0f * zz - xx) - yy)) * S(13);
+        res = res + ((kSH_C3_5 * z) * (xx - yy)) * S(14);
+        res = res + ((kSH_C3_6 * x) * (xx - 3.0f * yy)) * S(15);
+      }
+    }
+  }
+  res = f3(res.x + 0.5f, res.y + 0.5f, res.z + 0.5f);
+  clamped[0] = res.x < 0.0f;
+  clamped[1] = res.y < 0.0f;
+  clamped[2] = res.z < 0.0f;
+  return f3(fmax_(res.x, 0.0f), fmax_(res.y, 0.0f), fmax_(res.z, 0.0f));
+}
+
+// Forward preprocess, dsr/cuda_rasterizer/forward.cu:75-145, 148-251.
+__global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.P) return;
+  a.radii[idx] = 0;
+  a.tiles[idx] = 0;
+  const float m0 = a.means3D[3 * idx], m1 = a.means3D[3 * idx + 1], m2 = a.means3D[3 * idx + 2];
+  const float* vm = a.vm;
+  const float* pm = a.pm;
+  const float vx = ((vm[0] * m0 + vm[4] * m1) + vm[8] * m2) + vm[12];
+  const float vy = ((vm[1] * m0 + vm[5] * m1) + vm[9] * m2) + vm[13];
+  const float vz = ((vm[2] * m0 + vm[6] * m1) + vm[10] * m2) + vm[14];
+  if (vz <= 0.2f) {
+    if (a.prefiltered) a.counters[1] = kErrPrefiltered;
+    return;
+  }
+  F3 Tu, Tv, Tw, nrm;
+  if (a.tprec == nullptr) {
+    const float* q = a.rots + 4 * idx;
+    const float q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    const float n2 = ((q3 * q3 + q0 * q0) + q1 * q1) + q2 * q2;
+    const float s = 1.0f / sqrtf(n2);
+    const float w = q0 * s, x = q1 * s, y = q2 * s, z = q3 * s;
+    const F3 c0 = f3(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + w * z), 2.0f * (x * z - w * y));
+    const F3 c1 = f3(2.0f * (x * y - w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + w * x));
+    const F3 c2 = f3(2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y));
+    const float sx = a.mod * a.scales[2 * idx], sy = a.mod * a.scales[2 * idx + 1];
+    const F3 L0 = sx * c0, L1 = sy * c1, L2 = c2;
+    auto rowdot = [&](const F3& v, int cc) { return (v.x * pm[cc] + v.y * pm[4 + cc]) + v.z * pm[8 + cc]; };
+    const float a00 = rowdot(L0, 0), a01 = rowdot(L0, 1), a03 = rowdot(L0, 3);
+    const float a10 = rowdot(L1, 0), a11 = rowdot(L1, 1), a13 = rowdot(L1, 3);
+    const F3 pc = f3(m0, m1, m2);
+    const float a20 = rowdot(pc, 0) + pm[12], a21 = rowdot(pc, 1) + pm[13], a23 = rowdot(pc, 3) + pm[15];
+    const float hw = (float)a.W / 2.0f, hw1 = (float)(a.W - 1) / 2.0f;
+    const float hh = (float)a.H / 2.0f, hh1 = (float)(a.H - 1) / 2.0f;
+    Tu = f3(a00 * hw + a03 * hw1, a10 * hw + a13 * hw1, a20 * hw + a23 * hw1);
+    Tv = f3(a01 * hh + a03 * hh1, a11 * hh + a13 * hh1, a21 * hh + a23 * hh1);
+    Tw = f3(a03, a13, a23);
+    nrm = f3((vm[0] * L2.x + vm[4] * L2.y) + vm[8] * L2.z, (vm[1] * L2.x + vm[5] * L2.y) + vm[9] * L2.z,
+             (vm[2] * L2.x + vm[6] * L2.y) + vm[10] * L2.z);
+  } else {
+    const float* t = a.tprec + 9 * idx;
+    Tu = f3(t[0], t[1], t[2]);
+    Tv = f3(t[3], t[4], t[5]);
+    Tw = f3(t[6], t[7], t[8]);
+    nrm = f3(0.0f, 0.0f, 1.0f);
+  }
+  const float cosv = -((vx * nrm.x + vy * nrm.y) + vz * nrm.z);
+  if (cosv == 0.0f) return;
+  const float mult = cosv > 0.0f ? 1.0f : -1.0f;
+  nrm = mult * nrm;
+  const float d = (9.0f * (Tw.x * Tw.x) + 9.0f * (Tw.y * Tw.y)) + -1.0f * (Tw.z * Tw.z);
+  if (d == 0.0f) return;
+  const float inv = 1.0f / d;
+  const float fx = inv * 9.0f, fy = inv * 9.0f, fz = inv * -1.0f;
+  const float px = (fx * (Tu.x * Tw.x) + fy * (Tu.y * Tw.y)) + fz * (Tu.z * Tw.z);
+  const float py = (fx * (Tv.x * Tw.x) + fy * (Tv.y * Tw.y)) + fz * (Tv.z * Tw.z);
+  const float h0x = px * px - ((fx * (Tu.x * Tu.x) + fy * (Tu.y * Tu.y)) + fz * (Tu.z * Tu.z));
+  const float h0y = py * py - ((fx * (Tv.x * Tv.x) + fy * (Tv.y * Tv.y)) + fz * (Tv.z * Tv.z));
+  const float hx = sqrtf(fmax_(1e-4f, h0x));
+  const float hy = sqrtf(fmax_(1e-4f, h0y));
+  const float radius = ceilf(fmax_(fmax_(hx, hy), 3.0f * kFilterSize));
+  const int ir = f2i(radius);
+  int x0, y0, x1, y1;
+  get_rect(px, py, ir, a.B, a.gx, a.gy, x0, y0, x1, y1);
+  const uint32_t area = (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
+  if (area == 0) return;
+  F3 rgb;
+  if (a.colors == nullptr) {
+    const F3 dir0 = f3(m0 - a.campos[0], m1 - a.campos[1], m2 - a.campos[2]);
+    const float len = sqrtf((dir0.x * dir0.x + dir0.y * dir0.y) + dir0.z * dir0.z);
+    const F3 dir = f3(dir0.x / len, dir0.y / len, dir0.z / len);
+    uint8_t cl[3];
+    rgb = color_from_sh(a.D, a.M, a.shs + (size_t)idx * a.M * 3, dir, cl);
+    uchar4 c4;
+    c4.x = cl[0];
+    c4.y = cl[1];
+    c4.z = cl[2];
+    c4.w = 0;
+    reinterpret_cast<uchar4*>(a.clamped)[idx] = c4;
+  } else {
+    rgb = f3(a.colors[3 * idx], a.colors[3 * idx + 1], a.colors[3 * idx + 2]);
+  }
+  a.depths[idx] = vz;
+  a.radii[idx] = ir;
+  float4* r = a.rec + (size_t)kRecQuads * idx;
+  r[0] = make_float4(Tu.x, Tu.y, Tu.z, Tv.x);
+  r[1] = make_float4(Tv.y, Tv.z, Tw.x, Tw.y);
+  r[2] = make_float4(Tw.z, px, py, a.opac[idx]);
+  r[3] = make_float4(nrm.x, nrm.y, nrm.z, rgb.x);
+  r[4] = make_float4(rgb.y, rgb.z, vz, 0.0f);
+  a.tiles[idx] = area;
+  for (int ty = y0; ty < y1; ++ty)
+    for (int tx = x0; tx < x1; ++tx) atomicAdd(&a.tile_count[ty * a.gx + tx], 1u);
+}
+
+hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
+  if (a.P == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_preprocess, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// Exclusive scan of the tile histogram by one 1024-lane workgroup.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32_t* count,
+                                                              uint32_t* start, uint32_t* fill,
+                                                              uint32_t* ranges, uint32_t* counters) {
+  __shared__ uint32_t s_part[kScanThreads];
+  const int tid = threadIdx.x;
+  const int per = (T + kScanThreads - 1) / kScanThreads;
+  const int b = tid * per;
+  const int e = min(T, b + per);
+  uint32_t local = 0;
+  for (int i = b; i < e; ++i) local += count[i];
+  s_part[tid] = local;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over the 1024 partial sums.
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const uint32_t v = tid >= off ? s_part[tid - off] : 0u;
+    __syncthreads();
+    s_part[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = tid == 0 ? 0u : s_part[tid - 1];
+  for (int i = b; i < e; ++i) {
+    const uint32_t c = count[i];
+    start[i] = run;
+    fill[i] = run;
+    ranges[2 * i] = c ? run : 0u;
+    ranges[2 * i + 1] = c ? run + c : 0u;
+    run += c;
+  }
+  if (tid == kScanThreads - 1) {
+    start[T] = s_part[kScanThreads - 1];
+    counters[0] = s_part[kScanThreads - 1];
+  }
+}
+
+hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
+                             uint32_t* ranges, uint32_t* counters, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, T, count, start, fill, ranges,
+                     counters);
+  return hipGetLastError();
+}
+
+// duplicateWithKeys, rasterizer_impl.cu:70-111, writing straight into tile buckets.
+__global__ __launch_bounds__(256) void k_scatter(int P, int B, int gx, int gy, const float4* rec,
+                                                 const float* depths, const int* radii,
+                                                 uint32_t* tile_fill, uint64_t* keys) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P) return;
+  const int r = radii[idx];
+  if (r <= 0) return;
+  const float4 q2 = rec[(size_t)kRecQuads * idx + 2];
+  int x0, y0, x1, y1;
+  get_rect(q2.y, q2.z, r, B, gx, gy, x0, y0, x1, y1);
+  const uint64_t lo = ((uint64_t)__float_as_uint(depths[idx]) << 32) | (uint32_t)idx;
+  for (int ty = y0; ty < y1; ++ty)
+    for (int tx = x0; tx < x1; ++tx) {
+      const uint32_t slot = atomicAdd(&tile_fill[ty * gx + tx], 1u);
+      keys[slot] = lo;
+    }
+}
+
+hipError_t launch_scatter(int P, int B, int gx, int gy, const float4* rec, const float* depths,
+                          const int* radii, uint32_t* tile_fill, uint64_t* keys, hipStream_t s) {
+  if (P == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter, dim3((P + 255) / 256), dim3(256), 0, s, P, B, gx, gy, rec, depths, radii,
+                     tile_fill, keys);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Per-tile sort.
+// ---------------------------------------------------------------------------
+constexpr int kSortThreads = 256;
+constexpr int kSortCap = 4096;  // keys held in LDS (32 KiB)
+
+// Ascending bitonic sort of S (power of two) keys in LDS by the whole workgroup.
+SR_DEV void bitonic_lds(uint64_t* v, int S) {
+  for (int k = 2; k <= S; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < S; i += kSortThreads) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = v[i], b = v[ixj];
+          const bool up = (i & k) == 0;
+          if (up ? (a > b) : (a < b)) {
+            v[i] = b;
+            v[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+SR_DEV int next_pow2(int n) {
+  int s = 1;
+  while (s < n) s <<= 1;
+  return s;
+}
+
+// Number of elements of A among the first k of merge(A, B) (keys unique).
+SR_DEV int co_rank(const uint64_t* A, int na, const uint64_t* Bv, int nb, int k) {
+  int lo = max(0, k - nb), hi = min(k, na);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (A[mid] < Bv[k - mid - 1]) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_tiles(const uint32_t* ranges, uint64_t* keys,
+                                                             uint64_t* keys_tmp, uint32_t* point_list) {
+  __shared__ uint64_t s_keys[kSortCap];
+  const int tile = blockIdx.x;
+  const uint32_t beg = ranges[2 * tile], end = ranges[2 * tile + 1];
+  const int n = (int)(end - beg);
+  if (n <= 0) return;
+  const int tid = threadIdx.x;
+  if (n == 1) {
+    if (tid == 0) point_list[beg] = (uint32_t)keys[beg];
+    return;
+  }
+  if (n <= kSortCap) {
+    const int S = next_pow2(n);
+    for (int i = tid; i < S; i += kSortThreads) s_keys[i] = i < n ? keys[beg + i] : ~0ull;
+    __syncthreads();
+    bitonic_lds(s_keys, S);
+    for (int i = tid; i < n; i += kSortThreads) point_list[beg + i] = (uint32_t)s_keys[i];
+    return;
+  }
+  // Large tile: sort chunks of kSortCap keys in LDS, then merge runs pairwise.
+  uint64_t* src = keys + beg;
+  uint64_t* dst = keys_tmp + beg;
+  for (int c0 = 0; c0 < n; c0 += kSortCap) {
+    const int cn = min(kSortCap, n - c0);
+    const int S = next_pow2(cn);
+    for (int i = tid; i < S; i += kSortThreads) s_keys[i] = i < cn ? src[c0 + i] : ~0ull;
+    __syncthreads();
+    bitonic_lds(s_keys, S);
+    for (int i = tid; i < cn; i += kSortThreads) src[c0 + i] = s_keys[i];
+    __syncthreads();
+  }
+  for (int w = kSortCap; w < n; w <<= 1) {
+    for (int a0 = 0; a0 < n; a0 += 2 * w) {
+      const int na = min(w, n - a0);
+      const int nb = max(0, min(w, n - a0 - w));
+      const uint64_t* A = src + a0;
+      const uint64_t* Bv = src + a0 + na;
+      uint64_t* out = dst + a0;
+      const int total = na + nb;
+      const int per = (total + kSortThreads - 1) / kSortThreads;
+      const int k0 = min(total, tid * per), k1 = min(total, k0 + per);
+      if (k0 < k1) {
+        int i = co_rank(A, na, Bv, nb, k0);
+        int j = k0 - i;
+        for (int k = k0; k < k1; ++k) {
+          if (j >= nb || (i < na && A[i] < Bv[j])) out[k] = A[i++];
+          else out[k] = Bv[j++];
+        }
+      }
+    }
+    __syncthreads();
+    uint64_t* t = src;
+    src = dst;
+    dst = t;
+  }
+  for (int i = tid; i < n; i += kSortThreads) point_list[beg + i] = (uint32_t)src[i];
+}
+
+hipError_t launch_sort_tiles(int T, const uint32_t* ranges, uint64_t* keys, uint64_t* keys_tmp,
+                             uint32_t* point_list, hipStream_t s) {
+  if (T == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_tiles, dim3(T), dim3(kSortThreads), 0, s, ranges, keys, keys_tmp, point_list);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Forward blend, dsr/cuda_rasterizer/forward.cu:256-441.
+// ---------------------------------------------------------------------------
+template <int B>
+__global__ __launch_bounds__(B * B) void k_blend_fwd(BlendArgs a) {
+  constexpr int BS = B * B;
+  __shared__ float4 s_q0[BS], s_q1[BS], s_q2[BS], s_q3[BS], s_q4[BS];
+  const int tile = blockIdx.x;
+  const int tx = tile % a.gx, ty = tile / a.gx;
+  const int tid = threadIdx.x;
+  const uint32_t pxi = (uint32_t)(tx * B + tid % B), pyi = (uint32_t)(ty * B + tid / B);
+  const bool inside = pxi < (uint32_t)a.W && pyi < (uint32_t)a.H;
+  bool done = !inside;
+  const uint32_t rbeg = a.ranges[2 * tile], rend = a.ranges[2 * tile + 1];
+  const int rounds = (int)((rend - rbeg + BS - 1) / BS);
+  int toDo = (int)(rend - rbeg);
+  const float pixx = (float)pxi, pixy = (float)pyi;
+  float T = 1.0f;
+  uint32_t contributor = 0, last_contributor = 0;
+  float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, N0 = 0.0f, N1 = 0.0f, N2 = 0.0f;
+  float D = 0.0f, M1 = 0.0f, M2 = 0.0f, distortion = 0.0f, median_depth = 0.0f, median_contributor = -1.0f;
+  const float mcoef = kFar / (kFar - kNear);
+  for (int i = 0; i < rounds; i++, toDo -= BS) {
+    const int num_done = __syncthreads_count(done);
+    if (num_done == BS) break;
+    const uint32_t progress = (uint32_t)(i * BS + tid);
+    if (rbeg + progress < rend) {
+      const uint32_t id = a.point_list[rbeg + progress];
+      const float4* r = a.rec + (size_t)kRecQuads * id;
+      s_q0[tid] = r[0];
+      s_q1[tid] = r[1];
+      s_q2[tid] = r[2];
+      s_q3[tid] = r[3];
+      s_q4[tid] = r[4];
+    }
+    __syncthreads();
+    const int n = min(BS, toDo);
+    for (int j = 0; !done && j < n; j++) {
+      contributor++;
+      const float4 q0 = s_q0[j], q1 = s_q1[j], q2 = s_q2[j];
+      const float Tux = q0.x, Tuy = q0.y, Tuz = q0.z, Tvx = q0.w, Tvy = q1.x, Tvz = q1.y;
+      const float Twx = q1.z, Twy = q1.w, Twz = q2.x;
+      const float kx = pixx * Twx - Tux, ky = pixx * Twy - Tuy, kz = pixx * Twz - Tuz;
+      const float lx = pixy * Twx - Tvx, ly = pixy * Twy - Tvy, lz = pixy * Twz - Tvz;
+      const float px = ky * lz - kz * ly, py = kz * lx - kx * lz, pz = kx * ly - ky * lx;
+      if (pz == 0.0f) continue;
+      const float sx = px / pz, sy = py / pz;
+      const float rho3d = sx * sx + sy * sy;
+      const float dx = q2.y - pixx, dy = q2.z - pixy;
+      const float rho2d = 2.0f * (dx * dx + dy * dy);
+      const float rho = fmin_(rho3d, rho2d);
+      const float depth = (rho3d <= rho2d) ? (sx * Twx + sy * Twy) + Twz : Twz;
+      if (depth < kNear) continue;
+      const float power = -0.5f * rho;
+      if (power > 0.0f) continue;
+      const float alpha = fmin_(0.99f, q2.w * exp_pinned(power));
+      if (alpha < 1.0f / 255.0f) continue;
+      const float test_T = T * (1.0f - alpha);
+      if (test_T < 0.0001f) {
+        done = true;
+        continue;
+      }
+      const float w = alpha * T;
+      const float A = 1.0f - T;
+      const float mm = mcoef * (1.0f - kNear / depth);
+      distortion += (((mm * mm) * A + M2) - (2.0f * mm) * M1) * w;
+      D += depth * w;
+      M1 += mm * w;
+      M2 += (mm * mm) * w;
+      if (T > 0.5f) {
+        median_depth = depth;
+        median_contributor = (float)contributor;
+      }
+      const float4 q3 = s_q3[j], q4 = s_q4[j];
+      N0 += q3.x * w;
+      N1 += q3.y * w;
+      N2 += q3.z * w;
+      C0 += q3.w * w;
+      C1 += q4.x * w;
+      C2 += q4.y * w;
+      T = test_T;
+      last_contributor = contributor;
+    }
+  }
+  if (inside) {
+    const size_t HW = (size_t)a.H * a.W, pid = (size_t)a.W * pyi + pxi;
+    a.final_T[pid] = T;
+    a.n_contrib[pid] = last_contributor;
+    a.out_color[pid] = C0 + T * a.bg[0];
+    a.out_color[HW + pid] = C1 + T * a.bg[1];
+    a.out_color[2 * HW + pid] = C2 + T * a.bg[2];
+    a.n_contrib[pid + HW] = f2u(median_contributor);
+    a.final_T[pid + HW] = M1;
+    a.final_T[pid + 2 * HW] = M2;
+    a.out_others[pid] = D;
+    a.out_others[pid + HW] = 1.0f - T;
+    a.out_others[pid + 2 * HW] = N0;
+    a.out_others[pid + 3 * HW] = N1;
+    a.out_others[pid + 4 * HW] = N2;
+    a.out_others[pid + 5 * HW] = median_depth;
+    a.out_others[pid + 6 * HW] = distortion;
+  }
+}
+
+hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s) {
+  if (a.T == 0) return hipSuccess;
+  switch (a.B) {
+    case 8: hipLaunchKernelGGL(k_blend_fwd<8>, dim3(a.T), dim3(64), 0, s, a); break;
+    case 16: hipLaunchKernelGGL(k_blend_fwd<16>, dim3(a.T), dim3(256), 0, s, a); break;
+    case 32: hipLaunchKernelGGL(k_blend_fwd<32>, dim3(a.T), dim3(1024), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// markVisible / checkFrustum, rasterizer_impl.cu:54-66.
+__global__ void k_mark_visible(int P, const float* means3D, const float* vm, uint8_t* present) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const float m0 = means3D[3 * i], m1 = means3D[3 * i + 1], m2 = means3D[3 * i + 2];
+  const float vz = ((vm[2] * m0 + vm[6] * m1) + vm[10] * m2) + vm[14];
+  present[i] = vz <= 0.2f ? 0 : 1;
+}
+
+hipError_t launch_mark_visible(int P, const float* means3D, const float* vm, uint8_t* present,
+                               hipStream_t s) {
+  if (P == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mark_visible, dim3((P + 255) / 256), dim3(256), 0, s, P, means3D, vm, present);
+  return hipGetLastError();
+}
+
+}  // namespace sr

@@ -1,0 +1,169 @@
+// Internal layouts and kernel-launch entry points of the surfel rasteriser.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace sr {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+// Packed per-surfel record read by both blend kernels (5 x float4 = 80 B):
+//   q0 = (Tu.x, Tu.y, Tu.z, Tv.x)   q1 = (Tv.y, Tv.z, Tw.x, Tw.y)
+//   q2 = (Tw.z, mean2D.x, mean2D.y, opacity)
+//   q3 = (normal.x, normal.y, normal.z, r)   q4 = (g, b, depth, 0)
+// Tu/Tv/Tw are the columns of the ray-splat homography T as stored by the
+// reference (dsr/cuda_rasterizer/forward.cu:195-198); normal is the
+// camera-facing view-space normal (forward.cu:209-214).
+constexpr int kRecQuads = 5;
+
+// Geometry state: one entry per surfel.
+struct GeomLayout {
+  size_t rec, depths, tiles, clamped, dnormal, total;
+};
+inline GeomLayout geom_layout(int P) {
+  GeomLayout g{};
+  size_t off = 0;
+  g.rec = off; off += align_up((size_t)P * kRecQuads * 16);
+  g.depths = off; off += align_up((size_t)P * 4);
+  g.tiles = off; off += align_up((size_t)P * 4);
+  g.clamped = off; off += align_up((size_t)P * 4);  // 3 flags + pad per surfel
+  g.dnormal = off; off += align_up((size_t)P * 12);  // backward scratch: dL/dnormal
+  g.total = off;
+  return g;
+}
+
+// Image state: per-tile binning tables and per-pixel blend state.
+struct ImageLayout {
+  int gx, gy, T, N;
+  size_t counters, tile_count, tile_start, tile_fill, ranges, final_T, n_contrib, total;
+};
+inline ImageLayout image_layout(int W, int H, int B) {
+  ImageLayout l{};
+  l.gx = (W + B - 1) / B;
+  l.gy = (H + B - 1) / B;
+  l.T = l.gx * l.gy;
+  l.N = W * H;
+  size_t off = 0;
+  l.counters = off; off += kAlign;  // [0] = instance count, [1] = error flags
+  l.tile_count = off; off += align_up((size_t)l.T * 4);
+  l.tile_start = off; off += align_up(((size_t)l.T + 1) * 4);
+  l.tile_fill = off; off += align_up((size_t)l.T * 4);
+  l.ranges = off; off += align_up((size_t)l.T * 8);
+  l.final_T = off; off += align_up((size_t)l.N * 3 * 4);
+  l.n_contrib = off; off += align_up((size_t)l.N * 2 * 4);
+  l.total = off;
+  return l;
+}
+
+// Binning state: one entry per (surfel, tile) instance.
+struct BinLayout {
+  size_t keys, keys_tmp, point_list, total;
+};
+inline BinLayout bin_layout(int L) {
+  BinLayout b{};
+  size_t off = 0;
+  const size_t n = (size_t)(L > 0 ? L : 1);
+  b.keys = off; off += align_up(n * 8);
+  b.keys_tmp = off; off += align_up(n * 8);
+  b.point_list = off; off += align_up(n * 4);
+  b.total = off;
+  return b;
+}
+
+constexpr uint32_t kErrPrefiltered = 1u;
+
+struct PreprocessArgs {
+  int P, D, M, W, H, B, gx, gy, prefiltered;
+  float mod;
+  const float* means3D;
+  const float* scales;
+  const float* rots;
+  const float* opac;
+  const float* shs;
+  const float* colors;
+  const float* tprec;
+  const float* vm;
+  const float* pm;
+  const float* campos;
+  float4* rec;
+  float* depths;
+  int* radii;
+  uint32_t* tiles;
+  uint8_t* clamped;
+  uint32_t* tile_count;
+  uint32_t* counters;
+};
+
+struct BlendArgs {
+  int W, H, B, gx, gy, T;
+  const uint32_t* ranges;
+  const uint32_t* point_list;
+  const float4* rec;
+  const float* bg;
+  float* final_T;
+  uint32_t* n_contrib;
+  float* out_color;
+  float* out_others;
+};
+
+struct BlendBwdArgs {
+  int W, H, B, gx, gy, T;
+  const uint32_t* ranges;
+  const uint32_t* point_list;
+  const float4* rec;
+  const float* bg;
+  const float* final_T;
+  const uint32_t* n_contrib;
+  const float* dL_dpix;
+  const float* dL_dothers;
+  float* dL_dtransMat;
+  float* dL_dmean2D;
+  float* dL_dnormal;
+  float* dL_dopacity;
+  float* dL_dcolors;
+};
+
+struct PreprocessBwdArgs {
+  int P, D, M, W, H;
+  float tanx, tany;
+  const float* means3D;
+  const float* scales;
+  const float* rots;
+  const float* shs;
+  const float* tprec;
+  const float4* rec;
+  const int* radii;
+  const uint8_t* clamped;
+  const float* vm;
+  const float* pm;
+  const float* campos;
+  float* dL_dtransMat;
+  const float* dL_dnormal;
+  const float* dL_dcolors;
+  float* dL_dsh;
+  float* dL_dmean2D;
+  float* dL_dmean3D;
+  float* dL_dscales;
+  float* dL_drots;
+};
+
+// Launchers (return hipGetLastError()).
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
+hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s);
+hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
+                             uint32_t* ranges, uint32_t* counters, hipStream_t s);
+hipError_t launch_scatter(int P, int B, int gx, int gy, const float4* rec, const float* depths,
+                          const int* radii, uint32_t* tile_fill, uint64_t* keys, hipStream_t s);
+hipError_t launch_sort_tiles(int T, const uint32_t* ranges, uint64_t* keys, uint64_t* keys_tmp,
+                             uint32_t* point_list, hipStream_t s);
+hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s);
+hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s);
+hipError_t launch_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s);
+hipError_t launch_mark_visible(int P, const float* means3D, const float* vm, uint8_t* present,
+                               hipStream_t s);
+
+}  // namespace sr
