@@ -124,7 +124,7 @@ int sr_mark_visible(int P, const float* means3D, const float* viewmatrix,
 /* Byte sizes of the three state buffers (what sr_forward asks alloc for). */
 size_t sr_geom_bytes(int P);
 size_t sr_image_bytes(int W, int H, int tile);
-size_t sr_binning_bytes(int num_rendered);
+size_t sr_binning_bytes(int num_rendered, int tile);
 
 /* Byte offsets of the named arrays inside the state buffers, so callers and
  * tests can read internal state (e.g. "depths", "rec", "tiles_touched",

@@ -129,7 +129,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_geom_bytes.restype = sz
         lib.sr_image_bytes.argtypes = [i, i, i]
         lib.sr_image_bytes.restype = sz
-        lib.sr_binning_bytes.argtypes = [i]
+        lib.sr_binning_bytes.argtypes = [i, i]
         lib.sr_binning_bytes.restype = sz
         lib.sr_state_offset.argtypes = [C.c_char_p, i, i, i, i, i]
         lib.sr_state_offset.restype = sz

@@ -121,10 +121,11 @@ uint32_t* pinned_counters() {
 int validate(const sr_settings* s, const sr_surfels* g) {
   if (!s || !g) return fail(SR_ERR_INVALID, "settings and surfels must not be NULL");
   if (g->P < 0) return fail(SR_ERR_INVALID, "P must be >= 0");
-  if (s->image_width <= 0 || s->image_height <= 0)
-    return fail(SR_ERR_INVALID, "image size must be positive (got %d x %d)", s->image_width, s->image_height);
+  if (s->image_width <= 0 || s->image_height <= 0 || s->image_width > 32000 || s->image_height > 32000)
+    return fail(SR_ERR_INVALID, "image size must be in [1, 32000] (got %d x %d)", s->image_width, s->image_height);
   if (s->tile != 8 && s->tile != 16 && s->tile != 32)
     return fail(SR_ERR_INVALID, "tile must be 8, 16 or 32 (got %d)", s->tile);
+  if (g->P == 0) return SR_OK;  // the reference glue returns before looking at the inputs
   if ((g->shs == nullptr) == (g->colors_precomp == nullptr))
     return fail(SR_ERR_INVALID, "Please provide excatly one of either SHs or precomputed colors!");
   const bool sr = g->scales != nullptr && g->rotations != nullptr;
@@ -152,12 +153,12 @@ const char* sr_version(void) { return "surfel_rast gfx950 1.0"; }
 
 size_t sr_geom_bytes(int P) { return geom_layout(P < 0 ? 0 : P).total; }
 size_t sr_image_bytes(int W, int H, int tile) { return image_layout(W, H, tile > 0 ? tile : 16).total; }
-size_t sr_binning_bytes(int L) { return bin_layout(L).total; }
+size_t sr_binning_bytes(int L, int tile) { return bin_layout(L, tile > 0 ? tile : 16).total; }
 
 size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int L) {
   const GeomLayout g = geom_layout(P);
   const ImageLayout im = image_layout(W, H, tile > 0 ? tile : 16);
-  const BinLayout b = bin_layout(L);
+  const BinLayout b = bin_layout(L, tile > 0 ? tile : 16);
   const std::string n(name ? name : "");
   if (n == "rec") return g.rec;
   if (n == "depths") return g.depths;
@@ -171,6 +172,9 @@ size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int L) {
   if (n == "n_contrib") return im.n_contrib;
   if (n == "keys") return b.keys;
   if (n == "point_list") return b.point_list;
+  if (n == "sub") return b.sub;
+  if (n == "sub_count") return im.sub_count;
+  if (n == "cullbox") return g.cullbox;
   return (size_t)-1;
 }
 
@@ -230,6 +234,7 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
   pa.radii = radii;
   pa.tiles = (uint32_t*)(geom + gl.tiles);
   pa.clamped = (uint8_t*)(geom + gl.clamped);
+  pa.cullbox = (short4*)(geom + gl.cullbox);
   pa.tile_count = tile_count;
   pa.counters = counters;
   SR_STAGE(0, dbg, st, "preprocess", launch_preprocess(pa, st));
@@ -242,17 +247,34 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     return fail(SR_ERR_PREFILTERED, "Point is filtered although prefiltered is set. This shouldn't happen!");
   const int L = (int)host[0];
   *num_rendered = L;
-  const BinLayout bl = bin_layout(L);
+  const BinLayout bl = bin_layout(L, B);
   char* bin = (char*)alloc(ctx, bl.total, 1);
   if (!bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
   uint64_t* keys = (uint64_t*)(bin + bl.keys);
   uint64_t* keys_tmp = (uint64_t*)(bin + bl.keys_tmp);
   uint32_t* point_list = (uint32_t*)(bin + bl.point_list);
+  uint2* sub = (uint2*)(bin + bl.sub);
+  uint32_t* sub_count = (uint32_t*)(img + il.sub_count);
   if (L > 0) {
     SR_STAGE(2, dbg, st, "scatter",
              launch_scatter(P, B, il.gx, il.gy, (const float4*)(geom + gl.rec), (const float*)(geom + gl.depths),
                             radii, tile_fill, keys, st));
-    SR_STAGE(3, dbg, st, "sort", launch_sort_tiles(il.T, ranges, keys, keys_tmp, point_list, st));
+    SortArgs sa{};
+    sa.W = W;
+    sa.H = H;
+    sa.B = B;
+    sa.gx = il.gx;
+    sa.T = il.T;
+    sa.ranges = ranges;
+    sa.keys = keys;
+    sa.keys_tmp = keys_tmp;
+    sa.point_list = point_list;
+    sa.cullbox = (const short4*)(geom + gl.cullbox);
+    sa.sub = sub;
+    sa.sub_count = sub_count;
+    SR_STAGE(3, dbg, st, "sort", launch_sort_tiles(sa, st));
+  } else {
+    SR_STAGE(3, dbg, st, "clear", launch_fill_u32(sub_count, 0u, (size_t)il.T * quads_per_tile(B), st));
   }
   BlendArgs ba{};
   ba.W = W;
@@ -262,7 +284,8 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
   ba.gy = il.gy;
   ba.T = il.T;
   ba.ranges = ranges;
-  ba.point_list = point_list;
+  ba.sub = sub;
+  ba.sub_count = sub_count;
   ba.rec = (const float4*)(geom + gl.rec);
   ba.bg = s->bg;
   ba.final_T = (float*)(img + il.final_T);
@@ -290,7 +313,7 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
   const bool dbg = s->debug != 0;
   const GeomLayout gl = geom_layout(P);
   const ImageLayout il = image_layout(W, H, B);
-  const BinLayout bl = bin_layout(R);
+  const BinLayout bl = bin_layout(R, B);
   const char* geom = (const char*)geom_buf;
   const char* img = (const char*)img_buf;
   const char* bin = (const char*)bin_buf;
@@ -319,7 +342,8 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
   ba.gy = il.gy;
   ba.T = il.T;
   ba.ranges = (const uint32_t*)(img + il.ranges);
-  ba.point_list = bin ? (const uint32_t*)(bin + bl.point_list) : nullptr;
+  ba.sub = bin ? (const uint2*)(bin + bl.sub) : nullptr;
+  ba.sub_count = (const uint32_t*)(img + il.sub_count);
   ba.rec = (const float4*)(geom + gl.rec);
   ba.bg = s->bg;
   ba.final_T = (const float*)(img + il.final_T);

@@ -1,7 +1,8 @@
 // Backward kernels of the surfel rasteriser for gfx950 (CDNA4, wave64).
 //
-//   k_blend_bwd        one workgroup per tile, one lane per pixel, surfels
-//                      visited back to front (dsr/cuda_rasterizer/backward.cu:143-440).
+//   k_blend_bwd        one wave64 per 8 x 8 quadrant, one lane per pixel,
+//                      surfels visited back to front over the quadrant's
+//                      sub-list (dsr/cuda_rasterizer/backward.cu:143-440).
 //                      Each lane forms its pixel's 18 per-surfel partial
 //                      gradients; the wave sums them across its 64 lanes and
 //                      lanes 0..17 issue one FP32 atomic each, instead of up
@@ -28,26 +29,29 @@ SR_DEV float* grad_slot(const BlendBwdArgs& a, uint32_t id, int k) {
   return a.dL_dcolors + 3 * (size_t)id + (k - 15);
 }
 
-template <int B>
-__global__ __launch_bounds__(B * B) void k_blend_bwd(BlendBwdArgs a) {
-  constexpr int BS = B * B;
-  __shared__ float4 s_q0[BS], s_q1[BS], s_q2[BS], s_q3[BS], s_q4[BS];
-  __shared__ uint32_t s_id[BS];
-  const int tile = blockIdx.x;
+// One wave64 per 8 x 8 quadrant walks the quadrant's sub-list back to front.
+// contributor is the entry's 0-based position in the full tile list, so the
+// skip test (contributor >= last_contributor) and the median test match the
+// reference; entries behind every lane's last contributor are skipped whole.
+__global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
+  __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
+  __shared__ uint2 s_e[64];
+  const int QX = a.B / kQuad, Q = QX * QX;
+  const int tile = blockIdx.x / Q, q = blockIdx.x % Q;
   const int tx = tile % a.gx, ty = tile / a.gx;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const uint32_t pxi = (uint32_t)(tx * B + tid % B), pyi = (uint32_t)(ty * B + tid / B);
+  const int lane = threadIdx.x;
+  const uint32_t pxi = (uint32_t)(tx * a.B + (q % QX) * kQuad + (lane & 7));
+  const uint32_t pyi = (uint32_t)(ty * a.B + (q / QX) * kQuad + (lane >> 3));
   const bool inside = pxi < (uint32_t)a.W && pyi < (uint32_t)a.H;
   const size_t HW = (size_t)a.H * a.W, pid = (size_t)a.W * pyi + pxi;
   const uint32_t rbeg = a.ranges[2 * tile], rend = a.ranges[2 * tile + 1];
-  const int rounds = (int)((rend - rbeg + BS - 1) / BS);
-  int toDo = (int)(rend - rbeg);
+  const uint32_t ntile = rend - rbeg;
+  const int cnt = (int)a.sub_count[blockIdx.x];
+  const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
   const float pixx = (float)pxi, pixy = (float)pyi;
 
   const float T_final = inside ? a.final_T[pid] : 0.0f;
   float T = T_final;
-  uint32_t contributor = (uint32_t)toDo;
   const int last_contributor = inside ? (int)a.n_contrib[pid] : 0;
   const int median_contributor = inside ? (int)a.n_contrib[pid + HW] : 0;
   float dLp0 = 0.0f, dLp1 = 0.0f, dLp2 = 0.0f;
@@ -68,6 +72,10 @@ __global__ __launch_bounds__(B * B) void k_blend_bwd(BlendBwdArgs a) {
     final_D = a.final_T[pid + HW];
     final_D2 = a.final_T[pid + 2 * HW];
   }
+  // highest list position any lane of the wave still has to visit
+  int wave_last = last_contributor;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) wave_last = max(wave_last, __shfl_xor(wave_last, off, 64));
   const float final_A = 1.0f - T_final;
   float acc_r0 = 0.0f, acc_r1 = 0.0f, acc_r2 = 0.0f;
   float last_c0 = 0.0f, last_c1 = 0.0f, last_c2 = 0.0f;
@@ -82,131 +90,146 @@ __global__ __launch_bounds__(B * B) void k_blend_bwd(BlendBwdArgs a) {
   bg_dot += a.bg[1] * dLp1;
   bg_dot += a.bg[2] * dLp2;
 
-  for (int i = 0; i < rounds; i++, toDo -= BS) {
-    __syncthreads();
-    const uint32_t progress = (uint32_t)(i * BS + tid);
-    if (rbeg + progress < rend) {
-      const uint32_t id = a.point_list[rend - progress - 1];
-      s_id[tid] = id;
-      const float4* r = a.rec + (size_t)kRecQuads * id;
-      s_q0[tid] = r[0];
-      s_q1[tid] = r[1];
-      s_q2[tid] = r[2];
-      s_q3[tid] = r[3];
-      s_q4[tid] = r[4];
+  // first sub-list entry at or beyond wave_last: everything from there on is skipped
+  int hi = cnt;
+  {
+    int lo = 0;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((int)list[mid].y < wave_last) lo = mid + 1;
+      else hi = mid;
+    }
+  }
+  for (int c_end = hi; c_end > 0; c_end -= 64) {
+    const int c0 = max(0, c_end - 64);
+    const int m = c_end - c0;
+    if (lane < m) {
+      const uint2 v = list[c0 + lane];
+      s_e[lane] = v;
+      const float4* r = a.rec + (size_t)kRecQuads * v.x;
+      s_q0[lane] = r[0];
+      s_q1[lane] = r[1];
+      s_q2[lane] = r[2];
+      s_q3[lane] = r[3];
+      s_q4[lane] = r[4];
     }
     __syncthreads();
-    const int n = min(BS, toDo);
-    for (int j = 0; j < n; j++) {
+    for (int j = m - 1; j >= 0; j--) {
+      const uint2 ev = s_e[j];
+      const uint32_t contributor = ev.y;
       float g[kNumG];
 #pragma unroll
       for (int k = 0; k < kNumG; ++k) g[k] = 0.0f;
       bool has = false, brA = false, brB = false;
-      if (inside) {
-        contributor--;
-        do {
-          if (contributor >= (uint32_t)last_contributor) break;
-          const float4 q0 = s_q0[j], q1 = s_q1[j], q2 = s_q2[j];
-          const float Tux = q0.x, Tuy = q0.y, Tuz = q0.z, Tvx = q0.w, Tvy = q1.x, Tvz = q1.y;
-          const float Twx = q1.z, Twy = q1.w, Twz = q2.x;
-          const float kx = pixx * Twx - Tux, ky = pixx * Twy - Tuy, kz = pixx * Twz - Tuz;
-          const float lx = pixy * Twx - Tvx, ly = pixy * Twy - Tvy, lz = pixy * Twz - Tvz;
-          const float px = ky * lz - kz * ly, py = kz * lx - kx * lz, pz = kx * ly - ky * lx;
-          if (pz == 0.0f) break;
-          const float sx = px / pz, sy = py / pz;
-          const float rho3d = sx * sx + sy * sy;
-          const float dx = q2.y - pixx, dy = q2.z - pixy;
-          const float rho2d = 2.0f * (dx * dx + dy * dy);
-          const float rho = fmin_(rho3d, rho2d);
-          const float c_d = (rho3d <= rho2d) ? (sx * Twx + sy * Twy) + Twz : Twz;
-          if (c_d < kNear) break;
-          const float power = -0.5f * rho;
-          if (power > 0.0f) break;
-          const float opa = q2.w;
-          const float G = exp_pinned(power);
-          const float alpha = fmin_(0.99f, opa * G);
-          if (alpha < 1.0f / 255.0f) break;
-          has = true;
-          const float4 q3 = s_q3[j], q4 = s_q4[j];
-          T = T / (1.0f - alpha);
-          const float w = alpha * T;
-          float dL_dalpha = 0.0f;
-          acc_r0 = last_alpha * last_c0 + (1.0f - last_alpha) * acc_r0;
-          last_c0 = q3.w;
-          dL_dalpha += (q3.w - acc_r0) * dLp0;
-          g[15] = w * dLp0;
-          acc_r1 = last_alpha * last_c1 + (1.0f - last_alpha) * acc_r1;
-          last_c1 = q4.x;
-          dL_dalpha += (q4.x - acc_r1) * dLp1;
-          g[16] = w * dLp1;
-          acc_r2 = last_alpha * last_c2 + (1.0f - last_alpha) * acc_r2;
-          last_c2 = q4.y;
-          dL_dalpha += (q4.y - acc_r2) * dLp2;
-          g[17] = w * dLp2;
-          float dL_dz = 0.0f, dL_dweight = 0.0f;
-          const float m_d = mcoef * (1.0f - kNear / c_d);
-          const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
-          if ((int)contributor == median_contributor - 1) dL_dz += dL_dmedian;
-          dL_dweight += ((final_D2 + (m_d * m_d) * final_A) - (2.0f * m_d) * final_D) * dL_dreg;
-          dL_dalpha += dL_dweight - last_dL_dT;
-          last_dL_dT = dL_dweight * alpha + (1.0f - alpha) * last_dL_dT;
-          const float dL_dmd = ((2.0f * (T * alpha)) * (m_d * final_A - final_D)) * dL_dreg;
-          dL_dz += dL_dmd * dmd_dd;
-          accum_depth_rec = last_alpha * last_depth + (1.0f - last_alpha) * accum_depth_rec;
-          last_depth = c_d;
-          dL_dalpha += (c_d - accum_depth_rec) * dL_ddepth;
-          accum_alpha_rec = last_alpha + (1.0f - last_alpha) * accum_alpha_rec;
-          dL_dalpha += (1.0f - accum_alpha_rec) * dL_daccum;
-          acc_n0 = last_alpha * last_n0 + (1.0f - last_alpha) * acc_n0;
-          last_n0 = q3.x;
-          dL_dalpha += (q3.x - acc_n0) * dLn0;
-          g[11] = (alpha * T) * dLn0;
-          acc_n1 = last_alpha * last_n1 + (1.0f - last_alpha) * acc_n1;
-          last_n1 = q3.y;
-          dL_dalpha += (q3.y - acc_n1) * dLn1;
-          g[12] = (alpha * T) * dLn1;
-          acc_n2 = last_alpha * last_n2 + (1.0f - last_alpha) * acc_n2;
-          last_n2 = q3.z;
-          dL_dalpha += (q3.z - acc_n2) * dLn2;
-          g[13] = (alpha * T) * dLn2;
-          dL_dalpha *= T;
-          last_alpha = alpha;
-          dL_dalpha += (-T_final / (1.0f - alpha)) * bg_dot;
-          const float dL_dG = opa * dL_dalpha;
-          dL_dz += (alpha * T) * dL_ddepth;
-          if (rho3d <= rho2d) {
-            brA = true;
-            const float dL_dsx = (dL_dG * -G) * sx + dL_dz * Twx;
-            const float dL_dsy = (dL_dG * -G) * sy + dL_dz * Twy;
-            const float dsx_pz = dL_dsx / pz, dsy_pz = dL_dsy / pz;
-            const float dpx = dsx_pz, dpy = dsy_pz, dpz = -(dsx_pz * sx + dsy_pz * sy);
-            // dL_dk = cross(l, dL_dp), dL_dl = cross(dL_dp, k)
-            const float dkx = ly * dpz - lz * dpy, dky = lz * dpx - lx * dpz, dkz = lx * dpy - ly * dpx;
-            const float dlx = dpy * kz - dpz * ky, dly = dpz * kx - dpx * kz, dlz = dpx * ky - dpy * kx;
-            g[0] = -dkx;
-            g[1] = -dky;
-            g[2] = -dkz;
-            g[3] = -dlx;
-            g[4] = -dly;
-            g[5] = -dlz;
-            g[6] = (pixx * dkx + pixy * dlx) + dL_dz * sx;
-            g[7] = (pixx * dky + pixy * dly) + dL_dz * sy;
-            g[8] = (pixx * dkz + pixy * dlz) + dL_dz * 1.0f;
-          } else {
-            brB = true;
-            const float dG_ddelx = (-G * 2.0f) * dx;
-            const float dG_ddely = (-G * 2.0f) * dy;
-            g[9] = dL_dG * dG_ddelx;
-            g[10] = dL_dG * dG_ddely;
-            g[8] = dL_dz;
-          }
-          g[14] = G * dL_dalpha;
-        } while (false);
-      }
-      const uint64_t m = __ballot(has);
-      if (m == 0) continue;
-      const uint32_t id = s_id[j];
-      if (__popcll(m) == 1) {
+      do {
+        if (!inside || contributor >= (uint32_t)last_contributor) break;
+        const float4 q2 = s_q2[j];
+        const float dx = q2.y - pixx, dy = q2.z - pixy;
+        const float rho2d = 2.0f * (dx * dx + dy * dy);
+        const float4 q0 = s_q0[j], q1 = s_q1[j];
+        const float Tux = q0.x, Tuy = q0.y, Tuz = q0.z, Tvx = q0.w, Tvy = q1.x, Tvz = q1.y;
+        const float Twx = q1.z, Twy = q1.w, Twz = q2.x;
+        const float kx = pixx * Twx - Tux, ky = pixx * Twy - Tuy, kz = pixx * Twz - Tuz;
+        const float lx = pixy * Twx - Tvx, ly = pixy * Twy - Tvy, lz = pixy * Twz - Tvz;
+        const float pz = kx * ly - ky * lx;
+        if (pz == 0.0f) break;
+        const float px = ky * lz - kz * ly, py = kz * lx - kx * lz;
+        const float4 q4 = s_q4[j];
+        if (rho2d > q4.z) {
+          const float num = px * px + py * py, den = q4.z * (pz * pz);
+          if (num > den && num < 3.0e38f && den < 3.0e38f) break;
+        }
+        const float sx = px / pz, sy = py / pz;
+        const float rho3d = sx * sx + sy * sy;
+        const float rho = fmin_(rho3d, rho2d);
+        const float c_d = (rho3d <= rho2d) ? (sx * Twx + sy * Twy) + Twz : Twz;
+        if (c_d < kNear) break;
+        const float power = -0.5f * rho;
+        if (power > 0.0f) break;
+        const float opa = q2.w;
+        const float G = exp_pinned(power);
+        const float alpha = fmin_(0.99f, opa * G);
+        if (alpha < 1.0f / 255.0f) break;
+        has = true;
+        const float4 q3 = s_q3[j];
+        T = T / (1.0f - alpha);
+        const float w = alpha * T;
+        float dL_dalpha = 0.0f;
+        acc_r0 = last_alpha * last_c0 + (1.0f - last_alpha) * acc_r0;
+        last_c0 = q3.w;
+        dL_dalpha += (q3.w - acc_r0) * dLp0;
+        g[15] = w * dLp0;
+        acc_r1 = last_alpha * last_c1 + (1.0f - last_alpha) * acc_r1;
+        last_c1 = q4.x;
+        dL_dalpha += (q4.x - acc_r1) * dLp1;
+        g[16] = w * dLp1;
+        acc_r2 = last_alpha * last_c2 + (1.0f - last_alpha) * acc_r2;
+        last_c2 = q4.y;
+        dL_dalpha += (q4.y - acc_r2) * dLp2;
+        g[17] = w * dLp2;
+        float dL_dz = 0.0f, dL_dweight = 0.0f;
+        const float m_d = mcoef * (1.0f - kNear / c_d);
+        const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
+        if ((int)contributor == median_contributor - 1) dL_dz += dL_dmedian;
+        dL_dweight += ((final_D2 + (m_d * m_d) * final_A) - (2.0f * m_d) * final_D) * dL_dreg;
+        dL_dalpha += dL_dweight - last_dL_dT;
+        last_dL_dT = dL_dweight * alpha + (1.0f - alpha) * last_dL_dT;
+        const float dL_dmd = ((2.0f * (T * alpha)) * (m_d * final_A - final_D)) * dL_dreg;
+        dL_dz += dL_dmd * dmd_dd;
+        accum_depth_rec = last_alpha * last_depth + (1.0f - last_alpha) * accum_depth_rec;
+        last_depth = c_d;
+        dL_dalpha += (c_d - accum_depth_rec) * dL_ddepth;
+        accum_alpha_rec = last_alpha + (1.0f - last_alpha) * accum_alpha_rec;
+        dL_dalpha += (1.0f - accum_alpha_rec) * dL_daccum;
+        acc_n0 = last_alpha * last_n0 + (1.0f - last_alpha) * acc_n0;
+        last_n0 = q3.x;
+        dL_dalpha += (q3.x - acc_n0) * dLn0;
+        g[11] = (alpha * T) * dLn0;
+        acc_n1 = last_alpha * last_n1 + (1.0f - last_alpha) * acc_n1;
+        last_n1 = q3.y;
+        dL_dalpha += (q3.y - acc_n1) * dLn1;
+        g[12] = (alpha * T) * dLn1;
+        acc_n2 = last_alpha * last_n2 + (1.0f - last_alpha) * acc_n2;
+        last_n2 = q3.z;
+        dL_dalpha += (q3.z - acc_n2) * dLn2;
+        g[13] = (alpha * T) * dLn2;
+        dL_dalpha *= T;
+        last_alpha = alpha;
+        dL_dalpha += (-T_final / (1.0f - alpha)) * bg_dot;
+        const float dL_dG = opa * dL_dalpha;
+        dL_dz += (alpha * T) * dL_ddepth;
+        if (rho3d <= rho2d) {
+          brA = true;
+          const float dL_dsx = (dL_dG * -G) * sx + dL_dz * Twx;
+          const float dL_dsy = (dL_dG * -G) * sy + dL_dz * Twy;
+          const float dsx_pz = dL_dsx / pz, dsy_pz = dL_dsy / pz;
+          const float dpx = dsx_pz, dpy = dsy_pz, dpz = -(dsx_pz * sx + dsy_pz * sy);
+          // dL_dk = cross(l, dL_dp), dL_dl = cross(dL_dp, k)
+          const float dkx = ly * dpz - lz * dpy, dky = lz * dpx - lx * dpz, dkz = lx * dpy - ly * dpx;
+          const float dlx = dpy * kz - dpz * ky, dly = dpz * kx - dpx * kz, dlz = dpx * ky - dpy * kx;
+          g[0] = -dkx;
+          g[1] = -dky;
+          g[2] = -dkz;
+          g[3] = -dlx;
+          g[4] = -dly;
+          g[5] = -dlz;
+          g[6] = (pixx * dkx + pixy * dlx) + dL_dz * sx;
+          g[7] = (pixx * dky + pixy * dly) + dL_dz * sy;
+          g[8] = (pixx * dkz + pixy * dlz) + dL_dz * 1.0f;
+        } else {
+          brB = true;
+          const float dG_ddelx = (-G * 2.0f) * dx;
+          const float dG_ddely = (-G * 2.0f) * dy;
+          g[9] = dL_dG * dG_ddelx;
+          g[10] = dL_dG * dG_ddely;
+          g[8] = dL_dz;
+        }
+        g[14] = G * dL_dalpha;
+      } while (false);
+      const uint64_t mk = __ballot(has);
+      if (mk == 0ull) continue;
+      const uint32_t id = ev.x;
+      if (__popcll(mk) == 1) {
         if (has) {
           if (brA) {
 #pragma unroll
@@ -240,17 +263,15 @@ __global__ __launch_bounds__(B * B) void k_blend_bwd(BlendBwdArgs a) {
       const bool skip = (lane < 8 && !anyA) || ((lane == 9 || lane == 10) && !anyB) || lane >= kNumG;
       if (!skip) atomic_add_f32(grad_slot(a, id, lane), v);
     }
+    __syncthreads();
   }
 }
 
 hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s) {
   if (a.T == 0) return hipSuccess;
-  switch (a.B) {
-    case 8: hipLaunchKernelGGL(k_blend_bwd<8>, dim3(a.T), dim3(64), 0, s, a); break;
-    case 16: hipLaunchKernelGGL(k_blend_bwd<16>, dim3(a.T), dim3(256), 0, s, a); break;
-    case 32: hipLaunchKernelGGL(k_blend_bwd<32>, dim3(a.T), dim3(1024), 0, s, a); break;
-    default: return hipErrorInvalidValue;
-  }
+  if (a.B != 8 && a.B != 16 && a.B != 32) return hipErrorInvalidValue;
+  const unsigned nblk = (unsigned)(a.T * quads_per_tile(a.B));
+  hipLaunchKernelGGL(k_blend_bwd, dim3(nblk), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

@@ -9,11 +9,11 @@
 //                  id) for every covered tile into that tile's bucket.
 //   k_sort_tiles   one workgroup per tile: sorts its bucket in LDS (bitonic,
 //                  up to 4096 keys; larger tiles sort 4096-key chunks and
-//                  merge them in global memory) and emits point_list.
-//   k_blend_fwd    one workgroup per tile, one lane per pixel: rounds of
-//                  cooperative fetches of packed surfel records into LDS, then
-//                  front-to-back alpha blending of colour, depth, alpha,
-//                  normal, median depth and depth distortion.
+//                  merge them in global memory), emits point_list, then splits
+//                  the sorted list into per-8x8-quadrant sub-lists.
+//   k_blend_fwd    one wave64 per 8 x 8 quadrant, one lane per pixel: front-
+//                  to-back alpha blending of colour, depth, alpha, normal,
+//                  median depth and depth distortion over the sub-list.
 //
 // Sorting each tile bucket by (depth bits, surfel id) gives exactly the order
 // of the reference's stable radix sort over (tile, depth) keys emitted in
@@ -95,6 +95,53 @@ SR_DEV F3 color_from_sh(int deg, int M, const float* sh, F3 dir, uint8_t* clampe
   clamped[1] = res.y < 0.0f;
   clamped[2] = res.z < 0.0f;
   return f3(fmax_(res.x, 0.0f), fmax_(res.y, 0.0f), fmax_(res.z, 0.0f));
+}
+
+// Screen cull box of a surfel: every pixel where the blend can reach
+// alpha >= 1/255 lies inside it.  With rho_max = 2 ln(255 opacity), a pixel
+// contributes only if rho3d <= rho_max or rho2d = 2|pix - mean2D|^2 <= rho_max
+// (alpha = min(0.99, opacity exp(-rho/2)), dsr/cuda_rasterizer/forward.cu:371-381).
+// The box is the union of the disk rho2d <= rho_reject and the bounding box
+// of the projected level set rho3d <= rho_reject (the compute_aabb conic with
+// cutoff^2 = rho_reject, forward.cu:119-145), where rho_reject exceeds
+// rho_max by a margin far above FP32 rounding, plus one pixel.  A level set
+// reaching the camera plane (d >= 0) projects unbounded: no box then.
+SR_DEV short4 cull_box(F3 Tu, F3 Tv, F3 Tw, float mx, float my, float opa, int W, int H, float& rho_rej) {
+  const short4 none = make_short4(32767, 32767, -32768, -32768);
+  const short4 all = make_short4(-32768, -32768, 32767, 32767);
+  if (!(opa >= 1.0f / 255.0f)) {
+    rho_rej = -1.0f;
+    return none;
+  }
+  const float rho_max = 2.0f * logf(255.0f * opa);
+  rho_rej = rho_max * 1.001f + 0.02f;
+  const float r2 = sqrtf(0.5f * rho_rej);
+  float x0 = mx - r2, x1 = mx + r2, y0 = my - r2, y1 = my + r2;
+  const float c2 = rho_rej;
+  const float ww = Tw.x * Tw.x + Tw.y * Tw.y;
+  const float d = c2 * ww - Tw.z * Tw.z;
+  if (!(d < -1e-3f * (Tw.z * Tw.z))) return all;
+  const float fx = c2 / d, fz = -1.0f / d;
+  const float cx = fx * (Tu.x * Tw.x + Tu.y * Tw.y) + fz * (Tu.z * Tw.z);
+  const float cy = fx * (Tv.x * Tw.x + Tv.y * Tw.y) + fz * (Tv.z * Tw.z);
+  const float hx = sqrtf(fmax_(0.0f, cx * cx - (fx * (Tu.x * Tu.x + Tu.y * Tu.y) + fz * (Tu.z * Tu.z))));
+  const float hy = sqrtf(fmax_(0.0f, cy * cy - (fx * (Tv.x * Tv.x + Tv.y * Tv.y) + fz * (Tv.z * Tv.z))));
+  const float mgx = 1.0f + 1e-3f * (fabsf(cx) + hx), mgy = 1.0f + 1e-3f * (fabsf(cy) + hy);
+  x0 = fmin_(x0, cx - hx - mgx);
+  x1 = fmax_(x1, cx + hx + mgx);
+  y0 = fmin_(y0, cy - hy - mgy);
+  y1 = fmax_(y1, cy + hy + mgy);
+  x0 = floorf(x0 - 1.0f);
+  y0 = floorf(y0 - 1.0f);
+  x1 = ceilf(x1 + 1.0f);
+  y1 = ceilf(y1 + 1.0f);
+  if (!(x0 == x0 && x1 == x1 && y0 == y0 && y1 == y1)) return all;  // NaN: no culling
+  x0 = fmax_(x0, -1.0f);
+  y0 = fmax_(y0, -1.0f);
+  x1 = fmin_(x1, (float)W);
+  y1 = fmin_(y1, (float)H);
+  if (x1 < x0 || y1 < y0) return none;
+  return make_short4((short)x0, (short)y0, (short)x1, (short)y1);
 }
 
 // Forward preprocess, dsr/cuda_rasterizer/forward.cu:75-145, 148-251.
@@ -180,14 +227,18 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
   } else {
     rgb = f3(a.colors[3 * idx], a.colors[3 * idx + 1], a.colors[3 * idx + 2]);
   }
+  const float opa = a.opac[idx];
+  float rho_rej;
+  const short4 box = cull_box(Tu, Tv, Tw, px, py, opa, a.W, a.H, rho_rej);
   a.depths[idx] = vz;
   a.radii[idx] = ir;
   float4* r = a.rec + (size_t)kRecQuads * idx;
   r[0] = make_float4(Tu.x, Tu.y, Tu.z, Tv.x);
   r[1] = make_float4(Tv.y, Tv.z, Tw.x, Tw.y);
-  r[2] = make_float4(Tw.z, px, py, a.opac[idx]);
+  r[2] = make_float4(Tw.z, px, py, opa);
   r[3] = make_float4(nrm.x, nrm.y, nrm.z, rgb.x);
-  r[4] = make_float4(rgb.y, rgb.z, vz, 0.0f);
+  r[4] = make_float4(rgb.y, rgb.z, rho_rej, 0.0f);
+  a.cullbox[idx] = box;
   a.tiles[idx] = area;
   for (int ty = y0; ty < y1; ++ty)
     for (int tx = x0; tx < x1; ++tx) atomicAdd(&a.tile_count[ty * a.gx + tx], 1u);
@@ -312,122 +363,199 @@ SR_DEV int co_rank(const uint64_t* A, int na, const uint64_t* Bv, int nb, int k)
   return lo;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_tiles(const uint32_t* ranges, uint64_t* keys,
-                                                             uint64_t* keys_tmp, uint32_t* point_list) {
+// Order-preserving split of a tile's sorted list into one sub-list per 8 x 8
+// quadrant: entry e (position e in the tile list) goes to quadrant q when its
+// cull box meets the quadrant.  Sub-list q of tile t starts at
+// Q * beg + q * n and holds (surfel id, position) pairs; it is exactly the
+// tile list with entries removed that cannot reach alpha >= 1/255 anywhere in
+// the quadrant, so blending it gives the reference's result.
+constexpr int kMaxQuads = 16;
+SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
+  constexpr int kWaves = kSortThreads / 64;
+  __shared__ uint32_t s_wcnt[kMaxQuads][kWaves];
+  __shared__ uint32_t s_total[kMaxQuads];
+  const int QX = a.B / kQuad, Q = QX * QX;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tx = tile % a.gx, ty = tile / a.gx;
+  uint2* base = a.sub + (size_t)Q * beg;
+  if (tid < kMaxQuads) s_total[tid] = 0;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int c = 0; c < n; c += kSortThreads) {
+    const int e = c + tid;
+    const bool valid = e < n;
+    uint32_t id = 0;
+    short4 bx = make_short4(32767, 32767, -32768, -32768);
+    if (valid) {
+      id = a.point_list[beg + e];
+      bx = a.cullbox[id];
+    }
+    uint32_t hitmask = 0;
+    uint8_t rank[kMaxQuads];
+#pragma unroll
+    for (int q = 0; q < kMaxQuads; ++q) {
+      rank[q] = 0;
+      if (q < Q) {
+        const int qx0 = tx * a.B + (q % QX) * kQuad, qy0 = ty * a.B + (q / QX) * kQuad;
+        const bool hit = valid && qx0 < a.W && qy0 < a.H && bx.x <= qx0 + kQuad - 1 && bx.z >= qx0 &&
+                         bx.y <= qy0 + kQuad - 1 && bx.w >= qy0;
+        const uint64_t m = __ballot(hit);
+        if (lane == 0) s_wcnt[q][wave] = (uint32_t)__popcll(m);
+        rank[q] = (uint8_t)__popcll(m & lt);
+        hitmask |= (uint32_t)hit << q;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMaxQuads; ++q) {
+      if (q < Q && ((hitmask >> q) & 1u)) {
+        uint32_t off = s_total[q] + rank[q];
+        for (int w = 0; w < wave; ++w) off += s_wcnt[q][w];
+        base[(size_t)q * n + off] = make_uint2(id, (uint32_t)e);
+      }
+    }
+    __syncthreads();
+    if (tid < Q) {
+      uint32_t t = 0;
+      for (int w = 0; w < kWaves; ++w) t += s_wcnt[tid][w];
+      s_total[tid] += t;
+    }
+    __syncthreads();
+  }
+  if (tid < Q) a.sub_count[(size_t)tile * Q + tid] = s_total[tid];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_tiles(SortArgs a) {
   __shared__ uint64_t s_keys[kSortCap];
   const int tile = blockIdx.x;
-  const uint32_t beg = ranges[2 * tile], end = ranges[2 * tile + 1];
+  const uint32_t beg = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
   const int n = (int)(end - beg);
-  if (n <= 0) return;
   const int tid = threadIdx.x;
-  if (n == 1) {
-    if (tid == 0) point_list[beg] = (uint32_t)keys[beg];
+  if (n <= 0) {
+    const int Q = quads_per_tile(a.B);
+    if (tid < Q) a.sub_count[(size_t)tile * Q + tid] = 0;
     return;
   }
   if (n <= kSortCap) {
     const int S = next_pow2(n);
-    for (int i = tid; i < S; i += kSortThreads) s_keys[i] = i < n ? keys[beg + i] : ~0ull;
+    for (int i = tid; i < S; i += kSortThreads) s_keys[i] = i < n ? a.keys[beg + i] : ~0ull;
     __syncthreads();
     bitonic_lds(s_keys, S);
-    for (int i = tid; i < n; i += kSortThreads) point_list[beg + i] = (uint32_t)s_keys[i];
-    return;
-  }
-  // Large tile: sort chunks of kSortCap keys in LDS, then merge runs pairwise.
-  uint64_t* src = keys + beg;
-  uint64_t* dst = keys_tmp + beg;
-  for (int c0 = 0; c0 < n; c0 += kSortCap) {
-    const int cn = min(kSortCap, n - c0);
-    const int S = next_pow2(cn);
-    for (int i = tid; i < S; i += kSortThreads) s_keys[i] = i < cn ? src[c0 + i] : ~0ull;
-    __syncthreads();
-    bitonic_lds(s_keys, S);
-    for (int i = tid; i < cn; i += kSortThreads) src[c0 + i] = s_keys[i];
-    __syncthreads();
-  }
-  for (int w = kSortCap; w < n; w <<= 1) {
-    for (int a0 = 0; a0 < n; a0 += 2 * w) {
-      const int na = min(w, n - a0);
-      const int nb = max(0, min(w, n - a0 - w));
-      const uint64_t* A = src + a0;
-      const uint64_t* Bv = src + a0 + na;
-      uint64_t* out = dst + a0;
-      const int total = na + nb;
-      const int per = (total + kSortThreads - 1) / kSortThreads;
-      const int k0 = min(total, tid * per), k1 = min(total, k0 + per);
-      if (k0 < k1) {
-        int i = co_rank(A, na, Bv, nb, k0);
-        int j = k0 - i;
-        for (int k = k0; k < k1; ++k) {
-          if (j >= nb || (i < na && A[i] < Bv[j])) out[k] = A[i++];
-          else out[k] = Bv[j++];
+    for (int i = tid; i < n; i += kSortThreads) a.point_list[beg + i] = (uint32_t)s_keys[i];
+  } else {
+    // Large tile: sort chunks of kSortCap keys in LDS, then merge runs pairwise.
+    uint64_t* src = a.keys + beg;
+    uint64_t* dst = a.keys_tmp + beg;
+    for (int c0 = 0; c0 < n; c0 += kSortCap) {
+      const int cn = min(kSortCap, n - c0);
+      const int S = next_pow2(cn);
+      for (int i = tid; i < S; i += kSortThreads) s_keys[i] = i < cn ? src[c0 + i] : ~0ull;
+      __syncthreads();
+      bitonic_lds(s_keys, S);
+      for (int i = tid; i < cn; i += kSortThreads) src[c0 + i] = s_keys[i];
+      __syncthreads();
+    }
+    for (int w = kSortCap; w < n; w <<= 1) {
+      for (int a0 = 0; a0 < n; a0 += 2 * w) {
+        const int na = min(w, n - a0);
+        const int nb = max(0, min(w, n - a0 - w));
+        const uint64_t* A = src + a0;
+        const uint64_t* Bv = src + a0 + na;
+        uint64_t* out = dst + a0;
+        const int total = na + nb;
+        const int per = (total + kSortThreads - 1) / kSortThreads;
+        const int k0 = min(total, tid * per), k1 = min(total, k0 + per);
+        if (k0 < k1) {
+          int i = co_rank(A, na, Bv, nb, k0);
+          int j = k0 - i;
+          for (int k = k0; k < k1; ++k) {
+            if (j >= nb || (i < na && A[i] < Bv[j])) out[k] = A[i++];
+            else out[k] = Bv[j++];
+          }
         }
       }
+      __syncthreads();
+      uint64_t* t = src;
+      src = dst;
+      dst = t;
     }
-    __syncthreads();
-    uint64_t* t = src;
-    src = dst;
-    dst = t;
+    for (int i = tid; i < n; i += kSortThreads) a.point_list[beg + i] = (uint32_t)src[i];
   }
-  for (int i = tid; i < n; i += kSortThreads) point_list[beg + i] = (uint32_t)src[i];
+  __syncthreads();
+  split_quadrants(a, tile, beg, n);
 }
 
-hipError_t launch_sort_tiles(int T, const uint32_t* ranges, uint64_t* keys, uint64_t* keys_tmp,
-                             uint32_t* point_list, hipStream_t s) {
-  if (T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sort_tiles, dim3(T), dim3(kSortThreads), 0, s, ranges, keys, keys_tmp, point_list);
+hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
+  if (a.T == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_tiles, dim3(a.T), dim3(kSortThreads), 0, s, a);
   return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Forward blend, dsr/cuda_rasterizer/forward.cu:256-441.
+// Forward blend, dsr/cuda_rasterizer/forward.cu:256-441.  One wave64 per
+// 8 x 8 quadrant walks the quadrant's sub-list front to back; entries fetched
+// 64 at a time into LDS.  "contributor" is the entry's position in the full
+// tile list, as in the reference, so n_contrib and the median index match.
+// A pair is rejected before the divisions when both the screen-space and
+// the ray-splat exponents provably exceed rho_reject (alpha < 1/255 then,
+// which the reference skips too); accepted pairs follow the reference's
+// arithmetic step for step.
 // ---------------------------------------------------------------------------
-template <int B>
-__global__ __launch_bounds__(B * B) void k_blend_fwd(BlendArgs a) {
-  constexpr int BS = B * B;
-  __shared__ float4 s_q0[BS], s_q1[BS], s_q2[BS], s_q3[BS], s_q4[BS];
-  const int tile = blockIdx.x;
+__global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
+  __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
+  __shared__ uint32_t s_pos[64];
+  const int QX = a.B / kQuad, Q = QX * QX;
+  const int tile = blockIdx.x / Q, q = blockIdx.x % Q;
   const int tx = tile % a.gx, ty = tile / a.gx;
-  const int tid = threadIdx.x;
-  const uint32_t pxi = (uint32_t)(tx * B + tid % B), pyi = (uint32_t)(ty * B + tid / B);
+  const int lane = threadIdx.x;
+  const uint32_t pxi = (uint32_t)(tx * a.B + (q % QX) * kQuad + (lane & 7));
+  const uint32_t pyi = (uint32_t)(ty * a.B + (q / QX) * kQuad + (lane >> 3));
   const bool inside = pxi < (uint32_t)a.W && pyi < (uint32_t)a.H;
   bool done = !inside;
   const uint32_t rbeg = a.ranges[2 * tile], rend = a.ranges[2 * tile + 1];
-  const int rounds = (int)((rend - rbeg + BS - 1) / BS);
-  int toDo = (int)(rend - rbeg);
+  const uint32_t ntile = rend - rbeg;
+  const int cnt = (int)a.sub_count[blockIdx.x];
+  const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
   const float pixx = (float)pxi, pixy = (float)pyi;
   float T = 1.0f;
-  uint32_t contributor = 0, last_contributor = 0;
+  uint32_t last_contributor = 0;
   float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, N0 = 0.0f, N1 = 0.0f, N2 = 0.0f;
   float D = 0.0f, M1 = 0.0f, M2 = 0.0f, distortion = 0.0f, median_depth = 0.0f, median_contributor = -1.0f;
   const float mcoef = kFar / (kFar - kNear);
-  for (int i = 0; i < rounds; i++, toDo -= BS) {
-    const int num_done = __syncthreads_count(done);
-    if (num_done == BS) break;
-    const uint32_t progress = (uint32_t)(i * BS + tid);
-    if (rbeg + progress < rend) {
-      const uint32_t id = a.point_list[rbeg + progress];
-      const float4* r = a.rec + (size_t)kRecQuads * id;
-      s_q0[tid] = r[0];
-      s_q1[tid] = r[1];
-      s_q2[tid] = r[2];
-      s_q3[tid] = r[3];
-      s_q4[tid] = r[4];
+  for (int c = 0; c < cnt; c += 64) {
+    if (__ballot(!done) == 0ull) break;
+    if (c + lane < cnt) {
+      const uint2 v = list[c + lane];
+      s_pos[lane] = v.y;
+      const float4* r = a.rec + (size_t)kRecQuads * v.x;
+      s_q0[lane] = r[0];
+      s_q1[lane] = r[1];
+      s_q2[lane] = r[2];
+      s_q3[lane] = r[3];
+      s_q4[lane] = r[4];
     }
     __syncthreads();
-    const int n = min(BS, toDo);
-    for (int j = 0; !done && j < n; j++) {
-      contributor++;
-      const float4 q0 = s_q0[j], q1 = s_q1[j], q2 = s_q2[j];
+    const int m = min(64, cnt - c);
+    for (int j = 0; j < m; j++) {
+      if (done) continue;
+      const float4 q2 = s_q2[j];
+      const float dx = q2.y - pixx, dy = q2.z - pixy;
+      const float rho2d = 2.0f * (dx * dx + dy * dy);
+      const float4 q0 = s_q0[j], q1 = s_q1[j];
       const float Tux = q0.x, Tuy = q0.y, Tuz = q0.z, Tvx = q0.w, Tvy = q1.x, Tvz = q1.y;
       const float Twx = q1.z, Twy = q1.w, Twz = q2.x;
       const float kx = pixx * Twx - Tux, ky = pixx * Twy - Tuy, kz = pixx * Twz - Tuz;
       const float lx = pixy * Twx - Tvx, ly = pixy * Twy - Tvy, lz = pixy * Twz - Tvz;
-      const float px = ky * lz - kz * ly, py = kz * lx - kx * lz, pz = kx * ly - ky * lx;
+      const float pz = kx * ly - ky * lx;
       if (pz == 0.0f) continue;
+      const float px = ky * lz - kz * ly, py = kz * lx - kx * lz;
+      const float rho_rej = s_q4[j].z;
+      if (rho2d > rho_rej) {
+        const float num = px * px + py * py, den = rho_rej * (pz * pz);
+        if (num > den && num < 3.0e38f && den < 3.0e38f) continue;
+      }
       const float sx = px / pz, sy = py / pz;
       const float rho3d = sx * sx + sy * sy;
-      const float dx = q2.y - pixx, dy = q2.z - pixy;
-      const float rho2d = 2.0f * (dx * dx + dy * dy);
       const float rho = fmin_(rho3d, rho2d);
       const float depth = (rho3d <= rho2d) ? (sx * Twx + sy * Twy) + Twz : Twz;
       if (depth < kNear) continue;
@@ -440,6 +568,7 @@ __global__ __launch_bounds__(B * B) void k_blend_fwd(BlendArgs a) {
         done = true;
         continue;
       }
+      const uint32_t contributor = s_pos[j] + 1u;
       const float w = alpha * T;
       const float A = 1.0f - T;
       const float mm = mcoef * (1.0f - kNear / depth);
@@ -461,6 +590,7 @@ __global__ __launch_bounds__(B * B) void k_blend_fwd(BlendArgs a) {
       T = test_T;
       last_contributor = contributor;
     }
+    __syncthreads();
   }
   if (inside) {
     const size_t HW = (size_t)a.H * a.W, pid = (size_t)a.W * pyi + pxi;
@@ -484,12 +614,9 @@ __global__ __launch_bounds__(B * B) void k_blend_fwd(BlendArgs a) {
 
 hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s) {
   if (a.T == 0) return hipSuccess;
-  switch (a.B) {
-    case 8: hipLaunchKernelGGL(k_blend_fwd<8>, dim3(a.T), dim3(64), 0, s, a); break;
-    case 16: hipLaunchKernelGGL(k_blend_fwd<16>, dim3(a.T), dim3(256), 0, s, a); break;
-    case 32: hipLaunchKernelGGL(k_blend_fwd<32>, dim3(a.T), dim3(1024), 0, s, a); break;
-    default: return hipErrorInvalidValue;
-  }
+  if (a.B != 8 && a.B != 16 && a.B != 32) return hipErrorInvalidValue;
+  const unsigned nblk = (unsigned)(a.T * quads_per_tile(a.B));
+  hipLaunchKernelGGL(k_blend_fwd, dim3(nblk), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

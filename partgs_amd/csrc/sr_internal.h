@@ -13,15 +13,24 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 // Packed per-surfel record read by both blend kernels (5 x float4 = 80 B):
 //   q0 = (Tu.x, Tu.y, Tu.z, Tv.x)   q1 = (Tv.y, Tv.z, Tw.x, Tw.y)
 //   q2 = (Tw.z, mean2D.x, mean2D.y, opacity)
-//   q3 = (normal.x, normal.y, normal.z, r)   q4 = (g, b, depth, 0)
+//   q3 = (normal.x, normal.y, normal.z, r)   q4 = (g, b, rho_reject, 0)
 // Tu/Tv/Tw are the columns of the ray-splat homography T as stored by the
 // reference (dsr/cuda_rasterizer/forward.cu:195-198); normal is the
-// camera-facing view-space normal (forward.cu:209-214).
+// camera-facing view-space normal (forward.cu:209-214).  rho_reject bounds
+// the Gaussian exponent: any pixel with min(rho3d, rho2d) > rho_reject has
+// alpha < 1/255 and is skipped by the reference, so the blend kernels may
+// skip it early (rho_reject < 0: the surfel never reaches 1/255).
 constexpr int kRecQuads = 5;
+
+// The blend kernels run one wave64 per 8 x 8 pixel quadrant of a tile; each
+// quadrant walks an order-preserving sub-list of its tile's sorted list,
+// holding only the entries whose screen cull box meets the quadrant.
+constexpr int kQuad = 8;
+__host__ __device__ inline int quads_per_tile(int B) { return (B / kQuad) * (B / kQuad); }
 
 // Geometry state: one entry per surfel.
 struct GeomLayout {
-  size_t rec, depths, tiles, clamped, dnormal, total;
+  size_t rec, depths, tiles, clamped, dnormal, cullbox, total;
 };
 inline GeomLayout geom_layout(int P) {
   GeomLayout g{};
@@ -31,6 +40,7 @@ inline GeomLayout geom_layout(int P) {
   g.tiles = off; off += align_up((size_t)P * 4);
   g.clamped = off; off += align_up((size_t)P * 4);  // 3 flags + pad per surfel
   g.dnormal = off; off += align_up((size_t)P * 12);  // backward scratch: dL/dnormal
+  g.cullbox = off; off += align_up((size_t)P * 8);   // short4 inclusive pixel box
   g.total = off;
   return g;
 }
@@ -38,7 +48,7 @@ inline GeomLayout geom_layout(int P) {
 // Image state: per-tile binning tables and per-pixel blend state.
 struct ImageLayout {
   int gx, gy, T, N;
-  size_t counters, tile_count, tile_start, tile_fill, ranges, final_T, n_contrib, total;
+  size_t counters, tile_count, tile_start, tile_fill, ranges, sub_count, final_T, n_contrib, total;
 };
 inline ImageLayout image_layout(int W, int H, int B) {
   ImageLayout l{};
@@ -52,6 +62,7 @@ inline ImageLayout image_layout(int W, int H, int B) {
   l.tile_start = off; off += align_up(((size_t)l.T + 1) * 4);
   l.tile_fill = off; off += align_up((size_t)l.T * 4);
   l.ranges = off; off += align_up((size_t)l.T * 8);
+  l.sub_count = off; off += align_up((size_t)l.T * quads_per_tile(B) * 4);
   l.final_T = off; off += align_up((size_t)l.N * 3 * 4);
   l.n_contrib = off; off += align_up((size_t)l.N * 2 * 4);
   l.total = off;
@@ -60,15 +71,16 @@ inline ImageLayout image_layout(int W, int H, int B) {
 
 // Binning state: one entry per (surfel, tile) instance.
 struct BinLayout {
-  size_t keys, keys_tmp, point_list, total;
+  size_t keys, keys_tmp, point_list, sub, total;
 };
-inline BinLayout bin_layout(int L) {
+inline BinLayout bin_layout(int L, int B) {
   BinLayout b{};
   size_t off = 0;
   const size_t n = (size_t)(L > 0 ? L : 1);
   b.keys = off; off += align_up(n * 8);
   b.keys_tmp = off; off += align_up(n * 8);
   b.point_list = off; off += align_up(n * 4);
+  b.sub = off; off += align_up(n * quads_per_tile(B) * 8);  // uint2 (surfel id, position in tile)
   b.total = off;
   return b;
 }
@@ -93,14 +105,27 @@ struct PreprocessArgs {
   int* radii;
   uint32_t* tiles;
   uint8_t* clamped;
+  short4* cullbox;
   uint32_t* tile_count;
   uint32_t* counters;
+};
+
+struct SortArgs {
+  int W, H, B, gx, T;
+  const uint32_t* ranges;
+  uint64_t* keys;
+  uint64_t* keys_tmp;
+  uint32_t* point_list;
+  const short4* cullbox;
+  uint2* sub;
+  uint32_t* sub_count;
 };
 
 struct BlendArgs {
   int W, H, B, gx, gy, T;
   const uint32_t* ranges;
-  const uint32_t* point_list;
+  const uint2* sub;
+  const uint32_t* sub_count;
   const float4* rec;
   const float* bg;
   float* final_T;
@@ -112,7 +137,8 @@ struct BlendArgs {
 struct BlendBwdArgs {
   int W, H, B, gx, gy, T;
   const uint32_t* ranges;
-  const uint32_t* point_list;
+  const uint2* sub;
+  const uint32_t* sub_count;
   const float4* rec;
   const float* bg;
   const float* final_T;
@@ -158,8 +184,7 @@ hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint
                              uint32_t* ranges, uint32_t* counters, hipStream_t s);
 hipError_t launch_scatter(int P, int B, int gx, int gy, const float4* rec, const float* depths,
                           const int* radii, uint32_t* tile_fill, uint64_t* keys, hipStream_t s);
-hipError_t launch_sort_tiles(int T, const uint32_t* ranges, uint64_t* keys, uint64_t* keys_tmp,
-                             uint32_t* point_list, hipStream_t s);
+hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s);
 hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s);
 hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s);
 hipError_t launch_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s);

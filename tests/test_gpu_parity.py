@@ -133,6 +133,41 @@ def test_large_tile_merge_path():
     _check_backward(scene, res, o, seed=5)
 
 
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_early_reject_stress(tile):
+    """Opacities around 1/255, grazing and edge-on surfels, huge and sub-pixel
+    scales: the quadrant culling and the early rejection must not drop a
+    pair the reference blends."""
+    from partgs_amd.scenes import random_rotations, quat_from_matrix
+
+    scene = surfel_cloud(11, P=6000, width=136, height=104)
+    rng = np.random.default_rng(11)
+    P = scene.P
+    opa = np.concatenate([rng.uniform(0.9 / 255, 1.6 / 255, P // 3), rng.uniform(0.0, 1.0, P - P // 3)])
+    scene.opacities = torch.from_numpy(opa.reshape(-1, 1)).float()
+    sc = np.exp(rng.uniform(np.log(1e-4), np.log(0.6), (P, 2)))
+    scene.scales = torch.from_numpy(sc).float()
+    # a third of the surfels almost edge-on to the camera
+    cam = scene.camera
+    c2w = torch.linalg.inv(cam.world_view_transform.T).numpy().astype(np.float64)
+    R = random_rotations(rng, P)
+    k = P // 3
+    view_dir = c2w[:3, 2]
+    for i in range(k):
+        n = np.cross(view_dir, rng.normal(size=3))
+        n /= np.linalg.norm(n)
+        n = n + rng.normal(scale=1e-3, size=3)
+        n /= np.linalg.norm(n)
+        t = np.cross(n, rng.normal(size=3))
+        t /= np.linalg.norm(t)
+        R[i] = np.stack([t, np.cross(n, t), n], axis=1)
+    scene.rotations = torch.from_numpy(quat_from_matrix(R)).float()
+    o, _ = oracle_run(scene, tile=tile)
+    res = gpu_forward(scene, tile=tile)
+    _check_forward(scene, res, o)
+    _check_backward(scene, res, o, seed=11)
+
+
 def test_empty_and_culled():
     scene = surfel_cloud(6, P=1000, width=64, height=48)
     # P = 0: zero images, like the reference glue
