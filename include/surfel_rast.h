@@ -124,7 +124,7 @@ int sr_mark_visible(int P, const float* means3D, const float* viewmatrix,
 /* Byte sizes of the three state buffers (what sr_forward asks alloc for). */
 size_t sr_geom_bytes(int P);
 size_t sr_image_bytes(int W, int H, int tile);
-size_t sr_binning_bytes(int num_rendered, int tile);
+size_t sr_binning_bytes(int num_rendered, int W, int H, int tile);
 
 /* Byte offsets of the named arrays inside the state buffers, so callers and
  * tests can read internal state (e.g. "depths", "rec", "tiles_touched",
@@ -134,8 +134,9 @@ size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int num_
 
 /* Per-stage GPU timing with HIP events on the caller's stream.
  * Stages: 0 preprocess, 1 tile scan, 2 instance scatter, 3 tile sort,
- * 4 forward blend, 5 gradient clear, 6 backward blend, 7 backward preprocess. */
-#define SR_NUM_STAGES 8
+ * 4 forward blend, 5 gradient clear, 6 backward blend, 7 backward preprocess,
+ * 8 acceptance masks. */
+#define SR_NUM_STAGES 9
 int sr_set_timing(int enabled);
 /* Synchronises pending events; ms[i] = total milliseconds, count[i] = launches. */
 int sr_get_timing(double* ms, int* count, int n);

@@ -22,7 +22,7 @@ SR_ERR_HIP = 2
 SR_ERR_ALLOC = 3
 SR_ERR_PREFILTERED = 4
 SR_ERR_CAPACITY = 5
-SR_NUM_STAGES = 8
+SR_NUM_STAGES = 9
 STAGE_NAMES = (
     "preprocess",
     "tile_scan",
@@ -32,6 +32,7 @@ STAGE_NAMES = (
     "grad_clear",
     "blend_bwd",
     "preprocess_bwd",
+    "accept",
 )
 
 EXPORTED = (
@@ -129,7 +130,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_geom_bytes.restype = sz
         lib.sr_image_bytes.argtypes = [i, i, i]
         lib.sr_image_bytes.restype = sz
-        lib.sr_binning_bytes.argtypes = [i, i]
+        lib.sr_binning_bytes.argtypes = [i, i, i, i]
         lib.sr_binning_bytes.restype = sz
         lib.sr_state_offset.argtypes = [C.c_char_p, i, i, i, i, i]
         lib.sr_state_offset.restype = sz

@@ -153,12 +153,15 @@ const char* sr_version(void) { return "surfel_rast gfx950 1.0"; }
 
 size_t sr_geom_bytes(int P) { return geom_layout(P < 0 ? 0 : P).total; }
 size_t sr_image_bytes(int W, int H, int tile) { return image_layout(W, H, tile > 0 ? tile : 16).total; }
-size_t sr_binning_bytes(int L, int tile) { return bin_layout(L, tile > 0 ? tile : 16).total; }
+size_t sr_binning_bytes(int L, int W, int H, int tile) {
+  const int B = tile > 0 ? tile : 16;
+  return bin_layout(L, B, image_layout(W, H, B).T).total;
+}
 
 size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int L) {
   const GeomLayout g = geom_layout(P);
   const ImageLayout im = image_layout(W, H, tile > 0 ? tile : 16);
-  const BinLayout b = bin_layout(L, tile > 0 ? tile : 16);
+  const BinLayout b = bin_layout(L, tile > 0 ? tile : 16, im.T);
   const std::string n(name ? name : "");
   if (n == "rec") return g.rec;
   if (n == "depths") return g.depths;
@@ -175,6 +178,7 @@ size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int L) {
   if (n == "sub") return b.sub;
   if (n == "sub_count") return im.sub_count;
   if (n == "cullbox") return g.cullbox;
+  if (n == "masks") return b.masks;
   return (size_t)-1;
 }
 
@@ -247,7 +251,7 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     return fail(SR_ERR_PREFILTERED, "Point is filtered although prefiltered is set. This shouldn't happen!");
   const int L = (int)host[0];
   *num_rendered = L;
-  const BinLayout bl = bin_layout(L, B);
+  const BinLayout bl = bin_layout(L, B, il.T);
   char* bin = (char*)alloc(ctx, bl.total, 1);
   if (!bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
   uint64_t* keys = (uint64_t*)(bin + bl.keys);
@@ -272,7 +276,22 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     sa.cullbox = (const short4*)(geom + gl.cullbox);
     sa.sub = sub;
     sa.sub_count = sub_count;
+    sa.work = (uint2*)(bin + bl.work);
+    sa.work_count = counters + 2;
     SR_STAGE(3, dbg, st, "sort", launch_sort_tiles(sa, st));
+    AcceptArgs aa{};
+    aa.W = W;
+    aa.H = H;
+    aa.B = B;
+    aa.gx = il.gx;
+    aa.ranges = ranges;
+    aa.sub = sub;
+    aa.sub_count = sub_count;
+    aa.rec = (const float4*)(geom + gl.rec);
+    aa.work = (const uint2*)(bin + bl.work);
+    aa.work_count = counters + 2;
+    aa.masks = (uint64_t*)(bin + bl.masks);
+    SR_STAGE(8, dbg, st, "accept", launch_accept(aa, work_capacity(L, B, il.T), st));
   } else {
     SR_STAGE(3, dbg, st, "clear", launch_fill_u32(sub_count, 0u, (size_t)il.T * quads_per_tile(B), st));
   }
@@ -286,6 +305,7 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
   ba.ranges = ranges;
   ba.sub = sub;
   ba.sub_count = sub_count;
+  ba.masks = (const uint64_t*)(bin + bl.masks);
   ba.rec = (const float4*)(geom + gl.rec);
   ba.bg = s->bg;
   ba.final_T = (float*)(img + il.final_T);
@@ -313,7 +333,7 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
   const bool dbg = s->debug != 0;
   const GeomLayout gl = geom_layout(P);
   const ImageLayout il = image_layout(W, H, B);
-  const BinLayout bl = bin_layout(R, B);
+  const BinLayout bl = bin_layout(R, B, il.T);
   const char* geom = (const char*)geom_buf;
   const char* img = (const char*)img_buf;
   const char* bin = (const char*)bin_buf;
@@ -344,6 +364,7 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
   ba.ranges = (const uint32_t*)(img + il.ranges);
   ba.sub = bin ? (const uint2*)(bin + bl.sub) : nullptr;
   ba.sub_count = (const uint32_t*)(img + il.sub_count);
+  ba.masks = bin ? (const uint64_t*)(bin + bl.masks) : nullptr;
   ba.rec = (const float4*)(geom + gl.rec);
   ba.bg = s->bg;
   ba.final_T = (const float*)(img + il.final_T);

@@ -4,14 +4,16 @@
 //                      surfels visited back to front over the quadrant's
 //                      sub-list (dsr/cuda_rasterizer/backward.cu:143-440).
 //                      Each lane forms its pixel's 18 per-surfel partial
-//                      gradients; the wave sums them across its 64 lanes and
-//                      lanes 0..17 issue one FP32 atomic each, instead of up
-//                      to 64 x 13 atomics per surfel.
+//                      gradients; they are summed per entry in LDS and each
+//                      chunk of 64 entries is flushed with one FP32 atomic
+//                      per (entry, slot), instead of up to 64 x 13 atomics
+//                      per surfel.
 //   k_preprocess_bwd   per surfel: AABB-centre adjoint, dL/dT -> dL/d(mean,
 //                      scale, rotation), SH backward, and the densification
 //                      statistic written into dL_dmean2D (backward.cu:443-630).
 #include "sr_device.h"
 #include "sr_internal.h"
+#include "sr_pair.h"
 
 namespace sr {
 
@@ -29,13 +31,15 @@ SR_DEV float* grad_slot(const BlendBwdArgs& a, uint32_t id, int k) {
   return a.dL_dcolors + 3 * (size_t)id + (k - 15);
 }
 
-// One wave64 per 8 x 8 quadrant walks the quadrant's sub-list back to front.
-// contributor is the entry's 0-based position in the full tile list, so the
-// skip test (contributor >= last_contributor) and the median test match the
-// reference; entries behind every lane's last contributor are skipped whole.
+// One wave64 per 8 x 8 quadrant; each lane walks, back to front, the entries
+// its acceptance mask marks (the pairs the reference blends), skipping those
+// at or behind its last contributor, exactly as the reference does.  Partial
+// gradients of the 64 entries of a chunk are summed in LDS across the lanes,
+// then each entry's sums go to global memory with one FP32 atomic per slot.
 __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
   __shared__ uint2 s_e[64];
+  __shared__ float s_acc[kNumG][64];
   const int QX = a.B / kQuad, Q = QX * QX;
   const int tile = blockIdx.x / Q, q = blockIdx.x % Q;
   const int tx = tile % a.gx, ty = tile / a.gx;
@@ -44,15 +48,15 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   const uint32_t pyi = (uint32_t)(ty * a.B + (q / QX) * kQuad + (lane >> 3));
   const bool inside = pxi < (uint32_t)a.W && pyi < (uint32_t)a.H;
   const size_t HW = (size_t)a.H * a.W, pid = (size_t)a.W * pyi + pxi;
-  const uint32_t rbeg = a.ranges[2 * tile], rend = a.ranges[2 * tile + 1];
-  const uint32_t ntile = rend - rbeg;
+  const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
   const int cnt = (int)a.sub_count[blockIdx.x];
   const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
+  const uint64_t* masks = a.masks + mask_base(Q, rbeg, ntile, tile, q);
   const float pixx = (float)pxi, pixy = (float)pyi;
 
   const float T_final = inside ? a.final_T[pid] : 0.0f;
   float T = T_final;
-  const int last_contributor = inside ? (int)a.n_contrib[pid] : 0;
+  const uint32_t last_contributor = inside ? a.n_contrib[pid] : 0u;
   const int median_contributor = inside ? (int)a.n_contrib[pid + HW] : 0;
   float dLp0 = 0.0f, dLp1 = 0.0f, dLp2 = 0.0f;
   float dL_ddepth = 0.0f, dL_daccum = 0.0f, dL_dreg = 0.0f, dL_dmedian = 0.0f;
@@ -72,10 +76,6 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
     final_D = a.final_T[pid + HW];
     final_D2 = a.final_T[pid + 2 * HW];
   }
-  // highest list position any lane of the wave still has to visit
-  int wave_last = last_contributor;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) wave_last = max(wave_last, __shfl_xor(wave_last, off, 64));
   const float final_A = 1.0f - T_final;
   float acc_r0 = 0.0f, acc_r1 = 0.0f, acc_r2 = 0.0f;
   float last_c0 = 0.0f, last_c1 = 0.0f, last_c2 = 0.0f;
@@ -90,21 +90,14 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   bg_dot += a.bg[1] * dLp1;
   bg_dot += a.bg[2] * dLp2;
 
-  // first sub-list entry at or beyond wave_last: everything from there on is skipped
-  int hi = cnt;
-  {
-    int lo = 0;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if ((int)list[mid].y < wave_last) lo = mid + 1;
-      else hi = mid;
-    }
-  }
-  for (int c_end = hi; c_end > 0; c_end -= 64) {
-    const int c0 = max(0, c_end - 64);
-    const int m = c_end - c0;
+  for (int c = (cnt + 63) / 64 - 1; c >= 0; --c) {
+    uint64_t mask = (inside && last_contributor > 0) ? masks[(size_t)c * 64 + lane] : 0ull;
+    if (__ballot(mask != 0ull) == 0ull) continue;
+    const int e0 = c * 64;
+    const int m = min(64, cnt - e0);
+    __syncthreads();
     if (lane < m) {
-      const uint2 v = list[c0 + lane];
+      const uint2 v = list[e0 + lane];
       s_e[lane] = v;
       const float4* r = a.rec + (size_t)kRecQuads * v.x;
       s_q0[lane] = r[0];
@@ -113,157 +106,104 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
       s_q3[lane] = r[3];
       s_q4[lane] = r[4];
     }
+#pragma unroll
+    for (int k = 0; k < kNumG; ++k) s_acc[k][lane] = 0.0f;
     __syncthreads();
-    for (int j = m - 1; j >= 0; j--) {
-      const uint2 ev = s_e[j];
-      const uint32_t contributor = ev.y;
-      float g[kNumG];
-#pragma unroll
-      for (int k = 0; k < kNumG; ++k) g[k] = 0.0f;
-      bool has = false, brA = false, brB = false;
-      do {
-        if (!inside || contributor >= (uint32_t)last_contributor) break;
-        const float4 q2 = s_q2[j];
-        const float dx = q2.y - pixx, dy = q2.z - pixy;
-        const float rho2d = 2.0f * (dx * dx + dy * dy);
-        const float4 q0 = s_q0[j], q1 = s_q1[j];
-        const float Tux = q0.x, Tuy = q0.y, Tuz = q0.z, Tvx = q0.w, Tvy = q1.x, Tvz = q1.y;
-        const float Twx = q1.z, Twy = q1.w, Twz = q2.x;
-        const float kx = pixx * Twx - Tux, ky = pixx * Twy - Tuy, kz = pixx * Twz - Tuz;
-        const float lx = pixy * Twx - Tvx, ly = pixy * Twy - Tvy, lz = pixy * Twz - Tvz;
-        const float pz = kx * ly - ky * lx;
-        if (pz == 0.0f) break;
-        const float px = ky * lz - kz * ly, py = kz * lx - kx * lz;
-        const float4 q4 = s_q4[j];
-        if (rho2d > q4.z) {
-          const float num = px * px + py * py, den = q4.z * (pz * pz);
-          if (num > den && num < 3.0e38f && den < 3.0e38f) break;
-        }
-        const float sx = px / pz, sy = py / pz;
-        const float rho3d = sx * sx + sy * sy;
-        const float rho = fmin_(rho3d, rho2d);
-        const float c_d = (rho3d <= rho2d) ? (sx * Twx + sy * Twy) + Twz : Twz;
-        if (c_d < kNear) break;
-        const float power = -0.5f * rho;
-        if (power > 0.0f) break;
-        const float opa = q2.w;
-        const float G = exp_pinned(power);
-        const float alpha = fmin_(0.99f, opa * G);
-        if (alpha < 1.0f / 255.0f) break;
-        has = true;
-        const float4 q3 = s_q3[j];
-        T = T / (1.0f - alpha);
-        const float w = alpha * T;
-        float dL_dalpha = 0.0f;
-        acc_r0 = last_alpha * last_c0 + (1.0f - last_alpha) * acc_r0;
-        last_c0 = q3.w;
-        dL_dalpha += (q3.w - acc_r0) * dLp0;
-        g[15] = w * dLp0;
-        acc_r1 = last_alpha * last_c1 + (1.0f - last_alpha) * acc_r1;
-        last_c1 = q4.x;
-        dL_dalpha += (q4.x - acc_r1) * dLp1;
-        g[16] = w * dLp1;
-        acc_r2 = last_alpha * last_c2 + (1.0f - last_alpha) * acc_r2;
-        last_c2 = q4.y;
-        dL_dalpha += (q4.y - acc_r2) * dLp2;
-        g[17] = w * dLp2;
-        float dL_dz = 0.0f, dL_dweight = 0.0f;
-        const float m_d = mcoef * (1.0f - kNear / c_d);
-        const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
-        if ((int)contributor == median_contributor - 1) dL_dz += dL_dmedian;
-        dL_dweight += ((final_D2 + (m_d * m_d) * final_A) - (2.0f * m_d) * final_D) * dL_dreg;
-        dL_dalpha += dL_dweight - last_dL_dT;
-        last_dL_dT = dL_dweight * alpha + (1.0f - alpha) * last_dL_dT;
-        const float dL_dmd = ((2.0f * (T * alpha)) * (m_d * final_A - final_D)) * dL_dreg;
-        dL_dz += dL_dmd * dmd_dd;
-        accum_depth_rec = last_alpha * last_depth + (1.0f - last_alpha) * accum_depth_rec;
-        last_depth = c_d;
-        dL_dalpha += (c_d - accum_depth_rec) * dL_ddepth;
-        accum_alpha_rec = last_alpha + (1.0f - last_alpha) * accum_alpha_rec;
-        dL_dalpha += (1.0f - accum_alpha_rec) * dL_daccum;
-        acc_n0 = last_alpha * last_n0 + (1.0f - last_alpha) * acc_n0;
-        last_n0 = q3.x;
-        dL_dalpha += (q3.x - acc_n0) * dLn0;
-        g[11] = (alpha * T) * dLn0;
-        acc_n1 = last_alpha * last_n1 + (1.0f - last_alpha) * acc_n1;
-        last_n1 = q3.y;
-        dL_dalpha += (q3.y - acc_n1) * dLn1;
-        g[12] = (alpha * T) * dLn1;
-        acc_n2 = last_alpha * last_n2 + (1.0f - last_alpha) * acc_n2;
-        last_n2 = q3.z;
-        dL_dalpha += (q3.z - acc_n2) * dLn2;
-        g[13] = (alpha * T) * dLn2;
-        dL_dalpha *= T;
-        last_alpha = alpha;
-        dL_dalpha += (-T_final / (1.0f - alpha)) * bg_dot;
-        const float dL_dG = opa * dL_dalpha;
-        dL_dz += (alpha * T) * dL_ddepth;
-        if (rho3d <= rho2d) {
-          brA = true;
-          const float dL_dsx = (dL_dG * -G) * sx + dL_dz * Twx;
-          const float dL_dsy = (dL_dG * -G) * sy + dL_dz * Twy;
-          const float dsx_pz = dL_dsx / pz, dsy_pz = dL_dsy / pz;
-          const float dpx = dsx_pz, dpy = dsy_pz, dpz = -(dsx_pz * sx + dsy_pz * sy);
-          // dL_dk = cross(l, dL_dp), dL_dl = cross(dL_dp, k)
-          const float dkx = ly * dpz - lz * dpy, dky = lz * dpx - lx * dpz, dkz = lx * dpy - ly * dpx;
-          const float dlx = dpy * kz - dpz * ky, dly = dpz * kx - dpx * kz, dlz = dpx * ky - dpy * kx;
-          g[0] = -dkx;
-          g[1] = -dky;
-          g[2] = -dkz;
-          g[3] = -dlx;
-          g[4] = -dly;
-          g[5] = -dlz;
-          g[6] = (pixx * dkx + pixy * dlx) + dL_dz * sx;
-          g[7] = (pixx * dky + pixy * dly) + dL_dz * sy;
-          g[8] = (pixx * dkz + pixy * dlz) + dL_dz * 1.0f;
-        } else {
-          brB = true;
-          const float dG_ddelx = (-G * 2.0f) * dx;
-          const float dG_ddely = (-G * 2.0f) * dy;
-          g[9] = dL_dG * dG_ddelx;
-          g[10] = dL_dG * dG_ddely;
-          g[8] = dL_dz;
-        }
-        g[14] = G * dL_dalpha;
-      } while (false);
-      const uint64_t mk = __ballot(has);
-      if (mk == 0ull) continue;
-      const uint32_t id = ev.x;
-      if (__popcll(mk) == 1) {
-        if (has) {
-          if (brA) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) atomic_add_f32(grad_slot(a, id, k), g[k]);
-          } else {
-            atomic_add_f32(grad_slot(a, id, 8), g[8]);
-            atomic_add_f32(grad_slot(a, id, 9), g[9]);
-            atomic_add_f32(grad_slot(a, id, 10), g[10]);
-          }
-#pragma unroll
-          for (int k = 11; k < kNumG; ++k) atomic_add_f32(grad_slot(a, id, k), g[k]);
-        }
-        continue;
+    while (mask != 0ull) {
+      const int j = 63 - __builtin_clzll(mask);
+      mask &= ~(1ull << j);
+      const uint32_t contributor = s_e[j].y;  // 0-based position in the tile list
+      if (contributor >= last_contributor) continue;
+      const float4 q2 = s_q2[j];
+      Pair e;
+      if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], q2, 0.0f, e)) continue;
+      const float4 q3 = s_q3[j], q4 = s_q4[j];
+      const float alpha = e.alpha, G = e.G, c_d = e.depth;
+      const float opa = q2.w;
+      T = T / (1.0f - alpha);
+      const float w = alpha * T;
+      float dL_dalpha = 0.0f;
+      acc_r0 = last_alpha * last_c0 + (1.0f - last_alpha) * acc_r0;
+      last_c0 = q3.w;
+      dL_dalpha += (q3.w - acc_r0) * dLp0;
+      atomicAdd(&s_acc[15][j], w * dLp0);
+      acc_r1 = last_alpha * last_c1 + (1.0f - last_alpha) * acc_r1;
+      last_c1 = q4.x;
+      dL_dalpha += (q4.x - acc_r1) * dLp1;
+      atomicAdd(&s_acc[16][j], w * dLp1);
+      acc_r2 = last_alpha * last_c2 + (1.0f - last_alpha) * acc_r2;
+      last_c2 = q4.y;
+      dL_dalpha += (q4.y - acc_r2) * dLp2;
+      atomicAdd(&s_acc[17][j], w * dLp2);
+      float dL_dz = 0.0f, dL_dweight = 0.0f;
+      const float m_d = mcoef * (1.0f - kNear / c_d);
+      const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
+      if ((int)contributor == median_contributor - 1) dL_dz += dL_dmedian;
+      dL_dweight += ((final_D2 + (m_d * m_d) * final_A) - (2.0f * m_d) * final_D) * dL_dreg;
+      dL_dalpha += dL_dweight - last_dL_dT;
+      last_dL_dT = dL_dweight * alpha + (1.0f - alpha) * last_dL_dT;
+      const float dL_dmd = ((2.0f * (T * alpha)) * (m_d * final_A - final_D)) * dL_dreg;
+      dL_dz += dL_dmd * dmd_dd;
+      accum_depth_rec = last_alpha * last_depth + (1.0f - last_alpha) * accum_depth_rec;
+      last_depth = c_d;
+      dL_dalpha += (c_d - accum_depth_rec) * dL_ddepth;
+      accum_alpha_rec = last_alpha + (1.0f - last_alpha) * accum_alpha_rec;
+      dL_dalpha += (1.0f - accum_alpha_rec) * dL_daccum;
+      acc_n0 = last_alpha * last_n0 + (1.0f - last_alpha) * acc_n0;
+      last_n0 = q3.x;
+      dL_dalpha += (q3.x - acc_n0) * dLn0;
+      atomicAdd(&s_acc[11][j], (alpha * T) * dLn0);
+      acc_n1 = last_alpha * last_n1 + (1.0f - last_alpha) * acc_n1;
+      last_n1 = q3.y;
+      dL_dalpha += (q3.y - acc_n1) * dLn1;
+      atomicAdd(&s_acc[12][j], (alpha * T) * dLn1);
+      acc_n2 = last_alpha * last_n2 + (1.0f - last_alpha) * acc_n2;
+      last_n2 = q3.z;
+      dL_dalpha += (q3.z - acc_n2) * dLn2;
+      atomicAdd(&s_acc[13][j], (alpha * T) * dLn2);
+      dL_dalpha *= T;
+      last_alpha = alpha;
+      dL_dalpha += (-T_final / (1.0f - alpha)) * bg_dot;
+      const float dL_dG = opa * dL_dalpha;
+      dL_dz += (alpha * T) * dL_ddepth;
+      if (e.rho3d <= e.rho2d) {
+        const float sx = e.sx, sy = e.sy, pz = e.pz;
+        const float kx = e.kx, ky = e.ky, kz = e.kz, lx = e.lx, ly = e.ly, lz = e.lz;
+        const float4 q1 = s_q1[j];
+        const float dL_dsx = (dL_dG * -G) * sx + dL_dz * q1.z;
+        const float dL_dsy = (dL_dG * -G) * sy + dL_dz * q1.w;
+        const float dsx_pz = dL_dsx / pz, dsy_pz = dL_dsy / pz;
+        const float dpx = dsx_pz, dpy = dsy_pz, dpz = -(dsx_pz * sx + dsy_pz * sy);
+        // dL_dk = cross(l, dL_dp), dL_dl = cross(dL_dp, k)
+        const float dkx = ly * dpz - lz * dpy, dky = lz * dpx - lx * dpz, dkz = lx * dpy - ly * dpx;
+        const float dlx = dpy * kz - dpz * ky, dly = dpz * kx - dpx * kz, dlz = dpx * ky - dpy * kx;
+        atomicAdd(&s_acc[0][j], -dkx);
+        atomicAdd(&s_acc[1][j], -dky);
+        atomicAdd(&s_acc[2][j], -dkz);
+        atomicAdd(&s_acc[3][j], -dlx);
+        atomicAdd(&s_acc[4][j], -dly);
+        atomicAdd(&s_acc[5][j], -dlz);
+        atomicAdd(&s_acc[6][j], (pixx * dkx + pixy * dlx) + dL_dz * sx);
+        atomicAdd(&s_acc[7][j], (pixx * dky + pixy * dly) + dL_dz * sy);
+        atomicAdd(&s_acc[8][j], (pixx * dkz + pixy * dlz) + dL_dz * 1.0f);
+      } else {
+        const float dG_ddelx = (-G * 2.0f) * e.dx;
+        const float dG_ddely = (-G * 2.0f) * e.dy;
+        atomicAdd(&s_acc[9][j], dL_dG * dG_ddelx);
+        atomicAdd(&s_acc[10][j], dL_dG * dG_ddely);
+        atomicAdd(&s_acc[8][j], dL_dz);
       }
-      const bool anyA = __ballot(brA) != 0ull, anyB = __ballot(brB) != 0ull;
-      if (anyA) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) g[k] = wave_sum(g[k]);
-      }
-      g[8] = wave_sum(g[8]);
-      if (anyB) {
-        g[9] = wave_sum(g[9]);
-        g[10] = wave_sum(g[10]);
-      }
-#pragma unroll
-      for (int k = 11; k < kNumG; ++k) g[k] = wave_sum(g[k]);
-      // lane k adds slot k
-      float v = 0.0f;
-#pragma unroll
-      for (int k = 0; k < kNumG; ++k) v = (lane == k) ? g[k] : v;
-      const bool skip = (lane < 8 && !anyA) || ((lane == 9 || lane == 10) && !anyB) || lane >= kNumG;
-      if (!skip) atomic_add_f32(grad_slot(a, id, lane), v);
+      atomicAdd(&s_acc[14][j], G * dL_dalpha);
     }
     __syncthreads();
+    if (lane < m) {
+      const uint32_t id = s_e[lane].x;
+#pragma unroll
+      for (int k = 0; k < kNumG; ++k) {
+        const float v = s_acc[k][lane];
+        if (v != 0.0f) atomic_add_f32(grad_slot(a, id, k), v);
+      }
+    }
   }
 }
 

@@ -20,6 +20,7 @@
 // surfel order (dsr/cuda_rasterizer/rasterizer_impl.cu:70-111, 301-309).
 #include "sr_device.h"
 #include "sr_internal.h"
+#include "sr_pair.h"
 
 namespace sr {
 
@@ -422,7 +423,15 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
     }
     __syncthreads();
   }
-  if (tid < Q) a.sub_count[(size_t)tile * Q + tid] = s_total[tid];
+  if (tid < Q) {
+    const uint32_t cnt = s_total[tid];
+    a.sub_count[(size_t)tile * Q + tid] = cnt;
+    const uint32_t nch = (cnt + 63u) / 64u;
+    if (nch > 0) {
+      const uint32_t w0 = atomicAdd(a.work_count, nch);
+      for (uint32_t c = 0; c < nch; ++c) a.work[w0 + c] = make_uint2((uint32_t)(tile * Q + tid), c);
+    }
+  }
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_sort_tiles(SortArgs a) {
@@ -492,14 +501,66 @@ hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// Acceptance pass.  One wave per (quadrant, chunk of 64 sub-list entries),
+// one lane per pixel: bit j of the lane's mask is set when entry j of the
+// chunk reaches alpha >= 1/255 at that pixel.  The chunks of all quadrants
+// are independent, so this, the bulk of the pair evaluations, runs balanced
+// over the whole GPU; the blend kernels then visit only accepted pairs.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
+  __shared__ float4 s_q0[64], s_q1[64], s_q2[64];
+  __shared__ float s_rej[64];
+  const int QX = a.B / kQuad, Q = QX * QX;
+  const int lane = threadIdx.x;
+  const uint32_t nwork = *a.work_count;
+  for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const uint2 item = a.work[w];
+    const int qi = (int)item.x, c = (int)item.y;
+    const int tile = qi / Q, q = qi % Q;
+    const int tx = tile % a.gx, ty = tile / a.gx;
+    const uint32_t pxi = (uint32_t)(tx * a.B + (q % QX) * kQuad + (lane & 7));
+    const uint32_t pyi = (uint32_t)(ty * a.B + (q / QX) * kQuad + (lane >> 3));
+    const bool inside = pxi < (uint32_t)a.W && pyi < (uint32_t)a.H;
+    const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
+    const int cnt = (int)a.sub_count[qi];
+    const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
+    const int e0 = c * 64;
+    const int m = min(64, cnt - e0);
+    __syncthreads();
+    if (lane < m) {
+      const float4* r = a.rec + (size_t)kRecQuads * list[e0 + lane].x;
+      s_q0[lane] = r[0];
+      s_q1[lane] = r[1];
+      s_q2[lane] = r[2];
+      s_rej[lane] = r[4].z;
+    }
+    __syncthreads();
+    uint64_t mask = 0;
+    if (inside) {
+      const float pixx = (float)pxi, pixy = (float)pyi;
+      for (int j = 0; j < m; ++j) {
+        Pair e;
+        if (eval_pair<true>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], s_rej[j], e)) mask |= 1ull << j;
+      }
+    }
+    a.masks[mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane] = mask;
+  }
+}
+
+hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s) {
+  if (max_items == 0) return hipSuccess;
+  const unsigned grid = (unsigned)(max_items < 65536 ? max_items : 65536);
+  hipLaunchKernelGGL(k_accept, dim3(grid), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Forward blend, dsr/cuda_rasterizer/forward.cu:256-441.  One wave64 per
-// 8 x 8 quadrant walks the quadrant's sub-list front to back; entries fetched
-// 64 at a time into LDS.  "contributor" is the entry's position in the full
-// tile list, as in the reference, so n_contrib and the median index match.
-// A pair is rejected before the divisions when both the screen-space and
-// the ray-splat exponents provably exceed rho_reject (alpha < 1/255 then,
-// which the reference skips too); accepted pairs follow the reference's
-// arithmetic step for step.
+// 8 x 8 quadrant; each lane walks, front to back, only the entries its
+// acceptance mask marks, recomputing each pair exactly and applying the
+// reference's update.  Entries a lane skips are the ones the reference
+// skips, so T, the accumulators and the contributor counts (positions in the
+// full tile list) come out bit-identical.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
@@ -512,20 +573,25 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   const uint32_t pyi = (uint32_t)(ty * a.B + (q / QX) * kQuad + (lane >> 3));
   const bool inside = pxi < (uint32_t)a.W && pyi < (uint32_t)a.H;
   bool done = !inside;
-  const uint32_t rbeg = a.ranges[2 * tile], rend = a.ranges[2 * tile + 1];
-  const uint32_t ntile = rend - rbeg;
+  const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
   const int cnt = (int)a.sub_count[blockIdx.x];
   const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
+  const uint64_t* masks = a.masks + mask_base(Q, rbeg, ntile, tile, q);
   const float pixx = (float)pxi, pixy = (float)pyi;
   float T = 1.0f;
   uint32_t last_contributor = 0;
   float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, N0 = 0.0f, N1 = 0.0f, N2 = 0.0f;
   float D = 0.0f, M1 = 0.0f, M2 = 0.0f, distortion = 0.0f, median_depth = 0.0f, median_contributor = -1.0f;
   const float mcoef = kFar / (kFar - kNear);
-  for (int c = 0; c < cnt; c += 64) {
-    if (__ballot(!done) == 0ull) break;
-    if (c + lane < cnt) {
-      const uint2 v = list[c + lane];
+  const int nch = (cnt + 63) / 64;
+  for (int c = 0; c < nch; ++c) {
+    uint64_t mask = done ? 0ull : masks[(size_t)c * 64 + lane];
+    if (__ballot(mask != 0ull) == 0ull) continue;
+    const int e0 = c * 64;
+    const int m = min(64, cnt - e0);
+    __syncthreads();
+    if (lane < m) {
+      const uint2 v = list[e0 + lane];
       s_pos[lane] = v.y;
       const float4* r = a.rec + (size_t)kRecQuads * v.x;
       s_q0[lane] = r[0];
@@ -535,38 +601,16 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
       s_q4[lane] = r[4];
     }
     __syncthreads();
-    const int m = min(64, cnt - c);
-    for (int j = 0; j < m; j++) {
-      if (done) continue;
-      const float4 q2 = s_q2[j];
-      const float dx = q2.y - pixx, dy = q2.z - pixy;
-      const float rho2d = 2.0f * (dx * dx + dy * dy);
-      const float4 q0 = s_q0[j], q1 = s_q1[j];
-      const float Tux = q0.x, Tuy = q0.y, Tuz = q0.z, Tvx = q0.w, Tvy = q1.x, Tvz = q1.y;
-      const float Twx = q1.z, Twy = q1.w, Twz = q2.x;
-      const float kx = pixx * Twx - Tux, ky = pixx * Twy - Tuy, kz = pixx * Twz - Tuz;
-      const float lx = pixy * Twx - Tvx, ly = pixy * Twy - Tvy, lz = pixy * Twz - Tvz;
-      const float pz = kx * ly - ky * lx;
-      if (pz == 0.0f) continue;
-      const float px = ky * lz - kz * ly, py = kz * lx - kx * lz;
-      const float rho_rej = s_q4[j].z;
-      if (rho2d > rho_rej) {
-        const float num = px * px + py * py, den = rho_rej * (pz * pz);
-        if (num > den && num < 3.0e38f && den < 3.0e38f) continue;
-      }
-      const float sx = px / pz, sy = py / pz;
-      const float rho3d = sx * sx + sy * sy;
-      const float rho = fmin_(rho3d, rho2d);
-      const float depth = (rho3d <= rho2d) ? (sx * Twx + sy * Twy) + Twz : Twz;
-      if (depth < kNear) continue;
-      const float power = -0.5f * rho;
-      if (power > 0.0f) continue;
-      const float alpha = fmin_(0.99f, q2.w * exp_pinned(power));
-      if (alpha < 1.0f / 255.0f) continue;
+    while (mask != 0ull) {
+      const int j = __builtin_ctzll(mask);
+      mask &= mask - 1ull;
+      Pair e;
+      if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;
+      const float alpha = e.alpha, depth = e.depth;
       const float test_T = T * (1.0f - alpha);
       if (test_T < 0.0001f) {
         done = true;
-        continue;
+        break;
       }
       const uint32_t contributor = s_pos[j] + 1u;
       const float w = alpha * T;
@@ -590,7 +634,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
       T = test_T;
       last_contributor = contributor;
     }
-    __syncthreads();
+    if (__ballot(!done) == 0ull) break;
   }
   if (inside) {
     const size_t HW = (size_t)a.H * a.W, pid = (size_t)a.W * pyi + pxi;

@@ -71,16 +71,30 @@ inline ImageLayout image_layout(int W, int H, int B) {
 
 // Binning state: one entry per (surfel, tile) instance.
 struct BinLayout {
-  size_t keys, keys_tmp, point_list, sub, total;
+  size_t keys, keys_tmp, point_list, sub, masks, work, total;
 };
-inline BinLayout bin_layout(int L, int B) {
+// Per-pixel acceptance masks: quadrant (t, q) owns mask slots
+// [mask_base, mask_base + 64 * chunks), chunk c of its sub-list at
+// mask_base + 64 c, one 64-bit word per pixel (bit j: entry 64 c + j reaches
+// alpha >= 1/255 at that pixel).
+__host__ __device__ inline size_t mask_base(int Q, uint32_t beg, uint32_t n, int tile, int q) {
+  return (size_t)Q * beg + (size_t)q * n + (size_t)kQuad * kQuad * ((size_t)tile * Q + q);
+}
+inline size_t work_capacity(int L, int B, int T) {
+  const size_t Q = (size_t)quads_per_tile(B);
+  return Q * ((size_t)T + ((size_t)(L > 0 ? L : 1) + 63) / 64);
+}
+inline BinLayout bin_layout(int L, int B, int T) {
   BinLayout b{};
   size_t off = 0;
   const size_t n = (size_t)(L > 0 ? L : 1);
+  const size_t Q = (size_t)quads_per_tile(B);
   b.keys = off; off += align_up(n * 8);
   b.keys_tmp = off; off += align_up(n * 8);
   b.point_list = off; off += align_up(n * 4);
-  b.sub = off; off += align_up(n * quads_per_tile(B) * 8);  // uint2 (surfel id, position in tile)
+  b.sub = off; off += align_up(n * Q * 8);  // uint2 (surfel id, position in tile)
+  b.masks = off; off += align_up((n * Q + (size_t)kQuad * kQuad * Q * (size_t)T) * 8);
+  b.work = off; off += align_up(work_capacity(L, B, T) * 8);
   b.total = off;
   return b;
 }
@@ -119,6 +133,19 @@ struct SortArgs {
   const short4* cullbox;
   uint2* sub;
   uint32_t* sub_count;
+  uint2* work;          // (quadrant index, chunk) items for the acceptance pass
+  uint32_t* work_count;
+};
+
+struct AcceptArgs {
+  int W, H, B, gx;
+  const uint32_t* ranges;
+  const uint2* sub;
+  const uint32_t* sub_count;
+  const float4* rec;
+  const uint2* work;
+  const uint32_t* work_count;
+  uint64_t* masks;
 };
 
 struct BlendArgs {
@@ -126,6 +153,7 @@ struct BlendArgs {
   const uint32_t* ranges;
   const uint2* sub;
   const uint32_t* sub_count;
+  const uint64_t* masks;
   const float4* rec;
   const float* bg;
   float* final_T;
@@ -139,6 +167,7 @@ struct BlendBwdArgs {
   const uint32_t* ranges;
   const uint2* sub;
   const uint32_t* sub_count;
+  const uint64_t* masks;
   const float4* rec;
   const float* bg;
   const float* final_T;
@@ -185,6 +214,7 @@ hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint
 hipError_t launch_scatter(int P, int B, int gx, int gy, const float4* rec, const float* depths,
                           const int* radii, uint32_t* tile_fill, uint64_t* keys, hipStream_t s);
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s);
+hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s);
 hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s);
 hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s);
 hipError_t launch_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s);
