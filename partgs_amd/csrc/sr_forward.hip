@@ -7,10 +7,11 @@
 //                  tile start offsets, tile ranges and the instance count.
 //   k_scatter      per surfel: writes a 64-bit key (depth bits << 32 | surfel
 //                  id) for every covered tile into that tile's bucket.
-//   k_sort_tiles   one workgroup per tile: sorts its bucket in LDS (bitonic,
-//                  up to 4096 keys; larger tiles sort 4096-key chunks and
-//                  merge them in global memory), emits point_list, then splits
-//                  the sorted list into per-8x8-quadrant sub-lists.
+//   k_sort_tiles   one workgroup per tile: sorts its bucket in LDS (bitonic;
+//                  256 lanes up to 2048 keys, 1024 lanes up to 16384 keys,
+//                  larger tiles sort 16K-key chunks and merge them in global
+//                  memory), emits point_list, then splits the sorted list
+//                  into per-8x8-quadrant sub-lists.
 //   k_blend_fwd    one wave64 per 8 x 8 quadrant, one lane per pixel: front-
 //                  to-back alpha blending of colour, depth, alpha, normal,
 //                  median depth and depth distortion over the sub-list.
@@ -145,10 +146,9 @@ SR_DEV short4 cull_box(F3 Tu, F3 Tv, F3 Tw, float mx, float my, float opa, int W
   return make_short4((short)x0, (short)y0, (short)x1, (short)y1);
 }
 
-// Forward preprocess, dsr/cuda_rasterizer/forward.cu:75-145, 148-251.
-__global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.P) return;
+// Forward preprocess of one surfel, dsr/cuda_rasterizer/forward.cu:75-145,
+// 148-251.  Returns false for culled surfels, else its tile rectangle.
+SR_DEV bool preprocess_one(const PreprocessArgs& a, int idx, int& x0, int& y0, int& x1, int& y1) {
   a.radii[idx] = 0;
   a.tiles[idx] = 0;
   const float m0 = a.means3D[3 * idx], m1 = a.means3D[3 * idx + 1], m2 = a.means3D[3 * idx + 2];
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
   const float vz = ((vm[2] * m0 + vm[6] * m1) + vm[10] * m2) + vm[14];
   if (vz <= 0.2f) {
     if (a.prefiltered) a.counters[1] = kErrPrefiltered;
-    return;
+    return false;
   }
   F3 Tu, Tv, Tw, nrm;
   if (a.tprec == nullptr) {
@@ -193,11 +193,11 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
     nrm = f3(0.0f, 0.0f, 1.0f);
   }
   const float cosv = -((vx * nrm.x + vy * nrm.y) + vz * nrm.z);
-  if (cosv == 0.0f) return;
+  if (cosv == 0.0f) return false;
   const float mult = cosv > 0.0f ? 1.0f : -1.0f;
   nrm = mult * nrm;
   const float d = (9.0f * (Tw.x * Tw.x) + 9.0f * (Tw.y * Tw.y)) + -1.0f * (Tw.z * Tw.z);
-  if (d == 0.0f) return;
+  if (d == 0.0f) return false;
   const float inv = 1.0f / d;
   const float fx = inv * 9.0f, fy = inv * 9.0f, fz = inv * -1.0f;
   const float px = (fx * (Tu.x * Tw.x) + fy * (Tu.y * Tw.y)) + fz * (Tu.z * Tw.z);
@@ -208,10 +208,9 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
   const float hy = sqrtf(fmax_(1e-4f, h0y));
   const float radius = ceilf(fmax_(fmax_(hx, hy), 3.0f * kFilterSize));
   const int ir = f2i(radius);
-  int x0, y0, x1, y1;
   get_rect(px, py, ir, a.B, a.gx, a.gy, x0, y0, x1, y1);
   const uint32_t area = (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
-  if (area == 0) return;
+  if (area == 0) return false;
   F3 rgb;
   if (a.colors == nullptr) {
     const F3 dir0 = f3(m0 - a.campos[0], m1 - a.campos[1], m2 - a.campos[2]);
@@ -241,9 +240,40 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
   r[4] = make_float4(rgb.y, rgb.z, rho_rej, 0.0f);
   a.cullbox[idx] = box;
   a.tiles[idx] = area;
-  for (int ty = y0; ty < y1; ++ty)
-    for (int tx = x0; tx < x1; ++tx) atomicAdd(&a.tile_count[ty * a.gx + tx], 1u);
+  return true;
 }
+
+// Tile histograms with up to kHistTiles tiles are gathered in LDS per
+// workgroup and flushed with one global atomic per non-empty bin, which
+// keeps hot tiles (thousands of instances) off a single global counter.
+constexpr int kHistTiles = 4096;
+
+__global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
+  __shared__ uint32_t s_hist[kHistTiles];
+  const int T = a.gx * a.gy;
+  const bool lds = T <= kHistTiles;
+  if (lds)
+    for (int t = threadIdx.x; t < T; t += blockDim.x) s_hist[t] = 0;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+  const bool ok = idx < a.P && preprocess_one(a, idx, x0, y0, x1, y1);
+  if (lds) __syncthreads();
+  if (ok) {
+    for (int ty = y0; ty < y1; ++ty)
+      for (int tx = x0; tx < x1; ++tx) {
+        if (lds) atomicAdd(&s_hist[ty * a.gx + tx], 1u);
+        else atomicAdd(&a.tile_count[ty * a.gx + tx], 1u);
+      }
+  }
+  if (lds) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+      const uint32_t c = s_hist[t];
+      if (c) atomicAdd(&a.tile_count[t], c);
+    }
+  }
+}
+
 
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
   if (a.P == 0) return hipSuccess;
@@ -294,23 +324,50 @@ hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint
   return hipGetLastError();
 }
 
-// duplicateWithKeys, rasterizer_impl.cu:70-111, writing straight into tile buckets.
+// duplicateWithKeys, rasterizer_impl.cu:70-111, writing straight into tile
+// buckets.  With few enough tiles, each workgroup first counts its instances
+// per tile in LDS, reserves one contiguous block per tile with a single
+// global atomic, then hands out slots inside the block with LDS atomics.
 __global__ __launch_bounds__(256) void k_scatter(int P, int B, int gx, int gy, const float4* rec,
                                                  const float* depths, const int* radii,
                                                  uint32_t* tile_fill, uint64_t* keys) {
+  __shared__ uint32_t s_cnt[kHistTiles];
+  __shared__ uint32_t s_base[kHistTiles];
+  const int T = gx * gy;
+  const bool lds = T <= kHistTiles;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= P) return;
-  const int r = radii[idx];
-  if (r <= 0) return;
-  const float4 q2 = rec[(size_t)kRecQuads * idx + 2];
-  int x0, y0, x1, y1;
-  get_rect(q2.y, q2.z, r, B, gx, gy, x0, y0, x1, y1);
-  const uint64_t lo = ((uint64_t)__float_as_uint(depths[idx]) << 32) | (uint32_t)idx;
-  for (int ty = y0; ty < y1; ++ty)
-    for (int tx = x0; tx < x1; ++tx) {
-      const uint32_t slot = atomicAdd(&tile_fill[ty * gx + tx], 1u);
-      keys[slot] = lo;
-    }
+  int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+  uint64_t lo = 0;
+  const bool ok = idx < P && radii[idx] > 0;
+  if (ok) {
+    const float4 q2 = rec[(size_t)kRecQuads * idx + 2];
+    get_rect(q2.y, q2.z, radii[idx], B, gx, gy, x0, y0, x1, y1);
+    lo = ((uint64_t)__float_as_uint(depths[idx]) << 32) | (uint32_t)idx;
+  }
+  if (!lds) {
+    if (ok)
+      for (int ty = y0; ty < y1; ++ty)
+        for (int tx = x0; tx < x1; ++tx) keys[atomicAdd(&tile_fill[ty * gx + tx], 1u)] = lo;
+    return;
+  }
+  for (int t = threadIdx.x; t < T; t += blockDim.x) s_cnt[t] = 0;
+  __syncthreads();
+  if (ok)
+    for (int ty = y0; ty < y1; ++ty)
+      for (int tx = x0; tx < x1; ++tx) atomicAdd(&s_cnt[ty * gx + tx], 1u);
+  __syncthreads();
+  for (int t = threadIdx.x; t < T; t += blockDim.x) {
+    const uint32_t c = s_cnt[t];
+    if (c) s_base[t] = atomicAdd(&tile_fill[t], c);
+    s_cnt[t] = 0;
+  }
+  __syncthreads();
+  if (ok)
+    for (int ty = y0; ty < y1; ++ty)
+      for (int tx = x0; tx < x1; ++tx) {
+        const int t = ty * gx + tx;
+        keys[s_base[t] + atomicAdd(&s_cnt[t], 1u)] = lo;
+      }
 }
 
 hipError_t launch_scatter(int P, int B, int gx, int gy, const float4* rec, const float* depths,
@@ -322,23 +379,27 @@ hipError_t launch_scatter(int P, int B, int gx, int gy, const float4* rec, const
 }
 
 // ---------------------------------------------------------------------------
-// Per-tile sort.
+// Per-tile sort.  Tiles with at most kSmallMax instances are sorted by a
+// 256-lane workgroup entirely in LDS; larger tiles by a 1024-lane workgroup
+// holding up to kLargeCap keys (128 KiB of LDS), beyond which 16K-key chunks
+// are sorted in LDS and merged pairwise in global memory (merge path).
 // ---------------------------------------------------------------------------
-constexpr int kSortThreads = 256;
-constexpr int kSortCap = 4096;  // keys held in LDS (32 KiB)
+constexpr int kSmallMax = 2048;
+constexpr int kLargeCap = 16384;
 
-// Ascending bitonic sort of S (power of two) keys in LDS by the whole workgroup.
+// Ascending bitonic sort of S (power of two) keys in LDS by NT lanes.
+template <int NT>
 SR_DEV void bitonic_lds(uint64_t* v, int S) {
   for (int k = 2; k <= S; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < S; i += kSortThreads) {
+      for (int i = threadIdx.x; i < S; i += NT) {
         const int ixj = i ^ j;
         if (ixj > i) {
-          const uint64_t a = v[i], b = v[ixj];
+          const uint64_t x = v[i], y = v[ixj];
           const bool up = (i & k) == 0;
-          if (up ? (a > b) : (a < b)) {
-            v[i] = b;
-            v[ixj] = a;
+          if (up ? (x > y) : (x < y)) {
+            v[i] = y;
+            v[ixj] = x;
           }
         }
       }
@@ -369,10 +430,12 @@ SR_DEV int co_rank(const uint64_t* A, int na, const uint64_t* Bv, int nb, int k)
 // cull box meets the quadrant.  Sub-list q of tile t starts at
 // Q * beg + q * n and holds (surfel id, position) pairs; it is exactly the
 // tile list with entries removed that cannot reach alpha >= 1/255 anywhere in
-// the quadrant, so blending it gives the reference's result.
+// the quadrant, so blending it gives the reference's result.  Each non-empty
+// sub-list also appends its 64-entry chunks to the acceptance work list.
 constexpr int kMaxQuads = 16;
+template <int NT>
 SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
-  constexpr int kWaves = kSortThreads / 64;
+  constexpr int kWaves = NT / 64;
   __shared__ uint32_t s_wcnt[kMaxQuads][kWaves];
   __shared__ uint32_t s_total[kMaxQuads];
   const int QX = a.B / kQuad, Q = QX * QX;
@@ -382,7 +445,7 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
   if (tid < kMaxQuads) s_total[tid] = 0;
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int c = 0; c < n; c += kSortThreads) {
+  for (int c = 0; c < n; c += NT) {
     const int e = c + tid;
     const bool valid = e < n;
     uint32_t id = 0;
@@ -434,37 +497,38 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
   }
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_tiles(SortArgs a) {
-  __shared__ uint64_t s_keys[kSortCap];
+template <int NT, int CAP, bool kLarge>
+__global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
+  __shared__ uint64_t s_keys[CAP];
   const int tile = blockIdx.x;
   const uint32_t beg = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
   const int n = (int)(end - beg);
   const int tid = threadIdx.x;
+  if (kLarge != (n > kSmallMax)) return;
   if (n <= 0) {
     const int Q = quads_per_tile(a.B);
     if (tid < Q) a.sub_count[(size_t)tile * Q + tid] = 0;
     return;
   }
-  if (n <= kSortCap) {
+  if (n <= CAP) {
     const int S = next_pow2(n);
-    for (int i = tid; i < S; i += kSortThreads) s_keys[i] = i < n ? a.keys[beg + i] : ~0ull;
+    for (int i = tid; i < S; i += NT) s_keys[i] = i < n ? a.keys[beg + i] : ~0ull;
     __syncthreads();
-    bitonic_lds(s_keys, S);
-    for (int i = tid; i < n; i += kSortThreads) a.point_list[beg + i] = (uint32_t)s_keys[i];
+    bitonic_lds<NT>(s_keys, S);
+    for (int i = tid; i < n; i += NT) a.point_list[beg + i] = (uint32_t)s_keys[i];
   } else {
-    // Large tile: sort chunks of kSortCap keys in LDS, then merge runs pairwise.
     uint64_t* src = a.keys + beg;
     uint64_t* dst = a.keys_tmp + beg;
-    for (int c0 = 0; c0 < n; c0 += kSortCap) {
-      const int cn = min(kSortCap, n - c0);
+    for (int c0 = 0; c0 < n; c0 += CAP) {
+      const int cn = min(CAP, n - c0);
       const int S = next_pow2(cn);
-      for (int i = tid; i < S; i += kSortThreads) s_keys[i] = i < cn ? src[c0 + i] : ~0ull;
+      for (int i = tid; i < S; i += NT) s_keys[i] = i < cn ? src[c0 + i] : ~0ull;
       __syncthreads();
-      bitonic_lds(s_keys, S);
-      for (int i = tid; i < cn; i += kSortThreads) src[c0 + i] = s_keys[i];
+      bitonic_lds<NT>(s_keys, S);
+      for (int i = tid; i < cn; i += NT) src[c0 + i] = s_keys[i];
       __syncthreads();
     }
-    for (int w = kSortCap; w < n; w <<= 1) {
+    for (int w = CAP; w < n; w <<= 1) {
       for (int a0 = 0; a0 < n; a0 += 2 * w) {
         const int na = min(w, n - a0);
         const int nb = max(0, min(w, n - a0 - w));
@@ -472,7 +536,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tiles(SortArgs a) {
         const uint64_t* Bv = src + a0 + na;
         uint64_t* out = dst + a0;
         const int total = na + nb;
-        const int per = (total + kSortThreads - 1) / kSortThreads;
+        const int per = (total + NT - 1) / NT;
         const int k0 = min(total, tid * per), k1 = min(total, k0 + per);
         if (k0 < k1) {
           int i = co_rank(A, na, Bv, nb, k0);
@@ -488,15 +552,18 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tiles(SortArgs a) {
       src = dst;
       dst = t;
     }
-    for (int i = tid; i < n; i += kSortThreads) a.point_list[beg + i] = (uint32_t)src[i];
+    for (int i = tid; i < n; i += NT) a.point_list[beg + i] = (uint32_t)src[i];
   }
   __syncthreads();
-  split_quadrants(a, tile, beg, n);
+  split_quadrants<NT>(a, tile, beg, n);
 }
 
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
   if (a.T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sort_tiles, dim3(a.T), dim3(kSortThreads), 0, s, a);
+  hipLaunchKernelGGL((k_sort_tiles<256, kSmallMax, false>), dim3(a.T), dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_sort_tiles<1024, kLargeCap, true>), dim3(a.T), dim3(1024), 0, s, a);
   return hipGetLastError();
 }
 

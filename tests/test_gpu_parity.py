@@ -115,9 +115,11 @@ def test_precomputed_transmat():
     _check_backward(scene, res, o, seed=4)
 
 
-def test_large_tile_merge_path():
-    """More than 4096 instances in one tile exercises the chunk + merge sort."""
-    scene = surfel_cloud(5, P=12000, width=32, height=32)
+@pytest.mark.parametrize("P", [12000, 40000])
+def test_large_tile_sorts(P):
+    """Tiles above 2048 instances take the 1024-lane LDS sort; above 16384 the
+    chunk + merge-path sort."""
+    scene = surfel_cloud(5, P=P, width=32, height=32)
     cam = scene.camera
     # pull every centre onto the optical axis region so all land in few tiles
     c2w = torch.linalg.inv(cam.world_view_transform.T)
@@ -127,7 +129,7 @@ def test_large_tile_merge_path():
     scene.means3D = (centre + 0.05 * (torch.rand(scene.P, 3, generator=g) - 0.5)).float()
     o, _ = oracle_run(scene)
     st_counts = np.diff(o.get("ranges"), axis=1).max()
-    assert st_counts > 4096, st_counts
+    assert st_counts > (16384 if P > 20000 else 2048), st_counts
     res = gpu_forward(scene)
     _check_forward(scene, res, o)
     _check_backward(scene, res, o, seed=5)
