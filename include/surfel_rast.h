@@ -142,6 +142,39 @@ int sr_set_timing(int enabled);
 int sr_get_timing(double* ms, int* count, int n);
 void sr_reset_timing(void);
 
+/* ---- superquadric block layer -------------------------------------------
+ * Blocks -> attached surfels, BlockGaussianModel.prepare_scaling_rot and
+ * get_xyz / get_scaling / get_rotation / get_opacity
+ * (games/block_mesh_splatting/scene/block_gaussian_model.py:103-121,178-256).
+ * The block mesh is a regular (eta, omega) grid: face (i, j) is the triangle
+ * (V[i][j], V[i+1][j], V[i][(j+1) % n_omega]) and carries K surfels with
+ * barycentric weights bary (K x 3; NULL means K = 1 at the centroid).
+ * Surfel index = ((b * n_eta + i) * n_omega + j) * K + k. */
+typedef struct {
+  int B;                 /* number of blocks */
+  int n_eta, n_omega;    /* faces per block = n_eta * n_omega */
+  int K;                 /* surfels per face */
+  const float* eta;      /* device, n_eta + 1 vertex latitudes */
+  const float* omega;    /* device, n_omega vertex longitudes */
+  const float* bary;     /* device, K x 3, or NULL */
+  float ratio;           /* ratio_block_scene (parametric_sq scale) */
+  float scale_min;       /* scale_block_min added to exp(sq_s) */
+  float surfel_scale;    /* per-surfel _scale factor on (s1, s2) */
+  const float* params;   /* device, B x 12: sq_s(3), sq_r(4, w x y z), sq_t(3), sq_eps(2) */
+  const float* occupancy;/* device, B: sq_occ logits */
+} sb_blocks;
+
+/* No counterpart in the reference as a native kernel: the reference builds
+ * these surfels with ~20 torch ops per iteration (prepare_scaling_rot).
+ * Outputs P x 3, P x 2, P x 4, P (P = B * n_eta * n_omega * K). */
+int sb_generate_fwd(const sb_blocks* blk, float* means3D, float* scales, float* rotations, float* opacities,
+                    sr_stream_t stream);
+/* Adjoint of sb_generate_fwd: dL_dparams B x 12 and dL_doccupancy B are
+ * overwritten with the gradients of the block parameters. */
+int sb_generate_bwd(const sb_blocks* blk, const float* dL_dmeans3D, const float* dL_dscales,
+                    const float* dL_drotations, const float* dL_dopacities, float* dL_dparams,
+                    float* dL_doccupancy, sr_stream_t stream);
+
 const char* sr_last_error(void);
 const char* sr_version(void);
 

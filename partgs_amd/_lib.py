@@ -48,6 +48,8 @@ EXPORTED = (
     "sr_reset_timing",
     "sr_last_error",
     "sr_version",
+    "sb_generate_fwd",
+    "sb_generate_bwd",
 )
 
 
@@ -142,6 +144,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_reset_timing.restype = None
         lib.sr_last_error.argtypes = []
         lib.sr_last_error.restype = C.c_char_p
+        lib.sb_generate_fwd.argtypes = [vp, vp, vp, vp, vp, vp]
+        lib.sb_generate_fwd.restype = i
+        lib.sb_generate_bwd.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        lib.sb_generate_bwd.restype = i
         lib.sr_version.argtypes = []
         lib.sr_version.restype = C.c_char_p
         _lib = lib

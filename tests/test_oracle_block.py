@@ -1,0 +1,76 @@
+"""Pins the block-layer oracle (oracle/block_oracle.py) against geometry and
+the reference's own rot_to_quat_batch outputs."""
+import os
+
+import numpy as np
+
+from oracle import block_oracle as BO
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+VERTS = np.eye(3)  # three surfels per face, one on each vertex
+
+
+def quat_to_matrix(q):
+    """Standard (w, x, y, z) -> rotation matrix, written independently."""
+    w, x, y, z = q
+    return np.array([
+        [w * w + x * x - y * y - z * z, 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), w * w - x * x + y * y - z * z, 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), w * w - x * x - y * y + z * z],
+    ])
+
+
+def test_ellipsoid_closed_form():
+    rng = np.random.default_rng(0)
+    params, occ = BO.random_blocks(rng, 3)
+    params[:, 10:12] = 0.0  # sigmoid(0) * 1.8 + 0.1 = 1: an ellipsoid
+    eta, om = BO.grid(12, 16)
+    means, _, _, _ = BO.generate(params, occ, eta, om, bary=VERTS)
+    per = 12 * 16 * 3
+    for b in range(3):
+        S = np.exp(params[b, 0:3]) + 0.2
+        q = params[b, 3:7] / np.linalg.norm(params[b, 3:7])
+        R = quat_to_matrix(q)
+        # vertices are ((v * 0.25 * S) @ R_ref) + t with R_ref = R(q) in row form
+        local = (means[b * per:(b + 1) * per] - params[b, 7:10]) @ R.T
+        unit = local / (0.25 * S)
+        np.testing.assert_allclose(np.linalg.norm(unit, axis=1), 1.0, atol=1e-12)
+
+
+def test_frames_orthonormal_and_aligned():
+    rng = np.random.default_rng(1)
+    params, occ = BO.random_blocks(rng, 2)
+    eta, om = BO.grid(10, 12)
+    verts, _, _, _ = BO.generate(params, occ, eta, om, bary=VERTS)
+    _, scales, rots, opac = BO.generate(params, occ, eta, om)
+    V = verts.reshape(-1, 3, 3)  # per face: V0, V1, V2
+    for f in range(V.shape[0]):
+        M = quat_to_matrix(rots[f])
+        np.testing.assert_allclose(M.T @ M, np.eye(3), atol=1e-12)
+        n = np.cross(V[f, 1] - V[f, 0], V[f, 2] - V[f, 0])
+        c = V[f].mean(0)
+        d1 = V[f, 1] - c
+        # the reference divides by (|v| + 1e-8), which skews the frame of tiny
+        # triangles; allow that bias
+        tol = 1e-9 + 1e-7 / np.linalg.norm(n) + 1e-7 / np.linalg.norm(d1)
+        np.testing.assert_allclose(M[:, 2], n / np.linalg.norm(n), atol=tol)
+        np.testing.assert_allclose(M[:, 0], d1 / np.linalg.norm(d1), atol=tol)
+        np.testing.assert_allclose(scales[f, 0], np.linalg.norm(d1) / 2, rtol=1e-7, atol=2e-8)
+        np.testing.assert_allclose(scales[f, 1], np.dot(V[f, 2] - c, M[:, 1]) / 2, rtol=1e-5, atol=1e-8 + tol * np.linalg.norm(V[f, 2] - c))
+        assert rots[f, 0] >= 0
+    np.testing.assert_allclose(opac[:, 0], np.repeat(1 / (1 + np.exp(-occ)), 10 * 12))
+
+
+def test_rot_to_quat_matches_reference_golden():
+    g = np.load(os.path.join(GOLDEN, "rot_to_quat.npz"))
+    q = BO.rot_to_quat_batch(g["R"].astype(np.float64))
+    np.testing.assert_allclose(q, g["quat"], atol=2e-6)
+
+
+def test_rot_to_quat_round_trip():
+    rng = np.random.default_rng(2)
+    q = rng.normal(size=(500, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    q[q[:, 0] < 0] *= -1
+    R = np.stack([quat_to_matrix(x) for x in q])
+    np.testing.assert_allclose(BO.rot_to_quat_batch(R), q, atol=1e-12)
