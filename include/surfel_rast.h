@@ -61,6 +61,8 @@ typedef struct {
   int prefiltered;
   int debug;               /* synchronise and check after every stage */
   int tile;
+  int pair_store;          /* accepted-pair store per tile instance (0 = 24);
+                              chunks beyond it recompute their pairs */
 } sr_settings;
 
 /* Per-surfel inputs (device).  Exactly one of shs / colors_precomp and exactly
@@ -124,7 +126,7 @@ int sr_mark_visible(int P, const float* means3D, const float* viewmatrix,
 /* Byte sizes of the three state buffers (what sr_forward asks alloc for). */
 size_t sr_geom_bytes(int P);
 size_t sr_image_bytes(int W, int H, int tile);
-size_t sr_binning_bytes(int num_rendered, int W, int H, int tile);
+size_t sr_binning_bytes(int num_rendered, int W, int H, int tile, int pair_store);
 
 /* Byte offsets of the named arrays inside the state buffers, so callers and
  * tests can read internal state (e.g. "depths", "rec", "tiles_touched",

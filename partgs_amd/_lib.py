@@ -68,6 +68,7 @@ class SrSettings(C.Structure):
         ("prefiltered", C.c_int),
         ("debug", C.c_int),
         ("tile", C.c_int),
+        ("pair_store", C.c_int),
     ]
 
 
@@ -132,7 +133,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_geom_bytes.restype = sz
         lib.sr_image_bytes.argtypes = [i, i, i]
         lib.sr_image_bytes.restype = sz
-        lib.sr_binning_bytes.argtypes = [i, i, i, i]
+        lib.sr_binning_bytes.argtypes = [i, i, i, i, i]
         lib.sr_binning_bytes.restype = sz
         lib.sr_state_offset.argtypes = [C.c_char_p, i, i, i, i, i]
         lib.sr_state_offset.restype = sz

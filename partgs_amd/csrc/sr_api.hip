@@ -123,6 +123,7 @@ int validate(const sr_settings* s, const sr_surfels* g) {
   if (g->P < 0) return fail(SR_ERR_INVALID, "P must be >= 0");
   if (s->image_width <= 0 || s->image_height <= 0 || s->image_width > 32000 || s->image_height > 32000)
     return fail(SR_ERR_INVALID, "image size must be in [1, 32000] (got %d x %d)", s->image_width, s->image_height);
+  if (s->pair_store < 0) return fail(SR_ERR_INVALID, "pair_store must be >= 0");
   if (s->tile != 8 && s->tile != 16 && s->tile != 32)
     return fail(SR_ERR_INVALID, "tile must be 8, 16 or 32 (got %d)", s->tile);
   if (g->P == 0) return SR_OK;  // the reference glue returns before looking at the inputs
@@ -153,9 +154,9 @@ const char* sr_version(void) { return "surfel_rast gfx950 1.0"; }
 
 size_t sr_geom_bytes(int P) { return geom_layout(P < 0 ? 0 : P).total; }
 size_t sr_image_bytes(int W, int H, int tile) { return image_layout(W, H, tile > 0 ? tile : 16).total; }
-size_t sr_binning_bytes(int L, int W, int H, int tile) {
+size_t sr_binning_bytes(int L, int W, int H, int tile, int pair_store) {
   const int B = tile > 0 ? tile : 16;
-  return bin_layout(L, B, image_layout(W, H, B).T).total;
+  return bin_layout(L, B, image_layout(W, H, B).T, pair_store).total;
 }
 
 size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int L) {
@@ -251,7 +252,7 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     return fail(SR_ERR_PREFILTERED, "Point is filtered although prefiltered is set. This shouldn't happen!");
   const int L = (int)host[0];
   *num_rendered = L;
-  const BinLayout bl = bin_layout(L, B, il.T);
+  const BinLayout bl = bin_layout(L, B, il.T, s->pair_store);
   char* bin = (char*)alloc(ctx, bl.total, 1);
   if (!bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
   uint64_t* keys = (uint64_t*)(bin + bl.keys);
@@ -291,6 +292,11 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     aa.work = (const uint2*)(bin + bl.work);
     aa.work_count = counters + 2;
     aa.masks = (uint64_t*)(bin + bl.masks);
+    aa.frag_off = (uint32_t*)(bin + bl.frag_off);
+    aa.frag4 = (float4*)(bin + bl.frag4);
+    aa.frag1 = (float*)(bin + bl.frag1);
+    aa.frag_fill = (unsigned long long*)(counters + 4);
+    aa.frag_cap = (uint32_t)bl.frag_cap;
     SR_STAGE(8, dbg, st, "accept", launch_accept(aa, work_capacity(L, B, il.T), st));
   } else {
     SR_STAGE(3, dbg, st, "clear", launch_fill_u32(sub_count, 0u, (size_t)il.T * quads_per_tile(B), st));
@@ -306,6 +312,8 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
   ba.sub = sub;
   ba.sub_count = sub_count;
   ba.masks = (const uint64_t*)(bin + bl.masks);
+  ba.frag_off = (const uint32_t*)(bin + bl.frag_off);
+  ba.frag4 = (const float4*)(bin + bl.frag4);
   ba.rec = (const float4*)(geom + gl.rec);
   ba.bg = s->bg;
   ba.final_T = (float*)(img + il.final_T);
@@ -333,7 +341,7 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
   const bool dbg = s->debug != 0;
   const GeomLayout gl = geom_layout(P);
   const ImageLayout il = image_layout(W, H, B);
-  const BinLayout bl = bin_layout(R, B, il.T);
+  const BinLayout bl = bin_layout(R, B, il.T, s->pair_store);
   const char* geom = (const char*)geom_buf;
   const char* img = (const char*)img_buf;
   const char* bin = (const char*)bin_buf;
@@ -365,6 +373,9 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
   ba.sub = bin ? (const uint2*)(bin + bl.sub) : nullptr;
   ba.sub_count = (const uint32_t*)(img + il.sub_count);
   ba.masks = bin ? (const uint64_t*)(bin + bl.masks) : nullptr;
+  ba.frag_off = bin ? (const uint32_t*)(bin + bl.frag_off) : nullptr;
+  ba.frag4 = bin ? (const float4*)(bin + bl.frag4) : nullptr;
+  ba.frag1 = bin ? (const float*)(bin + bl.frag1) : nullptr;
   ba.rec = (const float4*)(geom + gl.rec);
   ba.bg = s->bg;
   ba.final_T = (const float*)(img + il.final_T);

@@ -90,9 +90,12 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   bg_dot += a.bg[1] * dLp1;
   bg_dot += a.bg[2] * dLp2;
 
+  const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
   for (int c = (cnt + 63) / 64 - 1; c >= 0; --c) {
     uint64_t mask = (inside && last_contributor > 0) ? masks[(size_t)c * 64 + lane] : 0ull;
     if (__ballot(mask != 0ull) == 0ull) continue;
+    const uint32_t foff = mask ? frag_off[(size_t)c * 64 + lane] : kNoFrag;
+    int r = __popcll(mask);  // store index one past this lane's last pair of the chunk
     const int e0 = c * 64;
     const int m = min(64, cnt - e0);
     __syncthreads();
@@ -112,11 +115,16 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
     while (mask != 0ull) {
       const int j = 63 - __builtin_clzll(mask);
       mask &= ~(1ull << j);
+      --r;
       const uint32_t contributor = s_e[j].y;  // 0-based position in the tile list
       if (contributor >= last_contributor) continue;
       const float4 q2 = s_q2[j];
       Pair e;
-      if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], q2, 0.0f, e)) continue;
+      if (foff != kNoFrag) {
+        load_pair(pixx, pixy, s_q0[j], s_q1[j], q2, a.frag4[foff + r], a.frag1[foff + r], e);
+      } else if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], q2, 0.0f, e)) {
+        continue;
+      }
       const float4 q3 = s_q3[j], q4 = s_q4[j];
       const float alpha = e.alpha, G = e.G, c_d = e.depth;
       const float opa = q2.w;

@@ -18,8 +18,9 @@
 // the 12 directional derivatives of every output are exact; they are
 // contracted with the upstream gradients and summed per block (wave
 // reduction, then one atomic per block parameter and wave).  Surfels whose
-// triangle is nearly degenerate (|n| < 1e-3 |e1| |e2|, e.g. at the poles) are
-// generated and differentiated in FP64.
+// triangle is nearly degenerate (|n| < 1e-3 |e1| |e2|, or |n| or |V1 - c|
+// below 1e-4, where the 1e-8 guards of the normalisations matter: small
+// faces at the poles) are generated and differentiated in FP64.
 #include <hip/hip_runtime.h>
 
 #include "../../include/surfel_rast.h"
@@ -225,8 +226,10 @@ SR_DEV bool surfel(const Block<S>& b, T eta0, T eta1, T om0, T om1, T ratio, T s
   S qn = fsqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
   if (val(qn) < T(1e-12)) qn = qn - qn + T(1e-12);
   for (int k = 0; k < 4; ++k) o.q[k] = q[k] / qn;
+  // Degenerate for FP32: nearly collinear edges, or a face so small that the
+  // 1e-8 guards of the normalisations become visible (> 1e-4 relative).
   const T le = val(fsqrt(dot(e1, e1))) * val(fsqrt(dot(e2, e2)));
-  return val(nn) < T(1e-3) * le;
+  return val(nn) < T(1e-3) * le || val(nn) < T(1e-4) || val(v1n) < T(1e-4);
 }
 
 struct Args {

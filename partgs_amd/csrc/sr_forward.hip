@@ -603,14 +603,43 @@ __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
     }
     __syncthreads();
     uint64_t mask = 0;
+    const float pixx = (float)pxi, pixy = (float)pyi;
     if (inside) {
-      const float pixx = (float)pxi, pixy = (float)pyi;
       for (int j = 0; j < m; ++j) {
         Pair e;
         if (eval_pair<true>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], s_rej[j], e)) mask |= 1ull << j;
       }
     }
-    a.masks[mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane] = mask;
+    const size_t slot = mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane;
+    a.masks[slot] = mask;
+    // reserve a contiguous run of the pair store for the chunk, lane by lane
+    const uint32_t cntp = (uint32_t)__popcll(mask);
+    uint32_t incl = cntp;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    unsigned long long base = 0;
+    if (lane == 0 && total > 0) base = atomicAdd(a.frag_fill, (unsigned long long)total);
+    base = __shfl(base, 0, 64);
+    const bool fits = total > 0 && base + total <= (unsigned long long)a.frag_cap;
+    const uint32_t my = (uint32_t)base + (incl - cntp);
+    a.frag_off[slot] = fits ? my : kNoFrag;
+    if (fits && cntp > 0) {
+      uint64_t mm = mask;
+      uint32_t r = 0;
+      while (mm != 0ull) {
+        const int j = __builtin_ctzll(mm);
+        mm &= mm - 1ull;
+        Pair e;
+        eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e);
+        a.frag4[my + r] = make_float4(e.alpha, e.G, e.depth, e.sx);
+        a.frag1[my + r] = e.sy;
+        ++r;
+      }
+    }
   }
 }
 
@@ -651,9 +680,11 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   float D = 0.0f, M1 = 0.0f, M2 = 0.0f, distortion = 0.0f, median_depth = 0.0f, median_contributor = -1.0f;
   const float mcoef = kFar / (kFar - kNear);
   const int nch = (cnt + 63) / 64;
+  const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
   for (int c = 0; c < nch; ++c) {
     uint64_t mask = done ? 0ull : masks[(size_t)c * 64 + lane];
     if (__ballot(mask != 0ull) == 0ull) continue;
+    uint32_t foff = mask ? frag_off[(size_t)c * 64 + lane] : kNoFrag;
     const int e0 = c * 64;
     const int m = min(64, cnt - e0);
     __syncthreads();
@@ -671,9 +702,17 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
     while (mask != 0ull) {
       const int j = __builtin_ctzll(mask);
       mask &= mask - 1ull;
-      Pair e;
-      if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;
-      const float alpha = e.alpha, depth = e.depth;
+      float alpha, depth;
+      if (foff != kNoFrag) {
+        const float4 f = a.frag4[foff++];
+        alpha = f.x;
+        depth = f.z;
+      } else {
+        Pair e;
+        if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;
+        alpha = e.alpha;
+        depth = e.depth;
+      }
       const float test_T = T * (1.0f - alpha);
       if (test_T < 0.0001f) {
         done = true;

@@ -71,8 +71,17 @@ inline ImageLayout image_layout(int W, int H, int B) {
 
 // Binning state: one entry per (surfel, tile) instance.
 struct BinLayout {
-  size_t keys, keys_tmp, point_list, sub, masks, work, total;
+  size_t keys, keys_tmp, point_list, sub, masks, work, frag_off, frag4, frag1, total;
+  size_t frag_cap;
 };
+// Accepted-pair store: per accepted (pixel, entry) pair the acceptance pass
+// keeps (alpha, G, depth, s.x) and s.y, where G = exp(-rho / 2) and s the
+// intersection in splat coordinates; a lane's pairs of one chunk are
+// contiguous from frag_off[mask slot].  Capacity kFragPerInstance pairs per
+// tile instance; chunks that do not fit get frag_off = kNoFrag and the blend
+// kernels recompute their pairs instead.
+constexpr size_t kFragPerInstance = 24;
+constexpr uint32_t kNoFrag = 0xFFFFFFFFu;
 // Per-pixel acceptance masks: quadrant (t, q) owns mask slots
 // [mask_base, mask_base + 64 * chunks), chunk c of its sub-list at
 // mask_base + 64 c, one 64-bit word per pixel (bit j: entry 64 c + j reaches
@@ -84,7 +93,7 @@ inline size_t work_capacity(int L, int B, int T) {
   const size_t Q = (size_t)quads_per_tile(B);
   return Q * ((size_t)T + ((size_t)(L > 0 ? L : 1) + 63) / 64);
 }
-inline BinLayout bin_layout(int L, int B, int T) {
+inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
   BinLayout b{};
   size_t off = 0;
   const size_t n = (size_t)(L > 0 ? L : 1);
@@ -95,6 +104,12 @@ inline BinLayout bin_layout(int L, int B, int T) {
   b.sub = off; off += align_up(n * Q * 8);  // uint2 (surfel id, position in tile)
   b.masks = off; off += align_up((n * Q + (size_t)kQuad * kQuad * Q * (size_t)T) * 8);
   b.work = off; off += align_up(work_capacity(L, B, T) * 8);
+  const size_t nslots = n * Q + (size_t)kQuad * kQuad * Q * (size_t)T;
+  b.frag_off = off; off += align_up(nslots * 4);
+  b.frag_cap = n * (pair_store > 0 ? (size_t)pair_store : kFragPerInstance);
+  if (b.frag_cap > 0x7fffffffu) b.frag_cap = 0x7fffffffu;
+  b.frag4 = off; off += align_up(b.frag_cap * 16);
+  b.frag1 = off; off += align_up(b.frag_cap * 4);
   b.total = off;
   return b;
 }
@@ -146,6 +161,11 @@ struct AcceptArgs {
   const uint2* work;
   const uint32_t* work_count;
   uint64_t* masks;
+  uint32_t* frag_off;
+  float4* frag4;
+  float* frag1;
+  unsigned long long* frag_fill;
+  uint32_t frag_cap;
 };
 
 struct BlendArgs {
@@ -154,6 +174,8 @@ struct BlendArgs {
   const uint2* sub;
   const uint32_t* sub_count;
   const uint64_t* masks;
+  const uint32_t* frag_off;
+  const float4* frag4;
   const float4* rec;
   const float* bg;
   float* final_T;
@@ -168,6 +190,9 @@ struct BlendBwdArgs {
   const uint2* sub;
   const uint32_t* sub_count;
   const uint64_t* masks;
+  const uint32_t* frag_off;
+  const float4* frag4;
+  const float* frag1;
   const float4* rec;
   const float* bg;
   const float* final_T;

@@ -56,4 +56,29 @@ SR_DEV bool eval_pair(float pixx, float pixy, const float4& q0, const float4& q1
   return !(e.alpha < 1.0f / 255.0f);
 }
 
+// Rebuilds a Pair from its stored (alpha, G, depth, s.x) and s.y: the cheap
+// planes k, l, p.z and the screen offsets are recomputed exactly as in
+// eval_pair, so every field equals eval_pair's.
+SR_DEV void load_pair(float pixx, float pixy, const float4& q0, const float4& q1, const float4& q2,
+                      const float4& f4, float f1, Pair& e) {
+  const float Tux = q0.x, Tuy = q0.y, Tuz = q0.z, Tvx = q0.w, Tvy = q1.x, Tvz = q1.y;
+  const float Twx = q1.z, Twy = q1.w, Twz = q2.x;
+  e.dx = q2.y - pixx;
+  e.dy = q2.z - pixy;
+  e.rho2d = 2.0f * (e.dx * e.dx + e.dy * e.dy);
+  e.kx = pixx * Twx - Tux;
+  e.ky = pixx * Twy - Tuy;
+  e.kz = pixx * Twz - Tuz;
+  e.lx = pixy * Twx - Tvx;
+  e.ly = pixy * Twy - Tvy;
+  e.lz = pixy * Twz - Tvz;
+  e.pz = e.kx * e.ly - e.ky * e.lx;
+  e.alpha = f4.x;
+  e.G = f4.y;
+  e.depth = f4.z;
+  e.sx = f4.w;
+  e.sy = f1;
+  e.rho3d = e.sx * e.sx + e.sy * e.sy;
+}
+
 }  // namespace sr

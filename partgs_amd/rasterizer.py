@@ -64,7 +64,7 @@ class _Arena:
 
 
 def _settings(bg, scale_modifier, viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height, image_width,
-              degree, campos, prefiltered, debug, tile):
+              degree, campos, prefiltered, debug, tile, pair_store=0):
     s = _lib.SrSettings()
     s.image_height = int(image_height)
     s.image_width = int(image_width)
@@ -79,6 +79,7 @@ def _settings(bg, scale_modifier, viewmatrix, projmatrix, tan_fovx, tan_fovy, im
     s.prefiltered = 1 if prefiltered else 0
     s.debug = 1 if debug else 0
     s.tile = int(tile)
+    s.pair_store = int(pair_store)
     return s
 
 
@@ -98,7 +99,7 @@ def _surfels(P, means3D, sh, colors, opacity, scales, rotations, transMat_precom
 
 def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, scale_modifier, transMat_precomp,
                             viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height, image_width, sh, degree,
-                            campos, prefiltered, debug, tile: int = 16):
+                            campos, prefiltered, debug, tile: int = 16, pair_store: int = 0):
     """RasterizeGaussiansCUDA (rasterize_points.cu:40-134).
 
     Returns (num_rendered, color (3,H,W), others (7,H,W), radii (P,) int32,
@@ -121,7 +122,7 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
     others = torch.empty((7, H, W), dtype=torch.float32, device=dev)
     radii = torch.zeros((P,), dtype=torch.int32, device=dev)
     s = _settings(a["bg"], scale_modifier, a["viewmatrix"], a["projmatrix"], tan_fovx, tan_fovy, H, W, degree,
-                  a["campos"], prefiltered, debug, tile)
+                  a["campos"], prefiltered, debug, tile, pair_store)
     g = _surfels(P, means3D, a["sh"], a["colors"], a["opacity"], a["scales"], a["rotations"], a["transMat_precomp"])
     arena = _Arena(dev)
     nr = C.c_int(0)
@@ -136,7 +137,8 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
 
 def rasterize_gaussians_bwd(bg, means3D, radii, colors, scales, rotations, scale_modifier, transMat_precomp,
                             viewmatrix, projmatrix, tan_fovx, tan_fovy, dL_dout_color, dL_dout_others, sh, degree,
-                            campos, geomBuffer, R, binningBuffer, imageBuffer, debug, tile: int = 16):
+                            campos, geomBuffer, R, binningBuffer, imageBuffer, debug, tile: int = 16,
+                            pair_store: int = 0):
     """RasterizeGaussiansBackwardCUDA (rasterize_points.cu:137-233).
 
     Returns (dL_dmeans2D, dL_dcolors, dL_dopacity, dL_dmeans3D, dL_dtransMat,
@@ -155,17 +157,18 @@ def rasterize_gaussians_bwd(bg, means3D, radii, colors, scales, rotations, scale
     H, W = a["dL_dout_color"].size(1), a["dL_dout_color"].size(2)
     M = int(a["sh"].size(1)) if a["sh"].numel() != 0 and a["sh"].size(0) != 0 else 0
     opts = dict(dtype=torch.float32, device=dev)
-    dL_dmeans3D = torch.zeros((P, 3), **opts)
-    dL_dmeans2D = torch.zeros((P, 3), **opts)
-    dL_dcolors = torch.zeros((P, 3), **opts)
-    dL_dopacity = torch.zeros((P, 1), **opts)
-    dL_dtransMat = torch.zeros((P, 9), **opts)
-    dL_dsh = torch.zeros((P, M, 3), **opts)
-    dL_dscales = torch.zeros((P, 2), **opts)
-    dL_drotations = torch.zeros((P, 4), **opts)
+    # the library clears every gradient array before accumulating into it
+    dL_dmeans3D = torch.empty((P, 3), **opts)
+    dL_dmeans2D = torch.empty((P, 3), **opts)
+    dL_dcolors = torch.empty((P, 3), **opts)
+    dL_dopacity = torch.empty((P, 1), **opts)
+    dL_dtransMat = torch.empty((P, 9), **opts)
+    dL_dsh = torch.empty((P, M, 3), **opts)
+    dL_dscales = torch.empty((P, 2), **opts)
+    dL_drotations = torch.empty((P, 4), **opts)
     if P != 0:
         s = _settings(a["bg"], scale_modifier, a["viewmatrix"], a["projmatrix"], tan_fovx, tan_fovy, H, W, degree,
-                      a["campos"], False, debug, tile)
+                      a["campos"], False, debug, tile, pair_store)
         g = _surfels(P, means3D, a["sh"], a["colors"], torch.empty(0, **opts), a["scales"], a["rotations"],
                      a["transMat_precomp"])
         gr = _lib.SrGrads()

@@ -29,13 +29,24 @@ def _run_gpu(params, occ, n_eta, n_omega):
     return grid, p, o, outs
 
 
+def _rotmat(q):
+    """Rotation matrices of (w, x, y, z) quaternions (numpy or torch), flattened."""
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    rows = [w * w + x * x - y * y - z * z, 2 * (x * y - w * z), 2 * (x * z + w * y),
+            2 * (x * y + w * z), w * w - x * x + y * y - z * z, 2 * (y * z - w * x),
+            2 * (x * z - w * y), 2 * (y * z + w * x), w * w - x * x - y * y + z * z]
+    return torch.stack(rows, 1) if isinstance(q, torch.Tensor) else np.stack(rows, 1)
+
+
 def test_block_forward_matches_oracle():
     params, occ = _blocks(0, 16)
     grid, p, o, outs = _run_gpu(params, occ, 64, 64)
     print(f"[blocks] B=16 grid 64x64 P={outs[0].shape[0]}", flush=True)
     ref = BO.generate(p.detach().cpu().numpy().astype(np.float64), o.detach().cpu().numpy().astype(np.float64),
                       grid.eta.cpu().numpy().astype(np.float64), grid.omega.cpu().numpy().astype(np.float64))
-    for got, want, name in zip(outs, ref, ("means3D", "scales", "rotations", "opacities")):
+    outs = [outs[0], outs[1], _rotmat(outs[2]), outs[3]]
+    ref = [ref[0], ref[1], _rotmat(ref[2]), ref[3]]
+    for got, want, name in zip(outs, ref, ("means3D", "scales", "rotation matrices", "opacities")):
         got = got.detach().cpu().numpy().astype(np.float64)
         err = np.abs(got - want)
         tol = 2e-5 * np.abs(want).max() + 1e-4 * np.abs(want)
@@ -43,14 +54,17 @@ def test_block_forward_matches_oracle():
 
 
 def test_block_backward_matches_fd64():
-    """64 random block parameters, FP64 central differences with h = 1e-4."""
+    """64 random block parameters, FP64 central differences with h = 1e-4.
+    The loss reads the rotation through R(q): the reference's w >= 0
+    standardisation flips q -> -q (the same rotation) discontinuously."""
     params, occ = _blocks(1, 16)
     params = params.astype(np.float32).astype(np.float64)
     occ = occ.astype(np.float32).astype(np.float64)
     grid, p, o, outs = _run_gpu(params, occ, 64, 64)
     rng = np.random.default_rng(7)
-    gs = [rng.normal(size=t.shape) for t in outs]
-    loss = sum((t * torch.tensor(g, dtype=torch.float32, device="cuda")).sum() for t, g in zip(outs, gs))
+    feats = [outs[0], outs[1], _rotmat(outs[2]), outs[3]]
+    gs = [rng.normal(size=t.shape) for t in feats]
+    loss = sum((t * torch.tensor(g, dtype=torch.float32, device="cuda")).sum() for t, g in zip(feats, gs))
     loss.backward()
     gp = p.grad.cpu().numpy().astype(np.float64)
     go = o.grad.cpu().numpy().astype(np.float64)
@@ -58,7 +72,8 @@ def test_block_backward_matches_fd64():
     om = grid.omega.cpu().numpy().astype(np.float64)
 
     def f(pp, oo):
-        return sum(float((a * g).sum()) for a, g in zip(BO.generate(pp, oo, eta, om), gs))
+        m, sc, r, a = BO.generate(pp, oo, eta, om)
+        return sum(float((x * g).sum()) for x, g in zip((m, sc, _rotmat(r), a), gs))
 
     h = 1e-4
     picks = rng.choice(16 * 13, size=64, replace=False)

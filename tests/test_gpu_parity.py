@@ -86,10 +86,13 @@ def test_tile_sizes(tile):
     _check_backward(scene, res, o, seed=1)
 
 
-def test_superquadric_sh3_small():
+@pytest.mark.parametrize("pair_store", [0, 1])
+def test_superquadric_sh3_small(pair_store):
+    """pair_store = 1 overflows the accepted-pair store, so most chunks take
+    the recompute path."""
     scene = superquadric_surfels(0, P=20000, width=160, height=120)
     o, _ = oracle_run(scene)
-    res = gpu_forward(scene)
+    res = gpu_forward(scene, pair_store=pair_store)
     _check_forward(scene, res, o)
     _check_backward(scene, res, o, seed=2)
 
