@@ -144,6 +144,59 @@ int sr_set_timing(int enabled);
 int sr_get_timing(double* ms, int* count, int n);
 void sr_reset_timing(void);
 
+/* ---- batched views ------------------------------------------------------
+ * Many views of one surfel set (uneven image sizes allowed) in one pass:
+ * every stage is one launch over all views, nothing is allocated and the
+ * host never waits, so a forward + backward can be captured in a hipGraph
+ * and replayed.  The caller owns a workspace of sr_batch_workspace_bytes()
+ * sized for `capacity` (surfel, tile) instances over all views; when a batch
+ * needs more, the forward sets SR_STATUS_CAPACITY in status[1], leaves the
+ * outputs undefined, and status[0] holds the instances needed.
+ *
+ * No counterpart in the reference, which renders one view per call
+ * (renderer/gaussian_renderer/__init__.py:9-149 inside the training loop);
+ * per view the results equal sr_forward / sr_backward on that view. */
+#define SR_MAX_VIEWS 32
+#define SR_STATUS_CAPACITY 2u
+
+typedef struct {
+  int V;                      /* 1..SR_MAX_VIEWS */
+  const int* widths;          /* host, V */
+  const int* heights;         /* host, V */
+  const float* tanfovx;       /* host, V */
+  const float* tanfovy;       /* host, V */
+  const float* viewmatrices;  /* device, V x 16 */
+  const float* projmatrices;  /* device, V x 16 */
+  const float* campos;        /* device, V x 3 */
+  const float* bg;            /* device, 3 (shared) */
+  float scale_modifier;
+  int sh_degree;
+  int tile;                   /* 8, 16 or 32 */
+  int pair_store;             /* as in sr_settings */
+} sr_views;
+
+size_t sr_batch_workspace_bytes(const sr_views* views, int P, int capacity);
+
+/* Outputs: out_color / out_others hold each view's 3 x H x W / 7 x H x W
+ * planes back to back in view order (view v starts at 3 / 7 times the pixel
+ * count of views 0..v-1); radii is V x P; status (device, 4 words) receives
+ * [0] instances, [1] flags (SR_STATUS_CAPACITY), [2] acceptance work items.
+ * The surfels need scales and rotations (no precomputed T, which is per
+ * view); prefiltered is not supported. */
+int sr_forward_batched(const sr_views* views, const sr_surfels* g, void* workspace, size_t workspace_bytes,
+                       int capacity, float* out_color, float* out_others, int* radii, uint32_t* status,
+                       sr_stream_t stream);
+
+/* Backward of the last sr_forward_batched on the same workspace.  dL_dcolor /
+ * dL_dothers are laid out like the forward outputs.  grads: dL_dmeans2D is
+ * V x P x 3 (each view's screen-space statistic, as the reference's
+ * per-view calls return it); dL_dcolors, dL_dopacity, dL_dmeans3D, dL_dsh,
+ * dL_dscales and dL_drotations are per surfel, summed over the views;
+ * dL_dtransMat is not used. */
+int sr_backward_batched(const sr_views* views, const sr_surfels* g, void* workspace, size_t workspace_bytes,
+                        int capacity, const int* radii, const float* dL_dcolor, const float* dL_dothers,
+                        sr_grads* grads, sr_stream_t stream);
+
 /* ---- superquadric block layer -------------------------------------------
  * Blocks -> attached surfels, BlockGaussianModel.prepare_scaling_rot and
  * get_xyz / get_scaling / get_rotation / get_opacity

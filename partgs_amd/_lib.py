@@ -50,7 +50,13 @@ EXPORTED = (
     "sr_version",
     "sb_generate_fwd",
     "sb_generate_bwd",
+    "sr_batch_workspace_bytes",
+    "sr_forward_batched",
+    "sr_backward_batched",
 )
+
+SR_MAX_VIEWS = 32
+SR_STATUS_CAPACITY = 2
 
 
 class SrSettings(C.Structure):
@@ -96,6 +102,24 @@ class SrGrads(C.Structure):
         ("dL_dsh", C.c_void_p),
         ("dL_dscales", C.c_void_p),
         ("dL_drotations", C.c_void_p),
+    ]
+
+
+class SrViews(C.Structure):
+    _fields_ = [
+        ("V", C.c_int),
+        ("widths", C.POINTER(C.c_int)),
+        ("heights", C.POINTER(C.c_int)),
+        ("tanfovx", C.POINTER(C.c_float)),
+        ("tanfovy", C.POINTER(C.c_float)),
+        ("viewmatrices", C.c_void_p),
+        ("projmatrices", C.c_void_p),
+        ("campos", C.c_void_p),
+        ("bg", C.c_void_p),
+        ("scale_modifier", C.c_float),
+        ("sh_degree", C.c_int),
+        ("tile", C.c_int),
+        ("pair_store", C.c_int),
     ]
 
 
@@ -149,6 +173,13 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sb_generate_fwd.restype = i
         lib.sb_generate_bwd.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         lib.sb_generate_bwd.restype = i
+        lib.sr_batch_workspace_bytes.argtypes = [C.POINTER(SrViews), i, i]
+        lib.sr_batch_workspace_bytes.restype = sz
+        lib.sr_forward_batched.argtypes = [C.POINTER(SrViews), C.POINTER(SrSurfels), vp, sz, i, vp, vp, vp, vp, vp]
+        lib.sr_forward_batched.restype = i
+        lib.sr_backward_batched.argtypes = [C.POINTER(SrViews), C.POINTER(SrSurfels), vp, sz, i, vp, vp, vp,
+                                            C.POINTER(SrGrads), vp]
+        lib.sr_backward_batched.restype = i
         lib.sr_version.argtypes = []
         lib.sr_version.restype = C.c_char_p
         _lib = lib

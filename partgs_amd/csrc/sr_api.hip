@@ -145,6 +145,280 @@ int validate(const sr_settings* s, const sr_surfels* g) {
   return SR_OK;
 }
 
+
+// ---- pipeline stages shared by the single-view and batched entry points ----
+struct Pipeline {
+  ViewSet vs;
+  int P = 0, s_D = 0, prefiltered = 0, pair_store = 0;
+  float mod = 1.0f;
+  const float* vm = nullptr;
+  const float* pm = nullptr;
+  const float* campos = nullptr;
+  const float* bg = nullptr;
+  char* geom = nullptr;
+  char* img = nullptr;
+  char* bin = nullptr;
+  GeomLayout gl{};
+  ImageLayout il{};
+  BinLayout bl{};
+  int capacity = 0;
+  uint32_t* counters() const { return (uint32_t*)(img + il.counters); }
+};
+
+struct RecordGrads {
+  float* transMat;
+  float* mean2D;
+  float* normal;
+  float* opacity;
+  float* colors;
+};
+
+// Workspace of the batched entry points: geometry, image and binning state
+// plus the per-record gradient scratch, each section 256-byte aligned.
+struct BatchLayout {
+  size_t geom, img, bin, g_transMat, g_opacity, g_colors, total;
+  GeomLayout gl;
+  ImageLayout il;
+  BinLayout bl;
+};
+BatchLayout batch_layout(const ViewSet& vs, int P, int capacity, int pair_store) {
+  BatchLayout b{};
+  const size_t R = (size_t)vs.V * (size_t)(P > 0 ? P : 0);
+  b.gl = geom_layout(R);
+  b.il = image_layout(vs);
+  b.bl = bin_layout(capacity, vs.B, (int)vs.T, pair_store);
+  size_t off = 0;
+  b.geom = off; off += align_up(b.gl.total);
+  b.img = off; off += align_up(b.il.total);
+  b.bin = off; off += align_up(b.bl.total);
+  b.g_transMat = off; off += align_up(R * 9 * 4);
+  b.g_opacity = off; off += align_up(R * 4);
+  b.g_colors = off; off += align_up(R * 3 * 4);
+  b.total = off;
+  return b;
+}
+
+int validate_views(const sr_views* v, const sr_surfels* g, int capacity) {
+  if (!v || !g) return fail(SR_ERR_INVALID, "views and surfels must not be NULL");
+  if (v->V < 1 || v->V > SR_MAX_VIEWS) return fail(SR_ERR_INVALID, "V must be in [1, %d]", SR_MAX_VIEWS);
+  if (!v->widths || !v->heights || !v->tanfovx || !v->tanfovy)
+    return fail(SR_ERR_INVALID, "image sizes and fields of view are required");
+  if (!v->viewmatrices || !v->projmatrices || !v->campos || !v->bg)
+    return fail(SR_ERR_INVALID, "viewmatrices, projmatrices, campos and bg are required");
+  if (v->tile != 8 && v->tile != 16 && v->tile != 32)
+    return fail(SR_ERR_INVALID, "tile must be 8, 16 or 32 (got %d)", v->tile);
+  if (v->pair_store < 0) return fail(SR_ERR_INVALID, "pair_store must be >= 0");
+  if (capacity < 1) return fail(SR_ERR_INVALID, "capacity must be >= 1");
+  size_t pixels = 0;
+  for (int i = 0; i < v->V; ++i) {
+    if (v->widths[i] <= 0 || v->heights[i] <= 0 || v->widths[i] > 32000 || v->heights[i] > 32000)
+      return fail(SR_ERR_INVALID, "view %d: image size must be in [1, 32000]", i);
+    pixels += (size_t)v->widths[i] * v->heights[i];
+  }
+  if (pixels > 0x7fffffffu) return fail(SR_ERR_INVALID, "too many pixels in one batch");
+  if (g->P < 1) return fail(SR_ERR_INVALID, "P must be >= 1");
+  if ((size_t)g->P * v->V > 0x7fffffffu) return fail(SR_ERR_INVALID, "P x V must stay below 2^31");
+  if ((g->shs == nullptr) == (g->colors_precomp == nullptr))
+    return fail(SR_ERR_INVALID, "Please provide excatly one of either SHs or precomputed colors!");
+  if (!g->scales || !g->rotations || g->transMat_precomp)
+    return fail(SR_ERR_INVALID, "batched views need scales and rotations (a precomputed T is per view)");
+  if (!g->means3D || !g->opacities) return fail(SR_ERR_INVALID, "means3D and opacities required");
+  if (g->shs && (g->M <= 0 || v->sh_degree > 3 || (v->sh_degree + 1) * (v->sh_degree + 1) > g->M))
+    return fail(SR_ERR_INVALID, "sh_degree %d does not fit M = %d", v->sh_degree, g->M);
+  return SR_OK;
+}
+
+Pipeline batch_pipeline(const sr_views* v, const sr_surfels* g, const ViewSet& vs, const BatchLayout& bl,
+                        char* ws, int capacity) {
+  Pipeline pl{};
+  pl.vs = vs;
+  pl.P = g->P;
+  pl.s_D = v->sh_degree;
+  pl.prefiltered = 0;
+  pl.mod = v->scale_modifier;
+  pl.pair_store = v->pair_store;
+  pl.vm = v->viewmatrices;
+  pl.pm = v->projmatrices;
+  pl.campos = v->campos;
+  pl.bg = v->bg;
+  pl.geom = ws + bl.geom;
+  pl.img = ws + bl.img;
+  pl.bin = ws + bl.bin;
+  pl.gl = bl.gl;
+  pl.il = bl.il;
+  pl.bl = bl.bl;
+  pl.capacity = capacity;
+  return pl;
+}
+
+// Clear counters and histogram, preprocess every (surfel, view), scan the
+// tile histogram; the instance count lands in counters[0].
+int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, bool dbg, hipStream_t st) {
+  uint32_t* counters = pl.counters();
+  uint32_t* tile_count = (uint32_t*)(pl.img + pl.il.tile_count);
+  SR_STAGE(1, dbg, st, "clear", launch_fill_u32(counters, 0u, kAlign / 4, st));
+  SR_STAGE(1, dbg, st, "clear", launch_fill_u32(tile_count, 0u, (size_t)pl.il.T, st));
+  PreprocessArgs pa{};
+  pa.vs = pl.vs;
+  pa.P = pl.P;
+  pa.D = pl.s_D;
+  pa.M = g->M;
+  pa.prefiltered = pl.prefiltered;
+  pa.mod = pl.mod;
+  pa.means3D = g->means3D;
+  pa.scales = g->scales;
+  pa.rots = g->rotations;
+  pa.opac = g->opacities;
+  pa.shs = g->shs;
+  pa.colors = g->colors_precomp;
+  pa.tprec = g->transMat_precomp;
+  pa.vm = pl.vm;
+  pa.pm = pl.pm;
+  pa.campos = pl.campos;
+  pa.rec = (float4*)(pl.geom + pl.gl.rec);
+  pa.depths = (float*)(pl.geom + pl.gl.depths);
+  pa.radii = radii;
+  pa.tiles = (uint32_t*)(pl.geom + pl.gl.tiles);
+  pa.clamped = (uint8_t*)(pl.geom + pl.gl.clamped);
+  pa.cullbox = (short4*)(pl.geom + pl.gl.cullbox);
+  pa.tile_count = tile_count;
+  pa.counters = counters;
+  SR_STAGE(0, dbg, st, "preprocess", launch_preprocess(pa, st));
+  SR_STAGE(1, dbg, st, "scan",
+           launch_scan_tiles(pl.il.T, tile_count, (uint32_t*)(pl.img + pl.il.tile_start),
+                             (uint32_t*)(pl.img + pl.il.tile_fill), (uint32_t*)(pl.img + pl.il.ranges), counters,
+                             capacity, st));
+  return SR_OK;
+}
+
+// Scatter into tile buckets, sort every tile, split into quadrant sub-lists,
+// then the acceptance pass over all (quadrant, chunk) work items.
+int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
+  uint32_t* counters = pl.counters();
+  char* bin = pl.bin;
+  const BinLayout& bl = pl.bl;
+  ScatterArgs sc{};
+  sc.vs = pl.vs;
+  sc.P = pl.P;
+  sc.rec = (const float4*)(pl.geom + pl.gl.rec);
+  sc.depths = (const float*)(pl.geom + pl.gl.depths);
+  sc.radii = radii;
+  sc.counters = counters;
+  sc.tile_fill = (uint32_t*)(pl.img + pl.il.tile_fill);
+  sc.keys = (uint64_t*)(bin + bl.keys);
+  SR_STAGE(2, dbg, st, "scatter", launch_scatter(sc, st));
+  SortArgs sa{};
+  sa.vs = pl.vs;
+  sa.ranges = (const uint32_t*)(pl.img + pl.il.ranges);
+  sa.counters = counters;
+  sa.keys = (uint64_t*)(bin + bl.keys);
+  sa.keys_tmp = (uint64_t*)(bin + bl.keys_tmp);
+  sa.point_list = (uint32_t*)(bin + bl.point_list);
+  sa.cullbox = (const short4*)(pl.geom + pl.gl.cullbox);
+  sa.sub = (uint2*)(bin + bl.sub);
+  sa.sub_count = (uint32_t*)(pl.img + pl.il.sub_count);
+  sa.work = (uint2*)(bin + bl.work);
+  sa.work_count = counters + 2;
+  SR_STAGE(3, dbg, st, "sort", launch_sort_tiles(sa, st));
+  AcceptArgs aa{};
+  aa.vs = pl.vs;
+  aa.ranges = sa.ranges;
+  aa.counters = counters;
+  aa.sub = sa.sub;
+  aa.sub_count = sa.sub_count;
+  aa.rec = sc.rec;
+  aa.work = sa.work;
+  aa.work_count = counters + 2;
+  aa.masks = (uint64_t*)(bin + bl.masks);
+  aa.frag_off = (uint32_t*)(bin + bl.frag_off);
+  aa.fragA = (float2*)(bin + bl.fragA);
+  aa.fragB = (float2*)(bin + bl.fragB);
+  aa.fragC = (float*)(bin + bl.fragC);
+  aa.frag_fill = (unsigned long long*)(counters + 4);
+  aa.frag_cap = (uint32_t)bl.frag_cap;
+  SR_STAGE(8, dbg, st, "accept", launch_accept(aa, work_capacity(pl.capacity, pl.vs.B, pl.il.T), st));
+  return SR_OK;
+}
+
+int run_blend_fwd(Pipeline& pl, float* out_color, float* out_others, bool dbg, hipStream_t st) {
+  BlendArgs ba{};
+  ba.vs = pl.vs;
+  ba.ranges = (const uint32_t*)(pl.img + pl.il.ranges);
+  ba.counters = pl.counters();
+  ba.sub = (const uint2*)(pl.bin + pl.bl.sub);
+  ba.sub_count = (const uint32_t*)(pl.img + pl.il.sub_count);
+  ba.masks = (const uint64_t*)(pl.bin + pl.bl.masks);
+  ba.frag_off = (const uint32_t*)(pl.bin + pl.bl.frag_off);
+  ba.fragA = (const float2*)(pl.bin + pl.bl.fragA);
+  ba.rec = (const float4*)(pl.geom + pl.gl.rec);
+  ba.bg = pl.bg;
+  ba.final_T = (float*)(pl.img + pl.il.final_T);
+  ba.n_contrib = (uint32_t*)(pl.img + pl.il.n_contrib);
+  ba.out_color = out_color;
+  ba.out_others = out_others;
+  SR_STAGE(4, dbg, st, "blend_fwd", launch_blend_fwd(ba, st));
+  return SR_OK;
+}
+
+int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const float* dL_dcolor,
+                 const float* dL_dothers, const RecordGrads& rg, sr_grads* gr, bool blend, bool dbg,
+                 hipStream_t st) {
+  if (blend) {
+    BlendBwdArgs ba{};
+    ba.vs = pl.vs;
+    ba.ranges = (const uint32_t*)(pl.img + pl.il.ranges);
+    ba.counters = pl.counters();
+    ba.sub = (const uint2*)(pl.bin + pl.bl.sub);
+    ba.sub_count = (const uint32_t*)(pl.img + pl.il.sub_count);
+    ba.masks = (const uint64_t*)(pl.bin + pl.bl.masks);
+    ba.frag_off = (const uint32_t*)(pl.bin + pl.bl.frag_off);
+    ba.fragA = (const float2*)(pl.bin + pl.bl.fragA);
+    ba.fragB = (const float2*)(pl.bin + pl.bl.fragB);
+    ba.fragC = (const float*)(pl.bin + pl.bl.fragC);
+    ba.rec = (const float4*)(pl.geom + pl.gl.rec);
+    ba.bg = pl.bg;
+    ba.final_T = (const float*)(pl.img + pl.il.final_T);
+    ba.n_contrib = (const uint32_t*)(pl.img + pl.il.n_contrib);
+    ba.dL_dpix = dL_dcolor;
+    ba.dL_dothers = dL_dothers;
+    ba.dL_dtransMat = rg.transMat;
+    ba.dL_dmean2D = rg.mean2D;
+    ba.dL_dnormal = rg.normal;
+    ba.dL_dopacity = rg.opacity;
+    ba.dL_dcolors = rg.colors;
+    SR_STAGE(6, dbg, st, "blend_bwd", launch_blend_bwd(ba, st));
+  }
+  PreprocessBwdArgs pb{};
+  pb.vs = pl.vs;
+  pb.P = pl.P;
+  pb.D = pl.s_D;
+  pb.M = g->M;
+  pb.means3D = g->means3D;
+  pb.scales = g->scales;
+  pb.rots = g->rotations;
+  pb.shs = g->shs;
+  pb.tprec = g->transMat_precomp;
+  pb.rec = (const float4*)(pl.geom + pl.gl.rec);
+  pb.radii = radii;
+  pb.clamped = (const uint8_t*)(pl.geom + pl.gl.clamped);
+  pb.vm = pl.vm;
+  pb.pm = pl.pm;
+  pb.campos = pl.campos;
+  pb.dL_dtransMat = rg.transMat;
+  pb.dL_dnormal = rg.normal;
+  pb.dL_dcolors = rg.colors;
+  pb.dL_dopacity = rg.opacity;
+  pb.dL_dopacity_out = gr->dL_dopacity;
+  pb.dL_dcolors_out = gr->dL_dcolors;
+  pb.dL_dsh = gr->dL_dsh;
+  pb.dL_dmean2D = rg.mean2D;
+  pb.dL_dmean3D = gr->dL_dmeans3D;
+  pb.dL_dscales = gr->dL_dscales;
+  pb.dL_drots = gr->dL_drotations;
+  SR_STAGE(7, dbg, st, "preprocess_bwd", launch_preprocess_bwd(pb, st));
+  return SR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -152,7 +426,7 @@ extern "C" {
 const char* sr_last_error(void) { return g_err.c_str(); }
 const char* sr_version(void) { return "surfel_rast gfx950 1.0"; }
 
-size_t sr_geom_bytes(int P) { return geom_layout(P < 0 ? 0 : P).total; }
+size_t sr_geom_bytes(int P) { return geom_layout((size_t)(P < 0 ? 0 : P)).total; }
 size_t sr_image_bytes(int W, int H, int tile) { return image_layout(W, H, tile > 0 ? tile : 16).total; }
 size_t sr_binning_bytes(int L, int W, int H, int tile, int pair_store) {
   const int B = tile > 0 ? tile : 16;
@@ -160,7 +434,7 @@ size_t sr_binning_bytes(int L, int W, int H, int tile, int pair_store) {
 }
 
 size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int L) {
-  const GeomLayout g = geom_layout(P);
+  const GeomLayout g = geom_layout((size_t)(P < 0 ? 0 : P));
   const ImageLayout im = image_layout(W, H, tile > 0 ? tile : 16);
   const BinLayout b = bin_layout(L, tile > 0 ? tile : 16, im.T);
   const std::string n(name ? name : "");
@@ -200,128 +474,49 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     SR_STAGE(4, dbg, st, "clear", launch_fill_f32(out_others, 0.0f, 7 * (size_t)W * H, st));
     return SR_OK;
   }
-  const GeomLayout gl = geom_layout(P);
-  const ImageLayout il = image_layout(W, H, B);
+  const ViewSet vs = single_view(W, H, B, s->tanfovx, s->tanfovy);
+  const GeomLayout gl = geom_layout((size_t)P);
+  const ImageLayout il = image_layout(vs);
   char* geom = (char*)alloc(ctx, gl.total, 0);
   char* img = (char*)alloc(ctx, il.total, 2);
   if (!geom || !img) return fail(SR_ERR_ALLOC, "allocation callback failed");
-  uint32_t* counters = (uint32_t*)(img + il.counters);
-  uint32_t* tile_count = (uint32_t*)(img + il.tile_count);
-  uint32_t* tile_start = (uint32_t*)(img + il.tile_start);
-  uint32_t* tile_fill = (uint32_t*)(img + il.tile_fill);
-  uint32_t* ranges = (uint32_t*)(img + il.ranges);
-  // clear counters and the tile histogram (fill kernels, no memset nodes)
-  SR_STAGE(1, dbg, st, "clear", launch_fill_u32(counters, 0u, kAlign / 4, st));
-  SR_STAGE(1, dbg, st, "clear", launch_fill_u32(tile_count, 0u, (size_t)il.T, st));
-  PreprocessArgs pa{};
-  pa.P = P;
-  pa.D = s->sh_degree;
-  pa.M = g->M;
-  pa.W = W;
-  pa.H = H;
-  pa.B = B;
-  pa.gx = il.gx;
-  pa.gy = il.gy;
-  pa.prefiltered = s->prefiltered;
-  pa.mod = s->scale_modifier;
-  pa.means3D = g->means3D;
-  pa.scales = g->scales;
-  pa.rots = g->rotations;
-  pa.opac = g->opacities;
-  pa.shs = g->shs;
-  pa.colors = g->colors_precomp;
-  pa.tprec = g->transMat_precomp;
-  pa.vm = s->viewmatrix;
-  pa.pm = s->projmatrix;
-  pa.campos = s->campos;
-  pa.rec = (float4*)(geom + gl.rec);
-  pa.depths = (float*)(geom + gl.depths);
-  pa.radii = radii;
-  pa.tiles = (uint32_t*)(geom + gl.tiles);
-  pa.clamped = (uint8_t*)(geom + gl.clamped);
-  pa.cullbox = (short4*)(geom + gl.cullbox);
-  pa.tile_count = tile_count;
-  pa.counters = counters;
-  SR_STAGE(0, dbg, st, "preprocess", launch_preprocess(pa, st));
-  SR_STAGE(1, dbg, st, "scan", launch_scan_tiles(il.T, tile_count, tile_start, tile_fill, ranges, counters, st));
+  Pipeline pl{};
+  pl.vs = vs;
+  pl.P = P;
+  pl.s_D = s->sh_degree;
+  pl.prefiltered = s->prefiltered;
+  pl.mod = s->scale_modifier;
+  pl.pair_store = s->pair_store;
+  pl.vm = s->viewmatrix;
+  pl.pm = s->projmatrix;
+  pl.campos = s->campos;
+  pl.bg = s->bg;
+  pl.geom = geom;
+  pl.img = img;
+  pl.gl = gl;
+  pl.il = il;
+  rc = run_front(pl, g, radii, 0xFFFFFFFFu, dbg, st);
+  if (rc != SR_OK) return rc;
   uint32_t* host = pinned_counters();
   if (!host) return fail(SR_ERR_HIP, "hipHostMalloc failed");
-  SR_HIP(hipMemcpyAsync(host, counters, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SR_HIP(hipMemcpyAsync(host, pl.counters(), 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   SR_HIP(hipStreamSynchronize(st));
   if (host[1] & kErrPrefiltered)
     return fail(SR_ERR_PREFILTERED, "Point is filtered although prefiltered is set. This shouldn't happen!");
   const int L = (int)host[0];
   *num_rendered = L;
-  const BinLayout bl = bin_layout(L, B, il.T, s->pair_store);
-  char* bin = (char*)alloc(ctx, bl.total, 1);
-  if (!bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
-  uint64_t* keys = (uint64_t*)(bin + bl.keys);
-  uint64_t* keys_tmp = (uint64_t*)(bin + bl.keys_tmp);
-  uint32_t* point_list = (uint32_t*)(bin + bl.point_list);
-  uint2* sub = (uint2*)(bin + bl.sub);
-  uint32_t* sub_count = (uint32_t*)(img + il.sub_count);
+  pl.bl = bin_layout(L, B, il.T, s->pair_store);
+  pl.bin = (char*)alloc(ctx, pl.bl.total, 1);
+  if (!pl.bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
+  pl.capacity = L;
   if (L > 0) {
-    SR_STAGE(2, dbg, st, "scatter",
-             launch_scatter(P, B, il.gx, il.gy, (const float4*)(geom + gl.rec), (const float*)(geom + gl.depths),
-                            radii, tile_fill, keys, st));
-    SortArgs sa{};
-    sa.W = W;
-    sa.H = H;
-    sa.B = B;
-    sa.gx = il.gx;
-    sa.T = il.T;
-    sa.ranges = ranges;
-    sa.keys = keys;
-    sa.keys_tmp = keys_tmp;
-    sa.point_list = point_list;
-    sa.cullbox = (const short4*)(geom + gl.cullbox);
-    sa.sub = sub;
-    sa.sub_count = sub_count;
-    sa.work = (uint2*)(bin + bl.work);
-    sa.work_count = counters + 2;
-    SR_STAGE(3, dbg, st, "sort", launch_sort_tiles(sa, st));
-    AcceptArgs aa{};
-    aa.W = W;
-    aa.H = H;
-    aa.B = B;
-    aa.gx = il.gx;
-    aa.ranges = ranges;
-    aa.sub = sub;
-    aa.sub_count = sub_count;
-    aa.rec = (const float4*)(geom + gl.rec);
-    aa.work = (const uint2*)(bin + bl.work);
-    aa.work_count = counters + 2;
-    aa.masks = (uint64_t*)(bin + bl.masks);
-    aa.frag_off = (uint32_t*)(bin + bl.frag_off);
-    aa.frag4 = (float4*)(bin + bl.frag4);
-    aa.frag1 = (float*)(bin + bl.frag1);
-    aa.frag_fill = (unsigned long long*)(counters + 4);
-    aa.frag_cap = (uint32_t)bl.frag_cap;
-    SR_STAGE(8, dbg, st, "accept", launch_accept(aa, work_capacity(L, B, il.T), st));
+    rc = run_binning(pl, radii, dbg, st);
+    if (rc != SR_OK) return rc;
   } else {
-    SR_STAGE(3, dbg, st, "clear", launch_fill_u32(sub_count, 0u, (size_t)il.T * quads_per_tile(B), st));
+    SR_STAGE(3, dbg, st, "clear",
+             launch_fill_u32((uint32_t*)(img + il.sub_count), 0u, (size_t)il.T * quads_per_tile(B), st));
   }
-  BlendArgs ba{};
-  ba.W = W;
-  ba.H = H;
-  ba.B = B;
-  ba.gx = il.gx;
-  ba.gy = il.gy;
-  ba.T = il.T;
-  ba.ranges = ranges;
-  ba.sub = sub;
-  ba.sub_count = sub_count;
-  ba.masks = (const uint64_t*)(bin + bl.masks);
-  ba.frag_off = (const uint32_t*)(bin + bl.frag_off);
-  ba.frag4 = (const float4*)(bin + bl.frag4);
-  ba.rec = (const float4*)(geom + gl.rec);
-  ba.bg = s->bg;
-  ba.final_T = (float*)(img + il.final_T);
-  ba.n_contrib = (uint32_t*)(img + il.n_contrib);
-  ba.out_color = out_color;
-  ba.out_others = out_others;
-  SR_STAGE(4, dbg, st, "blend_fwd", launch_blend_fwd(ba, st));
-  return SR_OK;
+  return run_blend_fwd(pl, out_color, out_others, dbg, st);
 }
 
 int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* radii, void* geom_buf,
@@ -339,12 +534,24 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
       !gr->dL_dscales || !gr->dL_drotations || (g->shs && !gr->dL_dsh))
     return fail(SR_ERR_INVALID, "gradient output missing");
   const bool dbg = s->debug != 0;
-  const GeomLayout gl = geom_layout(P);
-  const ImageLayout il = image_layout(W, H, B);
-  const BinLayout bl = bin_layout(R, B, il.T, s->pair_store);
-  const char* geom = (const char*)geom_buf;
-  const char* img = (const char*)img_buf;
-  const char* bin = (const char*)bin_buf;
+  Pipeline pl{};
+  pl.vs = single_view(W, H, B, s->tanfovx, s->tanfovy);
+  pl.P = P;
+  pl.s_D = s->sh_degree;
+  pl.vm = s->viewmatrix;
+  pl.pm = s->projmatrix;
+  pl.campos = s->campos;
+  pl.bg = s->bg;
+  pl.gl = geom_layout((size_t)P);
+  pl.il = image_layout(pl.vs);
+  pl.bl = bin_layout(R, B, pl.il.T, s->pair_store);
+  pl.geom = (char*)geom_buf;
+  pl.img = (char*)img_buf;
+  pl.bin = (char*)bin_buf;
+  pl.capacity = R;
+  // With one view the per-record gradient arrays are the outputs themselves.
+  RecordGrads rg{gr->dL_dtransMat, gr->dL_dmeans2D, (float*)((char*)geom_buf + pl.gl.dnormal), gr->dL_dopacity,
+                 gr->dL_dcolors};
   {
     StageScope sc(5, st);
     hipError_t e = launch_fill_f32(gr->dL_dmeans2D, 0.0f, 3 * (size_t)P, st);
@@ -355,68 +562,74 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
     if (e == hipSuccess && g->shs) e = launch_fill_f32(gr->dL_dsh, 0.0f, (size_t)P * g->M * 3, st);
     if (e == hipSuccess) e = launch_fill_f32(gr->dL_dscales, 0.0f, 2 * (size_t)P, st);
     if (e == hipSuccess) e = launch_fill_f32(gr->dL_drotations, 0.0f, 4 * (size_t)P, st);
+    if (e == hipSuccess) e = launch_fill_f32(rg.normal, 0.0f, 3 * (size_t)P, st);
     if (e != hipSuccess) return fail(SR_ERR_HIP, "clear grads: %s", hipGetErrorString(e));
   }
-  // dL/dnormal is an intermediate between the two backward kernels; it lives
-  // in the geometry buffer (the reference keeps it in a glue-side tensor,
-  // rasterize_points.cu:190).
-  float* dL_dnormal = (float*)((char*)geom_buf + gl.dnormal);
-  SR_STAGE(5, dbg, st, "clear", launch_fill_f32(dL_dnormal, 0.0f, 3 * (size_t)P, st));
-  BlendBwdArgs ba{};
-  ba.W = W;
-  ba.H = H;
-  ba.B = B;
-  ba.gx = il.gx;
-  ba.gy = il.gy;
-  ba.T = il.T;
-  ba.ranges = (const uint32_t*)(img + il.ranges);
-  ba.sub = bin ? (const uint2*)(bin + bl.sub) : nullptr;
-  ba.sub_count = (const uint32_t*)(img + il.sub_count);
-  ba.masks = bin ? (const uint64_t*)(bin + bl.masks) : nullptr;
-  ba.frag_off = bin ? (const uint32_t*)(bin + bl.frag_off) : nullptr;
-  ba.frag4 = bin ? (const float4*)(bin + bl.frag4) : nullptr;
-  ba.frag1 = bin ? (const float*)(bin + bl.frag1) : nullptr;
-  ba.rec = (const float4*)(geom + gl.rec);
-  ba.bg = s->bg;
-  ba.final_T = (const float*)(img + il.final_T);
-  ba.n_contrib = (const uint32_t*)(img + il.n_contrib);
-  ba.dL_dpix = dL_dcolor;
-  ba.dL_dothers = dL_dothers;
-  ba.dL_dtransMat = gr->dL_dtransMat;
-  ba.dL_dmean2D = gr->dL_dmeans2D;
-  ba.dL_dnormal = dL_dnormal;
-  ba.dL_dopacity = gr->dL_dopacity;
-  ba.dL_dcolors = gr->dL_dcolors;
-  if (R > 0) SR_STAGE(6, dbg, st, "blend_bwd", launch_blend_bwd(ba, st));
-  PreprocessBwdArgs pb{};
-  pb.P = P;
-  pb.D = s->sh_degree;
-  pb.M = g->M;
-  pb.W = W;
-  pb.H = H;
-  pb.tanx = s->tanfovx;
-  pb.tany = s->tanfovy;
-  pb.means3D = g->means3D;
-  pb.scales = g->scales;
-  pb.rots = g->rotations;
-  pb.shs = g->shs;
-  pb.tprec = g->transMat_precomp;
-  pb.rec = (const float4*)(geom + gl.rec);
-  pb.radii = radii;
-  pb.clamped = (const uint8_t*)(geom + gl.clamped);
-  pb.vm = s->viewmatrix;
-  pb.pm = s->projmatrix;
-  pb.campos = s->campos;
-  pb.dL_dtransMat = gr->dL_dtransMat;
-  pb.dL_dnormal = dL_dnormal;
-  pb.dL_dcolors = gr->dL_dcolors;
-  pb.dL_dsh = gr->dL_dsh;
-  pb.dL_dmean2D = gr->dL_dmeans2D;
-  pb.dL_dmean3D = gr->dL_dmeans3D;
-  pb.dL_dscales = gr->dL_dscales;
-  pb.dL_drots = gr->dL_drotations;
-  SR_STAGE(7, dbg, st, "preprocess_bwd", launch_preprocess_bwd(pb, st));
+  return run_backward(pl, g, radii, dL_dcolor, dL_dothers, rg, gr, R > 0 && bin_buf != nullptr, dbg, st);
+}
+
+size_t sr_batch_workspace_bytes(const sr_views* v, int P, int capacity) {
+  if (!v || v->V < 1 || v->V > SR_MAX_VIEWS || P < 0 || capacity < 0 || !v->widths || !v->heights) return 0;
+  const int B = v->tile > 0 ? v->tile : 16;
+  return batch_layout(make_viewset(v->V, v->widths, v->heights, v->tanfovx, v->tanfovy, B), P, capacity,
+                      v->pair_store).total;
+}
+
+int sr_forward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t ws_bytes, int capacity,
+                       float* out_color, float* out_others, int* radii, uint32_t* status, sr_stream_t stream_) {
+  hipStream_t st = (hipStream_t)stream_;
+  int rc = validate_views(v, g, capacity);
+  if (rc != SR_OK) return rc;
+  if (!ws || !out_color || !out_others || !radii || !status) return fail(SR_ERR_INVALID, "NULL argument");
+  const ViewSet vs = make_viewset(v->V, v->widths, v->heights, v->tanfovx, v->tanfovy, v->tile);
+  const BatchLayout bl = batch_layout(vs, g->P, capacity, v->pair_store);
+  if (ws_bytes < bl.total)
+    return fail(SR_ERR_INVALID, "workspace holds %zu bytes, needs %zu", ws_bytes, bl.total);
+  Pipeline pl = batch_pipeline(v, g, vs, bl, (char*)ws, capacity);
+  rc = run_front(pl, g, radii, (uint32_t)capacity, false, st);
+  if (rc == SR_OK) rc = run_binning(pl, radii, false, st);
+  if (rc == SR_OK) rc = run_blend_fwd(pl, out_color, out_others, false, st);
+  if (rc != SR_OK) return rc;
+  hipError_t e = launch_copy_status(pl.counters(), status, st);
+  if (e != hipSuccess) return fail(SR_ERR_HIP, "status: %s", hipGetErrorString(e));
   return SR_OK;
+}
+
+int sr_backward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t ws_bytes, int capacity,
+                        const int* radii, const float* dL_dcolor, const float* dL_dothers, sr_grads* gr,
+                        sr_stream_t stream_) {
+  hipStream_t st = (hipStream_t)stream_;
+  int rc = validate_views(v, g, capacity);
+  if (rc != SR_OK) return rc;
+  if (!ws || !radii || !dL_dcolor || !dL_dothers || !gr) return fail(SR_ERR_INVALID, "NULL argument");
+  if (!gr->dL_dmeans2D || !gr->dL_dcolors || !gr->dL_dopacity || !gr->dL_dmeans3D || !gr->dL_dscales ||
+      !gr->dL_drotations || (g->shs && !gr->dL_dsh))
+    return fail(SR_ERR_INVALID, "gradient output missing");
+  const ViewSet vs = make_viewset(v->V, v->widths, v->heights, v->tanfovx, v->tanfovy, v->tile);
+  const BatchLayout bl = batch_layout(vs, g->P, capacity, v->pair_store);
+  if (ws_bytes < bl.total)
+    return fail(SR_ERR_INVALID, "workspace holds %zu bytes, needs %zu", ws_bytes, bl.total);
+  Pipeline pl = batch_pipeline(v, g, vs, bl, (char*)ws, capacity);
+  const size_t P = (size_t)g->P, R = (size_t)vs.V * P;
+  char* w = (char*)ws;
+  RecordGrads rg{(float*)(w + bl.g_transMat), gr->dL_dmeans2D, (float*)(w + bl.geom + pl.gl.dnormal),
+                 (float*)(w + bl.g_opacity), (float*)(w + bl.g_colors)};
+  {
+    StageScope sc(5, st);
+    hipError_t e = launch_fill_f32(rg.transMat, 0.0f, 9 * R, st);
+    if (e == hipSuccess) e = launch_fill_f32(rg.mean2D, 0.0f, 3 * R, st);
+    if (e == hipSuccess) e = launch_fill_f32(rg.normal, 0.0f, 3 * R, st);
+    if (e == hipSuccess) e = launch_fill_f32(rg.opacity, 0.0f, R, st);
+    if (e == hipSuccess) e = launch_fill_f32(rg.colors, 0.0f, 3 * R, st);
+    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dcolors, 0.0f, 3 * P, st);
+    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dopacity, 0.0f, P, st);
+    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dmeans3D, 0.0f, 3 * P, st);
+    if (e == hipSuccess && g->shs) e = launch_fill_f32(gr->dL_dsh, 0.0f, P * g->M * 3, st);
+    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dscales, 0.0f, 2 * P, st);
+    if (e == hipSuccess) e = launch_fill_f32(gr->dL_drotations, 0.0f, 4 * P, st);
+    if (e != hipSuccess) return fail(SR_ERR_HIP, "clear grads: %s", hipGetErrorString(e));
+  }
+  return run_backward(pl, g, radii, dL_dcolor, dL_dothers, rg, gr, true, false, st);
 }
 
 int sr_mark_visible(int P, const float* means3D, const float* viewmatrix, const float* projmatrix,

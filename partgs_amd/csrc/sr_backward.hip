@@ -20,6 +20,7 @@ namespace sr {
 // Partial-gradient slots: 0..8 dL_dtransMat, 9..10 dL_dmean2D.xy,
 // 11..13 dL_dnormal, 14 dL_dopacity, 15..17 dL_dcolors.
 constexpr int kNumG = 18;
+constexpr int kPre = 8;  // stored pairs a lane fetches ahead
 
 SR_DEV void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
 
@@ -40,41 +41,48 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
   __shared__ uint2 s_e[64];
   __shared__ float s_acc[kNumG][64];
-  const int QX = a.B / kQuad, Q = QX * QX;
+  const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
+  if (a.counters[1] & kErrCapacity) return;
   const int tile = blockIdx.x / Q, q = blockIdx.x % Q;
-  const int tx = tile % a.gx, ty = tile / a.gx;
+  const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
+  const int lt = tile - (int)vi.tile_base;
+  const int tx = lt % vi.gx, ty = lt / vi.gx;
   const int lane = threadIdx.x;
-  const uint32_t pxi = (uint32_t)(tx * a.B + (q % QX) * kQuad + (lane & 7));
-  const uint32_t pyi = (uint32_t)(ty * a.B + (q / QX) * kQuad + (lane >> 3));
-  const bool inside = pxi < (uint32_t)a.W && pyi < (uint32_t)a.H;
-  const size_t HW = (size_t)a.H * a.W, pid = (size_t)a.W * pyi + pxi;
+  const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (lane & 7));
+  const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (lane >> 3));
+  const bool inside = pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
+  const size_t HW = (size_t)vi.H * vi.W, pid = (size_t)vi.W * pyi + pxi, pb = vi.pix_base;
+  const float* final_T = a.final_T + 3 * pb;
+  const uint32_t* n_contrib = a.n_contrib + 2 * pb;
+  const float* dL_dpix = a.dL_dpix + 3 * pb;
+  const float* dL_dothers = a.dL_dothers + 7 * pb;
   const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
   const int cnt = (int)a.sub_count[blockIdx.x];
   const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
   const uint64_t* masks = a.masks + mask_base(Q, rbeg, ntile, tile, q);
   const float pixx = (float)pxi, pixy = (float)pyi;
 
-  const float T_final = inside ? a.final_T[pid] : 0.0f;
+  const float T_final = inside ? final_T[pid] : 0.0f;
   float T = T_final;
-  const uint32_t last_contributor = inside ? a.n_contrib[pid] : 0u;
-  const int median_contributor = inside ? (int)a.n_contrib[pid + HW] : 0;
+  const uint32_t last_contributor = inside ? n_contrib[pid] : 0u;
+  const int median_contributor = inside ? (int)n_contrib[pid + HW] : 0;
   float dLp0 = 0.0f, dLp1 = 0.0f, dLp2 = 0.0f;
   float dL_ddepth = 0.0f, dL_daccum = 0.0f, dL_dreg = 0.0f, dL_dmedian = 0.0f;
   float dLn0 = 0.0f, dLn1 = 0.0f, dLn2 = 0.0f;
   float final_D = 0.0f, final_D2 = 0.0f;
   if (inside) {
-    dLp0 = a.dL_dpix[pid];
-    dLp1 = a.dL_dpix[HW + pid];
-    dLp2 = a.dL_dpix[2 * HW + pid];
-    dL_ddepth = a.dL_dothers[pid];
-    dL_daccum = a.dL_dothers[HW + pid];
-    dLn0 = a.dL_dothers[2 * HW + pid];
-    dLn1 = a.dL_dothers[3 * HW + pid];
-    dLn2 = a.dL_dothers[4 * HW + pid];
-    dL_dmedian = a.dL_dothers[5 * HW + pid];
-    dL_dreg = a.dL_dothers[6 * HW + pid];
-    final_D = a.final_T[pid + HW];
-    final_D2 = a.final_T[pid + 2 * HW];
+    dLp0 = dL_dpix[pid];
+    dLp1 = dL_dpix[HW + pid];
+    dLp2 = dL_dpix[2 * HW + pid];
+    dL_ddepth = dL_dothers[pid];
+    dL_daccum = dL_dothers[HW + pid];
+    dLn0 = dL_dothers[2 * HW + pid];
+    dLn1 = dL_dothers[3 * HW + pid];
+    dLn2 = dL_dothers[4 * HW + pid];
+    dL_dmedian = dL_dothers[5 * HW + pid];
+    dL_dreg = dL_dothers[6 * HW + pid];
+    final_D = final_T[pid + HW];
+    final_D2 = final_T[pid + 2 * HW];
   }
   const float final_A = 1.0f - T_final;
   float acc_r0 = 0.0f, acc_r1 = 0.0f, acc_r2 = 0.0f;
@@ -112,19 +120,10 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
 #pragma unroll
     for (int k = 0; k < kNumG; ++k) s_acc[k][lane] = 0.0f;
     __syncthreads();
-    while (mask != 0ull) {
-      const int j = 63 - __builtin_clzll(mask);
-      mask &= ~(1ull << j);
-      --r;
+    // Reverse step of the reference (backward.cu:282-437) for accepted entry j.
+    auto body = [&](const Pair& e, int j) {
       const uint32_t contributor = s_e[j].y;  // 0-based position in the tile list
-      if (contributor >= last_contributor) continue;
       const float4 q2 = s_q2[j];
-      Pair e;
-      if (foff != kNoFrag) {
-        load_pair(pixx, pixy, s_q0[j], s_q1[j], q2, a.frag4[foff + r], a.frag1[foff + r], e);
-      } else if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], q2, 0.0f, e)) {
-        continue;
-      }
       const float4 q3 = s_q3[j], q4 = s_q4[j];
       const float alpha = e.alpha, G = e.G, c_d = e.depth;
       const float opa = q2.w;
@@ -202,6 +201,47 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
         atomicAdd(&s_acc[8][j], dL_dz);
       }
       atomicAdd(&s_acc[14][j], G * dL_dalpha);
+    };
+    if (foff != kNoFrag) {
+      // Pairs are stored in entry order; walk them from the back, fetching
+      // kPre ahead so the loads overlap.
+      while (mask != 0ull) {
+        int jj[kPre];
+        float2 fa[kPre], fb[kPre];
+        float fc[kPre];
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) {
+          jj[k] = -1;
+          fa[k] = fb[k] = make_float2(0.0f, 0.0f);
+          fc[k] = 0.0f;
+          if (mask != 0ull) {
+            const int j = 63 - __builtin_clzll(mask);
+            mask &= ~(1ull << j);
+            --r;
+            jj[k] = j;
+            fa[k] = a.fragA[foff + r];
+            fb[k] = a.fragB[foff + r];
+            fc[k] = a.fragC[foff + r];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) {
+          const int j = jj[k];
+          if (j < 0 || s_e[j].y >= last_contributor) continue;
+          Pair e;
+          load_pair(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], fa[k], fb[k], fc[k], e);
+          body(e, j);
+        }
+      }
+    } else {
+      while (mask != 0ull) {
+        const int j = 63 - __builtin_clzll(mask);
+        mask &= ~(1ull << j);
+        if (s_e[j].y >= last_contributor) continue;  // at or behind the last contributor
+        Pair e;
+        if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;
+        body(e, j);
+      }
     }
     __syncthreads();
     if (lane < m) {
@@ -216,9 +256,9 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
 }
 
 hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s) {
-  if (a.T == 0) return hipSuccess;
-  if (a.B != 8 && a.B != 16 && a.B != 32) return hipErrorInvalidValue;
-  const unsigned nblk = (unsigned)(a.T * quads_per_tile(a.B));
+  if (a.vs.T == 0) return hipSuccess;
+  if (a.vs.B != 8 && a.vs.B != 16 && a.vs.B != 32) return hipErrorInvalidValue;
+  const unsigned nblk = (unsigned)(a.vs.T * quads_per_tile(a.vs.B));
   hipLaunchKernelGGL(k_blend_bwd, dim3(nblk), dim3(64), 0, s, a);
   return hipGetLastError();
 }
@@ -239,10 +279,10 @@ SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, 
   const float len = sqrtf((dir_orig.x * dir_orig.x + dir_orig.y * dir_orig.y) + dir_orig.z * dir_orig.z);
   const F3 dir = f3(dir_orig.x / len, dir_orig.y / len, dir_orig.z / len);
   auto S = [&](int k) { return f3(sh[3 * k], sh[3 * k + 1], sh[3 * k + 2]); };
-  auto put = [&](int k, float c) {
-    dsh[3 * k] = c * dL_dRGB.x;
-    dsh[3 * k + 1] = c * dL_dRGB.y;
-    dsh[3 * k + 2] = c * dL_dRGB.z;
+  auto put = [&](int k, float c) {  // dsh starts cleared; views add up
+    dsh[3 * k] += c * dL_dRGB.x;
+    dsh[3 * k + 1] += c * dL_dRGB.y;
+    dsh[3 * k + 2] += c * dL_dRGB.z;
   };
   F3 dx = f3(0.0f, 0.0f, 0.0f), dy = dx, dz = dx;
   const float x = dir.x, y = dir.y, z = dir.z;
@@ -311,17 +351,40 @@ SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, 
   return r;
 }
 
-// Backward preprocess, dsr/cuda_rasterizer/backward.cu:443-630.
-__global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.P || !(a.radii[idx] > 0)) return;
-  const float focal_x = (float)a.W / (2.0f * a.tanx);
-  const float focal_y = (float)a.H / (2.0f * a.tany);
-  const int W = f2i((focal_x * a.tanx) * 2.0f);
-  const int H = f2i((focal_y * a.tany) * 2.0f);
-  const float* pm = a.pm;
-  const float* vm = a.vm;
-  float* gT = a.dL_dtransMat + 9 * (size_t)idx;
+// SH backward of one record: adds c * dL_dRGB to the surfel's dL_dsh and
+// returns the gradient of the mean through the view direction.
+SR_DEV F3 sh_grad(const PreprocessBwdArgs& a, int v, int idx, size_t r, const float* campos, float m0, float m1,
+                  float m2) {
+  (void)v;
+  F3 dRGB = f3(a.dL_dcolors[3 * r], a.dL_dcolors[3 * r + 1], a.dL_dcolors[3 * r + 2]);
+  const uchar4 cl = reinterpret_cast<const uchar4*>(a.clamped)[r];
+  if (cl.x) dRGB.x = 0.0f;
+  if (cl.y) dRGB.y = 0.0f;
+  if (cl.z) dRGB.z = 0.0f;
+  const F3 dir_orig = f3(m0 - campos[0], m1 - campos[1], m2 - campos[2]);
+  return sh_backward(a.D, a.M, a.shs + (size_t)idx * a.M * 3, dir_orig, dRGB, a.dL_dsh + (size_t)idx * a.M * 3);
+}
+
+// Gradients of one surfel summed over the views that see it.
+struct SurfelGrad {
+  float m[3], sc[2], q[4];
+};
+
+// Backward preprocess of surfel idx in view v (record r),
+// dsr/cuda_rasterizer/backward.cu:443-630.  Adds the view's contributions to
+// g (and to dL_dsh, owned by this thread); rewrites the record's dL_dtransMat
+// (precomputed T) and dL_dmean2D (the densification statistic) in place.
+SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size_t r, SurfelGrad& g) {
+  const ViewInfo& vi = a.vs.v[v];
+  const float tanx = vi.tanx, tany = vi.tany;
+  const float focal_x = (float)vi.W / (2.0f * tanx);
+  const float focal_y = (float)vi.H / (2.0f * tany);
+  const int W = f2i((focal_x * tanx) * 2.0f);
+  const int H = f2i((focal_y * tany) * 2.0f);
+  const float* pm = a.pm + 16 * v;
+  const float* vm = a.vm + 16 * v;
+  const float* campos = a.campos + 3 * v;
+  float* gT = a.dL_dtransMat + 9 * r;
   const float m0 = a.means3D[3 * idx], m1 = a.means3D[3 * idx + 1], m2 = a.means3D[3 * idx + 2];
   const bool precomp = a.tprec != nullptr;
   F3 T0, T1, T2, nrm = f3(0.0f, 0.0f, 0.0f), Rc0, Rc1;
@@ -356,8 +419,8 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
       P4[1][i] = pm[4 * i + 1] * hh + pm[4 * i + 3] * hh1;
       P4[2][i] = pm[4 * i + 3];
     }
-    auto dotM = [&](const F3& v, float v4, int k) {
-      return ((v.x * P4[k][0] + v.y * P4[k][1]) + v.z * P4[k][2]) + v4 * P4[k][3];
+    auto dotM = [&](const F3& u, float u4, int k) {
+      return ((u.x * P4[k][0] + u.y * P4[k][1]) + u.z * P4[k][2]) + u4 * P4[k][3];
     };
     const F3 pc = f3(m0, m1, m2);
     T0 = f3(dotM(L0c, 0.0f, 0), dotM(L1c, 0.0f, 0), dotM(pc, 1.0f, 0));
@@ -367,7 +430,8 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
              (vm[2] * Rc2.x + vm[6] * Rc2.y) + vm[10] * Rc2.z);
   }
   F3 dT0 = f3(gT[0], gT[1], gT[2]), dT1 = f3(gT[3], gT[4], gT[5]), dT2 = f3(gT[6], gT[7], gT[8]);
-  const float gmx = a.dL_dmean2D[3 * idx], gmy = a.dL_dmean2D[3 * idx + 1];
+  float* gm2 = a.dL_dmean2D + 3 * r;
+  const float gmx = gm2[0], gmy = gm2[1];
   if (gmx != 0.0f || gmy != 0.0f) {
     const F3 tv = f3(9.0f, 9.0f, -1.0f);
     const float d = (tv.x * (T2.x * T2.x) + tv.y * (T2.y * T2.y)) + tv.z * (T2.z * T2.z);
@@ -397,7 +461,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
 #pragma unroll
       for (int i = 0; i < 3; ++i)
         dM[cc][i] = (P4[0][i] * comp(dT0, cc) + P4[1][i] * comp(dT1, cc)) + P4[2][i] * comp(dT2, cc);
-    const float* gn = a.dL_dnormal + 3 * (size_t)idx;
+    const float* gn = a.dL_dnormal + 3 * r;
     F3 dL_dtn = f3((vm[0] * gn[0] + vm[1] * gn[1]) + vm[2] * gn[2], (vm[4] * gn[0] + vm[5] * gn[1]) + vm[6] * gn[2],
                    (vm[8] * gn[0] + vm[9] * gn[1]) + vm[10] * gn[2]);
     const float pvx = ((vm[0] * m0 + vm[4] * m1) + vm[8] * m2) + vm[12];
@@ -413,40 +477,76 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
     const F3 dR2 = dL_dtn;
     const float vR[3][3] = {{dR0.x, dR0.y, dR0.z}, {dR1.x, dR1.y, dR1.z}, {dR2.x, dR2.y, dR2.z}};
     const float w = qw, x = qx, y = qy, z = qz;
-    float* gq = a.dL_drots + 4 * (size_t)idx;
-    gq[0] = 2.0f * ((x * (vR[1][2] - vR[2][1]) + y * (vR[2][0] - vR[0][2])) + z * (vR[0][1] - vR[1][0]));
-    gq[1] = 2.0f * ((((-2.0f * x) * (vR[1][1] + vR[2][2]) + y * (vR[0][1] + vR[1][0])) + z * (vR[0][2] + vR[2][0])) +
-                    w * (vR[1][2] - vR[2][1]));
-    gq[2] = 2.0f * (((x * (vR[0][1] + vR[1][0]) - (2.0f * y) * (vR[0][0] + vR[2][2])) + z * (vR[1][2] + vR[2][1])) +
-                    w * (vR[2][0] - vR[0][2]));
-    gq[3] = 2.0f * (((x * (vR[0][2] + vR[2][0]) + y * (vR[1][2] + vR[2][1])) - (2.0f * z) * (vR[0][0] + vR[1][1])) +
-                    w * (vR[0][1] - vR[1][0]));
-    a.dL_dscales[2 * idx] = (dRS0.x * Rc0.x + dRS0.y * Rc0.y) + dRS0.z * Rc0.z;
-    a.dL_dscales[2 * idx + 1] = (dRS1.x * Rc1.x + dRS1.y * Rc1.y) + dRS1.z * Rc1.z;
-    a.dL_dmean3D[3 * idx] = dM[2][0];
-    a.dL_dmean3D[3 * idx + 1] = dM[2][1];
-    a.dL_dmean3D[3 * idx + 2] = dM[2][2];
+    g.q[0] += 2.0f * ((x * (vR[1][2] - vR[2][1]) + y * (vR[2][0] - vR[0][2])) + z * (vR[0][1] - vR[1][0]));
+    g.q[1] += 2.0f * ((((-2.0f * x) * (vR[1][1] + vR[2][2]) + y * (vR[0][1] + vR[1][0])) + z * (vR[0][2] + vR[2][0])) +
+                      w * (vR[1][2] - vR[2][1]));
+    g.q[2] += 2.0f * (((x * (vR[0][1] + vR[1][0]) - (2.0f * y) * (vR[0][0] + vR[2][2])) + z * (vR[1][2] + vR[2][1])) +
+                      w * (vR[2][0] - vR[0][2]));
+    g.q[3] += 2.0f * (((x * (vR[0][2] + vR[2][0]) + y * (vR[1][2] + vR[2][1])) - (2.0f * z) * (vR[0][0] + vR[1][1])) +
+                      w * (vR[0][1] - vR[1][0]));
+    g.sc[0] += (dRS0.x * Rc0.x + dRS0.y * Rc0.y) + dRS0.z * Rc0.z;
+    g.sc[1] += (dRS1.x * Rc1.x + dRS1.y * Rc1.y) + dRS1.z * Rc1.z;
+    F3 dm3 = f3(dM[2][0], dM[2][1], dM[2][2]);
+    if (a.shs != nullptr) dm3 = dm3 + sh_grad(a, v, idx, r, campos, m0, m1, m2);
+    g.m[0] += dm3.x;
+    g.m[1] += dm3.y;
+    g.m[2] += dm3.z;
+  } else if (a.shs != nullptr) {
+    const F3 dm3 = sh_grad(a, v, idx, r, campos, m0, m1, m2);
+    g.m[0] += dm3.x;
+    g.m[1] += dm3.y;
+    g.m[2] += dm3.z;
   }
-  if (a.shs != nullptr) {
-    F3 dRGB = f3(a.dL_dcolors[3 * idx], a.dL_dcolors[3 * idx + 1], a.dL_dcolors[3 * idx + 2]);
-    const uchar4 cl = reinterpret_cast<const uchar4*>(a.clamped)[idx];
-    if (cl.x) dRGB.x = 0.0f;
-    if (cl.y) dRGB.y = 0.0f;
-    if (cl.z) dRGB.z = 0.0f;
-    const F3 dir_orig = f3(m0 - a.campos[0], m1 - a.campos[1], m2 - a.campos[2]);
-    const F3 dm = sh_backward(a.D, a.M, a.shs + (size_t)idx * a.M * 3, dir_orig, dRGB,
-                              a.dL_dsh + (size_t)idx * a.M * 3);
-    a.dL_dmean3D[3 * idx] += dm.x;
-    a.dL_dmean3D[3 * idx + 1] += dm.y;
-    a.dL_dmean3D[3 * idx + 2] += dm.z;
+  const float depth = a.rec[(size_t)kRecQuads * r + 2].x;  // Tw.z = transMat[8]
+  gm2[0] = (float)(((double)(gT[2] * depth) * 0.5) * (double)(float)W);
+  gm2[1] = (float)(((double)(gT[5] * depth) * 0.5) * (double)(float)H);
+}
+
+// One thread per surfel, looping over the views; with one view every sum is
+// 0 + x, so the results equal the single-view formulas bit for bit.
+__global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.P) return;
+  SurfelGrad g{};
+  bool any = false;
+  const bool sum_rec = a.dL_dopacity_out != a.dL_dopacity;
+  float op = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+  for (int v = 0; v < a.vs.V; ++v) {
+    const size_t r = (size_t)v * a.P + idx;
+    if (!(a.radii[r] > 0)) continue;
+    any = true;
+    preprocess_bwd_view(a, v, idx, r, g);
+    if (sum_rec) {
+      op += a.dL_dopacity[r];
+      c0 += a.dL_dcolors[3 * r];
+      c1 += a.dL_dcolors[3 * r + 1];
+      c2 += a.dL_dcolors[3 * r + 2];
+    }
   }
-  const float depth = a.rec[(size_t)kRecQuads * idx + 2].x;  // Tw.z = transMat[8]
-  a.dL_dmean2D[3 * idx] = (float)(((double)(gT[2] * depth) * 0.5) * (double)(float)W);
-  a.dL_dmean2D[3 * idx + 1] = (float)(((double)(gT[5] * depth) * 0.5) * (double)(float)H);
+  if (!any) return;
+  if (a.tprec == nullptr) {
+    a.dL_drots[4 * idx] = g.q[0];
+    a.dL_drots[4 * idx + 1] = g.q[1];
+    a.dL_drots[4 * idx + 2] = g.q[2];
+    a.dL_drots[4 * idx + 3] = g.q[3];
+    a.dL_dscales[2 * idx] = g.sc[0];
+    a.dL_dscales[2 * idx + 1] = g.sc[1];
+  }
+  if (a.tprec == nullptr || a.shs != nullptr) {
+    a.dL_dmean3D[3 * idx] = g.m[0];
+    a.dL_dmean3D[3 * idx + 1] = g.m[1];
+    a.dL_dmean3D[3 * idx + 2] = g.m[2];
+  }
+  if (sum_rec) {
+    a.dL_dopacity_out[idx] = op;
+    a.dL_dcolors_out[3 * idx] = c0;
+    a.dL_dcolors_out[3 * idx + 1] = c1;
+    a.dL_dcolors_out[3 * idx + 2] = c2;
+  }
 }
 
 hipError_t launch_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s) {
-  if (a.P == 0) return hipSuccess;
+  if (a.P == 0 || a.vs.V == 0) return hipSuccess;
   hipLaunchKernelGGL(k_preprocess_bwd, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }

@@ -146,19 +146,23 @@ SR_DEV short4 cull_box(F3 Tu, F3 Tv, F3 Tw, float mx, float my, float opa, int W
   return make_short4((short)x0, (short)y0, (short)x1, (short)y1);
 }
 
-// Forward preprocess of one surfel, dsr/cuda_rasterizer/forward.cu:75-145,
-// 148-251.  Returns false for culled surfels, else its tile rectangle.
-SR_DEV bool preprocess_one(const PreprocessArgs& a, int idx, int& x0, int& y0, int& x1, int& y1) {
-  a.radii[idx] = 0;
-  a.tiles[idx] = 0;
+// Forward preprocess of surfel idx in view v (record r = v * P + idx),
+// dsr/cuda_rasterizer/forward.cu:75-145, 148-251.  Returns false for culled
+// surfels, else its tile rectangle in the view's tile grid.
+SR_DEV bool preprocess_one(const PreprocessArgs& a, int v, int idx, size_t r, int& x0, int& y0, int& x1,
+                           int& y1) {
+  a.radii[r] = 0;
+  a.tiles[r] = 0;
   const float m0 = a.means3D[3 * idx], m1 = a.means3D[3 * idx + 1], m2 = a.means3D[3 * idx + 2];
-  const float* vm = a.vm;
-  const float* pm = a.pm;
+  const float* vm = a.vm + 16 * v;
+  const float* pm = a.pm + 16 * v;
+  const float* campos = a.campos + 3 * v;
+  const int W = a.vs.v[v].W, H = a.vs.v[v].H, gx = a.vs.v[v].gx, gy = a.vs.v[v].gy;
   const float vx = ((vm[0] * m0 + vm[4] * m1) + vm[8] * m2) + vm[12];
   const float vy = ((vm[1] * m0 + vm[5] * m1) + vm[9] * m2) + vm[13];
   const float vz = ((vm[2] * m0 + vm[6] * m1) + vm[10] * m2) + vm[14];
   if (vz <= 0.2f) {
-    if (a.prefiltered) a.counters[1] = kErrPrefiltered;
+    if (a.prefiltered) atomicOr(&a.counters[1], kErrPrefiltered);
     return false;
   }
   F3 Tu, Tv, Tw, nrm;
@@ -178,8 +182,8 @@ SR_DEV bool preprocess_one(const PreprocessArgs& a, int idx, int& x0, int& y0, i
     const float a10 = rowdot(L1, 0), a11 = rowdot(L1, 1), a13 = rowdot(L1, 3);
     const F3 pc = f3(m0, m1, m2);
     const float a20 = rowdot(pc, 0) + pm[12], a21 = rowdot(pc, 1) + pm[13], a23 = rowdot(pc, 3) + pm[15];
-    const float hw = (float)a.W / 2.0f, hw1 = (float)(a.W - 1) / 2.0f;
-    const float hh = (float)a.H / 2.0f, hh1 = (float)(a.H - 1) / 2.0f;
+    const float hw = (float)W / 2.0f, hw1 = (float)(W - 1) / 2.0f;
+    const float hh = (float)H / 2.0f, hh1 = (float)(H - 1) / 2.0f;
     Tu = f3(a00 * hw + a03 * hw1, a10 * hw + a13 * hw1, a20 * hw + a23 * hw1);
     Tv = f3(a01 * hh + a03 * hh1, a11 * hh + a13 * hh1, a21 * hh + a23 * hh1);
     Tw = f3(a03, a13, a23);
@@ -208,12 +212,12 @@ SR_DEV bool preprocess_one(const PreprocessArgs& a, int idx, int& x0, int& y0, i
   const float hy = sqrtf(fmax_(1e-4f, h0y));
   const float radius = ceilf(fmax_(fmax_(hx, hy), 3.0f * kFilterSize));
   const int ir = f2i(radius);
-  get_rect(px, py, ir, a.B, a.gx, a.gy, x0, y0, x1, y1);
+  get_rect(px, py, ir, a.vs.B, gx, gy, x0, y0, x1, y1);
   const uint32_t area = (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
   if (area == 0) return false;
   F3 rgb;
   if (a.colors == nullptr) {
-    const F3 dir0 = f3(m0 - a.campos[0], m1 - a.campos[1], m2 - a.campos[2]);
+    const F3 dir0 = f3(m0 - campos[0], m1 - campos[1], m2 - campos[2]);
     const float len = sqrtf((dir0.x * dir0.x + dir0.y * dir0.y) + dir0.z * dir0.z);
     const F3 dir = f3(dir0.x / len, dir0.y / len, dir0.z / len);
     uint8_t cl[3];
@@ -223,23 +227,23 @@ SR_DEV bool preprocess_one(const PreprocessArgs& a, int idx, int& x0, int& y0, i
     c4.y = cl[1];
     c4.z = cl[2];
     c4.w = 0;
-    reinterpret_cast<uchar4*>(a.clamped)[idx] = c4;
+    reinterpret_cast<uchar4*>(a.clamped)[r] = c4;
   } else {
     rgb = f3(a.colors[3 * idx], a.colors[3 * idx + 1], a.colors[3 * idx + 2]);
   }
   const float opa = a.opac[idx];
   float rho_rej;
-  const short4 box = cull_box(Tu, Tv, Tw, px, py, opa, a.W, a.H, rho_rej);
-  a.depths[idx] = vz;
-  a.radii[idx] = ir;
-  float4* r = a.rec + (size_t)kRecQuads * idx;
-  r[0] = make_float4(Tu.x, Tu.y, Tu.z, Tv.x);
-  r[1] = make_float4(Tv.y, Tv.z, Tw.x, Tw.y);
-  r[2] = make_float4(Tw.z, px, py, opa);
-  r[3] = make_float4(nrm.x, nrm.y, nrm.z, rgb.x);
-  r[4] = make_float4(rgb.y, rgb.z, rho_rej, 0.0f);
-  a.cullbox[idx] = box;
-  a.tiles[idx] = area;
+  const short4 box = cull_box(Tu, Tv, Tw, px, py, opa, W, H, rho_rej);
+  a.depths[r] = vz;
+  a.radii[r] = ir;
+  float4* rc = a.rec + (size_t)kRecQuads * r;
+  rc[0] = make_float4(Tu.x, Tu.y, Tu.z, Tv.x);
+  rc[1] = make_float4(Tv.y, Tv.z, Tw.x, Tw.y);
+  rc[2] = make_float4(Tw.z, px, py, opa);
+  rc[3] = make_float4(nrm.x, nrm.y, nrm.z, rgb.x);
+  rc[4] = make_float4(rgb.y, rgb.z, rho_rej, 0.0f);
+  a.cullbox[r] = box;
+  a.tiles[r] = area;
   return true;
 }
 
@@ -248,36 +252,40 @@ SR_DEV bool preprocess_one(const PreprocessArgs& a, int idx, int& x0, int& y0, i
 // keeps hot tiles (thousands of instances) off a single global counter.
 constexpr int kHistTiles = 4096;
 
+// Grid: (surfel blocks, views); a workgroup's surfels all belong to view
+// blockIdx.y, so its tile histogram is that view's.
 __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
   __shared__ uint32_t s_hist[kHistTiles];
-  const int T = a.gx * a.gy;
+  const int v = blockIdx.y;
+  const int gx = a.vs.v[v].gx;
+  const int T = gx * a.vs.v[v].gy;
+  uint32_t* tile_count = a.tile_count + a.vs.v[v].tile_base;
   const bool lds = T <= kHistTiles;
   if (lds)
     for (int t = threadIdx.x; t < T; t += blockDim.x) s_hist[t] = 0;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
-  const bool ok = idx < a.P && preprocess_one(a, idx, x0, y0, x1, y1);
+  const bool ok = idx < a.P && preprocess_one(a, v, idx, (size_t)v * a.P + idx, x0, y0, x1, y1);
   if (lds) __syncthreads();
   if (ok) {
     for (int ty = y0; ty < y1; ++ty)
       for (int tx = x0; tx < x1; ++tx) {
-        if (lds) atomicAdd(&s_hist[ty * a.gx + tx], 1u);
-        else atomicAdd(&a.tile_count[ty * a.gx + tx], 1u);
+        if (lds) atomicAdd(&s_hist[ty * gx + tx], 1u);
+        else atomicAdd(&tile_count[ty * gx + tx], 1u);
       }
   }
   if (lds) {
     __syncthreads();
     for (int t = threadIdx.x; t < T; t += blockDim.x) {
       const uint32_t c = s_hist[t];
-      if (c) atomicAdd(&a.tile_count[t], c);
+      if (c) atomicAdd(&tile_count[t], c);
     }
   }
 }
 
-
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
-  if (a.P == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_preprocess, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+  if (a.P == 0 || a.vs.V == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_preprocess, dim3((a.P + 255) / 256, a.vs.V), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -285,7 +293,8 @@ hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
 constexpr int kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32_t* count,
                                                               uint32_t* start, uint32_t* fill,
-                                                              uint32_t* ranges, uint32_t* counters) {
+                                                              uint32_t* ranges, uint32_t* counters,
+                                                              uint32_t capacity) {
   __shared__ uint32_t s_part[kScanThreads];
   const int tid = threadIdx.x;
   const int per = (T + kScanThreads - 1) / kScanThreads;
@@ -314,13 +323,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
   if (tid == kScanThreads - 1) {
     start[T] = s_part[kScanThreads - 1];
     counters[0] = s_part[kScanThreads - 1];
+    if (s_part[kScanThreads - 1] > capacity) atomicOr(&counters[1], kErrCapacity);
   }
 }
 
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
-                             uint32_t* ranges, uint32_t* counters, hipStream_t s) {
+                             uint32_t* ranges, uint32_t* counters, uint32_t capacity, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, T, count, start, fill, ranges,
-                     counters);
+                     counters, capacity);
   return hipGetLastError();
 }
 
@@ -328,22 +338,26 @@ hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint
 // buckets.  With few enough tiles, each workgroup first counts its instances
 // per tile in LDS, reserves one contiguous block per tile with a single
 // global atomic, then hands out slots inside the block with LDS atomics.
-__global__ __launch_bounds__(256) void k_scatter(int P, int B, int gx, int gy, const float4* rec,
-                                                 const float* depths, const int* radii,
-                                                 uint32_t* tile_fill, uint64_t* keys) {
+__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   __shared__ uint32_t s_cnt[kHistTiles];
   __shared__ uint32_t s_base[kHistTiles];
+  if (a.counters[1] & kErrCapacity) return;
+  const int v = blockIdx.y;
+  const int B = a.vs.B, gx = a.vs.v[v].gx, gy = a.vs.v[v].gy;
   const int T = gx * gy;
+  uint32_t* tile_fill = a.tile_fill + a.vs.v[v].tile_base;
   const bool lds = T <= kHistTiles;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t r = (size_t)v * a.P + idx;
   int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
   uint64_t lo = 0;
-  const bool ok = idx < P && radii[idx] > 0;
+  const bool ok = idx < a.P && a.radii[r] > 0;
   if (ok) {
-    const float4 q2 = rec[(size_t)kRecQuads * idx + 2];
-    get_rect(q2.y, q2.z, radii[idx], B, gx, gy, x0, y0, x1, y1);
-    lo = ((uint64_t)__float_as_uint(depths[idx]) << 32) | (uint32_t)idx;
+    const float4 q2 = a.rec[(size_t)kRecQuads * r + 2];
+    get_rect(q2.y, q2.z, a.radii[r], B, gx, gy, x0, y0, x1, y1);
+    lo = ((uint64_t)__float_as_uint(a.depths[r]) << 32) | (uint32_t)r;
   }
+  uint64_t* keys = a.keys;
   if (!lds) {
     if (ok)
       for (int ty = y0; ty < y1; ++ty)
@@ -370,11 +384,9 @@ __global__ __launch_bounds__(256) void k_scatter(int P, int B, int gx, int gy, c
       }
 }
 
-hipError_t launch_scatter(int P, int B, int gx, int gy, const float4* rec, const float* depths,
-                          const int* radii, uint32_t* tile_fill, uint64_t* keys, hipStream_t s) {
-  if (P == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scatter, dim3((P + 255) / 256), dim3(256), 0, s, P, B, gx, gy, rec, depths, radii,
-                     tile_fill, keys);
+hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s) {
+  if (a.P == 0 || a.vs.V == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter, dim3((a.P + 255) / 256, a.vs.V), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -438,9 +450,11 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
   constexpr int kWaves = NT / 64;
   __shared__ uint32_t s_wcnt[kMaxQuads][kWaves];
   __shared__ uint32_t s_total[kMaxQuads];
-  const int QX = a.B / kQuad, Q = QX * QX;
+  const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tx = tile % a.gx, ty = tile / a.gx;
+  const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
+  const int ltile = tile - (int)vi.tile_base;
+  const int tx = ltile % vi.gx, ty = ltile / vi.gx;
   uint2* base = a.sub + (size_t)Q * beg;
   if (tid < kMaxQuads) s_total[tid] = 0;
   __syncthreads();
@@ -460,8 +474,8 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
     for (int q = 0; q < kMaxQuads; ++q) {
       rank[q] = 0;
       if (q < Q) {
-        const int qx0 = tx * a.B + (q % QX) * kQuad, qy0 = ty * a.B + (q / QX) * kQuad;
-        const bool hit = valid && qx0 < a.W && qy0 < a.H && bx.x <= qx0 + kQuad - 1 && bx.z >= qx0 &&
+        const int qx0 = tx * B + (q % QX) * kQuad, qy0 = ty * B + (q / QX) * kQuad;
+        const bool hit = valid && qx0 < vi.W && qy0 < vi.H && bx.x <= qx0 + kQuad - 1 && bx.z >= qx0 &&
                          bx.y <= qy0 + kQuad - 1 && bx.w >= qy0;
         const uint64_t m = __ballot(hit);
         if (lane == 0) s_wcnt[q][wave] = (uint32_t)__popcll(m);
@@ -501,12 +515,13 @@ template <int NT, int CAP, bool kLarge>
 __global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
   __shared__ uint64_t s_keys[CAP];
   const int tile = blockIdx.x;
+  if (a.counters[1] & kErrCapacity) return;
   const uint32_t beg = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
   const int n = (int)(end - beg);
   const int tid = threadIdx.x;
   if (kLarge != (n > kSmallMax)) return;
   if (n <= 0) {
-    const int Q = quads_per_tile(a.B);
+    const int Q = quads_per_tile(a.vs.B);
     if (tid < Q) a.sub_count[(size_t)tile * Q + tid] = 0;
     return;
   }
@@ -559,11 +574,11 @@ __global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
 }
 
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
-  if (a.T == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_sort_tiles<256, kSmallMax, false>), dim3(a.T), dim3(256), 0, s, a);
+  if (a.vs.T == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_sort_tiles<256, kSmallMax, false>), dim3(a.vs.T), dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_sort_tiles<1024, kLargeCap, true>), dim3(a.T), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL((k_sort_tiles<1024, kLargeCap, true>), dim3(a.vs.T), dim3(1024), 0, s, a);
   return hipGetLastError();
 }
 
@@ -577,17 +592,20 @@ hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
 __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64];
   __shared__ float s_rej[64];
-  const int QX = a.B / kQuad, Q = QX * QX;
+  const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const int lane = threadIdx.x;
+  if (a.counters[1] & kErrCapacity) return;
   const uint32_t nwork = *a.work_count;
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
     const uint2 item = a.work[w];
     const int qi = (int)item.x, c = (int)item.y;
     const int tile = qi / Q, q = qi % Q;
-    const int tx = tile % a.gx, ty = tile / a.gx;
-    const uint32_t pxi = (uint32_t)(tx * a.B + (q % QX) * kQuad + (lane & 7));
-    const uint32_t pyi = (uint32_t)(ty * a.B + (q / QX) * kQuad + (lane >> 3));
-    const bool inside = pxi < (uint32_t)a.W && pyi < (uint32_t)a.H;
+    const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
+    const int lt = tile - (int)vi.tile_base;
+    const int tx = lt % vi.gx, ty = lt / vi.gx;
+    const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (lane & 7));
+    const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (lane >> 3));
+    const bool inside = pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
     const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
     const int cnt = (int)a.sub_count[qi];
     const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
@@ -635,8 +653,9 @@ __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
         mm &= mm - 1ull;
         Pair e;
         eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e);
-        a.frag4[my + r] = make_float4(e.alpha, e.G, e.depth, e.sx);
-        a.frag1[my + r] = e.sy;
+        a.fragA[my + r] = make_float2(e.alpha, e.depth);
+        a.fragB[my + r] = make_float2(e.G, e.sx);
+        a.fragC[my + r] = e.sy;
         ++r;
       }
     }
@@ -658,16 +677,21 @@ hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s) {
 // skips, so T, the accumulators and the contributor counts (positions in the
 // full tile list) come out bit-identical.
 // ---------------------------------------------------------------------------
+constexpr int kPre = 8;  // stored pairs a lane fetches ahead in the blend kernels
+
 __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
   __shared__ uint32_t s_pos[64];
-  const int QX = a.B / kQuad, Q = QX * QX;
+  const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
+  if (a.counters[1] & kErrCapacity) return;
   const int tile = blockIdx.x / Q, q = blockIdx.x % Q;
-  const int tx = tile % a.gx, ty = tile / a.gx;
+  const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
+  const int lt = tile - (int)vi.tile_base;
+  const int tx = lt % vi.gx, ty = lt / vi.gx;
   const int lane = threadIdx.x;
-  const uint32_t pxi = (uint32_t)(tx * a.B + (q % QX) * kQuad + (lane & 7));
-  const uint32_t pyi = (uint32_t)(ty * a.B + (q / QX) * kQuad + (lane >> 3));
-  const bool inside = pxi < (uint32_t)a.W && pyi < (uint32_t)a.H;
+  const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (lane & 7));
+  const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (lane >> 3));
+  const bool inside = pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
   bool done = !inside;
   const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
   const int cnt = (int)a.sub_count[blockIdx.x];
@@ -699,26 +723,12 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
       s_q4[lane] = r[4];
     }
     __syncthreads();
-    while (mask != 0ull) {
-      const int j = __builtin_ctzll(mask);
-      mask &= mask - 1ull;
-      float alpha, depth;
-      if (foff != kNoFrag) {
-        const float4 f = a.frag4[foff++];
-        alpha = f.x;
-        depth = f.z;
-      } else {
-        Pair e;
-        if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;
-        alpha = e.alpha;
-        depth = e.depth;
-      }
+    // One front-to-back step of the reference's update (forward.cu:383-415);
+    // false once T would drop below 1e-4 (the pixel is done).
+    auto step = [&](float alpha, float depth, uint32_t pos, const float4& q3, float q4x, float q4y) -> bool {
       const float test_T = T * (1.0f - alpha);
-      if (test_T < 0.0001f) {
-        done = true;
-        break;
-      }
-      const uint32_t contributor = s_pos[j] + 1u;
+      if (test_T < 0.0001f) return false;
+      const uint32_t contributor = pos + 1u;
       const float w = alpha * T;
       const float A = 1.0f - T;
       const float mm = mcoef * (1.0f - kNear / depth);
@@ -730,42 +740,86 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
         median_depth = depth;
         median_contributor = (float)contributor;
       }
-      const float4 q3 = s_q3[j], q4 = s_q4[j];
       N0 += q3.x * w;
       N1 += q3.y * w;
       N2 += q3.z * w;
       C0 += q3.w * w;
-      C1 += q4.x * w;
-      C2 += q4.y * w;
+      C1 += q4x * w;
+      C2 += q4y * w;
       T = test_T;
       last_contributor = contributor;
+      return true;
+    };
+    if (foff != kNoFrag) {
+      // The lane's pairs of this chunk are contiguous in entry order: fetch
+      // them kPre at a time so the loads overlap instead of stalling each step.
+      const float2* fa = a.fragA + foff;
+      while (mask != 0ull && !done) {
+        int jj[kPre];
+        float2 ad[kPre];
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) {
+          jj[k] = -1;
+          ad[k] = make_float2(0.0f, 1.0f);
+          if (mask != 0ull) {
+            jj[k] = __builtin_ctzll(mask);
+            mask &= mask - 1ull;
+            ad[k] = fa[k];
+          }
+        }
+        fa += kPre;
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) {
+          if (jj[k] >= 0 && !done) {
+            const int j = jj[k];
+            const float4 q3 = s_q3[j], q4 = s_q4[j];
+            if (!step(ad[k].x, ad[k].y, s_pos[j], q3, q4.x, q4.y)) done = true;
+          }
+        }
+      }
+    } else {
+      while (mask != 0ull) {
+        const int j = __builtin_ctzll(mask);
+        mask &= mask - 1ull;
+        Pair e;
+        if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;
+        const float4 q3 = s_q3[j], q4 = s_q4[j];
+        if (!step(e.alpha, e.depth, s_pos[j], q3, q4.x, q4.y)) {
+          done = true;
+          break;
+        }
+      }
     }
     if (__ballot(!done) == 0ull) break;
   }
   if (inside) {
-    const size_t HW = (size_t)a.H * a.W, pid = (size_t)a.W * pyi + pxi;
-    a.final_T[pid] = T;
-    a.n_contrib[pid] = last_contributor;
-    a.out_color[pid] = C0 + T * a.bg[0];
-    a.out_color[HW + pid] = C1 + T * a.bg[1];
-    a.out_color[2 * HW + pid] = C2 + T * a.bg[2];
-    a.n_contrib[pid + HW] = f2u(median_contributor);
-    a.final_T[pid + HW] = M1;
-    a.final_T[pid + 2 * HW] = M2;
-    a.out_others[pid] = D;
-    a.out_others[pid + HW] = 1.0f - T;
-    a.out_others[pid + 2 * HW] = N0;
-    a.out_others[pid + 3 * HW] = N1;
-    a.out_others[pid + 4 * HW] = N2;
-    a.out_others[pid + 5 * HW] = median_depth;
-    a.out_others[pid + 6 * HW] = distortion;
+    const size_t HW = (size_t)vi.H * vi.W, pid = (size_t)vi.W * pyi + pxi, pb = vi.pix_base;
+    float* final_T = a.final_T + 3 * pb;
+    uint32_t* n_contrib = a.n_contrib + 2 * pb;
+    float* out_color = a.out_color + 3 * pb;
+    float* out_others = a.out_others + 7 * pb;
+    final_T[pid] = T;
+    n_contrib[pid] = last_contributor;
+    out_color[pid] = C0 + T * a.bg[0];
+    out_color[HW + pid] = C1 + T * a.bg[1];
+    out_color[2 * HW + pid] = C2 + T * a.bg[2];
+    n_contrib[pid + HW] = f2u(median_contributor);
+    final_T[pid + HW] = M1;
+    final_T[pid + 2 * HW] = M2;
+    out_others[pid] = D;
+    out_others[pid + HW] = 1.0f - T;
+    out_others[pid + 2 * HW] = N0;
+    out_others[pid + 3 * HW] = N1;
+    out_others[pid + 4 * HW] = N2;
+    out_others[pid + 5 * HW] = median_depth;
+    out_others[pid + 6 * HW] = distortion;
   }
 }
 
 hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s) {
-  if (a.T == 0) return hipSuccess;
-  if (a.B != 8 && a.B != 16 && a.B != 32) return hipErrorInvalidValue;
-  const unsigned nblk = (unsigned)(a.T * quads_per_tile(a.B));
+  if (a.vs.T == 0) return hipSuccess;
+  if (a.vs.B != 8 && a.vs.B != 16 && a.vs.B != 32) return hipErrorInvalidValue;
+  const unsigned nblk = (unsigned)(a.vs.T * quads_per_tile(a.vs.B));
   hipLaunchKernelGGL(k_blend_fwd, dim3(nblk), dim3(64), 0, s, a);
   return hipGetLastError();
 }
@@ -783,6 +837,17 @@ hipError_t launch_mark_visible(int P, const float* means3D, const float* vm, uin
                                hipStream_t s) {
   if (P == 0) return hipSuccess;
   hipLaunchKernelGGL(k_mark_visible, dim3((P + 255) / 256), dim3(256), 0, s, P, means3D, vm, present);
+  return hipGetLastError();
+}
+
+// Copies the instance count and the error flags to a caller-visible status
+// word, so batched callers can check capacity without reading internal state.
+__global__ void k_copy_status(const uint32_t* counters, uint32_t* status) {
+  if (threadIdx.x < 4) status[threadIdx.x] = counters[threadIdx.x];
+}
+
+hipError_t launch_copy_status(const uint32_t* counters, uint32_t* status, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_status, dim3(1), dim3(64), 0, s, counters, status);
   return hipGetLastError();
 }
 

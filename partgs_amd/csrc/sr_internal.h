@@ -28,56 +28,111 @@ constexpr int kRecQuads = 5;
 constexpr int kQuad = 8;
 __host__ __device__ inline int quads_per_tile(int B) { return (B / kQuad) * (B / kQuad); }
 
-// Geometry state: one entry per surfel.
+// A batch of views rasterised together.  Every view has its own camera and
+// image size; all share the surfels and the tile edge B.  Surfel i seen from
+// view v is record r = v * P + i; tile t of view v is batch tile
+// tile_base + t; pixel p of view v is batch pixel pix_base + p, and a view's
+// C-channel image planes start at C * pix_base.  The table travels by value
+// in the kernel arguments (no device copy, nothing to upload before a launch).
+constexpr int kMaxViews = 32;
+struct ViewInfo {
+  int W, H, gx, gy;
+  uint32_t tile_base, pix_base;
+  float tanx, tany;
+};
+struct ViewSet {
+  int V, B;
+  uint32_t T;  // tiles of all views
+  uint32_t N;  // pixels of all views
+  ViewInfo v[kMaxViews];
+};
+inline ViewSet make_viewset(int V, const int* W, const int* H, const float* tanx, const float* tany, int B) {
+  ViewSet vs{};
+  vs.V = V;
+  vs.B = B;
+  uint32_t t = 0, n = 0;
+  for (int i = 0; i < V; ++i) {
+    ViewInfo& vi = vs.v[i];
+    vi.W = W[i];
+    vi.H = H[i];
+    vi.gx = (W[i] + B - 1) / B;
+    vi.gy = (H[i] + B - 1) / B;
+    vi.tile_base = t;
+    vi.pix_base = n;
+    vi.tanx = tanx ? tanx[i] : 0.0f;
+    vi.tany = tany ? tany[i] : 0.0f;
+    t += (uint32_t)(vi.gx * vi.gy);
+    n += (uint32_t)(W[i] * H[i]);
+  }
+  vs.T = t;
+  vs.N = n;
+  return vs;
+}
+inline ViewSet single_view(int W, int H, int B, float tanx = 0.0f, float tany = 0.0f) {
+  return make_viewset(1, &W, &H, &tanx, &tany, B);
+}
+// View owning batch tile t (t < vs.T).  The table is uniform across a
+// workgroup, so the search runs on scalar registers.
+__device__ inline int view_of_tile(const ViewSet& vs, uint32_t t) {
+  int lo = 0, hi = vs.V - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (vs.v[mid].tile_base <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Geometry state: one entry per record (surfel, view).
 struct GeomLayout {
   size_t rec, depths, tiles, clamped, dnormal, cullbox, total;
 };
-inline GeomLayout geom_layout(int P) {
+inline GeomLayout geom_layout(size_t R) {
   GeomLayout g{};
   size_t off = 0;
-  g.rec = off; off += align_up((size_t)P * kRecQuads * 16);
-  g.depths = off; off += align_up((size_t)P * 4);
-  g.tiles = off; off += align_up((size_t)P * 4);
-  g.clamped = off; off += align_up((size_t)P * 4);  // 3 flags + pad per surfel
-  g.dnormal = off; off += align_up((size_t)P * 12);  // backward scratch: dL/dnormal
-  g.cullbox = off; off += align_up((size_t)P * 8);   // short4 inclusive pixel box
+  g.rec = off; off += align_up(R * kRecQuads * 16);
+  g.depths = off; off += align_up(R * 4);
+  g.tiles = off; off += align_up(R * 4);
+  g.clamped = off; off += align_up(R * 4);  // 3 flags + pad per record
+  g.dnormal = off; off += align_up(R * 12);  // backward scratch: dL/dnormal
+  g.cullbox = off; off += align_up(R * 8);   // short4 inclusive pixel box
   g.total = off;
   return g;
 }
 
-// Image state: per-tile binning tables and per-pixel blend state.
+// Image state: per-tile binning tables and per-pixel blend state of all views.
 struct ImageLayout {
-  int gx, gy, T, N;
+  int T, N;
   size_t counters, tile_count, tile_start, tile_fill, ranges, sub_count, final_T, n_contrib, total;
 };
-inline ImageLayout image_layout(int W, int H, int B) {
+inline ImageLayout image_layout(const ViewSet& vs) {
   ImageLayout l{};
-  l.gx = (W + B - 1) / B;
-  l.gy = (H + B - 1) / B;
-  l.T = l.gx * l.gy;
-  l.N = W * H;
+  l.T = (int)vs.T;
+  l.N = (int)vs.N;
   size_t off = 0;
-  l.counters = off; off += kAlign;  // [0] = instance count, [1] = error flags
+  l.counters = off; off += kAlign;  // [0] instances, [1] error flags, [2] work items, [4..5] pair-store fill
   l.tile_count = off; off += align_up((size_t)l.T * 4);
   l.tile_start = off; off += align_up(((size_t)l.T + 1) * 4);
   l.tile_fill = off; off += align_up((size_t)l.T * 4);
   l.ranges = off; off += align_up((size_t)l.T * 8);
-  l.sub_count = off; off += align_up((size_t)l.T * quads_per_tile(B) * 4);
+  l.sub_count = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);
   l.final_T = off; off += align_up((size_t)l.N * 3 * 4);
   l.n_contrib = off; off += align_up((size_t)l.N * 2 * 4);
   l.total = off;
   return l;
 }
+inline ImageLayout image_layout(int W, int H, int B) { return image_layout(single_view(W, H, B)); }
 
 // Binning state: one entry per (surfel, tile) instance.
 struct BinLayout {
-  size_t keys, keys_tmp, point_list, sub, masks, work, frag_off, frag4, frag1, total;
+  size_t keys, keys_tmp, point_list, sub, masks, work, frag_off, fragA, fragB, fragC, total;
   size_t frag_cap;
 };
 // Accepted-pair store: per accepted (pixel, entry) pair the acceptance pass
-// keeps (alpha, G, depth, s.x) and s.y, where G = exp(-rho / 2) and s the
-// intersection in splat coordinates; a lane's pairs of one chunk are
-// contiguous from frag_off[mask slot].  Capacity kFragPerInstance pairs per
+// keeps fragA = (alpha, depth), fragB = (G, s.x) and fragC = s.y, where
+// G = exp(-rho / 2) and s the intersection in splat coordinates (the forward
+// blend reads fragA only); a lane's pairs of one chunk are contiguous from
+// frag_off[mask slot], in entry order.  Capacity kFragPerInstance pairs per
 // tile instance; chunks that do not fit get frag_off = kNoFrag and the blend
 // kernels recompute their pairs instead.
 constexpr size_t kFragPerInstance = 24;
@@ -108,16 +163,19 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
   b.frag_off = off; off += align_up(nslots * 4);
   b.frag_cap = n * (pair_store > 0 ? (size_t)pair_store : kFragPerInstance);
   if (b.frag_cap > 0x7fffffffu) b.frag_cap = 0x7fffffffu;
-  b.frag4 = off; off += align_up(b.frag_cap * 16);
-  b.frag1 = off; off += align_up(b.frag_cap * 4);
+  b.fragA = off; off += align_up(b.frag_cap * 8);
+  b.fragB = off; off += align_up(b.frag_cap * 8);
+  b.fragC = off; off += align_up(b.frag_cap * 4);
   b.total = off;
   return b;
 }
 
 constexpr uint32_t kErrPrefiltered = 1u;
+constexpr uint32_t kErrCapacity = 2u;  // more tile instances than the binning capacity
 
 struct PreprocessArgs {
-  int P, D, M, W, H, B, gx, gy, prefiltered;
+  ViewSet vs;
+  int P, D, M, prefiltered;
   float mod;
   const float* means3D;
   const float* scales;
@@ -126,9 +184,9 @@ struct PreprocessArgs {
   const float* shs;
   const float* colors;
   const float* tprec;
-  const float* vm;
-  const float* pm;
-  const float* campos;
+  const float* vm;      // V x 16
+  const float* pm;      // V x 16
+  const float* campos;  // V x 3
   float4* rec;
   float* depths;
   int* radii;
@@ -139,9 +197,21 @@ struct PreprocessArgs {
   uint32_t* counters;
 };
 
+struct ScatterArgs {
+  ViewSet vs;
+  int P;
+  const float4* rec;
+  const float* depths;
+  const int* radii;
+  const uint32_t* counters;
+  uint32_t* tile_fill;
+  uint64_t* keys;
+};
+
 struct SortArgs {
-  int W, H, B, gx, T;
+  ViewSet vs;
   const uint32_t* ranges;
+  const uint32_t* counters;
   uint64_t* keys;
   uint64_t* keys_tmp;
   uint32_t* point_list;
@@ -153,8 +223,9 @@ struct SortArgs {
 };
 
 struct AcceptArgs {
-  int W, H, B, gx;
+  ViewSet vs;
   const uint32_t* ranges;
+  const uint32_t* counters;
   const uint2* sub;
   const uint32_t* sub_count;
   const float4* rec;
@@ -162,20 +233,22 @@ struct AcceptArgs {
   const uint32_t* work_count;
   uint64_t* masks;
   uint32_t* frag_off;
-  float4* frag4;
-  float* frag1;
+  float2* fragA;
+  float2* fragB;
+  float* fragC;
   unsigned long long* frag_fill;
   uint32_t frag_cap;
 };
 
 struct BlendArgs {
-  int W, H, B, gx, gy, T;
+  ViewSet vs;
   const uint32_t* ranges;
+  const uint32_t* counters;
   const uint2* sub;
   const uint32_t* sub_count;
   const uint64_t* masks;
   const uint32_t* frag_off;
-  const float4* frag4;
+  const float2* fragA;
   const float4* rec;
   const float* bg;
   float* final_T;
@@ -184,15 +257,20 @@ struct BlendArgs {
   float* out_others;
 };
 
+// Per-record gradient arrays (record r = v * P + i) filled by the backward
+// blend: dL_dtransMat 9, dL_dmean2D 3 (x, y used), dL_dnormal 3,
+// dL_dopacity 1 and dL_dcolors 3 floats per record.
 struct BlendBwdArgs {
-  int W, H, B, gx, gy, T;
+  ViewSet vs;
   const uint32_t* ranges;
+  const uint32_t* counters;
   const uint2* sub;
   const uint32_t* sub_count;
   const uint64_t* masks;
   const uint32_t* frag_off;
-  const float4* frag4;
-  const float* frag1;
+  const float2* fragA;
+  const float2* fragB;
+  const float* fragC;
   const float4* rec;
   const float* bg;
   const float* final_T;
@@ -206,9 +284,12 @@ struct BlendBwdArgs {
   float* dL_dcolors;
 };
 
+// Backward preprocess over all views of each surfel.  The per-record arrays
+// are the backward blend's; dL_dopacity_out / dL_dcolors_out receive the sums
+// over views (skipped when they alias the per-record arrays, V = 1).
 struct PreprocessBwdArgs {
-  int P, D, M, W, H;
-  float tanx, tany;
+  ViewSet vs;
+  int P, D, M;
   const float* means3D;
   const float* scales;
   const float* rots;
@@ -223,6 +304,9 @@ struct PreprocessBwdArgs {
   float* dL_dtransMat;
   const float* dL_dnormal;
   const float* dL_dcolors;
+  const float* dL_dopacity;
+  float* dL_dopacity_out;
+  float* dL_dcolors_out;
   float* dL_dsh;
   float* dL_dmean2D;
   float* dL_dmean3D;
@@ -235,9 +319,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s);
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
-                             uint32_t* ranges, uint32_t* counters, hipStream_t s);
-hipError_t launch_scatter(int P, int B, int gx, int gy, const float4* rec, const float* depths,
-                          const int* radii, uint32_t* tile_fill, uint64_t* keys, hipStream_t s);
+                             uint32_t* ranges, uint32_t* counters, uint32_t capacity, hipStream_t s);
+hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s);
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s);
 hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s);
 hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s);
@@ -245,5 +328,6 @@ hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s);
 hipError_t launch_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s);
 hipError_t launch_mark_visible(int P, const float* means3D, const float* vm, uint8_t* present,
                                hipStream_t s);
+hipError_t launch_copy_status(const uint32_t* counters, uint32_t* status, hipStream_t s);
 
 }  // namespace sr

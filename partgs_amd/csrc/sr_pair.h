@@ -56,11 +56,11 @@ SR_DEV bool eval_pair(float pixx, float pixy, const float4& q0, const float4& q1
   return !(e.alpha < 1.0f / 255.0f);
 }
 
-// Rebuilds a Pair from its stored (alpha, G, depth, s.x) and s.y: the cheap
-// planes k, l, p.z and the screen offsets are recomputed exactly as in
+// Rebuilds a Pair from its stored (alpha, depth), (G, s.x) and s.y: the
+// cheap planes k, l, p.z and the screen offsets are recomputed exactly as in
 // eval_pair, so every field equals eval_pair's.
 SR_DEV void load_pair(float pixx, float pixy, const float4& q0, const float4& q1, const float4& q2,
-                      const float4& f4, float f1, Pair& e) {
+                      const float2& fa, const float2& fb, float fc, Pair& e) {
   const float Tux = q0.x, Tuy = q0.y, Tuz = q0.z, Tvx = q0.w, Tvy = q1.x, Tvz = q1.y;
   const float Twx = q1.z, Twy = q1.w, Twz = q2.x;
   e.dx = q2.y - pixx;
@@ -73,11 +73,11 @@ SR_DEV void load_pair(float pixx, float pixy, const float4& q0, const float4& q1
   e.ly = pixy * Twy - Tvy;
   e.lz = pixy * Twz - Tvz;
   e.pz = e.kx * e.ly - e.ky * e.lx;
-  e.alpha = f4.x;
-  e.G = f4.y;
-  e.depth = f4.z;
-  e.sx = f4.w;
-  e.sy = f1;
+  e.alpha = fa.x;
+  e.depth = fa.y;
+  e.G = fb.x;
+  e.sx = fb.y;
+  e.sy = fc;
   e.rho3d = e.sx * e.sx + e.sy * e.sy;
 }
 

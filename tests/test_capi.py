@@ -67,3 +67,41 @@ def test_invalid_arguments_are_reported_without_gpu():
     rc = lib.sr_forward(ctypes.byref(s), ctypes.byref(g), fn, None, 1, 1, 1, ctypes.byref(nr), None)
     assert rc == _lib.SR_ERR_INVALID
     assert b"SHs or precomputed colors" in lib.sr_last_error()
+
+
+def _views(V=2, tile=16):
+    import ctypes as C
+    v = _lib.SrViews()
+    v.V = V
+    w = (C.c_int * max(V, 1))(*([64] * max(V, 1)))
+    h = (C.c_int * max(V, 1))(*([48] * max(V, 1)))
+    t = (C.c_float * max(V, 1))(*([0.5] * max(V, 1)))
+    v.widths, v.heights, v.tanfovx, v.tanfovy = w, h, t, t
+    v.viewmatrices = v.projmatrices = v.campos = v.bg = 16  # never dereferenced: validation fails first
+    v.tile = tile
+    v._keep = (w, h, t)
+    return v
+
+
+def test_batched_workspace_and_validation_without_gpu():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    v = _views()
+    small = lib.sr_batch_workspace_bytes(ctypes.byref(v), 1000, 10)
+    big = lib.sr_batch_workspace_bytes(ctypes.byref(v), 1000, 100000)
+    assert 0 < small < big
+    assert lib.sr_batch_workspace_bytes(ctypes.byref(_views(V=0)), 1000, 10) == 0
+    assert lib.sr_batch_workspace_bytes(ctypes.byref(_views(V=_lib.SR_MAX_VIEWS + 1)), 1000, 10) == 0
+    g = _lib.SrSurfels()
+    g.P = 10
+    g.means3D = g.opacities = g.scales = g.rotations = g.colors_precomp = 16
+    g.transMat_precomp = 16
+    rc = lib.sr_forward_batched(ctypes.byref(v), ctypes.byref(g), 16, big, 100000, 16, 16, 16, 16, None)
+    assert rc == _lib.SR_ERR_INVALID and b"precomputed T" in lib.sr_last_error()
+    g.transMat_precomp = None
+    rc = lib.sr_forward_batched(ctypes.byref(_views(tile=12)), ctypes.byref(g), 16, big, 100000, 16, 16, 16, 16,
+                                None)
+    assert rc == _lib.SR_ERR_INVALID and b"tile" in lib.sr_last_error()
+    rc = lib.sr_forward_batched(ctypes.byref(v), ctypes.byref(g), 16, small, 100000, 16, 16, 16, 16, None)
+    assert rc == _lib.SR_ERR_INVALID and b"workspace" in lib.sr_last_error()

@@ -103,7 +103,7 @@ def test_render_matches_direct_rasteriser_call():
     from partgs_amd.renderer import render
     from partgs_amd.scenes import superquadric_surfels
 
-    scene = superquadric_surfels(5, P=20000, n_sq=4, W=160, H=120, sh_degree=2)
+    scene = superquadric_surfels(5, P=20000, n_sq=4, width=160, height=120, sh_degree=2)
     dev = torch.device("cuda")
     cam = scene.camera.to(dev)
     pc = _Model(scene, dev)
