@@ -1,0 +1,188 @@
+"""Train and evaluate the learned block proposal.
+
+  python -m partgs_amd.proposal_train --train 256 --eval 16 --fit-iters 200 --out DIR
+
+1. Dataset: synthetic block scenes (partgs_amd.block_fit.random_block_scene,
+   4-16 superquadrics, 20,000 noisy surface points each), each labelled by
+   the plain pipeline: the k-means initialisation of partgs_amd.init with 16
+   blocks, refined by the render-based plain optimiser (fit_blocks).
+2. Training: the point-set MLP (partgs_amd.proposal) on the MFMA kernels,
+   loss = per-scene Hungarian-matched weighted L2 between the 16 predicted
+   and the 16 fitted blocks (q compared up to sign).
+3. Evaluation on held-out scenes: the plain optimiser run from the plain
+   initialisation and from the network's proposal with the same optimiser
+   and loss; reported: iterations the proposal needs to reach 1.01 x the
+   plain arm's final loss, against the plain arm's iteration count.  The
+   plain initialisation stays the fallback: a scene where the proposal's
+   final loss is worse keeps the plain result.
+
+Writes DIR/proposal_weights.npz and DIR/proposal_report.json; prints one
+progress line per stage item.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+from .block_fit import fit_blocks, iterations_to_reach, logit, random_block_scene
+from .init import plain_block_init
+from .proposal import N_BLOCKS, ProposalNet
+
+W_PARAM = torch.tensor([1.0] * 3 + [0.5] * 4 + [1.0] * 3 + [0.1] * 2)
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def canonical(params: torch.Tensor) -> torch.Tensor:
+    """Unit quaternions with w >= 0 (the block layer normalises q anyway)."""
+    p = params.clone()
+    q = p[..., 3:7]
+    q = q / q.norm(dim=-1, keepdim=True).clamp_min(1e-8)
+    q = torch.where(q[..., :1] < 0, -q, q)
+    p[..., 3:7] = q
+    return p
+
+
+def plain_init(points: torch.Tensor, seed: int):
+    params, occ = plain_block_init(points, n_blocks=N_BLOCKS, ratio_block_scene=0.25, seed=seed)
+    return params, occ
+
+
+def label_scene(seed: int, fit_iters: int):
+    scene = random_block_scene(seed)
+    p0, o0 = plain_init(scene.points, seed)
+    fit = fit_blocks(scene, p0, o0, iters=fit_iters)
+    return scene, canonical(fit.params), fit.losses
+
+
+def matched_loss(pred: torch.Tensor, label: torch.Tensor):
+    """Hungarian-matched weighted L2 per scene; pred requires grad (B x 16 x 12)."""
+    from scipy.optimize import linear_sum_assignment
+
+    w = W_PARAM.to(pred.device)
+    total = 0.0
+    pd = pred.detach()
+    for b in range(pred.shape[0]):
+        d = pd[b][:, None, :] - label[b][None, :, :]
+        dq_neg = pd[b][:, None, 3:7] + label[b][None, :, 3:7]
+        cost_q = torch.minimum((d[..., 3:7] ** 2).sum(-1), (dq_neg ** 2).sum(-1))
+        cost = ((d ** 2) * w).sum(-1) - (d[..., 3:7] ** 2 * w[3:7]).sum(-1) + w[3] * cost_q
+        ri, ci = linear_sum_assignment(cost.cpu().numpy())
+        ri = torch.as_tensor(ri, device=pred.device)
+        ci = torch.as_tensor(ci, device=pred.device)
+        tgt = label[b][ci].clone()
+        flip = ((pred[b][ri, 3:7] - tgt[:, 3:7]) ** 2).sum(-1) > ((pred[b][ri, 3:7] + tgt[:, 3:7]) ** 2).sum(-1)
+        tgt[:, 3:7] = torch.where(flip[:, None], -tgt[:, 3:7], tgt[:, 3:7])
+        total = total + (((pred[b][ri] - tgt) ** 2) * w).sum() / N_BLOCKS
+    return total / pred.shape[0]
+
+
+def train(net: ProposalNet, points: torch.Tensor, labels: torch.Tensor, epochs: int, batch: int, lr: float,
+          seed: int = 0):
+    n = points.shape[0]
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    hist = []
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g)
+        ep_loss = 0.0
+        for s in range(0, n, batch):
+            idx = perm[s:s + batch].to(points.device)
+            out, cache = net.forward(points[idx])
+            o = out.detach().requires_grad_(True)
+            loss = matched_loss(o, labels[idx])
+            loss.backward()
+            grads = net.backward(o.grad, cache)
+            net.adam_step(grads, lr=lr)
+            ep_loss += float(loss) * idx.numel()
+        hist.append(ep_loss / n)
+        if ep % max(1, epochs // 10) == 0 or ep == epochs - 1:
+            _log(f"[proposal] epoch {ep} loss {hist[-1]:.4f}")
+    return hist
+
+
+def evaluate(net: ProposalNet, seeds, fit_iters: int):
+    rows = []
+    for sd in seeds:
+        scene = random_block_scene(sd)
+        p0, o0 = plain_init(scene.points, sd)
+        plain = fit_blocks(scene, p0, o0, iters=fit_iters)
+        with torch.no_grad():
+            out, _ = net.forward(scene.points[None])
+        pl = canonical(out[0])
+        ol = torch.full((N_BLOCKS,), logit(0.7), device=pl.device)
+        learned = fit_blocks(scene, pl, ol, iters=fit_iters)
+        final_plain = float(np.mean(plain.losses[-5:]))
+        level = 1.01 * final_plain
+        it_learned = iterations_to_reach(learned.losses, level)
+        it_plain = iterations_to_reach(plain.losses, level)
+        final_learned = float(np.mean(learned.losses[-5:]))
+        rows.append({"seed": sd, "final_plain": final_plain, "final_learned": final_learned,
+                     "iters_plain": fit_iters, "iters_plain_first_hit": it_plain, "iters_learned": it_learned,
+                     "init_loss_plain": plain.losses[0], "init_loss_learned": learned.losses[0],
+                     "fallback_to_plain": final_learned > final_plain})
+        _log(f"[proposal] eval seed {sd}: plain final {final_plain:.4f} (first hit {it_plain}) learned hits at "
+             f"{it_learned}, final {final_learned:.4f}")
+    return rows
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=256)
+    ap.add_argument("--eval", type=int, default=16)
+    ap.add_argument("--fit-iters", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", default="proposal_out")
+    ap.add_argument("--weights", default=None, help="skip training and load these weights")
+    args = ap.parse_args(argv)
+    os.makedirs(args.out, exist_ok=True)
+    torch.manual_seed(0)
+    net = ProposalNet(seed=0)
+    report = {"train_scenes": args.train, "eval_scenes": args.eval, "fit_iters": args.fit_iters}
+    t0 = time.time()
+    if args.weights:
+        sd = np.load(args.weights)
+        net.load_state_dict({k: torch.tensor(sd[k]) for k in sd.files})
+    else:
+        pts, labels = [], []
+        for i in range(args.train):
+            scene, label, losses = label_scene(10_000 + i, args.fit_iters)
+            pts.append(scene.points)
+            labels.append(label)
+            if i % 16 == 0:
+                _log(f"[proposal] labelled {i + 1}/{args.train} scenes, fit loss {losses[0]:.4f} -> "
+                     f"{losses[-1]:.4f} ({time.time() - t0:.0f} s)")
+        points = torch.stack(pts)
+        labels = torch.stack(labels)
+        report["label_seconds"] = time.time() - t0
+        t1 = time.time()
+        hist = train(net, points, labels, args.epochs, args.batch, args.lr)
+        report["train_seconds"] = time.time() - t1
+        report["train_loss"] = [hist[0], hist[-1]]
+        np.savez(os.path.join(args.out, "proposal_weights.npz"),
+                 **{k: v.numpy() for k, v in net.state_dict().items()})
+    rows = evaluate(net, range(90_000, 90_000 + args.eval), args.fit_iters)
+    ratios = [r["iters_learned"] / r["iters_plain"] for r in rows]
+    report["eval"] = rows
+    report["median_iters_ratio_vs_plain_total"] = float(np.median(ratios))
+    report["median_iters_ratio_vs_plain_first_hit"] = float(
+        np.median([r["iters_learned"] / r["iters_plain_first_hit"] for r in rows]))
+    report["reached_within_1pct"] = float(np.mean([r["iters_learned"] <= args.fit_iters for r in rows]))
+    report["fallback_fraction"] = float(np.mean([r["fallback_to_plain"] for r in rows]))
+    with open(os.path.join(args.out, "proposal_report.json"), "w") as f:
+        json.dump(report, f, indent=1)
+    _log(json.dumps({k: v for k, v in report.items() if k != "eval"}))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
