@@ -41,6 +41,25 @@ def pipeline_bytes(P, M, L, N, T):
     return (430 + 36 * M) * P + 196 * L + 120 * N + 24 * T
 
 
+STAGE_KERNEL = {"preprocess": "k_preprocess", "tile_scan": "k_scan_tiles", "scatter": "k_scatter",
+                "tile_sort": "k_sort_tiles", "blend_fwd": "k_blend_fwd", "blend_bwd": "k_blend_bwd",
+                "preprocess_bwd": "k_preprocess_bwd", "accept": "k_accept"}
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage's kernel from the committed counter
+    summary (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this benchmark,
+    each scaled by the same counter's reading for a copy of known size), or
+    None when no summary covers it."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic_c2b.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(STAGE_KERNEL.get(stage, stage))
+    return None if k is None else int(k["bytes_per_launch"])
+
+
 def cpu_baseline(scene, gc, go, budget_s=20.0):
     import numpy as np  # noqa: F401
 
@@ -238,7 +257,7 @@ def main():
                    "width": W, "height": H, "sh_degree": scene.sh_degree, "tile": args.tile,
                    "num_rendered": int(L), "visible": P_vis},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dominant),
                      "alg_bytes": int(dom_bytes), "avg_ms": round(per_stage_ms[dominant], 5)},
         "pipeline": {"alg_bytes": int(b_pipe), "kernel_ms": round(kern_ms, 5),
                      "hbm_frac": round(b_pipe / (kern_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 5),

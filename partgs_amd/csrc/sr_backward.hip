@@ -99,27 +99,37 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   bg_dot += a.bg[2] * dLp2;
 
   const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
-  for (int c = (cnt + 63) / 64 - 1; c >= 0; --c) {
-    uint64_t mask = (inside && last_contributor > 0) ? masks[(size_t)c * 64 + lane] : 0ull;
-    if (__ballot(mask != 0ull) == 0ull) continue;
-    const uint32_t foff = mask ? frag_off[(size_t)c * 64 + lane] : kNoFrag;
+  const int nch = (cnt + 63) / 64;
+  const bool active = inside && last_contributor > 0;
+  ChunkRegs nx{};
+  uint2 v2 = make_uint2(0u, 0u);
+  if (nch > 0) {
+    nx = chunk_load(masks, frag_off, a.rec, lane, nch - 1, chunk_entry(list, cnt, lane, nch - 1, nch));
+    v2 = chunk_entry(list, cnt, lane, nch - 2, nch);
+  }
+  for (int c = nch - 1; c >= 0; --c) {
+    uint64_t mask = active ? nx.mask : 0ull;
+    const uint32_t foff = mask ? nx.foff : kNoFrag;
     int r = __popcll(mask);  // store index one past this lane's last pair of the chunk
-    const int e0 = c * 64;
-    const int m = min(64, cnt - e0);
+    const int m = min(64, cnt - c * 64);
     __syncthreads();
     if (lane < m) {
-      const uint2 v = list[e0 + lane];
-      s_e[lane] = v;
-      const float4* r = a.rec + (size_t)kRecQuads * v.x;
-      s_q0[lane] = r[0];
-      s_q1[lane] = r[1];
-      s_q2[lane] = r[2];
-      s_q3[lane] = r[3];
-      s_q4[lane] = r[4];
+      s_e[lane] = nx.v;
+      s_q0[lane] = nx.q0;
+      s_q1[lane] = nx.q1;
+      s_q2[lane] = nx.q2;
+      s_q3[lane] = nx.q3;
+      s_q4[lane] = nx.q4;
     }
 #pragma unroll
     for (int k = 0; k < kNumG; ++k) s_acc[k][lane] = 0.0f;
     __syncthreads();
+    if (c > 0) {  // in flight while this chunk runs
+      const uint2 ve = v2;
+      v2 = chunk_entry(list, cnt, lane, c - 2, nch);
+      nx = chunk_load(masks, frag_off, a.rec, lane, c - 1, ve);
+    }
+    if (__ballot(mask != 0ull) == 0ull) continue;
     // Reverse step of the reference (backward.cu:282-437) for accepted entry j.
     auto body = [&](const Pair& e, int j) {
       const uint32_t contributor = s_e[j].y;  // 0-based position in the tile list

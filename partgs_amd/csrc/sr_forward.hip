@@ -705,24 +705,32 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   const float mcoef = kFar / (kFar - kNear);
   const int nch = (cnt + 63) / 64;
   const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
+  ChunkRegs nx{};
+  uint2 v2 = make_uint2(0u, 0u);
+  if (nch > 0) {
+    nx = chunk_load(masks, frag_off, a.rec, lane, 0, chunk_entry(list, cnt, lane, 0, nch));
+    v2 = chunk_entry(list, cnt, lane, 1, nch);
+  }
   for (int c = 0; c < nch; ++c) {
-    uint64_t mask = done ? 0ull : masks[(size_t)c * 64 + lane];
-    if (__ballot(mask != 0ull) == 0ull) continue;
-    uint32_t foff = mask ? frag_off[(size_t)c * 64 + lane] : kNoFrag;
-    const int e0 = c * 64;
-    const int m = min(64, cnt - e0);
+    const int m = min(64, cnt - c * 64);
+    uint64_t mask = done ? 0ull : nx.mask;
+    uint32_t foff = mask ? nx.foff : kNoFrag;
     __syncthreads();
     if (lane < m) {
-      const uint2 v = list[e0 + lane];
-      s_pos[lane] = v.y;
-      const float4* r = a.rec + (size_t)kRecQuads * v.x;
-      s_q0[lane] = r[0];
-      s_q1[lane] = r[1];
-      s_q2[lane] = r[2];
-      s_q3[lane] = r[3];
-      s_q4[lane] = r[4];
+      s_pos[lane] = nx.v.y;
+      s_q0[lane] = nx.q0;
+      s_q1[lane] = nx.q1;
+      s_q2[lane] = nx.q2;
+      s_q3[lane] = nx.q3;
+      s_q4[lane] = nx.q4;
     }
     __syncthreads();
+    if (c + 1 < nch) {  // in flight while this chunk blends
+      const uint2 ve = v2;
+      v2 = chunk_entry(list, cnt, lane, c + 2, nch);
+      nx = chunk_load(masks, frag_off, a.rec, lane, c + 1, ve);
+    }
+    if (__ballot(mask != 0ull) == 0ull) continue;
     // One front-to-back step of the reference's update (forward.cu:383-415);
     // false once T would drop below 1e-4 (the pixel is done).
     auto step = [&](float alpha, float depth, uint32_t pos, const float4& q3, float q4x, float q4y) -> bool {

@@ -81,4 +81,36 @@ SR_DEV void load_pair(float pixx, float pixy, const float4& q0, const float4& q1
   e.rho3d = e.sx * e.sx + e.sy * e.sy;
 }
 
+// Software pipeline over a quadrant's 64-entry chunks for the blend kernels:
+// while one chunk is processed, the per-lane mask and pair-store offset of
+// the next chunk, the five record quads of its entries and the sub-list
+// entry of the chunk after it are already in flight, so a chunk's staging
+// costs one memory latency that overlaps work instead of two in series.
+// Lanes past the end get entry (0, 0) and read record 0, so every load is
+// unconditional and in bounds; nothing reads those values.
+SR_DEV uint2 chunk_entry(const uint2* list, int cnt, int lane, int c, int nch) {
+  const int e = c * 64 + lane;
+  return (c >= 0 && c < nch && e < cnt) ? list[e] : make_uint2(0u, 0u);
+}
+struct ChunkRegs {
+  uint64_t mask;
+  uint32_t foff;
+  uint2 v;
+  float4 q0, q1, q2, q3, q4;
+};
+SR_DEV ChunkRegs chunk_load(const uint64_t* masks, const uint32_t* frag_off, const float4* rec, int lane, int c,
+                            uint2 ve) {
+  ChunkRegs k;
+  k.mask = masks[(size_t)c * 64 + lane];
+  k.foff = frag_off[(size_t)c * 64 + lane];
+  k.v = ve;
+  const float4* r = rec + (size_t)kRecQuads * ve.x;
+  k.q0 = r[0];
+  k.q1 = r[1];
+  k.q2 = r[2];
+  k.q3 = r[3];
+  k.q4 = r[4];
+  return k;
+}
+
 }  // namespace sr

@@ -56,7 +56,10 @@ def test_block_forward_matches_oracle():
 def test_block_backward_matches_fd64():
     """64 random block parameters, FP64 central differences with h = 1e-4.
     The loss reads the rotation through R(q): the reference's w >= 0
-    standardisation flips q -> -q (the same rotation) discontinuously."""
+    standardisation flips q -> -q (the same rotation) discontinuously.
+    The map has kinks (the relu floor on a face's second scale, frame flips
+    of nearly collinear edges); an entry whose FD at h and at h / 2 disagree
+    has one within h and is excluded, and at most 8 of the 64 may be."""
     params, occ = _blocks(1, 16)
     params = params.astype(np.float32).astype(np.float64)
     occ = occ.astype(np.float32).astype(np.float64)
@@ -75,30 +78,33 @@ def test_block_backward_matches_fd64():
         m, sc, r, a = BO.generate(pp, oo, eta, om)
         return sum(float((x * g).sum()) for x, g in zip((m, sc, _rotmat(r), a), gs))
 
+    def fd(b, j, h):
+        pp, oo = params.copy(), occ.copy()
+        tgt = (pp, (b, j)) if j < 12 else (oo, (b,))
+        tgt[0][tgt[1]] += h
+        lp = f(pp, oo)
+        tgt[0][tgt[1]] -= 2 * h
+        lm = f(pp, oo)
+        return (lp - lm) / (2 * h)
+
     h = 1e-4
     picks = rng.choice(16 * 13, size=64, replace=False)
-    fd, an = [], []
+    fds, an, kinks = [], [], 0
     for k in picks:
         b, j = divmod(int(k), 13)
-        pp, oo = params.copy(), occ.copy()
-        if j < 12:
-            pp[b, j] += h
-            lp = f(pp, oo)
-            pp[b, j] -= 2 * h
-            lm = f(pp, oo)
-            an.append(gp[b, j])
-        else:
-            oo[b] += h
-            lp = f(pp, oo)
-            oo[b] -= 2 * h
-            lm = f(pp, oo)
-            an.append(go[b])
-        fd.append((lp - lm) / (2 * h))
-    fd, an = np.array(fd), np.array(an)
-    scale = np.abs(fd).max()
-    err = np.abs(fd - an)
-    print(f"[blocks] fd max {scale:.3e} max abs err {err.max():.3e} rel {err.max() / scale:.2e}", flush=True)
-    assert (err <= 1e-3 * scale + 2e-3 * np.abs(fd)).all(), (err.max(), scale)
+        d1, d2 = fd(b, j, h), fd(b, j, h / 2)
+        if abs(d1 - d2) > 1e-4 * (abs(d1) + 1.0):
+            kinks += 1
+            continue
+        fds.append(d1)
+        an.append(gp[b, j] if j < 12 else go[b])
+    fds, an = np.array(fds), np.array(an)
+    scale = np.abs(fds).max()
+    err = np.abs(fds - an)
+    print(f"[blocks] {len(fds)} smooth entries, {kinks} with a kink within h; fd max {scale:.3e} "
+          f"max abs err {err.max():.3e} rel {err.max() / scale:.2e}", flush=True)
+    assert kinks <= 8
+    assert (err <= 1e-3 * scale + 2e-3 * np.abs(fds)).all(), (err.max(), scale)
 
 
 def test_blocks_through_rasteriser_autograd():
