@@ -22,7 +22,7 @@ SR_ERR_HIP = 2
 SR_ERR_ALLOC = 3
 SR_ERR_PREFILTERED = 4
 SR_ERR_CAPACITY = 5
-SR_NUM_STAGES = 9
+SR_NUM_STAGES = 10
 STAGE_NAMES = (
     "preprocess",
     "tile_scan",
@@ -33,6 +33,7 @@ STAGE_NAMES = (
     "blend_bwd",
     "preprocess_bwd",
     "accept",
+    "blend_bwd_grad",
 )
 
 EXPORTED = (

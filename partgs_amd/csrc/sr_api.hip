@@ -165,6 +165,19 @@ struct Pipeline {
   uint32_t* counters() const { return (uint32_t*)(img + il.counters); }
 };
 
+PairStore pair_store(char* bin, const BinLayout& bl) {
+  PairStore p{};
+  p.fA = (float2*)(bin + bl.fA);
+  p.fP = (uint32_t*)(bin + bl.fP);
+  p.fN = (float4*)(bin + bl.fN);
+  p.fM = (float2*)(bin + bl.fM);
+  p.fD = (float2*)(bin + bl.fD);
+  p.fB = (float2*)(bin + bl.fB);
+  p.fC = (float*)(bin + bl.fC);
+  p.fW = (float4*)(bin + bl.fW);
+  return p;
+}
+
 struct RecordGrads {
   float* transMat;
   float* mean2D;
@@ -319,6 +332,7 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   sa.sub_count = (uint32_t*)(pl.img + pl.il.sub_count);
   sa.work = (uint2*)(bin + bl.work);
   sa.work_count = counters + 2;
+  sa.q_flag = (uint32_t*)(pl.img + pl.il.q_flag);
   SR_STAGE(3, dbg, st, "sort", launch_sort_tiles(sa, st));
   AcceptArgs aa{};
   aa.vs = pl.vs;
@@ -331,9 +345,8 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   aa.work_count = counters + 2;
   aa.masks = (uint64_t*)(bin + bl.masks);
   aa.frag_off = (uint32_t*)(bin + bl.frag_off);
-  aa.fragA = (float2*)(bin + bl.fragA);
-  aa.fragB = (float2*)(bin + bl.fragB);
-  aa.fragC = (float*)(bin + bl.fragC);
+  aa.ps = pair_store(bin, bl);
+  aa.q_flag = sa.q_flag;
   aa.frag_fill = (unsigned long long*)(counters + 4);
   aa.frag_cap = (uint32_t)bl.frag_cap;
   SR_STAGE(8, dbg, st, "accept", launch_accept(aa, work_capacity(pl.capacity, pl.vs.B, pl.il.T), st));
@@ -349,7 +362,8 @@ int run_blend_fwd(Pipeline& pl, float* out_color, float* out_others, bool dbg, h
   ba.sub_count = (const uint32_t*)(pl.img + pl.il.sub_count);
   ba.masks = (const uint64_t*)(pl.bin + pl.bl.masks);
   ba.frag_off = (const uint32_t*)(pl.bin + pl.bl.frag_off);
-  ba.fragA = (const float2*)(pl.bin + pl.bl.fragA);
+  ba.ps = pair_store(pl.bin, pl.bl);
+  ba.q_flag = (const uint32_t*)(pl.img + pl.il.q_flag);
   ba.rec = (const float4*)(pl.geom + pl.gl.rec);
   ba.bg = pl.bg;
   ba.final_T = (float*)(pl.img + pl.il.final_T);
@@ -372,9 +386,10 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
     ba.sub_count = (const uint32_t*)(pl.img + pl.il.sub_count);
     ba.masks = (const uint64_t*)(pl.bin + pl.bl.masks);
     ba.frag_off = (const uint32_t*)(pl.bin + pl.bl.frag_off);
-    ba.fragA = (const float2*)(pl.bin + pl.bl.fragA);
-    ba.fragB = (const float2*)(pl.bin + pl.bl.fragB);
-    ba.fragC = (const float*)(pl.bin + pl.bl.fragC);
+    ba.ps = pair_store(pl.bin, pl.bl);
+    ba.q_flag = (const uint32_t*)(pl.img + pl.il.q_flag);
+    ba.work = (const uint2*)(pl.bin + pl.bl.work);
+    ba.work_count = pl.counters() + 2;
     ba.rec = (const float4*)(pl.geom + pl.gl.rec);
     ba.bg = pl.bg;
     ba.final_T = (const float*)(pl.img + pl.il.final_T);
@@ -387,6 +402,8 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
     ba.dL_dopacity = rg.opacity;
     ba.dL_dcolors = rg.colors;
     SR_STAGE(6, dbg, st, "blend_bwd", launch_blend_bwd(ba, st));
+    SR_STAGE(9, dbg, st, "blend_bwd_grad",
+             launch_blend_bwd_grad(ba, work_capacity(pl.capacity, pl.vs.B, pl.il.T), st));
   }
   PreprocessBwdArgs pb{};
   pb.vs = pl.vs;
@@ -515,6 +532,8 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
   } else {
     SR_STAGE(3, dbg, st, "clear",
              launch_fill_u32((uint32_t*)(img + il.sub_count), 0u, (size_t)il.T * quads_per_tile(B), st));
+    SR_STAGE(3, dbg, st, "clear",
+             launch_fill_u32((uint32_t*)(img + il.q_flag), 0u, (size_t)il.T * quads_per_tile(B), st));
   }
   return run_blend_fwd(pl, out_color, out_others, dbg, st);
 }
@@ -554,14 +573,12 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
                  gr->dL_dcolors};
   {
     StageScope sc(5, st);
+    // the blend accumulates into the per-record arrays; the backward
+    // preprocess writes every other output itself
     hipError_t e = launch_fill_f32(gr->dL_dmeans2D, 0.0f, 3 * (size_t)P, st);
     if (e == hipSuccess) e = launch_fill_f32(gr->dL_dcolors, 0.0f, 3 * (size_t)P, st);
     if (e == hipSuccess) e = launch_fill_f32(gr->dL_dopacity, 0.0f, (size_t)P, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dmeans3D, 0.0f, 3 * (size_t)P, st);
     if (e == hipSuccess) e = launch_fill_f32(gr->dL_dtransMat, 0.0f, 9 * (size_t)P, st);
-    if (e == hipSuccess && g->shs) e = launch_fill_f32(gr->dL_dsh, 0.0f, (size_t)P * g->M * 3, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dscales, 0.0f, 2 * (size_t)P, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_drotations, 0.0f, 4 * (size_t)P, st);
     if (e == hipSuccess) e = launch_fill_f32(rg.normal, 0.0f, 3 * (size_t)P, st);
     if (e != hipSuccess) return fail(SR_ERR_HIP, "clear grads: %s", hipGetErrorString(e));
   }
@@ -621,12 +638,6 @@ int sr_backward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t
     if (e == hipSuccess) e = launch_fill_f32(rg.normal, 0.0f, 3 * R, st);
     if (e == hipSuccess) e = launch_fill_f32(rg.opacity, 0.0f, R, st);
     if (e == hipSuccess) e = launch_fill_f32(rg.colors, 0.0f, 3 * R, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dcolors, 0.0f, 3 * P, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dopacity, 0.0f, P, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dmeans3D, 0.0f, 3 * P, st);
-    if (e == hipSuccess && g->shs) e = launch_fill_f32(gr->dL_dsh, 0.0f, P * g->M * 3, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dscales, 0.0f, 2 * P, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_drotations, 0.0f, 4 * P, st);
     if (e != hipSuccess) return fail(SR_ERR_HIP, "clear grads: %s", hipGetErrorString(e));
   }
   return run_backward(pl, g, radii, dL_dcolor, dL_dothers, rg, gr, true, false, st);

@@ -32,11 +32,164 @@ SR_DEV float* grad_slot(const BlendBwdArgs& a, uint32_t id, int k) {
   return a.dL_dcolors + 3 * (size_t)id + (k - 15);
 }
 
-// One wave64 per 8 x 8 quadrant; each lane walks, back to front, the entries
-// its acceptance mask marks (the pairs the reference blends), skipping those
-// at or behind its last contributor, exactly as the reference does.  Partial
-// gradients of the 64 entries of a chunk are summed in LDS across the lanes,
-// then each entry's sums go to global memory with one FP32 atomic per slot.
+// Per-pixel inputs of the backward blend (backward.cu:168-236).
+struct PixelBwd {
+  float T_final, final_A, final_D, final_D2, bg_dot;
+  uint32_t last_contributor;
+  int median_contributor;
+  float dLp0, dLp1, dLp2, dLn0, dLn1, dLn2;
+  float dL_ddepth, dL_daccum, dL_dreg, dL_dmedian;
+};
+
+SR_DEV PixelBwd pixel_bwd(const BlendBwdArgs& a, const ViewInfo& vi, bool inside, uint32_t pxi, uint32_t pyi) {
+  PixelBwd p{};
+  const size_t HW = (size_t)vi.H * vi.W, pid = (size_t)vi.W * pyi + pxi, pb = vi.pix_base;
+  if (inside) {
+    const float* final_T = a.final_T + 3 * pb;
+    const uint32_t* n_contrib = a.n_contrib + 2 * pb;
+    const float* dL_dpix = a.dL_dpix + 3 * pb;
+    const float* dL_dothers = a.dL_dothers + 7 * pb;
+    p.T_final = final_T[pid];
+    p.final_D = final_T[pid + HW];
+    p.final_D2 = final_T[pid + 2 * HW];
+    p.last_contributor = n_contrib[pid];
+    p.median_contributor = (int)n_contrib[pid + HW];
+    p.dLp0 = dL_dpix[pid];
+    p.dLp1 = dL_dpix[HW + pid];
+    p.dLp2 = dL_dpix[2 * HW + pid];
+    p.dL_ddepth = dL_dothers[pid];
+    p.dL_daccum = dL_dothers[HW + pid];
+    p.dLn0 = dL_dothers[2 * HW + pid];
+    p.dLn1 = dL_dothers[3 * HW + pid];
+    p.dLn2 = dL_dothers[4 * HW + pid];
+    p.dL_dmedian = dL_dothers[5 * HW + pid];
+    p.dL_dreg = dL_dothers[6 * HW + pid];
+  }
+  p.final_A = 1.0f - p.T_final;
+  float bg_dot = 0.0f;
+  bg_dot += a.bg[0] * p.dLp0;
+  bg_dot += a.bg[1] * p.dLp1;
+  bg_dot += a.bg[2] * p.dLp2;
+  p.bg_dot = bg_dot;
+  return p;
+}
+
+// The back-to-front recurrences of one pixel (backward.cu:316-379): fed the
+// pairs it blended in reverse order, each step yields the pair's weight
+// w = alpha T, dL/dalpha and dL/ddepth, from which every per-pair gradient
+// follows without further pixel state.
+struct ChainBwd {
+  float T;
+  float acc_r0 = 0.0f, acc_r1 = 0.0f, acc_r2 = 0.0f;
+  float last_c0 = 0.0f, last_c1 = 0.0f, last_c2 = 0.0f;
+  float last_depth = 0.0f, accum_depth_rec = 0.0f, accum_alpha_rec = 0.0f;
+  float last_n0 = 0.0f, last_n1 = 0.0f, last_n2 = 0.0f;
+  float acc_n0 = 0.0f, acc_n1 = 0.0f, acc_n2 = 0.0f;
+  float last_dL_dT = 0.0f, last_alpha = 0.0f;
+  // nc = (normal.xyz, r); m_d, dmd_dd = distortion weight and its derivative
+  SR_DEV float4 step(const PixelBwd& p, float alpha, float c_d, float m_d, float dmd_dd, uint32_t contributor,
+                     const float4& nc, float g, float bl) {
+    T = T / (1.0f - alpha);
+    const float w = alpha * T;
+    float dL_dalpha = 0.0f;
+    acc_r0 = last_alpha * last_c0 + (1.0f - last_alpha) * acc_r0;
+    last_c0 = nc.w;
+    dL_dalpha += (nc.w - acc_r0) * p.dLp0;
+    acc_r1 = last_alpha * last_c1 + (1.0f - last_alpha) * acc_r1;
+    last_c1 = g;
+    dL_dalpha += (g - acc_r1) * p.dLp1;
+    acc_r2 = last_alpha * last_c2 + (1.0f - last_alpha) * acc_r2;
+    last_c2 = bl;
+    dL_dalpha += (bl - acc_r2) * p.dLp2;
+    float dL_dz = 0.0f, dL_dweight = 0.0f;
+    if ((int)contributor == p.median_contributor - 1) dL_dz += p.dL_dmedian;
+    dL_dweight += ((p.final_D2 + (m_d * m_d) * p.final_A) - (2.0f * m_d) * p.final_D) * p.dL_dreg;
+    dL_dalpha += dL_dweight - last_dL_dT;
+    last_dL_dT = dL_dweight * alpha + (1.0f - alpha) * last_dL_dT;
+    const float dL_dmd = ((2.0f * (T * alpha)) * (m_d * p.final_A - p.final_D)) * p.dL_dreg;
+    dL_dz += dL_dmd * dmd_dd;
+    accum_depth_rec = last_alpha * last_depth + (1.0f - last_alpha) * accum_depth_rec;
+    last_depth = c_d;
+    dL_dalpha += (c_d - accum_depth_rec) * p.dL_ddepth;
+    accum_alpha_rec = last_alpha + (1.0f - last_alpha) * accum_alpha_rec;
+    dL_dalpha += (1.0f - accum_alpha_rec) * p.dL_daccum;
+    acc_n0 = last_alpha * last_n0 + (1.0f - last_alpha) * acc_n0;
+    last_n0 = nc.x;
+    dL_dalpha += (nc.x - acc_n0) * p.dLn0;
+    acc_n1 = last_alpha * last_n1 + (1.0f - last_alpha) * acc_n1;
+    last_n1 = nc.y;
+    dL_dalpha += (nc.y - acc_n1) * p.dLn1;
+    acc_n2 = last_alpha * last_n2 + (1.0f - last_alpha) * acc_n2;
+    last_n2 = nc.z;
+    dL_dalpha += (nc.z - acc_n2) * p.dLn2;
+    dL_dalpha *= T;
+    last_alpha = alpha;
+    dL_dalpha += (-p.T_final / (1.0f - alpha)) * p.bg_dot;
+    dL_dz += (alpha * T) * p.dL_ddepth;
+    return make_float4(w, dL_dalpha, dL_dz, 0.0f);
+  }
+};
+
+// The 18 partial gradients of one pair (backward.cu:380-437) from its
+// weight w, dL/dalpha and dL/ddepth, summed into s_acc[.][j] with LDS atomics.
+SR_DEV void pair_grads(const PixelBwd& p, const Pair& e, const float4& q1, float opa, float w, float dL_dalpha,
+                       float dL_dz, float pixx, float pixy, float (*s_acc)[64], int j) {
+  atomicAdd(&s_acc[15][j], w * p.dLp0);
+  atomicAdd(&s_acc[16][j], w * p.dLp1);
+  atomicAdd(&s_acc[17][j], w * p.dLp2);
+  atomicAdd(&s_acc[11][j], w * p.dLn0);
+  atomicAdd(&s_acc[12][j], w * p.dLn1);
+  atomicAdd(&s_acc[13][j], w * p.dLn2);
+  const float G = e.G;
+  const float dL_dG = opa * dL_dalpha;
+  if (e.rho3d <= e.rho2d) {
+    const float sx = e.sx, sy = e.sy, pz = e.pz;
+    const float kx = e.kx, ky = e.ky, kz = e.kz, lx = e.lx, ly = e.ly, lz = e.lz;
+    const float dL_dsx = (dL_dG * -G) * sx + dL_dz * q1.z;
+    const float dL_dsy = (dL_dG * -G) * sy + dL_dz * q1.w;
+    const float dsx_pz = dL_dsx / pz, dsy_pz = dL_dsy / pz;
+    const float dpx = dsx_pz, dpy = dsy_pz, dpz = -(dsx_pz * sx + dsy_pz * sy);
+    // dL_dk = cross(l, dL_dp), dL_dl = cross(dL_dp, k)
+    const float dkx = ly * dpz - lz * dpy, dky = lz * dpx - lx * dpz, dkz = lx * dpy - ly * dpx;
+    const float dlx = dpy * kz - dpz * ky, dly = dpz * kx - dpx * kz, dlz = dpx * ky - dpy * kx;
+    atomicAdd(&s_acc[0][j], -dkx);
+    atomicAdd(&s_acc[1][j], -dky);
+    atomicAdd(&s_acc[2][j], -dkz);
+    atomicAdd(&s_acc[3][j], -dlx);
+    atomicAdd(&s_acc[4][j], -dly);
+    atomicAdd(&s_acc[5][j], -dlz);
+    atomicAdd(&s_acc[6][j], (pixx * dkx + pixy * dlx) + dL_dz * sx);
+    atomicAdd(&s_acc[7][j], (pixx * dky + pixy * dly) + dL_dz * sy);
+    atomicAdd(&s_acc[8][j], (pixx * dkz + pixy * dlz) + dL_dz * 1.0f);
+  } else {
+    const float dG_ddelx = (-G * 2.0f) * e.dx;
+    const float dG_ddely = (-G * 2.0f) * e.dy;
+    atomicAdd(&s_acc[9][j], dL_dG * dG_ddelx);
+    atomicAdd(&s_acc[10][j], dL_dG * dG_ddely);
+    atomicAdd(&s_acc[8][j], dL_dz);
+  }
+  atomicAdd(&s_acc[14][j], G * dL_dalpha);
+  (void)pixx;
+}
+
+// Flushes a chunk's per-entry sums: one FP32 atomic per (entry, slot).
+SR_DEV void flush_grads(const BlendBwdArgs& a, float (*s_acc)[64], uint32_t id, int lane) {
+#pragma unroll
+  for (int k = 0; k < kNumG; ++k) {
+    const float v = s_acc[k][lane];
+    if (v != 0.0f) atomic_add_f32(grad_slot(a, id, k), v);
+  }
+}
+
+// Backward blend, pass 1 of 2.  One wave64 per 8 x 8 quadrant; each lane
+// walks, back to front, the pairs its pixel blended (the accepted pairs in
+// front of its last contributor, exactly the reference's), running only the
+// per-pixel recurrences and writing each pair's (w, dL/dalpha, dL/ddepth)
+// to the pair store.  The pairs come from the store as contiguous per-lane
+// streams; nothing is staged and there are no barriers, so the serial
+// per-pixel chain carries only the recurrences.  A quadrant some of whose
+// pairs missed the store (q_flag) instead runs the whole backward here over
+// staged records, recomputing those pairs.
 __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
   __shared__ uint2 s_e[64];
@@ -51,56 +204,60 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (lane & 7));
   const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (lane >> 3));
   const bool inside = pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
-  const size_t HW = (size_t)vi.H * vi.W, pid = (size_t)vi.W * pyi + pxi, pb = vi.pix_base;
-  const float* final_T = a.final_T + 3 * pb;
-  const uint32_t* n_contrib = a.n_contrib + 2 * pb;
-  const float* dL_dpix = a.dL_dpix + 3 * pb;
-  const float* dL_dothers = a.dL_dothers + 7 * pb;
   const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
   const int cnt = (int)a.sub_count[blockIdx.x];
   const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
   const uint64_t* masks = a.masks + mask_base(Q, rbeg, ntile, tile, q);
+  const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
   const float pixx = (float)pxi, pixy = (float)pyi;
+  const PixelBwd px = pixel_bwd(a, vi, inside, pxi, pyi);
+  ChainBwd ch;
+  ch.T = px.T_final;
+  const int nch = (cnt + 63) / 64;
+  const bool active = inside && px.last_contributor > 0;
 
-  const float T_final = inside ? final_T[pid] : 0.0f;
-  float T = T_final;
-  const uint32_t last_contributor = inside ? n_contrib[pid] : 0u;
-  const int median_contributor = inside ? (int)n_contrib[pid + HW] : 0;
-  float dLp0 = 0.0f, dLp1 = 0.0f, dLp2 = 0.0f;
-  float dL_ddepth = 0.0f, dL_daccum = 0.0f, dL_dreg = 0.0f, dL_dmedian = 0.0f;
-  float dLn0 = 0.0f, dLn1 = 0.0f, dLn2 = 0.0f;
-  float final_D = 0.0f, final_D2 = 0.0f;
-  if (inside) {
-    dLp0 = dL_dpix[pid];
-    dLp1 = dL_dpix[HW + pid];
-    dLp2 = dL_dpix[2 * HW + pid];
-    dL_ddepth = dL_dothers[pid];
-    dL_daccum = dL_dothers[HW + pid];
-    dLn0 = dL_dothers[2 * HW + pid];
-    dLn1 = dL_dothers[3 * HW + pid];
-    dLn2 = dL_dothers[4 * HW + pid];
-    dL_dmedian = dL_dothers[5 * HW + pid];
-    dL_dreg = dL_dothers[6 * HW + pid];
-    final_D = final_T[pid + HW];
-    final_D2 = final_T[pid + 2 * HW];
+  if (a.q_flag[blockIdx.x] == 0u) {
+    uint64_t n_mask = (nch > 0 && active) ? masks[(size_t)(nch - 1) * 64 + lane] : 0ull;
+    uint32_t n_foff = (nch > 0 && active) ? frag_off[(size_t)(nch - 1) * 64 + lane] : 0u;
+    for (int c = nch - 1; c >= 0; --c) {
+      const uint64_t mask = n_mask;
+      const uint32_t foff = n_foff;
+      if (c > 0 && active) {
+        n_mask = masks[(size_t)(c - 1) * 64 + lane];
+        n_foff = frag_off[(size_t)(c - 1) * 64 + lane];
+      }
+      int f = (int)foff + __popcll(mask) - 1;  // this lane's last stored pair of the chunk
+      const int f0 = (int)foff;
+      while (f >= f0) {
+        float2 fa[kPre], fm[kPre], fd[kPre];
+        float4 fn[kPre];
+        uint32_t fp[kPre];
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) {
+          if (f - k >= f0) {
+            fa[k] = a.ps.fA[f - k];
+            fp[k] = a.ps.fP[f - k];
+            fn[k] = a.ps.fN[f - k];
+            fm[k] = a.ps.fM[f - k];
+            fd[k] = a.ps.fD[f - k];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kPre; ++k) {
+          if (f - k >= f0 && fp[k] < px.last_contributor) {
+            a.ps.fW[f - k] = ch.step(px, fa[k].x, fa[k].y, fd[k].x, fd[k].y, fp[k], fn[k], fm[k].x, fm[k].y);
+          }
+        }
+        f -= kPre;
+      }
+    }
+    return;
   }
-  const float final_A = 1.0f - T_final;
-  float acc_r0 = 0.0f, acc_r1 = 0.0f, acc_r2 = 0.0f;
-  float last_c0 = 0.0f, last_c1 = 0.0f, last_c2 = 0.0f;
-  float last_depth = 0.0f, accum_depth_rec = 0.0f, accum_alpha_rec = 0.0f;
-  float last_n0 = 0.0f, last_n1 = 0.0f, last_n2 = 0.0f;
-  float acc_n0 = 0.0f, acc_n1 = 0.0f, acc_n2 = 0.0f;
-  float last_dL_dT = 0.0f, last_alpha = 0.0f;
+
+  // Staged path: per chunk, stage the records, run the recurrences and the
+  // gradients together, sum per entry in LDS and flush.
   const float mcoef = kFar / (kFar - kNear);
   const float dmd_num = kFar * kNear, dmd_den0 = kFar - kNear;
-  float bg_dot = 0.0f;
-  bg_dot += a.bg[0] * dLp0;
-  bg_dot += a.bg[1] * dLp1;
-  bg_dot += a.bg[2] * dLp2;
-
-  const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
-  const int nch = (cnt + 63) / 64;
-  const bool active = inside && last_contributor > 0;
   ChunkRegs nx{};
   uint2 v2 = make_uint2(0u, 0u);
   if (nch > 0) {
@@ -129,139 +286,30 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
       v2 = chunk_entry(list, cnt, lane, c - 2, nch);
       nx = chunk_load(masks, frag_off, a.rec, lane, c - 1, ve);
     }
-    if (__ballot(mask != 0ull) == 0ull) continue;
-    // Reverse step of the reference (backward.cu:282-437) for accepted entry j.
-    auto body = [&](const Pair& e, int j) {
-      const uint32_t contributor = s_e[j].y;  // 0-based position in the tile list
-      const float4 q2 = s_q2[j];
-      const float4 q3 = s_q3[j], q4 = s_q4[j];
-      const float alpha = e.alpha, G = e.G, c_d = e.depth;
-      const float opa = q2.w;
-      T = T / (1.0f - alpha);
-      const float w = alpha * T;
-      float dL_dalpha = 0.0f;
-      acc_r0 = last_alpha * last_c0 + (1.0f - last_alpha) * acc_r0;
-      last_c0 = q3.w;
-      dL_dalpha += (q3.w - acc_r0) * dLp0;
-      atomicAdd(&s_acc[15][j], w * dLp0);
-      acc_r1 = last_alpha * last_c1 + (1.0f - last_alpha) * acc_r1;
-      last_c1 = q4.x;
-      dL_dalpha += (q4.x - acc_r1) * dLp1;
-      atomicAdd(&s_acc[16][j], w * dLp1);
-      acc_r2 = last_alpha * last_c2 + (1.0f - last_alpha) * acc_r2;
-      last_c2 = q4.y;
-      dL_dalpha += (q4.y - acc_r2) * dLp2;
-      atomicAdd(&s_acc[17][j], w * dLp2);
-      float dL_dz = 0.0f, dL_dweight = 0.0f;
-      const float m_d = mcoef * (1.0f - kNear / c_d);
-      const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
-      if ((int)contributor == median_contributor - 1) dL_dz += dL_dmedian;
-      dL_dweight += ((final_D2 + (m_d * m_d) * final_A) - (2.0f * m_d) * final_D) * dL_dreg;
-      dL_dalpha += dL_dweight - last_dL_dT;
-      last_dL_dT = dL_dweight * alpha + (1.0f - alpha) * last_dL_dT;
-      const float dL_dmd = ((2.0f * (T * alpha)) * (m_d * final_A - final_D)) * dL_dreg;
-      dL_dz += dL_dmd * dmd_dd;
-      accum_depth_rec = last_alpha * last_depth + (1.0f - last_alpha) * accum_depth_rec;
-      last_depth = c_d;
-      dL_dalpha += (c_d - accum_depth_rec) * dL_ddepth;
-      accum_alpha_rec = last_alpha + (1.0f - last_alpha) * accum_alpha_rec;
-      dL_dalpha += (1.0f - accum_alpha_rec) * dL_daccum;
-      acc_n0 = last_alpha * last_n0 + (1.0f - last_alpha) * acc_n0;
-      last_n0 = q3.x;
-      dL_dalpha += (q3.x - acc_n0) * dLn0;
-      atomicAdd(&s_acc[11][j], (alpha * T) * dLn0);
-      acc_n1 = last_alpha * last_n1 + (1.0f - last_alpha) * acc_n1;
-      last_n1 = q3.y;
-      dL_dalpha += (q3.y - acc_n1) * dLn1;
-      atomicAdd(&s_acc[12][j], (alpha * T) * dLn1);
-      acc_n2 = last_alpha * last_n2 + (1.0f - last_alpha) * acc_n2;
-      last_n2 = q3.z;
-      dL_dalpha += (q3.z - acc_n2) * dLn2;
-      atomicAdd(&s_acc[13][j], (alpha * T) * dLn2);
-      dL_dalpha *= T;
-      last_alpha = alpha;
-      dL_dalpha += (-T_final / (1.0f - alpha)) * bg_dot;
-      const float dL_dG = opa * dL_dalpha;
-      dL_dz += (alpha * T) * dL_ddepth;
-      if (e.rho3d <= e.rho2d) {
-        const float sx = e.sx, sy = e.sy, pz = e.pz;
-        const float kx = e.kx, ky = e.ky, kz = e.kz, lx = e.lx, ly = e.ly, lz = e.lz;
-        const float4 q1 = s_q1[j];
-        const float dL_dsx = (dL_dG * -G) * sx + dL_dz * q1.z;
-        const float dL_dsy = (dL_dG * -G) * sy + dL_dz * q1.w;
-        const float dsx_pz = dL_dsx / pz, dsy_pz = dL_dsy / pz;
-        const float dpx = dsx_pz, dpy = dsy_pz, dpz = -(dsx_pz * sx + dsy_pz * sy);
-        // dL_dk = cross(l, dL_dp), dL_dl = cross(dL_dp, k)
-        const float dkx = ly * dpz - lz * dpy, dky = lz * dpx - lx * dpz, dkz = lx * dpy - ly * dpx;
-        const float dlx = dpy * kz - dpz * ky, dly = dpz * kx - dpx * kz, dlz = dpx * ky - dpy * kx;
-        atomicAdd(&s_acc[0][j], -dkx);
-        atomicAdd(&s_acc[1][j], -dky);
-        atomicAdd(&s_acc[2][j], -dkz);
-        atomicAdd(&s_acc[3][j], -dlx);
-        atomicAdd(&s_acc[4][j], -dly);
-        atomicAdd(&s_acc[5][j], -dlz);
-        atomicAdd(&s_acc[6][j], (pixx * dkx + pixy * dlx) + dL_dz * sx);
-        atomicAdd(&s_acc[7][j], (pixx * dky + pixy * dly) + dL_dz * sy);
-        atomicAdd(&s_acc[8][j], (pixx * dkz + pixy * dlz) + dL_dz * 1.0f);
-      } else {
-        const float dG_ddelx = (-G * 2.0f) * e.dx;
-        const float dG_ddely = (-G * 2.0f) * e.dy;
-        atomicAdd(&s_acc[9][j], dL_dG * dG_ddelx);
-        atomicAdd(&s_acc[10][j], dL_dG * dG_ddely);
-        atomicAdd(&s_acc[8][j], dL_dz);
-      }
-      atomicAdd(&s_acc[14][j], G * dL_dalpha);
-    };
-    if (foff != kNoFrag) {
-      // Pairs are stored in entry order; walk them from the back, fetching
-      // kPre ahead so the loads overlap.
-      while (mask != 0ull) {
-        int jj[kPre];
-        float2 fa[kPre], fb[kPre];
-        float fc[kPre];
-#pragma unroll
-        for (int k = 0; k < kPre; ++k) {
-          jj[k] = -1;
-          fa[k] = fb[k] = make_float2(0.0f, 0.0f);
-          fc[k] = 0.0f;
-          if (mask != 0ull) {
-            const int j = 63 - __builtin_clzll(mask);
-            mask &= ~(1ull << j);
-            --r;
-            jj[k] = j;
-            fa[k] = a.fragA[foff + r];
-            fb[k] = a.fragB[foff + r];
-            fc[k] = a.fragC[foff + r];
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < kPre; ++k) {
-          const int j = jj[k];
-          if (j < 0 || s_e[j].y >= last_contributor) continue;
-          Pair e;
-          load_pair(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], fa[k], fb[k], fc[k], e);
-          body(e, j);
-        }
-      }
-    } else {
+    if (__ballot(mask != 0ull) != 0ull) {
       while (mask != 0ull) {
         const int j = 63 - __builtin_clzll(mask);
         mask &= ~(1ull << j);
-        if (s_e[j].y >= last_contributor) continue;  // at or behind the last contributor
+        --r;
+        const uint32_t contributor = s_e[j].y;  // 0-based position in the tile list
+        if (contributor >= px.last_contributor) continue;
+        const float4 q2 = s_q2[j];
         Pair e;
-        if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;
-        body(e, j);
+        if (foff != kNoFrag) {
+          load_pair(pixx, pixy, s_q0[j], s_q1[j], q2, a.ps.fA[foff + r], a.ps.fB[foff + r], a.ps.fC[foff + r], e);
+        } else if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], q2, 0.0f, e)) {
+          continue;
+        }
+        const float c_d = e.depth;
+        const float m_d = mcoef * (1.0f - kNear / c_d);
+        const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
+        const float4 q3 = s_q3[j], q4 = s_q4[j];
+        const float4 g = ch.step(px, e.alpha, c_d, m_d, dmd_dd, contributor, q3, q4.x, q4.y);
+        pair_grads(px, e, s_q1[j], q2.w, g.x, g.y, g.z, pixx, pixy, s_acc, j);
       }
     }
     __syncthreads();
-    if (lane < m) {
-      const uint32_t id = s_e[lane].x;
-#pragma unroll
-      for (int k = 0; k < kNumG; ++k) {
-        const float v = s_acc[k][lane];
-        if (v != 0.0f) atomic_add_f32(grad_slot(a, id, k), v);
-      }
-    }
+    if (lane < m) flush_grads(a, s_acc, s_e[lane].x, lane);
   }
 }
 
@@ -270,6 +318,108 @@ hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s) {
   if (a.vs.B != 8 && a.vs.B != 16 && a.vs.B != 32) return hipErrorInvalidValue;
   const unsigned nblk = (unsigned)(a.vs.T * quads_per_tile(a.vs.B));
   hipLaunchKernelGGL(k_blend_bwd, dim3(nblk), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+// Backward blend, pass 2 of 2.  The per-pair gradients, balanced over the
+// whole GPU on the acceptance work list: one wave per (quadrant, 64-entry
+// chunk), one lane per pixel.  A lane's pairs are independent here (the
+// pixel recurrences ran in pass 1), so each forms its 18 partial gradients
+// from the stored (G, s) and (w, dL/dalpha, dL/ddepth), sums them per entry
+// in LDS and flushes once per (entry, slot).
+__global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
+  __shared__ float4 s_q0[64], s_q1[64], s_q2[64];
+  __shared__ uint32_t s_id[64];
+  __shared__ float s_acc[kNumG][64];
+  const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
+  const int lane = threadIdx.x;
+  if (a.counters[1] & kErrCapacity) return;
+  const uint32_t nwork = *a.work_count;
+  for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const uint2 item = a.work[w];
+    const int qi = (int)item.x, c = (int)item.y;
+    if (a.q_flag[qi] != 0u) continue;  // done by the staged path of pass 1
+    const int tile = qi / Q, q = qi % Q;
+    const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
+    const int lt = tile - (int)vi.tile_base;
+    const int tx = lt % vi.gx, ty = lt / vi.gx;
+    const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (lane & 7));
+    const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (lane >> 3));
+    const bool inside = pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
+    const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
+    const int cnt = (int)a.sub_count[qi];
+    const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
+    const size_t slot = mask_base(Q, rbeg, ntile, tile, q) + (size_t)c * 64 + lane;
+    const int e0 = c * 64;
+    const int m = min(64, cnt - e0);
+    const uint64_t mask0 = inside ? a.masks[slot] : 0ull;
+    const uint32_t foff = mask0 ? a.frag_off[slot] : 0u;
+    PixelBwd px{};
+    if (inside && mask0) {
+      const size_t HW = (size_t)vi.H * vi.W, pid = (size_t)vi.W * pyi + pxi, pb = vi.pix_base;
+      px.last_contributor = a.n_contrib[2 * pb + pid];
+      const float* dL_dpix = a.dL_dpix + 3 * pb;
+      const float* dL_dothers = a.dL_dothers + 7 * pb;
+      px.dLp0 = dL_dpix[pid];
+      px.dLp1 = dL_dpix[HW + pid];
+      px.dLp2 = dL_dpix[2 * HW + pid];
+      px.dLn0 = dL_dothers[2 * HW + pid];
+      px.dLn1 = dL_dothers[3 * HW + pid];
+      px.dLn2 = dL_dothers[4 * HW + pid];
+    }
+    __syncthreads();
+    if (lane < m) {
+      const uint2 v = list[e0 + lane];
+      const float4* r = a.rec + (size_t)kRecQuads * v.x;
+      s_q0[lane] = r[0];
+      s_q1[lane] = r[1];
+      s_q2[lane] = r[2];
+      s_id[lane] = v.x;
+    }
+#pragma unroll
+    for (int k = 0; k < kNumG; ++k) s_acc[k][lane] = 0.0f;
+    __syncthreads();
+    const float pixx = (float)pxi, pixy = (float)pyi;
+    uint64_t mask = mask0;
+    uint32_t f = foff;
+    while (mask != 0ull) {
+      int jj[kPre];
+      float2 fb[kPre];
+      float fc[kPre];
+      float4 fw[kPre];
+      uint32_t fp[kPre];
+#pragma unroll
+      for (int k = 0; k < kPre; ++k) {
+        jj[k] = -1;
+        if (mask != 0ull) {
+          jj[k] = __builtin_ctzll(mask);
+          mask &= mask - 1ull;
+          fb[k] = a.ps.fB[f + k];
+          fc[k] = a.ps.fC[f + k];
+          fw[k] = a.ps.fW[f + k];
+          fp[k] = a.ps.fP[f + k];
+        }
+      }
+      f += kPre;
+#pragma unroll
+      for (int k = 0; k < kPre; ++k) {
+        const int j = jj[k];
+        if (j < 0 || fp[k] >= px.last_contributor) continue;
+        Pair e;
+        const float4 q1 = s_q1[j], q2 = s_q2[j];
+        load_pair(pixx, pixy, s_q0[j], q1, q2, make_float2(0.0f, 0.0f), fb[k], fc[k], e);
+        pair_grads(px, e, q1, q2.w, fw[k].x, fw[k].y, fw[k].z, pixx, pixy, s_acc, j);
+      }
+    }
+    __syncthreads();
+    if (lane < m) flush_grads(a, s_acc, s_id[lane], lane);
+  }
+}
+
+hipError_t launch_blend_bwd_grad(const BlendBwdArgs& a, size_t max_items, hipStream_t s) {
+  if (max_items == 0) return hipSuccess;
+  const unsigned grid = (unsigned)(max_items < 65536 ? max_items : 65536);
+  hipLaunchKernelGGL(k_blend_bwd_grad, dim3(grid), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
@@ -285,15 +435,24 @@ SR_DEV float dot3(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 SR_DEV float comp(F3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 
 // SH backward, dsr/cuda_rasterizer/backward.cu:20-139 (same association).
+template <bool kAdd>
 SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, float* dsh) {
   const float len = sqrtf((dir_orig.x * dir_orig.x + dir_orig.y * dir_orig.y) + dir_orig.z * dir_orig.z);
   const F3 dir = f3(dir_orig.x / len, dir_orig.y / len, dir_orig.z / len);
   auto S = [&](int k) { return f3(sh[3 * k], sh[3 * k + 1], sh[3 * k + 2]); };
-  auto put = [&](int k, float c) {  // dsh starts cleared; views add up
-    dsh[3 * k] += c * dL_dRGB.x;
-    dsh[3 * k + 1] += c * dL_dRGB.y;
-    dsh[3 * k + 2] += c * dL_dRGB.z;
+  auto put = [&](int k, float c) {  // the first view stores, later views add
+    if (kAdd) {
+      dsh[3 * k] += c * dL_dRGB.x;
+      dsh[3 * k + 1] += c * dL_dRGB.y;
+      dsh[3 * k + 2] += c * dL_dRGB.z;
+    } else {
+      dsh[3 * k] = c * dL_dRGB.x;
+      dsh[3 * k + 1] = c * dL_dRGB.y;
+      dsh[3 * k + 2] = c * dL_dRGB.z;
+    }
   };
+  if (!kAdd)
+    for (int k = (deg + 1) * (deg + 1); k < M; ++k) dsh[3 * k] = dsh[3 * k + 1] = dsh[3 * k + 2] = 0.0f;
   F3 dx = f3(0.0f, 0.0f, 0.0f), dy = dx, dz = dx;
   const float x = dir.x, y = dir.y, z = dir.z;
   put(0, kSH_C0);
@@ -361,18 +520,20 @@ SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, 
   return r;
 }
 
-// SH backward of one record: adds c * dL_dRGB to the surfel's dL_dsh and
-// returns the gradient of the mean through the view direction.
-SR_DEV F3 sh_grad(const PreprocessBwdArgs& a, int v, int idx, size_t r, const float* campos, float m0, float m1,
-                  float m2) {
-  (void)v;
+// SH backward of one record: stores (first view) or adds c * dL_dRGB to the
+// surfel's dL_dsh and returns the gradient of the mean through the view
+// direction.
+SR_DEV F3 sh_grad(const PreprocessBwdArgs& a, bool first, int idx, size_t r, const float* campos, float m0,
+                  float m1, float m2) {
   F3 dRGB = f3(a.dL_dcolors[3 * r], a.dL_dcolors[3 * r + 1], a.dL_dcolors[3 * r + 2]);
   const uchar4 cl = reinterpret_cast<const uchar4*>(a.clamped)[r];
   if (cl.x) dRGB.x = 0.0f;
   if (cl.y) dRGB.y = 0.0f;
   if (cl.z) dRGB.z = 0.0f;
   const F3 dir_orig = f3(m0 - campos[0], m1 - campos[1], m2 - campos[2]);
-  return sh_backward(a.D, a.M, a.shs + (size_t)idx * a.M * 3, dir_orig, dRGB, a.dL_dsh + (size_t)idx * a.M * 3);
+  float* dsh = a.dL_dsh + (size_t)idx * a.M * 3;
+  const float* sh = a.shs + (size_t)idx * a.M * 3;
+  return first ? sh_backward<false>(a.D, a.M, sh, dir_orig, dRGB, dsh) : sh_backward<true>(a.D, a.M, sh, dir_orig, dRGB, dsh);
 }
 
 // Gradients of one surfel summed over the views that see it.
@@ -384,7 +545,7 @@ struct SurfelGrad {
 // dsr/cuda_rasterizer/backward.cu:443-630.  Adds the view's contributions to
 // g (and to dL_dsh, owned by this thread); rewrites the record's dL_dtransMat
 // (precomputed T) and dL_dmean2D (the densification statistic) in place.
-SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size_t r, SurfelGrad& g) {
+SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size_t r, bool first, SurfelGrad& g) {
   const ViewInfo& vi = a.vs.v[v];
   const float tanx = vi.tanx, tany = vi.tany;
   const float focal_x = (float)vi.W / (2.0f * tanx);
@@ -497,12 +658,12 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
     g.sc[0] += (dRS0.x * Rc0.x + dRS0.y * Rc0.y) + dRS0.z * Rc0.z;
     g.sc[1] += (dRS1.x * Rc1.x + dRS1.y * Rc1.y) + dRS1.z * Rc1.z;
     F3 dm3 = f3(dM[2][0], dM[2][1], dM[2][2]);
-    if (a.shs != nullptr) dm3 = dm3 + sh_grad(a, v, idx, r, campos, m0, m1, m2);
+    if (a.shs != nullptr) dm3 = dm3 + sh_grad(a, first, idx, r, campos, m0, m1, m2);
     g.m[0] += dm3.x;
     g.m[1] += dm3.y;
     g.m[2] += dm3.z;
   } else if (a.shs != nullptr) {
-    const F3 dm3 = sh_grad(a, v, idx, r, campos, m0, m1, m2);
+    const F3 dm3 = sh_grad(a, first, idx, r, campos, m0, m1, m2);
     g.m[0] += dm3.x;
     g.m[1] += dm3.y;
     g.m[2] += dm3.z;
@@ -513,7 +674,9 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
 }
 
 // One thread per surfel, looping over the views; with one view every sum is
-// 0 + x, so the results equal the single-view formulas bit for bit.
+// 0 + x, so the results equal the single-view formulas bit for bit.  Every
+// per-surfel output is written exactly once (zeros for a surfel no view
+// sees), so none of them needs clearing beforehand.
 __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.P) return;
@@ -524,8 +687,8 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
   for (int v = 0; v < a.vs.V; ++v) {
     const size_t r = (size_t)v * a.P + idx;
     if (!(a.radii[r] > 0)) continue;
+    preprocess_bwd_view(a, v, idx, r, !any, g);
     any = true;
-    preprocess_bwd_view(a, v, idx, r, g);
     if (sum_rec) {
       op += a.dL_dopacity[r];
       c0 += a.dL_dcolors[3 * r];
@@ -533,20 +696,19 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
       c2 += a.dL_dcolors[3 * r + 2];
     }
   }
-  if (!any) return;
-  if (a.tprec == nullptr) {
-    a.dL_drots[4 * idx] = g.q[0];
-    a.dL_drots[4 * idx + 1] = g.q[1];
-    a.dL_drots[4 * idx + 2] = g.q[2];
-    a.dL_drots[4 * idx + 3] = g.q[3];
-    a.dL_dscales[2 * idx] = g.sc[0];
-    a.dL_dscales[2 * idx + 1] = g.sc[1];
+  if (!any && a.shs != nullptr) {
+    float* dsh = a.dL_dsh + (size_t)idx * a.M * 3;
+    for (int k = 0; k < 3 * a.M; ++k) dsh[k] = 0.0f;
   }
-  if (a.tprec == nullptr || a.shs != nullptr) {
-    a.dL_dmean3D[3 * idx] = g.m[0];
-    a.dL_dmean3D[3 * idx + 1] = g.m[1];
-    a.dL_dmean3D[3 * idx + 2] = g.m[2];
-  }
+  a.dL_drots[4 * idx] = g.q[0];
+  a.dL_drots[4 * idx + 1] = g.q[1];
+  a.dL_drots[4 * idx + 2] = g.q[2];
+  a.dL_drots[4 * idx + 3] = g.q[3];
+  a.dL_dscales[2 * idx] = g.sc[0];
+  a.dL_dscales[2 * idx + 1] = g.sc[1];
+  a.dL_dmean3D[3 * idx] = g.m[0];
+  a.dL_dmean3D[3 * idx + 1] = g.m[1];
+  a.dL_dmean3D[3 * idx + 2] = g.m[2];
   if (sum_rec) {
     a.dL_dopacity_out[idx] = op;
     a.dL_dcolors_out[3 * idx] = c0;

@@ -503,6 +503,7 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
   if (tid < Q) {
     const uint32_t cnt = s_total[tid];
     a.sub_count[(size_t)tile * Q + tid] = cnt;
+    a.q_flag[(size_t)tile * Q + tid] = 0u;
     const uint32_t nch = (cnt + 63u) / 64u;
     if (nch > 0) {
       const uint32_t w0 = atomicAdd(a.work_count, nch);
@@ -522,7 +523,10 @@ __global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
   if (kLarge != (n > kSmallMax)) return;
   if (n <= 0) {
     const int Q = quads_per_tile(a.vs.B);
-    if (tid < Q) a.sub_count[(size_t)tile * Q + tid] = 0;
+    if (tid < Q) {
+      a.sub_count[(size_t)tile * Q + tid] = 0;
+      a.q_flag[(size_t)tile * Q + tid] = 0u;
+    }
     return;
   }
   if (n <= CAP) {
@@ -590,8 +594,8 @@ hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
 // over the whole GPU; the blend kernels then visit only accepted pairs.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
-  __shared__ float4 s_q0[64], s_q1[64], s_q2[64];
-  __shared__ float s_rej[64];
+  __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
+  __shared__ uint32_t s_pos[64];
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const int lane = threadIdx.x;
   if (a.counters[1] & kErrCapacity) return;
@@ -613,11 +617,14 @@ __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
     const int m = min(64, cnt - e0);
     __syncthreads();
     if (lane < m) {
-      const float4* r = a.rec + (size_t)kRecQuads * list[e0 + lane].x;
+      const uint2 v = list[e0 + lane];
+      const float4* r = a.rec + (size_t)kRecQuads * v.x;
       s_q0[lane] = r[0];
       s_q1[lane] = r[1];
       s_q2[lane] = r[2];
-      s_rej[lane] = r[4].z;
+      s_q3[lane] = r[3];
+      s_q4[lane] = r[4];
+      s_pos[lane] = v.y;
     }
     __syncthreads();
     uint64_t mask = 0;
@@ -625,7 +632,7 @@ __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
     if (inside) {
       for (int j = 0; j < m; ++j) {
         Pair e;
-        if (eval_pair<true>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], s_rej[j], e)) mask |= 1ull << j;
+        if (eval_pair<true>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], s_q4[j].z, e)) mask |= 1ull << j;
       }
     }
     const size_t slot = mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane;
@@ -645,17 +652,28 @@ __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
     const bool fits = total > 0 && base + total <= (unsigned long long)a.frag_cap;
     const uint32_t my = (uint32_t)base + (incl - cntp);
     a.frag_off[slot] = fits ? my : kNoFrag;
+    if (total > 0 && !fits && lane == 0) a.q_flag[qi] = 1u;
     if (fits && cntp > 0) {
+      const float mcoef = kFar / (kFar - kNear);
+      const float dmd_num = kFar * kNear, dmd_den0 = kFar - kNear;
       uint64_t mm = mask;
-      uint32_t r = 0;
+      uint32_t r = my;
       while (mm != 0ull) {
         const int j = __builtin_ctzll(mm);
         mm &= mm - 1ull;
         Pair e;
         eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e);
-        a.fragA[my + r] = make_float2(e.alpha, e.depth);
-        a.fragB[my + r] = make_float2(e.G, e.sx);
-        a.fragC[my + r] = e.sy;
+        const float4 q3 = s_q3[j], q4 = s_q4[j];
+        // the distortion weight m(depth) and its derivative, as the blends form them
+        const float m_d = mcoef * (1.0f - kNear / e.depth);
+        const float dmd_dd = dmd_num / ((dmd_den0 * e.depth) * e.depth);
+        a.ps.fA[r] = make_float2(e.alpha, e.depth);
+        a.ps.fP[r] = s_pos[j];
+        a.ps.fN[r] = q3;
+        a.ps.fM[r] = make_float2(q4.x, q4.y);
+        a.ps.fD[r] = make_float2(m_d, dmd_dd);
+        a.ps.fB[r] = make_float2(e.G, e.sx);
+        a.ps.fC[r] = e.sy;
         ++r;
       }
     }
@@ -697,6 +715,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   const int cnt = (int)a.sub_count[blockIdx.x];
   const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
   const uint64_t* masks = a.masks + mask_base(Q, rbeg, ntile, tile, q);
+  const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
   const float pixx = (float)pxi, pixy = (float)pyi;
   float T = 1.0f;
   uint32_t last_contributor = 0;
@@ -704,101 +723,125 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   float D = 0.0f, M1 = 0.0f, M2 = 0.0f, distortion = 0.0f, median_depth = 0.0f, median_contributor = -1.0f;
   const float mcoef = kFar / (kFar - kNear);
   const int nch = (cnt + 63) / 64;
-  const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
-  ChunkRegs nx{};
-  uint2 v2 = make_uint2(0u, 0u);
-  if (nch > 0) {
-    nx = chunk_load(masks, frag_off, a.rec, lane, 0, chunk_entry(list, cnt, lane, 0, nch));
-    v2 = chunk_entry(list, cnt, lane, 1, nch);
-  }
-  for (int c = 0; c < nch; ++c) {
-    const int m = min(64, cnt - c * 64);
-    uint64_t mask = done ? 0ull : nx.mask;
-    uint32_t foff = mask ? nx.foff : kNoFrag;
-    __syncthreads();
-    if (lane < m) {
-      s_pos[lane] = nx.v.y;
-      s_q0[lane] = nx.q0;
-      s_q1[lane] = nx.q1;
-      s_q2[lane] = nx.q2;
-      s_q3[lane] = nx.q3;
-      s_q4[lane] = nx.q4;
+  // One front-to-back step of the reference's update (forward.cu:383-415);
+  // false once T would drop below 1e-4 (the pixel is done).  mm is the
+  // distortion weight mcoef (1 - near / depth).
+  auto step = [&](float alpha, float depth, float mm, uint32_t pos, const float4& nc, float g, float bl) -> bool {
+    const float test_T = T * (1.0f - alpha);
+    if (test_T < 0.0001f) return false;
+    const uint32_t contributor = pos + 1u;
+    const float w = alpha * T;
+    const float A = 1.0f - T;
+    distortion += (((mm * mm) * A + M2) - (2.0f * mm) * M1) * w;
+    D += depth * w;
+    M1 += mm * w;
+    M2 += (mm * mm) * w;
+    if (T > 0.5f) {
+      median_depth = depth;
+      median_contributor = (float)contributor;
     }
-    __syncthreads();
-    if (c + 1 < nch) {  // in flight while this chunk blends
-      const uint2 ve = v2;
-      v2 = chunk_entry(list, cnt, lane, c + 2, nch);
-      nx = chunk_load(masks, frag_off, a.rec, lane, c + 1, ve);
-    }
-    if (__ballot(mask != 0ull) == 0ull) continue;
-    // One front-to-back step of the reference's update (forward.cu:383-415);
-    // false once T would drop below 1e-4 (the pixel is done).
-    auto step = [&](float alpha, float depth, uint32_t pos, const float4& q3, float q4x, float q4y) -> bool {
-      const float test_T = T * (1.0f - alpha);
-      if (test_T < 0.0001f) return false;
-      const uint32_t contributor = pos + 1u;
-      const float w = alpha * T;
-      const float A = 1.0f - T;
-      const float mm = mcoef * (1.0f - kNear / depth);
-      distortion += (((mm * mm) * A + M2) - (2.0f * mm) * M1) * w;
-      D += depth * w;
-      M1 += mm * w;
-      M2 += (mm * mm) * w;
-      if (T > 0.5f) {
-        median_depth = depth;
-        median_contributor = (float)contributor;
+    N0 += nc.x * w;
+    N1 += nc.y * w;
+    N2 += nc.z * w;
+    C0 += nc.w * w;
+    C1 += g * w;
+    C2 += bl * w;
+    T = test_T;
+    last_contributor = contributor;
+    return true;
+  };
+  if (a.q_flag[blockIdx.x] == 0u) {
+    // Every pair of the quadrant is in the store, which holds all a step
+    // reads: each lane streams its own pairs, kPre loads in flight, with no
+    // record staging and no barriers.
+    uint64_t n_mask = nch > 0 ? masks[lane] : 0ull;
+    uint32_t n_foff = nch > 0 ? frag_off[lane] : 0u;
+    for (int c = 0; c < nch; ++c) {
+      const uint64_t mask = done ? 0ull : n_mask;
+      uint32_t f = n_foff;
+      if (c + 1 < nch) {
+        n_mask = masks[(size_t)(c + 1) * 64 + lane];
+        n_foff = frag_off[(size_t)(c + 1) * 64 + lane];
       }
-      N0 += q3.x * w;
-      N1 += q3.y * w;
-      N2 += q3.z * w;
-      C0 += q3.w * w;
-      C1 += q4x * w;
-      C2 += q4y * w;
-      T = test_T;
-      last_contributor = contributor;
-      return true;
-    };
-    if (foff != kNoFrag) {
-      // The lane's pairs of this chunk are contiguous in entry order: fetch
-      // them kPre at a time so the loads overlap instead of stalling each step.
-      const float2* fa = a.fragA + foff;
-      while (mask != 0ull && !done) {
-        int jj[kPre];
-        float2 ad[kPre];
+      int left = __popcll(mask);
+      while (left > 0 && !done) {
+        float2 fa[kPre], fm[kPre];
+        float4 fn[kPre];
+        uint32_t fp[kPre];
+        float fd[kPre];
 #pragma unroll
         for (int k = 0; k < kPre; ++k) {
-          jj[k] = -1;
-          ad[k] = make_float2(0.0f, 1.0f);
-          if (mask != 0ull) {
-            jj[k] = __builtin_ctzll(mask);
-            mask &= mask - 1ull;
-            ad[k] = fa[k];
+          if (k < left) {
+            fa[k] = a.ps.fA[f + k];
+            fp[k] = a.ps.fP[f + k];
+            fn[k] = a.ps.fN[f + k];
+            fm[k] = a.ps.fM[f + k];
+            fd[k] = a.ps.fD[f + k].x;
           }
         }
-        fa += kPre;
 #pragma unroll
         for (int k = 0; k < kPre; ++k) {
-          if (jj[k] >= 0 && !done) {
-            const int j = jj[k];
-            const float4 q3 = s_q3[j], q4 = s_q4[j];
-            if (!step(ad[k].x, ad[k].y, s_pos[j], q3, q4.x, q4.y)) done = true;
+          if (k < left && !done) {
+            if (!step(fa[k].x, fa[k].y, fd[k], fp[k], fn[k], fm[k].x, fm[k].y)) done = true;
           }
         }
+        f += kPre;
+        left -= kPre;
       }
-    } else {
+      if (__ballot(!done) == 0ull) break;
+    }
+  } else {
+    // Some chunk missed the store: stage the records chunk by chunk and
+    // recompute the pairs of chunks without stored pairs.
+    ChunkRegs nx{};
+    uint2 v2 = make_uint2(0u, 0u);
+    if (nch > 0) {
+      nx = chunk_load(masks, frag_off, a.rec, lane, 0, chunk_entry(list, cnt, lane, 0, nch));
+      v2 = chunk_entry(list, cnt, lane, 1, nch);
+    }
+    for (int c = 0; c < nch; ++c) {
+      const int m = min(64, cnt - c * 64);
+      uint64_t mask = done ? 0ull : nx.mask;
+      const uint32_t foff = mask ? nx.foff : kNoFrag;
+      __syncthreads();
+      if (lane < m) {
+        s_pos[lane] = nx.v.y;
+        s_q0[lane] = nx.q0;
+        s_q1[lane] = nx.q1;
+        s_q2[lane] = nx.q2;
+        s_q3[lane] = nx.q3;
+        s_q4[lane] = nx.q4;
+      }
+      __syncthreads();
+      if (c + 1 < nch) {  // in flight while this chunk blends
+        const uint2 ve = v2;
+        v2 = chunk_entry(list, cnt, lane, c + 2, nch);
+        nx = chunk_load(masks, frag_off, a.rec, lane, c + 1, ve);
+      }
+      if (__ballot(mask != 0ull) == 0ull) continue;
+      uint32_t f = foff;
       while (mask != 0ull) {
         const int j = __builtin_ctzll(mask);
         mask &= mask - 1ull;
-        Pair e;
-        if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;
+        float alpha, depth;
+        if (foff != kNoFrag) {
+          const float2 fa = a.ps.fA[f++];
+          alpha = fa.x;
+          depth = fa.y;
+        } else {
+          Pair e;
+          if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;
+          alpha = e.alpha;
+          depth = e.depth;
+        }
         const float4 q3 = s_q3[j], q4 = s_q4[j];
-        if (!step(e.alpha, e.depth, s_pos[j], q3, q4.x, q4.y)) {
+        if (!step(alpha, depth, mcoef * (1.0f - kNear / depth), s_pos[j], q3, q4.x, q4.y)) {
           done = true;
           break;
         }
       }
+      if (__ballot(!done) == 0ull) break;
     }
-    if (__ballot(!done) == 0ull) break;
   }
   if (inside) {
     const size_t HW = (size_t)vi.H * vi.W, pid = (size_t)vi.W * pyi + pxi, pb = vi.pix_base;

@@ -103,7 +103,7 @@ inline GeomLayout geom_layout(size_t R) {
 // Image state: per-tile binning tables and per-pixel blend state of all views.
 struct ImageLayout {
   int T, N;
-  size_t counters, tile_count, tile_start, tile_fill, ranges, sub_count, final_T, n_contrib, total;
+  size_t counters, tile_count, tile_start, tile_fill, ranges, sub_count, q_flag, final_T, n_contrib, total;
 };
 inline ImageLayout image_layout(const ViewSet& vs) {
   ImageLayout l{};
@@ -116,6 +116,7 @@ inline ImageLayout image_layout(const ViewSet& vs) {
   l.tile_fill = off; off += align_up((size_t)l.T * 4);
   l.ranges = off; off += align_up((size_t)l.T * 8);
   l.sub_count = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);
+  l.q_flag = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);  // 1: a chunk missed the pair store
   l.final_T = off; off += align_up((size_t)l.N * 3 * 4);
   l.n_contrib = off; off += align_up((size_t)l.N * 2 * 4);
   l.total = off;
@@ -125,16 +126,22 @@ inline ImageLayout image_layout(int W, int H, int B) { return image_layout(singl
 
 // Binning state: one entry per (surfel, tile) instance.
 struct BinLayout {
-  size_t keys, keys_tmp, point_list, sub, masks, work, frag_off, fragA, fragB, fragC, total;
+  size_t keys, keys_tmp, point_list, sub, masks, work, frag_off, total;
+  size_t fA, fP, fN, fM, fD, fB, fC, fW;
   size_t frag_cap;
 };
 // Accepted-pair store: per accepted (pixel, entry) pair the acceptance pass
-// keeps fragA = (alpha, depth), fragB = (G, s.x) and fragC = s.y, where
-// G = exp(-rho / 2) and s the intersection in splat coordinates (the forward
-// blend reads fragA only); a lane's pairs of one chunk are contiguous from
-// frag_off[mask slot], in entry order.  Capacity kFragPerInstance pairs per
-// tile instance; chunks that do not fit get frag_off = kNoFrag and the blend
-// kernels recompute their pairs instead.
+// keeps everything the per-pixel blend chains read, so they stream it
+// without staging records:
+//   fA = (alpha, depth)   fP = position of the entry in the tile list
+//   fN = (normal.xyz, r)  fM = (g, b)   fD = (m, dm/ddepth) of the distortion
+//   fB = (G, s.x)         fC = s.y     (G = exp(-rho/2), s in splat coordinates)
+// and fW = (w, dL/dalpha, dL/ddepth, 0) is written by the backward chain pass
+// for the per-pair gradient pass.  A lane's pairs of one chunk are
+// contiguous from frag_off[mask slot], in entry order.  Capacity
+// kFragPerInstance pairs per tile instance; a chunk that does not fit gets
+// frag_off = kNoFrag, and its quadrant (q_flag) is blended by the staged
+// path that recomputes such pairs.
 constexpr size_t kFragPerInstance = 24;
 constexpr uint32_t kNoFrag = 0xFFFFFFFFu;
 // Per-pixel acceptance masks: quadrant (t, q) owns mask slots
@@ -163,12 +170,29 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
   b.frag_off = off; off += align_up(nslots * 4);
   b.frag_cap = n * (pair_store > 0 ? (size_t)pair_store : kFragPerInstance);
   if (b.frag_cap > 0x7fffffffu) b.frag_cap = 0x7fffffffu;
-  b.fragA = off; off += align_up(b.frag_cap * 8);
-  b.fragB = off; off += align_up(b.frag_cap * 8);
-  b.fragC = off; off += align_up(b.frag_cap * 4);
+  b.fA = off; off += align_up(b.frag_cap * 8);
+  b.fP = off; off += align_up(b.frag_cap * 4);
+  b.fN = off; off += align_up(b.frag_cap * 16);
+  b.fM = off; off += align_up(b.frag_cap * 8);
+  b.fD = off; off += align_up(b.frag_cap * 8);
+  b.fB = off; off += align_up(b.frag_cap * 8);
+  b.fC = off; off += align_up(b.frag_cap * 4);
+  b.fW = off; off += align_up(b.frag_cap * 16);
   b.total = off;
   return b;
 }
+
+// Device pointers of the accepted-pair store.
+struct PairStore {
+  float2* fA;
+  uint32_t* fP;
+  float4* fN;
+  float2* fM;
+  float2* fD;
+  float2* fB;
+  float* fC;
+  float4* fW;
+};
 
 constexpr uint32_t kErrPrefiltered = 1u;
 constexpr uint32_t kErrCapacity = 2u;  // more tile instances than the binning capacity
@@ -220,6 +244,7 @@ struct SortArgs {
   uint32_t* sub_count;
   uint2* work;          // (quadrant index, chunk) items for the acceptance pass
   uint32_t* work_count;
+  uint32_t* q_flag;
 };
 
 struct AcceptArgs {
@@ -233,9 +258,8 @@ struct AcceptArgs {
   const uint32_t* work_count;
   uint64_t* masks;
   uint32_t* frag_off;
-  float2* fragA;
-  float2* fragB;
-  float* fragC;
+  PairStore ps;
+  uint32_t* q_flag;
   unsigned long long* frag_fill;
   uint32_t frag_cap;
 };
@@ -248,7 +272,8 @@ struct BlendArgs {
   const uint32_t* sub_count;
   const uint64_t* masks;
   const uint32_t* frag_off;
-  const float2* fragA;
+  PairStore ps;
+  const uint32_t* q_flag;
   const float4* rec;
   const float* bg;
   float* final_T;
@@ -268,9 +293,10 @@ struct BlendBwdArgs {
   const uint32_t* sub_count;
   const uint64_t* masks;
   const uint32_t* frag_off;
-  const float2* fragA;
-  const float2* fragB;
-  const float* fragC;
+  PairStore ps;
+  const uint32_t* q_flag;
+  const uint2* work;  // (quadrant, chunk) items of the acceptance pass
+  const uint32_t* work_count;
   const float4* rec;
   const float* bg;
   const float* final_T;
@@ -325,6 +351,7 @@ hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s);
 hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s);
 hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s);
 hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s);
+hipError_t launch_blend_bwd_grad(const BlendBwdArgs& a, size_t max_items, hipStream_t s);
 hipError_t launch_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s);
 hipError_t launch_mark_visible(int P, const float* means3D, const float* vm, uint8_t* present,
                                hipStream_t s);
