@@ -137,8 +137,9 @@ size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int num_
 /* Per-stage GPU timing with HIP events on the caller's stream.
  * Stages: 0 preprocess, 1 tile scan, 2 instance scatter, 3 tile sort,
  * 4 forward blend, 5 gradient clear, 6 backward blend (per-pixel pass),
- * 7 backward preprocess, 8 acceptance masks, 9 backward blend (per-pair pass). */
-#define SR_NUM_STAGES 10
+ * 7 backward preprocess, 8 acceptance masks, 9 backward blend (per-pair pass),
+ * 10 accepted-pair store (offsets and fill). */
+#define SR_NUM_STAGES 11
 int sr_set_timing(int enabled);
 /* Synchronises pending events; ms[i] = total milliseconds, count[i] = launches. */
 int sr_get_timing(double* ms, int* count, int n);

@@ -22,7 +22,7 @@ SR_ERR_HIP = 2
 SR_ERR_ALLOC = 3
 SR_ERR_PREFILTERED = 4
 SR_ERR_CAPACITY = 5
-SR_NUM_STAGES = 10
+SR_NUM_STAGES = 11
 STAGE_NAMES = (
     "preprocess",
     "tile_scan",
@@ -34,6 +34,7 @@ STAGE_NAMES = (
     "preprocess_bwd",
     "accept",
     "blend_bwd_grad",
+    "pair_store",
 )
 
 EXPORTED = (

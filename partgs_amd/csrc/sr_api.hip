@@ -345,11 +345,18 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   aa.work_count = counters + 2;
   aa.masks = (uint64_t*)(bin + bl.masks);
   aa.frag_off = (uint32_t*)(bin + bl.frag_off);
+  aa.lane_range = (uint2*)(pl.img + pl.il.lane_range);
+  aa.q_cost = (uint32_t*)(pl.img + pl.il.q_cost);
   aa.ps = pair_store(bin, bl);
   aa.q_flag = sa.q_flag;
   aa.frag_fill = (unsigned long long*)(counters + 4);
   aa.frag_cap = (uint32_t)bl.frag_cap;
-  SR_STAGE(8, dbg, st, "accept", launch_accept(aa, work_capacity(pl.capacity, pl.vs.B, pl.il.T), st));
+  const size_t items = work_capacity(pl.capacity, pl.vs.B, pl.il.T);
+  SR_STAGE(8, dbg, st, "accept", launch_accept(aa, items, st));
+  SR_STAGE(10, dbg, st, "pair_offsets", launch_pair_offsets(aa, st));
+  SR_STAGE(10, dbg, st, "order",
+           launch_order_quads(pl.il.T * quads_per_tile(pl.vs.B), aa.q_cost, (uint32_t*)(pl.img + pl.il.q_order), st));
+  SR_STAGE(10, dbg, st, "pair_write", launch_pair_write(aa, items, st));
   return SR_OK;
 }
 
@@ -362,6 +369,8 @@ int run_blend_fwd(Pipeline& pl, float* out_color, float* out_others, bool dbg, h
   ba.sub_count = (const uint32_t*)(pl.img + pl.il.sub_count);
   ba.masks = (const uint64_t*)(pl.bin + pl.bl.masks);
   ba.frag_off = (const uint32_t*)(pl.bin + pl.bl.frag_off);
+  ba.lane_range = (const uint2*)(pl.img + pl.il.lane_range);
+  ba.q_order = (const uint32_t*)(pl.img + pl.il.q_order);
   ba.ps = pair_store(pl.bin, pl.bl);
   ba.q_flag = (const uint32_t*)(pl.img + pl.il.q_flag);
   ba.rec = (const float4*)(pl.geom + pl.gl.rec);
@@ -386,6 +395,8 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
     ba.sub_count = (const uint32_t*)(pl.img + pl.il.sub_count);
     ba.masks = (const uint64_t*)(pl.bin + pl.bl.masks);
     ba.frag_off = (const uint32_t*)(pl.bin + pl.bl.frag_off);
+    ba.lane_range = (const uint2*)(pl.img + pl.il.lane_range);
+    ba.q_order = (const uint32_t*)(pl.img + pl.il.q_order);
     ba.ps = pair_store(pl.bin, pl.bl);
     ba.q_flag = (const uint32_t*)(pl.img + pl.il.q_flag);
     ba.work = (const uint2*)(pl.bin + pl.bl.work);
@@ -534,6 +545,9 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
              launch_fill_u32((uint32_t*)(img + il.sub_count), 0u, (size_t)il.T * quads_per_tile(B), st));
     SR_STAGE(3, dbg, st, "clear",
              launch_fill_u32((uint32_t*)(img + il.q_flag), 0u, (size_t)il.T * quads_per_tile(B), st));
+    SR_STAGE(3, dbg, st, "clear",
+             launch_fill_u32((uint32_t*)(img + il.lane_range), 0u, (size_t)il.T * quads_per_tile(B) * 128, st));
+    SR_STAGE(3, dbg, st, "clear", launch_iota_u32((uint32_t*)(img + il.q_order), (size_t)il.T * quads_per_tile(B), st));
   }
   return run_blend_fwd(pl, out_color, out_others, dbg, st);
 }

@@ -196,16 +196,22 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   __shared__ float s_acc[kNumG][64];
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   if (a.counters[1] & kErrCapacity) return;
-  const int tile = blockIdx.x / Q, q = blockIdx.x % Q;
+  const int qi = (int)a.q_order[blockIdx.x / kSplit], part = blockIdx.x % kSplit;
+  const int tile = qi / Q, q = qi % Q;
   const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
   const int lt = tile - (int)vi.tile_base;
   const int tx = lt % vi.gx, ty = lt / vi.gx;
   const int lane = threadIdx.x;
-  const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (lane & 7));
-  const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (lane >> 3));
-  const bool inside = pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
+  const bool staged = a.q_flag[qi] != 0u;
+  if (staged && part != 0) return;  // the staged path uses all 64 lanes of part 0
+  // pixel of this lane: all 64 in the staged path, 64 / kSplit per part otherwise
+  const int pix = staged ? lane : part * (64 / kSplit) + lane;
+  const bool lane_ok = staged || lane < 64 / kSplit;
+  const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (pix & 7));
+  const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (pix >> 3));
+  const bool inside = lane_ok && pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
   const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
-  const int cnt = (int)a.sub_count[blockIdx.x];
+  const int cnt = (int)a.sub_count[qi];
   const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
   const uint64_t* masks = a.masks + mask_base(Q, rbeg, ntile, tile, q);
   const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
@@ -216,39 +222,48 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   const int nch = (cnt + 63) / 64;
   const bool active = inside && px.last_contributor > 0;
 
-  if (a.q_flag[blockIdx.x] == 0u) {
-    uint64_t n_mask = (nch > 0 && active) ? masks[(size_t)(nch - 1) * 64 + lane] : 0ull;
-    uint32_t n_foff = (nch > 0 && active) ? frag_off[(size_t)(nch - 1) * 64 + lane] : 0u;
-    for (int c = nch - 1; c >= 0; --c) {
-      const uint64_t mask = n_mask;
-      const uint32_t foff = n_foff;
-      if (c > 0 && active) {
-        n_mask = masks[(size_t)(c - 1) * 64 + lane];
-        n_foff = frag_off[(size_t)(c - 1) * 64 + lane];
+  if (!staged) {
+    // Each pixel's pairs are one contiguous run in entry order: walk it from
+    // the back, the next kPre pairs in flight while the current ones run.
+    const uint2 rg = (active && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
+    const int f0 = (int)rg.x;
+    int f = f0 + (int)rg.y - 1;
+    float2 fa[kPre], fm[kPre], fd[kPre];
+    float4 fn[kPre];
+    uint32_t fp[kPre];
+    auto fetch = [&](int at) {
+#pragma unroll
+      for (int k = 0; k < kPre; ++k) {
+        if (at - k >= f0) {
+          fa[k] = a.ps.fA[at - k];
+          fp[k] = a.ps.fP[at - k];
+          fn[k] = a.ps.fN[at - k];
+          fm[k] = a.ps.fM[at - k];
+          fd[k] = a.ps.fD[at - k];
+        }
       }
-      int f = (int)foff + __popcll(mask) - 1;  // this lane's last stored pair of the chunk
-      const int f0 = (int)foff;
-      while (f >= f0) {
-        float2 fa[kPre], fm[kPre], fd[kPre];
-        float4 fn[kPre];
-        uint32_t fp[kPre];
+    };
+    fetch(f);
+    while (f >= f0) {
+      float2 ca[kPre], cm[kPre], cd[kPre];
+      float4 cn[kPre];
+      uint32_t cp[kPre];
 #pragma unroll
-        for (int k = 0; k < kPre; ++k) {
-          if (f - k >= f0) {
-            fa[k] = a.ps.fA[f - k];
-            fp[k] = a.ps.fP[f - k];
-            fn[k] = a.ps.fN[f - k];
-            fm[k] = a.ps.fM[f - k];
-            fd[k] = a.ps.fD[f - k];
-          }
-        }
+      for (int k = 0; k < kPre; ++k) {
+        ca[k] = fa[k];
+        cm[k] = fm[k];
+        cd[k] = fd[k];
+        cn[k] = fn[k];
+        cp[k] = fp[k];
+      }
+      const int at = f;
+      f -= kPre;
+      if (f >= f0) fetch(f);  // next batch in flight
 #pragma unroll
-        for (int k = 0; k < kPre; ++k) {
-          if (f - k >= f0 && fp[k] < px.last_contributor) {
-            a.ps.fW[f - k] = ch.step(px, fa[k].x, fa[k].y, fd[k].x, fd[k].y, fp[k], fn[k], fm[k].x, fm[k].y);
-          }
+      for (int k = 0; k < kPre; ++k) {
+        if (at - k >= f0 && cp[k] < px.last_contributor) {
+          a.ps.fW[at - k] = ch.step(px, ca[k].x, ca[k].y, cd[k].x, cd[k].y, cp[k], cn[k], cm[k].x, cm[k].y);
         }
-        f -= kPre;
       }
     }
     return;
@@ -317,7 +332,7 @@ hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s) {
   if (a.vs.T == 0) return hipSuccess;
   if (a.vs.B != 8 && a.vs.B != 16 && a.vs.B != 32) return hipErrorInvalidValue;
   const unsigned nblk = (unsigned)(a.vs.T * quads_per_tile(a.vs.B));
-  hipLaunchKernelGGL(k_blend_bwd, dim3(nblk), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_blend_bwd, dim3(nblk * kSplit), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

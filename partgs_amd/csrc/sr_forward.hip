@@ -594,8 +594,8 @@ hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
 // over the whole GPU; the blend kernels then visit only accepted pairs.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
-  __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
-  __shared__ uint32_t s_pos[64];
+  __shared__ float4 s_q0[64], s_q1[64], s_q2[64];
+  __shared__ float s_rej[64];
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const int lane = threadIdx.x;
   if (a.counters[1] & kErrCapacity) return;
@@ -617,14 +617,11 @@ __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
     const int m = min(64, cnt - e0);
     __syncthreads();
     if (lane < m) {
-      const uint2 v = list[e0 + lane];
-      const float4* r = a.rec + (size_t)kRecQuads * v.x;
+      const float4* r = a.rec + (size_t)kRecQuads * list[e0 + lane].x;
       s_q0[lane] = r[0];
       s_q1[lane] = r[1];
       s_q2[lane] = r[2];
-      s_q3[lane] = r[3];
-      s_q4[lane] = r[4];
-      s_pos[lane] = v.y;
+      s_rej[lane] = r[4].z;
     }
     __syncthreads();
     uint64_t mask = 0;
@@ -632,51 +629,10 @@ __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
     if (inside) {
       for (int j = 0; j < m; ++j) {
         Pair e;
-        if (eval_pair<true>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], s_q4[j].z, e)) mask |= 1ull << j;
+        if (eval_pair<true>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], s_rej[j], e)) mask |= 1ull << j;
       }
     }
-    const size_t slot = mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane;
-    a.masks[slot] = mask;
-    // reserve a contiguous run of the pair store for the chunk, lane by lane
-    const uint32_t cntp = (uint32_t)__popcll(mask);
-    uint32_t incl = cntp;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t v = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += v;
-    }
-    const uint32_t total = __shfl(incl, 63, 64);
-    unsigned long long base = 0;
-    if (lane == 0 && total > 0) base = atomicAdd(a.frag_fill, (unsigned long long)total);
-    base = __shfl(base, 0, 64);
-    const bool fits = total > 0 && base + total <= (unsigned long long)a.frag_cap;
-    const uint32_t my = (uint32_t)base + (incl - cntp);
-    a.frag_off[slot] = fits ? my : kNoFrag;
-    if (total > 0 && !fits && lane == 0) a.q_flag[qi] = 1u;
-    if (fits && cntp > 0) {
-      const float mcoef = kFar / (kFar - kNear);
-      const float dmd_num = kFar * kNear, dmd_den0 = kFar - kNear;
-      uint64_t mm = mask;
-      uint32_t r = my;
-      while (mm != 0ull) {
-        const int j = __builtin_ctzll(mm);
-        mm &= mm - 1ull;
-        Pair e;
-        eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e);
-        const float4 q3 = s_q3[j], q4 = s_q4[j];
-        // the distortion weight m(depth) and its derivative, as the blends form them
-        const float m_d = mcoef * (1.0f - kNear / e.depth);
-        const float dmd_dd = dmd_num / ((dmd_den0 * e.depth) * e.depth);
-        a.ps.fA[r] = make_float2(e.alpha, e.depth);
-        a.ps.fP[r] = s_pos[j];
-        a.ps.fN[r] = q3;
-        a.ps.fM[r] = make_float2(q4.x, q4.y);
-        a.ps.fD[r] = make_float2(m_d, dmd_dd);
-        a.ps.fB[r] = make_float2(e.G, e.sx);
-        a.ps.fC[r] = e.sy;
-        ++r;
-      }
-    }
+    a.masks[mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane] = mask;
   }
 }
 
@@ -684,6 +640,187 @@ hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s) {
   if (max_items == 0) return hipSuccess;
   const unsigned grid = (unsigned)(max_items < 65536 ? max_items : 65536);
   hipLaunchKernelGGL(k_accept, dim3(grid), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+// Pair-store offsets: one wave per quadrant, one lane per pixel.  Each
+// pixel gets one contiguous run of the store for all its accepted pairs (in
+// entry order, across every chunk), so the per-pixel blend chains read one
+// stream; frag_off[chunk, pixel] records where each chunk's pairs begin.  A
+// quadrant whose pairs do not fit is flagged and left to the staged path.
+__global__ __launch_bounds__(64) void k_pair_offsets(AcceptArgs a) {
+  const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
+  const int lane = threadIdx.x;
+  if (a.counters[1] & kErrCapacity) return;
+  const int qi = blockIdx.x;
+  const int tile = qi / Q, q = qi % Q;
+  const int cnt = (int)a.sub_count[qi];
+  const int nch = (cnt + 63) / 64;
+  const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
+  const size_t base = mask_base(Q, rbeg, ntile, tile, q) + lane;
+  constexpr int kU = 8;  // mask loads in flight
+  uint32_t tot = 0;
+  for (int c0 = 0; c0 < nch; c0 += kU) {
+    uint64_t mk[kU];
+#pragma unroll
+    for (int k = 0; k < kU; ++k) mk[k] = (c0 + k < nch) ? a.masks[base + (size_t)(c0 + k) * 64] : 0ull;
+#pragma unroll
+    for (int k = 0; k < kU; ++k) tot += (uint32_t)__popcll(mk[k]);
+  }
+  uint32_t incl = tot;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  unsigned long long qbase = 0;
+  if (lane == 0 && total > 0) qbase = atomicAdd(a.frag_fill, (unsigned long long)total);
+  qbase = __shfl(qbase, 0, 64);
+  const bool fits = qbase + total <= (unsigned long long)a.frag_cap;
+  // cost of the quadrant for the blend kernels' longest-first order: its
+  // longest pixel chain (the staged path, when it does not fit, its list)
+  uint32_t mx = tot;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off, 64));
+  if (lane == 0) a.q_cost[qi] = fits ? mx : (uint32_t)cnt;
+  if (!fits) {
+    if (lane == 0) a.q_flag[qi] = 1u;
+    a.lane_range[(size_t)qi * 64 + lane] = make_uint2(0u, 0u);
+    for (int c = 0; c < nch; ++c) a.frag_off[base + (size_t)c * 64] = kNoFrag;
+    return;
+  }
+  uint32_t run = (uint32_t)qbase + (incl - tot);
+  a.lane_range[(size_t)qi * 64 + lane] = make_uint2(run, tot);
+  for (int c0 = 0; c0 < nch; c0 += kU) {
+    uint64_t mk[kU];
+#pragma unroll
+    for (int k = 0; k < kU; ++k) mk[k] = (c0 + k < nch) ? a.masks[base + (size_t)(c0 + k) * 64] : 0ull;
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      if (c0 + k < nch) a.frag_off[base + (size_t)(c0 + k) * 64] = run;
+      run += (uint32_t)__popcll(mk[k]);
+    }
+  }
+}
+
+// Orders the quadrants by decreasing cost (64 logarithmic buckets, one
+// 1024-lane workgroup), so the blend kernels dispatch the longest per-pixel
+// chains first and the tail of the launch is short work.  The order within a
+// bucket is arbitrary; each quadrant's result does not depend on it.
+__global__ __launch_bounds__(1024) void k_order_quads(int nq, const uint32_t* cost, uint32_t* order) {
+  __shared__ uint32_t s_cnt[64], s_base[64];
+  const int tid = threadIdx.x;
+  if (tid < 64) s_cnt[tid] = 0;
+  __syncthreads();
+  auto bucket = [](uint32_t c) {  // 63 for the costliest
+    const int lg = c == 0u ? 0 : 32 - __clz(c);  // 1 + floor(log2 c)
+    int b = lg * 2 + (lg >= 2 ? (int)((c >> (lg - 2)) & 1u) : 0);
+    return 63 - min(b, 63);
+  };
+  for (int i = tid; i < nq; i += 1024) atomicAdd(&s_cnt[bucket(cost[i])], 1u);
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (int b = 0; b < 64; ++b) {
+      s_base[b] = run;
+      run += s_cnt[b];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nq; i += 1024) order[atomicAdd(&s_base[bucket(cost[i])], 1u)] = (uint32_t)i;
+}
+
+hipError_t launch_order_quads(int nq, const uint32_t* cost, uint32_t* order, hipStream_t s) {
+  if (nq <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_order_quads, dim3(1), dim3(1024), 0, s, nq, cost, order);
+  return hipGetLastError();
+}
+
+__global__ void k_iota_u32(uint32_t* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
+}
+
+hipError_t launch_iota_u32(uint32_t* p, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_offsets(const AcceptArgs& a, hipStream_t s) {
+  if (a.vs.T == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pair_offsets, dim3((unsigned)(a.vs.T * quads_per_tile(a.vs.B))), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+// Pair-store fill: balanced like the acceptance pass, one wave per
+// (quadrant, chunk); each lane recomputes its accepted pairs of the chunk and
+// writes them at its offsets, with the entry data the blend chains need.
+__global__ __launch_bounds__(64) void k_pair_write(AcceptArgs a) {
+  __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
+  __shared__ uint32_t s_pos[64];
+  const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
+  const int lane = threadIdx.x;
+  if (a.counters[1] & kErrCapacity) return;
+  const float mcoef = kFar / (kFar - kNear);
+  const float dmd_num = kFar * kNear, dmd_den0 = kFar - kNear;
+  const uint32_t nwork = *a.work_count;
+  for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const uint2 item = a.work[w];
+    const int qi = (int)item.x, c = (int)item.y;
+    if (a.q_flag[qi] != 0u) continue;
+    const int tile = qi / Q, q = qi % Q;
+    const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
+    const int lt = tile - (int)vi.tile_base;
+    const int tx = lt % vi.gx, ty = lt / vi.gx;
+    const float pixx = (float)(tx * B + (q % QX) * kQuad + (lane & 7));
+    const float pixy = (float)(ty * B + (q / QX) * kQuad + (lane >> 3));
+    const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
+    const int cnt = (int)a.sub_count[qi];
+    const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
+    const int e0 = c * 64;
+    const int m = min(64, cnt - e0);
+    const size_t slot = mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane;
+    uint64_t mm = a.masks[slot];
+    uint32_t r = mm ? a.frag_off[slot] : 0u;
+    __syncthreads();
+    if (lane < m) {
+      const uint2 v = list[e0 + lane];
+      const float4* rc = a.rec + (size_t)kRecQuads * v.x;
+      s_q0[lane] = rc[0];
+      s_q1[lane] = rc[1];
+      s_q2[lane] = rc[2];
+      s_q3[lane] = rc[3];
+      s_q4[lane] = rc[4];
+      s_pos[lane] = v.y;
+    }
+    __syncthreads();
+    while (mm != 0ull) {
+      const int j = __builtin_ctzll(mm);
+      mm &= mm - 1ull;
+      Pair e;
+      eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e);
+      const float4 q3 = s_q3[j], q4 = s_q4[j];
+      // the distortion weight m(depth) and its derivative, as the blends form them
+      const float m_d = mcoef * (1.0f - kNear / e.depth);
+      const float dmd_dd = dmd_num / ((dmd_den0 * e.depth) * e.depth);
+      a.ps.fA[r] = make_float2(e.alpha, e.depth);
+      a.ps.fP[r] = s_pos[j];
+      a.ps.fN[r] = q3;
+      a.ps.fM[r] = make_float2(q4.x, q4.y);
+      a.ps.fD[r] = make_float2(m_d, dmd_dd);
+      a.ps.fB[r] = make_float2(e.G, e.sx);
+      a.ps.fC[r] = e.sy;
+      ++r;
+    }
+  }
+}
+
+hipError_t launch_pair_write(const AcceptArgs& a, size_t max_items, hipStream_t s) {
+  if (max_items == 0) return hipSuccess;
+  const unsigned grid = (unsigned)(max_items < 65536 ? max_items : 65536);
+  hipLaunchKernelGGL(k_pair_write, dim3(grid), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
@@ -702,17 +839,23 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   __shared__ uint32_t s_pos[64];
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   if (a.counters[1] & kErrCapacity) return;
-  const int tile = blockIdx.x / Q, q = blockIdx.x % Q;
+  const int qi = (int)a.q_order[blockIdx.x / kSplit], part = blockIdx.x % kSplit;
+  const int tile = qi / Q, q = qi % Q;
   const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
   const int lt = tile - (int)vi.tile_base;
   const int tx = lt % vi.gx, ty = lt / vi.gx;
   const int lane = threadIdx.x;
-  const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (lane & 7));
-  const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (lane >> 3));
-  const bool inside = pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
+  const bool staged = a.q_flag[qi] != 0u;
+  if (staged && part != 0) return;  // the staged path uses all 64 lanes of part 0
+  // pixel of this lane: all 64 in the staged path, 64 / kSplit per part otherwise
+  const int pix = staged ? lane : part * (64 / kSplit) + lane;
+  const bool lane_ok = staged || lane < 64 / kSplit;
+  const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (pix & 7));
+  const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (pix >> 3));
+  const bool inside = lane_ok && pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
   bool done = !inside;
   const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
-  const int cnt = (int)a.sub_count[blockIdx.x];
+  const int cnt = (int)a.sub_count[qi];
   const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
   const uint64_t* masks = a.masks + mask_base(Q, rbeg, ntile, tile, q);
   const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
@@ -750,45 +893,56 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
     last_contributor = contributor;
     return true;
   };
-  if (a.q_flag[blockIdx.x] == 0u) {
+  if (!staged) {
     // Every pair of the quadrant is in the store, which holds all a step
-    // reads: each lane streams its own pairs, kPre loads in flight, with no
-    // record staging and no barriers.
-    uint64_t n_mask = nch > 0 ? masks[lane] : 0ull;
-    uint32_t n_foff = nch > 0 ? frag_off[lane] : 0u;
-    for (int c = 0; c < nch; ++c) {
-      const uint64_t mask = done ? 0ull : n_mask;
-      uint32_t f = n_foff;
-      if (c + 1 < nch) {
-        n_mask = masks[(size_t)(c + 1) * 64 + lane];
-        n_foff = frag_off[(size_t)(c + 1) * 64 + lane];
-      }
-      int left = __popcll(mask);
-      while (left > 0 && !done) {
-        float2 fa[kPre], fm[kPre];
-        float4 fn[kPre];
-        uint32_t fp[kPre];
-        float fd[kPre];
+    // reads, and each pixel's pairs form one contiguous run in entry order:
+    // every lane streams its own run, the next kPre pairs in flight while
+    // the current ones blend, with no staging and no barriers.
+    // lane_range is written by the pair-offset pass, which runs only when the
+    // view has instances: read it for non-empty quadrants only
+    const uint2 rg = (inside && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
+    uint32_t f = rg.x;
+    int left = (int)rg.y;
+    float2 fa[kPre], fm[kPre];
+    float4 fn[kPre];
+    uint32_t fp[kPre];
+    float fd[kPre];
+    auto fetch = [&](uint32_t at, int n) {
 #pragma unroll
-        for (int k = 0; k < kPre; ++k) {
-          if (k < left) {
-            fa[k] = a.ps.fA[f + k];
-            fp[k] = a.ps.fP[f + k];
-            fn[k] = a.ps.fN[f + k];
-            fm[k] = a.ps.fM[f + k];
-            fd[k] = a.ps.fD[f + k].x;
-          }
+      for (int k = 0; k < kPre; ++k) {
+        if (k < n) {
+          fa[k] = a.ps.fA[at + k];
+          fp[k] = a.ps.fP[at + k];
+          fn[k] = a.ps.fN[at + k];
+          fm[k] = a.ps.fM[at + k];
+          fd[k] = a.ps.fD[at + k].x;
         }
-#pragma unroll
-        for (int k = 0; k < kPre; ++k) {
-          if (k < left && !done) {
-            if (!step(fa[k].x, fa[k].y, fd[k], fp[k], fn[k], fm[k].x, fm[k].y)) done = true;
-          }
-        }
-        f += kPre;
-        left -= kPre;
       }
-      if (__ballot(!done) == 0ull) break;
+    };
+    fetch(f, left);
+    while (left > 0 && !done) {
+      float2 ca[kPre], cm[kPre];
+      float4 cn[kPre];
+      uint32_t cp[kPre];
+      float cd[kPre];
+#pragma unroll
+      for (int k = 0; k < kPre; ++k) {
+        ca[k] = fa[k];
+        cm[k] = fm[k];
+        cn[k] = fn[k];
+        cp[k] = fp[k];
+        cd[k] = fd[k];
+      }
+      const int n = left;
+      f += kPre;
+      left -= kPre;
+      if (left > 0) fetch(f, left);  // next batch in flight
+#pragma unroll
+      for (int k = 0; k < kPre; ++k) {
+        if (k < n && !done) {
+          if (!step(ca[k].x, ca[k].y, cd[k], cp[k], cn[k], cm[k].x, cm[k].y)) done = true;
+        }
+      }
     }
   } else {
     // Some chunk missed the store: stage the records chunk by chunk and
@@ -871,7 +1025,7 @@ hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s) {
   if (a.vs.T == 0) return hipSuccess;
   if (a.vs.B != 8 && a.vs.B != 16 && a.vs.B != 32) return hipErrorInvalidValue;
   const unsigned nblk = (unsigned)(a.vs.T * quads_per_tile(a.vs.B));
-  hipLaunchKernelGGL(k_blend_fwd, dim3(nblk), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_blend_fwd, dim3(nblk * kSplit), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

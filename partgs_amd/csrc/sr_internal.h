@@ -27,6 +27,10 @@ constexpr int kRecQuads = 5;
 // holding only the entries whose screen cull box meets the quadrant.
 constexpr int kQuad = 8;
 __host__ __device__ inline int quads_per_tile(int B) { return (B / kQuad) * (B / kQuad); }
+// The per-pixel blend chains run 64 / kSplit pixels of a quadrant per wave:
+// a lane streams its own pairs, so spreading a heavy quadrant over more
+// waves spreads its memory traffic over more in-flight requests.
+constexpr int kSplit = 4;
 
 // A batch of views rasterised together.  Every view has its own camera and
 // image size; all share the surfels and the tile edge B.  Surfel i seen from
@@ -103,7 +107,8 @@ inline GeomLayout geom_layout(size_t R) {
 // Image state: per-tile binning tables and per-pixel blend state of all views.
 struct ImageLayout {
   int T, N;
-  size_t counters, tile_count, tile_start, tile_fill, ranges, sub_count, q_flag, final_T, n_contrib, total;
+  size_t counters, tile_count, tile_start, tile_fill, ranges, sub_count, q_flag, lane_range, q_cost, q_order,
+      final_T, n_contrib, total;
 };
 inline ImageLayout image_layout(const ViewSet& vs) {
   ImageLayout l{};
@@ -116,7 +121,10 @@ inline ImageLayout image_layout(const ViewSet& vs) {
   l.tile_fill = off; off += align_up((size_t)l.T * 4);
   l.ranges = off; off += align_up((size_t)l.T * 8);
   l.sub_count = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);
-  l.q_flag = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);  // 1: a chunk missed the pair store
+  l.q_flag = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);  // 1: the quadrant missed the pair store
+  l.lane_range = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 64 * 8);  // (first pair, pairs) per pixel
+  l.q_cost = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);   // longest pixel chain
+  l.q_order = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);  // quadrants, longest chain first
   l.final_T = off; off += align_up((size_t)l.N * 3 * 4);
   l.n_contrib = off; off += align_up((size_t)l.N * 2 * 4);
   l.total = off;
@@ -130,18 +138,19 @@ struct BinLayout {
   size_t fA, fP, fN, fM, fD, fB, fC, fW;
   size_t frag_cap;
 };
-// Accepted-pair store: per accepted (pixel, entry) pair the acceptance pass
+// Accepted-pair store: per accepted (pixel, entry) pair the pair-write pass
 // keeps everything the per-pixel blend chains read, so they stream it
 // without staging records:
 //   fA = (alpha, depth)   fP = position of the entry in the tile list
 //   fN = (normal.xyz, r)  fM = (g, b)   fD = (m, dm/ddepth) of the distortion
 //   fB = (G, s.x)         fC = s.y     (G = exp(-rho/2), s in splat coordinates)
 // and fW = (w, dL/dalpha, dL/ddepth, 0) is written by the backward chain pass
-// for the per-pair gradient pass.  A lane's pairs of one chunk are
-// contiguous from frag_off[mask slot], in entry order.  Capacity
-// kFragPerInstance pairs per tile instance; a chunk that does not fit gets
-// frag_off = kNoFrag, and its quadrant (q_flag) is blended by the staged
-// path that recomputes such pairs.
+// for the per-pair gradient pass.  All pairs of one pixel of a quadrant are
+// contiguous and in entry order: lane_range holds (first, count) per pixel,
+// frag_off the first pair of each (chunk, pixel).  Capacity
+// kFragPerInstance pairs per tile instance; a quadrant that does not fit is
+// flagged (q_flag, frag_off = kNoFrag) and blended by the staged path that
+// recomputes its pairs.
 constexpr size_t kFragPerInstance = 24;
 constexpr uint32_t kNoFrag = 0xFFFFFFFFu;
 // Per-pixel acceptance masks: quadrant (t, q) owns mask slots
@@ -258,6 +267,8 @@ struct AcceptArgs {
   const uint32_t* work_count;
   uint64_t* masks;
   uint32_t* frag_off;
+  uint2* lane_range;
+  uint32_t* q_cost;
   PairStore ps;
   uint32_t* q_flag;
   unsigned long long* frag_fill;
@@ -272,6 +283,8 @@ struct BlendArgs {
   const uint32_t* sub_count;
   const uint64_t* masks;
   const uint32_t* frag_off;
+  const uint2* lane_range;
+  const uint32_t* q_order;
   PairStore ps;
   const uint32_t* q_flag;
   const float4* rec;
@@ -293,6 +306,8 @@ struct BlendBwdArgs {
   const uint32_t* sub_count;
   const uint64_t* masks;
   const uint32_t* frag_off;
+  const uint2* lane_range;
+  const uint32_t* q_order;
   PairStore ps;
   const uint32_t* q_flag;
   const uint2* work;  // (quadrant, chunk) items of the acceptance pass
@@ -349,6 +364,10 @@ hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint
 hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s);
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s);
 hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s);
+hipError_t launch_pair_offsets(const AcceptArgs& a, hipStream_t s);
+hipError_t launch_pair_write(const AcceptArgs& a, size_t max_items, hipStream_t s);
+hipError_t launch_order_quads(int nq, const uint32_t* cost, uint32_t* order, hipStream_t s);
+hipError_t launch_iota_u32(uint32_t* p, size_t n, hipStream_t s);
 hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s);
 hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s);
 hipError_t launch_blend_bwd_grad(const BlendBwdArgs& a, size_t max_items, hipStream_t s);
