@@ -167,13 +167,7 @@ struct Pipeline {
 
 PairStore pair_store(char* bin, const BinLayout& bl) {
   PairStore p{};
-  p.fA = (float2*)(bin + bl.fA);
-  p.fP = (uint32_t*)(bin + bl.fP);
-  p.fN = (float4*)(bin + bl.fN);
-  p.fM = (float2*)(bin + bl.fM);
-  p.fD = (float2*)(bin + bl.fD);
-  p.fB = (float2*)(bin + bl.fB);
-  p.fC = (float*)(bin + bl.fC);
+  p.rec = (float4*)(bin + bl.prec);
   p.fW = (float4*)(bin + bl.fW);
   return p;
 }

@@ -228,41 +228,35 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
     const uint2 rg = (active && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
     const int f0 = (int)rg.x;
     int f = f0 + (int)rg.y - 1;
-    float2 fa[kPre], fm[kPre], fd[kPre];
-    float4 fn[kPre];
-    uint32_t fp[kPre];
+    float4 g0[kPre], g1[kPre], g2[kPre];
     auto fetch = [&](int at) {
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
         if (at - k >= f0) {
-          fa[k] = a.ps.fA[at - k];
-          fp[k] = a.ps.fP[at - k];
-          fn[k] = a.ps.fN[at - k];
-          fm[k] = a.ps.fM[at - k];
-          fd[k] = a.ps.fD[at - k];
+          const float4* pr = a.ps.rec + 4 * (size_t)(at - k);
+          g0[k] = pr[0];
+          g1[k] = pr[1];
+          g2[k] = pr[2];
         }
       }
     };
     fetch(f);
     while (f >= f0) {
-      float2 ca[kPre], cm[kPre], cd[kPre];
-      float4 cn[kPre];
-      uint32_t cp[kPre];
+      float4 c0[kPre], c1[kPre], c2[kPre];
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
-        ca[k] = fa[k];
-        cm[k] = fm[k];
-        cd[k] = fd[k];
-        cn[k] = fn[k];
-        cp[k] = fp[k];
+        c0[k] = g0[k];
+        c1[k] = g1[k];
+        c2[k] = g2[k];
       }
       const int at = f;
       f -= kPre;
       if (f >= f0) fetch(f);  // next batch in flight
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
-        if (at - k >= f0 && cp[k] < px.last_contributor) {
-          a.ps.fW[at - k] = ch.step(px, ca[k].x, ca[k].y, cd[k].x, cd[k].y, cp[k], cn[k], cm[k].x, cm[k].y);
+        const uint32_t pos = __float_as_uint(c2[k].z);
+        if (at - k >= f0 && pos < px.last_contributor) {
+          a.ps.fW[at - k] = ch.step(px, c0[k].x, c0[k].y, c0[k].z, c0[k].w, pos, c1[k], c2[k].x, c2[k].y);
         }
       }
     }
@@ -311,7 +305,9 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
         const float4 q2 = s_q2[j];
         Pair e;
         if (foff != kNoFrag) {
-          load_pair(pixx, pixy, s_q0[j], s_q1[j], q2, a.ps.fA[foff + r], a.ps.fB[foff + r], a.ps.fC[foff + r], e);
+          const float4* pr = a.ps.rec + 4 * (size_t)(foff + r);
+          const float4 p0 = pr[0], p2 = pr[2], p3 = pr[3];
+          load_pair(pixx, pixy, s_q0[j], s_q1[j], q2, make_float2(p0.x, p0.y), make_float2(p2.w, p3.x), p3.y, e);
         } else if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], q2, 0.0f, e)) {
           continue;
         }
@@ -399,30 +395,27 @@ __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
     uint32_t f = foff;
     while (mask != 0ull) {
       int jj[kPre];
-      float2 fb[kPre];
-      float fc[kPre];
-      float4 fw[kPre];
-      uint32_t fp[kPre];
+      float4 f2[kPre], f3[kPre], fw[kPre];
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
         jj[k] = -1;
         if (mask != 0ull) {
           jj[k] = __builtin_ctzll(mask);
           mask &= mask - 1ull;
-          fb[k] = a.ps.fB[f + k];
-          fc[k] = a.ps.fC[f + k];
+          const float4* pr = a.ps.rec + 4 * (size_t)(f + k);
+          f2[k] = pr[2];
+          f3[k] = pr[3];
           fw[k] = a.ps.fW[f + k];
-          fp[k] = a.ps.fP[f + k];
         }
       }
       f += kPre;
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
         const int j = jj[k];
-        if (j < 0 || fp[k] >= px.last_contributor) continue;
+        if (j < 0 || __float_as_uint(f2[k].z) >= px.last_contributor) continue;
         Pair e;
         const float4 q1 = s_q1[j], q2 = s_q2[j];
-        load_pair(pixx, pixy, s_q0[j], q1, q2, make_float2(0.0f, 0.0f), fb[k], fc[k], e);
+        load_pair(pixx, pixy, s_q0[j], q1, q2, make_float2(0.0f, 0.0f), make_float2(f2[k].w, f3[k].x), f3[k].y, e);
         pair_grads(px, e, q1, q2.w, fw[k].x, fw[k].y, fw[k].z, pixx, pixy, s_acc, j);
       }
     }

@@ -805,13 +805,11 @@ __global__ __launch_bounds__(64) void k_pair_write(AcceptArgs a) {
       // the distortion weight m(depth) and its derivative, as the blends form them
       const float m_d = mcoef * (1.0f - kNear / e.depth);
       const float dmd_dd = dmd_num / ((dmd_den0 * e.depth) * e.depth);
-      a.ps.fA[r] = make_float2(e.alpha, e.depth);
-      a.ps.fP[r] = s_pos[j];
-      a.ps.fN[r] = q3;
-      a.ps.fM[r] = make_float2(q4.x, q4.y);
-      a.ps.fD[r] = make_float2(m_d, dmd_dd);
-      a.ps.fB[r] = make_float2(e.G, e.sx);
-      a.ps.fC[r] = e.sy;
+      float4* pr = a.ps.rec + 4 * (size_t)r;
+      pr[0] = make_float4(e.alpha, e.depth, m_d, dmd_dd);
+      pr[1] = q3;
+      pr[2] = make_float4(q4.x, q4.y, __uint_as_float(s_pos[j]), e.G);
+      pr[3] = make_float4(e.sx, e.sy, 0.0f, 0.0f);
       ++r;
     }
   }
@@ -903,35 +901,26 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
     const uint2 rg = (inside && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
     uint32_t f = rg.x;
     int left = (int)rg.y;
-    float2 fa[kPre], fm[kPre];
-    float4 fn[kPre];
-    uint32_t fp[kPre];
-    float fd[kPre];
+    float4 f0[kPre], f1[kPre], f2[kPre];
     auto fetch = [&](uint32_t at, int n) {
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
         if (k < n) {
-          fa[k] = a.ps.fA[at + k];
-          fp[k] = a.ps.fP[at + k];
-          fn[k] = a.ps.fN[at + k];
-          fm[k] = a.ps.fM[at + k];
-          fd[k] = a.ps.fD[at + k].x;
+          const float4* pr = a.ps.rec + 4 * (size_t)(at + k);
+          f0[k] = pr[0];
+          f1[k] = pr[1];
+          f2[k] = pr[2];
         }
       }
     };
     fetch(f, left);
     while (left > 0 && !done) {
-      float2 ca[kPre], cm[kPre];
-      float4 cn[kPre];
-      uint32_t cp[kPre];
-      float cd[kPre];
+      float4 c0[kPre], c1[kPre], c2[kPre];
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
-        ca[k] = fa[k];
-        cm[k] = fm[k];
-        cn[k] = fn[k];
-        cp[k] = fp[k];
-        cd[k] = fd[k];
+        c0[k] = f0[k];
+        c1[k] = f1[k];
+        c2[k] = f2[k];
       }
       const int n = left;
       f += kPre;
@@ -940,7 +929,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
         if (k < n && !done) {
-          if (!step(ca[k].x, ca[k].y, cd[k], cp[k], cn[k], cm[k].x, cm[k].y)) done = true;
+          if (!step(c0[k].x, c0[k].y, c0[k].z, __float_as_uint(c2[k].z), c1[k], c2[k].x, c2[k].y)) done = true;
         }
       }
     }
@@ -979,9 +968,9 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
         mask &= mask - 1ull;
         float alpha, depth;
         if (foff != kNoFrag) {
-          const float2 fa = a.ps.fA[f++];
-          alpha = fa.x;
-          depth = fa.y;
+          const float4 p0 = a.ps.rec[4 * (size_t)(f++)];
+          alpha = p0.x;
+          depth = p0.y;
         } else {
           Pair e;
           if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;

@@ -135,17 +135,19 @@ inline ImageLayout image_layout(int W, int H, int B) { return image_layout(singl
 // Binning state: one entry per (surfel, tile) instance.
 struct BinLayout {
   size_t keys, keys_tmp, point_list, sub, masks, work, frag_off, total;
-  size_t fA, fP, fN, fM, fD, fB, fC, fW;
+  size_t prec, fW;
   size_t frag_cap;
 };
 // Accepted-pair store: per accepted (pixel, entry) pair the pair-write pass
-// keeps everything the per-pixel blend chains read, so they stream it
-// without staging records:
-//   fA = (alpha, depth)   fP = position of the entry in the tile list
-//   fN = (normal.xyz, r)  fM = (g, b)   fD = (m, dm/ddepth) of the distortion
-//   fB = (G, s.x)         fC = s.y     (G = exp(-rho/2), s in splat coordinates)
-// and fW = (w, dL/dalpha, dL/ddepth, 0) is written by the backward chain pass
-// for the per-pair gradient pass.  All pairs of one pixel of a quadrant are
+// keeps one aligned 64-byte record with everything the per-pixel blend
+// chains read, so they stream it without staging records:
+//   p0 = (alpha, depth, m, dm/ddepth)   m = distortion weight of the depth
+//   p1 = (normal.xyz, r)
+//   p2 = (g, b, position of the entry in the tile list (bits), G)
+//   p3 = (s.x, s.y, 0, 0)               G = exp(-rho/2), s in splat coordinates
+// A lane's consecutive pairs fill whole lines, so the fill writes no partial
+// lines.  fW = (w, dL/dalpha, dL/ddepth, 0) is written by the backward chain
+// pass for the per-pair gradient pass.  All pairs of one pixel of a quadrant are
 // contiguous and in entry order: lane_range holds (first, count) per pixel,
 // frag_off the first pair of each (chunk, pixel).  Capacity
 // kFragPerInstance pairs per tile instance; a quadrant that does not fit is
@@ -179,13 +181,7 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
   b.frag_off = off; off += align_up(nslots * 4);
   b.frag_cap = n * (pair_store > 0 ? (size_t)pair_store : kFragPerInstance);
   if (b.frag_cap > 0x7fffffffu) b.frag_cap = 0x7fffffffu;
-  b.fA = off; off += align_up(b.frag_cap * 8);
-  b.fP = off; off += align_up(b.frag_cap * 4);
-  b.fN = off; off += align_up(b.frag_cap * 16);
-  b.fM = off; off += align_up(b.frag_cap * 8);
-  b.fD = off; off += align_up(b.frag_cap * 8);
-  b.fB = off; off += align_up(b.frag_cap * 8);
-  b.fC = off; off += align_up(b.frag_cap * 4);
+  b.prec = off; off += align_up(b.frag_cap * 64);
   b.fW = off; off += align_up(b.frag_cap * 16);
   b.total = off;
   return b;
@@ -193,13 +189,7 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
 
 // Device pointers of the accepted-pair store.
 struct PairStore {
-  float2* fA;
-  uint32_t* fP;
-  float4* fN;
-  float2* fM;
-  float2* fD;
-  float2* fB;
-  float* fC;
+  float4* rec;  // 4 per pair
   float4* fW;
 };
 
