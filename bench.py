@@ -31,8 +31,14 @@ def algorithmic_bytes(P_vis, M, L, N, T):
         "tile_sort": 8 * L + 4 * L,
         "blend_fwd": 8 * T + 76 * L + 60 * N,
         "grad_clear": 0,
-        "blend_bwd": 8 * T + 76 * L + 60 * N + 76 * P_vis,
+        # the backward blend's gather and pixel inputs in its per-pixel pass,
+        # its per-surfel gradient output in the per-pair pass
+        "blend_bwd": 8 * T + 76 * L + 60 * N,
+        "blend_bwd_grad": 76 * P_vis,
         "preprocess_bwd": (151 + 12 * M) * P_vis + (48 + 12 * M) * P_vis,
+        # passes with no counterpart in the reference's byte model
+        "accept": 0,
+        "pair_store": 0,
     }
 
 
@@ -230,7 +236,8 @@ def main():
     M = scene.shs.shape[1]
     N = W * H
     T = ((W + args.tile - 1) // args.tile) * ((H + args.tile - 1) // args.tile)
-    per_stage_ms = {k: v[0] / max(v[1], 1) for k, v in stages.items() if v[1] > 0}
+    # per step: a stage may launch several kernels (clears, the pair-store passes)
+    per_stage_ms = {k: v[0] / args.steps for k, v in stages.items() if v[1] > 0}
     kern_ms = sum(v[0] for v in stages.values()) / args.steps
     dominant = max(per_stage_ms, key=lambda k: per_stage_ms[k])
     alg = algorithmic_bytes(P_vis, M, L, N, T)
