@@ -363,7 +363,7 @@ int run_blend_fwd(Pipeline& pl, float* out_color, float* out_others, bool dbg, h
   ba.sub_count = (const uint32_t*)(pl.img + pl.il.sub_count);
   ba.masks = (const uint64_t*)(pl.bin + pl.bl.masks);
   ba.frag_off = (const uint32_t*)(pl.bin + pl.bl.frag_off);
-  ba.lane_range = (const uint2*)(pl.img + pl.il.lane_range);
+  ba.lane_range = (uint2*)(pl.img + pl.il.lane_range);
   ba.q_order = (const uint32_t*)(pl.img + pl.il.q_order);
   ba.ps = pair_store(pl.bin, pl.bl);
   ba.q_flag = (const uint32_t*)(pl.img + pl.il.q_flag);

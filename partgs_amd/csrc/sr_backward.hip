@@ -131,15 +131,18 @@ struct ChainBwd {
 };
 
 // The 18 partial gradients of one pair (backward.cu:380-437) from its
-// weight w, dL/dalpha and dL/ddepth, summed into s_acc[.][j] with LDS atomics.
-SR_DEV void pair_grads(const PixelBwd& p, const Pair& e, const float4& q1, float opa, float w, float dL_dalpha,
-                       float dL_dz, float pixx, float pixy, float (*s_acc)[64], int j) {
-  atomicAdd(&s_acc[15][j], w * p.dLp0);
-  atomicAdd(&s_acc[16][j], w * p.dLp1);
-  atomicAdd(&s_acc[17][j], w * p.dLp2);
-  atomicAdd(&s_acc[11][j], w * p.dLn0);
-  atomicAdd(&s_acc[12][j], w * p.dLn1);
-  atomicAdd(&s_acc[13][j], w * p.dLn2);
+// weight w, dL/dalpha and dL/ddepth and its pixel's dL/dpixel (dLp) and
+// dL/dnormal (dLn), handed to add(slot, value).
+template <class Add>
+SR_DEV void pair_grads(float dLp0, float dLp1, float dLp2, float dLn0, float dLn1, float dLn2, const Pair& e,
+                       const float4& q1, float opa, float w, float dL_dalpha, float dL_dz, float pixx, float pixy,
+                       Add add) {
+  add(15, w * dLp0);
+  add(16, w * dLp1);
+  add(17, w * dLp2);
+  add(11, w * dLn0);
+  add(12, w * dLn1);
+  add(13, w * dLn2);
   const float G = e.G;
   const float dL_dG = opa * dL_dalpha;
   if (e.rho3d <= e.rho2d) {
@@ -152,24 +155,23 @@ SR_DEV void pair_grads(const PixelBwd& p, const Pair& e, const float4& q1, float
     // dL_dk = cross(l, dL_dp), dL_dl = cross(dL_dp, k)
     const float dkx = ly * dpz - lz * dpy, dky = lz * dpx - lx * dpz, dkz = lx * dpy - ly * dpx;
     const float dlx = dpy * kz - dpz * ky, dly = dpz * kx - dpx * kz, dlz = dpx * ky - dpy * kx;
-    atomicAdd(&s_acc[0][j], -dkx);
-    atomicAdd(&s_acc[1][j], -dky);
-    atomicAdd(&s_acc[2][j], -dkz);
-    atomicAdd(&s_acc[3][j], -dlx);
-    atomicAdd(&s_acc[4][j], -dly);
-    atomicAdd(&s_acc[5][j], -dlz);
-    atomicAdd(&s_acc[6][j], (pixx * dkx + pixy * dlx) + dL_dz * sx);
-    atomicAdd(&s_acc[7][j], (pixx * dky + pixy * dly) + dL_dz * sy);
-    atomicAdd(&s_acc[8][j], (pixx * dkz + pixy * dlz) + dL_dz * 1.0f);
+    add(0, -dkx);
+    add(1, -dky);
+    add(2, -dkz);
+    add(3, -dlx);
+    add(4, -dly);
+    add(5, -dlz);
+    add(6, (pixx * dkx + pixy * dlx) + dL_dz * sx);
+    add(7, (pixx * dky + pixy * dly) + dL_dz * sy);
+    add(8, (pixx * dkz + pixy * dlz) + dL_dz * 1.0f);
   } else {
     const float dG_ddelx = (-G * 2.0f) * e.dx;
     const float dG_ddely = (-G * 2.0f) * e.dy;
-    atomicAdd(&s_acc[9][j], dL_dG * dG_ddelx);
-    atomicAdd(&s_acc[10][j], dL_dG * dG_ddely);
-    atomicAdd(&s_acc[8][j], dL_dz);
+    add(9, dL_dG * dG_ddelx);
+    add(10, dL_dG * dG_ddely);
+    add(8, dL_dz);
   }
-  atomicAdd(&s_acc[14][j], G * dL_dalpha);
-  (void)pixx;
+  add(14, G * dL_dalpha);
 }
 
 // Flushes a chunk's per-entry sums: one FP32 atomic per (entry, slot).
@@ -223,8 +225,9 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   const bool active = inside && px.last_contributor > 0;
 
   if (!staged) {
-    // Each pixel's pairs are one contiguous run in entry order: walk it from
-    // the back, the next kPre pairs in flight while the current ones run.
+    // Each pixel's blended pairs are one contiguous run in entry order (the
+    // forward cut the run at the pixel's last contributor): walk it from the
+    // back, the next kPre pairs in flight while the current ones run.
     const uint2 rg = (active && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
     const int f0 = (int)rg.x;
     int f = f0 + (int)rg.y - 1;
@@ -254,8 +257,8 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
       if (f >= f0) fetch(f);  // next batch in flight
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
-        const uint32_t pos = __float_as_uint(c2[k].z);
-        if (at - k >= f0 && pos < px.last_contributor) {
+        if (at - k >= f0) {
+          const uint32_t pos = __float_as_uint(c2[k].z);
           a.ps.fW[at - k] = ch.step(px, c0[k].x, c0[k].y, c0[k].z, c0[k].w, pos, c1[k], c2[k].x, c2[k].y);
         }
       }
@@ -316,7 +319,8 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
         const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
         const float4 q3 = s_q3[j], q4 = s_q4[j];
         const float4 g = ch.step(px, e.alpha, c_d, m_d, dmd_dd, contributor, q3, q4.x, q4.y);
-        pair_grads(px, e, s_q1[j], q2.w, g.x, g.y, g.z, pixx, pixy, s_acc, j);
+        pair_grads(px.dLp0, px.dLp1, px.dLp2, px.dLn0, px.dLn1, px.dLn2, e, s_q1[j], q2.w, g.x, g.y, g.z, pixx,
+                   pixy, [&](int k, float v) { atomicAdd(&s_acc[k][j], v); });
       }
     }
     __syncthreads();
@@ -332,16 +336,50 @@ hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// 64 x 64 bit-matrix transpose across a wave: lane i passes row i (bit j =
+// M[i][j]) and gets column i back (bit j = M[j][i]).  Recursive block swap:
+// at each of the six levels the lanes that differ in bit s exchange the
+// off-diagonal s x s blocks of their 2s x 2s block.
+SR_DEV uint64_t wave_transpose64(uint64_t x, int lane) {
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  {
+    const uint32_t plo = (uint32_t)__shfl_xor((int)lo, 32), phi = (uint32_t)__shfl_xor((int)hi, 32);
+    if (lane & 32)
+      lo = phi;
+    else
+      hi = plo;
+  }
+  auto level = [&](uint32_t v, int sh, uint32_t m) -> uint32_t {
+    const uint32_t y = (uint32_t)__shfl_xor((int)v, sh);
+    return (lane & sh) ? (((y & ~m) >> sh) | (v & ~m)) : ((v & m) | ((y & m) << sh));
+  };
+  lo = level(lo, 16, 0x0000FFFFu);
+  hi = level(hi, 16, 0x0000FFFFu);
+  lo = level(lo, 8, 0x00FF00FFu);
+  hi = level(hi, 8, 0x00FF00FFu);
+  lo = level(lo, 4, 0x0F0F0F0Fu);
+  hi = level(hi, 4, 0x0F0F0F0Fu);
+  lo = level(lo, 2, 0x33333333u);
+  hi = level(hi, 2, 0x33333333u);
+  lo = level(lo, 1, 0x55555555u);
+  hi = level(hi, 1, 0x55555555u);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
 // Backward blend, pass 2 of 2.  The per-pair gradients, balanced over the
 // whole GPU on the acceptance work list: one wave per (quadrant, 64-entry
-// chunk), one lane per pixel.  A lane's pairs are independent here (the
-// pixel recurrences ran in pass 1), so each forms its 18 partial gradients
-// from the stored (G, s) and (w, dL/dalpha, dL/ddepth), sums them per entry
-// in LDS and flushes once per (entry, slot).
+// chunk).  A pair's gradients need no pixel recurrence any more (pass 1
+// stored its w, dL/dalpha and dL/ddepth), so the wave works entry-major:
+// lane j owns entry j of the chunk, keeps its record in registers and sums
+// its 18 partial gradients over the pixels that blended it, in registers;
+// the acceptance masks (one word per pixel) are transposed into one word of
+// pixels per entry, trimmed first to the pairs in front of each pixel's last
+// contributor.  One FP32 atomic per (entry, slot) flushes the sums.
 __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
-  __shared__ float4 s_q0[64], s_q1[64], s_q2[64];
-  __shared__ uint32_t s_id[64];
-  __shared__ float s_acc[kNumG][64];
+  __shared__ uint4 s_pm[64];    // per pixel: trimmed mask (lo, hi), first stored pair of the chunk
+  __shared__ float4 s_pg[64];   // per pixel: dL/dpixel (rgb), dL/dnormal.x
+  __shared__ float2 s_pn[64];   // per pixel: dL/dnormal.yz
+  __shared__ uint32_t s_pos[64];
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const int lane = threadIdx.x;
   if (a.counters[1] & kErrCapacity) return;
@@ -354,8 +392,9 @@ __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
     const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
     const int lt = tile - (int)vi.tile_base;
     const int tx = lt % vi.gx, ty = lt / vi.gx;
-    const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (lane & 7));
-    const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (lane >> 3));
+    const int bx = tx * B + (q % QX) * kQuad, by = ty * B + (q / QX) * kQuad;
+    // lane as pixel (lane & 7, lane >> 3) of the quadrant
+    const uint32_t pxi = (uint32_t)(bx + (lane & 7)), pyi = (uint32_t)(by + (lane >> 3));
     const bool inside = pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
     const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
     const int cnt = (int)a.sub_count[qi];
@@ -363,64 +402,89 @@ __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
     const size_t slot = mask_base(Q, rbeg, ntile, tile, q) + (size_t)c * 64 + lane;
     const int e0 = c * 64;
     const int m = min(64, cnt - e0);
-    const uint64_t mask0 = inside ? a.masks[slot] : 0ull;
-    const uint32_t foff = mask0 ? a.frag_off[slot] : 0u;
-    PixelBwd px{};
-    if (inside && mask0) {
+    uint64_t mask = inside ? a.masks[slot] : 0ull;
+    const uint32_t foff = mask ? a.frag_off[slot] : 0u;
+    uint32_t lc = 0u;
+    float4 pg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float2 pn = make_float2(0.0f, 0.0f);
+    if (mask) {
       const size_t HW = (size_t)vi.H * vi.W, pid = (size_t)vi.W * pyi + pxi, pb = vi.pix_base;
-      px.last_contributor = a.n_contrib[2 * pb + pid];
+      lc = a.n_contrib[2 * pb + pid];
       const float* dL_dpix = a.dL_dpix + 3 * pb;
       const float* dL_dothers = a.dL_dothers + 7 * pb;
-      px.dLp0 = dL_dpix[pid];
-      px.dLp1 = dL_dpix[HW + pid];
-      px.dLp2 = dL_dpix[2 * HW + pid];
-      px.dLn0 = dL_dothers[2 * HW + pid];
-      px.dLn1 = dL_dothers[3 * HW + pid];
-      px.dLn2 = dL_dothers[4 * HW + pid];
+      pg = make_float4(dL_dpix[pid], dL_dpix[HW + pid], dL_dpix[2 * HW + pid], dL_dothers[2 * HW + pid]);
+      pn = make_float2(dL_dothers[3 * HW + pid], dL_dothers[4 * HW + pid]);
     }
-    __syncthreads();
-    if (lane < m) {
-      const uint2 v = list[e0 + lane];
+    // lane as entry e0 + lane of the chunk
+    const bool has = lane < m;
+    const uint2 v = has ? list[e0 + lane] : make_uint2(0u, 0xFFFFFFFFu);
+    float4 q0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q1 = q0, q2 = q0;
+    if (has) {
       const float4* r = a.rec + (size_t)kRecQuads * v.x;
-      s_q0[lane] = r[0];
-      s_q1[lane] = r[1];
-      s_q2[lane] = r[2];
-      s_id[lane] = v.x;
+      q0 = r[0];
+      q1 = r[1];
+      q2 = r[2];
     }
-#pragma unroll
-    for (int k = 0; k < kNumG; ++k) s_acc[k][lane] = 0.0f;
+    __syncthreads();  // the previous item's readers are done
+    s_pos[lane] = v.y;
     __syncthreads();
-    const float pixx = (float)pxi, pixy = (float)pyi;
-    uint64_t mask = mask0;
-    uint32_t f = foff;
-    while (mask != 0ull) {
-      int jj[kPre];
-      float4 f2[kPre], f3[kPre], fw[kPre];
+    if (mask) {
+      // entries in front of the last contributor: positions rise along the
+      // chunk, so they are the first J, found by binary search
+      int J = 0;
+#pragma unroll
+      for (int st = 64; st > 0; st >>= 1)
+        if (J + st <= 64 && s_pos[J + st - 1] < lc) J += st;
+      if (J < 64) mask &= (1ull << J) - 1ull;
+    }
+    s_pm[lane] = make_uint4((uint32_t)mask, (uint32_t)(mask >> 32), foff, 0u);
+    s_pg[lane] = pg;
+    s_pn[lane] = pn;
+    uint64_t col = wave_transpose64(mask, lane);  // bit p: pixel p blended this lane's entry
+    __syncthreads();
+    float acc[kNumG];
+#pragma unroll
+    for (int k = 0; k < kNumG; ++k) acc[k] = 0.0f;
+    const uint64_t below = (1ull << lane) - 1ull;  // this entry's rank within a pixel's mask
+    while (col != 0ull) {
+      int pp[kPre];
+      float4 fw[kPre];
+      float gx[kPre], s0[kPre], s1[kPre];
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
-        jj[k] = -1;
-        if (mask != 0ull) {
-          jj[k] = __builtin_ctzll(mask);
-          mask &= mask - 1ull;
-          const float4* pr = a.ps.rec + 4 * (size_t)(f + k);
-          f2[k] = pr[2];
-          f3[k] = pr[3];
-          fw[k] = a.ps.fW[f + k];
+        pp[k] = -1;
+        if (col != 0ull) {
+          const int p = __builtin_ctzll(col);
+          col &= col - 1ull;
+          pp[k] = p;
+          const uint4 pm = s_pm[p];
+          const uint64_t mp = (uint64_t)pm.x | ((uint64_t)pm.y << 32);
+          const uint32_t f = pm.z + (uint32_t)__popcll(mp & below);
+          const float* pr = reinterpret_cast<const float*>(a.ps.rec + 4 * (size_t)f);
+          gx[k] = pr[11];  // p2.w = G
+          s0[k] = pr[12];  // p3.x, p3.y = s
+          s1[k] = pr[13];
+          fw[k] = a.ps.fW[f];
         }
       }
-      f += kPre;
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
-        const int j = jj[k];
-        if (j < 0 || __float_as_uint(f2[k].z) >= px.last_contributor) continue;
+        const int p = pp[k];
+        if (p < 0) continue;
+        const float4 g4 = s_pg[p];
+        const float2 n2 = s_pn[p];
+        const float pixx = (float)(bx + (p & 7)), pixy = (float)(by + (p >> 3));
         Pair e;
-        const float4 q1 = s_q1[j], q2 = s_q2[j];
-        load_pair(pixx, pixy, s_q0[j], q1, q2, make_float2(0.0f, 0.0f), make_float2(f2[k].w, f3[k].x), f3[k].y, e);
-        pair_grads(px, e, q1, q2.w, fw[k].x, fw[k].y, fw[k].z, pixx, pixy, s_acc, j);
+        load_pair(pixx, pixy, q0, q1, q2, make_float2(0.0f, 0.0f), make_float2(gx[k], s0[k]), s1[k], e);
+        pair_grads(g4.x, g4.y, g4.z, g4.w, n2.x, n2.y, e, q1, q2.w, fw[k].x, fw[k].y, fw[k].z, pixx, pixy,
+                   [&](int s, float val) { acc[s] += val; });
       }
     }
-    __syncthreads();
-    if (lane < m) flush_grads(a, s_acc, s_id[lane], lane);
+    if (has) {
+#pragma unroll
+      for (int k = 0; k < kNumG; ++k)
+        if (acc[k] != 0.0f) atomic_add_f32(grad_slot(a, v.x, k), acc[k]);
+    }
   }
 }
 

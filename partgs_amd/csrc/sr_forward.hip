@@ -914,6 +914,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
       }
     };
     fetch(f, left);
+    int blended = 0;
     while (left > 0 && !done) {
       float4 c0[kPre], c1[kPre], c2[kPre];
 #pragma unroll
@@ -929,10 +930,15 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
         if (k < n && !done) {
-          if (!step(c0[k].x, c0[k].y, c0[k].z, __float_as_uint(c2[k].z), c1[k], c2[k].x, c2[k].y)) done = true;
+          if (step(c0[k].x, c0[k].y, c0[k].z, __float_as_uint(c2[k].z), c1[k], c2[k].x, c2[k].y))
+            ++blended;
+          else
+            done = true;
         }
       }
     }
+    // the backward passes walk only the blended prefix of the run
+    if (inside && cnt > 0) a.lane_range[(size_t)qi * 64 + pix].y = (uint32_t)blended;
   } else {
     // Some chunk missed the store: stage the records chunk by chunk and
     // recompute the pairs of chunks without stored pairs.

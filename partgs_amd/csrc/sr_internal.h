@@ -149,7 +149,10 @@ struct BinLayout {
 // lines.  fW = (w, dL/dalpha, dL/ddepth, 0) is written by the backward chain
 // pass for the per-pair gradient pass.  All pairs of one pixel of a quadrant are
 // contiguous and in entry order: lane_range holds (first, count) per pixel,
-// frag_off the first pair of each (chunk, pixel).  Capacity
+// frag_off the first pair of each (chunk, pixel).  The forward blend
+// rewrites each count to the pairs the pixel actually blended (a prefix of
+// its run: the ones in front of its last contributor), so the backward
+// passes never touch the pairs behind the saturation point.  Capacity
 // kFragPerInstance pairs per tile instance; a quadrant that does not fit is
 // flagged (q_flag, frag_off = kNoFrag) and blended by the staged path that
 // recomputes its pairs.
@@ -273,7 +276,7 @@ struct BlendArgs {
   const uint32_t* sub_count;
   const uint64_t* masks;
   const uint32_t* frag_off;
-  const uint2* lane_range;
+  uint2* lane_range;  // count rewritten to the pairs blended
   const uint32_t* q_order;
   PairStore ps;
   const uint32_t* q_flag;
