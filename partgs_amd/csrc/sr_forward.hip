@@ -392,39 +392,13 @@ hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // Per-tile sort.  Tiles with at most kSmallMax instances are sorted by a
-// 256-lane workgroup entirely in LDS; larger tiles by a 1024-lane workgroup
-// holding up to kLargeCap keys (128 KiB of LDS), beyond which 16K-key chunks
-// are sorted in LDS and merged pairwise in global memory (merge path).
+// 256-lane workgroup entirely in LDS (8 keys per lane); larger tiles by a
+// 1024-lane workgroup holding up to kLargeCap keys (128 KiB of LDS, 16 keys
+// per lane), beyond which 16K-key chunks are sorted in LDS and merged
+// pairwise in global memory (merge path).
 // ---------------------------------------------------------------------------
 constexpr int kSmallMax = 2048;
 constexpr int kLargeCap = 16384;
-
-// Ascending bitonic sort of S (power of two) keys in LDS by NT lanes.
-template <int NT>
-SR_DEV void bitonic_lds(uint64_t* v, int S) {
-  for (int k = 2; k <= S; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < S; i += NT) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const uint64_t x = v[i], y = v[ixj];
-          const bool up = (i & k) == 0;
-          if (up ? (x > y) : (x < y)) {
-            v[i] = y;
-            v[ixj] = x;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
-
-SR_DEV int next_pow2(int n) {
-  int s = 1;
-  while (s < n) s <<= 1;
-  return s;
-}
 
 // Number of elements of A among the first k of merge(A, B) (keys unique).
 SR_DEV int co_rank(const uint64_t* A, int na, const uint64_t* Bv, int nb, int k) {
@@ -435,6 +409,80 @@ SR_DEV int co_rank(const uint64_t* A, int na, const uint64_t* Bv, int nb, int k)
     else hi = mid;
   }
   return lo;
+}
+
+// Ascending sort of S keys in LDS by NT lanes, K keys per lane (S a power
+// of two, K <= S <= NT K).  Each lane sorts its run of K in registers with a
+// bitonic network, then log2(S / K) merge-path levels double the sorted runs:
+// at each level a lane finds, by binary search, where its K outputs start in
+// the two runs it merges and merges them in registers.  A level costs a lane
+// about log2(S) + K dependent LDS reads, against the S / NT compare-exchange
+// sweeps per step of an LDS bitonic sort.  Keys must be unique apart from
+// the ~0 padding.
+template <int NT, int K>
+SR_DEV void merge_sort_lds(uint64_t* v, int S) {
+  const int tid = threadIdx.x;
+  const bool act = tid * K < S;
+  uint64_t r[K];
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) r[k] = v[tid * K + k];
+#pragma unroll
+    for (int kk = 2; kk <= K; kk <<= 1) {
+#pragma unroll
+      for (int j = kk >> 1; j > 0; j >>= 1) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          const int l = i ^ j;
+          if (l > i) {
+            const uint64_t x = r[i], y = r[l];
+            if (((i & kk) == 0) ? (x > y) : (x < y)) {
+              r[i] = y;
+              r[l] = x;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[tid * K + k] = r[k];  // the lane's own slots: no hazard
+  }
+  __syncthreads();
+  for (int w = K; w < S; w <<= 1) {
+    if (act) {
+      const int o = tid * K;
+      const int g = o & ~(2 * w - 1);  // first slot of the two runs this lane merges
+      const uint64_t* A = v + g;
+      const uint64_t* Bv = v + g + w;
+      int i = co_rank(A, w, Bv, w, o - g);
+      int j = (o - g) - i;
+      uint64_t av = i < w ? A[i] : ~0ull, bv = j < w ? Bv[j] : ~0ull;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const bool ta = av < bv;  // ties (padding only) from B, as co_rank splits them
+        r[k] = ta ? av : bv;
+        if (ta) {
+          ++i;
+          av = i < w ? A[i] : ~0ull;
+        } else {
+          ++j;
+          bv = j < w ? Bv[j] : ~0ull;
+        }
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[tid * K + k] = r[k];
+    }
+    __syncthreads();
+  }
+}
+
+SR_DEV int next_pow2(int n) {
+  int s = 1;
+  while (s < n) s <<= 1;
+  return s;
 }
 
 // Order-preserving split of a tile's sorted list into one sub-list per 8 x 8
@@ -530,20 +578,20 @@ __global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
     return;
   }
   if (n <= CAP) {
-    const int S = next_pow2(n);
+    const int S = max(next_pow2(n), CAP / NT);
     for (int i = tid; i < S; i += NT) s_keys[i] = i < n ? a.keys[beg + i] : ~0ull;
     __syncthreads();
-    bitonic_lds<NT>(s_keys, S);
+    merge_sort_lds<NT, CAP / NT>(s_keys, S);
     for (int i = tid; i < n; i += NT) a.point_list[beg + i] = (uint32_t)s_keys[i];
   } else {
     uint64_t* src = a.keys + beg;
     uint64_t* dst = a.keys_tmp + beg;
     for (int c0 = 0; c0 < n; c0 += CAP) {
       const int cn = min(CAP, n - c0);
-      const int S = next_pow2(cn);
+      const int S = max(next_pow2(cn), CAP / NT);
       for (int i = tid; i < S; i += NT) s_keys[i] = i < cn ? src[c0 + i] : ~0ull;
       __syncthreads();
-      bitonic_lds<NT>(s_keys, S);
+      merge_sort_lds<NT, CAP / NT>(s_keys, S);
       for (int i = tid; i < cn; i += NT) src[c0 + i] = s_keys[i];
       __syncthreads();
     }
