@@ -172,10 +172,11 @@ PairStore pair_store(char* bin, const BinLayout& bl) {
   return p;
 }
 
+// Per-record gradient outputs of the backward preprocess (record r = v P + i):
+// dL_dtransMat 9, dL_dmean2D 3, dL_dopacity 1, dL_dcolors 3 floats each.
 struct RecordGrads {
   float* transMat;
   float* mean2D;
-  float* normal;
   float* opacity;
   float* colors;
 };
@@ -288,13 +289,14 @@ int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, 
   pa.tiles = (uint32_t*)(pl.geom + pl.gl.tiles);
   pa.clamped = (uint8_t*)(pl.geom + pl.gl.clamped);
   pa.cullbox = (short4*)(pl.geom + pl.gl.cullbox);
+  pa.bsum = (uint32_t*)(pl.geom + pl.gl.bsum);
   pa.tile_count = tile_count;
   pa.counters = counters;
   SR_STAGE(0, dbg, st, "preprocess", launch_preprocess(pa, st));
   SR_STAGE(1, dbg, st, "scan",
            launch_scan_tiles(pl.il.T, tile_count, (uint32_t*)(pl.img + pl.il.tile_start),
                              (uint32_t*)(pl.img + pl.il.tile_fill), (uint32_t*)(pl.img + pl.il.ranges), counters,
-                             capacity, st));
+                             capacity, pl.vs.V * ((pl.P + 255) / 256), (uint32_t*)(pl.geom + pl.gl.bsum), st));
   return SR_OK;
 }
 
@@ -311,6 +313,9 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   sc.depths = (const float*)(pl.geom + pl.gl.depths);
   sc.radii = radii;
   sc.counters = counters;
+  sc.bsum = (const uint32_t*)(pl.geom + pl.gl.bsum);
+  sc.roff = (uint32_t*)(pl.geom + pl.gl.roff);
+  sc.inst_rec = (uint32_t*)(bin + bl.inst_rec);
   sc.tile_fill = (uint32_t*)(pl.img + pl.il.tile_fill);
   sc.keys = (uint64_t*)(bin + bl.keys);
   SR_STAGE(2, dbg, st, "scatter", launch_scatter(sc, st));
@@ -322,6 +327,9 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   sa.keys_tmp = (uint64_t*)(bin + bl.keys_tmp);
   sa.point_list = (uint32_t*)(bin + bl.point_list);
   sa.cullbox = (const short4*)(pl.geom + pl.gl.cullbox);
+  sa.inst_rec = (const uint32_t*)(bin + bl.inst_rec);
+  sa.inst_at = (uint32_t*)(bin + bl.inst_at);
+  sa.inst_quads = (uint16_t*)(bin + bl.inst_quads);
   sa.sub = (uint2*)(bin + bl.sub);
   sa.sub_count = (uint32_t*)(pl.img + pl.il.sub_count);
   sa.work = (uint2*)(bin + bl.work);
@@ -401,11 +409,8 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
     ba.n_contrib = (const uint32_t*)(pl.img + pl.il.n_contrib);
     ba.dL_dpix = dL_dcolor;
     ba.dL_dothers = dL_dothers;
-    ba.dL_dtransMat = rg.transMat;
-    ba.dL_dmean2D = rg.mean2D;
-    ba.dL_dnormal = rg.normal;
-    ba.dL_dopacity = rg.opacity;
-    ba.dL_dcolors = rg.colors;
+    ba.inst_at = (const uint32_t*)(pl.bin + pl.bl.inst_at);
+    ba.part = (float*)(pl.bin + pl.bl.part);
     SR_STAGE(6, dbg, st, "blend_bwd", launch_blend_bwd(ba, st));
     SR_STAGE(9, dbg, st, "blend_bwd_grad",
              launch_blend_bwd_grad(ba, work_capacity(pl.capacity, pl.vs.B, pl.il.T), st));
@@ -415,6 +420,11 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
   pb.P = pl.P;
   pb.D = pl.s_D;
   pb.M = g->M;
+  pb.counters = pl.counters();
+  pb.tiles = (const uint32_t*)(pl.geom + pl.gl.tiles);
+  pb.roff = (const uint32_t*)(pl.geom + pl.gl.roff);
+  pb.inst_quads = blend ? (const uint16_t*)(pl.bin + pl.bl.inst_quads) : nullptr;
+  pb.part = blend ? (const float*)(pl.bin + pl.bl.part) : nullptr;
   pb.means3D = g->means3D;
   pb.scales = g->scales;
   pb.rots = g->rotations;
@@ -427,7 +437,6 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
   pb.pm = pl.pm;
   pb.campos = pl.campos;
   pb.dL_dtransMat = rg.transMat;
-  pb.dL_dnormal = rg.normal;
   pb.dL_dcolors = rg.colors;
   pb.dL_dopacity = rg.opacity;
   pb.dL_dopacity_out = gr->dL_dopacity;
@@ -576,20 +585,9 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
   pl.img = (char*)img_buf;
   pl.bin = (char*)bin_buf;
   pl.capacity = R;
-  // With one view the per-record gradient arrays are the outputs themselves.
-  RecordGrads rg{gr->dL_dtransMat, gr->dL_dmeans2D, (float*)((char*)geom_buf + pl.gl.dnormal), gr->dL_dopacity,
-                 gr->dL_dcolors};
-  {
-    StageScope sc(5, st);
-    // the blend accumulates into the per-record arrays; the backward
-    // preprocess writes every other output itself
-    hipError_t e = launch_fill_f32(gr->dL_dmeans2D, 0.0f, 3 * (size_t)P, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dcolors, 0.0f, 3 * (size_t)P, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dopacity, 0.0f, (size_t)P, st);
-    if (e == hipSuccess) e = launch_fill_f32(gr->dL_dtransMat, 0.0f, 9 * (size_t)P, st);
-    if (e == hipSuccess) e = launch_fill_f32(rg.normal, 0.0f, 3 * (size_t)P, st);
-    if (e != hipSuccess) return fail(SR_ERR_HIP, "clear grads: %s", hipGetErrorString(e));
-  }
+  // With one view the per-record gradient arrays are the outputs themselves;
+  // the backward writes every output once, so nothing is cleared first.
+  RecordGrads rg{gr->dL_dtransMat, gr->dL_dmeans2D, gr->dL_dopacity, gr->dL_dcolors};
   return run_backward(pl, g, radii, dL_dcolor, dL_dothers, rg, gr, R > 0 && bin_buf != nullptr, dbg, st);
 }
 
@@ -635,19 +633,9 @@ int sr_backward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t
   if (ws_bytes < bl.total)
     return fail(SR_ERR_INVALID, "workspace holds %zu bytes, needs %zu", ws_bytes, bl.total);
   Pipeline pl = batch_pipeline(v, g, vs, bl, (char*)ws, capacity);
-  const size_t P = (size_t)g->P, R = (size_t)vs.V * P;
   char* w = (char*)ws;
-  RecordGrads rg{(float*)(w + bl.g_transMat), gr->dL_dmeans2D, (float*)(w + bl.geom + pl.gl.dnormal),
-                 (float*)(w + bl.g_opacity), (float*)(w + bl.g_colors)};
-  {
-    StageScope sc(5, st);
-    hipError_t e = launch_fill_f32(rg.transMat, 0.0f, 9 * R, st);
-    if (e == hipSuccess) e = launch_fill_f32(rg.mean2D, 0.0f, 3 * R, st);
-    if (e == hipSuccess) e = launch_fill_f32(rg.normal, 0.0f, 3 * R, st);
-    if (e == hipSuccess) e = launch_fill_f32(rg.opacity, 0.0f, R, st);
-    if (e == hipSuccess) e = launch_fill_f32(rg.colors, 0.0f, 3 * R, st);
-    if (e != hipSuccess) return fail(SR_ERR_HIP, "clear grads: %s", hipGetErrorString(e));
-  }
+  RecordGrads rg{(float*)(w + bl.g_transMat), gr->dL_dmeans2D, (float*)(w + bl.g_opacity),
+                 (float*)(w + bl.g_colors)};
   return run_backward(pl, g, radii, dL_dcolor, dL_dothers, rg, gr, true, false, st);
 }
 

@@ -1,16 +1,21 @@
-// Backward kernels of the surfel rasteriser for gfx950 (CDNA4, wave64).
+// Backward kernels of the surfel rasteriser for gfx950 (CDNA4, wave64),
+// dsr/cuda_rasterizer/backward.cu:143-630.
 //
-//   k_blend_bwd        one wave64 per 8 x 8 quadrant, one lane per pixel,
-//                      surfels visited back to front over the quadrant's
-//                      sub-list (dsr/cuda_rasterizer/backward.cu:143-440).
-//                      Each lane forms its pixel's 18 per-surfel partial
-//                      gradients; they are summed per entry in LDS and each
-//                      chunk of 64 entries is flushed with one FP32 atomic
-//                      per (entry, slot), instead of up to 64 x 13 atomics
-//                      per surfel.
-//   k_preprocess_bwd   per surfel: AABB-centre adjoint, dL/dT -> dL/d(mean,
-//                      scale, rotation), SH backward, and the densification
-//                      statistic written into dL_dmean2D (backward.cu:443-630).
+//   k_blend_bwd        pass 1: per pixel, the back-to-front recurrences over
+//                      the pairs it blended, storing each pair's weight,
+//                      dL/dalpha and dL/ddepth (a quadrant whose pairs missed
+//                      the pair store runs the whole backward here instead)
+//   k_blend_bwd_grad   pass 2: per (quadrant, 64-entry chunk), each entry's 18
+//                      partial gradients summed over the quadrant's pixels
+//   k_preprocess_bwd   per surfel: reduces its records' partial sums over
+//                      their tile instances, then the AABB-centre adjoint,
+//                      dL/dT -> dL/d(mean, scale, rotation), SH backward and
+//                      the densification statistic in dL_dmean2D
+//
+// The partial sums go to one slot per (tile-list position, quadrant) with
+// plain stores and are reduced per record by the backward preprocess: no
+// gradient is accumulated with global atomics, and the result does not
+// depend on scheduling order.
 #include "sr_device.h"
 #include "sr_internal.h"
 #include "sr_pair.h"
@@ -22,14 +27,20 @@ namespace sr {
 constexpr int kNumG = 18;
 constexpr int kPre = 8;  // stored pairs a lane fetches ahead
 
-SR_DEV void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
-
-SR_DEV float* grad_slot(const BlendBwdArgs& a, uint32_t id, int k) {
-  if (k < 9) return a.dL_dtransMat + 9 * (size_t)id + k;
-  if (k < 11) return a.dL_dmean2D + 3 * (size_t)id + (k - 9);
-  if (k < 14) return a.dL_dnormal + 3 * (size_t)id + (k - 11);
-  if (k == 14) return a.dL_dopacity + id;
-  return a.dL_dcolors + 3 * (size_t)id + (k - 15);
+// Stores the 18 partial sums (g(k) for slot k) of the sub-list entry at
+// tile-list position pos of quadrant q as five float4 in its (instance,
+// quadrant) slot; every entry of every sub-list is stored exactly once per
+// backward, zeros included, so the buffer needs no clearing.
+template <class G>
+SR_DEV void store_part(const BlendBwdArgs& a, uint32_t pos, int q, G g) {
+  const int Q = quads_per_tile(a.vs.B);
+  const uint32_t inst = a.inst_at[pos];
+  float4* d = reinterpret_cast<float4*>(a.part + ((size_t)inst * Q + q) * kPartStride);
+  d[0] = make_float4(g(0), g(1), g(2), g(3));
+  d[1] = make_float4(g(4), g(5), g(6), g(7));
+  d[2] = make_float4(g(8), g(9), g(10), g(11));
+  d[3] = make_float4(g(12), g(13), g(14), g(15));
+  d[4] = make_float4(g(16), g(17), 0.0f, 0.0f);
 }
 
 // Per-pixel inputs of the backward blend (backward.cu:168-236).
@@ -174,15 +185,6 @@ SR_DEV void pair_grads(float dLp0, float dLp1, float dLp2, float dLn0, float dLn
   add(14, G * dL_dalpha);
 }
 
-// Flushes a chunk's per-entry sums: one FP32 atomic per (entry, slot).
-SR_DEV void flush_grads(const BlendBwdArgs& a, float (*s_acc)[64], uint32_t id, int lane) {
-#pragma unroll
-  for (int k = 0; k < kNumG; ++k) {
-    const float v = s_acc[k][lane];
-    if (v != 0.0f) atomic_add_f32(grad_slot(a, id, k), v);
-  }
-}
-
 // Backward blend, pass 1 of 2.  One wave64 per 8 x 8 quadrant; each lane
 // walks, back to front, the pairs its pixel blended (the accepted pairs in
 // front of its last contributor, exactly the reference's), running only the
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   }
 
   // Staged path: per chunk, stage the records, run the recurrences and the
-  // gradients together, sum per entry in LDS and flush.
+  // gradients together, sum per entry in LDS and store the sums.
   const float mcoef = kFar / (kFar - kNear);
   const float dmd_num = kFar * kNear, dmd_den0 = kFar - kNear;
   ChunkRegs nx{};
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
       }
     }
     __syncthreads();
-    if (lane < m) flush_grads(a, s_acc, s_e[lane].x, lane);
+    if (lane < m) store_part(a, rbeg + s_e[lane].y, q, [&](int k) { return s_acc[k][lane]; });
   }
 }
 
@@ -374,7 +376,8 @@ SR_DEV uint64_t wave_transpose64(uint64_t x, int lane) {
 // its 18 partial gradients over the pixels that blended it, in registers;
 // the acceptance masks (one word per pixel) are transposed into one word of
 // pixels per entry, trimmed first to the pairs in front of each pixel's last
-// contributor.  One FP32 atomic per (entry, slot) flushes the sums.
+// contributor.  Lane j stores its sums to entry j's (position, quadrant)
+// slot of the partial-gradient buffer.
 __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
   __shared__ uint4 s_pm[64];    // per pixel: trimmed mask (lo, hi), first stored pair of the chunk
   __shared__ float4 s_pg[64];   // per pixel: dL/dpixel (rgb), dL/dnormal.x
@@ -480,11 +483,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
                    [&](int s, float val) { acc[s] += val; });
       }
     }
-    if (has) {
-#pragma unroll
-      for (int k = 0; k < kNumG; ++k)
-        if (acc[k] != 0.0f) atomic_add_f32(grad_slot(a, v.x, k), acc[k]);
-    }
+    if (has) store_part(a, rbeg + v.y, q, [&](int k) { return acc[k]; });
   }
 }
 
@@ -596,8 +595,7 @@ SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, 
 // surfel's dL_dsh and returns the gradient of the mean through the view
 // direction.
 SR_DEV F3 sh_grad(const PreprocessBwdArgs& a, bool first, int idx, size_t r, const float* campos, float m0,
-                  float m1, float m2) {
-  F3 dRGB = f3(a.dL_dcolors[3 * r], a.dL_dcolors[3 * r + 1], a.dL_dcolors[3 * r + 2]);
+                  float m1, float m2, F3 dRGB) {
   const uchar4 cl = reinterpret_cast<const uchar4*>(a.clamped)[r];
   if (cl.x) dRGB.x = 0.0f;
   if (cl.y) dRGB.y = 0.0f;
@@ -613,11 +611,40 @@ struct SurfelGrad {
   float m[3], sc[2], q[4];
 };
 
+// Record r's 18 partial gradients (the slots of the backward blend) summed
+// over the quadrants of each of its instances (a contiguous run), in
+// instance order: a fixed order, so the sums are the same on every run.
+SR_DEV void reduce_record(const PreprocessBwdArgs& a, size_t r, float (&gs)[18]) {
+#pragma unroll
+  for (int k = 0; k < 18; ++k) gs[k] = 0.0f;
+  const uint32_t n = a.tiles[r];
+  if (n == 0 || a.part == nullptr || (a.counters[1] & kErrCapacity)) return;
+  const int Q = quads_per_tile(a.vs.B);
+  const uint32_t i0 = a.roff[r];
+  for (uint32_t i = i0; i < i0 + n; ++i) {
+    uint32_t m = a.inst_quads[i];
+    while (m != 0u) {
+      const int q = __builtin_ctz(m);
+      m &= m - 1u;
+      const float4* p = reinterpret_cast<const float4*>(a.part + ((size_t)i * Q + q) * kPartStride);
+      const float4 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4];
+      gs[0] += p0.x; gs[1] += p0.y; gs[2] += p0.z; gs[3] += p0.w;
+      gs[4] += p1.x; gs[5] += p1.y; gs[6] += p1.z; gs[7] += p1.w;
+      gs[8] += p2.x; gs[9] += p2.y; gs[10] += p2.z; gs[11] += p2.w;
+      gs[12] += p3.x; gs[13] += p3.y; gs[14] += p3.z; gs[15] += p3.w;
+      gs[16] += p4.x; gs[17] += p4.y;
+    }
+  }
+}
+
 // Backward preprocess of surfel idx in view v (record r),
-// dsr/cuda_rasterizer/backward.cu:443-630.  Adds the view's contributions to
-// g (and to dL_dsh, owned by this thread); rewrites the record's dL_dtransMat
-// (precomputed T) and dL_dmean2D (the densification statistic) in place.
-SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size_t r, bool first, SurfelGrad& g) {
+// dsr/cuda_rasterizer/backward.cu:443-630, from the record's reduced blend
+// gradients gs.  Adds the view's contributions to g (and to dL_dsh, owned by
+// this thread) and writes the record's outputs: dL_dtransMat (with the
+// AABB-centre term for a precomputed T), dL_dmean2D (the densification
+// statistic), dL_dopacity and dL_dcolors.
+SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size_t r, bool first, SurfelGrad& g,
+                                const float (&gs)[18]) {
   const ViewInfo& vi = a.vs.v[v];
   const float tanx = vi.tanx, tany = vi.tany;
   const float focal_x = (float)vi.W / (2.0f * tanx);
@@ -672,9 +699,10 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
     nrm = f3((vm[0] * Rc2.x + vm[4] * Rc2.y) + vm[8] * Rc2.z, (vm[1] * Rc2.x + vm[5] * Rc2.y) + vm[9] * Rc2.z,
              (vm[2] * Rc2.x + vm[6] * Rc2.y) + vm[10] * Rc2.z);
   }
-  F3 dT0 = f3(gT[0], gT[1], gT[2]), dT1 = f3(gT[3], gT[4], gT[5]), dT2 = f3(gT[6], gT[7], gT[8]);
+  F3 dT0 = f3(gs[0], gs[1], gs[2]), dT1 = f3(gs[3], gs[4], gs[5]), dT2 = f3(gs[6], gs[7], gs[8]);
   float* gm2 = a.dL_dmean2D + 3 * r;
-  const float gmx = gm2[0], gmy = gm2[1];
+  const float gmx = gs[9], gmy = gs[10];
+  bool updated = false;
   if (gmx != 0.0f || gmy != 0.0f) {
     const F3 tv = f3(9.0f, 9.0f, -1.0f);
     const float d = (tv.x * (T2.x * T2.x) + tv.y * (T2.y * T2.y)) + tv.z * (T2.z * T2.z);
@@ -691,12 +719,22 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
     dT0 = dT0 + dL_dT0;
     dT1 = dT1 + dL_dT1;
     dT2 = dT2 + dL_dT3;
-    if (precomp) {
-      gT[0] = dT0.x; gT[1] = dT0.y; gT[2] = dT0.z;
-      gT[3] = dT1.x; gT[4] = dT1.y; gT[5] = dT1.z;
-      gT[6] = dT2.x; gT[7] = dT2.y; gT[8] = dT2.z;
-    }
+    updated = precomp;
   }
+  // dL_dtransMat: the blend's sums, with the AABB-centre term only for a
+  // precomputed T
+  if (updated) {
+    gT[0] = dT0.x; gT[1] = dT0.y; gT[2] = dT0.z;
+    gT[3] = dT1.x; gT[4] = dT1.y; gT[5] = dT1.z;
+    gT[6] = dT2.x; gT[7] = dT2.y; gT[8] = dT2.z;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) gT[k] = gs[k];
+  }
+  a.dL_dopacity[r] = gs[14];
+  a.dL_dcolors[3 * r] = gs[15];
+  a.dL_dcolors[3 * r + 1] = gs[16];
+  a.dL_dcolors[3 * r + 2] = gs[17];
   if (!precomp) {
     float dM[3][3];
 #pragma unroll
@@ -704,7 +742,7 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
 #pragma unroll
       for (int i = 0; i < 3; ++i)
         dM[cc][i] = (P4[0][i] * comp(dT0, cc) + P4[1][i] * comp(dT1, cc)) + P4[2][i] * comp(dT2, cc);
-    const float* gn = a.dL_dnormal + 3 * r;
+    const float gn[3] = {gs[11], gs[12], gs[13]};
     F3 dL_dtn = f3((vm[0] * gn[0] + vm[1] * gn[1]) + vm[2] * gn[2], (vm[4] * gn[0] + vm[5] * gn[1]) + vm[6] * gn[2],
                    (vm[8] * gn[0] + vm[9] * gn[1]) + vm[10] * gn[2]);
     const float pvx = ((vm[0] * m0 + vm[4] * m1) + vm[8] * m2) + vm[12];
@@ -730,12 +768,12 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
     g.sc[0] += (dRS0.x * Rc0.x + dRS0.y * Rc0.y) + dRS0.z * Rc0.z;
     g.sc[1] += (dRS1.x * Rc1.x + dRS1.y * Rc1.y) + dRS1.z * Rc1.z;
     F3 dm3 = f3(dM[2][0], dM[2][1], dM[2][2]);
-    if (a.shs != nullptr) dm3 = dm3 + sh_grad(a, first, idx, r, campos, m0, m1, m2);
+    if (a.shs != nullptr) dm3 = dm3 + sh_grad(a, first, idx, r, campos, m0, m1, m2, f3(gs[15], gs[16], gs[17]));
     g.m[0] += dm3.x;
     g.m[1] += dm3.y;
     g.m[2] += dm3.z;
   } else if (a.shs != nullptr) {
-    const F3 dm3 = sh_grad(a, first, idx, r, campos, m0, m1, m2);
+    const F3 dm3 = sh_grad(a, first, idx, r, campos, m0, m1, m2, f3(gs[15], gs[16], gs[17]));
     g.m[0] += dm3.x;
     g.m[1] += dm3.y;
     g.m[2] += dm3.z;
@@ -743,12 +781,13 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
   const float depth = a.rec[(size_t)kRecQuads * r + 2].x;  // Tw.z = transMat[8]
   gm2[0] = (float)(((double)(gT[2] * depth) * 0.5) * (double)(float)W);
   gm2[1] = (float)(((double)(gT[5] * depth) * 0.5) * (double)(float)H);
+  gm2[2] = 0.0f;
 }
 
 // One thread per surfel, looping over the views; with one view every sum is
 // 0 + x, so the results equal the single-view formulas bit for bit.  Every
-// per-surfel output is written exactly once (zeros for a surfel no view
-// sees), so none of them needs clearing beforehand.
+// per-surfel and per-record output is written exactly once (zeros for a
+// record no view sees), so none of them needs clearing beforehand.
 __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.P) return;
@@ -758,14 +797,23 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
   float op = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
   for (int v = 0; v < a.vs.V; ++v) {
     const size_t r = (size_t)v * a.P + idx;
-    if (!(a.radii[r] > 0)) continue;
-    preprocess_bwd_view(a, v, idx, r, !any, g);
+    if (!(a.radii[r] > 0)) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) a.dL_dtransMat[9 * r + k] = 0.0f;
+      a.dL_dmean2D[3 * r] = a.dL_dmean2D[3 * r + 1] = a.dL_dmean2D[3 * r + 2] = 0.0f;
+      a.dL_dopacity[r] = 0.0f;
+      a.dL_dcolors[3 * r] = a.dL_dcolors[3 * r + 1] = a.dL_dcolors[3 * r + 2] = 0.0f;
+      continue;
+    }
+    float gs[18];
+    reduce_record(a, r, gs);
+    preprocess_bwd_view(a, v, idx, r, !any, g, gs);
     any = true;
     if (sum_rec) {
-      op += a.dL_dopacity[r];
-      c0 += a.dL_dcolors[3 * r];
-      c1 += a.dL_dcolors[3 * r + 1];
-      c2 += a.dL_dcolors[3 * r + 2];
+      op += gs[14];
+      c0 += gs[15];
+      c1 += gs[16];
+      c2 += gs[17];
     }
   }
   if (!any && a.shs != nullptr) {

@@ -252,10 +252,32 @@ SR_DEV bool preprocess_one(const PreprocessArgs& a, int v, int idx, size_t r, in
 // keeps hot tiles (thousands of instances) off a single global counter.
 constexpr int kHistTiles = 4096;
 
+// Exclusive scan of one value per lane over a 256-lane workgroup (four
+// waves); every lane must call it.  Returns the lane's exclusive prefix and
+// the workgroup total.
+SR_DEV uint32_t block_scan256(uint32_t x, uint32_t* s_w, uint32_t& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t incl = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)incl, d);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();
+  uint32_t off = 0;
+  for (int w = 0; w < wave; ++w) off += s_w[w];
+  total = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+  return off + incl - x;
+}
+
 // Grid: (surfel blocks, views); a workgroup's surfels all belong to view
-// blockIdx.y, so its tile histogram is that view's.
+// blockIdx.y, so its tile histogram is that view's.  Each workgroup also
+// stores its records' instance total in bsum (view-major), which the tile
+// scan turns into the first instance index of every workgroup.
 __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
   __shared__ uint32_t s_hist[kHistTiles];
+  __shared__ uint32_t s_wsum[4];
   const int v = blockIdx.y;
   const int gx = a.vs.v[v].gx;
   const int T = gx * a.vs.v[v].gy;
@@ -266,7 +288,11 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
   const bool ok = idx < a.P && preprocess_one(a, v, idx, (size_t)v * a.P + idx, x0, y0, x1, y1);
-  if (lds) __syncthreads();
+  {
+    uint32_t total = 0;
+    (void)block_scan256(ok ? (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0) : 0u, s_wsum, total);
+    if (threadIdx.x == 0) a.bsum[(size_t)v * gridDim.x + blockIdx.x] = total;
+  }
   if (ok) {
     for (int ty = y0; ty < y1; ++ty)
       for (int tx = x0; tx < x1; ++tx) {
@@ -289,12 +315,14 @@ hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Exclusive scan of the tile histogram by one 1024-lane workgroup.
+// Exclusive scan of the tile histogram by one 1024-lane workgroup; then the
+// same for the per-workgroup instance totals of the preprocess (in place),
+// which numbers every record's instances in record order.
 constexpr int kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32_t* count,
                                                               uint32_t* start, uint32_t* fill,
                                                               uint32_t* ranges, uint32_t* counters,
-                                                              uint32_t capacity) {
+                                                              uint32_t capacity, int nb, uint32_t* bsum) {
   __shared__ uint32_t s_part[kScanThreads];
   const int tid = threadIdx.x;
   const int per = (T + kScanThreads - 1) / kScanThreads;
@@ -325,12 +353,32 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
     counters[0] = s_part[kScanThreads - 1];
     if (s_part[kScanThreads - 1] > capacity) atomicOr(&counters[1], kErrCapacity);
   }
+  __syncthreads();
+  const int pb = (nb + kScanThreads - 1) / kScanThreads;
+  const int b2 = tid * pb, e2 = min(nb, b2 + pb);
+  uint32_t loc = 0;
+  for (int i = b2; i < e2; ++i) loc += bsum[i];
+  s_part[tid] = loc;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const uint32_t v = tid >= off ? s_part[tid - off] : 0u;
+    __syncthreads();
+    s_part[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run2 = tid == 0 ? 0u : s_part[tid - 1];
+  for (int i = b2; i < e2; ++i) {
+    const uint32_t c = bsum[i];
+    bsum[i] = run2;
+    run2 += c;
+  }
 }
 
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
-                             uint32_t* ranges, uint32_t* counters, uint32_t capacity, hipStream_t s) {
+                             uint32_t* ranges, uint32_t* counters, uint32_t capacity, int nb, uint32_t* bsum,
+                             hipStream_t s) {
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, T, count, start, fill, ranges,
-                     counters, capacity);
+                     counters, capacity, nb, bsum);
   return hipGetLastError();
 }
 
@@ -338,9 +386,14 @@ hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint
 // buckets.  With few enough tiles, each workgroup first counts its instances
 // per tile in LDS, reserves one contiguous block per tile with a single
 // global atomic, then hands out slots inside the block with LDS atomics.
+// Instances are numbered in record order (record r's k-th tile, row-major
+// in its rectangle, is instance roff[r] + k) and the key's low word is the
+// instance number, which orders equal depths by record exactly as the
+// reference's (tile | depth, record) sort does; inst_rec maps it back.
 __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   __shared__ uint32_t s_cnt[kHistTiles];
   __shared__ uint32_t s_base[kHistTiles];
+  __shared__ uint32_t s_wsum[4];
   if (a.counters[1] & kErrCapacity) return;
   const int v = blockIdx.y;
   const int B = a.vs.B, gx = a.vs.v[v].gx, gy = a.vs.v[v].gy;
@@ -350,18 +403,30 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const size_t r = (size_t)v * a.P + idx;
   int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
-  uint64_t lo = 0;
+  uint64_t hi = 0;
   const bool ok = idx < a.P && a.radii[r] > 0;
   if (ok) {
     const float4 q2 = a.rec[(size_t)kRecQuads * r + 2];
     get_rect(q2.y, q2.z, a.radii[r], B, gx, gy, x0, y0, x1, y1);
-    lo = ((uint64_t)__float_as_uint(a.depths[r]) << 32) | (uint32_t)r;
+    hi = (uint64_t)__float_as_uint(a.depths[r]) << 32;
+  }
+  uint32_t total = 0;
+  const uint32_t inst0 =
+      a.bsum[(size_t)v * gridDim.x + blockIdx.x] +
+      block_scan256(ok ? (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0) : 0u, s_wsum, total);
+  if (idx < a.P) a.roff[r] = inst0;
+  if (ok) {
+    uint32_t i = inst0;
+    for (int ty = y0; ty < y1; ++ty)
+      for (int tx = x0; tx < x1; ++tx) a.inst_rec[i++] = (uint32_t)r;
   }
   uint64_t* keys = a.keys;
   if (!lds) {
-    if (ok)
+    if (ok) {
+      uint32_t i = inst0;
       for (int ty = y0; ty < y1; ++ty)
-        for (int tx = x0; tx < x1; ++tx) keys[atomicAdd(&tile_fill[ty * gx + tx], 1u)] = lo;
+        for (int tx = x0; tx < x1; ++tx) keys[atomicAdd(&tile_fill[ty * gx + tx], 1u)] = hi | i++;
+    }
     return;
   }
   for (int t = threadIdx.x; t < T; t += blockDim.x) s_cnt[t] = 0;
@@ -376,12 +441,14 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     s_cnt[t] = 0;
   }
   __syncthreads();
-  if (ok)
+  if (ok) {
+    uint32_t i = inst0;
     for (int ty = y0; ty < y1; ++ty)
       for (int tx = x0; tx < x1; ++tx) {
         const int t = ty * gx + tx;
-        keys[s_base[t] + atomicAdd(&s_cnt[t], 1u)] = lo;
+        keys[s_base[t] + atomicAdd(&s_cnt[t], 1u)] = hi | i++;
       }
+  }
 }
 
 hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s) {
@@ -531,6 +598,7 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
         hitmask |= (uint32_t)hit << q;
       }
     }
+    if (valid) a.inst_quads[a.inst_at[beg + e]] = (uint16_t)hitmask;  // for the gradient reduction
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kMaxQuads; ++q) {
@@ -582,7 +650,11 @@ __global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
     for (int i = tid; i < S; i += NT) s_keys[i] = i < n ? a.keys[beg + i] : ~0ull;
     __syncthreads();
     merge_sort_lds<NT, CAP / NT>(s_keys, S);
-    for (int i = tid; i < n; i += NT) a.point_list[beg + i] = (uint32_t)s_keys[i];
+    for (int i = tid; i < n; i += NT) {
+      const uint32_t inst = (uint32_t)s_keys[i];
+      a.inst_at[beg + i] = inst;
+      a.point_list[beg + i] = a.inst_rec[inst];
+    }
   } else {
     uint64_t* src = a.keys + beg;
     uint64_t* dst = a.keys_tmp + beg;
@@ -619,7 +691,11 @@ __global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
       src = dst;
       dst = t;
     }
-    for (int i = tid; i < n; i += NT) a.point_list[beg + i] = (uint32_t)src[i];
+    for (int i = tid; i < n; i += NT) {
+      const uint32_t inst = (uint32_t)src[i];
+      a.inst_at[beg + i] = inst;
+      a.point_list[beg + i] = a.inst_rec[inst];
+    }
   }
   __syncthreads();
   split_quadrants<NT>(a, tile, beg, n);

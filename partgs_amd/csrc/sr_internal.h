@@ -89,8 +89,11 @@ __device__ inline int view_of_tile(const ViewSet& vs, uint32_t t) {
 
 // Geometry state: one entry per record (surfel, view).
 struct GeomLayout {
-  size_t rec, depths, tiles, clamped, dnormal, cullbox, total;
+  size_t rec, depths, tiles, clamped, roff, cullbox, bsum, total;
 };
+// Workgroups of 256 records, per view, that preprocess and scatter use:
+// at most R / 256 + one partial workgroup per view.
+inline size_t record_groups(size_t R) { return R / 256 + kMaxViews + 1; }
 inline GeomLayout geom_layout(size_t R) {
   GeomLayout g{};
   size_t off = 0;
@@ -98,8 +101,9 @@ inline GeomLayout geom_layout(size_t R) {
   g.depths = off; off += align_up(R * 4);
   g.tiles = off; off += align_up(R * 4);
   g.clamped = off; off += align_up(R * 4);  // 3 flags + pad per record
-  g.dnormal = off; off += align_up(R * 12);  // backward scratch: dL/dnormal
+  g.roff = off; off += align_up(R * 4);     // first instance number of the record
   g.cullbox = off; off += align_up(R * 8);   // short4 inclusive pixel box
+  g.bsum = off; off += align_up(record_groups(R) * 4);  // instances per workgroup, then their scan
   g.total = off;
   return g;
 }
@@ -115,7 +119,8 @@ inline ImageLayout image_layout(const ViewSet& vs) {
   l.T = (int)vs.T;
   l.N = (int)vs.N;
   size_t off = 0;
-  l.counters = off; off += kAlign;  // [0] instances, [1] error flags, [2] work items, [4..5] pair-store fill
+  l.counters = off; off += kAlign;  // [0] instances, [1] error flags, [2] work items,
+                                    // [3] instance slots handed out, [4..5] pair-store fill
   l.tile_count = off; off += align_up((size_t)l.T * 4);
   l.tile_start = off; off += align_up(((size_t)l.T + 1) * 4);
   l.tile_fill = off; off += align_up((size_t)l.T * 4);
@@ -132,12 +137,24 @@ inline ImageLayout image_layout(const ViewSet& vs) {
 }
 inline ImageLayout image_layout(int W, int H, int B) { return image_layout(single_view(W, H, B)); }
 
-// Binning state: one entry per (surfel, tile) instance.
+// Binning state: one entry per (surfel, tile) instance.  Instances are
+// numbered in record order: record r's k-th tile (row-major in its tile
+// rectangle) is instance roff[r] + k.
+//   inst_rec   instance -> record
+//   inst_at    position in the concatenated sorted tile lists -> instance
+//   inst_quads per instance, the quadrants whose sub-lists hold it
+//   part       per (instance, quadrant): the 18 per-surfel partial
+//              gradients the backward blend summed over that quadrant's
+//              pixels (kPartStride floats); the backward preprocess sums a
+//              record's contiguous run of them, so no gradient needs a
+//              global atomic
 struct BinLayout {
   size_t keys, keys_tmp, point_list, sub, masks, work, frag_off, total;
+  size_t inst_rec, inst_at, inst_quads, part;
   size_t prec, fW;
   size_t frag_cap;
 };
+constexpr int kPartStride = 20;  // 18 partial gradients, padded to five float4
 // Accepted-pair store: per accepted (pixel, entry) pair the pair-write pass
 // keeps one aligned 64-byte record with everything the per-pixel blend
 // chains read, so they stream it without staging records:
@@ -182,6 +199,10 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
   b.work = off; off += align_up(work_capacity(L, B, T) * 8);
   const size_t nslots = n * Q + (size_t)kQuad * kQuad * Q * (size_t)T;
   b.frag_off = off; off += align_up(nslots * 4);
+  b.inst_rec = off; off += align_up(n * 4);
+  b.inst_at = off; off += align_up(n * 4);
+  b.inst_quads = off; off += align_up(n * 2);
+  b.part = off; off += align_up(n * Q * kPartStride * 4);
   b.frag_cap = n * (pair_store > 0 ? (size_t)pair_store : kFragPerInstance);
   if (b.frag_cap > 0x7fffffffu) b.frag_cap = 0x7fffffffu;
   b.prec = off; off += align_up(b.frag_cap * 64);
@@ -219,6 +240,7 @@ struct PreprocessArgs {
   uint32_t* tiles;
   uint8_t* clamped;
   short4* cullbox;
+  uint32_t* bsum;
   uint32_t* tile_count;
   uint32_t* counters;
 };
@@ -230,6 +252,9 @@ struct ScatterArgs {
   const float* depths;
   const int* radii;
   const uint32_t* counters;
+  const uint32_t* bsum;
+  uint32_t* roff;
+  uint32_t* inst_rec;
   uint32_t* tile_fill;
   uint64_t* keys;
 };
@@ -242,6 +267,9 @@ struct SortArgs {
   uint64_t* keys_tmp;
   uint32_t* point_list;
   const short4* cullbox;
+  const uint32_t* inst_rec;
+  uint32_t* inst_at;
+  uint16_t* inst_quads;
   uint2* sub;
   uint32_t* sub_count;
   uint2* work;          // (quadrant index, chunk) items for the acceptance pass
@@ -288,9 +316,10 @@ struct BlendArgs {
   float* out_others;
 };
 
-// Per-record gradient arrays (record r = v * P + i) filled by the backward
-// blend: dL_dtransMat 9, dL_dmean2D 3 (x, y used), dL_dnormal 3,
-// dL_dopacity 1 and dL_dcolors 3 floats per record.
+// Backward blend.  Each (instance, quadrant) entry receives its 18
+// partial gradients summed over the quadrant's pixels in part: slots 0..8
+// dL_dtransMat, 9..10 dL_dmean2D.xy, 11..13 dL_dnormal, 14 dL_dopacity,
+// 15..17 dL_dcolors.
 struct BlendBwdArgs {
   ViewSet vs;
   const uint32_t* ranges;
@@ -311,19 +340,23 @@ struct BlendBwdArgs {
   const uint32_t* n_contrib;
   const float* dL_dpix;
   const float* dL_dothers;
-  float* dL_dtransMat;
-  float* dL_dmean2D;
-  float* dL_dnormal;
-  float* dL_dopacity;
-  float* dL_dcolors;
+  const uint32_t* inst_at;
+  float* part;  // partial gradients per (instance, quadrant)
 };
 
-// Backward preprocess over all views of each surfel.  The per-record arrays
-// are the backward blend's; dL_dopacity_out / dL_dcolors_out receive the sums
-// over views (skipped when they alias the per-record arrays, V = 1).
+// Backward preprocess over all views of each surfel.  It first reduces each
+// visible record's partial gradients (part, over the quadrants of its
+// instances) and writes the per-record outputs dL_dtransMat, dL_dmean2D,
+// dL_dopacity and dL_dcolors; dL_dopacity_out / dL_dcolors_out receive the
+// sums over views (skipped when they alias the per-record arrays, V = 1).
 struct PreprocessBwdArgs {
   ViewSet vs;
   int P, D, M;
+  const uint32_t* counters;
+  const uint32_t* tiles;
+  const uint32_t* roff;
+  const uint16_t* inst_quads;
+  const float* part;
   const float* means3D;
   const float* scales;
   const float* rots;
@@ -336,9 +369,8 @@ struct PreprocessBwdArgs {
   const float* pm;
   const float* campos;
   float* dL_dtransMat;
-  const float* dL_dnormal;
-  const float* dL_dcolors;
-  const float* dL_dopacity;
+  float* dL_dcolors;
+  float* dL_dopacity;
   float* dL_dopacity_out;
   float* dL_dcolors_out;
   float* dL_dsh;
@@ -353,7 +385,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s);
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
-                             uint32_t* ranges, uint32_t* counters, uint32_t capacity, hipStream_t s);
+                             uint32_t* ranges, uint32_t* counters, uint32_t capacity, int nb, uint32_t* bsum,
+                             hipStream_t s);
 hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s);
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s);
 hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s);
