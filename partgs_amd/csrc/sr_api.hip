@@ -346,6 +346,7 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   aa.work = sa.work;
   aa.work_count = counters + 2;
   aa.masks = (uint64_t*)(bin + bl.masks);
+  aa.chunk_T = (float*)(bin + bl.chunk_T);
   aa.frag_off = (uint32_t*)(bin + bl.frag_off);
   aa.lane_range = (uint2*)(pl.img + pl.il.lane_range);
   aa.q_cost = (uint32_t*)(pl.img + pl.il.q_cost);

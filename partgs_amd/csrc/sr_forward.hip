@@ -560,8 +560,12 @@ SR_DEV int next_pow2(int n) {
 // the quadrant, so blending it gives the reference's result.  Each non-empty
 // sub-list also appends its 64-entry chunks to the acceptance work list.
 constexpr int kMaxQuads = 16;
-template <int NT>
-SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
+// kRegs: the caller holds the tile's sorted records and instances in
+// registers (rid, rin; lane tid owns entries tid + k NT), as after an LDS
+// sort; otherwise they are read back from point_list and inst_at.
+template <int NT, int K, bool kRegs>
+SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n, const uint32_t (&rid)[K],
+                            const uint32_t (&rin)[K]) {
   constexpr int kWaves = NT / 64;
   __shared__ uint32_t s_wcnt[kMaxQuads][kWaves];
   __shared__ uint32_t s_total[kMaxQuads];
@@ -574,13 +578,28 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
   if (tid < kMaxQuads) s_total[tid] = 0;
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int c = 0; c < n; c += NT) {
+  short4 bxs[K];  // register path: every cull box in flight at once
+  if constexpr (kRegs) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      bxs[k] = (tid + k * NT < n) ? a.cullbox[rid[k]] : make_short4(32767, 32767, -32768, -32768);
+  }
+  const int iters = kRegs ? K : (n + NT - 1) / NT;
+#pragma unroll
+  for (int k = 0; k < iters; ++k) {
+    const int c = k * NT;
+    if (c >= n) break;  // uniform: c depends on k only
     const int e = c + tid;
     const bool valid = e < n;
-    uint32_t id = 0;
+    uint32_t id = 0, inst = 0;
     short4 bx = make_short4(32767, 32767, -32768, -32768);
-    if (valid) {
+    if constexpr (kRegs) {
+      id = rid[k];
+      inst = rin[k];
+      bx = bxs[k];
+    } else if (valid) {
       id = a.point_list[beg + e];
+      inst = a.inst_at[beg + e];
       bx = a.cullbox[id];
     }
     uint32_t hitmask = 0;
@@ -598,7 +617,7 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n) {
         hitmask |= (uint32_t)hit << q;
       }
     }
-    if (valid) a.inst_quads[a.inst_at[beg + e]] = (uint16_t)hitmask;  // for the gradient reduction
+    if (valid) a.inst_quads[inst] = (uint16_t)hitmask;  // for the gradient reduction
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kMaxQuads; ++q) {
@@ -650,10 +669,32 @@ __global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
     for (int i = tid; i < S; i += NT) s_keys[i] = i < n ? a.keys[beg + i] : ~0ull;
     __syncthreads();
     merge_sort_lds<NT, CAP / NT>(s_keys, S);
-    for (int i = tid; i < n; i += NT) {
-      const uint32_t inst = (uint32_t)s_keys[i];
-      a.inst_at[beg + i] = inst;
-      a.point_list[beg + i] = a.inst_rec[inst];
+    // the small-tile workgroup keeps the sorted records in registers for the
+    // split; the 1024-lane one (four waves per SIMD) has no registers to
+    // spare and reads them back
+    constexpr int K = CAP / NT;
+    constexpr bool kRegs = K <= 8;
+    uint32_t rin[K], rid[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {  // every record lookup in flight at once
+      const int i = tid + k * NT;
+      rin[k] = i < n ? (uint32_t)s_keys[i] : 0u;
+      rid[k] = i < n ? a.inst_rec[rin[k]] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = tid + k * NT;
+      if (i < n) {
+        a.inst_at[beg + i] = rin[k];
+        a.point_list[beg + i] = rid[k];
+      }
+    }
+    if constexpr (kRegs) {
+      split_quadrants<NT, K, true>(a, tile, beg, n, rid, rin);
+    } else {
+      __syncthreads();
+      const uint32_t none[1] = {0u};
+      split_quadrants<NT, 1, false>(a, tile, beg, n, none, none);
     }
   } else {
     uint64_t* src = a.keys + beg;
@@ -696,9 +737,10 @@ __global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
       a.inst_at[beg + i] = inst;
       a.point_list[beg + i] = a.inst_rec[inst];
     }
+    __syncthreads();
+    const uint32_t none[1] = {0u};
+    split_quadrants<NT, 1, false>(a, tile, beg, n, none, none);
   }
-  __syncthreads();
-  split_quadrants<NT>(a, tile, beg, n);
 }
 
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
@@ -749,14 +791,20 @@ __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
     }
     __syncthreads();
     uint64_t mask = 0;
+    float tc = 1.0f;  // transmittance across the chunk's accepted pairs
     const float pixx = (float)pxi, pixy = (float)pyi;
     if (inside) {
       for (int j = 0; j < m; ++j) {
         Pair e;
-        if (eval_pair<true>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], s_rej[j], e)) mask |= 1ull << j;
+        if (eval_pair<true>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], s_rej[j], e)) {
+          mask |= 1ull << j;
+          tc *= 1.0f - e.alpha;
+        }
       }
     }
-    a.masks[mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane] = mask;
+    const size_t slot = mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane;
+    a.masks[slot] = mask;
+    a.chunk_T[slot] = tc;
   }
 }
 
@@ -767,11 +815,22 @@ hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Transmittance below which a pixel is certainly saturated.  The blend stops
+// a pixel at its first pair with T (1 - alpha) < 1e-4, T the running product
+// of (1 - alpha) over its accepted pairs in order.  The product over whole
+// chunks is the same product rounded in another order, off by at most a few
+// thousand ulp (one rounding per pair, 2^-24 each): below half the
+// threshold, the pixel stopped in an earlier chunk.
+constexpr float kSaturatedT = 0.5e-4f;
+
 // Pair-store offsets: one wave per quadrant, one lane per pixel.  Each
-// pixel gets one contiguous run of the store for all its accepted pairs (in
+// pixel gets one contiguous run of the store for its accepted pairs (in
 // entry order, across every chunk), so the per-pixel blend chains read one
-// stream; frag_off[chunk, pixel] records where each chunk's pairs begin.  A
-// quadrant whose pairs do not fit is flagged and left to the staged path.
+// stream; frag_off[chunk, pixel] records where each chunk's pairs begin.
+// Chunks behind a pixel's saturation point (by the acceptance pass's chunk
+// transmittances) are cut: their masks are cleared and their pairs never
+// stored, as the blend never reaches them.  A quadrant whose pairs do not
+// fit is flagged and left to the staged path.
 __global__ __launch_bounds__(64) void k_pair_offsets(AcceptArgs a) {
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const int lane = threadIdx.x;
@@ -784,13 +843,30 @@ __global__ __launch_bounds__(64) void k_pair_offsets(AcceptArgs a) {
   const size_t base = mask_base(Q, rbeg, ntile, tile, q) + lane;
   constexpr int kU = 8;  // mask loads in flight
   uint32_t tot = 0;
+  int cut = nch;  // first chunk the pixel cannot reach
+  float trun = 1.0f;
   for (int c0 = 0; c0 < nch; c0 += kU) {
     uint64_t mk[kU];
+    float tk[kU];
 #pragma unroll
-    for (int k = 0; k < kU; ++k) mk[k] = (c0 + k < nch) ? a.masks[base + (size_t)(c0 + k) * 64] : 0ull;
+    for (int k = 0; k < kU; ++k) {
+      const bool in = c0 + k < nch;
+      mk[k] = in ? a.masks[base + (size_t)(c0 + k) * 64] : 0ull;
+      tk[k] = in ? a.chunk_T[base + (size_t)(c0 + k) * 64] : 1.0f;
+    }
 #pragma unroll
-    for (int k = 0; k < kU; ++k) tot += (uint32_t)__popcll(mk[k]);
+    for (int k = 0; k < kU; ++k) {
+      if (c0 + k < cut) {
+        if (trun < kSaturatedT) {
+          cut = c0 + k;
+        } else {
+          tot += (uint32_t)__popcll(mk[k]);
+          trun *= tk[k];
+        }
+      }
+    }
   }
+  for (int c = cut; c < nch; ++c) a.masks[base + (size_t)c * 64] = 0ull;
   uint32_t incl = tot;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -819,7 +895,7 @@ __global__ __launch_bounds__(64) void k_pair_offsets(AcceptArgs a) {
   for (int c0 = 0; c0 < nch; c0 += kU) {
     uint64_t mk[kU];
 #pragma unroll
-    for (int k = 0; k < kU; ++k) mk[k] = (c0 + k < nch) ? a.masks[base + (size_t)(c0 + k) * 64] : 0ull;
+    for (int k = 0; k < kU; ++k) mk[k] = (c0 + k < cut) ? a.masks[base + (size_t)(c0 + k) * 64] : 0ull;
 #pragma unroll
     for (int k = 0; k < kU; ++k) {
       if (c0 + k < nch) a.frag_off[base + (size_t)(c0 + k) * 64] = run;

@@ -149,7 +149,7 @@ inline ImageLayout image_layout(int W, int H, int B) { return image_layout(singl
 //              record's contiguous run of them, so no gradient needs a
 //              global atomic
 struct BinLayout {
-  size_t keys, keys_tmp, point_list, sub, masks, work, frag_off, total;
+  size_t keys, keys_tmp, point_list, sub, masks, chunk_T, work, frag_off, total;
   size_t inst_rec, inst_at, inst_quads, part;
   size_t prec, fW;
   size_t frag_cap;
@@ -196,8 +196,9 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
   b.point_list = off; off += align_up(n * 4);
   b.sub = off; off += align_up(n * Q * 8);  // uint2 (surfel id, position in tile)
   b.masks = off; off += align_up((n * Q + (size_t)kQuad * kQuad * Q * (size_t)T) * 8);
-  b.work = off; off += align_up(work_capacity(L, B, T) * 8);
   const size_t nslots = n * Q + (size_t)kQuad * kQuad * Q * (size_t)T;
+  b.chunk_T = off; off += align_up(nslots * 4);  // per mask slot: product of (1 - alpha) over its pairs
+  b.work = off; off += align_up(work_capacity(L, B, T) * 8);
   b.frag_off = off; off += align_up(nslots * 4);
   b.inst_rec = off; off += align_up(n * 4);
   b.inst_at = off; off += align_up(n * 4);
@@ -287,6 +288,7 @@ struct AcceptArgs {
   const uint2* work;
   const uint32_t* work_count;
   uint64_t* masks;
+  float* chunk_T;
   uint32_t* frag_off;
   uint2* lane_range;
   uint32_t* q_cost;
