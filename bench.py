@@ -48,8 +48,9 @@ def pipeline_bytes(P, M, L, N, T):
 
 
 STAGE_KERNEL = {"preprocess": "k_preprocess", "tile_scan": "k_scan_tiles", "scatter": "k_scatter",
-                "tile_sort": "k_sort_tiles", "blend_fwd": "k_blend_fwd", "blend_bwd": "k_blend_bwd",
-                "preprocess_bwd": "k_preprocess_bwd", "accept": "k_accept"}
+                "tile_sort": "k_sort_large", "blend_fwd": "k_blend_fwd", "blend_bwd": "k_blend_bwd",
+                "preprocess_bwd": "k_preprocess_bwd", "accept": "k_accept", "blend_bwd_grad": "k_blend_bwd_grad",
+                "pair_store": "k_pair_write"}
 
 
 def measured_traffic(stage):

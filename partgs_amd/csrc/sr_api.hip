@@ -296,7 +296,8 @@ int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, 
   SR_STAGE(1, dbg, st, "scan",
            launch_scan_tiles(pl.il.T, tile_count, (uint32_t*)(pl.img + pl.il.tile_start),
                              (uint32_t*)(pl.img + pl.il.tile_fill), (uint32_t*)(pl.img + pl.il.ranges), counters,
-                             capacity, pl.vs.V * ((pl.P + 255) / 256), (uint32_t*)(pl.geom + pl.gl.bsum), st));
+                             capacity, pl.vs.V * ((pl.P + 255) / 256), (uint32_t*)(pl.geom + pl.gl.bsum),
+                             (uint32_t*)(pl.img + pl.il.large), st));
   return SR_OK;
 }
 
@@ -323,6 +324,7 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   sa.vs = pl.vs;
   sa.ranges = (const uint32_t*)(pl.img + pl.il.ranges);
   sa.counters = counters;
+  sa.large = (const uint32_t*)(pl.img + pl.il.large);
   sa.keys = (uint64_t*)(bin + bl.keys);
   sa.keys_tmp = (uint64_t*)(bin + bl.keys_tmp);
   sa.point_list = (uint32_t*)(bin + bl.point_list);

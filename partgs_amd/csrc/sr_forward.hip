@@ -315,14 +315,39 @@ hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Exclusive scan of the tile histogram by one 1024-lane workgroup; then the
-// same for the per-workgroup instance totals of the preprocess (in place),
-// which numbers every record's instances in record order.
+// Tile sort sizes: tiles with at most kSmallMax instances are sorted by a
+// 256-lane workgroup, larger ones by a 1024-lane workgroup holding up to
+// kLargeCap keys in LDS.
+constexpr int kSmallMax = 2048;
+constexpr int kLargeCap = 16384;
+
+// Inclusive Hillis-Steele scan of one value per lane of a 1024-lane
+// workgroup (s: 1024 words of LDS); returns the lane's exclusive prefix.
+SR_DEV uint32_t scan1024_excl(uint32_t* s, uint32_t x) {
+  const int tid = threadIdx.x;
+  __syncthreads();
+  s[tid] = x;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = tid >= off ? s[tid - off] : 0u;
+    __syncthreads();
+    s[tid] += v;
+    __syncthreads();
+  }
+  return tid == 0 ? 0u : s[tid - 1];
+}
+
+// Exclusive scan of the tile histogram by one 1024-lane workgroup, with the
+// list of tiles too large for the small-tile sort (count in counters[3]);
+// then the same scan for the per-workgroup instance totals of the
+// preprocess (in place), which numbers every record's instances in record
+// order.
 constexpr int kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32_t* count,
                                                               uint32_t* start, uint32_t* fill,
                                                               uint32_t* ranges, uint32_t* counters,
-                                                              uint32_t capacity, int nb, uint32_t* bsum) {
+                                                              uint32_t capacity, int nb, uint32_t* bsum,
+                                                              uint32_t* large) {
   __shared__ uint32_t s_part[kScanThreads];
   const int tid = threadIdx.x;
   const int per = (T + kScanThreads - 1) / kScanThreads;
@@ -353,6 +378,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
     counters[0] = s_part[kScanThreads - 1];
     if (s_part[kScanThreads - 1] > capacity) atomicOr(&counters[1], kErrCapacity);
   }
+  {
+    uint32_t nl = 0;
+    for (int i = b; i < e; ++i) nl += count[i] > (uint32_t)kSmallMax ? 1u : 0u;
+    uint32_t at = scan1024_excl(s_part, nl);
+    for (int i = b; i < e; ++i)
+      if (count[i] > (uint32_t)kSmallMax) large[at++] = (uint32_t)i;
+    if (tid == kScanThreads - 1) counters[3] = at;
+  }
   __syncthreads();
   const int pb = (nb + kScanThreads - 1) / kScanThreads;
   const int b2 = tid * pb, e2 = min(nb, b2 + pb);
@@ -376,9 +409,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
 
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
                              uint32_t* ranges, uint32_t* counters, uint32_t capacity, int nb, uint32_t* bsum,
-                             hipStream_t s) {
+                             uint32_t* large, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, T, count, start, fill, ranges,
-                     counters, capacity, nb, bsum);
+                     counters, capacity, nb, bsum, large);
   return hipGetLastError();
 }
 
@@ -464,8 +497,6 @@ hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s) {
 // per lane), beyond which 16K-key chunks are sorted in LDS and merged
 // pairwise in global memory (merge path).
 // ---------------------------------------------------------------------------
-constexpr int kSmallMax = 2048;
-constexpr int kLargeCap = 16384;
 
 // Number of elements of A among the first k of merge(A, B) (keys unique).
 SR_DEV int co_rank(const uint64_t* A, int na, const uint64_t* Bv, int nb, int k) {
@@ -478,22 +509,31 @@ SR_DEV int co_rank(const uint64_t* A, int na, const uint64_t* Bv, int nb, int k)
   return lo;
 }
 
+// LDS slot of key i in the merge sort: one pad slot after every K keys, so
+// the K-strided accesses of a wave (lane t at t K + k) spread over the banks
+// instead of piling onto two of them.
+template <int K>
+SR_DEV int lds_slot(int i) {
+  return i + i / K;
+}
+
 // Ascending sort of S keys in LDS by NT lanes, K keys per lane (S a power
-// of two, K <= S <= NT K).  Each lane sorts its run of K in registers with a
-// bitonic network, then log2(S / K) merge-path levels double the sorted runs:
-// at each level a lane finds, by binary search, where its K outputs start in
-// the two runs it merges and merges them in registers.  A level costs a lane
-// about log2(S) + K dependent LDS reads, against the S / NT compare-exchange
-// sweeps per step of an LDS bitonic sort.  Keys must be unique apart from
-// the ~0 padding.
+// of two, K <= S <= NT K; key i at lds_slot(i)).  Each lane sorts its run of
+// K in registers with a bitonic network, then log2(S / K) merge-path levels
+// double the sorted runs: at each level a lane finds, by binary search,
+// where its K outputs start in the two runs it merges and merges them in
+// registers.  A level costs a lane about log2(S) + K dependent LDS reads,
+// against the S / NT compare-exchange sweeps per step of an LDS bitonic
+// sort.  Keys must be unique apart from the ~0 padding.
 template <int NT, int K>
 SR_DEV void merge_sort_lds(uint64_t* v, int S) {
   const int tid = threadIdx.x;
   const bool act = tid * K < S;
+  auto at = [&](int i) -> uint64_t { return v[lds_slot<K>(i)]; };
   uint64_t r[K];
   if (act) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) r[k] = v[tid * K + k];
+    for (int k = 0; k < K; ++k) r[k] = at(tid * K + k);
 #pragma unroll
     for (int kk = 2; kk <= K; kk <<= 1) {
 #pragma unroll
@@ -512,35 +552,41 @@ SR_DEV void merge_sort_lds(uint64_t* v, int S) {
       }
     }
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[tid * K + k] = r[k];  // the lane's own slots: no hazard
+    for (int k = 0; k < K; ++k) v[lds_slot<K>(tid * K + k)] = r[k];  // the lane's own slots: no hazard
   }
   __syncthreads();
   for (int w = K; w < S; w <<= 1) {
     if (act) {
       const int o = tid * K;
-      const int g = o & ~(2 * w - 1);  // first slot of the two runs this lane merges
-      const uint64_t* A = v + g;
-      const uint64_t* Bv = v + g + w;
-      int i = co_rank(A, w, Bv, w, o - g);
-      int j = (o - g) - i;
-      uint64_t av = i < w ? A[i] : ~0ull, bv = j < w ? Bv[j] : ~0ull;
+      const int g = o & ~(2 * w - 1);  // first key of the two runs this lane merges
+      const int kq = o - g;
+      // co-rank: how many of the first kq merged keys come from run A
+      // (ties from B, matching the merge below)
+      int lo = max(0, kq - w), hi = min(kq, w);
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (at(g + mid) < at(g + w + kq - mid - 1)) lo = mid + 1;
+        else hi = mid;
+      }
+      int i = lo, j = kq - lo;
+      uint64_t av = i < w ? at(g + i) : ~0ull, bv = j < w ? at(g + w + j) : ~0ull;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const bool ta = av < bv;  // ties (padding only) from B, as co_rank splits them
+        const bool ta = av < bv;  // ties (padding only) from B
         r[k] = ta ? av : bv;
         if (ta) {
           ++i;
-          av = i < w ? A[i] : ~0ull;
+          av = i < w ? at(g + i) : ~0ull;
         } else {
           ++j;
-          bv = j < w ? Bv[j] : ~0ull;
+          bv = j < w ? at(g + w + j) : ~0ull;
         }
       }
     }
     __syncthreads();
     if (act) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) v[tid * K + k] = r[k];
+      for (int k = 0; k < K; ++k) v[lds_slot<K>(tid * K + k)] = r[k];
     }
     __syncthreads();
   }
@@ -578,11 +624,18 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n, co
   if (tid < kMaxQuads) s_total[tid] = 0;
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  short4 bxs[K];  // register path: every cull box in flight at once
-  if constexpr (kRegs) {
+  // register path with few keys per lane: every cull box in flight at once;
+  // with more, the next iteration's box is fetched one iteration ahead
+  constexpr bool kBoxes = kRegs && K <= 8;
+  short4 bxs[kBoxes ? K : 1];
+  if constexpr (kBoxes) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
       bxs[k] = (tid + k * NT < n) ? a.cullbox[rid[k]] : make_short4(32767, 32767, -32768, -32768);
+  }
+  short4 bx_next = make_short4(32767, 32767, -32768, -32768);
+  if constexpr (kRegs && !kBoxes) {
+    if (tid < n) bx_next = a.cullbox[rid[0]];
   }
   const int iters = kRegs ? K : (n + NT - 1) / NT;
 #pragma unroll
@@ -593,10 +646,15 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n, co
     const bool valid = e < n;
     uint32_t id = 0, inst = 0;
     short4 bx = make_short4(32767, 32767, -32768, -32768);
-    if constexpr (kRegs) {
+    if constexpr (kBoxes) {
       id = rid[k];
       inst = rin[k];
       bx = bxs[k];
+    } else if constexpr (kRegs) {
+      id = rid[k];
+      inst = rin[k];
+      bx = bx_next;
+      if (k + 1 < K && e + NT < n) bx_next = a.cullbox[rid[k + 1 < K ? k + 1 : k]];
     } else if (valid) {
       id = a.point_list[beg + e];
       inst = a.inst_at[beg + e];
@@ -647,38 +705,29 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n, co
   }
 }
 
-template <int NT, int CAP, bool kLarge>
-__global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
-  __shared__ uint64_t s_keys[CAP];
-  const int tile = blockIdx.x;
-  if (a.counters[1] & kErrCapacity) return;
+// Sorts tile `tile` (n > 0 instances) and splits it into quadrant sub-lists.
+template <int NT, int CAP>
+SR_DEV void sort_tile(const SortArgs& a, int tile, uint64_t* s_keys) {
+  constexpr int K = CAP / NT;  // keys per lane in the LDS sort
   const uint32_t beg = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
   const int n = (int)(end - beg);
   const int tid = threadIdx.x;
-  if (kLarge != (n > kSmallMax)) return;
-  if (n <= 0) {
-    const int Q = quads_per_tile(a.vs.B);
-    if (tid < Q) {
-      a.sub_count[(size_t)tile * Q + tid] = 0;
-      a.q_flag[(size_t)tile * Q + tid] = 0u;
-    }
-    return;
-  }
   if (n <= CAP) {
     const int S = max(next_pow2(n), CAP / NT);
-    for (int i = tid; i < S; i += NT) s_keys[i] = i < n ? a.keys[beg + i] : ~0ull;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {  // all K loads in flight at once
+      const int i = tid + k * NT;
+      if (i < S) s_keys[lds_slot<K>(i)] = i < n ? a.keys[beg + i] : ~0ull;
+    }
     __syncthreads();
     merge_sort_lds<NT, CAP / NT>(s_keys, S);
-    // the small-tile workgroup keeps the sorted records in registers for the
-    // split; the 1024-lane one (four waves per SIMD) has no registers to
-    // spare and reads them back
-    constexpr int K = CAP / NT;
-    constexpr bool kRegs = K <= 8;
+    // the lane's records (every lookup in flight at once), kept in
+    // registers for the split
     uint32_t rin[K], rid[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {  // every record lookup in flight at once
+    for (int k = 0; k < K; ++k) {
       const int i = tid + k * NT;
-      rin[k] = i < n ? (uint32_t)s_keys[i] : 0u;
+      rin[k] = i < n ? (uint32_t)s_keys[lds_slot<K>(i)] : 0u;
       rid[k] = i < n ? a.inst_rec[rin[k]] : 0u;
     }
 #pragma unroll
@@ -689,23 +738,17 @@ __global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
         a.point_list[beg + i] = rid[k];
       }
     }
-    if constexpr (kRegs) {
-      split_quadrants<NT, K, true>(a, tile, beg, n, rid, rin);
-    } else {
-      __syncthreads();
-      const uint32_t none[1] = {0u};
-      split_quadrants<NT, 1, false>(a, tile, beg, n, none, none);
-    }
+    split_quadrants<NT, K, true>(a, tile, beg, n, rid, rin);
   } else {
     uint64_t* src = a.keys + beg;
     uint64_t* dst = a.keys_tmp + beg;
     for (int c0 = 0; c0 < n; c0 += CAP) {
       const int cn = min(CAP, n - c0);
       const int S = max(next_pow2(cn), CAP / NT);
-      for (int i = tid; i < S; i += NT) s_keys[i] = i < cn ? src[c0 + i] : ~0ull;
+      for (int i = tid; i < S; i += NT) s_keys[lds_slot<K>(i)] = i < cn ? src[c0 + i] : ~0ull;
       __syncthreads();
       merge_sort_lds<NT, CAP / NT>(s_keys, S);
-      for (int i = tid; i < cn; i += NT) src[c0 + i] = s_keys[i];
+      for (int i = tid; i < cn; i += NT) src[c0 + i] = s_keys[lds_slot<K>(i)];
       __syncthreads();
     }
     for (int w = CAP; w < n; w <<= 1) {
@@ -743,12 +786,40 @@ __global__ __launch_bounds__(NT) void k_sort_tiles(SortArgs a) {
   }
 }
 
+// Tiles with at most kSmallMax instances (empty ones included): one
+// 256-lane workgroup per tile of the grid.
+__global__ __launch_bounds__(256) void k_sort_small(SortArgs a) {
+  __shared__ uint64_t s_keys[kSmallMax + kSmallMax / (kSmallMax / 256)];
+  const int tile = blockIdx.x;
+  if (a.counters[1] & kErrCapacity) return;
+  const int n = (int)(a.ranges[2 * tile + 1] - a.ranges[2 * tile]);
+  if (n > kSmallMax) return;
+  if (n <= 0) {
+    const int Q = quads_per_tile(a.vs.B);
+    if ((int)threadIdx.x < Q) {
+      a.sub_count[(size_t)tile * Q + threadIdx.x] = 0;
+      a.q_flag[(size_t)tile * Q + threadIdx.x] = 0u;
+    }
+    return;
+  }
+  sort_tile<256, kSmallMax>(a, tile, s_keys);
+}
+
+// Larger tiles: workgroup i sorts entry i of the scan's list of them, so
+// the large tiles take the first workgroups of the launch (dispatched first)
+// and the rest, one per remaining tile of the grid, exit at once.
+__global__ __launch_bounds__(1024) void k_sort_large(SortArgs a) {
+  __shared__ uint64_t s_keys[kLargeCap + kLargeCap / (kLargeCap / 1024)];
+  if (a.counters[1] & kErrCapacity) return;
+  if (blockIdx.x < a.counters[3]) sort_tile<1024, kLargeCap>(a, (int)a.large[blockIdx.x], s_keys);
+}
+
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
   if (a.vs.T == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_sort_tiles<256, kSmallMax, false>), dim3(a.vs.T), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_sort_small, dim3(a.vs.T), dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_sort_tiles<1024, kLargeCap, true>), dim3(a.vs.T), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(k_sort_large, dim3(a.vs.T), dim3(1024), 0, s, a);
   return hipGetLastError();
 }
 

@@ -111,8 +111,8 @@ inline GeomLayout geom_layout(size_t R) {
 // Image state: per-tile binning tables and per-pixel blend state of all views.
 struct ImageLayout {
   int T, N;
-  size_t counters, tile_count, tile_start, tile_fill, ranges, sub_count, q_flag, lane_range, q_cost, q_order,
-      final_T, n_contrib, total;
+  size_t counters, tile_count, tile_start, tile_fill, ranges, large, sub_count, q_flag, lane_range, q_cost,
+      q_order, final_T, n_contrib, total;
 };
 inline ImageLayout image_layout(const ViewSet& vs) {
   ImageLayout l{};
@@ -120,11 +120,12 @@ inline ImageLayout image_layout(const ViewSet& vs) {
   l.N = (int)vs.N;
   size_t off = 0;
   l.counters = off; off += kAlign;  // [0] instances, [1] error flags, [2] work items,
-                                    // [3] instance slots handed out, [4..5] pair-store fill
+                                    // [3] large tiles, [4..5] pair-store fill
   l.tile_count = off; off += align_up((size_t)l.T * 4);
   l.tile_start = off; off += align_up(((size_t)l.T + 1) * 4);
   l.tile_fill = off; off += align_up((size_t)l.T * 4);
   l.ranges = off; off += align_up((size_t)l.T * 8);
+  l.large = off; off += align_up((size_t)l.T * 4);  // tiles for the large-tile sort
   l.sub_count = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);
   l.q_flag = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);  // 1: the quadrant missed the pair store
   l.lane_range = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 64 * 8);  // (first pair, pairs) per pixel
@@ -264,6 +265,7 @@ struct SortArgs {
   ViewSet vs;
   const uint32_t* ranges;
   const uint32_t* counters;
+  const uint32_t* large;  // tiles above the small-tile sort's size, counters[3] of them
   uint64_t* keys;
   uint64_t* keys_tmp;
   uint32_t* point_list;
@@ -388,7 +390,7 @@ hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s);
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
                              uint32_t* ranges, uint32_t* counters, uint32_t capacity, int nb, uint32_t* bsum,
-                             hipStream_t s);
+                             uint32_t* large, hipStream_t s);
 hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s);
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s);
 hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s);
