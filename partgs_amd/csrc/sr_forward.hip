@@ -614,6 +614,7 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n, co
                             const uint32_t (&rin)[K]) {
   constexpr int kWaves = NT / 64;
   __shared__ uint32_t s_wcnt[kMaxQuads][kWaves];
+  __shared__ uint32_t s_woff[kMaxQuads][kWaves];
   __shared__ uint32_t s_total[kMaxQuads];
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -677,21 +678,27 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n, co
     }
     if (valid) a.inst_quads[inst] = (uint16_t)hitmask;  // for the gradient reduction
     __syncthreads();
+    // per quadrant, the waves' first slots: a scan over the waves, one lane
+    // per (quadrant, wave), each quadrant a kWaves-lane segment of one wave
+    if (tid < Q * kWaves) {
+      const int q = tid / kWaves, w = tid % kWaves;
+      const uint32_t c0 = s_wcnt[q][w];
+      uint32_t incl = c0;
+#pragma unroll
+      for (int d = 1; d < kWaves; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, d, kWaves);
+        if (w >= d) incl += y;
+      }
+      const uint32_t t0 = s_total[q];
+      s_woff[q][w] = t0 + incl - c0;
+      if (w == kWaves - 1) s_total[q] = t0 + incl;
+    }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < kMaxQuads; ++q) {
-      if (q < Q && ((hitmask >> q) & 1u)) {
-        uint32_t off = s_total[q] + rank[q];
-        for (int w = 0; w < wave; ++w) off += s_wcnt[q][w];
-        base[(size_t)q * n + off] = make_uint2(id, (uint32_t)e);
-      }
+      if (q < Q && ((hitmask >> q) & 1u))
+        base[(size_t)q * n + s_woff[q][wave] + rank[q]] = make_uint2(id, (uint32_t)e);
     }
-    __syncthreads();
-    if (tid < Q) {
-      uint32_t t = 0;
-      for (int w = 0; w < kWaves; ++w) t += s_wcnt[tid][w];
-      s_total[tid] += t;
-    }
-    __syncthreads();
   }
   if (tid < Q) {
     const uint32_t cnt = s_total[tid];
