@@ -452,7 +452,6 @@ __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
     while (col != 0ull) {
       int pp[kPre];
       float4 fw[kPre];
-      float gx[kPre], s0[kPre], s1[kPre];
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
         pp[k] = -1;
@@ -463,11 +462,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
           const uint4 pm = s_pm[p];
           const uint64_t mp = (uint64_t)pm.x | ((uint64_t)pm.y << 32);
           const uint32_t f = pm.z + (uint32_t)__popcll(mp & below);
-          const float* pr = reinterpret_cast<const float*>(a.ps.rec + 4 * (size_t)f);
-          gx[k] = pr[11];  // p2.w = G
-          s0[k] = pr[12];  // p3.x, p3.y = s
-          s1[k] = pr[13];
-          fw[k] = a.ps.fW[f];
+          fw[k] = a.ps.fW[f];  // (w, dL/dalpha, dL/ddepth)
         }
       }
 #pragma unroll
@@ -477,8 +472,10 @@ __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
         const float4 g4 = s_pg[p];
         const float2 n2 = s_pn[p];
         const float pixx = (float)(bx + (p & 7)), pixy = (float)(by + (p >> 3));
+        // the pair itself (s, G, planes) recomputed exactly as the acceptance
+        // pass evaluated it: cheaper than fetching it from the pair store
         Pair e;
-        load_pair(pixx, pixy, q0, q1, q2, make_float2(0.0f, 0.0f), make_float2(gx[k], s0[k]), s1[k], e);
+        (void)eval_pair<false>(pixx, pixy, q0, q1, q2, 0.0f, e);
         pair_grads(g4.x, g4.y, g4.z, g4.w, n2.x, n2.y, e, q1, q2.w, fw[k].x, fw[k].y, fw[k].z, pixx, pixy,
                    [&](int s, float val) { acc[s] += val; });
       }

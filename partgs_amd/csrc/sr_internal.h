@@ -165,7 +165,7 @@ constexpr int kPartStride = 20;  // 18 partial gradients, padded to five float4
 //   p3 = (s.x, s.y, 0, 0)               G = exp(-rho/2), s in splat coordinates
 // A lane's consecutive pairs fill whole lines, so the fill writes no partial
 // lines.  fW = (w, dL/dalpha, dL/ddepth, 0) is written by the backward chain
-// pass for the per-pair gradient pass.  All pairs of one pixel of a quadrant are
+// pass for the per-pair gradient pass, which recomputes the rest of the pair.  All pairs of one pixel of a quadrant are
 // contiguous and in entry order: lane_range holds (first, count) per pixel,
 // frag_off the first pair of each (chunk, pixel).  The forward blend
 // rewrites each count to the pairs the pixel actually blended (a prefix of
@@ -216,7 +216,7 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
 // Device pointers of the accepted-pair store.
 struct PairStore {
   float4* rec;  // 4 per pair
-  float4* fW;
+  float4* fW;   // 1 per pair
 };
 
 constexpr uint32_t kErrPrefiltered = 1u;
