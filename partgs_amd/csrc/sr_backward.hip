@@ -25,7 +25,8 @@ namespace sr {
 // Partial-gradient slots: 0..8 dL_dtransMat, 9..10 dL_dmean2D.xy,
 // 11..13 dL_dnormal, 14 dL_dopacity, 15..17 dL_dcolors.
 constexpr int kNumG = 18;
-constexpr int kPre = 8;  // stored pairs a lane fetches ahead
+constexpr int kPre = 8;       // pairs a lane fetches ahead in the per-pair pass
+constexpr int kChainPre = 8;  // stored pairs a lane fetches ahead in the chain pass
 
 // Stores the 18 partial sums (g(k) for slot k) of the sub-list entry at
 // tile-list position pos of quadrant q as five float4 in its (instance,
@@ -229,14 +230,14 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   if (!staged) {
     // Each pixel's blended pairs are one contiguous run in entry order (the
     // forward cut the run at the pixel's last contributor): walk it from the
-    // back, the next kPre pairs in flight while the current ones run.
+    // back, the next kChainPre pairs in flight while the current ones run.
     const uint2 rg = (active && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
     const int f0 = (int)rg.x;
     int f = f0 + (int)rg.y - 1;
-    float4 g0[kPre], g1[kPre], g2[kPre];
+    float4 g0[kChainPre], g1[kChainPre], g2[kChainPre];
     auto fetch = [&](int at) {
 #pragma unroll
-      for (int k = 0; k < kPre; ++k) {
+      for (int k = 0; k < kChainPre; ++k) {
         if (at - k >= f0) {
           const float4* pr = a.ps.rec + 4 * (size_t)(at - k);
           g0[k] = pr[0];
@@ -247,18 +248,18 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
     };
     fetch(f);
     while (f >= f0) {
-      float4 c0[kPre], c1[kPre], c2[kPre];
+      float4 c0[kChainPre], c1[kChainPre], c2[kChainPre];
 #pragma unroll
-      for (int k = 0; k < kPre; ++k) {
+      for (int k = 0; k < kChainPre; ++k) {
         c0[k] = g0[k];
         c1[k] = g1[k];
         c2[k] = g2[k];
       }
       const int at = f;
-      f -= kPre;
+      f -= kChainPre;
       if (f >= f0) fetch(f);  // next batch in flight
 #pragma unroll
-      for (int k = 0; k < kPre; ++k) {
+      for (int k = 0; k < kChainPre; ++k) {
         if (at - k >= f0) {
           const uint32_t pos = __float_as_uint(c2[k].z);
           a.ps.fW[at - k] = ch.step(px, c0[k].x, c0[k].y, c0[k].z, c0[k].w, pos, c1[k], c2[k].x, c2[k].y);
