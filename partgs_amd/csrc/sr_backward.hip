@@ -27,6 +27,8 @@ namespace sr {
 constexpr int kNumG = 18;
 constexpr int kPre = 8;       // pairs a lane fetches ahead in the per-pair pass
 constexpr int kChainPre = 8;  // stored pairs a lane fetches ahead in the chain pass
+// waves per quadrant in the chain pass (64 / kSplitBwd pixels each)
+constexpr int kSplitBwd = 8;
 
 // Stores the 18 partial sums (g(k) for slot k) of the sub-list entry at
 // tile-list position pos of quadrant q as five float4 in its (instance,
@@ -201,37 +203,34 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   __shared__ float s_acc[kNumG][64];
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   if (a.counters[1] & kErrCapacity) return;
-  const int qi = (int)a.q_order[blockIdx.x / kSplit], part = blockIdx.x % kSplit;
+  const int qi = (int)a.q_order[blockIdx.x / kSplitBwd], part = blockIdx.x % kSplitBwd;
   const int tile = qi / Q, q = qi % Q;
   const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
   const int lt = tile - (int)vi.tile_base;
   const int tx = lt % vi.gx, ty = lt / vi.gx;
   const int lane = threadIdx.x;
+  const int cnt = (int)a.sub_count[qi];
+  if (cnt == 0) return;  // no surfel reaches the quadrant: no gradient from it
   const bool staged = a.q_flag[qi] != 0u;
   if (staged && part != 0) return;  // the staged path uses all 64 lanes of part 0
-  // pixel of this lane: all 64 in the staged path, 64 / kSplit per part otherwise
-  const int pix = staged ? lane : part * (64 / kSplit) + lane;
-  const bool lane_ok = staged || lane < 64 / kSplit;
+  // pixel of this lane: all 64 in the staged path, 64 / kSplitBwd per part otherwise
+  const int pix = staged ? lane : part * (64 / kSplitBwd) + lane;
+  const bool lane_ok = staged || lane < 64 / kSplitBwd;
   const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (pix & 7));
   const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (pix >> 3));
   const bool inside = lane_ok && pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
-  const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
-  const int cnt = (int)a.sub_count[qi];
-  const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
-  const uint64_t* masks = a.masks + mask_base(Q, rbeg, ntile, tile, q);
-  const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
   const float pixx = (float)pxi, pixy = (float)pyi;
-  const PixelBwd px = pixel_bwd(a, vi, inside, pxi, pyi);
   ChainBwd ch;
-  ch.T = px.T_final;
-  const int nch = (cnt + 63) / 64;
-  const bool active = inside && px.last_contributor > 0;
 
   if (!staged) {
     // Each pixel's blended pairs are one contiguous run in entry order (the
     // forward cut the run at the pixel's last contributor): walk it from the
     // back, the next kChainPre pairs in flight while the current ones run.
-    const uint2 rg = (active && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
+    // A lane without blended pairs has nothing to do (no barriers follow).
+    const uint2 rg = inside ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
+    if (rg.y == 0u) return;
+    const PixelBwd px = pixel_bwd(a, vi, true, pxi, pyi);
+    ch.T = px.T_final;
     const int f0 = (int)rg.x;
     int f = f0 + (int)rg.y - 1;
     float4 g0[kChainPre], g1[kChainPre], g2[kChainPre];
@@ -271,6 +270,14 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
 
   // Staged path: per chunk, stage the records, run the recurrences and the
   // gradients together, sum per entry in LDS and store the sums.
+  const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
+  const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
+  const uint64_t* masks = a.masks + mask_base(Q, rbeg, ntile, tile, q);
+  const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
+  const PixelBwd px = pixel_bwd(a, vi, inside, pxi, pyi);
+  ch.T = px.T_final;
+  const int nch = (cnt + 63) / 64;
+  const bool active = inside && px.last_contributor > 0;
   const float mcoef = kFar / (kFar - kNear);
   const float dmd_num = kFar * kNear, dmd_den0 = kFar - kNear;
   ChunkRegs nx{};
@@ -335,7 +342,7 @@ hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s) {
   if (a.vs.T == 0) return hipSuccess;
   if (a.vs.B != 8 && a.vs.B != 16 && a.vs.B != 32) return hipErrorInvalidValue;
   const unsigned nblk = (unsigned)(a.vs.T * quads_per_tile(a.vs.B));
-  hipLaunchKernelGGL(k_blend_bwd, dim3(nblk * kSplit), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_blend_bwd, dim3(nblk * kSplitBwd), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

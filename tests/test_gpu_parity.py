@@ -77,6 +77,23 @@ def test_forward_backward_cloud_c1():
     _check_backward(scene, res, o)
 
 
+def test_backward_is_deterministic():
+    """Every gradient sum is formed in a fixed order without atomics: two
+    backward passes over the same forward state give the same bits, and a
+    second forward + backward does too."""
+    scene = superquadric_surfels(3, P=20000, width=160, height=120)
+    H, W = scene.camera.image_height, scene.camera.image_width
+    gc, go = upstream_grads(3, H, W)
+    res = gpu_forward(scene)
+    print(f"[parity] determinism L={res['L']}", flush=True)
+    g1 = gpu_backward(res, gc, go)
+    g2 = gpu_backward(res, gc, go)
+    g3 = gpu_backward(gpu_forward(scene), gc, go)
+    for k in g1:
+        np.testing.assert_array_equal(g1[k], g2[k], err_msg=k)
+        np.testing.assert_array_equal(g1[k], g3[k], err_msg=k)
+
+
 @pytest.mark.parametrize("tile", [8, 32])
 def test_tile_sizes(tile):
     scene = surfel_cloud(1, P=4096, width=200, height=150)
