@@ -238,7 +238,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
 #pragma unroll
       for (int k = 0; k < kChainPre; ++k) {
         if (at - k >= f0) {
-          const float4* pr = a.ps.rec + 4 * (size_t)(at - k);
+          const float4* pr = a.ps.rec + kPairQuads * (size_t)(at - k);
           g0[k] = pr[0];
           g1[k] = pr[1];
           g2[k] = pr[2];
@@ -288,8 +288,6 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   }
   for (int c = nch - 1; c >= 0; --c) {
     uint64_t mask = active ? nx.mask : 0ull;
-    const uint32_t foff = mask ? nx.foff : kNoFrag;
-    int r = __popcll(mask);  // store index one past this lane's last pair of the chunk
     const int m = min(64, cnt - c * 64);
     __syncthreads();
     if (lane < m) {
@@ -312,18 +310,11 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
       while (mask != 0ull) {
         const int j = 63 - __builtin_clzll(mask);
         mask &= ~(1ull << j);
-        --r;
         const uint32_t contributor = s_e[j].y;  // 0-based position in the tile list
         if (contributor >= px.last_contributor) continue;
         const float4 q2 = s_q2[j];
-        Pair e;
-        if (foff != kNoFrag) {
-          const float4* pr = a.ps.rec + 4 * (size_t)(foff + r);
-          const float4 p0 = pr[0], p2 = pr[2], p3 = pr[3];
-          load_pair(pixx, pixy, s_q0[j], s_q1[j], q2, make_float2(p0.x, p0.y), make_float2(p2.w, p3.x), p3.y, e);
-        } else if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], q2, 0.0f, e)) {
-          continue;
-        }
+        Pair e;  // a flagged quadrant has no stored pairs: recompute
+        if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], q2, 0.0f, e)) continue;
         const float c_d = e.depth;
         const float m_d = mcoef * (1.0f - kNear / c_d);
         const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);

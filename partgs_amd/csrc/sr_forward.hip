@@ -1083,11 +1083,10 @@ __global__ __launch_bounds__(64) void k_pair_write(AcceptArgs a) {
       // the distortion weight m(depth) and its derivative, as the blends form them
       const float m_d = mcoef * (1.0f - kNear / e.depth);
       const float dmd_dd = dmd_num / ((dmd_den0 * e.depth) * e.depth);
-      float4* pr = a.ps.rec + 4 * (size_t)r;
+      float4* pr = a.ps.rec + kPairQuads * (size_t)r;
       pr[0] = make_float4(e.alpha, e.depth, m_d, dmd_dd);
       pr[1] = q3;
-      pr[2] = make_float4(q4.x, q4.y, __uint_as_float(s_pos[j]), e.G);
-      pr[3] = make_float4(e.sx, e.sy, 0.0f, 0.0f);
+      pr[2] = make_float4(q4.x, q4.y, __uint_as_float(s_pos[j]), 0.0f);
       ++r;
     }
   }
@@ -1184,7 +1183,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
 #pragma unroll
       for (int k = 0; k < kPre; ++k) {
         if (k < n) {
-          const float4* pr = a.ps.rec + 4 * (size_t)(at + k);
+          const float4* pr = a.ps.rec + kPairQuads * (size_t)(at + k);
           f0[k] = pr[0];
           f1[k] = pr[1];
           f2[k] = pr[2];
@@ -1252,7 +1251,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
         mask &= mask - 1ull;
         float alpha, depth;
         if (foff != kNoFrag) {
-          const float4 p0 = a.ps.rec[4 * (size_t)(f++)];
+          const float4 p0 = a.ps.rec[kPairQuads * (size_t)(f++)];
           alpha = p0.x;
           depth = p0.y;
         } else {

@@ -157,14 +157,13 @@ struct BinLayout {
 };
 constexpr int kPartStride = 20;  // 18 partial gradients, padded to five float4
 // Accepted-pair store: per accepted (pixel, entry) pair the pair-write pass
-// keeps one aligned 64-byte record with everything the per-pixel blend
-// chains read, so they stream it without staging records:
+// keeps one 48-byte record with everything the per-pixel blend chains read,
+// so they stream it without staging records:
 //   p0 = (alpha, depth, m, dm/ddepth)   m = distortion weight of the depth
 //   p1 = (normal.xyz, r)
-//   p2 = (g, b, position of the entry in the tile list (bits), G)
-//   p3 = (s.x, s.y, 0, 0)               G = exp(-rho/2), s in splat coordinates
-// A lane's consecutive pairs fill whole lines, so the fill writes no partial
-// lines.  fW = (w, dL/dalpha, dL/ddepth, 0) is written by the backward chain
+//   p2 = (g, b, position of the entry in the tile list (bits), 0)
+// A lane's consecutive pairs are consecutive records, so the fill writes
+// whole lines.  (The per-pair gradient pass recomputes s and G.)  fW = (w, dL/dalpha, dL/ddepth, 0) is written by the backward chain
 // pass for the per-pair gradient pass, which recomputes the rest of the pair.  All pairs of one pixel of a quadrant are
 // contiguous and in entry order: lane_range holds (first, count) per pixel,
 // frag_off the first pair of each (chunk, pixel).  The forward blend
@@ -175,6 +174,7 @@ constexpr int kPartStride = 20;  // 18 partial gradients, padded to five float4
 // flagged (q_flag, frag_off = kNoFrag) and blended by the staged path that
 // recomputes its pairs.
 constexpr size_t kFragPerInstance = 24;
+constexpr int kPairQuads = 3;  // float4 per stored pair
 constexpr uint32_t kNoFrag = 0xFFFFFFFFu;
 // Per-pixel acceptance masks: quadrant (t, q) owns mask slots
 // [mask_base, mask_base + 64 * chunks), chunk c of its sub-list at
@@ -207,7 +207,7 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
   b.part = off; off += align_up(n * Q * kPartStride * 4);
   b.frag_cap = n * (pair_store > 0 ? (size_t)pair_store : kFragPerInstance);
   if (b.frag_cap > 0x7fffffffu) b.frag_cap = 0x7fffffffu;
-  b.prec = off; off += align_up(b.frag_cap * 64);
+  b.prec = off; off += align_up(b.frag_cap * kPairQuads * 16);
   b.fW = off; off += align_up(b.frag_cap * 16);
   b.total = off;
   return b;
@@ -215,7 +215,7 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
 
 // Device pointers of the accepted-pair store.
 struct PairStore {
-  float4* rec;  // 4 per pair
+  float4* rec;  // kPairQuads per pair
   float4* fW;   // 1 per pair
 };
 
