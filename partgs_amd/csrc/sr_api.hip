@@ -414,6 +414,7 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
     ba.dL_dothers = dL_dothers;
     ba.inst_at = (const uint32_t*)(pl.bin + pl.bl.inst_at);
     ba.part = (float*)(pl.bin + pl.bl.part);
+    ba.part_nz = (uint8_t*)(pl.bin + pl.bl.part_nz);
     SR_STAGE(6, dbg, st, "blend_bwd", launch_blend_bwd(ba, st));
     SR_STAGE(9, dbg, st, "blend_bwd_grad",
              launch_blend_bwd_grad(ba, work_capacity(pl.capacity, pl.vs.B, pl.il.T), st));
@@ -428,6 +429,7 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
   pb.roff = (const uint32_t*)(pl.geom + pl.gl.roff);
   pb.inst_quads = blend ? (const uint16_t*)(pl.bin + pl.bl.inst_quads) : nullptr;
   pb.part = blend ? (const float*)(pl.bin + pl.bl.part) : nullptr;
+  pb.part_nz = blend ? (const uint8_t*)(pl.bin + pl.bl.part_nz) : nullptr;
   pb.means3D = g->means3D;
   pb.scales = g->scales;
   pb.rots = g->rotations;

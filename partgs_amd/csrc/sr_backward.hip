@@ -32,13 +32,19 @@ constexpr int kSplitBwd = 8;
 
 // Stores the 18 partial sums (g(k) for slot k) of the sub-list entry at
 // tile-list position pos of quadrant q as five float4 in its (instance,
-// quadrant) slot; every entry of every sub-list is stored exactly once per
-// backward, zeros included, so the buffer needs no clearing.
+// quadrant) slot.  Every entry of every sub-list sets its flag exactly once
+// per backward and stores its sums when any is non-zero, so neither buffer
+// needs clearing.
 template <class G>
 SR_DEV void store_part(const BlendBwdArgs& a, uint32_t pos, int q, G g) {
   const int Q = quads_per_tile(a.vs.B);
-  const uint32_t inst = a.inst_at[pos];
-  float4* d = reinterpret_cast<float4*>(a.part + ((size_t)inst * Q + q) * kPartStride);
+  const size_t slot = (size_t)a.inst_at[pos] * Q + q;
+  bool nz = false;
+#pragma unroll
+  for (int k = 0; k < 18; ++k) nz |= g(k) != 0.0f;
+  a.part_nz[slot] = nz ? 1 : 0;  // most entries blend no pixel of the quadrant: only a flag
+  if (!nz) return;
+  float4* d = reinterpret_cast<float4*>(a.part + slot * kPartStride);
   d[0] = make_float4(g(0), g(1), g(2), g(3));
   d[1] = make_float4(g(4), g(5), g(6), g(7));
   d[2] = make_float4(g(8), g(9), g(10), g(11));
@@ -622,6 +628,7 @@ SR_DEV void reduce_record(const PreprocessBwdArgs& a, size_t r, float (&gs)[18])
     while (m != 0u) {
       const int q = __builtin_ctz(m);
       m &= m - 1u;
+      if (a.part_nz[(size_t)i * Q + q] == 0) continue;
       const float4* p = reinterpret_cast<const float4*>(a.part + ((size_t)i * Q + q) * kPartStride);
       const float4 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4];
       gs[0] += p0.x; gs[1] += p0.y; gs[2] += p0.z; gs[3] += p0.w;

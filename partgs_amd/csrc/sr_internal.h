@@ -151,7 +151,7 @@ inline ImageLayout image_layout(int W, int H, int B) { return image_layout(singl
 //              global atomic
 struct BinLayout {
   size_t keys, keys_tmp, point_list, sub, masks, chunk_T, work, frag_off, total;
-  size_t inst_rec, inst_at, inst_quads, part;
+  size_t inst_rec, inst_at, inst_quads, part, part_nz;
   size_t prec, fW;
   size_t frag_cap;
 };
@@ -205,6 +205,7 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
   b.inst_at = off; off += align_up(n * 4);
   b.inst_quads = off; off += align_up(n * 2);
   b.part = off; off += align_up(n * Q * kPartStride * 4);
+  b.part_nz = off; off += align_up(n * Q);
   b.frag_cap = n * (pair_store > 0 ? (size_t)pair_store : kFragPerInstance);
   if (b.frag_cap > 0x7fffffffu) b.frag_cap = 0x7fffffffu;
   b.prec = off; off += align_up(b.frag_cap * kPairQuads * 16);
@@ -345,7 +346,8 @@ struct BlendBwdArgs {
   const float* dL_dpix;
   const float* dL_dothers;
   const uint32_t* inst_at;
-  float* part;  // partial gradients per (instance, quadrant)
+  float* part;        // partial gradients per (instance, quadrant)
+  uint8_t* part_nz;   // 1 where that slot was stored (any non-zero sum)
 };
 
 // Backward preprocess over all views of each surfel.  It first reduces each
@@ -361,6 +363,7 @@ struct PreprocessBwdArgs {
   const uint32_t* roff;
   const uint16_t* inst_quads;
   const float* part;
+  const uint8_t* part_nz;
   const float* means3D;
   const float* scales;
   const float* rots;
