@@ -332,6 +332,7 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   sa.inst_rec = (const uint32_t*)(bin + bl.inst_rec);
   sa.inst_at = (uint32_t*)(bin + bl.inst_at);
   sa.inst_quads = (uint16_t*)(bin + bl.inst_quads);
+  sa.part_nz = (uint8_t*)(bin + bl.part_nz);
   sa.sub = (uint2*)(bin + bl.sub);
   sa.sub_count = (uint32_t*)(pl.img + pl.il.sub_count);
   sa.work = (uint2*)(bin + bl.work);

@@ -507,25 +507,14 @@ SR_DEV F3 had(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
 SR_DEV float dot3(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 SR_DEV float comp(F3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 
-// SH backward, dsr/cuda_rasterizer/backward.cu:20-139 (same association).
-template <bool kAdd>
-SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, float* dsh) {
+// SH backward, dsr/cuda_rasterizer/backward.cu:20-139 (same association):
+// put(k, c) receives the basis value of each active coefficient k (dL/dsh_k
+// = c dL/dRGB), S(k) gives coefficient k; returns dL/d(mean) through the
+// view direction.
+template <class Get, class Put>
+SR_DEV F3 sh_backward_core(int deg, F3 dir_orig, F3 dL_dRGB, Get S, Put put) {
   const float len = sqrtf((dir_orig.x * dir_orig.x + dir_orig.y * dir_orig.y) + dir_orig.z * dir_orig.z);
   const F3 dir = f3(dir_orig.x / len, dir_orig.y / len, dir_orig.z / len);
-  auto S = [&](int k) { return f3(sh[3 * k], sh[3 * k + 1], sh[3 * k + 2]); };
-  auto put = [&](int k, float c) {  // the first view stores, later views add
-    if (kAdd) {
-      dsh[3 * k] += c * dL_dRGB.x;
-      dsh[3 * k + 1] += c * dL_dRGB.y;
-      dsh[3 * k + 2] += c * dL_dRGB.z;
-    } else {
-      dsh[3 * k] = c * dL_dRGB.x;
-      dsh[3 * k + 1] = c * dL_dRGB.y;
-      dsh[3 * k + 2] = c * dL_dRGB.z;
-    }
-  };
-  if (!kAdd)
-    for (int k = (deg + 1) * (deg + 1); k < M; ++k) dsh[3 * k] = dsh[3 * k + 1] = dsh[3 * k + 2] = 0.0f;
   F3 dx = f3(0.0f, 0.0f, 0.0f), dy = dx, dz = dx;
   const float x = dir.x, y = dir.y, z = dir.z;
   put(0, kSH_C0);
@@ -593,6 +582,66 @@ SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, 
   return r;
 }
 
+// SH backward against memory: the first view stores c dL_dRGB (and zeros for
+// the inactive coefficients), later views add.  With 16 coefficients and
+// 16-byte aligned rows the surfel's 48 coefficients are read and its 48
+// gradients written as twelve float4 each (same values, same order of
+// operations as the scalar form).
+template <bool kAdd>
+SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, float* dsh) {
+  const int nact = (deg + 1) * (deg + 1);
+  if (M == 16 && ((reinterpret_cast<uintptr_t>(sh) | reinterpret_cast<uintptr_t>(dsh)) & 15u) == 0u) {
+    float shv[48];
+    const float4* s4 = reinterpret_cast<const float4*>(sh);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+      const float4 t = s4[i];
+      shv[4 * i] = t.x;
+      shv[4 * i + 1] = t.y;
+      shv[4 * i + 2] = t.z;
+      shv[4 * i + 3] = t.w;
+    }
+    float cv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) cv[k] = 0.0f;
+    const F3 r = sh_backward_core(
+        deg, dir_orig, dL_dRGB, [&](int k) { return f3(shv[3 * k], shv[3 * k + 1], shv[3 * k + 2]); },
+        [&](int k, float c) { cv[k] = c; });
+    float4* d4 = reinterpret_cast<float4*>(dsh);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+      float v[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int j = 4 * i + t, k = j / 3;
+        v[t] = k < nact ? cv[k] * comp(dL_dRGB, j % 3) : (kAdd ? -0.0f : 0.0f);
+      }
+      if (kAdd) {
+        const float4 o = d4[i];
+        d4[i] = make_float4(o.x + v[0], o.y + v[1], o.z + v[2], o.w + v[3]);
+      } else {
+        d4[i] = make_float4(v[0], v[1], v[2], v[3]);
+      }
+    }
+    return r;
+  }
+  if (!kAdd)
+    for (int k = nact; k < M; ++k) dsh[3 * k] = dsh[3 * k + 1] = dsh[3 * k + 2] = 0.0f;
+  return sh_backward_core(
+      deg, dir_orig, dL_dRGB, [&](int k) { return f3(sh[3 * k], sh[3 * k + 1], sh[3 * k + 2]); },
+      [&](int k, float c) {
+        if (kAdd) {
+          dsh[3 * k] += c * dL_dRGB.x;
+          dsh[3 * k + 1] += c * dL_dRGB.y;
+          dsh[3 * k + 2] += c * dL_dRGB.z;
+        } else {
+          dsh[3 * k] = c * dL_dRGB.x;
+          dsh[3 * k + 1] = c * dL_dRGB.y;
+          dsh[3 * k + 2] = c * dL_dRGB.z;
+        }
+      });
+}
+
 // SH backward of one record: stores (first view) or adds c * dL_dRGB to the
 // surfel's dL_dsh and returns the gradient of the mean through the view
 // direction.
@@ -623,6 +672,31 @@ SR_DEV void reduce_record(const PreprocessBwdArgs& a, size_t r, float (&gs)[18])
   if (n == 0 || a.part == nullptr || (a.counters[1] & kErrCapacity)) return;
   const int Q = quads_per_tile(a.vs.B);
   const uint32_t i0 = a.roff[r];
+  auto add_slot = [&](size_t slot) {
+    const float4* p = reinterpret_cast<const float4*>(a.part + slot * kPartStride);
+    const float4 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4];
+    gs[0] += p0.x; gs[1] += p0.y; gs[2] += p0.z; gs[3] += p0.w;
+    gs[4] += p1.x; gs[5] += p1.y; gs[6] += p1.z; gs[7] += p1.w;
+    gs[8] += p2.x; gs[9] += p2.y; gs[10] += p2.z; gs[11] += p2.w;
+    gs[12] += p3.x; gs[13] += p3.y; gs[14] += p3.z; gs[15] += p3.w;
+    gs[16] += p4.x; gs[17] += p4.y;
+  };
+  if (Q == 4) {
+    // the four flags of an instance are one word (zero for quadrants that do
+    // not hold it); four instances' flags in flight at a time
+    const uint32_t* nz = reinterpret_cast<const uint32_t*>(a.part_nz);
+    for (uint32_t i = i0; i < i0 + n; i += 4) {
+      uint32_t fl[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) fl[t] = i + t < i0 + n ? nz[i + t] : 0u;
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if ((fl[t] >> (8 * q)) & 0xffu) add_slot((size_t)(i + t) * 4 + q);
+    }
+    return;
+  }
   for (uint32_t i = i0; i < i0 + n; ++i) {
     uint32_t m = a.inst_quads[i];
     while (m != 0u) {

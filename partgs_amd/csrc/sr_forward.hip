@@ -676,7 +676,14 @@ SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n, co
         hitmask |= (uint32_t)hit << q;
       }
     }
-    if (valid) a.inst_quads[inst] = (uint16_t)hitmask;  // for the gradient reduction
+    if (valid) {  // for the gradient reduction: the instance's quadrants, their stored-sum flags cleared
+      a.inst_quads[inst] = (uint16_t)hitmask;
+      if (Q == 4) {
+        reinterpret_cast<uint32_t*>(a.part_nz)[inst] = 0u;
+      } else {
+        for (int q = 0; q < Q; ++q) a.part_nz[(size_t)inst * Q + q] = 0;
+      }
+    }
     __syncthreads();
     // per quadrant, the waves' first slots: a scan over the waves, one lane
     // per (quadrant, wave), each quadrant a kWaves-lane segment of one wave

@@ -274,6 +274,7 @@ struct SortArgs {
   const uint32_t* inst_rec;
   uint32_t* inst_at;
   uint16_t* inst_quads;
+  uint8_t* part_nz;  // cleared here for every (instance, quadrant)
   uint2* sub;
   uint32_t* sub_count;
   uint2* work;          // (quadrant index, chunk) items for the acceptance pass
