@@ -2,6 +2,12 @@
 
   python -m partgs_amd.proposal_train --train 256 --eval 16 --fit-iters 200 --out DIR
 
+Labelling dominates the cost (one plain fit per training scene), so it can
+be split over several runs: `--label-seed S --labels-out F --label-only`
+labels scenes S, S+1, ... and saves their seeds and fitted blocks (the
+points are regenerated from the seeds); `--labels-in F1 F2 ...` trains on
+saved labels instead of labelling.
+
 1. Dataset: synthetic block scenes (partgs_amd.block_fit.random_block_scene,
    4-16 superquadrics, 20,000 noisy surface points each), each labelled by
    the plain pipeline: the k-means initialisation of partgs_amd.init with 16
@@ -144,6 +150,10 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default="proposal_out")
     ap.add_argument("--weights", default=None, help="skip training and load these weights")
+    ap.add_argument("--label-seed", type=int, default=10_000, help="first training-scene seed")
+    ap.add_argument("--labels-out", default=None, help="save the labelled seeds and blocks here")
+    ap.add_argument("--labels-in", nargs="*", default=None, help="train on these saved labels")
+    ap.add_argument("--label-only", action="store_true", help="label (and save), then stop")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
     torch.manual_seed(0)
@@ -154,17 +164,34 @@ def main(argv=None):
         sd = np.load(args.weights)
         net.load_state_dict({k: torch.tensor(sd[k]) for k in sd.files})
     else:
-        pts, labels = [], []
-        for i in range(args.train):
-            scene, label, losses = label_scene(10_000 + i, args.fit_iters)
-            pts.append(scene.points)
-            labels.append(label)
-            if i % 16 == 0:
-                _log(f"[proposal] labelled {i + 1}/{args.train} scenes, fit loss {losses[0]:.4f} -> "
-                     f"{losses[-1]:.4f} ({time.time() - t0:.0f} s)")
+        pts, labels, seeds = [], [], []
+        if args.labels_in:
+            for fn in args.labels_in:
+                z = np.load(fn)
+                for sd, lb in zip(z["seeds"], z["labels"]):
+                    seeds.append(int(sd))
+                    pts.append(random_block_scene(int(sd)).points)
+                    labels.append(torch.tensor(lb, device=pts[-1].device))
+            _log(f"[proposal] loaded {len(seeds)} labelled scenes from {len(args.labels_in)} file(s)")
+        else:
+            for i in range(args.train):
+                sd = args.label_seed + i
+                scene, label, losses = label_scene(sd, args.fit_iters)
+                seeds.append(sd)
+                pts.append(scene.points)
+                labels.append(label)
+                if i % 16 == 0:
+                    _log(f"[proposal] labelled {i + 1}/{args.train} scenes, fit loss {losses[0]:.4f} -> "
+                         f"{losses[-1]:.4f} ({time.time() - t0:.0f} s)")
         points = torch.stack(pts)
         labels = torch.stack(labels)
+        report["train_scenes"] = len(seeds)
         report["label_seconds"] = time.time() - t0
+        if args.labels_out:
+            np.savez(args.labels_out, seeds=np.array(seeds, np.int64), labels=labels.cpu().numpy())
+        if args.label_only:
+            _log(json.dumps(report))
+            return 0
         t1 = time.time()
         hist = train(net, points, labels, args.epochs, args.batch, args.lr)
         report["train_seconds"] = time.time() - t1
