@@ -27,8 +27,10 @@ namespace sr {
 constexpr int kNumG = 18;
 constexpr int kPre = 8;       // pairs a lane fetches ahead in the per-pair pass
 constexpr int kChainPre = 8;  // stored pairs a lane fetches ahead in the chain pass
-// waves per quadrant in the chain pass (64 / kSplitBwd pixels each)
+// waves per quadrant in the chain pass (64 / kSplitBwd = 8 pixels each,
+// 8 lanes per pixel)
 constexpr int kSplitBwd = 8;
+constexpr int kChainSegMin = 16;  // shortest chain segment a lane replays
 
 // Stores the 18 partial sums (g(k) for slot k) of the sub-list entry at
 // tile-list position pos of quadrant q as five float4 in its (instance,
@@ -148,7 +150,32 @@ struct ChainBwd {
     dL_dz += (alpha * T) * p.dL_ddepth;
     return make_float4(w, dL_dalpha, dL_dz, 0.0f);
   }
+  // The state updates of step() alone, with the same operations in the same
+  // order (so a chain resumed from a saved state continues bit for bit).
+  SR_DEV void advance(const PixelBwd& p, float alpha, float c_d, float m_d, const float4& nc, float g, float bl) {
+    T = T / (1.0f - alpha);
+    acc_r0 = last_alpha * last_c0 + (1.0f - last_alpha) * acc_r0;
+    last_c0 = nc.w;
+    acc_r1 = last_alpha * last_c1 + (1.0f - last_alpha) * acc_r1;
+    last_c1 = g;
+    acc_r2 = last_alpha * last_c2 + (1.0f - last_alpha) * acc_r2;
+    last_c2 = bl;
+    float dL_dweight = 0.0f;
+    dL_dweight += ((p.final_D2 + (m_d * m_d) * p.final_A) - (2.0f * m_d) * p.final_D) * p.dL_dreg;
+    last_dL_dT = dL_dweight * alpha + (1.0f - alpha) * last_dL_dT;
+    accum_depth_rec = last_alpha * last_depth + (1.0f - last_alpha) * accum_depth_rec;
+    last_depth = c_d;
+    accum_alpha_rec = last_alpha + (1.0f - last_alpha) * accum_alpha_rec;
+    acc_n0 = last_alpha * last_n0 + (1.0f - last_alpha) * acc_n0;
+    last_n0 = nc.x;
+    acc_n1 = last_alpha * last_n1 + (1.0f - last_alpha) * acc_n1;
+    last_n1 = nc.y;
+    acc_n2 = last_alpha * last_n2 + (1.0f - last_alpha) * acc_n2;
+    last_n2 = nc.z;
+    last_alpha = alpha;
+  }
 };
+static_assert(sizeof(ChainBwd) == 18 * sizeof(float), "chain state is 18 floats");
 
 // The 18 partial gradients of one pair (backward.cu:380-437) from its
 // weight w, dL/dalpha and dL/ddepth and its pixel's dL/dpixel (dLp) and
@@ -207,6 +234,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
   __shared__ uint2 s_e[64];
   __shared__ float s_acc[kNumG][64];
+  __shared__ float s_state[8][7][18];  // chain state where each later segment of a run starts
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   if (a.counters[1] & kErrCapacity) return;
   const int qi = (int)a.q_order[blockIdx.x / kSplitBwd], part = blockIdx.x % kSplitBwd;
@@ -230,46 +258,86 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
 
   if (!staged) {
     // Each pixel's blended pairs are one contiguous run in entry order (the
-    // forward cut the run at the pixel's last contributor): walk it from the
-    // back, the next kChainPre pairs in flight while the current ones run.
-    // A lane without blended pairs has nothing to do (no barriers follow).
-    const uint2 rg = inside ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
-    if (rg.y == 0u) return;
-    const PixelBwd px = pixel_bwd(a, vi, true, pxi, pyi);
-    ch.T = px.T_final;
-    const int f0 = (int)rg.x;
-    int f = f0 + (int)rg.y - 1;
-    float4 g0[kChainPre], g1[kChainPre], g2[kChainPre];
-    auto fetch = [&](int at) {
+    // forward cut the run at the pixel's last contributor), walked from the
+    // back with the next kChainPre pairs in flight.  A wave holds 8 pixels
+    // and gives each 8 lanes: a long run is cut into up to 8 segments; the
+    // pixel's first lane walks the run once with the state updates alone
+    // (ChainBwd::advance, about half a step), saving the chain state where
+    // each segment starts, then every segment is replayed with full steps by
+    // its own lane from its saved state.  The replay repeats the sequential
+    // walk's operations exactly, so each pair's outputs are bit-identical.
+    const int ps = lane & 7, sg = lane >> 3;
+    const int pxl = part * 8 + ps;
+    const uint32_t qxi = (uint32_t)(tx * B + (q % QX) * kQuad + (pxl & 7));
+    const uint32_t qyi = (uint32_t)(ty * B + (q / QX) * kQuad + (pxl >> 3));
+    const bool pin = qxi < (uint32_t)vi.W && qyi < (uint32_t)vi.H;
+    const uint2 rg = pin ? a.lane_range[(size_t)qi * 64 + pxl] : make_uint2(0u, 0u);
+    const int n = (int)rg.y, f0 = (int)rg.x;
+    const int seg_len = max(kChainSegMin, (n + 7) / 8);
+    const int nseg = (n + seg_len - 1) / seg_len;
+    const PixelBwd px = pixel_bwd(a, vi, n > 0, qxi, qyi);
+    // walk pairs f0 + hi down to f0 + lo, calling fn(index, p0, p1, p2)
+    auto walk = [&](int lo, int hi, auto&& fn) {
+      float4 g0[kChainPre], g1[kChainPre], g2[kChainPre];
+      auto fetch = [&](int at) {
 #pragma unroll
-      for (int k = 0; k < kChainPre; ++k) {
-        if (at - k >= f0) {
-          const float4* pr = a.ps.rec + kPairQuads * (size_t)(at - k);
-          g0[k] = pr[0];
-          g1[k] = pr[1];
-          g2[k] = pr[2];
+        for (int k = 0; k < kChainPre; ++k) {
+          if (at - k >= lo) {
+            const float4* pr = a.ps.rec + kPairQuads * (size_t)(f0 + at - k);
+            g0[k] = pr[0];
+            g1[k] = pr[1];
+            g2[k] = pr[2];
+          }
         }
+      };
+      int f = hi;
+      if (f >= lo) fetch(f);
+      while (f >= lo) {
+        float4 c0[kChainPre], c1[kChainPre], c2[kChainPre];
+#pragma unroll
+        for (int k = 0; k < kChainPre; ++k) {
+          c0[k] = g0[k];
+          c1[k] = g1[k];
+          c2[k] = g2[k];
+        }
+        const int at = f;
+        f -= kChainPre;
+        if (f >= lo) fetch(f);  // next batch in flight
+#pragma unroll
+        for (int k = 0; k < kChainPre; ++k)
+          if (at - k >= lo) fn(at - k, c0[k], c1[k], c2[k]);
       }
     };
-    fetch(f);
-    while (f >= f0) {
-      float4 c0[kChainPre], c1[kChainPre], c2[kChainPre];
+    // state walk: segment s starts (backward order) after its pairs from
+    // (s + 1) seg_len on have run
+    float* st_out = &s_state[ps][0][0];
+    if (sg == 0 && nseg > 1) {
+      ChainBwd w0;
+      w0.T = px.T_final;
+      walk(seg_len, n - 1, [&](int k, const float4& c0, const float4& c1, const float4& c2) {
+        w0.advance(px, c0.x, c0.y, c0.z, c1, c2.x, c2.y);
+        if (k % seg_len == 0) {
+          const float* src = reinterpret_cast<const float*>(&w0);
+          float* dst = st_out + (k / seg_len - 1) * 18;
 #pragma unroll
-      for (int k = 0; k < kChainPre; ++k) {
-        c0[k] = g0[k];
-        c1[k] = g1[k];
-        c2[k] = g2[k];
-      }
-      const int at = f;
-      f -= kChainPre;
-      if (f >= f0) fetch(f);  // next batch in flight
-#pragma unroll
-      for (int k = 0; k < kChainPre; ++k) {
-        if (at - k >= f0) {
-          const uint32_t pos = __float_as_uint(c2[k].z);
-          a.ps.fW[at - k] = ch.step(px, c0[k].x, c0[k].y, c0[k].z, c0[k].w, pos, c1[k], c2[k].x, c2[k].y);
+          for (int i = 0; i < 18; ++i) dst[i] = src[i];
         }
+      });
+    }
+    __syncthreads();
+    if (sg < nseg) {
+      if (sg == nseg - 1) {
+        ch.T = px.T_final;
+      } else {
+        const float* src = st_out + sg * 18;
+        float* dst = reinterpret_cast<float*>(&ch);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) dst[i] = src[i];
       }
+      walk(sg * seg_len, min(n, (sg + 1) * seg_len) - 1,
+           [&](int k, const float4& c0, const float4& c1, const float4& c2) {
+             a.ps.fW[f0 + k] = ch.step(px, c0.x, c0.y, c0.z, c0.w, __float_as_uint(c2.z), c1, c2.x, c2.y);
+           });
     }
     return;
   }
