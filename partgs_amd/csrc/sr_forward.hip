@@ -828,13 +828,51 @@ __global__ __launch_bounds__(1024) void k_sort_large(SortArgs a) {
   if (blockIdx.x < a.counters[3]) sort_tile<1024, kLargeCap>(a, (int)a.large[blockIdx.x], s_keys);
 }
 
+// A second stream per (host thread, device) with two fork/join events, for
+// launches that may overlap: the small- and large-tile sorts touch disjoint
+// tiles, so the large tiles (the longest single sorts) run beside the small
+// ones instead of after them.  Default priority; created on first use.
+struct SideStream {
+  int device = -1;
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+static hipError_t side_stream(SideStream*& out) {
+  thread_local SideStream cache[16];
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
+  SideStream& c = cache[dev];
+  if (c.s == nullptr) {
+    e = hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.join, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+    c.device = dev;
+  }
+  out = &c;
+  return hipSuccess;
+}
+
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
   if (a.vs.T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sort_small, dim3(a.vs.T), dim3(256), 0, s, a);
-  hipError_t e = hipGetLastError();
+  SideStream* side = nullptr;
+  hipError_t e = side_stream(side);
+  if (e != hipSuccess) return e;
+  // fork: the small-tile sort on the side stream once everything before it
+  // on s is done, the large-tile sort on s; join before anything after
+  e = hipEventRecord(side->fork, s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(side->s, side->fork, 0);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sort_small, dim3(a.vs.T), dim3(256), 0, side->s, a);
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_sort_large, dim3(a.vs.T), dim3(1024), 0, s, a);
-  return hipGetLastError();
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipEventRecord(side->join, side->s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s, side->join, 0);
+  return e;
 }
 
 // ---------------------------------------------------------------------------
