@@ -141,6 +141,9 @@ size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int num_
  * 10 accepted-pair store (offsets and fill). */
 #define SR_NUM_STAGES 11
 int sr_set_timing(int enabled);
+/* Times only the stages whose bit is set in mask (bit i: stage i), so a
+ * caller can time one kernel without the other stages' events. */
+int sr_set_timing_stages(unsigned int mask);
 /* Synchronises pending events; ms[i] = total milliseconds, count[i] = launches. */
 int sr_get_timing(double* ms, int* count, int n);
 void sr_reset_timing(void);

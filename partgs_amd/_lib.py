@@ -46,6 +46,7 @@ EXPORTED = (
     "sr_binning_bytes",
     "sr_state_offset",
     "sr_set_timing",
+    "sr_set_timing_stages",
     "sr_get_timing",
     "sr_reset_timing",
     "sr_last_error",
@@ -165,6 +166,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_state_offset.restype = sz
         lib.sr_set_timing.argtypes = [i]
         lib.sr_set_timing.restype = i
+        lib.sr_set_timing_stages.argtypes = [C.c_uint]
+        lib.sr_set_timing_stages.restype = i
         lib.sr_get_timing.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int), i]
         lib.sr_get_timing.restype = i
         lib.sr_reset_timing.argtypes = []
@@ -215,6 +218,14 @@ def state_offset(name: str, P: int, W: int, H: int, tile: int, L: int) -> int:
 
 def set_timing(enabled: bool) -> None:
     check(load().sr_set_timing(1 if enabled else 0))
+
+
+def set_timing_stages(names) -> None:
+    """Time only the named stages (names from STAGE_NAMES)."""
+    mask = 0
+    for n in names:
+        mask |= 1 << STAGE_NAMES.index(n)
+    check(load().sr_set_timing_stages(mask))
 
 
 def get_timing() -> dict:

@@ -39,7 +39,7 @@ int fail(int code, const char* fmt, ...) {
 // ---- per-stage timing ------------------------------------------------------
 struct Timing {
   std::mutex mu;
-  bool on = false;
+  uint32_t mask = 0;  // bit i: stage i is timed
   struct Pending {
     int stage;
     hipEvent_t a, b;
@@ -72,7 +72,7 @@ struct StageScope {
   StageScope(int st, hipStream_t str) : stage(st), s(str) {
     Timing& t = timing();
     std::lock_guard<std::mutex> lk(t.mu);
-    if (!t.on) return;
+    if (stage < 0 || stage >= SR_NUM_STAGES || !((t.mask >> stage) & 1u)) return;
     a = take_event(t);
     b = take_event(t);
     if (a && b) (void)hipEventRecord(a, s);
@@ -326,7 +326,13 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   sa.counters = counters;
   sa.large = (const uint32_t*)(pl.img + pl.il.large);
   sa.keys = (uint64_t*)(bin + bl.keys);
-  sa.keys_tmp = (uint64_t*)(bin + bl.keys_tmp);
+  sa.lmeta = (uint2*)(bin + bl.lmeta);
+  sa.lpre = (uint16_t*)(bin + bl.lpre);
+  // a large tile holds more than kSmallMax of the capacity's instances
+  {
+    const size_t bound = (size_t)pl.capacity / (kSmallMax + 1) + 1;
+    sa.max_large = (uint32_t)(bound < (size_t)pl.il.T ? bound : (size_t)pl.il.T);
+  }
   sa.point_list = (uint32_t*)(bin + bl.point_list);
   sa.cullbox = (const short4*)(pl.geom + pl.gl.cullbox);
   sa.inst_rec = (const uint32_t*)(bin + bl.inst_rec);
@@ -656,9 +662,13 @@ int sr_mark_visible(int P, const float* means3D, const float* viewmatrix, const 
 }
 
 int sr_set_timing(int enabled) {
+  return sr_set_timing_stages(enabled ? (1u << SR_NUM_STAGES) - 1u : 0u);
+}
+
+int sr_set_timing_stages(unsigned int mask) {
   Timing& t = timing();
   std::lock_guard<std::mutex> lk(t.mu);
-  t.on = enabled != 0;
+  t.mask = mask & ((1u << SR_NUM_STAGES) - 1u);
   return SR_OK;
 }
 

@@ -7,11 +7,14 @@
 //                  tile start offsets, tile ranges and the instance count.
 //   k_scatter      per surfel: writes a 64-bit key (depth bits << 32 | surfel
 //                  id) for every covered tile into that tile's bucket.
-//   k_sort_tiles   one workgroup per tile: sorts its bucket in LDS (bitonic;
-//                  256 lanes up to 2048 keys, 1024 lanes up to 16384 keys,
-//                  larger tiles sort 16K-key chunks and merge them in global
-//                  memory), emits point_list, then splits the sorted list
-//                  into per-8x8-quadrant sub-lists.
+//   k_sort_tiles   one workgroup per tile of up to 2048 instances: sorts its
+//                  bucket in LDS (bitonic runs, merge-path levels), emits
+//                  point_list, then splits the sorted list into
+//                  per-8x8-quadrant sub-lists; larger tiles: one workgroup
+//                  per 2048-key chunk, sorted the same way.
+//   k_merge_chunks the large tiles' chunks merged: every entry placed in the
+//                  tile list and the sub-lists by its ranks in the other
+//                  chunks (binary search in LDS).
 //   k_blend_fwd    one wave64 per 8 x 8 quadrant, one lane per pixel: front-
 //                  to-back alpha blending of colour, depth, alpha, normal,
 //                  median depth and depth distortion over the sub-list.
@@ -315,11 +318,6 @@ hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Tile sort sizes: tiles with at most kSmallMax instances are sorted by a
-// 256-lane workgroup, larger ones by a 1024-lane workgroup holding up to
-// kLargeCap keys in LDS.
-constexpr int kSmallMax = 2048;
-constexpr int kLargeCap = 16384;
 
 // Inclusive Hillis-Steele scan of one value per lane of a 1024-lane
 // workgroup (s: 1024 words of LDS); returns the lane's exclusive prefix.
@@ -492,22 +490,10 @@ hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // Per-tile sort.  Tiles with at most kSmallMax instances are sorted by a
-// 256-lane workgroup entirely in LDS (8 keys per lane); larger tiles by a
-// 1024-lane workgroup holding up to kLargeCap keys (128 KiB of LDS, 16 keys
-// per lane), beyond which 16K-key chunks are sorted in LDS and merged
-// pairwise in global memory (merge path).
+// 256-lane workgroup entirely in LDS (8 keys per lane); larger tiles are cut
+// into kSmallMax-key chunks, each sorted the same way by its own workgroup,
+// and merged by rank (k_sort_tiles, k_merge_chunks).
 // ---------------------------------------------------------------------------
-
-// Number of elements of A among the first k of merge(A, B) (keys unique).
-SR_DEV int co_rank(const uint64_t* A, int na, const uint64_t* Bv, int nb, int k) {
-  int lo = max(0, k - nb), hi = min(k, na);
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (A[mid] < Bv[k - mid - 1]) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
 
 // LDS slot of key i in the merge sort: one pad slot after every K keys, so
 // the K-strided accesses of a wave (lane t at t K + k) spread over the banks
@@ -598,215 +584,371 @@ SR_DEV int next_pow2(int n) {
   return s;
 }
 
-// Order-preserving split of a tile's sorted list into one sub-list per 8 x 8
-// quadrant: entry e (position e in the tile list) goes to quadrant q when its
-// cull box meets the quadrant.  Sub-list q of tile t starts at
-// Q * beg + q * n and holds (surfel id, position) pairs; it is exactly the
-// tile list with entries removed that cannot reach alpha >= 1/255 anywhere in
-// the quadrant, so blending it gives the reference's result.  Each non-empty
-// sub-list also appends its 64-entry chunks to the acceptance work list.
+// Quadrant sub-lists.  Sub-list q of tile t is the tile's sorted list with
+// the entries removed whose conservative cull box misses quadrant q (such
+// an entry cannot reach alpha >= 1/255 anywhere in the quadrant, so blending
+// the sub-list gives the reference's result); it starts at Q * beg + q * n
+// and holds (surfel id, position in the tile list) pairs.  Each non-empty
+// sub-list appends its 64-entry chunks to the acceptance work list.
 constexpr int kMaxQuads = 16;
-// kRegs: the caller holds the tile's sorted records and instances in
-// registers (rid, rin; lane tid owns entries tid + k NT), as after an LDS
-// sort; otherwise they are read back from point_list and inst_at.
-template <int NT, int K, bool kRegs>
-SR_DEV void split_quadrants(const SortArgs& a, int tile, uint32_t beg, int n, const uint32_t (&rid)[K],
-                            const uint32_t (&rin)[K]) {
-  constexpr int kWaves = NT / 64;
-  __shared__ uint32_t s_wcnt[kMaxQuads][kWaves];
-  __shared__ uint32_t s_woff[kMaxQuads][kWaves];
-  __shared__ uint32_t s_total[kMaxQuads];
-  const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
+
+// Bit q set when cull box bx meets quadrant q (of QX x QX) of tile (tx, ty)
+// inside the image; the empty box (32767, 32767, -32768, -32768) meets none.
+SR_DEV uint32_t quad_hits(short4 bx, int tx, int ty, int B, int QX, int Q, const ViewInfo& vi) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxQuads; ++q) {
+    if (q < Q) {
+      const int qx0 = tx * B + (q % QX) * kQuad, qy0 = ty * B + (q / QX) * kQuad;
+      const bool hit = qx0 < vi.W && qy0 < vi.H && bx.x <= qx0 + kQuad - 1 && bx.z >= qx0 &&
+                       bx.y <= qy0 + kQuad - 1 && bx.w >= qy0;
+      m |= (uint32_t)hit << q;
+    }
+  }
+  return m;
+}
+
+// The hit bits of quadrants 4g..4g+3 in the four 16-bit fields of a packed
+// counter (at most kSmallMax < 2^16 entries are counted in one scan, so
+// sums of these never carry from one field into the next).
+SR_DEV uint64_t spread4(uint32_t h, int g) {
+  const uint32_t b = h >> (4 * g);
+  return (uint64_t)(b & 1u) | ((uint64_t)((b >> 1) & 1u) << 16) | ((uint64_t)((b >> 2) & 1u) << 32) |
+         ((uint64_t)((b >> 3) & 1u) << 48);
+}
+SR_DEV uint32_t field16(uint64_t v, int f) { return (uint32_t)(v >> (16 * f)) & 0xffffu; }
+
+// Exclusive scan of one 64-bit value per lane over a 256-lane workgroup;
+// total receives the sum over all lanes.
+SR_DEV uint64_t scan256_excl_u64(uint64_t v, uint64_t* s_w, uint64_t& total) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint64_t incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)incl, d);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(incl >> 32), d);
+    if (lane >= d) incl += ((uint64_t)hi << 32) | lo;
+  }
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();
+  const uint64_t w0 = s_w[0], w1 = s_w[1], w2 = s_w[2], w3 = s_w[3];
+  __syncthreads();
+  total = w0 + w1 + w2 + w3;
+  const uint64_t off = (wave > 0 ? w0 : 0ull) + (wave > 1 ? w1 : 0ull) + (wave > 2 ? w2 : 0ull);
+  return off + incl - v;
+}
+
+// After an LDS sort of cn keys (bucket entries c0 .. c0 + cn - 1 of tile
+// `tile`, whose bucket starts at beg and holds n), by a 256-lane workgroup:
+// lane tid takes the K consecutive sorted entries tid K .. tid K + K - 1,
+// looks up their records and cull boxes (all in flight at once), and one
+// packed scan per four quadrants numbers every entry's hits.
+// kWhole (c0 == beg, cn == n): writes point_list, inst_at and the quadrant
+// sub-lists, their lengths and the acceptance work items.  Otherwise (one
+// chunk of a larger tile): writes the sorted keys back in place, and per
+// entry (record, quadrant mask) to lmeta and, per quadrant, the number of
+// the chunk's earlier entries that hit it to lpre, for k_merge_chunks.
+constexpr int kSortThreads = 256;
+template <bool kWhole>
+SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint32_t c0, int cn,
+                         const uint64_t* s_keys) {
+  constexpr int K = kSmallMax / kSortThreads;
+  __shared__ uint64_t s_w[kSortThreads / 64];
+  const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
   const int ltile = tile - (int)vi.tile_base;
   const int tx = ltile % vi.gx, ty = ltile / vi.gx;
-  uint2* base = a.sub + (size_t)Q * beg;
-  if (tid < kMaxQuads) s_total[tid] = 0;
-  __syncthreads();
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  // register path with few keys per lane: every cull box in flight at once;
-  // with more, the next iteration's box is fetched one iteration ahead
-  constexpr bool kBoxes = kRegs && K <= 8;
-  short4 bxs[kBoxes ? K : 1];
-  if constexpr (kBoxes) {
+  const int tid = threadIdx.x;
+  uint64_t key[K];
+  uint32_t rid[K], hm[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-      bxs[k] = (tid + k * NT < n) ? a.cullbox[rid[k]] : make_short4(32767, 32767, -32768, -32768);
+  for (int k = 0; k < K; ++k) {
+    const int i = tid * K + k;
+    key[k] = i < cn ? s_keys[lds_slot<K>(i)] : ~0ull;
+    rid[k] = i < cn ? a.inst_rec[(uint32_t)key[k]] : 0u;
   }
-  short4 bx_next = make_short4(32767, 32767, -32768, -32768);
-  if constexpr (kRegs && !kBoxes) {
-    if (tid < n) bx_next = a.cullbox[rid[0]];
-  }
-  const int iters = kRegs ? K : (n + NT - 1) / NT;
+  short4 bx[K];
 #pragma unroll
-  for (int k = 0; k < iters; ++k) {
-    const int c = k * NT;
-    if (c >= n) break;  // uniform: c depends on k only
-    const int e = c + tid;
-    const bool valid = e < n;
-    uint32_t id = 0, inst = 0;
-    short4 bx = make_short4(32767, 32767, -32768, -32768);
-    if constexpr (kBoxes) {
-      id = rid[k];
-      inst = rin[k];
-      bx = bxs[k];
-    } else if constexpr (kRegs) {
-      id = rid[k];
-      inst = rin[k];
-      bx = bx_next;
-      if (k + 1 < K && e + NT < n) bx_next = a.cullbox[rid[k + 1 < K ? k + 1 : k]];
-    } else if (valid) {
-      id = a.point_list[beg + e];
-      inst = a.inst_at[beg + e];
-      bx = a.cullbox[id];
-    }
-    uint32_t hitmask = 0;
-    uint8_t rank[kMaxQuads];
+  for (int k = 0; k < K; ++k)
+    bx[k] = tid * K + k < cn ? a.cullbox[rid[k]] : make_short4(32767, 32767, -32768, -32768);
 #pragma unroll
-    for (int q = 0; q < kMaxQuads; ++q) {
-      rank[q] = 0;
-      if (q < Q) {
-        const int qx0 = tx * B + (q % QX) * kQuad, qy0 = ty * B + (q / QX) * kQuad;
-        const bool hit = valid && qx0 < vi.W && qy0 < vi.H && bx.x <= qx0 + kQuad - 1 && bx.z >= qx0 &&
-                         bx.y <= qy0 + kQuad - 1 && bx.w >= qy0;
-        const uint64_t m = __ballot(hit);
-        if (lane == 0) s_wcnt[q][wave] = (uint32_t)__popcll(m);
-        rank[q] = (uint8_t)__popcll(m & lt);
-        hitmask |= (uint32_t)hit << q;
+  for (int k = 0; k < K; ++k) {
+    const int i = tid * K + k;
+    if (i < cn) {
+      if constexpr (kWhole) {
+        a.point_list[c0 + i] = rid[k];
+        a.inst_at[c0 + i] = (uint32_t)key[k];
+      } else {
+        a.keys[c0 + i] = key[k];
       }
     }
-    if (valid) {  // for the gradient reduction: the instance's quadrants, their stored-sum flags cleared
-      a.inst_quads[inst] = (uint16_t)hitmask;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = tid * K + k;
+    hm[k] = quad_hits(bx[k], tx, ty, B, QX, Q, vi);
+    if (i < cn) {
+      // for the gradient reduction: the instance's quadrants, their
+      // stored-sum flags cleared
+      const uint32_t inst = (uint32_t)key[k];
+      a.inst_quads[inst] = (uint16_t)hm[k];
       if (Q == 4) {
         reinterpret_cast<uint32_t*>(a.part_nz)[inst] = 0u;
       } else {
         for (int q = 0; q < Q; ++q) a.part_nz[(size_t)inst * Q + q] = 0;
       }
-    }
-    __syncthreads();
-    // per quadrant, the waves' first slots: a scan over the waves, one lane
-    // per (quadrant, wave), each quadrant a kWaves-lane segment of one wave
-    if (tid < Q * kWaves) {
-      const int q = tid / kWaves, w = tid % kWaves;
-      const uint32_t c0 = s_wcnt[q][w];
-      uint32_t incl = c0;
-#pragma unroll
-      for (int d = 1; d < kWaves; d <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)incl, d, kWaves);
-        if (w >= d) incl += y;
-      }
-      const uint32_t t0 = s_total[q];
-      s_woff[q][w] = t0 + incl - c0;
-      if (w == kWaves - 1) s_total[q] = t0 + incl;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kMaxQuads; ++q) {
-      if (q < Q && ((hitmask >> q) & 1u))
-        base[(size_t)q * n + s_woff[q][wave] + rank[q]] = make_uint2(id, (uint32_t)e);
+      if constexpr (!kWhole) a.lmeta[c0 + i] = make_uint2(rid[k], hm[k]);
     }
   }
-  if (tid < Q) {
-    const uint32_t cnt = s_total[tid];
-    a.sub_count[(size_t)tile * Q + tid] = cnt;
-    a.q_flag[(size_t)tile * Q + tid] = 0u;
-    const uint32_t nch = (cnt + 63u) / 64u;
-    if (nch > 0) {
-      const uint32_t w0 = atomicAdd(a.work_count, nch);
-      for (uint32_t c = 0; c < nch; ++c) a.work[w0 + c] = make_uint2((uint32_t)(tile * Q + tid), c);
+  uint2* base = a.sub + (size_t)Q * beg;
+  for (int g = 0; g * 4 < Q; ++g) {  // quadrants 4g .. 4g + 3 (uniform loop)
+    uint64_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) cnt += spread4(hm[k], g);
+    uint64_t total;
+    uint64_t run = scan256_excl_u64(cnt, s_w, total);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = tid * K + k;
+      if (i < cn) {
+        if constexpr (kWhole) {
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            const int q = 4 * g + f;
+            if (q < Q && ((hm[k] >> q) & 1u))
+              base[(size_t)q * n + field16(run, f)] = make_uint2(rid[k], (uint32_t)i);
+          }
+        } else {
+          uint16_t* d = a.lpre + (size_t)(c0 + i) * Q + 4 * g;
+          if (Q >= 4) *reinterpret_cast<uint64_t*>(d) = run;
+          else *d = (uint16_t)run;
+        }
+      }
+      run += spread4(hm[k], g);
+    }
+    if constexpr (kWhole) {
+      if (tid >= 4 * g && tid < min(Q, 4 * g + 4)) {
+        const int q = tid;
+        const uint32_t c = field16(total, q - 4 * g);
+        a.sub_count[(size_t)tile * Q + q] = c;
+        a.q_flag[(size_t)tile * Q + q] = 0u;
+        const uint32_t nch = (c + 63u) / 64u;
+        if (nch > 0) {
+          const uint32_t w0 = atomicAdd(a.work_count, nch);
+          for (uint32_t ch = 0; ch < nch; ++ch) a.work[w0 + ch] = make_uint2((uint32_t)(tile * Q + q), ch);
+        }
+      }
     }
   }
 }
 
-// Sorts tile `tile` (n > 0 instances) and splits it into quadrant sub-lists.
-template <int NT, int CAP>
-SR_DEV void sort_tile(const SortArgs& a, int tile, uint64_t* s_keys) {
-  constexpr int K = CAP / NT;  // keys per lane in the LDS sort
-  const uint32_t beg = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
-  const int n = (int)(end - beg);
+// Loads bucket entries c0 .. c0 + cn - 1 (cn <= kSmallMax) into LDS, padded
+// to a power of two, and sorts them.
+SR_DEV void sort_run(const SortArgs& a, uint32_t c0, int cn, uint64_t* s_keys) {
+  constexpr int K = kSmallMax / kSortThreads;
   const int tid = threadIdx.x;
-  if (n <= CAP) {
-    const int S = max(next_pow2(n), CAP / NT);
+  const int S = max(next_pow2(cn), K);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {  // all K loads in flight at once
-      const int i = tid + k * NT;
-      if (i < S) s_keys[lds_slot<K>(i)] = i < n ? a.keys[beg + i] : ~0ull;
-    }
-    __syncthreads();
-    merge_sort_lds<NT, CAP / NT>(s_keys, S);
-    // the lane's records (every lookup in flight at once), kept in
-    // registers for the split
-    uint32_t rin[K], rid[K];
+  for (int k = 0; k < K; ++k) {  // all K loads in flight at once
+    const int i = tid + k * kSortThreads;
+    if (i < S) s_keys[lds_slot<K>(i)] = i < cn ? a.keys[c0 + i] : ~0ull;
+  }
+  __syncthreads();
+  merge_sort_lds<kSortThreads, K>(s_keys, S);
+}
+
+// Tiles with more than kSmallMax instances, in two passes over the scan's
+// list of them (up to max_large; chunk c of a tile is handled by row
+// c % kChunkRows of its workgroups).  The bucket is cut into chunks of
+// kSmallMax keys.  k_sort_tiles sorts each chunk in LDS (in place in
+// `keys`), looks up its records and their quadrants, and stores per entry
+// (record, quadrant mask) in lmeta and, per quadrant, the number of the
+// chunk's earlier entries that hit it (lpre).  k_merge_chunks then places
+// every entry: its position in the tile list is its rank in its own chunk
+// plus, for every other chunk, the number of that chunk's keys below it
+// (binary search, in LDS for tiles of up to kMergeLds keys); its position
+// in sub-list q is likewise its own lpre plus the other chunks' hits of q
+// below it.  Keys are unique, so this is the merge of the sorted chunks,
+// and the lists equal those of sorting the whole bucket at once.
+constexpr int kChunkRows = 8;
+constexpr int kMergeLds = 16384;
+// Workgroups per row for the large tiles (each strides over their list, so
+// any number of large tiles is covered; up to kLargeGroups of them each get
+// their own workgroups).
+constexpr uint32_t kLargeGroups = 1024;
+SR_DEV uint32_t large_groups(const SortArgs& a) { return min(a.max_large, kLargeGroups); }
+
+// Hits of quadrants 4g..4g+3 among the first j entries of a sorted chunk of
+// len entries starting at bucket entry c0 (j <= len), packed as by spread4.
+SR_DEV uint64_t chunk_hits_below(const SortArgs& a, uint32_t c0, int len, int j, int Q, int g) {
+  const int at = min(j, len - 1);
+  const uint16_t* p = a.lpre + (size_t)(c0 + (uint32_t)at) * Q + 4 * g;
+  uint64_t v = Q >= 4 ? *reinterpret_cast<const uint64_t*>(p) : (uint64_t)*p;
+  if (j == len) v += spread4(a.lmeta[c0 + (uint32_t)at].y, g);
+  return v;
+}
+
+// Places the entries of chunk c of a large tile (bucket at beg, n entries);
+// kLds: the tile's chunk-sorted keys are in s_tile, else read from `keys`.
+constexpr int kMergeThreads = 1024;
+template <bool kLds>
+SR_DEV void merge_chunk(const SortArgs& a, uint32_t beg, int n, int c, const uint64_t* s_tile) {
+  constexpr int E = kSmallMax / kMergeThreads;  // entries per lane
+  constexpr int kG = 8;                         // chunks searched together
+  const int P = (n + kSmallMax - 1) / kSmallMax;
+  const int Q = quads_per_tile(a.vs.B);
+  const int tid = threadIdx.x;
+  auto chunk_len = [&](int cc) { return min(kSmallMax, n - cc * kSmallMax); };
+  auto key_at = [&](int idx) -> uint64_t { return kLds ? s_tile[idx] : a.keys[beg + (uint32_t)idx]; };
+  const uint32_t c0 = beg + (uint32_t)c * kSmallMax;
+  const int cn = chunk_len(c);
+  uint64_t x[E];
+  uint32_t e[E], rq[E][kMaxQuads];
+  bool ok[E];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int i = tid + k * NT;
-      rin[k] = i < n ? (uint32_t)s_keys[lds_slot<K>(i)] : 0u;
-      rid[k] = i < n ? a.inst_rec[rin[k]] : 0u;
-    }
+  for (int r = 0; r < E; ++r) {
+    const int i = tid + r * kMergeThreads;
+    ok[r] = i < cn;
+    x[r] = key_at(c * kSmallMax + (ok[r] ? i : 0));
+    e[r] = (uint32_t)i;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int i = tid + k * NT;
-      if (i < n) {
-        a.inst_at[beg + i] = rin[k];
-        a.point_list[beg + i] = rid[k];
+    for (int g = 0; g < kMaxQuads / 4; ++g) {
+      if (g * 4 < Q) {
+        const uint64_t v = chunk_hits_below(a, c0, cn, ok[r] ? i : 0, Q, g);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) rq[r][4 * g + f] = field16(v, f);
       }
     }
-    split_quadrants<NT, K, true>(a, tile, beg, n, rid, rin);
-  } else {
-    uint64_t* src = a.keys + beg;
-    uint64_t* dst = a.keys_tmp + beg;
-    for (int c0 = 0; c0 < n; c0 += CAP) {
-      const int cn = min(CAP, n - c0);
-      const int S = max(next_pow2(cn), CAP / NT);
-      for (int i = tid; i < S; i += NT) s_keys[lds_slot<K>(i)] = i < cn ? src[c0 + i] : ~0ull;
-      __syncthreads();
-      merge_sort_lds<NT, CAP / NT>(s_keys, S);
-      for (int i = tid; i < cn; i += NT) src[c0 + i] = s_keys[lds_slot<K>(i)];
-      __syncthreads();
+  }
+  for (int cg = 0; cg < P; cg += kG) {
+    // lower bounds of x in chunks cg .. cg + kG - 1, searched together;
+    // every probe stays inside the tile's bucket
+    int pos[E][kG];
+#pragma unroll
+    for (int r = 0; r < E; ++r)
+#pragma unroll
+      for (int j = 0; j < kG; ++j) pos[r][j] = 0;
+    for (int step = kSmallMax; step > 0; step >>= 1) {
+      uint64_t probe[E][kG];
+#pragma unroll
+      for (int r = 0; r < E; ++r)
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+          const int cc = cg + j;
+          probe[r][j] = 0;
+          if (cc < P && cc != c) {  // uniform
+            const int p = min(pos[r][j] + step, chunk_len(cc));
+            probe[r][j] = key_at(cc * kSmallMax + p - 1);
+          }
+        }
+#pragma unroll
+      for (int r = 0; r < E; ++r)
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+          const int cc = cg + j;
+          if (cc < P && cc != c && pos[r][j] + step <= chunk_len(cc) && probe[r][j] < x[r]) pos[r][j] += step;
+        }
     }
-    for (int w = CAP; w < n; w <<= 1) {
-      for (int a0 = 0; a0 < n; a0 += 2 * w) {
-        const int na = min(w, n - a0);
-        const int nb = max(0, min(w, n - a0 - w));
-        const uint64_t* A = src + a0;
-        const uint64_t* Bv = src + a0 + na;
-        uint64_t* out = dst + a0;
-        const int total = na + nb;
-        const int per = (total + NT - 1) / NT;
-        const int k0 = min(total, tid * per), k1 = min(total, k0 + per);
-        if (k0 < k1) {
-          int i = co_rank(A, na, Bv, nb, k0);
-          int j = k0 - i;
-          for (int k = k0; k < k1; ++k) {
-            if (j >= nb || (i < na && A[i] < Bv[j])) out[k] = A[i++];
-            else out[k] = Bv[j++];
+#pragma unroll
+    for (int j = 0; j < kG; ++j) {
+      const int cc = cg + j;
+      if (cc >= P || cc == c) continue;
+      const uint32_t cc0 = beg + (uint32_t)cc * kSmallMax;
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        e[r] += (uint32_t)pos[r][j];
+#pragma unroll
+        for (int g = 0; g < kMaxQuads / 4; ++g) {
+          if (g * 4 < Q) {
+            const uint64_t v = chunk_hits_below(a, cc0, chunk_len(cc), pos[r][j], Q, g);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) rq[r][4 * g + f] += field16(v, f);
           }
         }
       }
-      __syncthreads();
-      uint64_t* t = src;
-      src = dst;
-      dst = t;
     }
-    for (int i = tid; i < n; i += NT) {
-      const uint32_t inst = (uint32_t)src[i];
-      a.inst_at[beg + i] = inst;
-      a.point_list[beg + i] = a.inst_rec[inst];
-    }
-    __syncthreads();
-    const uint32_t none[1] = {0u};
-    split_quadrants<NT, 1, false>(a, tile, beg, n, none, none);
+  }
+  uint2* base = a.sub + (size_t)Q * beg;
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    if (!ok[r]) continue;
+    const uint2 m = a.lmeta[c0 + tid + r * kMergeThreads];
+    a.point_list[beg + e[r]] = m.x;
+    a.inst_at[beg + e[r]] = (uint32_t)x[r];
+#pragma unroll
+    for (int q = 0; q < kMaxQuads; ++q)
+      if (q < Q && ((m.y >> q) & 1u)) base[(size_t)q * n + rq[r][q]] = make_uint2(m.x, e[r]);
   }
 }
 
-// Tiles with at most kSmallMax instances (empty ones included): one
-// 256-lane workgroup per tile of the grid.
-__global__ __launch_bounds__(256) void k_sort_small(SortArgs a) {
-  __shared__ uint64_t s_keys[kSmallMax + kSmallMax / (kSmallMax / 256)];
-  const int tile = blockIdx.x;
+__global__ __launch_bounds__(kMergeThreads) void k_merge_chunks(SortArgs a) {
+  __shared__ uint64_t s_tile[kMergeLds];
   if (a.counters[1] & kErrCapacity) return;
-  const int n = (int)(a.ranges[2 * tile + 1] - a.ranges[2 * tile]);
+  const int Q = quads_per_tile(a.vs.B);
+  const int tid = threadIdx.x;
+  for (uint32_t li = blockIdx.x; li < a.counters[3]; li += gridDim.x) {
+    const int tile = (int)a.large[li];
+    const uint32_t beg = a.ranges[2 * tile];
+    const int n = (int)(a.ranges[2 * tile + 1] - beg);
+    const int P = (n + kSmallMax - 1) / kSmallMax;
+    if ((int)blockIdx.y < P) {
+      if (n <= kMergeLds) {
+        for (int i = tid; i < n; i += kMergeThreads) s_tile[i] = a.keys[beg + i];
+        __syncthreads();
+        for (int c = blockIdx.y; c < P; c += kChunkRows) merge_chunk<true>(a, beg, n, c, s_tile);
+        __syncthreads();  // s_tile is refilled for the next tile
+      } else {
+        for (int c = blockIdx.y; c < P; c += kChunkRows) merge_chunk<false>(a, beg, n, c, s_tile);
+      }
+    }
+    // the tile's sub-list lengths and acceptance work items
+    if (blockIdx.y == 0 && tid < Q) {
+      const int q = tid;
+      uint32_t cnt = 0;
+      for (int cc = 0; cc < P; ++cc) {
+        const int len = min(kSmallMax, n - cc * kSmallMax);
+        cnt += field16(chunk_hits_below(a, beg + (uint32_t)cc * kSmallMax, len, len, Q, q / 4), q % 4);
+      }
+      a.sub_count[(size_t)tile * Q + q] = cnt;
+      a.q_flag[(size_t)tile * Q + q] = 0u;
+      const uint32_t nch = (cnt + 63u) / 64u;
+      if (nch > 0) {
+        const uint32_t w0 = atomicAdd(a.work_count, nch);
+        for (uint32_t ch = 0; ch < nch; ++ch) a.work[w0 + ch] = make_uint2((uint32_t)(tile * Q + q), ch);
+      }
+    }
+  }
+}
+
+// Every tile's sort in one launch of 256-lane workgroups.  The first
+// G * kChunkRows workgroups (G = large_groups) sort the chunks of the large
+// tiles (workgroup b: entries b % G + k G of their list, chunks b / G +
+// k kChunkRows), so they are dispatched first; workgroup G * kChunkRows + t
+// sorts tile t when it holds at most kSmallMax instances (and writes empty
+// sub-lists for an empty tile).
+__global__ __launch_bounds__(kSortThreads) void k_sort_tiles(SortArgs a) {
+  __shared__ uint64_t s_keys[kSmallMax + kSmallMax / (kSmallMax / kSortThreads)];
+  if (a.counters[1] & kErrCapacity) return;
+  const uint32_t G = large_groups(a), nchunk = G * kChunkRows;
+  if (blockIdx.x < nchunk) {
+    const uint32_t row = blockIdx.x / G;
+    for (uint32_t li = blockIdx.x % G; li < a.counters[3]; li += G) {
+      const int tile = (int)a.large[li];
+      const uint32_t beg = a.ranges[2 * tile];
+      const int n = (int)(a.ranges[2 * tile + 1] - beg);
+      for (int c = (int)row; c * kSmallMax < n; c += kChunkRows) {
+        const uint32_t c0 = beg + (uint32_t)c * kSmallMax;
+        const int cn = min(kSmallMax, n - c * kSmallMax);
+        sort_run(a, c0, cn, s_keys);
+        place_sorted<false>(a, tile, beg, n, c0, cn, s_keys);
+        __syncthreads();  // s_keys is refilled for the next chunk
+      }
+    }
+    return;
+  }
+  const int tile = (int)(blockIdx.x - nchunk);
+  const uint32_t beg = a.ranges[2 * tile];
+  const int n = (int)(a.ranges[2 * tile + 1] - beg);
   if (n > kSmallMax) return;
   if (n <= 0) {
     const int Q = quads_per_tile(a.vs.B);
@@ -816,63 +958,18 @@ __global__ __launch_bounds__(256) void k_sort_small(SortArgs a) {
     }
     return;
   }
-  sort_tile<256, kSmallMax>(a, tile, s_keys);
-}
-
-// Larger tiles: workgroup i sorts entry i of the scan's list of them, so
-// the large tiles take the first workgroups of the launch (dispatched first)
-// and the rest, one per remaining tile of the grid, exit at once.
-__global__ __launch_bounds__(1024) void k_sort_large(SortArgs a) {
-  __shared__ uint64_t s_keys[kLargeCap + kLargeCap / (kLargeCap / 1024)];
-  if (a.counters[1] & kErrCapacity) return;
-  if (blockIdx.x < a.counters[3]) sort_tile<1024, kLargeCap>(a, (int)a.large[blockIdx.x], s_keys);
-}
-
-// A second stream per (host thread, device) with two fork/join events, for
-// launches that may overlap: the small- and large-tile sorts touch disjoint
-// tiles, so the large tiles (the longest single sorts) run beside the small
-// ones instead of after them.  Default priority; created on first use.
-struct SideStream {
-  int device = -1;
-  hipStream_t s = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-};
-static hipError_t side_stream(SideStream*& out) {
-  thread_local SideStream cache[16];
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return e;
-  if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
-  SideStream& c = cache[dev];
-  if (c.s == nullptr) {
-    e = hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.join, hipEventDisableTiming);
-    if (e != hipSuccess) return e;
-    c.device = dev;
-  }
-  out = &c;
-  return hipSuccess;
+  sort_run(a, beg, n, s_keys);
+  place_sorted<true>(a, tile, beg, n, beg, n, s_keys);
 }
 
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
   if (a.vs.T == 0) return hipSuccess;
-  SideStream* side = nullptr;
-  hipError_t e = side_stream(side);
+  const uint32_t G = a.max_large < kLargeGroups ? a.max_large : kLargeGroups;
+  hipLaunchKernelGGL(k_sort_tiles, dim3(G * kChunkRows + a.vs.T), dim3(kSortThreads), 0, s, a);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // fork: the small-tile sort on the side stream once everything before it
-  // on s is done, the large-tile sort on s; join before anything after
-  e = hipEventRecord(side->fork, s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(side->s, side->fork, 0);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_sort_small, dim3(a.vs.T), dim3(256), 0, side->s, a);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_sort_large, dim3(a.vs.T), dim3(1024), 0, s, a);
-  e = hipGetLastError();
-  if (e == hipSuccess) e = hipEventRecord(side->join, side->s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(s, side->join, 0);
-  return e;
+  hipLaunchKernelGGL(k_merge_chunks, dim3(G, kChunkRows), dim3(kMergeThreads), 0, s, a);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

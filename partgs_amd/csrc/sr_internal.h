@@ -150,7 +150,7 @@ inline ImageLayout image_layout(int W, int H, int B) { return image_layout(singl
 //              record's contiguous run of them, so no gradient needs a
 //              global atomic
 struct BinLayout {
-  size_t keys, keys_tmp, point_list, sub, masks, chunk_T, work, frag_off, total;
+  size_t keys, lmeta, lpre, point_list, sub, masks, chunk_T, work, frag_off, total;
   size_t inst_rec, inst_at, inst_quads, part, part_nz;
   size_t prec, fW;
   size_t frag_cap;
@@ -193,7 +193,10 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
   const size_t n = (size_t)(L > 0 ? L : 1);
   const size_t Q = (size_t)quads_per_tile(B);
   b.keys = off; off += align_up(n * 8);
-  b.keys_tmp = off; off += align_up(n * 8);
+  // large-tile sort: per entry of a sorted 2048-key chunk, (record, quadrant
+  // mask) and, per quadrant, the chunk's hits before it (16-bit each)
+  b.lmeta = off; off += align_up(n * 8);
+  b.lpre = off; off += align_up(n * Q * 2);
   b.point_list = off; off += align_up(n * 4);
   b.sub = off; off += align_up(n * Q * 8);  // uint2 (surfel id, position in tile)
   b.masks = off; off += align_up((n * Q + (size_t)kQuad * kQuad * Q * (size_t)T) * 8);
@@ -262,13 +265,19 @@ struct ScatterArgs {
   uint64_t* keys;
 };
 
+// Tiles with at most kSmallMax instances are sorted by one 256-lane
+// workgroup each; larger ones in kSmallMax-key chunks that are then merged.
+constexpr int kSmallMax = 2048;
+
 struct SortArgs {
   ViewSet vs;
   const uint32_t* ranges;
   const uint32_t* counters;
   const uint32_t* large;  // tiles above the small-tile sort's size, counters[3] of them
+  uint32_t max_large;     // bound on counters[3] (grid size of the large-tile kernels)
   uint64_t* keys;
-  uint64_t* keys_tmp;
+  uint2* lmeta;
+  uint16_t* lpre;
   uint32_t* point_list;
   const short4* cullbox;
   const uint32_t* inst_rec;
