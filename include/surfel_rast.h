@@ -63,6 +63,16 @@ typedef struct {
   int tile;
   int pair_store;          /* accepted-pair store per tile instance (0 = 24);
                               chunks beyond it recompute their pairs */
+  int capacity;            /* tile instances the binning buffer is sized for.
+                              0: sized from the instance count, which the
+                              forward reads back before it binns (a stream
+                              synchronisation mid-forward, as the reference).
+                              > 0: the whole forward is enqueued at once and
+                              the count is read back after; more instances
+                              than this return SR_ERR_CAPACITY with
+                              *num_rendered set (call again with at least
+                              that).  The backward must be given the same
+                              value, as it locates the binning arrays by it. */
 } sr_settings;
 
 /* Per-surfel inputs (device).  Exactly one of shs / colors_precomp and exactly

@@ -78,6 +78,7 @@ class SrSettings(C.Structure):
         ("debug", C.c_int),
         ("tile", C.c_int),
         ("pair_store", C.c_int),
+        ("capacity", C.c_int),
     ]
 
 
@@ -189,6 +190,15 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_version.restype = C.c_char_p
         _lib = lib
         return lib
+
+
+class CapacityError(RuntimeError):
+    """More tile instances than the binning capacity the caller gave."""
+
+    def __init__(self, needed: int, capacity: int):
+        super().__init__(f"{needed} tile instances exceed the binning capacity {capacity}")
+        self.needed = needed
+        self.capacity = capacity
 
 
 def check(rc: int) -> None:

@@ -109,6 +109,13 @@ int check_stage(bool debug, hipStream_t s, const char* what) {
     if (rc_ != SR_OK) return rc_;                               \
   } while (0)
 
+// An event per host thread, marking the instance count's copy to the host.
+hipEvent_t counts_event() {
+  thread_local hipEvent_t e = nullptr;
+  if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+  return e;
+}
+
 // Pinned 8-byte landing zone for the instance count and error flags.
 uint32_t* pinned_counters() {
   thread_local uint32_t* p = nullptr;
@@ -509,6 +516,8 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
   if (g->P > 0 && (!radii || !g->opacities)) return fail(SR_ERR_INVALID, "radii and opacities must not be NULL");
   const int P = g->P, W = s->image_width, H = s->image_height, B = s->tile;
   const bool dbg = s->debug != 0;
+  const int cap = s->capacity;
+  if (cap < 0) return fail(SR_ERR_INVALID, "capacity must be >= 0 (got %d)", cap);
   *num_rendered = 0;
   if (P == 0) {
     // The reference glue skips the rasteriser for P == 0 and returns its
@@ -538,10 +547,37 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
   pl.img = img;
   pl.gl = gl;
   pl.il = il;
-  rc = run_front(pl, g, radii, 0xFFFFFFFFu, dbg, st);
-  if (rc != SR_OK) return rc;
   uint32_t* host = pinned_counters();
   if (!host) return fail(SR_ERR_HIP, "hipHostMalloc failed");
+  if (cap > 0) {
+    // Everything enqueued at once on a binning buffer of the caller's
+    // capacity; the kernels after the scan do nothing when the instances
+    // exceed it.  The instance count is copied back right after the scan
+    // and waited for only once the rest is enqueued, by when it has long
+    // arrived, so the stream never drains mid-forward.
+    pl.bl = bin_layout(cap, B, il.T, s->pair_store);
+    pl.bin = (char*)alloc(ctx, pl.bl.total, 1);
+    if (!pl.bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
+    pl.capacity = cap;
+    rc = run_front(pl, g, radii, (uint32_t)cap, dbg, st);
+    if (rc != SR_OK) return rc;
+    SR_HIP(hipMemcpyAsync(host, pl.counters(), 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    hipEvent_t counted = counts_event();
+    if (!counted) return fail(SR_ERR_HIP, "hipEventCreate failed");
+    SR_HIP(hipEventRecord(counted, st));
+    rc = run_binning(pl, radii, dbg, st);
+    if (rc == SR_OK) rc = run_blend_fwd(pl, out_color, out_others, dbg, st);
+    SR_HIP(hipEventSynchronize(counted));
+    if (rc != SR_OK) return rc;
+    if (host[1] & kErrPrefiltered)
+      return fail(SR_ERR_PREFILTERED, "Point is filtered although prefiltered is set. This shouldn't happen!");
+    *num_rendered = (int)host[0];
+    if (host[1] & kErrCapacity)
+      return fail(SR_ERR_CAPACITY, "%u tile instances exceed the capacity %d", host[0], cap);
+    return SR_OK;
+  }
+  rc = run_front(pl, g, radii, 0xFFFFFFFFu, dbg, st);
+  if (rc != SR_OK) return rc;
   SR_HIP(hipMemcpyAsync(host, pl.counters(), 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   SR_HIP(hipStreamSynchronize(st));
   if (host[1] & kErrPrefiltered)
@@ -592,11 +628,14 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
   pl.bg = s->bg;
   pl.gl = geom_layout((size_t)P);
   pl.il = image_layout(pl.vs);
-  pl.bl = bin_layout(R, B, pl.il.T, s->pair_store);
+  // the binning arrays are laid out for the forward's capacity (R when 0)
+  const int cap = s->capacity > 0 ? s->capacity : R;
+  if (cap < R) return fail(SR_ERR_INVALID, "capacity %d below num_rendered %d", s->capacity, R);
+  pl.bl = bin_layout(cap, B, pl.il.T, s->pair_store);
   pl.geom = (char*)geom_buf;
   pl.img = (char*)img_buf;
   pl.bin = (char*)bin_buf;
-  pl.capacity = R;
+  pl.capacity = cap;
   // With one view the per-record gradient arrays are the outputs themselves;
   // the backward writes every output once, so nothing is cleared first.
   RecordGrads rg{gr->dL_dtransMat, gr->dL_dmeans2D, gr->dL_dopacity, gr->dL_dcolors};

@@ -64,7 +64,7 @@ class _Arena:
 
 
 def _settings(bg, scale_modifier, viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height, image_width,
-              degree, campos, prefiltered, debug, tile, pair_store=0):
+              degree, campos, prefiltered, debug, tile, pair_store=0, capacity=0):
     s = _lib.SrSettings()
     s.image_height = int(image_height)
     s.image_width = int(image_width)
@@ -80,6 +80,7 @@ def _settings(bg, scale_modifier, viewmatrix, projmatrix, tan_fovx, tan_fovy, im
     s.debug = 1 if debug else 0
     s.tile = int(tile)
     s.pair_store = int(pair_store)
+    s.capacity = int(capacity)
     return s
 
 
@@ -99,11 +100,14 @@ def _surfels(P, means3D, sh, colors, opacity, scales, rotations, transMat_precom
 
 def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, scale_modifier, transMat_precomp,
                             viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height, image_width, sh, degree,
-                            campos, prefiltered, debug, tile: int = 16, pair_store: int = 0):
+                            campos, prefiltered, debug, tile: int = 16, pair_store: int = 0, capacity: int = 0):
     """RasterizeGaussiansCUDA (rasterize_points.cu:40-134).
 
     Returns (num_rendered, color (3,H,W), others (7,H,W), radii (P,) int32,
-    geomBuffer, binningBuffer, imgBuffer).
+    geomBuffer, binningBuffer, imgBuffer).  capacity > 0 sizes the binning
+    buffer for that many tile instances up front, so the stream is not
+    synchronised mid-forward; more instances raise CapacityError (whose
+    `needed` is the count).  The backward must be given the same capacity.
     """
     if means3D.ndimension() != 2 or means3D.size(1) != 3:
         raise RuntimeError("means3D must have dimensions (num_points, 3)")
@@ -122,7 +126,7 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
     others = torch.empty((7, H, W), dtype=torch.float32, device=dev)
     radii = torch.zeros((P,), dtype=torch.int32, device=dev)
     s = _settings(a["bg"], scale_modifier, a["viewmatrix"], a["projmatrix"], tan_fovx, tan_fovy, H, W, degree,
-                  a["campos"], prefiltered, debug, tile, pair_store)
+                  a["campos"], prefiltered, debug, tile, pair_store, capacity)
     g = _surfels(P, means3D, a["sh"], a["colors"], a["opacity"], a["scales"], a["rotations"], a["transMat_precomp"])
     arena = _Arena(dev)
     nr = C.c_int(0)
@@ -131,6 +135,8 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
                             _lib.ptr(radii), C.byref(nr), _lib.stream_handle(dev))
     if rc == _lib.SR_ERR_ALLOC and arena.error is not None:
         raise arena.error
+    if rc == _lib.SR_ERR_CAPACITY:
+        raise _lib.CapacityError(nr.value, capacity)
     _lib.check(rc)
     return nr.value, color, others, radii, arena.get(0), arena.get(1), arena.get(2)
 
@@ -138,7 +144,7 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
 def rasterize_gaussians_bwd(bg, means3D, radii, colors, scales, rotations, scale_modifier, transMat_precomp,
                             viewmatrix, projmatrix, tan_fovx, tan_fovy, dL_dout_color, dL_dout_others, sh, degree,
                             campos, geomBuffer, R, binningBuffer, imageBuffer, debug, tile: int = 16,
-                            pair_store: int = 0):
+                            pair_store: int = 0, capacity: int = 0):
     """RasterizeGaussiansBackwardCUDA (rasterize_points.cu:137-233).
 
     Returns (dL_dmeans2D, dL_dcolors, dL_dopacity, dL_dmeans3D, dL_dtransMat,
@@ -168,7 +174,7 @@ def rasterize_gaussians_bwd(bg, means3D, radii, colors, scales, rotations, scale
     dL_drotations = torch.empty((P, 4), **opts)
     if P != 0:
         s = _settings(a["bg"], scale_modifier, a["viewmatrix"], a["projmatrix"], tan_fovx, tan_fovy, H, W, degree,
-                      a["campos"], False, debug, tile, pair_store)
+                      a["campos"], False, debug, tile, pair_store, capacity)
         g = _surfels(P, means3D, a["sh"], a["colors"], torch.empty(0, **opts), a["scales"], a["rotations"],
                      a["transMat_precomp"])
         gr = _lib.SrGrads()
@@ -205,6 +211,18 @@ def mark_visible(means3D, viewmatrix, projmatrix):
     return present
 
 
+# Binning capacity per (device, tile): the last instance count with an
+# eighth of headroom.  The first forward of a key sizes its binning buffer
+# from the exact count (one stream synchronisation mid-forward, as the
+# reference); later ones enqueue the whole forward at once and only fall back
+# to a larger capacity when the count outgrows it.
+_capacity_hint = {}
+
+
+def _grown(n: int) -> int:
+    return n + n // 8 + 1024
+
+
 class _RasterizeGaussians(torch.autograd.Function):
     """Autograd wrapper, __init__.py:44-156."""
 
@@ -212,13 +230,27 @@ class _RasterizeGaussians(torch.autograd.Function):
     def forward(ctx, means3D, means2D, sh, colors_precomp, opacities, scales, rotations, cov3Ds_precomp,
                 raster_settings):
         tile = getattr(raster_settings, "tile", 16)
-        num_rendered, color, depth, radii, geomBuffer, binningBuffer, imgBuffer = rasterize_gaussians_fwd(
-            raster_settings.bg, means3D, colors_precomp, opacities, scales, rotations,
-            raster_settings.scale_modifier, cov3Ds_precomp, raster_settings.viewmatrix, raster_settings.projmatrix,
-            raster_settings.tanfovx, raster_settings.tanfovy, raster_settings.image_height,
-            raster_settings.image_width, sh, raster_settings.sh_degree, raster_settings.campos,
-            raster_settings.prefiltered, raster_settings.debug, tile)
+        key = (means3D.device, tile)
+        hint = 0 if raster_settings.debug else _capacity_hint.get(key, 0)
+        cap = hint
+        while True:
+            try:
+                num_rendered, color, depth, radii, geomBuffer, binningBuffer, imgBuffer = rasterize_gaussians_fwd(
+                    raster_settings.bg, means3D, colors_precomp, opacities, scales, rotations,
+                    raster_settings.scale_modifier, cov3Ds_precomp, raster_settings.viewmatrix,
+                    raster_settings.projmatrix, raster_settings.tanfovx, raster_settings.tanfovy,
+                    raster_settings.image_height, raster_settings.image_width, sh, raster_settings.sh_degree,
+                    raster_settings.campos, raster_settings.prefiltered, raster_settings.debug, tile,
+                    capacity=cap)
+                break
+            except _lib.CapacityError as e:
+                cap = _grown(e.needed)
+        # keep the hint tracking the count: set by an exact or a retried
+        # forward, shrunk when the count fell far below it
+        if cap != hint or cap == 0 or _grown(num_rendered) * 4 < cap:
+            _capacity_hint[key] = _grown(num_rendered)
         ctx.raster_settings = raster_settings
+        ctx.capacity = cap
         ctx.num_rendered = num_rendered
         ctx.save_for_backward(colors_precomp, means3D, scales, rotations, cov3Ds_precomp, radii, sh, geomBuffer,
                               binningBuffer, imgBuffer)
@@ -239,7 +271,8 @@ class _RasterizeGaussians(torch.autograd.Function):
          grad_rotations) = rasterize_gaussians_bwd(
             rs.bg, means3D, radii, colors_precomp, scales, rotations, rs.scale_modifier, cov3Ds_precomp,
             rs.viewmatrix, rs.projmatrix, rs.tanfovx, rs.tanfovy, grad_out_color, grad_depth, sh, rs.sh_degree,
-            rs.campos, geomBuffer, num_rendered, binningBuffer, imgBuffer, rs.debug, getattr(rs, "tile", 16))
+            rs.campos, geomBuffer, num_rendered, binningBuffer, imgBuffer, rs.debug, getattr(rs, "tile", 16),
+            capacity=ctx.capacity)
         return (grad_means3D, grad_means2D, grad_sh, grad_colors_precomp, grad_opacities, grad_scales,
                 grad_rotations, grad_cov3Ds_precomp, None)
 

@@ -14,7 +14,7 @@ REC_QUADS = 5
 
 
 def gpu_forward(scene, *, tile=16, scale_modifier=1.0, colors=None, transmat=None, prefiltered=False, debug=False,
-                pair_store=0):
+                pair_store=0, capacity=0):
     """Runs the HIP forward on a SurfelScene; returns a dict of outputs and state."""
     dev = torch.device("cuda")
     cam = scene.camera
@@ -28,10 +28,11 @@ def gpu_forward(scene, *, tile=16, scale_modifier=1.0, colors=None, transmat=Non
             cam.world_view_transform.to(dev), cam.full_proj_transform.to(dev), math.tan(cam.FoVx * 0.5),
             math.tan(cam.FoVy * 0.5), cam.image_height, cam.image_width, shs, scene.sh_degree,
             cam.camera_center.to(dev), prefiltered, debug, tile)
-    L, color, others, radii, geom, binb, img = rasterize_gaussians_fwd(*args, pair_store=pair_store)
+    L, color, others, radii, geom, binb, img = rasterize_gaussians_fwd(*args, pair_store=pair_store,
+                                                                       capacity=capacity)
     torch.cuda.synchronize()
     out = dict(L=L, color=color, others=others, radii=radii, geom=geom, bin=binb, img=img, args=args, tile=tile,
-               pair_store=pair_store)
+               pair_store=pair_store, capacity=capacity)
     return out
 
 
@@ -45,8 +46,8 @@ def read_state(res, scene):
     img = res["img"].cpu().numpy()
     binb = res["bin"].cpu().numpy()
 
-    def view(buf, name, dtype, count):
-        off = _lib.state_offset(name, P, W, H, B, L)
+    def view(buf, name, dtype, count):  # the binning arrays are laid out for the capacity
+        off = _lib.state_offset(name, P, W, H, B, res.get("capacity") or L)
         return np.frombuffer(buf, dtype=dtype, count=count, offset=off)
 
     st = {}
@@ -71,7 +72,8 @@ def gpu_backward(res, dL_dcolor, dL_dothers):
     dev = torch.device("cuda")
     out = rasterize_gaussians_bwd(a[0], a[1], res["radii"], a[2], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11],
                                   dL_dcolor.to(dev), dL_dothers.to(dev), a[14], a[15], a[16], res["geom"], res["L"],
-                                  res["bin"], res["img"], a[19], res["tile"], res["pair_store"])
+                                  res["bin"], res["img"], a[19], res["tile"], res["pair_store"],
+                                  res.get("capacity", 0))
     torch.cuda.synchronize()
     names = ("dL_dmeans2D", "dL_dcolors", "dL_dopacity", "dL_dmeans3D", "dL_dtransMat", "dL_dsh", "dL_dscales",
              "dL_drotations")

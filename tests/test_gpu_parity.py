@@ -208,6 +208,42 @@ def test_empty_and_culled():
     assert res["L"] == 0 == o.num_rendered
     _check_forward(scene, res, o)
     _check_backward(scene, res, o, seed=6)
+    # the same with the binning enqueued before the (zero) count is known
+    res = gpu_forward(scene, capacity=64)
+    assert res["L"] == 0
+    _check_forward(scene, res, o)
+    _check_backward(scene, res, o, seed=6)
+
+
+def test_capacity_mode_matches_exact():
+    """A binning capacity given up front (no synchronisation mid-forward)
+    gives the same instance count, state, images and gradients as sizing the
+    buffer from the count; too small a capacity raises CapacityError carrying
+    the count, and the exact count is enough."""
+    from partgs_amd._lib import CapacityError
+
+    scene = superquadric_surfels(4, P=20000, width=160, height=120)
+    H, W = scene.camera.image_height, scene.camera.image_width
+    gc, go = upstream_grads(4, H, W)
+    ref = gpu_forward(scene)
+    L = ref["L"]
+    print(f"[parity] capacity mode L={L}", flush=True)
+    g_ref = gpu_backward(ref, gc, go)
+    st_ref = read_state(ref, scene)
+    with pytest.raises(CapacityError) as ei:
+        gpu_forward(scene, capacity=L // 2)
+    assert ei.value.needed == L
+    for cap in (L, L + 1000):
+        res = gpu_forward(scene, capacity=cap)
+        assert res["L"] == L
+        np.testing.assert_array_equal(res["color"].cpu().numpy(), ref["color"].cpu().numpy())
+        np.testing.assert_array_equal(res["others"].cpu().numpy(), ref["others"].cpu().numpy())
+        st = read_state(res, scene)
+        for k in ("point_list", "ranges", "n_contrib", "final_T"):
+            np.testing.assert_array_equal(st[k], st_ref[k], err_msg=k)
+        g = gpu_backward(res, gc, go)
+        for k in g:
+            np.testing.assert_array_equal(g[k], g_ref[k], err_msg=k)
 
 
 def test_prefiltered_violation_raises():
@@ -247,5 +283,24 @@ def test_autograd_api_matches_low_level():
     for t, name in ((means3D, "dL_dmeans3D"), (opac, "dL_dopacity"), (scales, "dL_dscales"),
                     (rots, "dL_drotations"), (means2D, "dL_dmeans2D")):
         assert_close_scaled(t.grad.cpu().numpy(), g[name], 1e-5, name)
+    # later forwards reuse the binning capacity of earlier ones (no
+    # synchronisation mid-forward), growing it when outgrown: same results
+    from partgs_amd import rasterizer as R
+    key = (means3D.device, 16)
+    for hint in (None, 1):
+        if hint is not None:
+            R._capacity_hint[key] = hint  # far too small: the forward retries
+        leaves = (means3D, means2D, shs, opac, scales, rots)
+        for t in leaves:
+            t.grad = None
+        color2, _, allmap2 = GaussianRasterizer(rs)(means3D=means3D, means2D=means2D, shs=shs, opacities=opac,
+                                                    scales=scales, rotations=rots)
+        ((color2 * gc.cuda()).sum() + (allmap2 * go.cuda()).sum()).backward()
+        assert R._capacity_hint[key] >= res["L"]
+        np.testing.assert_array_equal(color2.detach().cpu().numpy(), res["color"].cpu().numpy())
+        np.testing.assert_array_equal(allmap2.detach().cpu().numpy(), res["others"].cpu().numpy())
+        for t, name in ((means3D, "dL_dmeans3D"), (opac, "dL_dopacity"), (scales, "dL_dscales"),
+                        (rots, "dL_drotations"), (means2D, "dL_dmeans2D")):
+            assert_close_scaled(t.grad.cpu().numpy(), g[name], 1e-5, name)
     vis = GaussianRasterizer(rs).markVisible(scene.means3D)
     assert bool(vis.all())
