@@ -271,8 +271,10 @@ Pipeline batch_pipeline(const sr_views* v, const sr_surfels* g, const ViewSet& v
 int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, bool dbg, hipStream_t st) {
   uint32_t* counters = pl.counters();
   uint32_t* tile_count = (uint32_t*)(pl.img + pl.il.tile_count);
-  SR_STAGE(1, dbg, st, "clear", launch_fill_u32(counters, 0u, kAlign / 4, st));
-  SR_STAGE(1, dbg, st, "clear", launch_fill_u32(tile_count, 0u, (size_t)pl.il.T, st));
+  // the counters and the tile histogram are adjacent: one clear for both
+  static_assert(kAlign % 4 == 0, "counters region in whole words");
+  if (tile_count != counters + kAlign / 4) return fail(SR_ERR_INVALID, "image layout: histogram not after counters");
+  SR_STAGE(1, dbg, st, "clear", launch_fill_u32(counters, 0u, kAlign / 4 + (size_t)pl.il.T, st));
   PreprocessArgs pa{};
   pa.vs = pl.vs;
   pa.P = pl.P;

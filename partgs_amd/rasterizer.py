@@ -124,7 +124,7 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
     H, W = int(image_height), int(image_width)
     color = torch.empty((3, H, W), dtype=torch.float32, device=dev)
     others = torch.empty((7, H, W), dtype=torch.float32, device=dev)
-    radii = torch.zeros((P,), dtype=torch.int32, device=dev)
+    radii = torch.empty((P,), dtype=torch.int32, device=dev)  # the preprocess writes every entry
     s = _settings(a["bg"], scale_modifier, a["viewmatrix"], a["projmatrix"], tan_fovx, tan_fovy, H, W, degree,
                   a["campos"], prefiltered, debug, tile, pair_store, capacity)
     g = _surfels(P, means3D, a["sh"], a["colors"], a["opacity"], a["scales"], a["rotations"], a["transMat_precomp"])
@@ -255,6 +255,9 @@ class _RasterizeGaussians(torch.autograd.Function):
         ctx.save_for_backward(colors_precomp, means3D, scales, rotations, cov3Ds_precomp, radii, sh, geomBuffer,
                               binningBuffer, imgBuffer)
         ctx.mark_non_differentiable(radii)
+        # outputs without an incoming gradient arrive as None (the backward
+        # fills zeros where it needs them) instead of zero tensors made here
+        ctx.set_materialize_grads(False)
         return color, radii, depth
 
     @staticmethod
