@@ -169,6 +169,7 @@ struct Pipeline {
   ImageLayout il{};
   BinLayout bl{};
   int capacity = 0;
+  bool side_huge = false;  // huge-tile sort on a second stream (graph-captured batches)
   uint32_t* counters() const { return (uint32_t*)(img + il.counters); }
 };
 
@@ -306,7 +307,8 @@ int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, 
            launch_scan_tiles(pl.il.T, tile_count, (uint32_t*)(pl.img + pl.il.tile_start),
                              (uint32_t*)(pl.img + pl.il.tile_fill), (uint32_t*)(pl.img + pl.il.ranges), counters,
                              capacity, pl.vs.V * ((pl.P + 255) / 256), (uint32_t*)(pl.geom + pl.gl.bsum),
-                             (uint32_t*)(pl.img + pl.il.large), st));
+                             (uint32_t*)(pl.img + pl.il.large), (uint32_t*)(pl.img + pl.il.lchunk_off),
+                             pl.side_huge ? kSmallMax : kLargeMax, st));
   return SR_OK;
 }
 
@@ -337,11 +339,8 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   sa.keys = (uint64_t*)(bin + bl.keys);
   sa.lmeta = (uint2*)(bin + bl.lmeta);
   sa.lpre = (uint16_t*)(bin + bl.lpre);
-  // a large tile holds more than kSmallMax of the capacity's instances
-  {
-    const size_t bound = (size_t)pl.capacity / (kSmallMax + 1) + 1;
-    sa.max_large = (uint32_t)(bound < (size_t)pl.il.T ? bound : (size_t)pl.il.T);
-  }
+  sa.lchunk_off = (const uint32_t*)(pl.img + pl.il.lchunk_off);
+  sa.side_huge = pl.side_huge ? 1 : 0;
   sa.point_list = (uint32_t*)(bin + bl.point_list);
   sa.cullbox = (const short4*)(pl.geom + pl.gl.cullbox);
   sa.inst_rec = (const uint32_t*)(bin + bl.inst_rec);
@@ -662,6 +661,13 @@ int sr_forward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t 
   if (ws_bytes < bl.total)
     return fail(SR_ERR_INVALID, "workspace holds %zu bytes, needs %zu", ws_bytes, bl.total);
   Pipeline pl = batch_pipeline(v, g, vs, bl, (char*)ws, capacity);
+  // Many views hold thousands of tiles above kSmallMax instances: sorting
+  // them in chunks and merging by rank does more work in all than one
+  // workgroup per tile, and throughput is what counts here (latency of the
+  // few largest tiles is what counts for one view).  So every tile above
+  // kSmallMax takes the one-workgroup sort, beside the small-tile sort on a
+  // second stream (a captured graph makes the fork cheap).
+  pl.side_huge = true;
   rc = run_front(pl, g, radii, (uint32_t)capacity, false, st);
   if (rc == SR_OK) rc = run_binning(pl, radii, false, st);
   if (rc == SR_OK) rc = run_blend_fwd(pl, out_color, out_others, false, st);

@@ -336,7 +336,10 @@ SR_DEV uint32_t scan1024_excl(uint32_t* s, uint32_t x) {
 }
 
 // Exclusive scan of the tile histogram by one 1024-lane workgroup, with the
-// list of tiles too large for the small-tile sort (count in counters[3]);
+// list of tiles of kSmallMax < n <= large_max (count in counters[3]) and the
+// number of each one's first kSmallMax-key chunk (lchunk_off, followed by
+// the total, which also goes to counters[6]), and the tiles above large_max
+// from the end of the same list (count in counters[7]);
 // then the same scan for the per-workgroup instance totals of the
 // preprocess (in place), which numbers every record's instances in record
 // order.
@@ -345,7 +348,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
                                                               uint32_t* start, uint32_t* fill,
                                                               uint32_t* ranges, uint32_t* counters,
                                                               uint32_t capacity, int nb, uint32_t* bsum,
-                                                              uint32_t* large) {
+                                                              uint32_t* large, uint32_t* lchunk_off,
+                                                              int large_max) {
   __shared__ uint32_t s_part[kScanThreads];
   const int tid = threadIdx.x;
   const int per = (T + kScanThreads - 1) / kScanThreads;
@@ -377,12 +381,34 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
     if (s_part[kScanThreads - 1] > capacity) atomicOr(&counters[1], kErrCapacity);
   }
   {
-    uint32_t nl = 0;
-    for (int i = b; i < e; ++i) nl += count[i] > (uint32_t)kSmallMax ? 1u : 0u;
+    auto chunks = [](uint32_t c) { return (c + (uint32_t)kSmallMax - 1u) / (uint32_t)kSmallMax; };
+    auto is_large = [&](uint32_t c) { return c > (uint32_t)kSmallMax && c <= (uint32_t)large_max; };
+    uint32_t nl = 0, nc = 0, nh = 0;
+    for (int i = b; i < e; ++i) {
+      if (is_large(count[i])) {
+        nl += 1u;
+        nc += chunks(count[i]);
+      }
+      nh += count[i] > (uint32_t)large_max ? 1u : 0u;
+    }
     uint32_t at = scan1024_excl(s_part, nl);
-    for (int i = b; i < e; ++i)
-      if (count[i] > (uint32_t)kSmallMax) large[at++] = (uint32_t)i;
-    if (tid == kScanThreads - 1) counters[3] = at;
+    uint32_t cat = scan1024_excl(s_part, nc);
+    uint32_t hat = scan1024_excl(s_part, nh);
+    for (int i = b; i < e; ++i) {
+      if (is_large(count[i])) {
+        large[at] = (uint32_t)i;
+        lchunk_off[at] = cat;
+        ++at;
+        cat += chunks(count[i]);
+      }
+      if (count[i] > (uint32_t)large_max) large[T - 1 - (int)hat++] = (uint32_t)i;
+    }
+    if (tid == kScanThreads - 1) {
+      counters[3] = at;
+      counters[6] = cat;
+      counters[7] = hat;
+      lchunk_off[at] = cat;
+    }
   }
   __syncthreads();
   const int pb = (nb + kScanThreads - 1) / kScanThreads;
@@ -407,9 +433,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
 
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
                              uint32_t* ranges, uint32_t* counters, uint32_t capacity, int nb, uint32_t* bsum,
-                             uint32_t* large, hipStream_t s) {
+                             uint32_t* large, uint32_t* lchunk_off, int large_max, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, T, count, start, fill, ranges,
-                     counters, capacity, nb, bsum, large);
+                     counters, capacity, nb, bsum, large, lchunk_off, large_max);
   return hipGetLastError();
 }
 
@@ -757,26 +783,33 @@ SR_DEV void sort_run(const SortArgs& a, uint32_t c0, int cn, uint64_t* s_keys) {
   merge_sort_lds<kSortThreads, K>(s_keys, S);
 }
 
-// Tiles with more than kSmallMax instances, in two passes over the scan's
-// list of them (up to max_large; chunk c of a tile is handled by row
-// c % kChunkRows of its workgroups).  The bucket is cut into chunks of
-// kSmallMax keys.  k_sort_tiles sorts each chunk in LDS (in place in
+// Tiles with more than kSmallMax instances, in two passes over the chunks
+// of the scan's list of them: chunk item i (of counters[6]) is chunk
+// i - lchunk_off[l] of large tile l, the last with lchunk_off[l] <= i; a
+// fixed number of workgroups stride over the items.  The bucket is cut into
+// chunks of kSmallMax keys.  k_sort_tiles sorts each chunk in LDS (in place in
 // `keys`), looks up its records and their quadrants, and stores per entry
 // (record, quadrant mask) in lmeta and, per quadrant, the number of the
 // chunk's earlier entries that hit it (lpre).  k_merge_chunks then places
 // every entry: its position in the tile list is its rank in its own chunk
 // plus, for every other chunk, the number of that chunk's keys below it
-// (binary search, in LDS for tiles of up to kMergeLds keys); its position
+// (binary search in LDS); its position
 // in sub-list q is likewise its own lpre plus the other chunks' hits of q
 // below it.  Keys are unique, so this is the merge of the sorted chunks,
 // and the lists equal those of sorting the whole bucket at once.
-constexpr int kChunkRows = 8;
-constexpr int kMergeLds = 16384;
-// Workgroups per row for the large tiles (each strides over their list, so
-// any number of large tiles is covered; up to kLargeGroups of them each get
-// their own workgroups).
-constexpr uint32_t kLargeGroups = 1024;
-SR_DEV uint32_t large_groups(const SortArgs& a) { return min(a.max_large, kLargeGroups); }
+constexpr uint32_t kChunkBlocks = 256;  // workgroups striding over the chunk items, per pass
+
+// Large tile and chunk of chunk item `it` (< counters[6]).
+SR_DEV void chunk_item(const SortArgs& a, uint32_t it, uint32_t& li, int& c) {
+  uint32_t lo = 0, hi = a.counters[3];  // lchunk_off[lo] <= it < lchunk_off[hi]
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a.lchunk_off[mid] <= it) lo = mid;
+    else hi = mid;
+  }
+  li = lo;
+  c = (int)(it - a.lchunk_off[lo]);
+}
 
 // Hits of quadrants 4g..4g+3 among the first j entries of a sorted chunk of
 // len entries starting at bucket entry c0 (j <= len), packed as by spread4.
@@ -788,10 +821,9 @@ SR_DEV uint64_t chunk_hits_below(const SortArgs& a, uint32_t c0, int len, int j,
   return v;
 }
 
-// Places the entries of chunk c of a large tile (bucket at beg, n entries);
-// kLds: the tile's chunk-sorted keys are in s_tile, else read from `keys`.
+// Places the entries of chunk c of a large tile (bucket at beg, n entries),
+// whose chunk-sorted keys are in s_tile.
 constexpr int kMergeThreads = 1024;
-template <bool kLds>
 SR_DEV void merge_chunk(const SortArgs& a, uint32_t beg, int n, int c, const uint64_t* s_tile) {
   constexpr int E = kSmallMax / kMergeThreads;  // entries per lane
   constexpr int kG = 8;                         // chunks searched together
@@ -799,7 +831,7 @@ SR_DEV void merge_chunk(const SortArgs& a, uint32_t beg, int n, int c, const uin
   const int Q = quads_per_tile(a.vs.B);
   const int tid = threadIdx.x;
   auto chunk_len = [&](int cc) { return min(kSmallMax, n - cc * kSmallMax); };
-  auto key_at = [&](int idx) -> uint64_t { return kLds ? s_tile[idx] : a.keys[beg + (uint32_t)idx]; };
+  auto key_at = [&](int idx) -> uint64_t { return s_tile[idx]; };
   const uint32_t c0 = beg + (uint32_t)c * kSmallMax;
   const int cn = chunk_len(c);
   uint64_t x[E];
@@ -881,28 +913,174 @@ SR_DEV void merge_chunk(const SortArgs& a, uint32_t beg, int n, int c, const uin
   }
 }
 
+// Number of elements of A among the first k of merge(A, B) (keys unique).
+SR_DEV int co_rank(const uint64_t* A, int na, const uint64_t* Bv, int nb, int k) {
+  int lo = max(0, k - nb), hi = min(k, na);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (A[mid] < Bv[k - mid - 1]) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// A tile above the chunk path's limit (kLargeMax instances: far denser than
+// a 16 x 16 tile of a typical view; in batches every tile above kSmallMax),
+// by one 1024-lane workgroup: kLargeMax-key chunks sorted in
+// LDS (16 keys per lane), merged pairwise in global memory (merge path; the tile's own lmeta
+// entries, unused for such a tile, are the second buffer), then split into
+// the quadrant sub-lists in rounds of 1024 entries (per-wave ballots and a
+// scan over the waves).  O(n log n), where ranking against every chunk
+// would be O(n^2 / kSmallMax).
+SR_DEV void sort_huge_tile(const SortArgs& a, int tile, uint64_t* s_keys) {
+  constexpr int NT = kMergeThreads, K = kLargeMax / NT, kChunk = kLargeMax, kWaves = NT / 64;
+  __shared__ uint32_t s_wcnt[kMaxQuads][kWaves];
+  __shared__ uint32_t s_woff[kMaxQuads][kWaves];
+  __shared__ uint32_t s_total[kMaxQuads];
+  const uint32_t beg = a.ranges[2 * tile];
+  const int n = (int)(a.ranges[2 * tile + 1] - beg);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint64_t* src = a.keys + beg;
+  uint64_t* dst = reinterpret_cast<uint64_t*>(a.lmeta + beg);
+  for (int c0 = 0; c0 < n; c0 += kChunk) {
+    const int cn = min(kChunk, n - c0);
+    const int S = max(next_pow2(cn), K);
+    __syncthreads();
+    for (int i = tid; i < S; i += NT) s_keys[lds_slot<K>(i)] = i < cn ? src[c0 + i] : ~0ull;
+    __syncthreads();
+    merge_sort_lds<NT, K>(s_keys, S);
+    for (int i = tid; i < cn; i += NT) src[c0 + i] = s_keys[lds_slot<K>(i)];
+  }
+  __syncthreads();
+  for (int w = kChunk; w < n; w <<= 1) {
+    for (int a0 = 0; a0 < n; a0 += 2 * w) {
+      const int na = min(w, n - a0);
+      const int nb = max(0, min(w, n - a0 - w));
+      const uint64_t* A = src + a0;
+      const uint64_t* Bv = src + a0 + na;
+      uint64_t* out = dst + a0;
+      const int total = na + nb;
+      const int per = (total + NT - 1) / NT;
+      const int k0 = min(total, tid * per), k1 = min(total, k0 + per);
+      if (k0 < k1) {
+        int i = co_rank(A, na, Bv, nb, k0);
+        int j = k0 - i;
+        for (int k = k0; k < k1; ++k) {
+          if (j >= nb || (i < na && A[i] < Bv[j])) out[k] = A[i++];
+          else out[k] = Bv[j++];
+        }
+      }
+    }
+    __syncthreads();
+    uint64_t* t = src;
+    src = dst;
+    dst = t;
+  }
+  for (int i = tid; i < n; i += NT) {
+    const uint32_t inst = (uint32_t)src[i];
+    a.inst_at[beg + i] = inst;
+    a.point_list[beg + i] = a.inst_rec[inst];
+  }
+  const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
+  const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
+  const int ltile = tile - (int)vi.tile_base;
+  const int tx = ltile % vi.gx, ty = ltile / vi.gx;
+  uint2* base = a.sub + (size_t)Q * beg;
+  if (tid < kMaxQuads) s_total[tid] = 0;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // the next round's record and cull box are fetched a round ahead
+  const short4 kNone = make_short4(32767, 32767, -32768, -32768);
+  uint32_t id_next = tid < n ? a.point_list[beg + tid] : 0u;
+  short4 bx_next = tid < n ? a.cullbox[id_next] : kNone;
+  for (int c = 0; c < n; c += NT) {  // uniform
+    const int e = c + tid;
+    const bool valid = e < n;
+    const uint32_t id = id_next;
+    const uint32_t inst = valid ? a.inst_at[beg + e] : 0u;
+    const short4 bx = bx_next;
+    if (e + NT < n) {
+      id_next = a.point_list[beg + e + NT];
+      bx_next = a.cullbox[id_next];
+    }
+    const uint32_t hm = valid ? quad_hits(bx, tx, ty, B, QX, Q, vi) : 0u;
+#pragma unroll
+    for (int q = 0; q < kMaxQuads; ++q) {
+      if (q < Q) {
+        const uint64_t m = __ballot((hm >> q) & 1u);
+        if (lane == 0) s_wcnt[q][wave] = (uint32_t)__popcll(m);
+      }
+    }
+    if (valid) {
+      a.inst_quads[inst] = (uint16_t)hm;
+      for (int q = 0; q < Q; ++q) a.part_nz[(size_t)inst * Q + q] = 0;
+    }
+    __syncthreads();
+    if (tid < Q * kWaves) {  // per quadrant, the waves' first slots
+      const int q = tid / kWaves, w = tid % kWaves;
+      const uint32_t c0 = s_wcnt[q][w];
+      uint32_t incl = c0;
+#pragma unroll
+      for (int d = 1; d < kWaves; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, d, kWaves);
+        if (w >= d) incl += y;
+      }
+      const uint32_t t0 = s_total[q];
+      s_woff[q][w] = t0 + incl - c0;
+      if (w == kWaves - 1) s_total[q] = t0 + incl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMaxQuads; ++q) {  // the ballots again: each lane's rank within its wave
+      if (q < Q) {
+        const uint64_t m = __ballot((hm >> q) & 1u);
+        if ((hm >> q) & 1u) base[(size_t)q * n + s_woff[q][wave] + (uint32_t)__popcll(m & lt)] = make_uint2(id, (uint32_t)e);
+      }
+    }
+  }
+  if (tid < Q) {
+    const uint32_t cnt = s_total[tid];
+    a.sub_count[(size_t)tile * Q + tid] = cnt;
+    a.q_flag[(size_t)tile * Q + tid] = 0u;
+    const uint32_t nch = (cnt + 63u) / 64u;
+    if (nch > 0) {
+      const uint32_t w0 = atomicAdd(a.work_count, nch);
+      for (uint32_t ch = 0; ch < nch; ++ch) a.work[w0 + ch] = make_uint2((uint32_t)(tile * Q + tid), ch);
+    }
+  }
+}
+
+// The counters[7] huge tiles, one workgroup each (striding).
+__global__ __launch_bounds__(kMergeThreads) void k_sort_huge(SortArgs a) {
+  __shared__ uint64_t s_keys[kLargeMax + kLargeMax / (kLargeMax / kMergeThreads)];
+  if (a.counters[1] & kErrCapacity) return;
+  for (uint32_t h = blockIdx.x; h < a.counters[7]; h += gridDim.x) {
+    __syncthreads();  // s_keys may still be read for the previous tile
+    sort_huge_tile(a, (int)a.large[a.vs.T - 1 - h], s_keys);
+  }
+}
+
+// The counters[6] chunk items of the large tiles, one workgroup each
+// (striding).
 __global__ __launch_bounds__(kMergeThreads) void k_merge_chunks(SortArgs a) {
-  __shared__ uint64_t s_tile[kMergeLds];
+  __shared__ uint64_t s_tile[kLargeMax];
   if (a.counters[1] & kErrCapacity) return;
   const int Q = quads_per_tile(a.vs.B);
   const int tid = threadIdx.x;
-  for (uint32_t li = blockIdx.x; li < a.counters[3]; li += gridDim.x) {
+  for (uint32_t it = blockIdx.x; it < a.counters[6]; it += gridDim.x) {
+    __syncthreads();  // s_tile may still be read for the previous item
+    uint32_t li;
+    int c;
+    chunk_item(a, it, li, c);
     const int tile = (int)a.large[li];
     const uint32_t beg = a.ranges[2 * tile];
     const int n = (int)(a.ranges[2 * tile + 1] - beg);
     const int P = (n + kSmallMax - 1) / kSmallMax;
-    if ((int)blockIdx.y < P) {
-      if (n <= kMergeLds) {
-        for (int i = tid; i < n; i += kMergeThreads) s_tile[i] = a.keys[beg + i];
-        __syncthreads();
-        for (int c = blockIdx.y; c < P; c += kChunkRows) merge_chunk<true>(a, beg, n, c, s_tile);
-        __syncthreads();  // s_tile is refilled for the next tile
-      } else {
-        for (int c = blockIdx.y; c < P; c += kChunkRows) merge_chunk<false>(a, beg, n, c, s_tile);
-      }
-    }
-    // the tile's sub-list lengths and acceptance work items
-    if (blockIdx.y == 0 && tid < Q) {
+    for (int i = tid; i < n; i += kMergeThreads) s_tile[i] = a.keys[beg + i];
+    __syncthreads();
+    merge_chunk(a, beg, n, c, s_tile);
+    // the tile's sub-list lengths and acceptance work items, by its chunk 0
+    if (c == 0 && tid < Q) {
       const int q = tid;
       uint32_t cnt = 0;
       for (int cc = 0; cc < P; ++cc) {
@@ -921,32 +1099,30 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge_chunks(SortArgs a) {
 }
 
 // Every tile's sort in one launch of 256-lane workgroups.  The first
-// G * kChunkRows workgroups (G = large_groups) sort the chunks of the large
-// tiles (workgroup b: entries b % G + k G of their list, chunks b / G +
-// k kChunkRows), so they are dispatched first; workgroup G * kChunkRows + t
-// sorts tile t when it holds at most kSmallMax instances (and writes empty
-// sub-lists for an empty tile).
+// kChunkBlocks workgroups sort the chunks of the large tiles (workgroup b:
+// chunk items b + k kChunkBlocks), so they are dispatched first; workgroup
+// kChunkBlocks + t sorts tile t when it holds at most kSmallMax instances
+// (and writes empty sub-lists for an empty tile).
 __global__ __launch_bounds__(kSortThreads) void k_sort_tiles(SortArgs a) {
   __shared__ uint64_t s_keys[kSmallMax + kSmallMax / (kSmallMax / kSortThreads)];
   if (a.counters[1] & kErrCapacity) return;
-  const uint32_t G = large_groups(a), nchunk = G * kChunkRows;
-  if (blockIdx.x < nchunk) {
-    const uint32_t row = blockIdx.x / G;
-    for (uint32_t li = blockIdx.x % G; li < a.counters[3]; li += G) {
+  if (blockIdx.x < kChunkBlocks) {
+    for (uint32_t it = blockIdx.x; it < a.counters[6]; it += kChunkBlocks) {
+      uint32_t li;
+      int c;
+      chunk_item(a, it, li, c);
       const int tile = (int)a.large[li];
       const uint32_t beg = a.ranges[2 * tile];
       const int n = (int)(a.ranges[2 * tile + 1] - beg);
-      for (int c = (int)row; c * kSmallMax < n; c += kChunkRows) {
-        const uint32_t c0 = beg + (uint32_t)c * kSmallMax;
-        const int cn = min(kSmallMax, n - c * kSmallMax);
-        sort_run(a, c0, cn, s_keys);
-        place_sorted<false>(a, tile, beg, n, c0, cn, s_keys);
-        __syncthreads();  // s_keys is refilled for the next chunk
-      }
+      const uint32_t c0 = beg + (uint32_t)c * kSmallMax;
+      const int cn = min(kSmallMax, n - c * kSmallMax);
+      sort_run(a, c0, cn, s_keys);
+      place_sorted<false>(a, tile, beg, n, c0, cn, s_keys);
+      __syncthreads();  // s_keys is refilled for the next chunk
     }
     return;
   }
-  const int tile = (int)(blockIdx.x - nchunk);
+  const int tile = (int)(blockIdx.x - kChunkBlocks);
   const uint32_t beg = a.ranges[2 * tile];
   const int n = (int)(a.ranges[2 * tile + 1] - beg);
   if (n > kSmallMax) return;
@@ -962,14 +1138,57 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tiles(SortArgs a) {
   place_sorted<true>(a, tile, beg, n, beg, n, s_keys);
 }
 
+// A second stream per (host thread, device) with fork/join events, for the
+// huge-tile sort beside the others.  Default priority; created on first use.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+static hipError_t side_stream(SideStream*& out) {
+  thread_local SideStream cache[16];
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
+  SideStream& c = cache[dev];
+  if (c.s == nullptr) {
+    e = hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c.join, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+  }
+  out = &c;
+  return hipSuccess;
+}
+
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
   if (a.vs.T == 0) return hipSuccess;
-  const uint32_t G = a.max_large < kLargeGroups ? a.max_large : kLargeGroups;
-  hipLaunchKernelGGL(k_sort_tiles, dim3(G * kChunkRows + a.vs.T), dim3(kSortThreads), 0, s, a);
-  hipError_t e = hipGetLastError();
+  hipError_t e = hipSuccess;
+  SideStream* side = nullptr;
+  if (a.side_huge) {  // fork before the tile sort, so the huge-tile sort does not wait for it
+    e = side_stream(side);
+    if (e == hipSuccess) e = hipEventRecord(side->fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side->s, side->fork, 0);
+    if (e != hipSuccess) return e;
+  }
+  // issued first: its many small workgroups are dispatched before the huge
+  // tiles' (each of which takes a whole CU's LDS)
+  hipLaunchKernelGGL(k_sort_tiles, dim3(kChunkBlocks + a.vs.T), dim3(kSortThreads), 0, s, a);
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_merge_chunks, dim3(G, kChunkRows), dim3(kMergeThreads), 0, s, a);
-  return hipGetLastError();
+  const hipStream_t hs = a.side_huge ? side->s : s;  // stream of the huge-tile sort
+  // batches: one workgroup per possible tile, so the dispatcher balances
+  // thousands of uneven sorts; one view: a few striding workgroups
+  const unsigned huge_blocks = a.side_huge ? (unsigned)a.vs.T : kChunkBlocks;
+  hipLaunchKernelGGL(k_sort_huge, dim3(huge_blocks), dim3(kMergeThreads), 0, hs, a);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_merge_chunks, dim3(kChunkBlocks), dim3(kMergeThreads), 0, s, a);
+  e = hipGetLastError();
+  if (e != hipSuccess || !a.side_huge) return e;
+  e = hipEventRecord(side->join, hs);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s, side->join, 0);
+  return e;
 }
 
 // ---------------------------------------------------------------------------

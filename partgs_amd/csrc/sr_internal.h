@@ -111,7 +111,7 @@ inline GeomLayout geom_layout(size_t R) {
 // Image state: per-tile binning tables and per-pixel blend state of all views.
 struct ImageLayout {
   int T, N;
-  size_t counters, tile_count, tile_start, tile_fill, ranges, large, sub_count, q_flag, lane_range, q_cost,
+  size_t counters, tile_count, tile_start, tile_fill, ranges, large, lchunk_off, sub_count, q_flag, lane_range, q_cost,
       q_order, final_T, n_contrib, total;
 };
 inline ImageLayout image_layout(const ViewSet& vs) {
@@ -120,12 +120,14 @@ inline ImageLayout image_layout(const ViewSet& vs) {
   l.N = (int)vs.N;
   size_t off = 0;
   l.counters = off; off += kAlign;  // [0] instances, [1] error flags, [2] work items,
-                                    // [3] large tiles, [4..5] pair-store fill
+                                    // [3] large tiles, [4..5] pair-store fill,
+                                    // [6] chunks of the large tiles, [7] huge tiles
   l.tile_count = off; off += align_up((size_t)l.T * 4);
   l.tile_start = off; off += align_up(((size_t)l.T + 1) * 4);
   l.tile_fill = off; off += align_up((size_t)l.T * 4);
   l.ranges = off; off += align_up((size_t)l.T * 8);
   l.large = off; off += align_up((size_t)l.T * 4);  // tiles for the large-tile sort
+  l.lchunk_off = off; off += align_up(((size_t)l.T + 1) * 4);  // their first chunk numbers
   l.sub_count = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);
   l.q_flag = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 4);  // 1: the quadrant missed the pair store
   l.lane_range = off; off += align_up((size_t)l.T * quads_per_tile(vs.B) * 64 * 8);  // (first pair, pairs) per pixel
@@ -266,15 +268,20 @@ struct ScatterArgs {
 };
 
 // Tiles with at most kSmallMax instances are sorted by one 256-lane
-// workgroup each; larger ones in kSmallMax-key chunks that are then merged.
+// workgroup each; up to kLargeMax in kSmallMax-key chunks that are then
+// merged by rank; larger ("huge") ones by one 1024-lane workgroup each.
 constexpr int kSmallMax = 2048;
+constexpr int kLargeMax = 16384;
 
 struct SortArgs {
   ViewSet vs;
   const uint32_t* ranges;
   const uint32_t* counters;
-  const uint32_t* large;  // tiles above the small-tile sort's size, counters[3] of them
-  uint32_t max_large;     // bound on counters[3] (grid size of the large-tile kernels)
+  const uint32_t* large;  // counters[3] tiles of kSmallMax < n <= large_max from the start, then
+                          // counters[7] tiles of n > large_max from the end (large[T - 1 - h]);
+                          // large_max is kLargeMax, or kSmallMax for batches (no chunk path)
+  const uint32_t* lchunk_off;  // per large tile its first kSmallMax-key chunk (counters[6] in all)
+  int side_huge;               // host: sort the huge tiles on a second stream beside the rest
   uint64_t* keys;
   uint2* lmeta;
   uint16_t* lpre;
@@ -403,7 +410,7 @@ hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s);
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
                              uint32_t* ranges, uint32_t* counters, uint32_t capacity, int nb, uint32_t* bsum,
-                             uint32_t* large, hipStream_t s);
+                             uint32_t* large, uint32_t* lchunk_off, int large_max, hipStream_t s);
 hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s);
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s);
 hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s);

@@ -137,8 +137,8 @@ def test_precomputed_transmat():
 
 @pytest.mark.parametrize("P", [12000, 40000])
 def test_large_tile_sorts(P):
-    """Tiles above 2048 instances take the 1024-lane LDS sort; above 16384 the
-    chunk + merge-path sort."""
+    """Tiles above 2048 instances are sorted as 2048-key chunks merged by rank;
+    above 16384, by one workgroup (LDS chunks, merge path in global memory)."""
     scene = surfel_cloud(5, P=P, width=32, height=32)
     cam = scene.camera
     # pull every centre onto the optical axis region so all land in few tiles
