@@ -685,13 +685,12 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
   const int ltile = tile - (int)vi.tile_base;
   const int tx = ltile % vi.gx, ty = ltile / vi.gx;
   const int tid = threadIdx.x;
-  uint64_t key[K];
-  uint32_t rid[K], hm[K];
+  uint32_t inst[K], rid[K], hm[K];  // instance number: the key's low half
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = tid * K + k;
-    key[k] = i < cn ? s_keys[lds_slot<K>(i)] : ~0ull;
-    rid[k] = i < cn ? a.inst_rec[(uint32_t)key[k]] : 0u;
+    inst[k] = i < cn ? (uint32_t)s_keys[lds_slot<K>(i)] : 0u;
+    rid[k] = i < cn ? a.inst_rec[inst[k]] : 0u;
   }
   short4 bx[K];
 #pragma unroll
@@ -703,9 +702,9 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
     if (i < cn) {
       if constexpr (kWhole) {
         a.point_list[c0 + i] = rid[k];
-        a.inst_at[c0 + i] = (uint32_t)key[k];
+        a.inst_at[c0 + i] = inst[k];
       } else {
-        a.keys[c0 + i] = key[k];
+        a.keys[c0 + i] = s_keys[lds_slot<K>(i)];
       }
     }
   }
@@ -716,12 +715,11 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
     if (i < cn) {
       // for the gradient reduction: the instance's quadrants, their
       // stored-sum flags cleared
-      const uint32_t inst = (uint32_t)key[k];
-      a.inst_quads[inst] = (uint16_t)hm[k];
+      a.inst_quads[inst[k]] = (uint16_t)hm[k];
       if (Q == 4) {
-        reinterpret_cast<uint32_t*>(a.part_nz)[inst] = 0u;
+        reinterpret_cast<uint32_t*>(a.part_nz)[inst[k]] = 0u;
       } else {
-        for (int q = 0; q < Q; ++q) a.part_nz[(size_t)inst * Q + q] = 0;
+        for (int q = 0; q < Q; ++q) a.part_nz[(size_t)inst[k] * Q + q] = 0;
       }
       if constexpr (!kWhole) a.lmeta[c0 + i] = make_uint2(rid[k], hm[k]);
     }
