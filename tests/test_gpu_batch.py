@@ -16,7 +16,7 @@ from partgs_amd import _lib
 from partgs_amd.batch import BatchConfig, BatchRasterizer, autotune, capture_step
 from partgs_amd.camera import random_sphere_camera
 from partgs_amd.rasterizer import rasterize_gaussians_bwd, rasterize_gaussians_fwd
-from partgs_amd.scenes import superquadric_surfels
+from partgs_amd.scenes import superquadric_surfels, surfel_cloud
 
 pytestmark = pytest.mark.gpu
 
@@ -64,6 +64,30 @@ def test_batched_forward_equals_single_views(tile):
         assert torch.equal(radii[v], ref["radii"]), v
         total += int(ref["radii"].gt(0).sum())
     assert total > 0 and need > 0
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_batched_dense_tiles_equal_single_views(tile):
+    """Tiles above 2048 instances: a batch sorts each with one workgroup, a
+    single view in 2048-key chunks merged by rank; both must give the same
+    images, bit for bit."""
+    scene = surfel_cloud(5, P=12000, width=32, height=32)
+    cam = scene.camera
+    c2w = torch.linalg.inv(cam.world_view_transform.T)
+    centre = cam.camera_center + 3.0 * c2w[:3, 2]
+    g = torch.Generator().manual_seed(5)
+    scene.means3D = (centre + 0.05 * (torch.rand(scene.P, 3, generator=g) - 0.5)).float()
+    s = scene.to("cuda")
+    cams = [cam.to("cuda"), cam.to("cuda")]
+    print(f"[batch] dense tiles, tile {tile}", flush=True)
+    br = BatchRasterizer(cams, s.bg, s.sh_degree, BatchConfig(tile=tile))
+    color, others, radii = br.forward_checked(s.means3D, s.shs, None, s.opacities, s.scales, s.rotations)
+    torch.cuda.synchronize()
+    for v, c in enumerate(cams):
+        ref = _single(s, c, tile)
+        assert torch.equal(br.split(color, 3)[v], ref["color"]), v
+        assert torch.equal(br.split(others, 7)[v], ref["others"]), v
+        assert torch.equal(radii[v], ref["radii"]), v
 
 
 def test_batched_backward_sums_single_views():

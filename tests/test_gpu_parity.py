@@ -135,10 +135,12 @@ def test_precomputed_transmat():
     _check_backward(scene, res, o, seed=4)
 
 
+@pytest.mark.parametrize("tile", [8, 16, 32])
 @pytest.mark.parametrize("P", [12000, 40000])
-def test_large_tile_sorts(P):
+def test_large_tile_sorts(P, tile):
     """Tiles above 2048 instances are sorted as 2048-key chunks merged by rank;
-    above 16384, by one workgroup (LDS chunks, merge path in global memory)."""
+    above 16384, by one workgroup (LDS chunks, merge path in global memory).
+    Tile 8 has one quadrant per tile, tile 32 sixteen (four packed counters)."""
     scene = surfel_cloud(5, P=P, width=32, height=32)
     cam = scene.camera
     # pull every centre onto the optical axis region so all land in few tiles
@@ -147,10 +149,10 @@ def test_large_tile_sorts(P):
     centre = cam.camera_center + 3.0 * fwd
     g = torch.Generator().manual_seed(5)
     scene.means3D = (centre + 0.05 * (torch.rand(scene.P, 3, generator=g) - 0.5)).float()
-    o, _ = oracle_run(scene)
+    o, _ = oracle_run(scene, tile=tile)
     st_counts = np.diff(o.get("ranges"), axis=1).max()
     assert st_counts > (16384 if P > 20000 else 2048), st_counts
-    res = gpu_forward(scene)
+    res = gpu_forward(scene, tile=tile)
     _check_forward(scene, res, o)
     _check_backward(scene, res, o, seed=5)
 
