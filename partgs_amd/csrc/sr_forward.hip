@@ -1355,7 +1355,17 @@ __global__ __launch_bounds__(1024) void k_order_quads(int nq, const uint32_t* co
     int b = lg * 2 + (lg >= 2 ? (int)((c >> (lg - 2)) & 1u) : 0);
     return 63 - min(b, 63);
   };
-  for (int i = tid; i < nq; i += 1024) atomicAdd(&s_cnt[bucket(cost[i])], 1u);
+  // Most quadrants are empty (cost 0, the last bucket): a wave counts and
+  // places those with one LDS atomic instead of one per lane.
+  const int lane = tid & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int i0 = 0; i0 < nq; i0 += 1024) {  // uniform
+    const int i = i0 + tid;
+    const uint32_t c = i < nq ? cost[i] : 1u;
+    const uint64_t zero = __ballot(i < nq && c == 0u);
+    if (lane == 0 && zero) atomicAdd(&s_cnt[63], (uint32_t)__popcll(zero));
+    if (i < nq && c != 0u) atomicAdd(&s_cnt[bucket(c)], 1u);
+  }
   __syncthreads();
   if (tid == 0) {
     uint32_t run = 0;
@@ -1365,7 +1375,18 @@ __global__ __launch_bounds__(1024) void k_order_quads(int nq, const uint32_t* co
     }
   }
   __syncthreads();
-  for (int i = tid; i < nq; i += 1024) order[atomicAdd(&s_base[bucket(cost[i])], 1u)] = (uint32_t)i;
+  for (int i0 = 0; i0 < nq; i0 += 1024) {
+    const int i = i0 + tid;
+    const uint32_t c = i < nq ? cost[i] : 1u;
+    const uint64_t zero = __ballot(i < nq && c == 0u);
+    uint32_t zbase = 0;
+    if (lane == 0 && zero) zbase = atomicAdd(&s_base[63], (uint32_t)__popcll(zero));
+    zbase = (uint32_t)__shfl((int)zbase, 0);
+    if (i < nq) {
+      if (c == 0u) order[zbase + (uint32_t)__popcll(zero & lt)] = (uint32_t)i;
+      else order[atomicAdd(&s_base[bucket(c)], 1u)] = (uint32_t)i;
+    }
+  }
 }
 
 hipError_t launch_order_quads(int nq, const uint32_t* cost, uint32_t* order, hipStream_t s) {
