@@ -675,6 +675,33 @@ SR_DEV uint64_t scan256_excl_u64(uint64_t v, uint64_t* s_w, uint64_t& total) {
 // entry (record, quadrant mask) to lmeta and, per quadrant, the number of
 // the chunk's earlier entries that hit it to lpre, for k_merge_chunks.
 constexpr int kSortThreads = 256;
+
+// Called by one whole wave; lane q < Q holds the length of the tile's
+// sub-list q.  Writes the lengths and clears the overflow flags, and appends
+// the sub-lists' 64-entry chunks to the acceptance work list with one atomic
+// for the tile (same-address atomics serialise).
+SR_DEV void append_work(const SortArgs& a, int tile, int Q, uint32_t cnt) {
+  const int lane = threadIdx.x & 63;
+  const bool has = lane < Q;
+  const uint32_t nch = has ? (cnt + 63u) / 64u : 0u;
+  uint32_t incl = nch;
+#pragma unroll
+  for (int d = 1; d < kMaxQuads; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)incl, d);
+    if (lane >= d) incl += y;
+  }
+  const uint32_t all = (uint32_t)__shfl((int)incl, kMaxQuads - 1);
+  uint32_t w0 = 0;
+  if (lane == 0 && all > 0) w0 = atomicAdd(a.work_count, all);
+  w0 = (uint32_t)__shfl((int)w0, 0);
+  if (has) {
+    a.sub_count[(size_t)tile * Q + lane] = cnt;
+    a.q_flag[(size_t)tile * Q + lane] = 0u;
+    const uint32_t first = w0 + incl - nch;
+    for (uint32_t ch = 0; ch < nch; ++ch) a.work[first + ch] = make_uint2((uint32_t)(tile * Q + lane), ch);
+  }
+}
+
 template <bool kWhole>
 SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint32_t c0, int cn,
                          const uint64_t* s_keys) {
@@ -725,6 +752,7 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
     }
   }
   uint2* base = a.sub + (size_t)Q * beg;
+  uint32_t my_cnt = 0;  // kWhole: lane q < Q gets sub-list q's length
   for (int g = 0; g * 4 < Q; ++g) {  // quadrants 4g .. 4g + 3 (uniform loop)
     uint64_t cnt = 0;
 #pragma unroll
@@ -750,19 +778,11 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
       }
       run += spread4(hm[k], g);
     }
-    if constexpr (kWhole) {
-      if (tid >= 4 * g && tid < min(Q, 4 * g + 4)) {
-        const int q = tid;
-        const uint32_t c = field16(total, q - 4 * g);
-        a.sub_count[(size_t)tile * Q + q] = c;
-        a.q_flag[(size_t)tile * Q + q] = 0u;
-        const uint32_t nch = (c + 63u) / 64u;
-        if (nch > 0) {
-          const uint32_t w0 = atomicAdd(a.work_count, nch);
-          for (uint32_t ch = 0; ch < nch; ++ch) a.work[w0 + ch] = make_uint2((uint32_t)(tile * Q + q), ch);
-        }
-      }
-    }
+    if constexpr (kWhole)
+      if (tid >= 4 * g && tid < min(Q, 4 * g + 4)) my_cnt = field16(total, tid - 4 * g);
+  }
+  if constexpr (kWhole) {
+    if (tid < 64) append_work(a, tile, Q, my_cnt);
   }
 }
 
@@ -1078,20 +1098,14 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge_chunks(SortArgs a) {
     __syncthreads();
     merge_chunk(a, beg, n, c, s_tile);
     // the tile's sub-list lengths and acceptance work items, by its chunk 0
-    if (c == 0 && tid < Q) {
-      const int q = tid;
+    if (c == 0 && tid < 64) {
       uint32_t cnt = 0;
-      for (int cc = 0; cc < P; ++cc) {
-        const int len = min(kSmallMax, n - cc * kSmallMax);
-        cnt += field16(chunk_hits_below(a, beg + (uint32_t)cc * kSmallMax, len, len, Q, q / 4), q % 4);
-      }
-      a.sub_count[(size_t)tile * Q + q] = cnt;
-      a.q_flag[(size_t)tile * Q + q] = 0u;
-      const uint32_t nch = (cnt + 63u) / 64u;
-      if (nch > 0) {
-        const uint32_t w0 = atomicAdd(a.work_count, nch);
-        for (uint32_t ch = 0; ch < nch; ++ch) a.work[w0 + ch] = make_uint2((uint32_t)(tile * Q + q), ch);
-      }
+      if (tid < Q)
+        for (int cc = 0; cc < P; ++cc) {
+          const int len = min(kSmallMax, n - cc * kSmallMax);
+          cnt += field16(chunk_hits_below(a, beg + (uint32_t)cc * kSmallMax, len, len, Q, tid / 4), tid % 4);
+        }
+      append_work(a, tile, Q, cnt);
     }
   }
 }
