@@ -655,15 +655,18 @@ SR_DEV F3 sh_backward_core(int deg, F3 dir_orig, F3 dL_dRGB, Get S, Put put) {
 // 16-byte aligned rows the surfel's 48 coefficients are read and its 48
 // gradients written as twelve float4 each (same values, same order of
 // operations as the scalar form).
-template <bool kAdd>
-SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, float* dsh) {
+// srow (when not null): the surfel's row staged in LDS by its wave, read
+// instead of sh; with to_row the gradients are written back into it (the
+// caller stores the wave's rows) instead of to dsh.
+template <bool kAdd, bool kRow, bool kToRow>
+SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, float* dsh, float4* srow) {
   const int nact = (deg + 1) * (deg + 1);
   if (M == 16 && ((reinterpret_cast<uintptr_t>(sh) | reinterpret_cast<uintptr_t>(dsh)) & 15u) == 0u) {
     float shv[48];
     const float4* s4 = reinterpret_cast<const float4*>(sh);
 #pragma unroll
     for (int i = 0; i < 12; ++i) {
-      const float4 t = s4[i];
+      const float4 t = kRow ? srow[i] : s4[i];
       shv[4 * i] = t.x;
       shv[4 * i + 1] = t.y;
       shv[4 * i + 2] = t.z;
@@ -687,6 +690,8 @@ SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, 
       if (kAdd) {
         const float4 o = d4[i];
         d4[i] = make_float4(o.x + v[0], o.y + v[1], o.z + v[2], o.w + v[3]);
+      } else if (kToRow) {
+        srow[i] = make_float4(v[0], v[1], v[2], v[3]);
       } else {
         d4[i] = make_float4(v[0], v[1], v[2], v[3]);
       }
@@ -713,8 +718,9 @@ SR_DEV F3 sh_backward(int deg, int M, const float* sh, F3 dir_orig, F3 dL_dRGB, 
 // SH backward of one record: stores (first view) or adds c * dL_dRGB to the
 // surfel's dL_dsh and returns the gradient of the mean through the view
 // direction.
+template <bool kRow, bool kToRow>
 SR_DEV F3 sh_grad(const PreprocessBwdArgs& a, bool first, int idx, size_t r, const float* campos, float m0,
-                  float m1, float m2, F3 dRGB) {
+                  float m1, float m2, F3 dRGB, float4* srow) {
   const uchar4 cl = reinterpret_cast<const uchar4*>(a.clamped)[r];
   if (cl.x) dRGB.x = 0.0f;
   if (cl.y) dRGB.y = 0.0f;
@@ -722,7 +728,8 @@ SR_DEV F3 sh_grad(const PreprocessBwdArgs& a, bool first, int idx, size_t r, con
   const F3 dir_orig = f3(m0 - campos[0], m1 - campos[1], m2 - campos[2]);
   float* dsh = a.dL_dsh + (size_t)idx * a.M * 3;
   const float* sh = a.shs + (size_t)idx * a.M * 3;
-  return first ? sh_backward<false>(a.D, a.M, sh, dir_orig, dRGB, dsh) : sh_backward<true>(a.D, a.M, sh, dir_orig, dRGB, dsh);
+  return first ? sh_backward<false, kRow, kToRow>(a.D, a.M, sh, dir_orig, dRGB, dsh, srow)
+               : sh_backward<true, kRow, false>(a.D, a.M, sh, dir_orig, dRGB, dsh, srow);
 }
 
 // Gradients of one surfel summed over the views that see it.
@@ -788,8 +795,9 @@ SR_DEV void reduce_record(const PreprocessBwdArgs& a, size_t r, float (&gs)[18])
 // this thread) and writes the record's outputs: dL_dtransMat (with the
 // AABB-centre term for a precomputed T), dL_dmean2D (the densification
 // statistic), dL_dopacity and dL_dcolors.
+template <bool kRow, bool kToRow>
 SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size_t r, bool first, SurfelGrad& g,
-                                const float (&gs)[18]) {
+                                const float (&gs)[18], float4* srow) {
   const ViewInfo& vi = a.vs.v[v];
   const float tanx = vi.tanx, tany = vi.tany;
   const float focal_x = (float)vi.W / (2.0f * tanx);
@@ -913,12 +921,13 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
     g.sc[0] += (dRS0.x * Rc0.x + dRS0.y * Rc0.y) + dRS0.z * Rc0.z;
     g.sc[1] += (dRS1.x * Rc1.x + dRS1.y * Rc1.y) + dRS1.z * Rc1.z;
     F3 dm3 = f3(dM[2][0], dM[2][1], dM[2][2]);
-    if (a.shs != nullptr) dm3 = dm3 + sh_grad(a, first, idx, r, campos, m0, m1, m2, f3(gs[15], gs[16], gs[17]));
+    if (a.shs != nullptr)
+      dm3 = dm3 + sh_grad<kRow, kToRow>(a, first, idx, r, campos, m0, m1, m2, f3(gs[15], gs[16], gs[17]), srow);
     g.m[0] += dm3.x;
     g.m[1] += dm3.y;
     g.m[2] += dm3.z;
   } else if (a.shs != nullptr) {
-    const F3 dm3 = sh_grad(a, first, idx, r, campos, m0, m1, m2, f3(gs[15], gs[16], gs[17]));
+    const F3 dm3 = sh_grad<kRow, kToRow>(a, first, idx, r, campos, m0, m1, m2, f3(gs[15], gs[16], gs[17]), srow);
     g.m[0] += dm3.x;
     g.m[1] += dm3.y;
     g.m[2] += dm3.z;
@@ -933,9 +942,51 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
 // 0 + x, so the results equal the single-view formulas bit for bit.  Every
 // per-surfel and per-record output is written exactly once (zeros for a
 // record no view sees), so none of them needs clearing beforehand.
-__global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
+// Degree-3 SH in 16-byte aligned rows move through LDS: each wave loads its
+// 64 surfels' rows (12 KB, contiguous) with coalesced loads and, for one
+// view, stores the gradient rows the same way, instead of each lane moving
+// its 192-byte row in 16-byte pieces 192 bytes apart.
+constexpr int kShRow = 12;  // float4 per degree-3 SH row
+template <bool kRow, bool kToRow>
+SR_DEV void preprocess_bwd_surfel(const PreprocessBwdArgs& a, int idx, float4* srow);
+// Three waves per SIMD (a few registers spill to scratch): with the 48 KB of
+// staged rows, two per SIMD was slower (0.086 against 0.071 ms at 300k surfels).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_preprocess_bwd(
+    PreprocessBwdArgs a) {
+  __shared__ float4 s_sh[256 * kShRow];
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.P) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool staged = a.shs != nullptr && a.M == 16 &&
+                      ((reinterpret_cast<uintptr_t>(a.shs) | reinterpret_cast<uintptr_t>(a.dL_dsh)) & 15u) == 0u;
+  const bool to_row = staged && a.vs.V == 1;  // uniform
+  const int w0 = blockIdx.x * blockDim.x + wave * 64;  // the wave's first surfel
+  const int nrow = min(64, a.P - w0) * kShRow;         // float4 of the wave's rows (<= 0: none)
+  float4* wrows = s_sh + wave * 64 * kShRow;
+  if (staged) {
+    const float4* src = reinterpret_cast<const float4*>(a.shs) + (size_t)w0 * kShRow;
+#pragma unroll
+    for (int t = 0; t < kShRow; ++t)
+      if (t * 64 + lane < nrow) wrows[t * 64 + lane] = src[t * 64 + lane];
+  }
+  __syncthreads();
+  if (idx < a.P) {
+    if (to_row) preprocess_bwd_surfel<true, true>(a, idx, wrows + lane * kShRow);
+    else if (staged) preprocess_bwd_surfel<true, false>(a, idx, wrows + lane * kShRow);
+    else preprocess_bwd_surfel<false, false>(a, idx, wrows);
+  }
+  if (to_row) {
+    __syncthreads();
+    float4* dst = reinterpret_cast<float4*>(a.dL_dsh) + (size_t)w0 * kShRow;
+#pragma unroll
+    for (int t = 0; t < kShRow; ++t)
+      if (t * 64 + lane < nrow) dst[t * 64 + lane] = wrows[t * 64 + lane];
+  }
+}
+
+// Gradients of surfel idx over all views; kRow: its SH row is staged in LDS
+// at srow, kToRow: its dL_dsh row goes there too.
+template <bool kRow, bool kToRow>
+SR_DEV void preprocess_bwd_surfel(const PreprocessBwdArgs& a, int idx, float4* srow) {
   SurfelGrad g{};
   bool any = false;
   const bool sum_rec = a.dL_dopacity_out != a.dL_dopacity;
@@ -952,7 +1003,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
     }
     float gs[18];
     reduce_record(a, r, gs);
-    preprocess_bwd_view(a, v, idx, r, !any, g, gs);
+    preprocess_bwd_view<kRow, kToRow>(a, v, idx, r, !any, g, gs, srow);
     any = true;
     if (sum_rec) {
       op += gs[14];
@@ -962,8 +1013,12 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreprocessBwdArgs a) {
     }
   }
   if (!any && a.shs != nullptr) {
-    float* dsh = a.dL_dsh + (size_t)idx * a.M * 3;
-    for (int k = 0; k < 3 * a.M; ++k) dsh[k] = 0.0f;
+    if (kToRow) {
+      for (int k = 0; k < kShRow; ++k) srow[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+      float* dsh = a.dL_dsh + (size_t)idx * a.M * 3;
+      for (int k = 0; k < 3 * a.M; ++k) dsh[k] = 0.0f;
+    }
   }
   a.dL_drots[4 * idx] = g.q[0];
   a.dL_drots[4 * idx + 1] = g.q[1];
