@@ -67,9 +67,10 @@ SR_DEV F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
 SR_DEV F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
 SR_DEV F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
 
-// SH -> RGB, dsr/cuda_rasterizer/forward.cu:20-71 (same association).
-SR_DEV F3 color_from_sh(int deg, int M, const float* sh, F3 dir, uint8_t* clamped) {
-  auto S = [&](int k) { return f3(sh[3 * k], sh[3 * k + 1], sh[3 * k + 2]); };
+// SH -> RGB, dsr/cuda_rasterizer/forward.cu:20-71 (same association); S(k)
+// is coefficient k.
+template <typename Get>
+SR_DEV F3 color_from_sh_core(int deg, Get S, F3 dir, uint8_t* clamped) {
   F3 res = kSH_C0 * S(0);
   if (deg > 0) {
     const float x = dir.x, y = dir.y, z = dir.z;
@@ -100,6 +101,26 @@ SR_DEV F3 color_from_sh(int deg, int M, const float* sh, F3 dir, uint8_t* clampe
   clamped[1] = res.y < 0.0f;
   clamped[2] = res.z < 0.0f;
   return f3(fmax_(res.x, 0.0f), fmax_(res.y, 0.0f), fmax_(res.z, 0.0f));
+}
+
+// Degree 3 over 16 coefficients in a 16-byte aligned row: the row is read as
+// twelve float4 (a quarter of the load instructions); otherwise only the
+// coefficients the degree uses are read, one float at a time.
+SR_DEV F3 color_from_sh(int deg, int M, const float* sh, F3 dir, uint8_t* clamped) {
+  if (deg == 3 && M == 16 && (reinterpret_cast<uintptr_t>(sh) & 15u) == 0u) {
+    float v[48];
+    const float4* s4 = reinterpret_cast<const float4*>(sh);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+      const float4 t = s4[i];
+      v[4 * i] = t.x;
+      v[4 * i + 1] = t.y;
+      v[4 * i + 2] = t.z;
+      v[4 * i + 3] = t.w;
+    }
+    return color_from_sh_core(deg, [&](int k) { return f3(v[3 * k], v[3 * k + 1], v[3 * k + 2]); }, dir, clamped);
+  }
+  return color_from_sh_core(deg, [&](int k) { return f3(sh[3 * k], sh[3 * k + 1], sh[3 * k + 2]); }, dir, clamped);
 }
 
 // Screen cull box of a surfel: every pixel where the blend can reach
