@@ -109,9 +109,14 @@ int check_stage(bool debug, hipStream_t s, const char* what) {
     if (rc_ != SR_OK) return rc_;                               \
   } while (0)
 
-// An event per host thread, marking the instance count's copy to the host.
+// An event per (host thread, device), marking the instance count's copy to
+// the host (an event is recorded only on streams of the device it was
+// created on).
 hipEvent_t counts_event() {
-  thread_local hipEvent_t e = nullptr;
+  thread_local hipEvent_t cache[16] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return nullptr;
+  hipEvent_t& e = cache[dev];
   if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
   return e;
 }
