@@ -104,15 +104,11 @@ def run_batched(args):
 
     from partgs_amd import _lib
     from partgs_amd.batch import BatchConfig, BatchRasterizer, autotune, capture_step
-    from partgs_amd.camera import random_sphere_camera
-    from partgs_amd.scenes import superquadric_surfels
+    from partgs_amd.scenes import batched_views, superquadric_surfels
 
     dev = torch.device("cuda:0")
-    rng = np.random.default_rng(0)
-    choices = [(400, 300), (640, 480), (800, 600)]
-    res = [choices[int(k)] for k in rng.integers(0, 3, size=32)]
     scene = superquadric_surfels(0, P=1_000_000, width=800, height=600, sh_degree=3).to(dev)
-    cams = [random_sphere_camera(1000 + k, 3.0, w, h).to(dev) for k, (w, h) in enumerate(res)]
+    cams = [c.to(dev) for c in batched_views(0, 32)]
     print(f"[bench] batched: {len(cams)} views, {scene.P} surfels", file=sys.stderr, flush=True)
     cfg = BatchConfig()
     if args.autotune:

@@ -73,6 +73,13 @@ typedef struct {
                               *num_rendered set (call again with at least
                               that).  The backward must be given the same
                               value, as it locates the binning arrays by it. */
+  int pair_capacity;       /* accepted-pair store size in pairs; 0: pair_store
+                              pairs per tile instance.  Quadrants whose pairs
+                              do not fit are blended by recomputing them (same
+                              results, slower).  sr_pair_count reports what a
+                              forward asked for, so a caller can size the next
+                              one from it.  The backward must be given the
+                              same value. */
 } sr_settings;
 
 /* Per-surfel inputs (device).  Exactly one of shs / colors_precomp and exactly
@@ -136,7 +143,17 @@ int sr_mark_visible(int P, const float* means3D, const float* viewmatrix,
 /* Byte sizes of the three state buffers (what sr_forward asks alloc for). */
 size_t sr_geom_bytes(int P);
 size_t sr_image_bytes(int W, int H, int tile);
-size_t sr_binning_bytes(int num_rendered, int W, int H, int tile, int pair_store);
+size_t sr_binning_bytes(int num_rendered, int W, int H, int tile, int pair_store, int pair_capacity);
+
+/* Accepted pairs (*pairs) and tile instances (*instances) of the calling
+ * thread's latest completed sr_forward on the current device: the pairs are
+ * those the forward asked the pair store for (the accepted pairs in front
+ * of each pixel's saturation point).  The counts are copied back
+ * asynchronously at the end of each forward and never waited for: returns
+ * SR_OK once such a copy has landed, SR_ERR_INVALID before.  A caller sizes
+ * the next forward's pair_capacity from them.  No counterpart in the
+ * reference. */
+int sr_pair_count(unsigned long long* pairs, unsigned int* instances);
 
 /* Byte offsets of the named arrays inside the state buffers, so callers and
  * tests can read internal state (e.g. "depths", "rec", "tiles_touched",
@@ -187,14 +204,17 @@ typedef struct {
   int sh_degree;
   int tile;                   /* 8, 16 or 32 */
   int pair_store;             /* as in sr_settings */
+  int pair_capacity;          /* as in sr_settings (pairs over all views) */
 } sr_views;
 
 size_t sr_batch_workspace_bytes(const sr_views* views, int P, int capacity);
 
 /* Outputs: out_color / out_others hold each view's 3 x H x W / 7 x H x W
  * planes back to back in view order (view v starts at 3 / 7 times the pixel
- * count of views 0..v-1); radii is V x P; status (device, 4 words) receives
- * [0] instances, [1] flags (SR_STATUS_CAPACITY), [2] acceptance work items.
+ * count of views 0..v-1); radii is V x P; status (device, 8 words) receives
+ * [0] instances, [1] flags (SR_STATUS_CAPACITY), [2] acceptance work items,
+ * [4..5] the accepted pairs asked of the pair store (64-bit, low word first;
+ * see sr_pair_count); the other words are internal.
  * The surfels need scales and rotations (no precomputed T, which is per
  * view); prefiltered is not supported. */
 int sr_forward_batched(const sr_views* views, const sr_surfels* g, void* workspace, size_t workspace_bytes,

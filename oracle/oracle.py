@@ -62,6 +62,12 @@ def load():
         lib.sro_set_threads.argtypes = [C.c_int]
         lib.sro_get_threads.restype = C.c_int
         lib.sro_mark_visible.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.sro_forward_record.argtypes = [C.c_void_p, C.POINTER(SroScene), C.c_int]
+        lib.sro_forward_record.restype = C.c_int
+        lib.sro_replay.argtypes = [C.c_void_p, C.POINTER(SroScene)]
+        lib.sro_replay.restype = C.c_int
+        lib.sro_alpha_capped.argtypes = [C.c_void_p]
+        lib.sro_alpha_capped.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -160,7 +166,40 @@ class Oracle:
 
     def forward(self, *, means3D, opacities, viewmatrix, projmatrix, campos, bg, W, H, tanfovx, tanfovy,
                 shs=None, sh_degree=0, colors_precomp=None, scales=None, rotations=None, transMat_precomp=None,
-                scale_modifier=1.0, tile=16, prefiltered=False, f64=False) -> int:
+                scale_modifier=1.0, tile=16, prefiltered=False, f64=False, record=False) -> int:
+        """Preprocess, binning and blend.  record=True keeps every discrete
+        decision for replay()."""
+        s = self._scene(means3D=means3D, opacities=opacities, viewmatrix=viewmatrix, projmatrix=projmatrix,
+                        campos=campos, bg=bg, W=W, H=H, tanfovx=tanfovx, tanfovy=tanfovy, shs=shs,
+                        sh_degree=sh_degree, colors_precomp=colors_precomp, scales=scales, rotations=rotations,
+                        transMat_precomp=transMat_precomp, scale_modifier=scale_modifier, tile=tile,
+                        prefiltered=prefiltered, f64=f64)
+        fn = self.lib.sro_forward_record if record else self.lib.sro_forward
+        rc = fn(self.h, C.byref(s), 1 if f64 else 0)
+        if rc < 0:
+            raise ValueError("oracle: bad arguments")
+        gx, gy = (W + tile - 1) // tile, (H + tile - 1) // tile
+        self.f64 = f64
+        self.dims = dict(P=s.P, M=s.M, W=int(W), H=int(H), T=gx * gy, L=self.lib.sro_num_rendered(self.h))
+        return rc
+
+    def replay(self, **kw) -> None:
+        """Re-evaluates the recorded forward on new inputs (same keywords as
+        forward) with its discrete decisions held: a smooth function of the
+        inputs whose derivative is the gradient the backward computes."""
+        kw.setdefault("f64", self.f64)
+        s = self._scene(**kw)
+        if self.lib.sro_replay(self.h, C.byref(s)) != 0:
+            raise ValueError("oracle: replay needs a recorded forward of the same shape")
+
+    @property
+    def alpha_capped(self) -> int:
+        """Blended pairs of the recorded forward whose alpha hit the 0.99 cap."""
+        return int(self.lib.sro_alpha_capped(self.h))
+
+    def _scene(self, *, means3D, opacities, viewmatrix, projmatrix, campos, bg, W, H, tanfovx, tanfovy,
+               shs=None, sh_degree=0, colors_precomp=None, scales=None, rotations=None, transMat_precomp=None,
+               scale_modifier=1.0, tile=16, prefiltered=False, f64=False):
         dt = np.float64 if f64 else np.float32
         arrs = {k: _arr(v, dt) for k, v in dict(
             means3D=means3D, shs=shs, colors_precomp=colors_precomp, opacities=opacities, scales=scales,
@@ -175,13 +214,7 @@ class Oracle:
         for k, v in arrs.items():
             setattr(s, k, None if v is None or v.size == 0 else v.ctypes.data)
         s.scale_modifier, s.tan_fovx, s.tan_fovy = float(scale_modifier), float(tanfovx), float(tanfovy)
-        rc = self.lib.sro_forward(self.h, C.byref(s), 1 if f64 else 0)
-        if rc < 0:
-            raise ValueError("oracle: bad arguments")
-        gx, gy = (W + tile - 1) // tile, (H + tile - 1) // tile
-        self.f64 = f64
-        self.dims = dict(P=P, M=M, W=int(W), H=int(H), T=gx * gy, L=self.lib.sro_num_rendered(self.h))
-        return rc
+        return s
 
     @property
     def num_rendered(self) -> int:

@@ -155,6 +155,15 @@ template <typename R> struct Ctx {
   // backward: render-stage sums in FP64, then final grads in R
   std::vector<double> acc_transMat, acc_mean2D, acc_normal, acc_opacity, acc_colors;
   std::vector<R> g_mean2D, g_colors, g_opacity, g_means3D, g_transMat, g_sh, g_scales, g_rots, g_normal;
+  // Frozen decisions (finite-difference support): a recorded forward keeps
+  // every discrete choice it made; a replay re-evaluates the same
+  // expressions on new inputs with those choices fixed, which makes the
+  // outputs a smooth function of the inputs whose gradient is the one the
+  // backward computes (it differentiates with the same choices held).
+  bool record = false, replay = false;
+  std::vector<int8_t> dec_mult;                  // normal flip sign per surfel
+  std::vector<std::vector<uint32_t>> dec_pix;    // per pixel: (list position << 3) | flags
+  int dec_alpha_clamped = 0;                     // blended pairs whose alpha hit the 0.99 cap
 };
 
 template <typename R> void copy_in(std::vector<R>& dst, const void* src, size_t n, int f64) {
@@ -210,6 +219,8 @@ template <typename R> V3<R> color_from_sh(const Ctx<R>& c, int idx, uint8_t* cla
     }
   }
   res = v3(res.x + (R)0.5f, res.y + (R)0.5f, res.z + (R)0.5f);
+  if (c.replay)  // clamping choices held from the recorded forward
+    return v3(clamped[0] ? (R)0 : res.x, clamped[1] ? (R)0 : res.y, clamped[2] ? (R)0 : res.z);
   clamped[0] = res.x < (R)0;
   clamped[1] = res.y < (R)0;
   clamped[2] = res.z < (R)0;
@@ -246,15 +257,19 @@ void get_rect(R px, R py, int r, int B, int gx, int gy, int& x0, int& y0, int& x
 // ---------------------------------------------------------------------------
 template <typename R> void preprocess_one(Ctx<R>& c, int idx) {
   const Scene& s = c.sc;
-  c.radii[idx] = 0;
-  c.tiles_touched[idx] = 0;
+  if (c.replay) {
+    if (c.radii[idx] <= 0) return;  // culled in the recorded forward
+  } else {
+    c.radii[idx] = 0;
+    c.tiles_touched[idx] = 0;
+  }
   const R* m = &c.means3D[3 * idx];
   const R* vm = c.vm.data();
   const R* pm = c.pm.data();
   const R vx = ((vm[0] * m[0] + vm[4] * m[1]) + vm[8] * m[2]) + vm[12];
   const R vy = ((vm[1] * m[0] + vm[5] * m[1]) + vm[9] * m[2]) + vm[13];
   const R vz = ((vm[2] * m[0] + vm[6] * m[1]) + vm[10] * m[2]) + vm[14];
-  if (vz <= (R)0.2f) {
+  if (vz <= (R)0.2f && !c.replay) {
     if (s.prefiltered) c.prefilter_violation = 1;
     return;
   }
@@ -287,8 +302,9 @@ template <typename R> void preprocess_one(Ctx<R>& c, int idx) {
   }
   // dual-visible normal flip, forward.cu:209-214
   const R cosv = -((vx * nrm.x + vy * nrm.y) + vz * nrm.z);
-  if (cosv == (R)0) return;
-  const R mult = cosv > (R)0 ? (R)1 : (R)-1;
+  if (cosv == (R)0 && !c.replay) return;
+  const R mult = c.replay ? (R)c.dec_mult[idx] : (cosv > (R)0 ? (R)1 : (R)-1);
+  if (c.record) c.dec_mult[idx] = (int8_t)(mult > (R)0 ? 1 : -1);
   nrm = mult * nrm;
   // compute_aabb with cutoff 3, forward.cu:119-145
   const R d = ((R)9 * (Tw.x * Tw.x) + (R)9 * (Tw.y * Tw.y)) + (R)-1 * (Tw.z * Tw.z);
@@ -302,10 +318,10 @@ template <typename R> void preprocess_one(Ctx<R>& c, int idx) {
   const R hx = std::sqrt(rmax((R)1e-4f, h0x));
   const R hy = std::sqrt(rmax((R)1e-4f, h0y));
   const R radius = std::ceil(rmax(rmax(hx, hy), (R)(3.0f * 0.707106f)));
-  const int ir = f2i(radius);
+  const int ir = c.replay ? c.radii[idx] : f2i(radius);
   int x0, y0, x1, y1;
   get_rect(px, py, ir, s.B, c.gx, c.gy, x0, y0, x1, y1);
-  const uint32_t area = (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
+  const uint32_t area = c.replay ? c.tiles_touched[idx] : (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
   if (area == 0) return;
   if (!s.has_colors) {
     const V3<R> rgb = color_from_sh(c, idx, &c.clamped[3 * idx]);
@@ -454,6 +470,10 @@ template <typename R> void render_pixel(Ctx<R>& c, int tile, int pxi, int pyi) {
     }
     for (int ch = 0; ch < 3; ++ch) N[ch] += no[ch] * w;
     for (int ch = 0; ch < 3; ++ch) C[ch] += c.rgb[3 * id + ch] * w;
+    if (c.record) {
+      const uint32_t fl = (rho3d <= rho2d ? 1u : 0u) | (alpha == (R)0.99f ? 2u : 0u) | (T > (R)0.5 ? 4u : 0u);
+      c.dec_pix[(size_t)W * pyi + pxi].push_back((e << 3) | fl);
+    }
     T = test_T;
     last_contributor = contributor;
   }
@@ -469,6 +489,72 @@ template <typename R> void render_pixel(Ctx<R>& c, int tile, int pxi, int pyi) {
   for (int ch = 0; ch < 3; ++ch) c.out_others[pid + (2 + ch) * HW] = N[ch];
   c.out_others[pid + 5 * HW] = median_depth;
   c.out_others[pid + 6 * HW] = distortion;
+}
+
+// Replay of one pixel: the recorded blended entries with their branch
+// (3D intersection or screen-space filter), alpha cap and median choices
+// held; the expressions are those of render_pixel.
+template <typename R> void render_pixel_replay(Ctx<R>& c, int pxi, int pyi) {
+  const int W = c.sc.W, H = c.sc.H;
+  const R pixx = (R)pxi, pixy = (R)pyi;
+  R T = (R)1;
+  R C[3] = {0, 0, 0}, N[3] = {0, 0, 0};
+  R D = 0, M1 = 0, M2 = 0, distortion = 0, median_depth = 0;
+  const R near_n = (R)0.2f, far_n = (R)100.0f;
+  const R mcoef = far_n / (far_n - near_n);
+  const size_t HW = (size_t)H * W, pid = (size_t)W * pyi + pxi;
+  for (const uint32_t word : c.dec_pix[pid]) {
+    const uint32_t e = word >> 3, fl = word & 7u;
+    const int id = (int)c.point_list[e];
+    const R* T9 = &c.transMat[9 * id];
+    const V3<R> Tu = v3(T9[0], T9[1], T9[2]), Tv = v3(T9[3], T9[4], T9[5]), Tw = v3(T9[6], T9[7], T9[8]);
+    const V3<R> k = pixx * Tw - Tu;
+    const V3<R> l = pixy * Tw - Tv;
+    const V3<R> p = cross(k, l);
+    const R sx = p.x / p.z, sy = p.y / p.z;
+    const R rho3d = sx * sx + sy * sy;
+    const R dx = c.means2D[2 * id] - pixx, dy = c.means2D[2 * id + 1] - pixy;
+    const R rho2d = (R)2 * (dx * dx + dy * dy);
+    const R rho = (fl & 1u) ? rho3d : rho2d;
+    const R depth = (fl & 1u) ? (sx * Tw.x + sy * Tw.y) + Tw.z : Tw.z;
+    const R* no = &c.normal_opacity[4 * id];
+    const R alpha = (fl & 2u) ? (R)0.99f : no[3] * rexp<R>((R)-0.5f * rho);
+    const R w = alpha * T;
+    const R A = (R)1 - T;
+    const R mm = mcoef * ((R)1 - near_n / depth);
+    distortion += (((mm * mm) * A + M2) - ((R)2 * mm) * M1) * w;
+    D += depth * w;
+    M1 += mm * w;
+    M2 += (mm * mm) * w;
+    if (fl & 4u) median_depth = depth;
+    for (int ch = 0; ch < 3; ++ch) N[ch] += no[ch] * w;
+    for (int ch = 0; ch < 3; ++ch) C[ch] += c.rgb[3 * id + ch] * w;
+    T = T * ((R)1 - alpha);
+  }
+  c.final_T[pid] = T;
+  for (int ch = 0; ch < 3; ++ch) c.out_color[ch * HW + pid] = C[ch] + T * c.bg[ch];
+  c.final_T[pid + HW] = M1;
+  c.final_T[pid + 2 * HW] = M2;
+  c.out_others[pid] = D;
+  c.out_others[pid + HW] = (R)1 - T;
+  for (int ch = 0; ch < 3; ++ch) c.out_others[pid + (2 + ch) * HW] = N[ch];
+  c.out_others[pid + 5 * HW] = median_depth;
+  c.out_others[pid + 6 * HW] = distortion;
+}
+
+// Replays a recorded forward on the inputs now loaded (same surfel count,
+// image and tile size): preprocess with the recorded culls, normal flips,
+// radii and colour clamps, the recorded tile lists, and the recorded blends.
+template <typename R> void run_replay(Ctx<R>& c) {
+  const Scene& s = c.sc;
+  c.replay = true;
+  c.record = false;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < s.P; ++i) preprocess_one(c, i);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int y = 0; y < s.H; ++y)
+    for (int x = 0; x < s.W; ++x) render_pixel_replay(c, x, y);
+  c.replay = false;
 }
 
 template <typename R> void run_forward(Ctx<R>& c) {
@@ -487,6 +573,10 @@ template <typename R> void run_forward(Ctx<R>& c) {
   c.rgb.assign(3 * (size_t)P, 0);
   c.clamped.assign(3 * (size_t)P, 0);
   c.prefilter_violation = 0;
+  if (c.record) {
+    c.dec_mult.assign(P, 0);
+    c.dec_pix.assign((size_t)s.W * s.H, std::vector<uint32_t>());
+  }
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < P; ++i) preprocess_one(c, i);
   binning(c);
@@ -503,6 +593,12 @@ template <typename R> void run_forward(Ctx<R>& c) {
         const int x = tx * B + i, y = ty * B + j;
         if (x < s.W && y < s.H) render_pixel(c, t, x, y);
       }
+  }
+  if (c.record) {
+    int n = 0;
+    for (const auto& v : c.dec_pix)
+      for (const uint32_t w : v) n += (w & 2u) ? 1 : 0;
+    c.dec_alpha_clamped = n;
   }
 }
 
@@ -1032,6 +1128,45 @@ int sro_forward(void* hv, const sro_scene_c* s, int f64) {
   load_scene(h->cf, s, 0);
   run_forward(h->cf);
   return h->cf.prefilter_violation;
+}
+
+// Forward that keeps its discrete decisions for sro_replay.  Returns as
+// sro_forward, or -2 when the instance count does not fit the record words.
+int sro_forward_record(void* hv, const sro_scene_c* s, int f64) {
+  Handle* h = (Handle*)hv;
+  if (!h) return -1;
+  h->cf.record = !f64;
+  h->cd.record = f64 != 0;
+  const int rc = sro_forward(hv, s, f64);
+  h->cf.record = h->cd.record = false;
+  const int L = f64 ? h->cd.L : h->cf.L;
+  if (L >= (1 << 29)) return -2;
+  return rc;
+}
+
+// Re-evaluates the forward recorded by sro_forward_record on new inputs
+// (same P, image, tile, precision and optional-input pattern) with every
+// discrete decision held: culls, normal flips, radii and tile lists, colour
+// clamps, and per pixel which entries blend, their branch, alpha cap and
+// median.  The outputs replace the recorded run's.
+int sro_replay(void* hv, const sro_scene_c* s) {
+  Handle* h = (Handle*)hv;
+  if (!h || !s) return -1;
+  auto go = [&](auto& c, int f64) {
+    if (c.dec_pix.empty() || s->P != c.sc.P || s->W != c.sc.W || s->H != c.sc.H || s->tile != c.sc.B) return -1;
+    load_scene(c, s, f64);
+    run_replay(c);
+    return 0;
+  };
+  return h->f64 ? go(h->cd, 1) : go(h->cf, 0);
+}
+
+// Blended pairs of the recorded forward whose alpha was capped at 0.99 (the
+// backward differentiates through the cap, so a replay of such a pair is
+// not what the gradient describes).
+int sro_alpha_capped(void* hv) {
+  Handle* h = (Handle*)hv;
+  return h->f64 ? h->cd.dec_alpha_clamped : h->cf.dec_alpha_clamped;
 }
 
 int sro_num_rendered(void* hv) {

@@ -56,6 +56,7 @@ EXPORTED = (
     "sr_batch_workspace_bytes",
     "sr_forward_batched",
     "sr_backward_batched",
+    "sr_pair_count",
 )
 
 SR_MAX_VIEWS = 32
@@ -79,6 +80,7 @@ class SrSettings(C.Structure):
         ("tile", C.c_int),
         ("pair_store", C.c_int),
         ("capacity", C.c_int),
+        ("pair_capacity", C.c_int),
     ]
 
 
@@ -124,6 +126,7 @@ class SrViews(C.Structure):
         ("sh_degree", C.c_int),
         ("tile", C.c_int),
         ("pair_store", C.c_int),
+        ("pair_capacity", C.c_int),
     ]
 
 
@@ -161,7 +164,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_geom_bytes.restype = sz
         lib.sr_image_bytes.argtypes = [i, i, i]
         lib.sr_image_bytes.restype = sz
-        lib.sr_binning_bytes.argtypes = [i, i, i, i, i]
+        lib.sr_binning_bytes.argtypes = [i, i, i, i, i, i]
         lib.sr_binning_bytes.restype = sz
         lib.sr_state_offset.argtypes = [C.c_char_p, i, i, i, i, i]
         lib.sr_state_offset.restype = sz
@@ -188,6 +191,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_backward_batched.restype = i
         lib.sr_version.argtypes = []
         lib.sr_version.restype = C.c_char_p
+        lib.sr_pair_count.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint)]
+        lib.sr_pair_count.restype = i
         _lib = lib
         return lib
 
@@ -224,6 +229,17 @@ def state_offset(name: str, P: int, W: int, H: int, tile: int, L: int) -> int:
     if off == (1 << 64) - 1:
         raise KeyError(name)
     return off
+
+
+def pair_count():
+    """(accepted pairs, tile instances) of the latest completed forward of
+    this thread on the current device, or None before one completed (never
+    waits)."""
+    n = C.c_ulonglong(0)
+    L = C.c_uint(0)
+    if load().sr_pair_count(C.byref(n), C.byref(L)) != SR_OK:
+        return None
+    return int(n.value), int(L.value)
 
 
 def set_timing(enabled: bool) -> None:

@@ -29,7 +29,9 @@ from . import _lib
 @dataclass
 class BatchConfig:
     tile: int = 16
-    pair_store: int = 0  # 0 = 24 pairs per instance
+    # accepted-pair store per instance; 0 sizes it from the pairs the last
+    # forward asked for (status words 4-5), with an eighth of headroom
+    pair_store: int = 0
 
 
 class BatchRasterizer:
@@ -68,8 +70,9 @@ class BatchRasterizer:
             self.pix_base.append(self.pix_base[-1] + n)
         self.N = sum(self.pixels)
         self.capacity = int(capacity) if capacity else 0
+        self.pair_capacity = 0
         self.workspace = None
-        self.status = torch.zeros(4, dtype=torch.int32, device=self.device)
+        self.status = torch.zeros(8, dtype=torch.int32, device=self.device)
         self._P = None
         self._host = dict(w=(C.c_int * self.V)(*self.widths), h=(C.c_int * self.V)(*self.heights),
                           tx=(C.c_float * self.V)(*self.tanx), ty=(C.c_float * self.V)(*self.tany))
@@ -88,6 +91,7 @@ class BatchRasterizer:
         v.sh_degree = self.sh_degree
         v.tile = self.config.tile
         v.pair_store = self.config.pair_store
+        v.pair_capacity = self.pair_capacity
         return v
 
     @staticmethod
@@ -104,22 +108,44 @@ class BatchRasterizer:
         g.transMat_precomp = None
         return g
 
-    def workspace_bytes(self, P: int, capacity: int) -> int:
-        return int(self.lib.sr_batch_workspace_bytes(C.byref(self._views()), P, capacity))
+    def workspace_bytes(self, P: int, capacity: int, pair_capacity: int | None = None) -> int:
+        saved = self.pair_capacity
+        if pair_capacity is not None:
+            self.pair_capacity = int(pair_capacity)
+        try:
+            return int(self.lib.sr_batch_workspace_bytes(C.byref(self._views()), P, capacity))
+        finally:
+            self.pair_capacity = saved
 
-    def ensure_workspace(self, P: int, capacity: int | None = None) -> None:
-        """Sizes the workspace for P surfels and a capacity of (surfel, tile)
-        instances.  With no capacity known yet it starts at 1: the first
-        forward then stops after the tile histogram and reports the exact
-        count, which the next call allocates for."""
+    def _auto_pairs(self, capacity: int, pairs: int | None = None) -> int:
+        """Pair-store size for a capacity: 0 (pair_store per instance) when
+        the config fixes pair_store, else the measured pair count with an
+        eighth of headroom, or 8 pairs per instance before any count."""
+        if self.config.pair_store > 0:
+            return 0
+        n = pairs if pairs is not None else 8 * capacity
+        return min(int(n * 1.125) + 4096, 0x7FFFFFFF)
+
+    def ensure_workspace(self, P: int, capacity: int | None = None, pair_capacity: int | None = None) -> None:
+        """Sizes the workspace for P surfels, a capacity of (surfel, tile)
+        instances and an accepted-pair store.  With no capacity known yet it
+        starts at 1: the first forward then stops after the tile histogram
+        and reports the exact count, which the next call allocates for."""
         cap = int(capacity or self.capacity or 1)
-        need = self.workspace_bytes(P, cap)
+        if pair_capacity is not None:
+            pcap = int(pair_capacity)
+        elif cap == self.capacity and self.pair_capacity > 0:
+            pcap = self.pair_capacity  # keep a store sized from a measured count
+        else:
+            pcap = self._auto_pairs(cap)
+        need = self.workspace_bytes(P, cap, pcap)
         if need == 0:
             raise RuntimeError("invalid batch description")
         if self.workspace is None or self.workspace.numel() < need:
             self.workspace = None
             self.workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
         self.capacity = cap
+        self.pair_capacity = pcap
         self._P = P
 
     # -- raw passes (graph-capturable) ---------------------------------------------
@@ -163,6 +189,11 @@ class BatchRasterizer:
         """(instances, flags) of the last forward; synchronises the stream."""
         st = self.status.cpu()
         return int(st[0]) & 0xFFFFFFFF, int(st[1])
+
+    def read_pairs(self) -> int:
+        """Accepted pairs the last forward asked the pair store for."""
+        st = self.status.cpu()
+        return (int(st[4]) & 0xFFFFFFFF) | ((int(st[5]) & 0xFFFFFFFF) << 32)
 
     def forward_checked(self, means3D, shs, colors, opacities, scales, rotations):
         """Forward that grows the workspace until the batch fits (host sync)."""
@@ -212,7 +243,7 @@ class _RasterizeBatch(torch.autograd.Function):
         color, others, radii = br.forward_checked(*args)
         ctx.br = br
         ctx.args = args
-        ctx.capacity = br.capacity
+        ctx.capacity = (br.capacity, br.pair_capacity)
         ctx.save_for_backward(radii)
         ctx.mark_non_differentiable(radii)
         return color, others, radii
@@ -223,7 +254,7 @@ class _RasterizeBatch(torch.autograd.Function):
         (radii,) = ctx.saved_tensors
         means3D, shs, colors, opac, scales, rots = ctx.args
         P = means3D.shape[0]
-        if br.capacity != ctx.capacity:
+        if (br.capacity, br.pair_capacity) != ctx.capacity:
             raise RuntimeError("the batch workspace was reallocated between forward and backward")
         M = shs.shape[1] if shs is not None else 0
         grads = br.new_grads(P, M)
@@ -266,7 +297,9 @@ def capture_step(br: BatchRasterizer, means3D, shs, colors, opacities, scales, r
     P = means3D.shape[0]
     br.forward_checked(means3D, shs, colors, opacities, scales, rotations)
     need, _ = br.read_status()
-    br.ensure_workspace(P, max(br.capacity, int(need * headroom) + 1024))
+    cap = max(br.capacity, int(need * headroom) + 1024)
+    pairs = br.read_pairs()
+    br.ensure_workspace(P, cap, br._auto_pairs(cap, int(pairs * headroom)))
     outs = br.new_outputs(P)
     M = shs.shape[1] if shs is not None else 0
     grads = br.new_grads(P, M)

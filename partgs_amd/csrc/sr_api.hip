@@ -136,6 +136,7 @@ int validate(const sr_settings* s, const sr_surfels* g) {
   if (s->image_width <= 0 || s->image_height <= 0 || s->image_width > 32000 || s->image_height > 32000)
     return fail(SR_ERR_INVALID, "image size must be in [1, 32000] (got %d x %d)", s->image_width, s->image_height);
   if (s->pair_store < 0) return fail(SR_ERR_INVALID, "pair_store must be >= 0");
+  if (s->pair_capacity < 0) return fail(SR_ERR_INVALID, "pair_capacity must be >= 0");
   if (s->tile != 8 && s->tile != 16 && s->tile != 32)
     return fail(SR_ERR_INVALID, "tile must be 8, 16 or 32 (got %d)", s->tile);
   if (g->P == 0) return SR_OK;  // the reference glue returns before looking at the inputs
@@ -161,7 +162,7 @@ int validate(const sr_settings* s, const sr_surfels* g) {
 // ---- pipeline stages shared by the single-view and batched entry points ----
 struct Pipeline {
   ViewSet vs;
-  int P = 0, s_D = 0, prefiltered = 0, pair_store = 0;
+  int P = 0, s_D = 0, prefiltered = 0, pair_store = 0, pair_capacity = 0;
   float mod = 1.0f;
   const float* vm = nullptr;
   const float* pm = nullptr;
@@ -185,6 +186,47 @@ PairStore pair_store(char* bin, const BinLayout& bl) {
   return p;
 }
 
+// Pinned landing zones and events for the accepted-pair count of recent
+// forwards, per (host thread, device): a ring of kPairSlots, so enqueuing a
+// forward only ever waits for the copy of the forward kPairSlots before it
+// (long finished), never for the one still running.
+constexpr int kPairSlots = 4;
+struct PairCount {
+  uint32_t* host = nullptr;  // kPairSlots x 8 words: counters[0..5] of a forward
+  hipEvent_t ev[kPairSlots] = {};
+  unsigned long long seq[kPairSlots] = {};  // 0 = slot never used
+  unsigned long long next_seq = 1;
+  int next = 0;
+};
+PairCount& pair_count_slot() {
+  thread_local PairCount slots[16];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+  return slots[dev];
+}
+
+// Enqueues the copy of the instance and pair counts (counters[0..5]) of the
+// forward just enqueued into the next slot of the ring.
+int post_pair_count(Pipeline& pl, hipStream_t st) {
+  PairCount& pc = pair_count_slot();
+  if (!pc.host) {
+    if (hipHostMalloc((void**)&pc.host, kPairSlots * 8 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+      pc.host = nullptr;
+      return fail(SR_ERR_HIP, "hipHostMalloc failed");
+    }
+    for (int k = 0; k < kPairSlots; ++k)
+      if (hipEventCreateWithFlags(&pc.ev[k], hipEventDisableTiming) != hipSuccess)
+        return fail(SR_ERR_HIP, "hipEventCreate failed");
+  }
+  const int k = pc.next;
+  if (pc.seq[k] != 0) SR_HIP(hipEventSynchronize(pc.ev[k]));  // the copy of kPairSlots forwards ago
+  SR_HIP(hipMemcpyAsync(pc.host + 8 * k, pl.counters(), 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SR_HIP(hipEventRecord(pc.ev[k], st));
+  pc.seq[k] = pc.next_seq++;
+  pc.next = (k + 1) % kPairSlots;
+  return SR_OK;
+}
+
 // Per-record gradient outputs of the backward preprocess (record r = v P + i):
 // dL_dtransMat 9, dL_dmean2D 3, dL_dopacity 1, dL_dcolors 3 floats each.
 struct RecordGrads {
@@ -202,12 +244,12 @@ struct BatchLayout {
   ImageLayout il;
   BinLayout bl;
 };
-BatchLayout batch_layout(const ViewSet& vs, int P, int capacity, int pair_store) {
+BatchLayout batch_layout(const ViewSet& vs, int P, int capacity, int pair_store, int pair_capacity) {
   BatchLayout b{};
   const size_t R = (size_t)vs.V * (size_t)(P > 0 ? P : 0);
   b.gl = geom_layout(R);
   b.il = image_layout(vs);
-  b.bl = bin_layout(capacity, vs.B, (int)vs.T, pair_store);
+  b.bl = bin_layout(capacity, vs.B, (int)vs.T, pair_store, pair_capacity);
   size_t off = 0;
   b.geom = off; off += align_up(b.gl.total);
   b.img = off; off += align_up(b.il.total);
@@ -229,6 +271,7 @@ int validate_views(const sr_views* v, const sr_surfels* g, int capacity) {
   if (v->tile != 8 && v->tile != 16 && v->tile != 32)
     return fail(SR_ERR_INVALID, "tile must be 8, 16 or 32 (got %d)", v->tile);
   if (v->pair_store < 0) return fail(SR_ERR_INVALID, "pair_store must be >= 0");
+  if (v->pair_capacity < 0) return fail(SR_ERR_INVALID, "pair_capacity must be >= 0");
   if (capacity < 1) return fail(SR_ERR_INVALID, "capacity must be >= 1");
   size_t pixels = 0;
   for (int i = 0; i < v->V; ++i) {
@@ -258,6 +301,7 @@ Pipeline batch_pipeline(const sr_views* v, const sr_surfels* g, const ViewSet& v
   pl.prefiltered = 0;
   pl.mod = v->scale_modifier;
   pl.pair_store = v->pair_store;
+  pl.pair_capacity = v->pair_capacity;
   pl.vm = v->viewmatrices;
   pl.pm = v->projmatrices;
   pl.campos = v->campos;
@@ -484,9 +528,27 @@ const char* sr_version(void) { return "surfel_rast gfx950 1.0"; }
 
 size_t sr_geom_bytes(int P) { return geom_layout((size_t)(P < 0 ? 0 : P)).total; }
 size_t sr_image_bytes(int W, int H, int tile) { return image_layout(W, H, tile > 0 ? tile : 16).total; }
-size_t sr_binning_bytes(int L, int W, int H, int tile, int pair_store) {
+size_t sr_binning_bytes(int L, int W, int H, int tile, int pair_store, int pair_capacity) {
   const int B = tile > 0 ? tile : 16;
-  return bin_layout(L, B, image_layout(W, H, B).T, pair_store).total;
+  return bin_layout(L, B, image_layout(W, H, B).T, pair_store, pair_capacity).total;
+}
+
+int sr_pair_count(unsigned long long* pairs, unsigned int* instances) {
+  if (!pairs || !instances) return fail(SR_ERR_INVALID, "NULL argument");
+  PairCount& pc = pair_count_slot();
+  // newest slot whose copy has landed
+  int best = -1;
+  for (int k = 0; k < kPairSlots; ++k) {
+    if (pc.seq[k] == 0 || (best >= 0 && pc.seq[k] < pc.seq[best])) continue;
+    const hipError_t q = hipEventQuery(pc.ev[k]);
+    if (q == hipSuccess) best = k;
+    else if (q != hipErrorNotReady) return fail(SR_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+  }
+  if (best < 0) return fail(SR_ERR_INVALID, "no forward has completed on this thread and device");
+  const uint32_t* c = pc.host + 8 * best;
+  *instances = c[0];
+  *pairs = (unsigned long long)c[4] | ((unsigned long long)c[5] << 32);
+  return SR_OK;
 }
 
 size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int L) {
@@ -545,6 +607,7 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
   pl.prefiltered = s->prefiltered;
   pl.mod = s->scale_modifier;
   pl.pair_store = s->pair_store;
+  pl.pair_capacity = s->pair_capacity;
   pl.vm = s->viewmatrix;
   pl.pm = s->projmatrix;
   pl.campos = s->campos;
@@ -561,7 +624,7 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     // exceed it.  The instance count is copied back right after the scan
     // and waited for only once the rest is enqueued, by when it has long
     // arrived, so the stream never drains mid-forward.
-    pl.bl = bin_layout(cap, B, il.T, s->pair_store);
+    pl.bl = bin_layout(cap, B, il.T, s->pair_store, s->pair_capacity);
     pl.bin = (char*)alloc(ctx, pl.bl.total, 1);
     if (!pl.bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
     pl.capacity = cap;
@@ -573,6 +636,7 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     SR_HIP(hipEventRecord(counted, st));
     rc = run_binning(pl, radii, dbg, st);
     if (rc == SR_OK) rc = run_blend_fwd(pl, out_color, out_others, dbg, st);
+    if (rc == SR_OK) rc = post_pair_count(pl, st);
     SR_HIP(hipEventSynchronize(counted));
     if (rc != SR_OK) return rc;
     if (host[1] & kErrPrefiltered)
@@ -590,13 +654,16 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     return fail(SR_ERR_PREFILTERED, "Point is filtered although prefiltered is set. This shouldn't happen!");
   const int L = (int)host[0];
   *num_rendered = L;
-  pl.bl = bin_layout(L, B, il.T, s->pair_store);
+  pl.bl = bin_layout(L, B, il.T, s->pair_store, s->pair_capacity);
   pl.bin = (char*)alloc(ctx, pl.bl.total, 1);
   if (!pl.bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
   pl.capacity = L;
   if (L > 0) {
     rc = run_binning(pl, radii, dbg, st);
     if (rc != SR_OK) return rc;
+    rc = run_blend_fwd(pl, out_color, out_others, dbg, st);
+    if (rc != SR_OK) return rc;
+    return post_pair_count(pl, st);
   } else {
     SR_STAGE(3, dbg, st, "clear",
              launch_fill_u32((uint32_t*)(img + il.sub_count), 0u, (size_t)il.T * quads_per_tile(B), st));
@@ -637,7 +704,7 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
   // the binning arrays are laid out for the forward's capacity (R when 0)
   const int cap = s->capacity > 0 ? s->capacity : R;
   if (cap < R) return fail(SR_ERR_INVALID, "capacity %d below num_rendered %d", s->capacity, R);
-  pl.bl = bin_layout(cap, B, pl.il.T, s->pair_store);
+  pl.bl = bin_layout(cap, B, pl.il.T, s->pair_store, s->pair_capacity);
   pl.geom = (char*)geom_buf;
   pl.img = (char*)img_buf;
   pl.bin = (char*)bin_buf;
@@ -652,7 +719,7 @@ size_t sr_batch_workspace_bytes(const sr_views* v, int P, int capacity) {
   if (!v || v->V < 1 || v->V > SR_MAX_VIEWS || P < 0 || capacity < 0 || !v->widths || !v->heights) return 0;
   const int B = v->tile > 0 ? v->tile : 16;
   return batch_layout(make_viewset(v->V, v->widths, v->heights, v->tanfovx, v->tanfovy, B), P, capacity,
-                      v->pair_store).total;
+                      v->pair_store, v->pair_capacity).total;
 }
 
 int sr_forward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t ws_bytes, int capacity,
@@ -662,7 +729,7 @@ int sr_forward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t 
   if (rc != SR_OK) return rc;
   if (!ws || !out_color || !out_others || !radii || !status) return fail(SR_ERR_INVALID, "NULL argument");
   const ViewSet vs = make_viewset(v->V, v->widths, v->heights, v->tanfovx, v->tanfovy, v->tile);
-  const BatchLayout bl = batch_layout(vs, g->P, capacity, v->pair_store);
+  const BatchLayout bl = batch_layout(vs, g->P, capacity, v->pair_store, v->pair_capacity);
   if (ws_bytes < bl.total)
     return fail(SR_ERR_INVALID, "workspace holds %zu bytes, needs %zu", ws_bytes, bl.total);
   Pipeline pl = batch_pipeline(v, g, vs, bl, (char*)ws, capacity);
@@ -693,7 +760,7 @@ int sr_backward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t
       !gr->dL_drotations || (g->shs && !gr->dL_dsh))
     return fail(SR_ERR_INVALID, "gradient output missing");
   const ViewSet vs = make_viewset(v->V, v->widths, v->heights, v->tanfovx, v->tanfovy, v->tile);
-  const BatchLayout bl = batch_layout(vs, g->P, capacity, v->pair_store);
+  const BatchLayout bl = batch_layout(vs, g->P, capacity, v->pair_store, v->pair_capacity);
   if (ws_bytes < bl.total)
     return fail(SR_ERR_INVALID, "workspace holds %zu bytes, needs %zu", ws_bytes, bl.total);
   Pipeline pl = batch_pipeline(v, g, vs, bl, (char*)ws, capacity);

@@ -1735,7 +1735,7 @@ hipError_t launch_mark_visible(int P, const float* means3D, const float* vm, uin
 // Copies the instance count and the error flags to a caller-visible status
 // word, so batched callers can check capacity without reading internal state.
 __global__ void k_copy_status(const uint32_t* counters, uint32_t* status) {
-  if (threadIdx.x < 4) status[threadIdx.x] = counters[threadIdx.x];
+  if (threadIdx.x < 8) status[threadIdx.x] = counters[threadIdx.x];
 }
 
 hipError_t launch_copy_status(const uint32_t* counters, uint32_t* status, hipStream_t s) {

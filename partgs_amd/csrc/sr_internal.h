@@ -189,7 +189,7 @@ inline size_t work_capacity(int L, int B, int T) {
   const size_t Q = (size_t)quads_per_tile(B);
   return Q * ((size_t)T + ((size_t)(L > 0 ? L : 1) + 63) / 64);
 }
-inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
+inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0, int pair_capacity = 0) {
   BinLayout b{};
   size_t off = 0;
   const size_t n = (size_t)(L > 0 ? L : 1);
@@ -211,7 +211,8 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0) {
   b.inst_quads = off; off += align_up(n * 2);
   b.part = off; off += align_up(n * Q * kPartStride * 4);
   b.part_nz = off; off += align_up(n * Q);
-  b.frag_cap = n * (pair_store > 0 ? (size_t)pair_store : kFragPerInstance);
+  b.frag_cap = pair_capacity > 0 ? (size_t)pair_capacity
+                                  : n * (pair_store > 0 ? (size_t)pair_store : kFragPerInstance);
   if (b.frag_cap > 0x7fffffffu) b.frag_cap = 0x7fffffffu;
   b.prec = off; off += align_up(b.frag_cap * kPairQuads * 16);
   b.fW = off; off += align_up(b.frag_cap * 16);

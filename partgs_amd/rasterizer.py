@@ -64,7 +64,7 @@ class _Arena:
 
 
 def _settings(bg, scale_modifier, viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height, image_width,
-              degree, campos, prefiltered, debug, tile, pair_store=0, capacity=0):
+              degree, campos, prefiltered, debug, tile, pair_store=0, capacity=0, pair_capacity=0):
     s = _lib.SrSettings()
     s.image_height = int(image_height)
     s.image_width = int(image_width)
@@ -81,6 +81,7 @@ def _settings(bg, scale_modifier, viewmatrix, projmatrix, tan_fovx, tan_fovy, im
     s.tile = int(tile)
     s.pair_store = int(pair_store)
     s.capacity = int(capacity)
+    s.pair_capacity = int(pair_capacity)
     return s
 
 
@@ -100,14 +101,18 @@ def _surfels(P, means3D, sh, colors, opacity, scales, rotations, transMat_precom
 
 def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, scale_modifier, transMat_precomp,
                             viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height, image_width, sh, degree,
-                            campos, prefiltered, debug, tile: int = 16, pair_store: int = 0, capacity: int = 0):
+                            campos, prefiltered, debug, tile: int = 16, pair_store: int = 0, capacity: int = 0,
+                            pair_capacity: int = 0):
     """RasterizeGaussiansCUDA (rasterize_points.cu:40-134).
 
     Returns (num_rendered, color (3,H,W), others (7,H,W), radii (P,) int32,
     geomBuffer, binningBuffer, imgBuffer).  capacity > 0 sizes the binning
     buffer for that many tile instances up front, so the stream is not
     synchronised mid-forward; more instances raise CapacityError (whose
-    `needed` is the count).  The backward must be given the same capacity.
+    `needed` is the count).  pair_capacity > 0 sizes the accepted-pair store
+    in pairs (0: pair_store per instance); pairs beyond it are recomputed,
+    with the same results.  The backward must be given the same capacity and
+    pair_capacity.
     """
     if means3D.ndimension() != 2 or means3D.size(1) != 3:
         raise RuntimeError("means3D must have dimensions (num_points, 3)")
@@ -126,7 +131,7 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
     others = torch.empty((7, H, W), dtype=torch.float32, device=dev)
     radii = torch.empty((P,), dtype=torch.int32, device=dev)  # the preprocess writes every entry
     s = _settings(a["bg"], scale_modifier, a["viewmatrix"], a["projmatrix"], tan_fovx, tan_fovy, H, W, degree,
-                  a["campos"], prefiltered, debug, tile, pair_store, capacity)
+                  a["campos"], prefiltered, debug, tile, pair_store, capacity, pair_capacity)
     g = _surfels(P, means3D, a["sh"], a["colors"], a["opacity"], a["scales"], a["rotations"], a["transMat_precomp"])
     arena = _Arena(dev)
     nr = C.c_int(0)
@@ -144,7 +149,7 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
 def rasterize_gaussians_bwd(bg, means3D, radii, colors, scales, rotations, scale_modifier, transMat_precomp,
                             viewmatrix, projmatrix, tan_fovx, tan_fovy, dL_dout_color, dL_dout_others, sh, degree,
                             campos, geomBuffer, R, binningBuffer, imageBuffer, debug, tile: int = 16,
-                            pair_store: int = 0, capacity: int = 0):
+                            pair_store: int = 0, capacity: int = 0, pair_capacity: int = 0):
     """RasterizeGaussiansBackwardCUDA (rasterize_points.cu:137-233).
 
     Returns (dL_dmeans2D, dL_dcolors, dL_dopacity, dL_dmeans3D, dL_dtransMat,
@@ -174,7 +179,7 @@ def rasterize_gaussians_bwd(bg, means3D, radii, colors, scales, rotations, scale
     dL_drotations = torch.empty((P, 4), **opts)
     if P != 0:
         s = _settings(a["bg"], scale_modifier, a["viewmatrix"], a["projmatrix"], tan_fovx, tan_fovy, H, W, degree,
-                      a["campos"], False, debug, tile, pair_store, capacity)
+                      a["campos"], False, debug, tile, pair_store, capacity, pair_capacity)
         g = _surfels(P, means3D, a["sh"], a["colors"], torch.empty(0, **opts), a["scales"], a["rotations"],
                      a["transMat_precomp"])
         gr = _lib.SrGrads()
@@ -211,12 +216,18 @@ def mark_visible(means3D, viewmatrix, projmatrix):
     return present
 
 
-# Binning capacity per (device, tile): the last instance count with an
-# eighth of headroom.  The first forward of a key sizes its binning buffer
-# from the exact count (one stream synchronisation mid-forward, as the
-# reference); later ones enqueue the whole forward at once and only fall back
-# to a larger capacity when the count outgrows it.
+# Binning capacity per (device, tile, width, height): the last instance
+# count with an eighth of headroom.  The first forward of a key sizes its
+# binning buffer from the exact count (one stream synchronisation
+# mid-forward, as the reference); later ones enqueue the whole forward at
+# once and only fall back to a larger capacity when the count outgrows it.
+# The accepted-pair store follows the pairs per instance of the latest
+# completed forward on the device with the same tile (read back without
+# waiting), with an eighth of headroom; pairs beyond the store are
+# recomputed with the same results.
 _capacity_hint = {}
+_pair_ratio = {}
+_PAIRS_PER_INSTANCE = 8.0  # before any count is known
 
 
 def _grown(n: int) -> int:
@@ -230,10 +241,17 @@ class _RasterizeGaussians(torch.autograd.Function):
     def forward(ctx, means3D, means2D, sh, colors_precomp, opacities, scales, rotations, cov3Ds_precomp,
                 raster_settings):
         tile = getattr(raster_settings, "tile", 16)
-        key = (means3D.device, tile)
+        key = (means3D.device, tile, int(raster_settings.image_width), int(raster_settings.image_height))
         hint = 0 if raster_settings.debug else _capacity_hint.get(key, 0)
         cap = hint
+        rkey = (means3D.device, tile)
+        with torch.cuda.device(means3D.device):
+            seen = _lib.pair_count()
+        if seen is not None and seen[1] > 0 and rkey in _pair_ratio:
+            _pair_ratio[rkey] = seen[0] / seen[1]
         while True:
+            ratio = _pair_ratio.get(rkey) or _PAIRS_PER_INSTANCE
+            pcap = min(int(ratio * cap * 1.125) + 4096, 0x7FFFFFFF) if cap > 0 else 0
             try:
                 num_rendered, color, depth, radii, geomBuffer, binningBuffer, imgBuffer = rasterize_gaussians_fwd(
                     raster_settings.bg, means3D, colors_precomp, opacities, scales, rotations,
@@ -241,7 +259,7 @@ class _RasterizeGaussians(torch.autograd.Function):
                     raster_settings.projmatrix, raster_settings.tanfovx, raster_settings.tanfovy,
                     raster_settings.image_height, raster_settings.image_width, sh, raster_settings.sh_degree,
                     raster_settings.campos, raster_settings.prefiltered, raster_settings.debug, tile,
-                    capacity=cap)
+                    capacity=cap, pair_capacity=pcap)
                 break
             except _lib.CapacityError as e:
                 cap = _grown(e.needed)
@@ -249,6 +267,9 @@ class _RasterizeGaussians(torch.autograd.Function):
         # forward, shrunk when the count fell far below it
         if cap != hint or cap == 0 or _grown(num_rendered) * 4 < cap:
             _capacity_hint[key] = _grown(num_rendered)
+        # from now on the pair store follows the measured pairs per instance
+        _pair_ratio.setdefault(rkey, 0.0)
+        ctx.pair_capacity = pcap
         ctx.raster_settings = raster_settings
         ctx.capacity = cap
         ctx.num_rendered = num_rendered
@@ -275,7 +296,7 @@ class _RasterizeGaussians(torch.autograd.Function):
             rs.bg, means3D, radii, colors_precomp, scales, rotations, rs.scale_modifier, cov3Ds_precomp,
             rs.viewmatrix, rs.projmatrix, rs.tanfovx, rs.tanfovy, grad_out_color, grad_depth, sh, rs.sh_degree,
             rs.campos, geomBuffer, num_rendered, binningBuffer, imgBuffer, rs.debug, getattr(rs, "tile", 16),
-            capacity=ctx.capacity)
+            capacity=ctx.capacity, pair_capacity=ctx.pair_capacity)
         return (grad_means3D, grad_means2D, grad_sh, grad_colors_precomp, grad_opacities, grad_scales,
                 grad_rotations, grad_cov3Ds_precomp, None)
 

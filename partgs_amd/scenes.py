@@ -165,3 +165,12 @@ def upstream_grads(seed: int, H: int, W: int):
     """N(0, 1) upstream gradients for color (3,H,W) and others (7,H,W)."""
     g = torch.Generator().manual_seed(seed)
     return torch.randn(3, H, W, generator=g), torch.randn(7, H, W, generator=g)
+
+
+def batched_views(seed: int = 0, n: int = 32, choices=((400, 300), (640, 480), (800, 600))):
+    """n cameras on the radius-3 sphere looking at the origin (camera k from
+    seed 1000 + k) with resolutions drawn from ``choices`` (seed ``seed``):
+    the multi-view batch of 1,000,000-surfel scenes."""
+    rng = np.random.default_rng(seed)
+    res = [tuple(choices[int(k)]) for k in rng.integers(0, len(choices), size=n)]
+    return [random_sphere_camera(1000 + k, 3.0, w, h) for k, (w, h) in enumerate(res)]

@@ -14,7 +14,7 @@ REC_QUADS = 5
 
 
 def gpu_forward(scene, *, tile=16, scale_modifier=1.0, colors=None, transmat=None, prefiltered=False, debug=False,
-                pair_store=0, capacity=0):
+                pair_store=0, capacity=0, pair_capacity=0):
     """Runs the HIP forward on a SurfelScene; returns a dict of outputs and state."""
     dev = torch.device("cuda")
     cam = scene.camera
@@ -29,10 +29,10 @@ def gpu_forward(scene, *, tile=16, scale_modifier=1.0, colors=None, transmat=Non
             math.tan(cam.FoVy * 0.5), cam.image_height, cam.image_width, shs, scene.sh_degree,
             cam.camera_center.to(dev), prefiltered, debug, tile)
     L, color, others, radii, geom, binb, img = rasterize_gaussians_fwd(*args, pair_store=pair_store,
-                                                                       capacity=capacity)
+                                                                       capacity=capacity, pair_capacity=pair_capacity)
     torch.cuda.synchronize()
     out = dict(L=L, color=color, others=others, radii=radii, geom=geom, bin=binb, img=img, args=args, tile=tile,
-               pair_store=pair_store, capacity=capacity)
+               pair_store=pair_store, capacity=capacity, pair_capacity=pair_capacity)
     return out
 
 
@@ -73,7 +73,7 @@ def gpu_backward(res, dL_dcolor, dL_dothers):
     out = rasterize_gaussians_bwd(a[0], a[1], res["radii"], a[2], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11],
                                   dL_dcolor.to(dev), dL_dothers.to(dev), a[14], a[15], a[16], res["geom"], res["L"],
                                   res["bin"], res["img"], a[19], res["tile"], res["pair_store"],
-                                  res.get("capacity", 0))
+                                  res.get("capacity", 0), res.get("pair_capacity", 0))
     torch.cuda.synchronize()
     names = ("dL_dmeans2D", "dL_dcolors", "dL_dopacity", "dL_dmeans3D", "dL_dtransMat", "dL_dsh", "dL_dscales",
              "dL_drotations")

@@ -304,5 +304,31 @@ def test_autograd_api_matches_low_level():
         for t, name in ((means3D, "dL_dmeans3D"), (opac, "dL_dopacity"), (scales, "dL_dscales"),
                         (rots, "dL_drotations"), (means2D, "dL_dmeans2D")):
             assert_close_scaled(t.grad.cpu().numpy(), g[name], 1e-5, name)
-    vis = GaussianRasterizer(rs).markVisible(scene.means3D)
-    assert bool(vis.all())
+
+
+def test_mark_visible_matches_oracle_across_near_plane():
+    """markVisible (dsr/cuda_rasterizer/rasterizer_impl.cu:54-66,141-153):
+    present = view z > 0.2.  Surfels placed at view depths in [0, 0.4], a
+    thousand of them within float rounding of 0.2, the rest on both sides."""
+    from oracle import oracle as O
+    from partgs_amd.rasterizer import GaussianRasterizationSettings, GaussianRasterizer
+
+    scene = surfel_cloud(9, P=20000, width=64, height=48)
+    cam = scene.camera
+    c2w = torch.linalg.inv(cam.world_view_transform.T).double()
+    rng = np.random.default_rng(9)
+    z = rng.uniform(0.0, 0.4, scene.P)
+    z[:1000] = 0.2 + rng.integers(-4, 5, 1000) * np.spacing(np.float32(0.2))
+    xy = rng.uniform(-0.3, 0.3, (scene.P, 2))
+    local = torch.tensor(np.stack([xy[:, 0], xy[:, 1], z, np.ones(scene.P)], 1))
+    means = (local @ c2w.T)[:, :3].float()
+    rs = GaussianRasterizationSettings(image_height=48, image_width=64, tanfovx=0.5, tanfovy=0.5,
+                                       bg=torch.ones(3, device="cuda"), scale_modifier=1.0,
+                                       viewmatrix=cam.world_view_transform.cuda(),
+                                       projmatrix=cam.full_proj_transform.cuda(), sh_degree=0,
+                                       campos=cam.camera_center.cuda(), prefiltered=False, debug=False)
+    vis = GaussianRasterizer(rs).markVisible(means.cuda()).cpu().numpy()
+    want = O.mark_visible(means, cam.world_view_transform)
+    print(f"[parity] mark_visible: {int(want.sum())} of {scene.P} in front", flush=True)
+    assert 0 < want.sum() < scene.P and 0 < want[:1000].sum() < 1000
+    np.testing.assert_array_equal(vis, want)
