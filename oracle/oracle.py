@@ -66,6 +66,7 @@ def load():
         lib.sro_forward_record.restype = C.c_int
         lib.sro_replay.argtypes = [C.c_void_p, C.POINTER(SroScene)]
         lib.sro_replay.restype = C.c_int
+        lib.sro_set_fp64_degenerate.argtypes = [C.c_int]
         lib.sro_alpha_capped.argtypes = [C.c_void_p]
         lib.sro_alpha_capped.restype = C.c_int
         _lib = lib
@@ -101,6 +102,12 @@ def sh_rgb(deg: int, sh, means, campos, f64: bool = False):
     return rgb, cl.astype(bool)
 
 
+def set_fp64_degenerate(on: bool) -> None:
+    """True (the default): near-degenerate surfels take the FP64 preprocess
+    like the HIP kernel; False: all FP32, the reference's own arithmetic."""
+    load().sro_set_fp64_degenerate(1 if on else 0)
+
+
 def higher_msb(n: int) -> int:
     return int(load().sro_higher_msb(n))
 
@@ -114,6 +121,7 @@ _SHAPES = {
     "normal_opacity": ("R", lambda d: (d["P"], 4)),
     "rgb": ("R", lambda d: (d["P"], 3)),
     "clamped": ("u1", lambda d: (d["P"], 3)),
+    "flip": ("i1", lambda d: (d["P"],)),
     "tiles_touched": ("u4", lambda d: (d["P"],)),
     "point_offsets": ("u4", lambda d: (d["P"],)),
     "keys": ("u8", lambda d: (d["L"],)),

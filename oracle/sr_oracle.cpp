@@ -41,6 +41,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -143,6 +144,7 @@ template <typename R> struct Ctx {
   std::vector<int> radii;
   std::vector<R> depths, means2D, transMat, normal_opacity, rgb;
   std::vector<uint8_t> clamped;
+  std::vector<int8_t> flip;  // camera-facing sign of the normal, x2 when the preprocess ran in FP64
   std::vector<uint32_t> tiles_touched, point_offsets;
   int prefilter_violation = 0;
   // binning state
@@ -255,6 +257,123 @@ void get_rect(R px, R py, int r, int B, int gx, int gy, int& x0, int& y0, int& x
 // Forward preprocess, dsr/cuda_rasterizer/forward.cu:75-145 (compute_transmat,
 // compute_aabb) and :148-251 (preprocessCUDA), in_frustum auxiliary.h:185-210.
 // ---------------------------------------------------------------------------
+// Homography rows Tu, Tv, Tw and view-space normal of surfel idx, evaluated
+// in precision G from the stored inputs (forward.cu:75-115, 183-207).
+template <typename G> struct Geo {
+  V3<G> Tu, Tv, Tw, n;
+};
+template <typename G, typename R> Geo<G> surfel_geom(const Ctx<R>& c, int idx) {
+  const Scene& s = c.sc;
+  Geo<G> g;
+  const R* m = &c.means3D[3 * idx];
+  if (!s.has_tprec) {
+    const R* qr = &c.rots[4 * idx];
+    const G qq[4] = {(G)qr[0], (G)qr[1], (G)qr[2], (G)qr[3]};
+    G w, x, y, z;
+    V3<G> col[3];
+    quat_cols(qq, w, x, y, z, col);
+    const G sx = (G)((R)s.mod * c.scales[2 * idx]), sy = (G)((R)s.mod * c.scales[2 * idx + 1]);
+    const V3<G> L0 = sx * col[0], L1 = sy * col[1], L2 = col[2];
+    const R* pm = c.pm.data();
+    const R* vm = c.vm.data();
+    // A(i,c) = sum_r M(r,i) F(r,c), F(r,c) = pm[4r+c]; only c = 0, 1, 3 are needed.
+    auto rowdot = [&](const V3<G>& a, int cc) {
+      return (a.x * (G)pm[cc] + a.y * (G)pm[4 + cc]) + a.z * (G)pm[8 + cc];
+    };
+    const G a00 = rowdot(L0, 0), a01 = rowdot(L0, 1), a03 = rowdot(L0, 3);
+    const G a10 = rowdot(L1, 0), a11 = rowdot(L1, 1), a13 = rowdot(L1, 3);
+    const V3<G> pc = v3((G)m[0], (G)m[1], (G)m[2]);
+    const G a20 = rowdot(pc, 0) + (G)pm[12], a21 = rowdot(pc, 1) + (G)pm[13], a23 = rowdot(pc, 3) + (G)pm[15];
+    const G hw = (G)s.W / (G)2, hw1 = (G)(s.W - 1) / (G)2;
+    const G hh = (G)s.H / (G)2, hh1 = (G)(s.H - 1) / (G)2;
+    g.Tu = v3(a00 * hw + a03 * hw1, a10 * hw + a13 * hw1, a20 * hw + a23 * hw1);
+    g.Tv = v3(a01 * hh + a03 * hh1, a11 * hh + a13 * hh1, a21 * hh + a23 * hh1);
+    g.Tw = v3(a03, a13, a23);
+    g.n = v3(((G)vm[0] * L2.x + (G)vm[4] * L2.y) + (G)vm[8] * L2.z,
+             ((G)vm[1] * L2.x + (G)vm[5] * L2.y) + (G)vm[9] * L2.z,
+             ((G)vm[2] * L2.x + (G)vm[6] * L2.y) + (G)vm[10] * L2.z);
+  } else {
+    const R* t = &c.tprec[9 * idx];
+    g.Tu = v3((G)t[0], (G)t[1], (G)t[2]);
+    g.Tv = v3((G)t[3], (G)t[4], (G)t[5]);
+    g.Tw = v3((G)t[6], (G)t[7], (G)t[8]);
+    g.n = v3((G)0, (G)0, (G)1);
+  }
+  return g;
+}
+
+// compute_aabb with cutoff 3 (forward.cu:119-145) up to the square roots.
+template <typename G> struct Box {
+  G d, px, py, h0x, h0y;
+};
+template <typename G> Box<G> aabb_terms(const Geo<G>& g) {
+  const V3<G> Tu = g.Tu, Tv = g.Tv, Tw = g.Tw;
+  Box<G> b;
+  b.d = ((G)9 * (Tw.x * Tw.x) + (G)9 * (Tw.y * Tw.y)) + (G)-1 * (Tw.z * Tw.z);
+  const G inv = (G)1 / b.d;
+  const G fx = inv * (G)9, fy = inv * (G)9, fz = inv * (G)-1;
+  b.px = (fx * (Tu.x * Tw.x) + fy * (Tu.y * Tw.y)) + fz * (Tu.z * Tw.z);
+  b.py = (fx * (Tv.x * Tw.x) + fy * (Tv.y * Tw.y)) + fz * (Tv.z * Tw.z);
+  b.h0x = b.px * b.px - ((fx * (Tu.x * Tu.x) + fy * (Tu.y * Tu.y)) + fz * (Tu.z * Tu.z));
+  b.h0y = b.py * b.py - ((fx * (Tv.x * Tv.x) + fy * (Tv.y * Tv.y)) + fz * (Tv.z * Tv.z));
+  return b;
+}
+
+// ceil(max(sqrt(max(1e-4, h0x)), sqrt(max(1e-4, h0y)), 3 * 0.707106))
+// (forward.cu:141, 230) in precision R.
+template <typename R> int radius_of(R h0x, R h0y) {
+  const R hx = std::sqrt(rmax((R)1e-4f, h0x));
+  const R hy = std::sqrt(rmax((R)1e-4f, h0y));
+  return f2i(std::ceil(rmax(rmax(hx, hy), (R)(3.0f * 0.707106f))));
+}
+
+// The radius from FP64 h0 exactly: the least integer r >= 3 with r^2 >= h0.
+int radius_exact(double h0x, double h0y) {
+  const double h0 = std::max(h0x, h0y);
+  if (!(h0 < 1.0e18)) return h0 == h0 ? 1000000000 : 3;
+  int r = (int)std::ceil(std::sqrt(std::max(h0, 9.0)));
+  while (r > 3 && (double)(r - 1) * (double)(r - 1) >= h0) --r;
+  while ((double)r * (double)r < h0) ++r;
+  return r;
+}
+
+// Near-degenerate surfels take their homography, normal, camera-facing test
+// and AABB in FP64 (selection on the FP32 values, as the HIP kernel makes
+// it; thresholds and reasons in partgs_amd/csrc/sr_forward.hip):
+// grazing |cos| <= 2^-16 sum|v_i|; conic |d| <= 2^-16 (9 Tw.x^2 +
+// 9 Tw.y^2 + Tw.z^2); tiny, every tangent entry of Tu, Tv below 2^-7 |Tw.z|;
+// radius, h0 within 2^-20 of its magnitude terms of a ceil step; edge, the
+// tile rectangle changes when the centre moves by 2^-18 (|p| + 16) pixels.
+bool g_fp64_degenerate = true;
+const float kDegTol = 1.0f / 65536.0f;
+bool radius_ambiguous(float h0, float mag) {
+  const float e = mag * (1.0f / 1048576.0f);
+  const float lo = std::sqrt(rmax(1e-4f, h0 - e)), hi = std::sqrt(rmax(1e-4f, h0 + e));
+  return std::ceil(rmax(lo, 3.0f * 0.707106f)) != std::ceil(rmax(hi, 3.0f * 0.707106f));
+}
+bool rect_ambiguous(float px, float py, int r, int B, int gx, int gy) {
+  const float ex = (std::fabs(px) + 16.0f) * (1.0f / 262144.0f), ey = (std::fabs(py) + 16.0f) * (1.0f / 262144.0f);
+  int a0, a1, a2, a3, b0, b1, b2, b3;
+  get_rect(px - ex, py - ey, r, B, gx, gy, a0, a1, a2, a3);
+  get_rect(px + ex, py + ey, r, B, gx, gy, b0, b1, b2, b3);
+  return a0 != b0 || a1 != b1 || a2 != b2 || a3 != b3;
+}
+bool degenerate(const Geo<float>& g, float vx, float vy, float vz, float cosv, const Box<float>& b, int B, int gx,
+                int gy) {
+  const float cabs = (std::fabs(vx) + std::fabs(vy)) + std::fabs(vz);  // |n| = 1
+  if (!(std::fabs(cosv) > kDegTol * cabs)) return true;
+  const float dabs = (9.0f * (g.Tw.x * g.Tw.x) + 9.0f * (g.Tw.y * g.Tw.y)) + g.Tw.z * g.Tw.z;
+  if (!(std::fabs(b.d) > kDegTol * dabs)) return true;
+  const float tmax = rmax(rmax(std::fabs(g.Tu.x), std::fabs(g.Tu.y)), rmax(std::fabs(g.Tv.x), std::fabs(g.Tv.y)));
+  if (tmax < (1.0f / 128.0f) * std::fabs(g.Tw.z)) return true;
+  const float inv = 1.0f / b.d;
+  const float fx = std::fabs(inv * 9.0f), fz = std::fabs(inv);
+  const float mx = b.px * b.px + ((fx * (g.Tu.x * g.Tu.x) + fx * (g.Tu.y * g.Tu.y)) + fz * (g.Tu.z * g.Tu.z));
+  const float my = b.py * b.py + ((fx * (g.Tv.x * g.Tv.x) + fx * (g.Tv.y * g.Tv.y)) + fz * (g.Tv.z * g.Tv.z));
+  if (radius_ambiguous(b.h0x, mx) || radius_ambiguous(b.h0y, my)) return true;
+  return rect_ambiguous(b.px, b.py, radius_of(b.h0x, b.h0y), B, gx, gy);
+}
+
 template <typename R> void preprocess_one(Ctx<R>& c, int idx) {
   const Scene& s = c.sc;
   if (c.replay) {
@@ -265,7 +384,6 @@ template <typename R> void preprocess_one(Ctx<R>& c, int idx) {
   }
   const R* m = &c.means3D[3 * idx];
   const R* vm = c.vm.data();
-  const R* pm = c.pm.data();
   const R vx = ((vm[0] * m[0] + vm[4] * m[1]) + vm[8] * m[2]) + vm[12];
   const R vy = ((vm[1] * m[0] + vm[5] * m[1]) + vm[9] * m[2]) + vm[13];
   const R vz = ((vm[2] * m[0] + vm[6] * m[1]) + vm[10] * m[2]) + vm[14];
@@ -273,54 +391,46 @@ template <typename R> void preprocess_one(Ctx<R>& c, int idx) {
     if (s.prefiltered) c.prefilter_violation = 1;
     return;
   }
-  V3<R> Tu, Tv, Tw, nrm;
-  if (!s.has_tprec) {
-    R w, x, y, z;
-    V3<R> col[3];
-    quat_cols(&c.rots[4 * idx], w, x, y, z, col);
-    const R sx = (R)s.mod * c.scales[2 * idx], sy = (R)s.mod * c.scales[2 * idx + 1];
-    const V3<R> L0 = sx * col[0], L1 = sy * col[1], L2 = col[2];
-    // A(i,c) = sum_r M(r,i) F(r,c), F(r,c) = pm[4r+c]; only c = 0, 1, 3 are needed.
-    auto rowdot = [&](const V3<R>& a, int cc) { return (a.x * pm[cc] + a.y * pm[4 + cc]) + a.z * pm[8 + cc]; };
-    const R a00 = rowdot(L0, 0), a01 = rowdot(L0, 1), a03 = rowdot(L0, 3);
-    const R a10 = rowdot(L1, 0), a11 = rowdot(L1, 1), a13 = rowdot(L1, 3);
-    const V3<R> pc = v3(m[0], m[1], m[2]);
-    const R a20 = rowdot(pc, 0) + pm[12], a21 = rowdot(pc, 1) + pm[13], a23 = rowdot(pc, 3) + pm[15];
-    const R hw = (R)s.W / (R)2, hw1 = (R)(s.W - 1) / (R)2;
-    const R hh = (R)s.H / (R)2, hh1 = (R)(s.H - 1) / (R)2;
-    Tu = v3(a00 * hw + a03 * hw1, a10 * hw + a13 * hw1, a20 * hw + a23 * hw1);
-    Tv = v3(a01 * hh + a03 * hh1, a11 * hh + a13 * hh1, a21 * hh + a23 * hh1);
-    Tw = v3(a03, a13, a23);
-    nrm = v3((vm[0] * L2.x + vm[4] * L2.y) + vm[8] * L2.z, (vm[1] * L2.x + vm[5] * L2.y) + vm[9] * L2.z,
-             (vm[2] * L2.x + vm[6] * L2.y) + vm[10] * L2.z);
-  } else {
-    const R* t = &c.tprec[9 * idx];
-    Tu = v3(t[0], t[1], t[2]);
-    Tv = v3(t[3], t[4], t[5]);
-    Tw = v3(t[6], t[7], t[8]);
-    nrm = v3((R)0, (R)0, (R)1);
-  }
+  const Geo<R> g = surfel_geom<R>(c, idx);
   // dual-visible normal flip, forward.cu:209-214
-  const R cosv = -((vx * nrm.x + vy * nrm.y) + vz * nrm.z);
-  if (cosv == (R)0 && !c.replay) return;
-  const R mult = c.replay ? (R)c.dec_mult[idx] : (cosv > (R)0 ? (R)1 : (R)-1);
+  const R cosv = -((vx * g.n.x + vy * g.n.y) + vz * g.n.z);
+  const Box<R> b = aabb_terms(g);
+  V3<R> Tu = g.Tu, Tv = g.Tv, Tw = g.Tw, nrm = g.n;
+  R px = b.px, py = b.py, mult;
+  int ir;
+  bool fp64 = false;
+  if constexpr (std::is_same<R, float>::value)
+    fp64 = g_fp64_degenerate && degenerate(g, vx, vy, vz, cosv, b, s.B, c.gx, c.gy);
+  if (!fp64) {
+    if (cosv == (R)0 && !c.replay) return;
+    if (b.d == (R)0 && !c.replay) return;
+    mult = cosv > (R)0 ? (R)1 : (R)-1;
+    ir = radius_of(b.h0x, b.h0y);
+  } else {
+    const Geo<double> G = surfel_geom<double>(c, idx);
+    const double dvx = (((double)vm[0] * m[0] + (double)vm[4] * m[1]) + (double)vm[8] * m[2]) + (double)vm[12];
+    const double dvy = (((double)vm[1] * m[0] + (double)vm[5] * m[1]) + (double)vm[9] * m[2]) + (double)vm[13];
+    const double dvz = (((double)vm[2] * m[0] + (double)vm[6] * m[1]) + (double)vm[10] * m[2]) + (double)vm[14];
+    const double dcos = -((dvx * G.n.x + dvy * G.n.y) + dvz * G.n.z);
+    if (dcos == 0.0 && !c.replay) return;
+    const Box<double> B = aabb_terms(G);
+    if (B.d == 0.0 && !c.replay) return;
+    mult = dcos > 0.0 ? (R)1 : (R)-1;
+    Tu = v3((R)G.Tu.x, (R)G.Tu.y, (R)G.Tu.z);
+    Tv = v3((R)G.Tv.x, (R)G.Tv.y, (R)G.Tv.z);
+    Tw = v3((R)G.Tw.x, (R)G.Tw.y, (R)G.Tw.z);
+    nrm = v3((R)G.n.x, (R)G.n.y, (R)G.n.z);
+    px = (R)B.px;
+    py = (R)B.py;
+    ir = radius_exact(B.h0x, B.h0y);
+  }
+  if (c.replay) mult = (R)c.dec_mult[idx];
   if (c.record) c.dec_mult[idx] = (int8_t)(mult > (R)0 ? 1 : -1);
+  c.flip[idx] = (int8_t)((mult > (R)0 ? 1 : -1) * (fp64 ? 2 : 1));
   nrm = mult * nrm;
-  // compute_aabb with cutoff 3, forward.cu:119-145
-  const R d = ((R)9 * (Tw.x * Tw.x) + (R)9 * (Tw.y * Tw.y)) + (R)-1 * (Tw.z * Tw.z);
-  if (d == (R)0) return;
-  const R inv = (R)1 / d;
-  const R fx = inv * (R)9, fy = inv * (R)9, fz = inv * (R)-1;
-  const R px = (fx * (Tu.x * Tw.x) + fy * (Tu.y * Tw.y)) + fz * (Tu.z * Tw.z);
-  const R py = (fx * (Tv.x * Tw.x) + fy * (Tv.y * Tw.y)) + fz * (Tv.z * Tw.z);
-  const R h0x = px * px - ((fx * (Tu.x * Tu.x) + fy * (Tu.y * Tu.y)) + fz * (Tu.z * Tu.z));
-  const R h0y = py * py - ((fx * (Tv.x * Tv.x) + fy * (Tv.y * Tv.y)) + fz * (Tv.z * Tv.z));
-  const R hx = std::sqrt(rmax((R)1e-4f, h0x));
-  const R hy = std::sqrt(rmax((R)1e-4f, h0y));
-  const R radius = std::ceil(rmax(rmax(hx, hy), (R)(3.0f * 0.707106f)));
-  const int ir = c.replay ? c.radii[idx] : f2i(radius);
+  if (c.replay) ir = c.radii[idx];
   int x0, y0, x1, y1;
-  get_rect(px, py, ir, s.B, c.gx, c.gy, x0, y0, x1, y1);
+  get_rect(px, py, ir, s.B, c.gx, c.gy, x0, y0, x1, y1);  // from the stored centre, as the binning does
   const uint32_t area = c.replay ? c.tiles_touched[idx] : (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
   if (area == 0) return;
   if (!s.has_colors) {
@@ -572,6 +682,7 @@ template <typename R> void run_forward(Ctx<R>& c) {
   c.normal_opacity.assign(4 * (size_t)P, 0);
   c.rgb.assign(3 * (size_t)P, 0);
   c.clamped.assign(3 * (size_t)P, 0);
+  c.flip.assign((size_t)P, 0);
   c.prefilter_violation = 0;
   if (c.record) {
     c.dec_mult.assign(P, 0);
@@ -837,7 +948,7 @@ template <typename R> void preprocess_bwd_one(Ctx<R>& c, int idx) {
   const R* pm = c.pm.data();
   const R* vm = c.vm.data();
   R* gT = &c.g_transMat[9 * idx];
-  V3<R> T0, T1, T2, nrm, L0c, L1c;
+  V3<R> T0, T1, T2, L0c, L1c;
   V3<R> Rc[3];
   R qw = 0, qx = 0, qy = 0, qz = 0, scx = 0, scy = 0;
   V3<R> Pc[3][2];  // P columns as (xyz, w) halves: P[k] = (a, b) with a = rows 0..2, b.x = row 3
@@ -871,9 +982,6 @@ template <typename R> void preprocess_bwd_one(Ctx<R>& c, int idx) {
     T0 = v3(dotM(L0c, 0, 0), dotM(L1c, 0, 0), dotM(pc, 1, 0));
     T1 = v3(dotM(L0c, 0, 1), dotM(L1c, 0, 1), dotM(pc, 1, 1));
     T2 = v3(dotM(L0c, 0, 2), dotM(L1c, 0, 2), dotM(pc, 1, 2));
-    const V3<R> L2 = Rc[2];
-    nrm = v3((vm[0] * L2.x + vm[4] * L2.y) + vm[8] * L2.z, (vm[1] * L2.x + vm[5] * L2.y) + vm[9] * L2.z,
-             (vm[2] * L2.x + vm[6] * L2.y) + vm[10] * L2.z);
   }
   (void)Pc;
   V3<R> dT0 = v3(gT[0], gT[1], gT[2]), dT1 = v3(gT[3], gT[4], gT[5]), dT2 = v3(gT[6], gT[7], gT[8]);
@@ -912,11 +1020,10 @@ template <typename R> void preprocess_bwd_one(Ctx<R>& c, int idx) {
     const R* gn = &c.g_normal[3 * idx];
     V3<R> dL_dtn = v3((vm[0] * gn[0] + vm[1] * gn[1]) + vm[2] * gn[2], (vm[4] * gn[0] + vm[5] * gn[1]) + vm[6] * gn[2],
                       (vm[8] * gn[0] + vm[9] * gn[1]) + vm[10] * gn[2]);
-    const R pvx = ((vm[0] * m[0] + vm[4] * m[1]) + vm[8] * m[2]) + vm[12];
-    const R pvy = ((vm[1] * m[0] + vm[5] * m[1]) + vm[9] * m[2]) + vm[13];
-    const R pvz = ((vm[2] * m[0] + vm[6] * m[1]) + vm[10] * m[2]) + vm[14];
-    const R cosv = -((pvx * nrm.x + pvy * nrm.y) + pvz * nrm.z);
-    const R mult = cosv > (R)0 ? (R)1 : (R)-1;
+    // the camera-facing sign the forward chose (backward.cu:550-555 re-derives
+    // it from p_view . normal, which the forward's FP32 test gives too; a
+    // surfel whose forward ran in FP64 keeps that decision)
+    const R mult = c.flip[idx] > 0 ? (R)1 : (R)-1;
     dL_dtn = mult * dL_dtn;
     const V3<R> dRS0 = v3(dM[0][0], dM[0][1], dM[0][2]);
     const V3<R> dRS1 = v3(dM[1][0], dM[1][1], dM[1][2]);
@@ -1033,6 +1140,7 @@ template <typename R> int get_named(Ctx<R>& c, const std::string& n, void* dst, 
   if (n == "normal_opacity") return copy_out(c.normal_opacity, dst, nbytes);
   if (n == "rgb") return copy_out(c.rgb, dst, nbytes);
   if (n == "clamped") return copy_out(c.clamped, dst, nbytes);
+  if (n == "flip") return copy_out(c.flip, dst, nbytes);
   if (n == "tiles_touched") return copy_out(c.tiles_touched, dst, nbytes);
   if (n == "point_offsets") return copy_out(c.point_offsets, dst, nbytes);
   if (n == "keys") return copy_out(c.keys, dst, nbytes);
@@ -1110,6 +1218,11 @@ void sro_sh_rgb(int deg, int M, const void* sh, const void* means, const void* c
   if (f64) run(0.0); else run(0.0f);
 }
 uint32_t sro_higher_msb(uint32_t n) { return higher_msb(n); }
+
+// on != 0 (the default): near-degenerate surfels take the FP64 preprocess,
+// as the HIP kernel does; 0: every surfel stays FP32, which is the
+// reference's own arithmetic (used to measure what FP64 changes).
+void sro_set_fp64_degenerate(int on) { g_fp64_degenerate = on != 0; }
 
 // Runs preprocess, binning and the forward blend.  Returns 0, 1 when a
 // surfel fails the near test although prefiltered is set (the reference

@@ -810,7 +810,7 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
   float* gT = a.dL_dtransMat + 9 * r;
   const float m0 = a.means3D[3 * idx], m1 = a.means3D[3 * idx + 1], m2 = a.means3D[3 * idx + 2];
   const bool precomp = a.tprec != nullptr;
-  F3 T0, T1, T2, nrm = f3(0.0f, 0.0f, 0.0f), Rc0, Rc1;
+  F3 T0, T1, T2, Rc0, Rc1;
   float qw = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f, scx = 0.0f, scy = 0.0f;
   float P4[3][4];
   if (precomp) {
@@ -830,7 +830,6 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
     const float w = qw, x = qx, y = qy, z = qz;
     Rc0 = f3(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + w * z), 2.0f * (x * z - w * y));
     Rc1 = f3(2.0f * (x * y - w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + w * x));
-    const F3 Rc2 = f3(2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y));
     scx = a.scales[2 * idx];
     scy = a.scales[2 * idx + 1];
     const F3 L0c = (1.0f * scx) * Rc0, L1c = (1.0f * scy) * Rc1;
@@ -849,8 +848,6 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
     T0 = f3(dotM(L0c, 0.0f, 0), dotM(L1c, 0.0f, 0), dotM(pc, 1.0f, 0));
     T1 = f3(dotM(L0c, 0.0f, 1), dotM(L1c, 0.0f, 1), dotM(pc, 1.0f, 1));
     T2 = f3(dotM(L0c, 0.0f, 2), dotM(L1c, 0.0f, 2), dotM(pc, 1.0f, 2));
-    nrm = f3((vm[0] * Rc2.x + vm[4] * Rc2.y) + vm[8] * Rc2.z, (vm[1] * Rc2.x + vm[5] * Rc2.y) + vm[9] * Rc2.z,
-             (vm[2] * Rc2.x + vm[6] * Rc2.y) + vm[10] * Rc2.z);
   }
   F3 dT0 = f3(gs[0], gs[1], gs[2]), dT1 = f3(gs[3], gs[4], gs[5]), dT2 = f3(gs[6], gs[7], gs[8]);
   float* gm2 = a.dL_dmean2D + 3 * r;
@@ -898,11 +895,11 @@ SR_DEV void preprocess_bwd_view(const PreprocessBwdArgs& a, int v, int idx, size
     const float gn[3] = {gs[11], gs[12], gs[13]};
     F3 dL_dtn = f3((vm[0] * gn[0] + vm[1] * gn[1]) + vm[2] * gn[2], (vm[4] * gn[0] + vm[5] * gn[1]) + vm[6] * gn[2],
                    (vm[8] * gn[0] + vm[9] * gn[1]) + vm[10] * gn[2]);
-    const float pvx = ((vm[0] * m0 + vm[4] * m1) + vm[8] * m2) + vm[12];
-    const float pvy = ((vm[1] * m0 + vm[5] * m1) + vm[9] * m2) + vm[13];
-    const float pvz = ((vm[2] * m0 + vm[6] * m1) + vm[10] * m2) + vm[14];
-    const float cosv = -((pvx * nrm.x + pvy * nrm.y) + pvz * nrm.z);
-    const float mult = cosv > 0.0f ? 1.0f : -1.0f;
+    // the camera-facing sign the forward chose (backward.cu:550-555 re-derives
+    // it from p_view . normal, which the forward's FP32 test gives too; for
+    // a surfel whose forward ran in FP64 the forward's decision is the one
+    // its outputs used)
+    const float mult = a.rec[(size_t)kRecQuads * r + 4].w > 0.0f ? 1.0f : -1.0f;
     dL_dtn = mult * dL_dtn;
     const F3 dRS0 = f3(dM[0][0], dM[0][1], dM[0][2]);
     const F3 dRS1 = f3(dM[1][0], dM[1][1], dM[1][2]);

@@ -170,9 +170,143 @@ SR_DEV short4 cull_box(F3 Tu, F3 Tv, F3 Tw, float mx, float my, float opa, int W
   return make_short4((short)x0, (short)y0, (short)x1, (short)y1);
 }
 
+// Ray-splat homography and view-space normal of surfel idx seen from view
+// v, evaluated in precision R (dsr/cuda_rasterizer/forward.cu:75-115,
+// 183-207): T = [L0, 0; L1, 0; p, 1]^T world2ndc ndc2pix, rows Tu, Tv, Tw.
+// With R = float the expressions and their order are the reference mode's.
+template <typename R> struct Geo {
+  R Tu[3], Tv[3], Tw[3], n[3];
+};
+SR_DEV float sqrt_r(float x) { return sqrtf(x); }
+SR_DEV double sqrt_r(double x) { return ::sqrt(x); }
+template <typename R>
+SR_DEV void surfel_geom(const PreprocessArgs& a, int idx, const float* vm, const float* pm, int W, int H,
+                        Geo<R>& g) {
+  const R m0 = (R)a.means3D[3 * idx], m1 = (R)a.means3D[3 * idx + 1], m2 = (R)a.means3D[3 * idx + 2];
+  if (a.tprec == nullptr) {
+    const float* q = a.rots + 4 * idx;
+    const R q0 = (R)q[0], q1 = (R)q[1], q2 = (R)q[2], q3 = (R)q[3];
+    const R n2 = ((q3 * q3 + q0 * q0) + q1 * q1) + q2 * q2;
+    const R s = (R)1 / sqrt_r(n2);
+    const R w = q0 * s, x = q1 * s, y = q2 * s, z = q3 * s;
+    const R c0[3] = {(R)1 - (R)2 * (y * y + z * z), (R)2 * (x * y + w * z), (R)2 * (x * z - w * y)};
+    const R c1[3] = {(R)2 * (x * y - w * z), (R)1 - (R)2 * (x * x + z * z), (R)2 * (y * z + w * x)};
+    const R c2[3] = {(R)2 * (x * z + w * y), (R)2 * (y * z - w * x), (R)1 - (R)2 * (x * x + y * y)};
+    const R sx = (R)(a.mod * a.scales[2 * idx]), sy = (R)(a.mod * a.scales[2 * idx + 1]);
+    const R L0[3] = {sx * c0[0], sx * c0[1], sx * c0[2]};
+    const R L1[3] = {sy * c1[0], sy * c1[1], sy * c1[2]};
+    auto rowdot = [&](const R* v, int cc) {
+      return (v[0] * (R)pm[cc] + v[1] * (R)pm[4 + cc]) + v[2] * (R)pm[8 + cc];
+    };
+    const R a00 = rowdot(L0, 0), a01 = rowdot(L0, 1), a03 = rowdot(L0, 3);
+    const R a10 = rowdot(L1, 0), a11 = rowdot(L1, 1), a13 = rowdot(L1, 3);
+    const R pc[3] = {m0, m1, m2};
+    const R a20 = rowdot(pc, 0) + (R)pm[12], a21 = rowdot(pc, 1) + (R)pm[13], a23 = rowdot(pc, 3) + (R)pm[15];
+    const R hw = (R)W / (R)2, hw1 = (R)(W - 1) / (R)2;
+    const R hh = (R)H / (R)2, hh1 = (R)(H - 1) / (R)2;
+    g.Tu[0] = a00 * hw + a03 * hw1; g.Tu[1] = a10 * hw + a13 * hw1; g.Tu[2] = a20 * hw + a23 * hw1;
+    g.Tv[0] = a01 * hh + a03 * hh1; g.Tv[1] = a11 * hh + a13 * hh1; g.Tv[2] = a21 * hh + a23 * hh1;
+    g.Tw[0] = a03; g.Tw[1] = a13; g.Tw[2] = a23;
+    for (int i = 0; i < 3; ++i)
+      g.n[i] = ((R)vm[i] * c2[0] + (R)vm[4 + i] * c2[1]) + (R)vm[8 + i] * c2[2];
+  } else {
+    const float* t = a.tprec + 9 * idx;
+    for (int i = 0; i < 3; ++i) {
+      g.Tu[i] = (R)t[i];
+      g.Tv[i] = (R)t[3 + i];
+      g.Tw[i] = (R)t[6 + i];
+    }
+    g.n[0] = (R)0; g.n[1] = (R)0; g.n[2] = (R)1;
+  }
+}
+
+// compute_aabb with cutoff 3 (dsr/cuda_rasterizer/forward.cu:119-145) up
+// to the square roots: the denominator d, the centre p and h0 = p^2 - f T T.
+template <typename R> struct Box {
+  R d, px, py, h0x, h0y;
+};
+template <typename R> SR_DEV Box<R> aabb_terms(const Geo<R>& g) {
+  Box<R> b;
+  const R* Tu = g.Tu;
+  const R* Tv = g.Tv;
+  const R* Tw = g.Tw;
+  b.d = ((R)9 * (Tw[0] * Tw[0]) + (R)9 * (Tw[1] * Tw[1])) + (R)-1 * (Tw[2] * Tw[2]);
+  const R inv = (R)1 / b.d;
+  const R fx = inv * (R)9, fy = inv * (R)9, fz = inv * (R)-1;
+  b.px = (fx * (Tu[0] * Tw[0]) + fy * (Tu[1] * Tw[1])) + fz * (Tu[2] * Tw[2]);
+  b.py = (fx * (Tv[0] * Tw[0]) + fy * (Tv[1] * Tw[1])) + fz * (Tv[2] * Tw[2]);
+  b.h0x = b.px * b.px - ((fx * (Tu[0] * Tu[0]) + fy * (Tu[1] * Tu[1])) + fz * (Tu[2] * Tu[2]));
+  b.h0y = b.py * b.py - ((fx * (Tv[0] * Tv[0]) + fy * (Tv[1] * Tv[1])) + fz * (Tv[2] * Tv[2]));
+  return b;
+}
+
+// Radius from h0 as the reference forms it, ceil(max(sqrt(max(1e-4, h0x)),
+// sqrt(max(1e-4, h0y)), 3 * 0.707106)) (forward.cu:141, 230), in FP32.
+SR_DEV int radius_f32(float h0x, float h0y) {
+  const float hx = sqrtf(fmax_(1e-4f, h0x));
+  const float hy = sqrtf(fmax_(1e-4f, h0y));
+  return f2i(ceilf(fmax_(fmax_(hx, hy), 3.0f * kFilterSize)));
+}
+
+// The same radius from FP64 h0, exactly: the least integer r >= 3 with
+// r^2 >= h0 (the square roots and the ceiling decided on exact squares).
+SR_DEV int radius_f64(double h0x, double h0y) {
+  const double h0 = fmax(h0x, h0y);
+  if (!(h0 < 1.0e18)) return h0 == h0 ? 1000000000 : 3;  // beyond every grid; NaN as the FP32 path's f2i
+  int r = (int)ceil(sqrt(fmax(h0, 9.0)));
+  while (r > 3 && (double)(r - 1) * (double)(r - 1) >= h0) --r;
+  while ((double)r * (double)r < h0) ++r;
+  return r;
+}
+
+// Near-degenerate surfels, whose FP32 preprocess can take a different
+// discrete decision than exact arithmetic would (all tests in FP32 on the
+// FP32 values, so the oracle reproduces the selection bit for bit):
+//   grazing: |cos| <= 2^-16 (|vx| + |vy| + |vz|), the camera-facing sign
+//            test (forward.cu:209-214) within the rounding error of the
+//            product and of the unit normal itself, cos == 0 included;
+//   conic:   |d| <= 2^-16 (9 Tw.x^2 + 9 Tw.y^2 + Tw.z^2), the AABB
+//            denominator (forward.cu:125-126) near its d == 0 exit;
+//   tiny:    every tangent entry of Tu, Tv below 2^-7 |Tw.z|, a footprint
+//            under 1/128 pixel, where h0 is all cancellation;
+//   radius:  h0x or h0y within 2^-20 of its magnitude terms of a square
+//            k^2 (k >= 3) where ceil(sqrt(.)) steps (forward.cu:141, 230);
+//   edge:    the tile rectangle (getRect, auxiliary.h:67-77) changes when
+//            the centre moves by 2^-18 (|p| + 16) pixels.
+constexpr float kDegTol = 1.0f / 65536.0f;
+SR_DEV bool radius_ambiguous(float h0, float mag) {
+  const float e = mag * (1.0f / 1048576.0f);
+  const float lo = sqrtf(fmax_(1e-4f, h0 - e)), hi = sqrtf(fmax_(1e-4f, h0 + e));
+  return ceilf(fmax_(lo, 3.0f * kFilterSize)) != ceilf(fmax_(hi, 3.0f * kFilterSize));
+}
+SR_DEV bool rect_ambiguous(float px, float py, int r, int B, int gx, int gy) {
+  const float ex = (fabsf(px) + 16.0f) * (1.0f / 262144.0f), ey = (fabsf(py) + 16.0f) * (1.0f / 262144.0f);
+  int a0, a1, a2, a3, b0, b1, b2, b3;
+  get_rect(px - ex, py - ey, r, B, gx, gy, a0, a1, a2, a3);
+  get_rect(px + ex, py + ey, r, B, gx, gy, b0, b1, b2, b3);
+  return a0 != b0 || a1 != b1 || a2 != b2 || a3 != b3;
+}
+SR_DEV bool degenerate(const Geo<float>& g, float vx, float vy, float vz, float cosv, const Box<float>& b, int B,
+                       int gx, int gy) {
+  const float cabs = (fabsf(vx) + fabsf(vy)) + fabsf(vz);  // |n| = 1; bounds the normal's own error too
+  if (!(fabsf(cosv) > kDegTol * cabs)) return true;
+  const float dabs = (9.0f * (g.Tw[0] * g.Tw[0]) + 9.0f * (g.Tw[1] * g.Tw[1])) + g.Tw[2] * g.Tw[2];
+  if (!(fabsf(b.d) > kDegTol * dabs)) return true;
+  const float tmax = fmax_(fmax_(fabsf(g.Tu[0]), fabsf(g.Tu[1])), fmax_(fabsf(g.Tv[0]), fabsf(g.Tv[1])));
+  if (tmax < (1.0f / 128.0f) * fabsf(g.Tw[2])) return true;
+  const float inv = 1.0f / b.d;
+  const float fx = fabsf(inv * 9.0f), fz = fabsf(inv);
+  const float mx = b.px * b.px + ((fx * (g.Tu[0] * g.Tu[0]) + fx * (g.Tu[1] * g.Tu[1])) + fz * (g.Tu[2] * g.Tu[2]));
+  const float my = b.py * b.py + ((fx * (g.Tv[0] * g.Tv[0]) + fx * (g.Tv[1] * g.Tv[1])) + fz * (g.Tv[2] * g.Tv[2]));
+  if (radius_ambiguous(b.h0x, mx) || radius_ambiguous(b.h0y, my)) return true;
+  return rect_ambiguous(b.px, b.py, radius_f32(b.h0x, b.h0y), B, gx, gy);
+}
+
 // Forward preprocess of surfel idx in view v (record r = v * P + idx),
 // dsr/cuda_rasterizer/forward.cu:75-145, 148-251.  Returns false for culled
-// surfels, else its tile rectangle in the view's tile grid.
+// surfels, else its tile rectangle in the view's tile grid.  Near-degenerate
+// surfels (see degenerate()) take the homography, normal, camera-facing
+// test and AABB in FP64 and store them rounded to FP32.
 SR_DEV bool preprocess_one(const PreprocessArgs& a, int v, int idx, size_t r, int& x0, int& y0, int& x1,
                            int& y1) {
   a.radii[r] = 0;
@@ -189,53 +323,43 @@ SR_DEV bool preprocess_one(const PreprocessArgs& a, int v, int idx, size_t r, in
     if (a.prefiltered) atomicOr(&a.counters[1], kErrPrefiltered);
     return false;
   }
+  Geo<float> g;
+  surfel_geom<float>(a, idx, vm, pm, W, H, g);
+  const float cosv = -((vx * g.n[0] + vy * g.n[1]) + vz * g.n[2]);
+  Box<float> b = aabb_terms(g);
   F3 Tu, Tv, Tw, nrm;
-  if (a.tprec == nullptr) {
-    const float* q = a.rots + 4 * idx;
-    const float q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-    const float n2 = ((q3 * q3 + q0 * q0) + q1 * q1) + q2 * q2;
-    const float s = 1.0f / sqrtf(n2);
-    const float w = q0 * s, x = q1 * s, y = q2 * s, z = q3 * s;
-    const F3 c0 = f3(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + w * z), 2.0f * (x * z - w * y));
-    const F3 c1 = f3(2.0f * (x * y - w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + w * x));
-    const F3 c2 = f3(2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y));
-    const float sx = a.mod * a.scales[2 * idx], sy = a.mod * a.scales[2 * idx + 1];
-    const F3 L0 = sx * c0, L1 = sy * c1, L2 = c2;
-    auto rowdot = [&](const F3& v, int cc) { return (v.x * pm[cc] + v.y * pm[4 + cc]) + v.z * pm[8 + cc]; };
-    const float a00 = rowdot(L0, 0), a01 = rowdot(L0, 1), a03 = rowdot(L0, 3);
-    const float a10 = rowdot(L1, 0), a11 = rowdot(L1, 1), a13 = rowdot(L1, 3);
-    const F3 pc = f3(m0, m1, m2);
-    const float a20 = rowdot(pc, 0) + pm[12], a21 = rowdot(pc, 1) + pm[13], a23 = rowdot(pc, 3) + pm[15];
-    const float hw = (float)W / 2.0f, hw1 = (float)(W - 1) / 2.0f;
-    const float hh = (float)H / 2.0f, hh1 = (float)(H - 1) / 2.0f;
-    Tu = f3(a00 * hw + a03 * hw1, a10 * hw + a13 * hw1, a20 * hw + a23 * hw1);
-    Tv = f3(a01 * hh + a03 * hh1, a11 * hh + a13 * hh1, a21 * hh + a23 * hh1);
-    Tw = f3(a03, a13, a23);
-    nrm = f3((vm[0] * L2.x + vm[4] * L2.y) + vm[8] * L2.z, (vm[1] * L2.x + vm[5] * L2.y) + vm[9] * L2.z,
-             (vm[2] * L2.x + vm[6] * L2.y) + vm[10] * L2.z);
+  float px, py, mult;
+  int ir;
+  if (!degenerate(g, vx, vy, vz, cosv, b, a.vs.B, gx, gy)) {
+    mult = cosv > 0.0f ? 1.0f : -1.0f;
+    Tu = f3(g.Tu[0], g.Tu[1], g.Tu[2]);
+    Tv = f3(g.Tv[0], g.Tv[1], g.Tv[2]);
+    Tw = f3(g.Tw[0], g.Tw[1], g.Tw[2]);
+    nrm = mult * f3(g.n[0], g.n[1], g.n[2]);
+    px = b.px;
+    py = b.py;
+    ir = radius_f32(b.h0x, b.h0y);
   } else {
-    const float* t = a.tprec + 9 * idx;
-    Tu = f3(t[0], t[1], t[2]);
-    Tv = f3(t[3], t[4], t[5]);
-    Tw = f3(t[6], t[7], t[8]);
-    nrm = f3(0.0f, 0.0f, 1.0f);
+    Geo<double> G;
+    surfel_geom<double>(a, idx, vm, pm, W, H, G);
+    const double dvx = (((double)vm[0] * m0 + (double)vm[4] * m1) + (double)vm[8] * m2) + (double)vm[12];
+    const double dvy = (((double)vm[1] * m0 + (double)vm[5] * m1) + (double)vm[9] * m2) + (double)vm[13];
+    const double dvz = (((double)vm[2] * m0 + (double)vm[6] * m1) + (double)vm[10] * m2) + (double)vm[14];
+    const double dcos = -((dvx * G.n[0] + dvy * G.n[1]) + dvz * G.n[2]);
+    if (dcos == 0.0) return false;
+    const Box<double> B = aabb_terms(G);
+    if (B.d == 0.0) return false;
+    mult = dcos > 0.0 ? 1.0f : -1.0f;
+    Tu = f3((float)G.Tu[0], (float)G.Tu[1], (float)G.Tu[2]);
+    Tv = f3((float)G.Tv[0], (float)G.Tv[1], (float)G.Tv[2]);
+    Tw = f3((float)G.Tw[0], (float)G.Tw[1], (float)G.Tw[2]);
+    nrm = mult * f3((float)G.n[0], (float)G.n[1], (float)G.n[2]);
+    px = (float)B.px;
+    py = (float)B.py;
+    ir = radius_f64(B.h0x, B.h0y);
+    mult *= 2.0f;  // marks the record as evaluated in FP64
   }
-  const float cosv = -((vx * nrm.x + vy * nrm.y) + vz * nrm.z);
-  if (cosv == 0.0f) return false;
-  const float mult = cosv > 0.0f ? 1.0f : -1.0f;
-  nrm = mult * nrm;
-  const float d = (9.0f * (Tw.x * Tw.x) + 9.0f * (Tw.y * Tw.y)) + -1.0f * (Tw.z * Tw.z);
-  if (d == 0.0f) return false;
-  const float inv = 1.0f / d;
-  const float fx = inv * 9.0f, fy = inv * 9.0f, fz = inv * -1.0f;
-  const float px = (fx * (Tu.x * Tw.x) + fy * (Tu.y * Tw.y)) + fz * (Tu.z * Tw.z);
-  const float py = (fx * (Tv.x * Tw.x) + fy * (Tv.y * Tw.y)) + fz * (Tv.z * Tw.z);
-  const float h0x = px * px - ((fx * (Tu.x * Tu.x) + fy * (Tu.y * Tu.y)) + fz * (Tu.z * Tu.z));
-  const float h0y = py * py - ((fx * (Tv.x * Tv.x) + fy * (Tv.y * Tv.y)) + fz * (Tv.z * Tv.z));
-  const float hx = sqrtf(fmax_(1e-4f, h0x));
-  const float hy = sqrtf(fmax_(1e-4f, h0y));
-  const float radius = ceilf(fmax_(fmax_(hx, hy), 3.0f * kFilterSize));
-  const int ir = f2i(radius);
+  // the rectangle from the stored (FP32) centre, as the scatter recomputes it
   get_rect(px, py, ir, a.vs.B, gx, gy, x0, y0, x1, y1);
   const uint32_t area = (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
   if (area == 0) return false;
@@ -265,7 +389,7 @@ SR_DEV bool preprocess_one(const PreprocessArgs& a, int v, int idx, size_t r, in
   rc[1] = make_float4(Tv.y, Tv.z, Tw.x, Tw.y);
   rc[2] = make_float4(Tw.z, px, py, opa);
   rc[3] = make_float4(nrm.x, nrm.y, nrm.z, rgb.x);
-  rc[4] = make_float4(rgb.y, rgb.z, rho_rej, 0.0f);
+  rc[4] = make_float4(rgb.y, rgb.z, rho_rej, mult);
   a.cullbox[r] = box;
   a.tiles[r] = area;
   return true;

@@ -13,13 +13,16 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 // Packed per-surfel record read by both blend kernels (5 x float4 = 80 B):
 //   q0 = (Tu.x, Tu.y, Tu.z, Tv.x)   q1 = (Tv.y, Tv.z, Tw.x, Tw.y)
 //   q2 = (Tw.z, mean2D.x, mean2D.y, opacity)
-//   q3 = (normal.x, normal.y, normal.z, r)   q4 = (g, b, rho_reject, 0)
+//   q3 = (normal.x, normal.y, normal.z, r)   q4 = (g, b, rho_reject, flip)
 // Tu/Tv/Tw are the columns of the ray-splat homography T as stored by the
 // reference (dsr/cuda_rasterizer/forward.cu:195-198); normal is the
 // camera-facing view-space normal (forward.cu:209-214).  rho_reject bounds
 // the Gaussian exponent: any pixel with min(rho3d, rho2d) > rho_reject has
 // alpha < 1/255 and is skipped by the reference, so the blend kernels may
-// skip it early (rho_reject < 0: the surfel never reaches 1/255).
+// skip it early (rho_reject < 0: the surfel never reaches 1/255).  flip is
+// the sign the forward gave the normal to face the camera, times 2 when the
+// surfel's preprocess ran in FP64 (near-degenerate); the backward takes the
+// sign from it rather than re-deciding it.
 constexpr int kRecQuads = 5;
 
 // The blend kernels run one wave64 per 8 x 8 pixel quadrant of a tile; each

@@ -57,6 +57,7 @@ def read_state(res, scene):
         st["means2D"] = rec[:, 9:11]
         st["normal_opacity"] = np.concatenate([rec[:, 12:15], rec[:, 11:12]], axis=1)
         st["rgb"] = np.concatenate([rec[:, 15:16], rec[:, 16:18]], axis=1)
+        st["flip"] = rec[:, 19].astype(np.int8)  # camera-facing sign, x2 for the FP64 preprocess
         st["depths"] = view(geom, "depths", np.float32, P)
         st["tiles_touched"] = view(geom, "tiles_touched", np.uint32, P)
         st["clamped"] = view(geom, "clamped", np.uint8, 4 * P).reshape(P, 4)[:, :3]
