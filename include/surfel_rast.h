@@ -259,10 +259,15 @@ typedef struct {
 int sb_generate_fwd(const sb_blocks* blk, float* means3D, float* scales, float* rotations, float* opacities,
                     sr_stream_t stream);
 /* Adjoint of sb_generate_fwd: dL_dparams B x 12 and dL_doccupancy B are
- * overwritten with the gradients of the block parameters. */
+ * overwritten with the gradients of the block parameters, summed in a fixed
+ * order (the same bits on every run).  workspace (device) holds at least
+ * sb_bwd_workspace_bytes(blk) bytes: 13 floats per surfel.  Replaces the
+ * autograd backward of the reference's torch ops in prepare_scaling_rot
+ * (block_gaussian_model.py:198-256). */
+size_t sb_bwd_workspace_bytes(const sb_blocks* blk);
 int sb_generate_bwd(const sb_blocks* blk, const float* dL_dmeans3D, const float* dL_dscales,
                     const float* dL_drotations, const float* dL_dopacities, float* dL_dparams,
-                    float* dL_doccupancy, sr_stream_t stream);
+                    float* dL_doccupancy, void* workspace, size_t workspace_bytes, sr_stream_t stream);
 
 const char* sr_last_error(void);
 const char* sr_version(void);

@@ -53,6 +53,7 @@ EXPORTED = (
     "sr_version",
     "sb_generate_fwd",
     "sb_generate_bwd",
+    "sb_bwd_workspace_bytes",
     "sr_batch_workspace_bytes",
     "sr_forward_batched",
     "sr_backward_batched",
@@ -180,7 +181,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_last_error.restype = C.c_char_p
         lib.sb_generate_fwd.argtypes = [vp, vp, vp, vp, vp, vp]
         lib.sb_generate_fwd.restype = i
-        lib.sb_generate_bwd.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        lib.sb_generate_bwd.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]
+        lib.sb_bwd_workspace_bytes.argtypes = [vp]
+        lib.sb_bwd_workspace_bytes.restype = sz
         lib.sb_generate_bwd.restype = i
         lib.sr_batch_workspace_bytes.argtypes = [C.POINTER(SrViews), i, i]
         lib.sr_batch_workspace_bytes.restype = sz

@@ -126,10 +126,11 @@ class _GenerateSurfels(torch.autograd.Function):
         d_params = torch.empty_like(params)
         d_occ = torch.empty_like(occ)
         d = _desc(grid, params, occ)
+        ws = torch.empty(int(lib.sb_bwd_workspace_bytes(C.byref(d))), dtype=torch.uint8, device=dev)
         with torch.cuda.device(dev):
             _lib.check(lib.sb_generate_bwd(C.byref(d), g_means.data_ptr(), g_scales.data_ptr(), g_rots.data_ptr(),
                                            g_opac.data_ptr(), d_params.data_ptr(), d_occ.data_ptr(),
-                                           _lib.stream_handle(dev)))
+                                           ws.data_ptr(), ws.numel(), _lib.stream_handle(dev)))
         return d_params, d_occ, None
 
 

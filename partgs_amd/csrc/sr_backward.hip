@@ -381,21 +381,35 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
       nx = chunk_load(masks, frag_off, a.rec, lane, c - 1, ve);
     }
     if (__ballot(mask != 0ull) != 0ull) {
-      while (mask != 0ull) {
-        const int j = 63 - __builtin_clzll(mask);
-        mask &= ~(1ull << j);
+      // entries from last to first, the wave in step (each lane still walks
+      // its own pixel's entries back to front); an entry's 18 partial
+      // gradients are summed over the pixels by a butterfly, in a fixed
+      // order, so the results do not depend on scheduling
+      for (int j = 63; j >= 0; --j) {
+        const bool hit = (mask >> j) & 1ull;
+        if (__ballot(hit) == 0ull) continue;
+        float gk[kNumG];
+#pragma unroll
+        for (int k = 0; k < kNumG; ++k) gk[k] = 0.0f;
         const uint32_t contributor = s_e[j].y;  // 0-based position in the tile list
-        if (contributor >= px.last_contributor) continue;
-        const float4 q2 = s_q2[j];
-        Pair e;  // a flagged quadrant has no stored pairs: recompute
-        if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], q2, 0.0f, e)) continue;
-        const float c_d = e.depth;
-        const float m_d = mcoef * (1.0f - kNear / c_d);
-        const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
-        const float4 q3 = s_q3[j], q4 = s_q4[j];
-        const float4 g = ch.step(px, e.alpha, c_d, m_d, dmd_dd, contributor, q3, q4.x, q4.y);
-        pair_grads(px.dLp0, px.dLp1, px.dLp2, px.dLn0, px.dLn1, px.dLn2, e, s_q1[j], q2.w, g.x, g.y, g.z, pixx,
-                   pixy, [&](int k, float v) { atomicAdd(&s_acc[k][j], v); });
+        if (hit && contributor < px.last_contributor) {
+          const float4 q2 = s_q2[j];
+          Pair e;  // a flagged quadrant has no stored pairs: recompute
+          if (eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], q2, 0.0f, e)) {
+            const float c_d = e.depth;
+            const float m_d = mcoef * (1.0f - kNear / c_d);
+            const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
+            const float4 q3 = s_q3[j], q4 = s_q4[j];
+            const float4 g = ch.step(px, e.alpha, c_d, m_d, dmd_dd, contributor, q3, q4.x, q4.y);
+            pair_grads(px.dLp0, px.dLp1, px.dLp2, px.dLn0, px.dLn1, px.dLn2, e, s_q1[j], q2.w, g.x, g.y, g.z,
+                       pixx, pixy, [&](int k, float v) { gk[k] += v; });
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kNumG; ++k) {
+          const float t = wave_sum(gk[k]);
+          if (lane == 0) s_acc[k][j] = t;
+        }
       }
     }
     __syncthreads();

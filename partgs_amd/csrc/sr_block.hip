@@ -16,8 +16,9 @@
 // The backward evaluates the same code on dual numbers, once per block
 // parameter (3 log-scales, 4 quaternion, 3 translation, 2 epsilon logits), so
 // the 12 directional derivatives of every output are exact; they are
-// contracted with the upstream gradients and summed per block (wave
-// reduction, then one atomic per block parameter and wave).  Surfels whose
+// contracted with the upstream gradients; the per-block sums are formed in
+// a fixed order (per-surfel contributions, then one workgroup per block
+// and parameter), so the gradients are deterministic.  Surfels whose
 // triangle is nearly degenerate (|n| < 1e-3 |e1| |e2|, or |n| or |V1 - c|
 // below 1e-4, where the 1e-8 guards of the normalisations matter: small
 // faces at the poles) are generated and differentiated in FP64.
@@ -312,52 +313,66 @@ SR_DEV T directional(const Args& a, int b, int i, int j, int k, int pi, const fl
   return acc;
 }
 
+// Pass 1: every surfel's 13 contributions (the 12 directional derivatives
+// contracted with its upstream gradients, and the occupancy term) go to
+// contrib[pi * P + sidx], parameter-major so each pass stores contiguously.
 __global__ __launch_bounds__(256) void k_block_bwd(Args a, const float* dmeans, const float* dscales,
-                                                   const float* drots, const float* dopac, float* dprm,
-                                                   float* docc) {
+                                                   const float* drots, const float* dopac, float* contrib) {
   const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
   const int P = a.B * a.n_eta * a.n_omega * a.K;
-  const bool live = sidx < P;
+  if (sidx >= P) return;
   int b = 0, i = 0, j = 0, k = 0;
-  float g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  bool degenerate = false;
-  if (live) {
-    face_of(a, sidx, b, i, j, k);
-    for (int t = 0; t < 3; ++t) g[t] = dmeans[3 * sidx + t];
-    for (int t = 0; t < 2; ++t) g[3 + t] = dscales[2 * sidx + t];
-    for (int t = 0; t < 4; ++t) g[5 + t] = drots[4 * sidx + t];
-    SurfelOut<float> o;
-    degenerate = gen_plain<float>(a, b, i, j, k, o);
+  float g[9];
+  face_of(a, sidx, b, i, j, k);
+  for (int t = 0; t < 3; ++t) g[t] = dmeans[3 * sidx + t];
+  for (int t = 0; t < 2; ++t) g[3 + t] = dscales[2 * sidx + t];
+  for (int t = 0; t < 4; ++t) g[5 + t] = drots[4 * sidx + t];
+  SurfelOut<float> o;
+  const bool degenerate = gen_plain<float>(a, b, i, j, k, o);
+  for (int pi = 0; pi < 12; ++pi)
+    contrib[(size_t)pi * P + sidx] =
+        degenerate ? (float)directional<double>(a, b, i, j, k, pi, g) : directional<float>(a, b, i, j, k, pi, g);
+  const float sg = 1.0f / (1.0f + expf(-a.occ[b]));
+  contrib[(size_t)12 * P + sidx] = dopac[sidx] * sg * (1.0f - sg);
+}
+
+// Pass 2: one workgroup per (block, parameter) sums the block's contiguous
+// run of contributions: each lane a strided partial sum in a fixed order,
+// then a fixed LDS tree.  No atomics, so the gradients are the same bits on
+// every run.
+__global__ __launch_bounds__(256) void k_block_reduce(int P, int per_block, const float* contrib, float* dprm,
+                                                      float* docc) {
+  __shared__ float s_sum[256];
+  const int b = blockIdx.x, pi = blockIdx.y;
+  const float* src = contrib + (size_t)pi * P + (size_t)b * per_block;
+  float acc = 0.0f;
+  for (int t = threadIdx.x; t < per_block; t += 256) acc += src[t];
+  s_sum[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+    __syncthreads();
   }
-  // a wave covers surfels of at most two blocks (4096-surfel blocks are the
-  // common case); reduce per block id with a masked wave sum
-  const int lane = threadIdx.x & 63;
-  const int b_first = __shfl(b, 0, 64);
-  for (int pi = 0; pi <= 12; ++pi) {
-    float v = 0.0f;
-    if (live) {
-      if (pi < 12) {
-        v = degenerate ? (float)directional<double>(a, b, i, j, k, pi, g) : directional<float>(a, b, i, j, k, pi, g);
-      } else {
-        const float s = 1.0f / (1.0f + expf(-a.occ[b]));
-        v = dopac[sidx] * s * (1.0f - s);
-      }
-    }
-    // lanes of block b_first, then the rest one block at a time
-    bool pending = live;
-    int cur = b_first;
-    while (__ballot(pending) != 0ull) {
-      const uint64_t mk = __ballot(pending);
-      cur = __shfl(b, __ffsll((unsigned long long)mk) - 1, 64);
-      const bool mine = pending && b == cur;
-      const float s = sr::wave_sum(mine ? v : 0.0f);
-      if (lane == 0) {
-        if (pi < 12) unsafeAtomicAdd(&dprm[12 * cur + pi], s);
-        else unsafeAtomicAdd(&docc[cur], s);
-      }
-      if (mine) pending = false;
-    }
+  if (threadIdx.x == 0) {
+    if (pi < 12) dprm[12 * b + pi] = s_sum[0];
+    else docc[b] = s_sum[0];
   }
+}
+
+// The adjoint on a contribution buffer of 13 P floats.
+int run_bwd(const Args& a, long long P, const float* dmeans, const float* dscales, const float* drots,
+            const float* dopac, float* dprm, float* docc, float* contrib, hipStream_t st) {
+  if (P == 0) {
+    if (sr::launch_fill_f32(dprm, 0.0f, 12 * (size_t)a.B, st) != hipSuccess) return SR_ERR_HIP;
+    if (sr::launch_fill_f32(docc, 0.0f, (size_t)a.B, st) != hipSuccess) return SR_ERR_HIP;
+    return SR_OK;
+  }
+  hipLaunchKernelGGL(k_block_bwd, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, a, dmeans, dscales, drots,
+                     dopac, contrib);
+  if (hipGetLastError() != hipSuccess) return SR_ERR_HIP;
+  hipLaunchKernelGGL(k_block_reduce, dim3((unsigned)a.B, 13), dim3(256), 0, st, (int)P,
+                     a.n_eta * a.n_omega * a.K, (const float*)contrib, dprm, docc);
+  return hipGetLastError() == hipSuccess ? SR_OK : SR_ERR_HIP;
 }
 
 }  // namespace sb
@@ -380,9 +395,15 @@ int sb_generate_fwd(const sb_blocks* blk, float* means3D, float* scales, float* 
   return hipGetLastError() == hipSuccess ? SR_OK : SR_ERR_HIP;
 }
 
+size_t sb_bwd_workspace_bytes(const sb_blocks* blk) {
+  if (!blk || blk->B < 0 || blk->n_eta <= 0 || blk->n_omega <= 0 || blk->K <= 0) return 0;
+  const long long P = (long long)blk->B * blk->n_eta * blk->n_omega * blk->K;
+  return (size_t)(P > 0 ? P : 1) * 13 * sizeof(float);
+}
+
 int sb_generate_bwd(const sb_blocks* blk, const float* dL_dmeans3D, const float* dL_dscales,
                     const float* dL_drotations, const float* dL_dopacities, float* dL_dparams,
-                    float* dL_doccupancy, sr_stream_t stream) {
+                    float* dL_doccupancy, void* workspace, size_t workspace_bytes, sr_stream_t stream) {
   if (!blk || !dL_dmeans3D || !dL_dscales || !dL_drotations || !dL_dopacities || !dL_dparams || !dL_doccupancy)
     return SR_ERR_INVALID;
   if (blk->B < 0 || blk->n_eta <= 0 || blk->n_omega <= 0 || blk->K <= 0 || !blk->eta || !blk->omega ||
@@ -391,14 +412,10 @@ int sb_generate_bwd(const sb_blocks* blk, const float* dL_dmeans3D, const float*
   sb::Args a{blk->B, blk->n_eta, blk->n_omega, blk->K, blk->eta, blk->omega, blk->bary, blk->ratio,
              blk->scale_min, blk->surfel_scale, blk->params, blk->occupancy};
   const long long P = (long long)blk->B * blk->n_eta * blk->n_omega * blk->K;
-  if (P > 0x7fffffffLL) return SR_ERR_INVALID;
-  hipStream_t st = (hipStream_t)stream;
-  if (sr::launch_fill_f32(dL_dparams, 0.0f, 12 * (size_t)blk->B, st) != hipSuccess) return SR_ERR_HIP;
-  if (sr::launch_fill_f32(dL_doccupancy, 0.0f, (size_t)blk->B, st) != hipSuccess) return SR_ERR_HIP;
-  if (P == 0) return SR_OK;
-  hipLaunchKernelGGL(sb::k_block_bwd, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, a, dL_dmeans3D,
-                     dL_dscales, dL_drotations, dL_dopacities, dL_dparams, dL_doccupancy);
-  return hipGetLastError() == hipSuccess ? SR_OK : SR_ERR_HIP;
+  if (P > 0x7fffffffLL / 13) return SR_ERR_INVALID;
+  if (!workspace || workspace_bytes < sb_bwd_workspace_bytes(blk)) return SR_ERR_INVALID;
+  return sb::run_bwd(a, P, dL_dmeans3D, dL_dscales, dL_drotations, dL_dopacities, dL_dparams, dL_doccupancy,
+                     (float*)workspace, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -147,3 +147,23 @@ def test_blocks_through_rasteriser_autograd():
     for a, b_, name in ((p.grad, p2.grad, "params"), (o.grad, o2.grad, "occupancy")):
         a, b_ = a.cpu().numpy(), b_.cpu().numpy()
         assert np.abs(a - b_).max() <= 1e-4 * np.abs(b_).max() + 1e-7, name
+
+
+def test_block_backward_is_deterministic():
+    """The block gradients are summed in a fixed order (per-surfel
+    contributions, then one workgroup per block and parameter): repeated
+    backward passes give the same bits."""
+    params, occ = _blocks(4, 16)
+    grid, p, o, outs = _run_gpu(params, occ, 64, 64)
+    print("[blocks] determinism B=16 64x64", flush=True)
+    rng = np.random.default_rng(11)
+    gs = [torch.tensor(rng.normal(size=t.shape), dtype=torch.float32, device="cuda") for t in outs]
+    res = []
+    for _ in range(3):
+        p.grad = None
+        o.grad = None
+        outs = generate_surfels(p, o, grid)
+        torch.autograd.backward(list(outs), gs)
+        res.append((p.grad.clone(), o.grad.clone()))
+    for a, b in res[1:]:
+        assert torch.equal(a, res[0][0]) and torch.equal(b, res[0][1])

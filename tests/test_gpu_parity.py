@@ -77,18 +77,20 @@ def test_forward_backward_cloud_c1():
     _check_backward(scene, res, o)
 
 
-def test_backward_is_deterministic():
+@pytest.mark.parametrize("pair_store", [0, 1])
+def test_backward_is_deterministic(pair_store):
     """Every gradient sum is formed in a fixed order without atomics: two
     backward passes over the same forward state give the same bits, and a
-    second forward + backward does too."""
+    second forward + backward does too.  pair_store = 1 sends most quadrants
+    through the path that recomputes their pairs."""
     scene = superquadric_surfels(3, P=20000, width=160, height=120)
     H, W = scene.camera.image_height, scene.camera.image_width
     gc, go = upstream_grads(3, H, W)
-    res = gpu_forward(scene)
-    print(f"[parity] determinism L={res['L']}", flush=True)
+    res = gpu_forward(scene, pair_store=pair_store)
+    print(f"[parity] determinism L={res['L']} pair_store={pair_store}", flush=True)
     g1 = gpu_backward(res, gc, go)
     g2 = gpu_backward(res, gc, go)
-    g3 = gpu_backward(gpu_forward(scene), gc, go)
+    g3 = gpu_backward(gpu_forward(scene, pair_store=pair_store), gc, go)
     for k in g1:
         np.testing.assert_array_equal(g1[k], g2[k], err_msg=k)
         np.testing.assert_array_equal(g1[k], g3[k], err_msg=k)
