@@ -234,6 +234,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
   __shared__ uint2 s_e[64];
   __shared__ float s_acc[kNumG][64];
+  __shared__ float s_tmp[kNumG][65];
   __shared__ float s_state[8][7][18];  // chain state where each later segment of a run starts
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   if (a.counters[1] & kErrCapacity) return;
@@ -383,14 +384,17 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
     if (__ballot(mask != 0ull) != 0ull) {
       // entries from last to first, the wave in step (each lane still walks
       // its own pixel's entries back to front); an entry's 18 partial
-      // gradients are summed over the pixels by a butterfly, in a fixed
-      // order, so the results do not depend on scheduling
+      // gradients are then summed over the pixels that blended it in pixel
+      // order from 0, exactly as the per-pair gradient pass sums a stored
+      // quadrant's, so the partials have the same bits whichever quadrants
+      // missed the pair store
       for (int j = 63; j >= 0; --j) {
         const bool hit = (mask >> j) & 1ull;
         if (__ballot(hit) == 0ull) continue;
         float gk[kNumG];
 #pragma unroll
         for (int k = 0; k < kNumG; ++k) gk[k] = 0.0f;
+        bool took = false;
         const uint32_t contributor = s_e[j].y;  // 0-based position in the tile list
         if (hit && contributor < px.last_contributor) {
           const float4 q2 = s_q2[j];
@@ -403,13 +407,19 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
             const float4 g = ch.step(px, e.alpha, c_d, m_d, dmd_dd, contributor, q3, q4.x, q4.y);
             pair_grads(px.dLp0, px.dLp1, px.dLp2, px.dLn0, px.dLn1, px.dLn2, e, s_q1[j], q2.w, g.x, g.y, g.z,
                        pixx, pixy, [&](int k, float v) { gk[k] += v; });
+            took = true;
           }
         }
+        const uint64_t tookm = __ballot(took);
 #pragma unroll
-        for (int k = 0; k < kNumG; ++k) {
-          const float t = wave_sum(gk[k]);
-          if (lane == 0) s_acc[k][j] = t;
+        for (int k = 0; k < kNumG; ++k) s_tmp[k][lane] = gk[k];
+        __syncthreads();
+        if (lane < kNumG) {
+          float acc = 0.0f;
+          for (uint64_t mm = tookm; mm != 0ull; mm &= mm - 1ull) acc += s_tmp[lane][__builtin_ctzll(mm)];
+          s_acc[lane][j] = acc;
         }
+        __syncthreads();
       }
     }
     __syncthreads();

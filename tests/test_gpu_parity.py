@@ -96,6 +96,24 @@ def test_backward_is_deterministic(pair_store):
         np.testing.assert_array_equal(g1[k], g3[k], err_msg=k)
 
 
+def test_recompute_path_gives_stored_path_bits():
+    """A quadrant whose pairs missed the accepted-pair store runs the backward
+    by recomputing them; it sums each entry's partial gradients over the
+    pixels in the same order as the stored path, so the gradients do not
+    depend on which quadrants fit (which the store's first-come reservation
+    decides)."""
+    scene = superquadric_surfels(3, P=20000, width=160, height=120)
+    H, W = scene.camera.image_height, scene.camera.image_width
+    gc, go = upstream_grads(3, H, W)
+    full = gpu_backward(gpu_forward(scene), gc, go)
+    print("[parity] stored vs recomputed pairs", flush=True)
+    for cap in (1, 20000, 200000):
+        res = gpu_forward(scene, capacity=40000, pair_capacity=cap)
+        g = gpu_backward(res, gc, go)
+        for k in g:
+            np.testing.assert_array_equal(g[k], full[k], err_msg=f"{k} pair_capacity={cap}")
+
+
 @pytest.mark.parametrize("tile", [8, 32])
 def test_tile_sizes(tile):
     scene = surfel_cloud(1, P=4096, width=200, height=150)
@@ -290,7 +308,7 @@ def test_autograd_api_matches_low_level():
     # later forwards reuse the binning capacity of earlier ones (no
     # synchronisation mid-forward), growing it when outgrown: same results
     from partgs_amd import rasterizer as R
-    key = (means3D.device, 16)
+    key = (means3D.device, 16, cam.image_width, cam.image_height)
     for hint in (None, 1):
         if hint is not None:
             R._capacity_hint[key] = hint  # far too small: the forward retries
