@@ -269,6 +269,19 @@ int sb_generate_bwd(const sb_blocks* blk, const float* dL_dmeans3D, const float*
                     const float* dL_drotations, const float* dL_dopacities, float* dL_dparams,
                     float* dL_doccupancy, void* workspace, size_t workspace_bytes, sr_stream_t stream);
 
+/* sb_generate_bwd without a workspace and without any allocation: for
+ * callers that cannot allocate (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The 13-float-per-surfel contributions go to a fixed scratch
+ * array of 13 x 262144 floats (13.6 MB) that the library holds on every GPU
+ * it runs on until the process ends; more than 262144 surfels (B x n_eta x
+ * n_omega x K) return SR_ERR_INVALID.  Results equal sb_generate_bwd's bit
+ * for bit.  Calls share the GPU's scratch: calls that overlap on one GPU
+ * (other streams, other threads) race and must be ordered by the caller. */
+int sb_generate_bwd_noalloc(const sb_blocks* blk, const float* dL_dmeans3D, const float* dL_dscales,
+                            const float* dL_drotations, const float* dL_dopacities, float* dL_dparams,
+                            float* dL_doccupancy, sr_stream_t stream);
+
 const char* sr_last_error(void);
 const char* sr_version(void);
 

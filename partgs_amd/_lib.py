@@ -54,6 +54,7 @@ EXPORTED = (
     "sb_generate_fwd",
     "sb_generate_bwd",
     "sb_bwd_workspace_bytes",
+    "sb_generate_bwd_noalloc",
     "sr_batch_workspace_bytes",
     "sr_forward_batched",
     "sr_backward_batched",
@@ -182,6 +183,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sb_generate_fwd.argtypes = [vp, vp, vp, vp, vp, vp]
         lib.sb_generate_fwd.restype = i
         lib.sb_generate_bwd.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]
+        lib.sb_generate_bwd_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        lib.sb_generate_bwd_noalloc.restype = i
         lib.sb_bwd_workspace_bytes.argtypes = [vp]
         lib.sb_bwd_workspace_bytes.restype = sz
         lib.sb_generate_bwd.restype = i

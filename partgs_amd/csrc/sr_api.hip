@@ -30,6 +30,16 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+}  // namespace
+
+// Sets the calling thread's error message (for entry points in other files).
+int sr::set_error(int code, const char* msg) {
+  g_err = msg ? msg : "";
+  return code;
+}
+
+namespace {
+
 #define SR_HIP(expr)                                                                  \
   do {                                                                                \
     hipError_t e_ = (expr);                                                           \

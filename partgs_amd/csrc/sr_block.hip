@@ -24,6 +24,8 @@
 // faces at the poles) are generated and differentiated in FP64.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "../../include/surfel_rast.h"
 #include "sr_device.h"
 #include "sr_internal.h"
@@ -375,6 +377,38 @@ int run_bwd(const Args& a, long long P, const float* dmeans, const float* dscale
   return hipGetLastError() == hipSuccess ? SR_OK : SR_ERR_HIP;
 }
 
+// Scratch of sb_generate_bwd_noalloc: the 13 contributions of up to
+// kNoAllocSurfels surfels (13.6 MB), one copy per GPU the library runs on,
+// held until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: kernels get the address hipGetSymbolAddress
+// returns for the current device, cached per device.
+constexpr long long kNoAllocSurfels = 262144;
+__device__ __align__(256) float scratch[13 * kNoAllocSurfels];
+
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[64] = {};
+
+// The device address of `scratch` on the current device (looked up once per
+// device), or nullptr with the error set.
+float* scratch_on_current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    sr::set_error(SR_ERR_HIP, "sb_generate_bwd_noalloc: no current HIP device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      sr::set_error(SR_ERR_HIP, "sb_generate_bwd_noalloc: hipGetSymbolAddress(scratch) failed");
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = (float*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace sb
 
 extern "C" {
@@ -416,6 +450,29 @@ int sb_generate_bwd(const sb_blocks* blk, const float* dL_dmeans3D, const float*
   if (!workspace || workspace_bytes < sb_bwd_workspace_bytes(blk)) return SR_ERR_INVALID;
   return sb::run_bwd(a, P, dL_dmeans3D, dL_dscales, dL_drotations, dL_dopacities, dL_dparams, dL_doccupancy,
                      (float*)workspace, (hipStream_t)stream);
+}
+
+int sb_generate_bwd_noalloc(const sb_blocks* blk, const float* dL_dmeans3D, const float* dL_dscales,
+                            const float* dL_drotations, const float* dL_dopacities, float* dL_dparams,
+                            float* dL_doccupancy, sr_stream_t stream) {
+  if (!blk || !dL_dmeans3D || !dL_dscales || !dL_drotations || !dL_dopacities || !dL_dparams || !dL_doccupancy)
+    return sr::set_error(SR_ERR_INVALID, "sb_generate_bwd_noalloc: NULL argument");
+  if (blk->B < 0 || blk->n_eta <= 0 || blk->n_omega <= 0 || blk->K <= 0 || !blk->eta || !blk->omega ||
+      !blk->params || !blk->occupancy || (!blk->bary && blk->K != 1))
+    return sr::set_error(SR_ERR_INVALID, "sb_generate_bwd_noalloc: invalid block description");
+  const long long P = (long long)blk->B * blk->n_eta * blk->n_omega * blk->K;
+  if (P > sb::kNoAllocSurfels)
+    return sr::set_error(SR_ERR_INVALID,
+                         "sb_generate_bwd_noalloc: more than 262144 surfels (the fixed scratch's limit); "
+                         "use sb_generate_bwd with a workspace");
+  float* contrib = sb::scratch_on_current_device();
+  if (!contrib) return SR_ERR_HIP;
+  sb::Args a{blk->B, blk->n_eta, blk->n_omega, blk->K, blk->eta, blk->omega, blk->bary, blk->ratio,
+             blk->scale_min, blk->surfel_scale, blk->params, blk->occupancy};
+  // every contribution the reduction reads is written first by the same
+  // call, so the scratch needs no clearing
+  return sb::run_bwd(a, P, dL_dmeans3D, dL_dscales, dL_drotations, dL_dopacities, dL_dparams, dL_doccupancy,
+                     contrib, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -411,6 +411,8 @@ struct PreprocessBwdArgs {
 // Launchers (return hipGetLastError()).
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
+// Sets the message sr_last_error() returns on the calling thread; returns code.
+int set_error(int code, const char* msg);
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s);
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
                              uint32_t* ranges, uint32_t* counters, uint32_t capacity, int nb, uint32_t* bsum,
