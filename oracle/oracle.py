@@ -67,6 +67,10 @@ def load():
         lib.sro_replay.argtypes = [C.c_void_p, C.POINTER(SroScene)]
         lib.sro_replay.restype = C.c_int
         lib.sro_set_fp64_degenerate.argtypes = [C.c_int]
+        lib.sro_forward_part.argtypes = [C.c_void_p, C.POINTER(SroScene), C.c_int, C.c_int, C.c_void_p]
+        lib.sro_forward_part.restype = C.c_int
+        lib.sro_backward_part.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.sro_backward_part.restype = C.c_int
         lib.sro_alpha_capped.argtypes = [C.c_void_p]
         lib.sro_alpha_capped.restype = C.c_int
         _lib = lib
@@ -128,7 +132,9 @@ _SHAPES = {
     "point_list": ("u4", lambda d: (d["L"],)),
     "ranges": ("u4", lambda d: (d["T"], 2)),
     "out_color": ("R", lambda d: (3, d["H"], d["W"])),
-    "out_others": ("R", lambda d: (7, d["H"], d["W"])),
+    "out_others": ("R", lambda d: (d.get("O", 7), d["H"], d["W"])),
+    "out_semantic": ("R", lambda d: (d.get("S", 0), d["H"], d["W"])),
+    "dL_dsemantics": ("R", lambda d: (d["P"], d.get("S", 0))),
     "final_T": ("R", lambda d: (3, d["H"], d["W"])),
     "n_contrib": ("u4", lambda d: (2, d["H"], d["W"])),
     "acc_dL_dtransMat": ("f8", lambda d: (d["P"], 9)),
@@ -190,6 +196,28 @@ class Oracle:
         self.f64 = f64
         self.dims = dict(P=s.P, M=s.M, W=int(W), H=int(H), T=gx * gy, L=self.lib.sro_num_rendered(self.h))
         return rc
+
+    def forward_part(self, *, semantics, f64=False, **kw) -> int:
+        """The _part variant (diff-surfel-rasterization_part): same keywords
+        as forward plus semantics (P x S, S <= 16); 8 aux channels."""
+        s = self._scene(f64=f64, **kw)
+        sem = _arr(semantics, np.float64 if f64 else np.float32)
+        self._keep.append(sem)
+        S = int(sem.shape[1]) if sem.ndim == 2 else 0
+        rc = self.lib.sro_forward_part(self.h, C.byref(s), 1 if f64 else 0, S,
+                                       sem.ctypes.data if sem.size else None)
+        if rc < 0:
+            raise ValueError("oracle: bad arguments")
+        W, H, tile = int(kw["W"]), int(kw["H"]), int(kw.get("tile", 16))
+        gx, gy = (W + tile - 1) // tile, (H + tile - 1) // tile
+        self.f64 = f64
+        self.dims = dict(P=s.P, M=s.M, W=W, H=H, T=gx * gy, L=self.lib.sro_num_rendered(self.h), S=S, O=8)
+        return rc
+
+    def backward_part(self, dL_dcolor, dL_dsemantic, dL_dothers):
+        dt = np.float64 if self.f64 else np.float32
+        gc, gs, go = _arr(dL_dcolor, dt), _arr(dL_dsemantic, dt), _arr(dL_dothers, dt)
+        self.lib.sro_backward_part(self.h, gc.ctypes.data, gs.ctypes.data if gs.size else None, go.ctypes.data)
 
     def replay(self, **kw) -> None:
         """Re-evaluates the recorded forward on new inputs (same keywords as

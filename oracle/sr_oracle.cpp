@@ -134,12 +134,16 @@ struct Scene {
   int P, D, M, W, H, B, prefiltered;
   bool has_shs, has_colors, has_sr, has_tprec;
   double mod, tanx, tany;
+  bool part = false;  // the diff-surfel-rasterization_part variant
+  int S = 0;          // its semantic channels (<= 16)
 };
 
 template <typename R> struct Ctx {
   Scene sc{};
   int gx = 0, gy = 0, T = 0, N = 0;
   std::vector<R> means3D, shs, colors, opac, scales, rots, tprec, vm, pm, campos, bg;
+  std::vector<R> sem, out_sem, g_sem;  // _part: semantics (P x S), image (S x H x W), gradient
+  std::vector<double> acc_sem;
   // geometry state (dsr/cuda_rasterizer/rasterizer_impl.h:29-44)
   std::vector<int> radii;
   std::vector<R> depths, means2D, transMat, normal_opacity, rgb;
@@ -667,6 +671,400 @@ template <typename R> void run_replay(Ctx<R>& c) {
   c.replay = false;
 }
 
+
+// ===========================================================================
+// The _part variant (submodules/diff-surfel-rasterization_part, "dsp/"):
+// what renderer/gaussian_renderer_2d/__init__.py:86-96 calls.  Its math
+// differs from the plain rasteriser's: the homography is built in camera
+// space with intrinsics (fx, fy, W/2, H/2) (dsp/cuda_rasterizer/forward.cu:
+// 75-128), the AABB uses t = (1, 1, -1) and radius ceil(3 max(extent,
+// FilterSize)) (:133-163, 235-239), pixel centres sit at +0.5 (:292), up to
+// 16 semantic channels blend beside the colour (:440, 460) and an eighth
+// aux channel holds the median weight (:471).  Several constants of the
+// reference are double literals (FilterSize, FilterInvSquare, NEAR_PLANE,
+// FAR_PLANE; dsp/cuda_rasterizer/auxiliary.h:19-36), so the expressions
+// that use them are evaluated in double and rounded, as C++ promotion does.
+// ===========================================================================
+inline void atomic_addd(double* p, double v) {
+#pragma omp atomic
+  *p += v;
+}
+
+const double kP_FS = 0.7071067811865476;  // FilterSize
+const double kP_NEAR = 0.2, kP_FAR = 100.0;
+
+// 1/(FilterSize*FilterSize) * x, the macro's evaluation order.
+template <typename R> inline R part_fis_mul(R x) { return (R)((1.0 / (kP_FS * kP_FS)) * (double)x); }
+
+template <typename R> void preprocess_one_part(Ctx<R>& c, int idx) {
+  const Scene& s = c.sc;
+  c.radii[idx] = 0;
+  c.tiles_touched[idx] = 0;
+  const R* m = &c.means3D[3 * idx];
+  const R* vm = c.vm.data();
+  const R vx = ((vm[0] * m[0] + vm[4] * m[1]) + vm[8] * m[2]) + vm[12];
+  const R vy = ((vm[1] * m[0] + vm[5] * m[1]) + vm[9] * m[2]) + vm[13];
+  const R vz = ((vm[2] * m[0] + vm[6] * m[1]) + vm[10] * m[2]) + vm[14];
+  if (vz <= (R)0.2f) {  // in_frustum, dsp/cuda_rasterizer/auxiliary.h:180-191
+    if (s.prefiltered) c.prefilter_violation = 1;
+    return;
+  }
+  // focal = W / (2 tan(fov/2)) (rasterizer_impl.cu:223-224); intrins.zw =
+  // float(W)/2.0, float(H)/2.0 (dsp forward.cu:213)
+  const R fx = (R)(float)s.W / ((R)2.0f * (R)(float)s.tanx);
+  const R fy = (R)(float)s.H / ((R)2.0f * (R)(float)s.tany);
+  const R cx = (R)((double)(float)s.W / 2.0), cy = (R)((double)(float)s.H / 2.0);
+  auto Wv = [&](const V3<R>& v) {
+    return v3((vm[0] * v.x + vm[4] * v.y) + vm[8] * v.z, (vm[1] * v.x + vm[5] * v.y) + vm[9] * v.z,
+              (vm[2] * v.x + vm[6] * v.y) + vm[10] * v.z);
+  };
+  V3<R> Tu, Tv, Tw, nrm;
+  if (!s.has_tprec) {
+    // computeTransMat, dsp forward.cu:75-128 (scale_to_mat(.., 1.0f): the
+    // scale modifier is not applied)
+    R w, x, y, z;
+    V3<R> col[3];
+    quat_cols(&c.rots[4 * idx], w, x, y, z, col);
+    const R sx = c.scales[2 * idx], sy = c.scales[2 * idx + 1];
+    const V3<R> M0 = Wv(sx * col[0]), M1 = Wv(sy * col[1]);
+    const V3<R> pv = v3(vx, vy, vz);
+    V3<R> tn = Wv(col[2]);
+    const R cosv = ((-tn.x * pv.x) + (-tn.y * pv.y)) + (-tn.z * pv.z);
+    if (cosv == (R)0) return;  // BACKFACE_CULL
+    const R mult = cosv > (R)0 ? (R)1 : (R)-1;  // DUAL_VISIABLE
+    tn = mult * tn;
+    Tu = v3(fx * M0.x + cx * M0.z, fx * M1.x + cx * M1.z, fx * pv.x + cx * pv.z);
+    Tv = v3(fy * M0.y + cy * M0.z, fy * M1.y + cy * M1.z, fy * pv.y + cy * pv.z);
+    Tw = v3(M0.z, M1.z, pv.z);
+    nrm = tn;
+  } else {
+    // the reference leaves the normal unset here; (0, 0, 1) as the plain
+    // rasteriser's precomputed path
+    const R* t = &c.tprec[9 * idx];
+    Tu = v3(t[0], t[1], t[2]);
+    Tv = v3(t[3], t[4], t[5]);
+    Tw = v3(t[6], t[7], t[8]);
+    nrm = v3((R)0, (R)0, (R)1);
+  }
+  // computeAABB, dsp forward.cu:133-163
+  const R d = ((R)1 * (Tw.x * Tw.x) + (R)1 * (Tw.y * Tw.y)) + (R)-1 * (Tw.z * Tw.z);
+  if (d == (R)0) return;
+  const R inv = (R)1.0f / d;
+  const R f0 = (R)1 * inv, f1 = (R)1 * inv, f2 = (R)-1 * inv;
+  const R px = (f0 * (Tu.x * Tw.x) + f1 * (Tu.y * Tw.y)) + f2 * (Tu.z * Tw.z);
+  const R py = (f0 * (Tv.x * Tw.x) + f1 * (Tv.y * Tw.y)) + f2 * (Tv.z * Tw.z);
+  const R h0x = px * px - ((f0 * (Tu.x * Tu.x) + f1 * (Tu.y * Tu.y)) + f2 * (Tu.z * Tu.z));
+  const R h0y = py * py - ((f0 * (Tv.x * Tv.x) + f1 * (Tv.y * Tv.y)) + f2 * (Tv.z * Tv.z));
+  const R hx = std::sqrt(rmax((R)0, h0x)), hy = std::sqrt(rmax((R)0, h0y));
+  // radius = ceil(3 * max(max(ext.x, ext.y), FilterSize)) with the double
+  // FilterSize (forward.cu:235-239)
+  const R ext = std::fmax(hx, hy);
+  const float radius = (float)std::ceil(3.0 * std::fmax((double)ext, kP_FS));
+  const int ir = f2i(radius);
+  int x0, y0, x1, y1;
+  get_rect(px, py, ir, s.B, c.gx, c.gy, x0, y0, x1, y1);
+  const uint32_t area = (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
+  if (area == 0) return;
+  if (!s.has_colors) {
+    const V3<R> rgb = color_from_sh(c, idx, &c.clamped[3 * idx]);
+    c.rgb[3 * idx] = rgb.x;
+    c.rgb[3 * idx + 1] = rgb.y;
+    c.rgb[3 * idx + 2] = rgb.z;
+  } else {
+    for (int k = 0; k < 3; ++k) c.rgb[3 * idx + k] = c.colors[3 * idx + k];
+  }
+  c.flip[idx] = 1;
+  c.depths[idx] = vz;
+  c.radii[idx] = ir;
+  c.means2D[2 * idx] = px;
+  c.means2D[2 * idx + 1] = py;
+  R* T9 = &c.transMat[9 * idx];
+  T9[0] = Tu.x; T9[1] = Tu.y; T9[2] = Tu.z;
+  T9[3] = Tv.x; T9[4] = Tv.y; T9[5] = Tv.z;
+  T9[6] = Tw.x; T9[7] = Tw.y; T9[8] = Tw.z;
+  R* no = &c.normal_opacity[4 * idx];
+  no[0] = nrm.x; no[1] = nrm.y; no[2] = nrm.z; no[3] = c.opac[idx];
+  c.tiles_touched[idx] = area;
+}
+
+// The per-pair evaluation both passes share (dsp forward.cu:352-385,
+// backward.cu:293-318).  Returns false for a skipped pair.
+template <typename R> struct PartPair {
+  V3<R> k, l, p;
+  R sx, sy, rho3d, rho2d, dx, dy, depth, G, alpha;
+};
+template <typename R> bool part_pair(const Ctx<R>& c, int id, R pixx, R pixy, PartPair<R>& e) {
+  const R* T9 = &c.transMat[9 * id];
+  const V3<R> Tu = v3(T9[0], T9[1], T9[2]), Tv = v3(T9[3], T9[4], T9[5]), Tw = v3(T9[6], T9[7], T9[8]);
+  e.k = v3(-Tu.x + pixx * Tw.x, -Tu.y + pixx * Tw.y, -Tu.z + pixx * Tw.z);
+  e.l = v3(-Tv.x + pixy * Tw.x, -Tv.y + pixy * Tw.y, -Tv.z + pixy * Tw.z);
+  e.p = cross(e.k, e.l);
+  if (e.p.z == (R)0) return false;
+  e.sx = e.p.x / e.p.z;
+  e.sy = e.p.y / e.p.z;
+  e.rho3d = e.sx * e.sx + e.sy * e.sy;
+  e.dx = c.means2D[2 * id] - pixx;
+  e.dy = c.means2D[2 * id + 1] - pixy;
+  e.rho2d = part_fis_mul<R>(e.dx * e.dx + e.dy * e.dy);
+  const R rho = rmin(e.rho3d, e.rho2d);
+  e.depth = (e.rho3d <= e.rho2d) ? (e.sx * Tw.x + e.sy * Tw.y) + Tw.z : Tw.z;
+  if ((double)e.depth < kP_NEAR) return false;
+  const R power = (R)-0.5f * rho;
+  if (power > (R)0) return false;
+  e.G = rexp<R>(power);
+  e.alpha = rmin((R)0.99f, c.normal_opacity[4 * id + 3] * e.G);
+  return !(e.alpha < (R)(1.0f / 255.0f));
+}
+
+// mapped depth (FAR d - FAR NEAR) / ((FAR - NEAR) d) in double (forward.cu:393)
+template <typename R> inline R part_mapped(R depth) {
+  return (R)((kP_FAR * (double)depth - kP_FAR * kP_NEAR) / ((kP_FAR - kP_NEAR) * (double)depth));
+}
+
+// Forward blend of one pixel, dsp/cuda_rasterizer/forward.cu:256-474.
+template <typename R> void render_pixel_part(Ctx<R>& c, int tile, int pxi, int pyi) {
+  const int W = c.sc.W, H = c.sc.H, S = c.sc.S;
+  const uint32_t rs = c.ranges[2 * tile], re = c.ranges[2 * tile + 1];
+  const R pixx = (R)((double)(float)pxi + 0.5), pixy = (R)((double)(float)pyi + 0.5);
+  R T = (R)1;
+  uint32_t contributor = 0, last_contributor = 0;
+  R C[3] = {0, 0, 0}, N[3] = {0, 0, 0}, Sem[16] = {0};
+  R D = 0, dist1 = 0, dist2 = 0, distortion = 0, median_depth = 0, median_weight = 0, median_contributor = (R)-1;
+  for (uint32_t e = rs; e < re; ++e) {
+    contributor++;
+    const int id = (int)c.point_list[e];
+    PartPair<R> q;
+    if (!part_pair(c, id, pixx, pixy, q)) continue;
+    const R alpha = q.alpha, depth = q.depth;
+    const R test_T = T * ((R)1 - alpha);
+    if (test_T < (R)0.0001f) break;
+    const R A = (R)1 - T;
+    const R md = part_mapped(depth);
+    const R error = ((md * md) * A + dist2) - ((R)2 * md) * dist1;
+    distortion += (error * alpha) * T;
+    if (T > (R)0.5) {
+      median_depth = depth;
+      median_weight = alpha * T;
+      median_contributor = (R)contributor;
+    }
+    const R* no = &c.normal_opacity[4 * id];
+    for (int ch = 0; ch < 3; ++ch) N[ch] += (no[ch] * alpha) * T;
+    D += (depth * alpha) * T;
+    dist1 += (md * alpha) * T;
+    dist2 += ((md * md) * alpha) * T;
+    for (int ch = 0; ch < 3; ++ch) C[ch] += (c.rgb[3 * id + ch] * alpha) * T;
+    for (int ch = 0; ch < S; ++ch) Sem[ch] += (c.sem[(size_t)id * S + ch] * alpha) * T;
+    T = test_T;
+    last_contributor = contributor;
+  }
+  const size_t HW = (size_t)H * W, pid = (size_t)W * pyi + pxi;
+  c.final_T[pid] = T;
+  c.n_contrib[pid] = last_contributor;
+  for (int ch = 0; ch < 3; ++ch) c.out_color[ch * HW + pid] = C[ch] + T * c.bg[ch];
+  for (int ch = 0; ch < S; ++ch) c.out_sem[ch * HW + pid] = Sem[ch];
+  c.n_contrib[pid + HW] = f2u(median_contributor);
+  c.final_T[pid + HW] = dist1;
+  c.final_T[pid + 2 * HW] = dist2;
+  c.out_others[pid] = D;
+  c.out_others[pid + HW] = (R)1 - T;
+  for (int ch = 0; ch < 3; ++ch) c.out_others[pid + (2 + ch) * HW] = N[ch];
+  c.out_others[pid + 5 * HW] = median_depth;
+  c.out_others[pid + 6 * HW] = distortion;
+  c.out_others[pid + 7 * HW] = median_weight;
+}
+
+// Backward blend of one pixel, dsp/cuda_rasterizer/backward.cu:140-470.
+template <typename R>
+void render_bwd_pixel_part(Ctx<R>& c, int tile, int pxi, int pyi, const R* dL_dpix, const R* dL_dsem,
+                           const R* dL_dothers) {
+  const int W = c.sc.W, H = c.sc.H, S = c.sc.S;
+  const size_t HW = (size_t)H * W, pid = (size_t)W * pyi + pxi;
+  const uint32_t rs = c.ranges[2 * tile], re = c.ranges[2 * tile + 1];
+  const R pixx = (R)((double)(float)pxi + 0.5), pixy = (R)((double)(float)pyi + 0.5);
+  const R T_final = c.final_T[pid];
+  R T = T_final;
+  uint32_t contributor = re - rs;
+  const int last_contributor = (int)c.n_contrib[pid];
+  const int median_contributor = (int)c.n_contrib[pid + HW];
+  R accum_rec[3] = {0, 0, 0}, dL_dpixel[3], last_color[3] = {0, 0, 0}, dL_dpixel_sem[16] = {0};
+  const R dL_ddepth = dL_dothers[pid], dL_daccum = dL_dothers[HW + pid], dL_dreg = dL_dothers[6 * HW + pid];
+  R dL_dnormal2D[3];
+  for (int i = 0; i < 3; ++i) dL_dnormal2D[i] = dL_dothers[(2 + i) * HW + pid];
+  const R dL_dmedian_depth = dL_dothers[5 * HW + pid], dL_dmax_dweight = dL_dothers[7 * HW + pid];
+  R last_depth = 0, last_normal[3] = {0, 0, 0}, accum_depth_rec = 0, accum_alpha_rec = 0;
+  R accum_normal_rec[3] = {0, 0, 0};
+  const R final_D = c.final_T[pid + HW], final_D2 = c.final_T[pid + 2 * HW];
+  const R final_A = (R)1 - T_final;
+  R last_dL_dT = 0, last_alpha = 0;
+  for (int i = 0; i < 3; ++i) dL_dpixel[i] = dL_dpix[i * HW + pid];
+  for (int i = 0; i < S; ++i) dL_dpixel_sem[i] = dL_dsem[i * HW + pid];
+  for (uint32_t e = re; e > rs; --e) {
+    contributor--;
+    if (contributor >= (uint32_t)last_contributor) continue;
+    const int id = (int)c.point_list[e - 1];
+    PartPair<R> q;
+    if (!part_pair(c, id, pixx, pixy, q)) continue;
+    const R G = q.G, alpha = q.alpha, c_d = q.depth;
+    T = T / ((R)1 - alpha);
+    const R dchannel_dcolor = alpha * T;
+    R dL_dalpha = 0;
+    for (int ch = 0; ch < 3; ++ch) {
+      const R col = c.rgb[3 * id + ch];
+      accum_rec[ch] = last_alpha * last_color[ch] + ((R)1 - last_alpha) * accum_rec[ch];
+      last_color[ch] = col;
+      dL_dalpha += (col - accum_rec[ch]) * dL_dpixel[ch];
+      atomic_addd(&c.acc_colors[3 * id + ch], (double)(dchannel_dcolor * dL_dpixel[ch]));
+    }
+    // semantics: their gradient only (the alpha term is commented out in
+    // the reference, backward.cu:346)
+    for (int ch = 0; ch < S; ++ch)
+      atomic_addd(&c.acc_sem[(size_t)id * S + ch], (double)(dchannel_dcolor * dL_dpixel_sem[ch]));
+    R dL_dz = 0, dL_dweight = 0;
+    const R m_d = part_mapped(c_d);
+    const R dmd_dd = (R)((kP_FAR * kP_NEAR) / (((kP_FAR - kP_NEAR) * (double)c_d) * (double)c_d));
+    if ((int)contributor == median_contributor - 1) {
+      dL_dz += dL_dmedian_depth;
+      dL_dweight += dL_dmax_dweight;
+    }
+    dL_dweight += ((final_D2 + (m_d * m_d) * final_A) - ((R)2 * m_d) * final_D) * dL_dreg;
+    dL_dalpha += dL_dweight - last_dL_dT;
+    last_dL_dT = dL_dweight * alpha + ((R)1 - alpha) * last_dL_dT;
+    const R dL_dmd = (((R)2.0f * (T * alpha)) * (m_d * final_A - final_D)) * dL_dreg;
+    dL_dz += dL_dmd * dmd_dd;
+    accum_depth_rec = last_alpha * last_depth + ((R)1 - last_alpha) * accum_depth_rec;
+    last_depth = c_d;
+    dL_dalpha += (c_d - accum_depth_rec) * dL_ddepth;
+    accum_alpha_rec = (R)((double)last_alpha * 1.0 + (double)(((R)1 - last_alpha) * accum_alpha_rec));
+    dL_dalpha += ((R)1 - accum_alpha_rec) * dL_daccum;
+    const R* no = &c.normal_opacity[4 * id];
+    for (int ch = 0; ch < 3; ++ch) {
+      accum_normal_rec[ch] = last_alpha * last_normal[ch] + ((R)1 - last_alpha) * accum_normal_rec[ch];
+      last_normal[ch] = no[ch];
+      dL_dalpha += (no[ch] - accum_normal_rec[ch]) * dL_dnormal2D[ch];
+      atomic_addd(&c.acc_normal[3 * id + ch], (double)((alpha * T) * dL_dnormal2D[ch]));
+    }
+    dL_dalpha *= T;
+    last_alpha = alpha;
+    R bg_dot_dpixel = 0;
+    for (int i = 0; i < 3; ++i) bg_dot_dpixel += c.bg[i] * dL_dpixel[i];
+    dL_dalpha += (-T_final / ((R)1 - alpha)) * bg_dot_dpixel;
+    const R dL_dG = no[3] * dL_dalpha;
+    dL_dz += (alpha * T) * dL_ddepth;
+    const R* T9 = &c.transMat[9 * id];
+    if (q.rho3d <= q.rho2d) {
+      const R dL_dsx = (dL_dG * -G) * q.sx + dL_dz * T9[6];
+      const R dL_dsy = (dL_dG * -G) * q.sy + dL_dz * T9[7];
+      const R dsx_pz = dL_dsx / q.p.z, dsy_pz = dL_dsy / q.p.z;
+      const V3<R> dL_dp = v3(dsx_pz, dsy_pz, -(dsx_pz * q.sx + dsy_pz * q.sy));
+      const V3<R> dL_dk = cross(q.l, dL_dp);
+      const V3<R> dL_dl = cross(dL_dp, q.k);
+      double* g = &c.acc_transMat[9 * id];
+      atomic_addd(&g[0], (double)(-dL_dk.x));
+      atomic_addd(&g[1], (double)(-dL_dk.y));
+      atomic_addd(&g[2], (double)(-dL_dk.z));
+      atomic_addd(&g[3], (double)(-dL_dl.x));
+      atomic_addd(&g[4], (double)(-dL_dl.y));
+      atomic_addd(&g[5], (double)(-dL_dl.z));
+      atomic_addd(&g[6], (double)((pixx * dL_dk.x + pixy * dL_dl.x) + dL_dz * q.sx));
+      atomic_addd(&g[7], (double)((pixx * dL_dk.y + pixy * dL_dl.y) + dL_dz * q.sy));
+      atomic_addd(&g[8], (double)((pixx * dL_dk.z + pixy * dL_dl.z) + dL_dz * (R)1));
+    } else {
+      // -G * FilterInvSquare * d.x: ((-G * 1) / (FS * FS)) * d.x in double
+      const R dG_ddelx = (R)(((double)(-G) / (kP_FS * kP_FS)) * (double)q.dx);
+      const R dG_ddely = (R)(((double)(-G) / (kP_FS * kP_FS)) * (double)q.dy);
+      atomic_addd(&c.acc_mean2D[3 * id], (double)(dL_dG * dG_ddelx));
+      atomic_addd(&c.acc_mean2D[3 * id + 1], (double)(dL_dG * dG_ddely));
+      atomic_addd(&c.acc_transMat[9 * id + 8], (double)dL_dz);
+    }
+    atomic_addd(&c.acc_opacity[id], (double)(G * dL_dalpha));
+  }
+}
+
+// Backward preprocess, dsp/cuda_rasterizer/backward.cu:473-700: the AABB
+// adjoint added into dL_dtransMat for every visible surfel and the
+// densification statistic (computeAABB, :621-671), then the camera-space
+// homography's adjoint (computeTransMat, :473-545) and the SH backward.
+template <typename R> void preprocess_bwd_one_part(Ctx<R>& c, int idx) {
+  const Scene& s = c.sc;
+  if (!(c.radii[idx] > 0)) return;
+  const R fx = (R)(float)s.W / ((R)2.0f * (R)(float)s.tanx);
+  const R fy = (R)(float)s.H / ((R)2.0f * (R)(float)s.tany);
+  const R tx = (R)(float)s.tanx, ty = (R)(float)s.tany;
+  const R Wh = fx * tx, Hh = fy * ty;  // "W", "H" of BACKWARD::preprocess (:685-686)
+  R* gT = &c.g_transMat[9 * idx];
+  const R* T9 = &c.transMat[9 * idx];
+  const V3<R> T0 = v3(T9[0], T9[1], T9[2]), T1 = v3(T9[3], T9[4], T9[5]), T3 = v3(T9[6], T9[7], T9[8]);
+  const R gmx = c.g_mean2D[3 * idx], gmy = c.g_mean2D[3 * idx + 1];
+  {
+    const V3<R> tv = v3<R>(1, 1, -1);
+    const R d = (tv.x * (T3.x * T3.x) + tv.y * (T3.y * T3.y)) + tv.z * (T3.z * T3.z);
+    const R invd = (R)1.0f / d;
+    const V3<R> f = v3(tv.x * invd, tv.y * invd, tv.z * invd);
+    auto had = [](V3<R> a, V3<R> b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); };
+    const V3<R> dL_dT0 = had(gmx * f, T3);
+    const V3<R> dL_dT1 = had(gmy * f, T3);
+    V3<R> dL_dT3 = had(gmx * f, T0) + had(gmy * f, T1);
+    const V3<R> dL_df = had(gmx * T0, T3) + had(gmy * T1, T3);
+    const R dotdf = (dL_df.x * f.x + dL_df.y * f.y) + dL_df.z * f.z;
+    const R dL_dd = (R)((double)dotdf * (-1.0 / (double)d));
+    const V3<R> dd_dT3 = v3((tv.x * T3.x) * (R)2.0f, (tv.y * T3.y) * (R)2.0f, (tv.z * T3.z) * (R)2.0f);
+    dL_dT3 = dL_dT3 + dL_dd * dd_dT3;
+    gT[0] += dL_dT0.x; gT[1] += dL_dT0.y; gT[2] += dL_dT0.z;
+    gT[3] += dL_dT1.x; gT[4] += dL_dT1.y; gT[5] += dL_dT1.z;
+    gT[6] += dL_dT3.x; gT[7] += dL_dT3.y; gT[8] += dL_dT3.z;
+    const R z = T9[8];
+    c.g_mean2D[3 * idx] = (gT[2] * z) * Wh;
+    c.g_mean2D[3 * idx + 1] = (gT[5] * z) * Hh;
+  }
+  const R* m = &c.means3D[3 * idx];
+  if (!s.has_tprec) {
+    const R* vm = c.vm.data();
+    const R cx = fx * tx, cy = fy * ty;  // intrins.zw of the backward (:596)
+    R qw, qx, qy, qz;
+    V3<R> Rc[3];
+    quat_cols(&c.rots[4 * idx], qw, qx, qy, qz, Rc);
+    const R sx = c.scales[2 * idx], sy = c.scales[2 * idx + 1];
+    // dL_dM = (P^T dL_dT^T) (:512-518): column j = (fx dTu_j, fy dTv_j, (cx dTu_j + cy dTv_j) + dTw_j)
+    V3<R> dM[3];
+    for (int j = 0; j < 3; ++j) dM[j] = v3(fx * gT[j], fy * gT[3 + j], (cx * gT[j] + cy * gT[3 + j]) + gT[6 + j]);
+    auto Wt = [&](const V3<R>& v) {
+      return v3((vm[0] * v.x + vm[1] * v.y) + vm[2] * v.z, (vm[4] * v.x + vm[5] * v.y) + vm[6] * v.z,
+                (vm[8] * v.x + vm[9] * v.y) + vm[10] * v.z);
+    };
+    const V3<R> dRS0 = Wt(dM[0]), dRS1 = Wt(dM[1]), dpw = Wt(dM[2]);
+    const R* gn = &c.g_normal[3 * idx];
+    V3<R> dL_dtn = Wt(v3(gn[0], gn[1], gn[2]));
+    // DUAL_VISIABLE (:528-533): the sign from tn = W R[2] against p_view
+    const V3<R> L2 = Rc[2];
+    const V3<R> tn = v3((vm[0] * L2.x + vm[4] * L2.y) + vm[8] * L2.z, (vm[1] * L2.x + vm[5] * L2.y) + vm[9] * L2.z,
+                        (vm[2] * L2.x + vm[6] * L2.y) + vm[10] * L2.z);
+    const R pvx = ((vm[0] * m[0] + vm[4] * m[1]) + vm[8] * m[2]) + vm[12];
+    const R pvy = ((vm[1] * m[0] + vm[5] * m[1]) + vm[9] * m[2]) + vm[13];
+    const R pvz = ((vm[2] * m[0] + vm[6] * m[1]) + vm[10] * m[2]) + vm[14];
+    const R cosv = ((-tn.x * pvx) + (-tn.y * pvy)) + (-tn.z * pvz);
+    const R mult = cosv > (R)0 ? (R)1 : (R)-1;
+    dL_dtn = mult * dL_dtn;
+    const V3<R> dR0 = v3(dRS0.x * sx, dRS0.y * sx, dRS0.z * sx);
+    const V3<R> dR1 = v3(dRS1.x * sy, dRS1.y * sy, dRS1.z * sy);
+    const R vR[3][3] = {{dR0.x, dR0.y, dR0.z}, {dR1.x, dR1.y, dR1.z}, {dL_dtn.x, dL_dtn.y, dL_dtn.z}};
+    const R w = qw, x = qx, y = qy, z = qz;
+    R* gq = &c.g_rots[4 * idx];
+    gq[0] = (R)2 * ((x * (vR[1][2] - vR[2][1]) + y * (vR[2][0] - vR[0][2])) + z * (vR[0][1] - vR[1][0]));
+    gq[1] = (R)2 * ((((R)-2 * x) * (vR[1][1] + vR[2][2]) + y * (vR[0][1] + vR[1][0])) + z * (vR[0][2] + vR[2][0]) +
+                    w * (vR[1][2] - vR[2][1]));
+    gq[2] = (R)2 * (((x * (vR[0][1] + vR[1][0]) - ((R)2 * y) * (vR[0][0] + vR[2][2])) + z * (vR[1][2] + vR[2][1])) +
+                    w * (vR[2][0] - vR[0][2]));
+    gq[3] = (R)2 * (((x * (vR[0][2] + vR[2][0]) + y * (vR[1][2] + vR[2][1])) - ((R)2 * z) * (vR[0][0] + vR[1][1])) +
+                    w * (vR[0][1] - vR[1][0]));
+    c.g_scales[2 * idx] = (dRS0.x * Rc[0].x + dRS0.y * Rc[0].y) + dRS0.z * Rc[0].z;
+    c.g_scales[2 * idx + 1] = (dRS1.x * Rc[1].x + dRS1.y * Rc[1].y) + dRS1.z * Rc[1].z;
+    c.g_means3D[3 * idx] = dpw.x;
+    c.g_means3D[3 * idx + 1] = dpw.y;
+    c.g_means3D[3 * idx + 2] = dpw.z;
+  }
+  if (s.has_shs) sh_backward(c, idx);
+}
+
 template <typename R> void run_forward(Ctx<R>& c) {
   const Scene& s = c.sc;
   const int P = s.P;
@@ -688,11 +1086,17 @@ template <typename R> void run_forward(Ctx<R>& c) {
     c.dec_mult.assign(P, 0);
     c.dec_pix.assign((size_t)s.W * s.H, std::vector<uint32_t>());
   }
+  if (s.part) {
 #pragma omp parallel for schedule(static)
-  for (int i = 0; i < P; ++i) preprocess_one(c, i);
+    for (int i = 0; i < P; ++i) preprocess_one_part(c, i);
+  } else {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < P; ++i) preprocess_one(c, i);
+  }
   binning(c);
   c.out_color.assign(3 * (size_t)c.N, 0);
-  c.out_others.assign(7 * (size_t)c.N, 0);
+  c.out_others.assign((s.part ? 8 : 7) * (size_t)c.N, 0);
+  c.out_sem.assign((size_t)s.S * c.N, 0);
   c.final_T.assign(3 * (size_t)c.N, 0);
   c.n_contrib.assign(2 * (size_t)c.N, 0);
   const int B = s.B;
@@ -702,7 +1106,10 @@ template <typename R> void run_forward(Ctx<R>& c) {
     for (int j = 0; j < B; ++j)
       for (int i = 0; i < B; ++i) {
         const int x = tx * B + i, y = ty * B + j;
-        if (x < s.W && y < s.H) render_pixel(c, t, x, y);
+        if (x < s.W && y < s.H) {
+          if (s.part) render_pixel_part(c, t, x, y);
+          else render_pixel(c, t, x, y);
+        }
       }
   }
   if (c.record) {
@@ -713,10 +1120,6 @@ template <typename R> void run_forward(Ctx<R>& c) {
   }
 }
 
-inline void atomic_addd(double* p, double v) {
-#pragma omp atomic
-  *p += v;
-}
 
 // ---------------------------------------------------------------------------
 // Per-pixel backward blend, dsr/cuda_rasterizer/backward.cu:143-440.
@@ -952,7 +1355,7 @@ template <typename R> void preprocess_bwd_one(Ctx<R>& c, int idx) {
   V3<R> Rc[3];
   R qw = 0, qx = 0, qy = 0, qz = 0, scx = 0, scy = 0;
   V3<R> Pc[3][2];  // P columns as (xyz, w) halves: P[k] = (a, b) with a = rows 0..2, b.x = row 3
-  R P4[3][4];
+  R P4[3][4] = {};
   const bool precomp = s.has_tprec;
   const R* m = &c.means3D[3 * idx];
   if (precomp) {
@@ -1054,9 +1457,11 @@ template <typename R> void preprocess_bwd_one(Ctx<R>& c, int idx) {
   c.g_mean2D[3 * idx + 1] = (R)(((double)(gT[5] * depth) * 0.5) * (double)(R)(float)H);
 }
 
-template <typename R> void run_backward(Ctx<R>& c, const R* dL_dpix, const R* dL_dothers) {
+template <typename R>
+void run_backward(Ctx<R>& c, const R* dL_dpix, const R* dL_dothers, const R* dL_dsem = nullptr) {
   const Scene& s = c.sc;
   const int P = s.P;
+  c.acc_sem.assign((size_t)P * s.S, 0.0);
   c.acc_transMat.assign(9 * (size_t)P, 0.0);
   c.acc_mean2D.assign(3 * (size_t)P, 0.0);
   c.acc_normal.assign(3 * (size_t)P, 0.0);
@@ -1069,7 +1474,10 @@ template <typename R> void run_backward(Ctx<R>& c, const R* dL_dpix, const R* dL
     for (int j = 0; j < B; ++j)
       for (int i = 0; i < B; ++i) {
         const int x = tx * B + i, y = ty * B + j;
-        if (x < s.W && y < s.H) render_bwd_pixel(c, t, x, y, dL_dpix, dL_dothers);
+        if (x < s.W && y < s.H) {
+          if (s.part) render_bwd_pixel_part(c, t, x, y, dL_dpix, dL_dsem, dL_dothers);
+          else render_bwd_pixel(c, t, x, y, dL_dpix, dL_dothers);
+        }
       }
   }
   auto round_into = [](std::vector<R>& dst, const std::vector<double>& src) {
@@ -1085,8 +1493,14 @@ template <typename R> void run_backward(Ctx<R>& c, const R* dL_dpix, const R* dL
   c.g_sh.assign((size_t)P * s.M * 3, (R)0);
   c.g_scales.assign(2 * (size_t)P, (R)0);
   c.g_rots.assign(4 * (size_t)P, (R)0);
+  round_into(c.g_sem, c.acc_sem);
+  if (s.part) {
 #pragma omp parallel for schedule(static)
-  for (int i = 0; i < P; ++i) preprocess_bwd_one(c, i);
+    for (int i = 0; i < P; ++i) preprocess_bwd_one_part(c, i);
+  } else {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < P; ++i) preprocess_bwd_one(c, i);
+  }
 }
 
 struct Handle {
@@ -1148,6 +1562,8 @@ template <typename R> int get_named(Ctx<R>& c, const std::string& n, void* dst, 
   if (n == "ranges") return copy_out(c.ranges, dst, nbytes);
   if (n == "out_color") return copy_out(c.out_color, dst, nbytes);
   if (n == "out_others") return copy_out(c.out_others, dst, nbytes);
+  if (n == "out_semantic") return copy_out(c.out_sem, dst, nbytes);
+  if (n == "dL_dsemantics") return copy_out(c.g_sem, dst, nbytes);
   if (n == "final_T") return copy_out(c.final_T, dst, nbytes);
   if (n == "n_contrib") return copy_out(c.n_contrib, dst, nbytes);
   if (n == "acc_dL_dtransMat") return copy_out(c.acc_transMat, dst, nbytes);
@@ -1280,6 +1696,37 @@ int sro_replay(void* hv, const sro_scene_c* s) {
 int sro_alpha_capped(void* hv) {
   Handle* h = (Handle*)hv;
   return h->f64 ? h->cd.dec_alpha_clamped : h->cf.dec_alpha_clamped;
+}
+
+// The _part variant's forward: as sro_forward, plus S <= 16 semantic
+// channels per surfel (semantics P x S, the precision of the run).
+int sro_forward_part(void* hv, const sro_scene_c* s, int f64, int S, const void* semantics) {
+  Handle* h = (Handle*)hv;
+  if (!h || !s || S < 0 || S > 16 || (S > 0 && !semantics)) return -1;
+  if (!s || s->P < 0 || s->W <= 0 || s->H <= 0 || s->tile <= 0) return -1;
+  if ((s->shs == nullptr) == (s->colors_precomp == nullptr)) return -1;
+  if ((s->scales == nullptr || s->rotations == nullptr) == (s->transMat_precomp == nullptr)) return -1;
+  h->f64 = f64;
+  auto go = [&](auto& c) {
+    load_scene(c, s, f64);
+    c.sc.part = true;
+    c.sc.S = S;
+    copy_in(c.sem, semantics, (size_t)s->P * S, f64);
+    run_forward(c);
+    return c.prefilter_violation;
+  };
+  return f64 ? go(h->cd) : go(h->cf);
+}
+
+// Backward of sro_forward_part: dL_dcolor 3 x H x W, dL_dsemantic S x H x W,
+// dL_dothers 8 x H x W.
+int sro_backward_part(void* hv, const void* dL_dcolor, const void* dL_dsem, const void* dL_dothers) {
+  Handle* h = (Handle*)hv;
+  if (h->f64)
+    run_backward(h->cd, (const double*)dL_dcolor, (const double*)dL_dothers, (const double*)dL_dsem);
+  else
+    run_backward(h->cf, (const float*)dL_dcolor, (const float*)dL_dothers, (const float*)dL_dsem);
+  return 0;
 }
 
 int sro_num_rendered(void* hv) {
