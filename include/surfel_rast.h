@@ -136,6 +136,32 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
                 const float* dL_dcolor, const float* dL_dothers, sr_grads* grads,
                 sr_stream_t stream);
 
+/* ---- the _part variant --------------------------------------------------
+ * The rasteriser of submodules/diff-surfel-rasterization_part, which
+ * renderer/gaussian_renderer_2d/__init__.py:86-96 calls: camera-space
+ * homography, pixel centres at +0.5, S <= 16 semantic channels blended
+ * beside the colour, an 8-channel out_others (depth, alpha, normal xyz,
+ * median depth, distortion, median weight).  The settings, surfels,
+ * allocation callback and state buffers are those of sr_forward (its
+ * capacity and pair_store fields are not used).
+ *   sr_forward_part  <- RasterizeGaussiansCUDA, dsp/rasterize_points.cu:40-146
+ *   sr_backward_part <- RasterizeGaussiansBackwardCUDA, dsp/rasterize_points.cu:149-252
+ * (dsp = submodules/diff-surfel-rasterization_part).
+ * semantics is P x S (device), out_semantic S x H x W, out_others 8 x H x W. */
+int sr_forward_part(const sr_settings* s, const sr_surfels* g, int S, const float* semantics,
+                    sr_alloc_fn alloc, void* alloc_ctx, float* out_color, float* out_semantic,
+                    float* out_others, int* radii, int* num_rendered, sr_stream_t stream);
+/* Backward of sr_forward_part: dL_dsemantic S x H x W, dL_dothers 8 x H x W;
+ * dL_dsemantics (P x S) receives the semantics' gradient.  As the
+ * reference, the semantic images' gradient does not reach the geometry
+ * (dsp/cuda_rasterizer/backward.cu:346).  With a precomputed transMat the
+ * scale and rotation gradients are zero (the reference reads inputs it was
+ * not given there). */
+int sr_backward_part(const sr_settings* s, const sr_surfels* g, int S, const float* semantics, int R,
+                     const int* radii, void* geom_buf, const void* bin_buf, const void* img_buf,
+                     const float* dL_dcolor, const float* dL_dsemantic, const float* dL_dothers,
+                     sr_grads* grads, float* dL_dsemantics, sr_stream_t stream);
+
 /* present[i] = 1 when surfel i passes the near test (view z > 0.2). */
 int sr_mark_visible(int P, const float* means3D, const float* viewmatrix,
                     const float* projmatrix, uint8_t* present, sr_stream_t stream);

@@ -59,6 +59,8 @@ EXPORTED = (
     "sr_forward_batched",
     "sr_backward_batched",
     "sr_pair_count",
+    "sr_forward_part",
+    "sr_backward_part",
 )
 
 SR_MAX_VIEWS = 32
@@ -198,6 +200,12 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_version.argtypes = []
         lib.sr_version.restype = C.c_char_p
         lib.sr_pair_count.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint)]
+        lib.sr_forward_part.argtypes = [C.POINTER(SrSettings), C.POINTER(SrSurfels), i, vp, ALLOC_FN, vp, vp, vp,
+                                        vp, vp, C.POINTER(C.c_int), vp]
+        lib.sr_forward_part.restype = i
+        lib.sr_backward_part.argtypes = [C.POINTER(SrSettings), C.POINTER(SrSurfels), i, vp, i, vp, vp, vp, vp, vp,
+                                         vp, vp, C.POINTER(SrGrads), vp, vp]
+        lib.sr_backward_part.restype = i
         lib.sr_pair_count.restype = i
         _lib = lib
         return lib

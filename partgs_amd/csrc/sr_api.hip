@@ -186,6 +186,8 @@ struct Pipeline {
   BinLayout bl{};
   int capacity = 0;
   bool side_huge = false;  // huge-tile sort on a second stream (graph-captured batches)
+  bool part = false;       // the _part variant
+  int S = 0;               // its semantic channels
   uint32_t* counters() const { return (uint32_t*)(img + il.counters); }
 };
 
@@ -341,6 +343,7 @@ int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, 
   pa.D = pl.s_D;
   pa.M = g->M;
   pa.prefiltered = pl.prefiltered;
+  pa.part = pl.part ? 1 : 0;
   pa.mod = pl.mod;
   pa.means3D = g->means3D;
   pa.scales = g->scales;
@@ -373,7 +376,7 @@ int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, 
 
 // Scatter into tile buckets, sort every tile, split into quadrant sub-lists,
 // then the acceptance pass over all (quadrant, chunk) work items.
-int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
+int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st, bool pairs = true) {
   uint32_t* counters = pl.counters();
   char* bin = pl.bin;
   const BinLayout& bl = pl.bl;
@@ -412,6 +415,7 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st) {
   sa.work_count = counters + 2;
   sa.q_flag = (uint32_t*)(pl.img + pl.il.q_flag);
   SR_STAGE(3, dbg, st, "sort", launch_sort_tiles(sa, st));
+  if (!pairs) return SR_OK;  // the _part variant's blend walks the sub-lists itself
   AcceptArgs aa{};
   aa.vs = pl.vs;
   aa.ranges = sa.ranges;
@@ -723,6 +727,181 @@ int sr_backward(const sr_settings* s, const sr_surfels* g, int R, const int* rad
   // the backward writes every output once, so nothing is cleared first.
   RecordGrads rg{gr->dL_dtransMat, gr->dL_dmeans2D, gr->dL_dopacity, gr->dL_dcolors};
   return run_backward(pl, g, radii, dL_dcolor, dL_dothers, rg, gr, R > 0 && bin_buf != nullptr, dbg, st);
+}
+
+// ---- the _part variant -----------------------------------------------------
+namespace {
+PartArgs part_args(const Pipeline& pl, const float* sem) {
+  PartArgs pa{};
+  pa.vs = pl.vs;
+  pa.S = pl.S;
+  pa.ranges = (const uint32_t*)(pl.img + pl.il.ranges);
+  pa.sub = (const uint2*)(pl.bin + pl.bl.sub);
+  pa.sub_count = (const uint32_t*)(pl.img + pl.il.sub_count);
+  pa.rec = (const float4*)(pl.geom + pl.gl.rec);
+  pa.sem = sem;
+  pa.bg = pl.bg;
+  return pa;
+}
+
+int validate_part(const sr_settings* s, const sr_surfels* g, int S, const float* semantics) {
+  int rc = validate(s, g);
+  if (rc != SR_OK) return rc;
+  if (S < 0 || S > 16) return fail(SR_ERR_INVALID, "semantic channels must be in [0, 16] (got %d)", S);
+  if (g->P > 0 && S > 0 && !semantics) return fail(SR_ERR_INVALID, "semantics must not be NULL");
+  return SR_OK;
+}
+}  // namespace
+
+int sr_forward_part(const sr_settings* s, const sr_surfels* g, int S, const float* semantics, sr_alloc_fn alloc,
+                    void* ctx, float* out_color, float* out_semantic, float* out_others, int* radii,
+                    int* num_rendered, sr_stream_t stream_) {
+  hipStream_t st = (hipStream_t)stream_;
+  int rc = validate_part(s, g, S, semantics);
+  if (rc != SR_OK) return rc;
+  if (!alloc || !out_color || !out_others || !num_rendered || (S > 0 && !out_semantic))
+    return fail(SR_ERR_INVALID, "NULL output");
+  if (g->P > 0 && (!radii || !g->opacities)) return fail(SR_ERR_INVALID, "radii and opacities must not be NULL");
+  const int P = g->P, W = s->image_width, H = s->image_height, B = s->tile;
+  const bool dbg = s->debug != 0;
+  *num_rendered = 0;
+  if (P == 0) {
+    SR_STAGE(4, dbg, st, "clear", launch_fill_f32(out_color, 0.0f, 3 * (size_t)W * H, st));
+    SR_STAGE(4, dbg, st, "clear", launch_fill_f32(out_others, 0.0f, 8 * (size_t)W * H, st));
+    if (S > 0) SR_STAGE(4, dbg, st, "clear", launch_fill_f32(out_semantic, 0.0f, (size_t)S * W * H, st));
+    return SR_OK;
+  }
+  Pipeline pl{};
+  pl.vs = single_view(W, H, B, s->tanfovx, s->tanfovy);
+  pl.P = P;
+  pl.s_D = s->sh_degree;
+  pl.prefiltered = s->prefiltered;
+  pl.mod = s->scale_modifier;
+  pl.part = true;
+  pl.S = S;
+  pl.vm = s->viewmatrix;
+  pl.pm = s->projmatrix;
+  pl.campos = s->campos;
+  pl.bg = s->bg;
+  pl.gl = geom_layout((size_t)P);
+  pl.il = image_layout(pl.vs);
+  pl.geom = (char*)alloc(ctx, pl.gl.total, 0);
+  pl.img = (char*)alloc(ctx, pl.il.total, 2);
+  if (!pl.geom || !pl.img) return fail(SR_ERR_ALLOC, "allocation callback failed");
+  uint32_t* host = pinned_counters();
+  if (!host) return fail(SR_ERR_HIP, "hipHostMalloc failed");
+  rc = run_front(pl, g, radii, 0xFFFFFFFFu, dbg, st);
+  if (rc != SR_OK) return rc;
+  SR_HIP(hipMemcpyAsync(host, pl.counters(), 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SR_HIP(hipStreamSynchronize(st));
+  if (host[1] & kErrPrefiltered)
+    return fail(SR_ERR_PREFILTERED, "Point is filtered although prefiltered is set. This shouldn't happen!");
+  const int L = (int)host[0];
+  *num_rendered = L;
+  pl.bl = bin_layout(L, B, pl.il.T, 0, 1, S);
+  pl.bin = (char*)alloc(ctx, pl.bl.total, 1);
+  if (!pl.bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
+  pl.capacity = L;
+  if (L > 0) {
+    rc = run_binning(pl, radii, dbg, st, false);
+    if (rc != SR_OK) return rc;
+  } else {
+    SR_STAGE(3, dbg, st, "clear",
+             launch_fill_u32((uint32_t*)(pl.img + pl.il.sub_count), 0u, (size_t)pl.il.T * quads_per_tile(B), st));
+  }
+  PartArgs pa = part_args(pl, semantics);
+  pa.final_T = (float*)(pl.img + pl.il.final_T);
+  pa.n_contrib = (uint32_t*)(pl.img + pl.il.n_contrib);
+  pa.out_color = out_color;
+  pa.out_sem = out_semantic;
+  pa.out_others = out_others;
+  SR_STAGE(4, dbg, st, "blend_fwd", launch_part_blend_fwd(pa, st));
+  return SR_OK;
+}
+
+int sr_backward_part(const sr_settings* s, const sr_surfels* g, int S, const float* semantics, int R,
+                     const int* radii, void* geom_buf, const void* bin_buf, const void* img_buf,
+                     const float* dL_dcolor, const float* dL_dsemantic, const float* dL_dothers, sr_grads* gr,
+                     float* dL_dsemantics, sr_stream_t stream_) {
+  hipStream_t st = (hipStream_t)stream_;
+  int rc = validate_part(s, g, S, semantics);
+  if (rc != SR_OK) return rc;
+  if (!gr || !dL_dcolor || !dL_dothers || (S > 0 && (!dL_dsemantic || !dL_dsemantics)))
+    return fail(SR_ERR_INVALID, "NULL gradient argument");
+  const int P = g->P, W = s->image_width, H = s->image_height, B = s->tile;
+  if (P == 0) return SR_OK;
+  if (!geom_buf || !img_buf || !radii) return fail(SR_ERR_INVALID, "forward state missing");
+  if (R < 0) return fail(SR_ERR_INVALID, "num_rendered must be >= 0");
+  if (!gr->dL_dmeans2D || !gr->dL_dcolors || !gr->dL_dopacity || !gr->dL_dmeans3D || !gr->dL_dtransMat ||
+      !gr->dL_dscales || !gr->dL_drotations || (g->shs && !gr->dL_dsh))
+    return fail(SR_ERR_INVALID, "gradient output missing");
+  const bool dbg = s->debug != 0;
+  Pipeline pl{};
+  pl.vs = single_view(W, H, B, s->tanfovx, s->tanfovy);
+  pl.P = P;
+  pl.s_D = s->sh_degree;
+  pl.part = true;
+  pl.S = S;
+  pl.vm = s->viewmatrix;
+  pl.pm = s->projmatrix;
+  pl.campos = s->campos;
+  pl.bg = s->bg;
+  pl.gl = geom_layout((size_t)P);
+  pl.il = image_layout(pl.vs);
+  pl.bl = bin_layout(R, B, pl.il.T, 0, 1, S);
+  pl.geom = (char*)geom_buf;
+  pl.img = (char*)img_buf;
+  pl.bin = (char*)bin_buf;
+  pl.capacity = R;
+  const bool blend = R > 0 && bin_buf != nullptr;
+  if (blend) {
+    PartArgs pa = part_args(pl, semantics);
+    pa.final_T_in = (const float*)(pl.img + pl.il.final_T);
+    pa.n_contrib_in = (const uint32_t*)(pl.img + pl.il.n_contrib);
+    pa.dL_dpix = dL_dcolor;
+    pa.dL_dsem = dL_dsemantic;
+    pa.dL_dothers = dL_dothers;
+    pa.inst_at = (const uint32_t*)(pl.bin + pl.bl.inst_at);
+    pa.part = (float*)(pl.bin + pl.bl.part);
+    pa.sem_part = (float*)(pl.bin + pl.bl.sem_part);
+    pa.part_nz = (uint8_t*)(pl.bin + pl.bl.part_nz);
+    SR_STAGE(6, dbg, st, "blend_bwd", launch_part_blend_bwd(pa, st));
+  }
+  PreprocessBwdArgs pb{};
+  pb.vs = pl.vs;
+  pb.P = P;
+  pb.D = pl.s_D;
+  pb.M = g->M;
+  pb.counters = pl.counters();
+  pb.tiles = (const uint32_t*)(pl.geom + pl.gl.tiles);
+  pb.roff = (const uint32_t*)(pl.geom + pl.gl.roff);
+  pb.inst_quads = blend ? (const uint16_t*)(pl.bin + pl.bl.inst_quads) : nullptr;
+  pb.part = blend ? (const float*)(pl.bin + pl.bl.part) : nullptr;
+  pb.part_nz = blend ? (const uint8_t*)(pl.bin + pl.bl.part_nz) : nullptr;
+  pb.sem_part = blend ? (const float*)(pl.bin + pl.bl.sem_part) : nullptr;
+  pb.S = S;
+  pb.means3D = g->means3D;
+  pb.scales = g->scales;
+  pb.rots = g->rotations;
+  pb.shs = g->shs;
+  pb.tprec = g->transMat_precomp;
+  pb.rec = (const float4*)(pl.geom + pl.gl.rec);
+  pb.radii = radii;
+  pb.clamped = (const uint8_t*)(pl.geom + pl.gl.clamped);
+  pb.vm = pl.vm;
+  pb.pm = pl.pm;
+  pb.campos = pl.campos;
+  pb.dL_dtransMat = gr->dL_dtransMat;
+  pb.dL_dcolors = gr->dL_dcolors;
+  pb.dL_dopacity = gr->dL_dopacity;
+  pb.dL_dsh = gr->dL_dsh;
+  pb.dL_dmean2D = gr->dL_dmeans2D;
+  pb.dL_dmean3D = gr->dL_dmeans3D;
+  pb.dL_dscales = gr->dL_dscales;
+  pb.dL_drots = gr->dL_drotations;
+  pb.dL_dsem = dL_dsemantics;
+  SR_STAGE(7, dbg, st, "preprocess_bwd", launch_part_preprocess_bwd(pb, st));
+  return SR_OK;
 }
 
 size_t sr_batch_workspace_bytes(const sr_views* v, int P, int capacity) {

@@ -1064,4 +1064,160 @@ hipError_t launch_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Backward preprocess of the _part variant (one view; dsp = submodules/
+// diff-surfel-rasterization_part/cuda_rasterizer): per visible surfel the
+// blend's partial gradients summed over its instances, then the AABB-centre
+// adjoint added into dL_dtransMat for every visible surfel and the
+// densification statistic (computeAABB, dsp/backward.cu:621-671), then the
+// camera-space homography's adjoint (computeTransMat, :473-545) and the SH
+// backward.  Every output is written (zeros for a culled surfel).
+__global__ __launch_bounds__(256) void k_part_preprocess_bwd(PreprocessBwdArgs a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.P) return;
+  const int S = a.S;
+  float* gT = a.dL_dtransMat + 9 * idx;
+  float* gm2 = a.dL_dmean2D + 3 * idx;
+  if (!(a.radii[idx] > 0)) {
+    for (int k = 0; k < 9; ++k) gT[k] = 0.0f;
+    for (int k = 0; k < 3; ++k) gm2[k] = a.dL_dmean3D[3 * idx + k] = a.dL_dcolors[3 * idx + k] = 0.0f;
+    a.dL_dopacity[idx] = 0.0f;
+    a.dL_dscales[2 * idx] = a.dL_dscales[2 * idx + 1] = 0.0f;
+    for (int k = 0; k < 4; ++k) a.dL_drots[4 * idx + k] = 0.0f;
+    for (int ch = 0; ch < S; ++ch) a.dL_dsem[(size_t)idx * S + ch] = 0.0f;
+    if (a.shs != nullptr)
+      for (int k = 0; k < 3 * a.M; ++k) a.dL_dsh[(size_t)idx * a.M * 3 + k] = 0.0f;
+    return;
+  }
+  float gs[18];
+  reduce_record(a, (size_t)idx, gs);
+  // semantic partials, in the same (instance, quadrant) order
+  {
+    float sm[16];
+    for (int ch = 0; ch < 16; ++ch) sm[ch] = 0.0f;
+    const uint32_t n = a.tiles[idx];
+    const int Q = quads_per_tile(a.vs.B);
+    const uint32_t i0 = a.roff[idx];
+    if (S > 0 && a.part != nullptr)
+      for (uint32_t i = i0; i < i0 + n; ++i) {
+        uint32_t qm = a.inst_quads[i];
+        while (qm != 0u) {
+          const int q = __builtin_ctz(qm);
+          qm &= qm - 1u;
+          const size_t slot = (size_t)i * Q + q;
+          if (a.part_nz[slot] == 0) continue;
+          for (int ch = 0; ch < S; ++ch) sm[ch] += a.sem_part[slot * S + ch];
+        }
+      }
+    for (int ch = 0; ch < S; ++ch) a.dL_dsem[(size_t)idx * S + ch] = sm[ch];
+  }
+  const ViewInfo& vi = a.vs.v[0];
+  const float tanx = vi.tanx, tany = vi.tany;
+  const float fx = (float)vi.W / (2.0f * tanx), fy = (float)vi.H / (2.0f * tany);
+  const float Wh = fx * tanx, Hh = fy * tany;  // "W", "H" of BACKWARD::preprocess (:685-686)
+  const float4* rc = a.rec + (size_t)kRecQuads * idx;
+  const float4 r0 = rc[0], r1 = rc[1], r2 = rc[2];
+  const F3 T0 = f3(r0.x, r0.y, r0.z), T1 = f3(r0.w, r1.x, r1.y), T3 = f3(r1.z, r1.w, r2.x);
+  F3 dT0 = f3(gs[0], gs[1], gs[2]), dT1 = f3(gs[3], gs[4], gs[5]), dT2 = f3(gs[6], gs[7], gs[8]);
+  {
+    const float gmx = gs[9], gmy = gs[10];
+    const F3 tv = f3(1.0f, 1.0f, -1.0f);
+    const float d = (tv.x * (T3.x * T3.x) + tv.y * (T3.y * T3.y)) + tv.z * (T3.z * T3.z);
+    const float invd = 1.0f / d;
+    const F3 f = f3(tv.x * invd, tv.y * invd, tv.z * invd);
+    const F3 dL_dT0 = had(gmx * f, T3);
+    const F3 dL_dT1 = had(gmy * f, T3);
+    F3 dL_dT3 = had(gmx * f, T0) + had(gmy * f, T1);
+    const F3 dL_df = had(gmx * T0, T3) + had(gmy * T1, T3);
+    const float dotdf = (dL_df.x * f.x + dL_df.y * f.y) + dL_df.z * f.z;
+    const float dL_dd = (float)((double)dotdf * (-1.0 / (double)d));
+    const F3 dd_dT3 = f3((tv.x * T3.x) * 2.0f, (tv.y * T3.y) * 2.0f, (tv.z * T3.z) * 2.0f);
+    dL_dT3 = dL_dT3 + dL_dd * dd_dT3;
+    dT0 = dT0 + dL_dT0;
+    dT1 = dT1 + dL_dT1;
+    dT2 = dT2 + dL_dT3;
+  }
+  gT[0] = dT0.x; gT[1] = dT0.y; gT[2] = dT0.z;
+  gT[3] = dT1.x; gT[4] = dT1.y; gT[5] = dT1.z;
+  gT[6] = dT2.x; gT[7] = dT2.y; gT[8] = dT2.z;
+  const float z = r2.x;
+  gm2[0] = (dT0.z * z) * Wh;
+  gm2[1] = (dT1.z * z) * Hh;
+  gm2[2] = 0.0f;
+  a.dL_dopacity[idx] = gs[14];
+  a.dL_dcolors[3 * idx] = gs[15];
+  a.dL_dcolors[3 * idx + 1] = gs[16];
+  a.dL_dcolors[3 * idx + 2] = gs[17];
+  const float m0 = a.means3D[3 * idx], m1 = a.means3D[3 * idx + 1], m2 = a.means3D[3 * idx + 2];
+  F3 dm3 = f3(0.0f, 0.0f, 0.0f);
+  if (a.tprec == nullptr) {
+    const float* vm = a.vm;
+    const float cx = fx * tanx, cy = fy * tany;  // intrins.zw of the backward (:596)
+    const float* q = a.rots + 4 * idx;
+    const float q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    const float n2 = ((q3 * q3 + q0 * q0) + q1 * q1) + q2 * q2;
+    const float s = 1.0f / sqrtf(n2);
+    const float w = q0 * s, x = q1 * s, y = q2 * s, zq = q3 * s;
+    const F3 Rc0 = f3(1.0f - 2.0f * (y * y + zq * zq), 2.0f * (x * y + w * zq), 2.0f * (x * zq - w * y));
+    const F3 Rc1 = f3(2.0f * (x * y - w * zq), 1.0f - 2.0f * (x * x + zq * zq), 2.0f * (y * zq + w * x));
+    const F3 Rc2 = f3(2.0f * (x * zq + w * y), 2.0f * (y * zq - w * x), 1.0f - 2.0f * (x * x + y * y));
+    const float sx = a.scales[2 * idx], sy = a.scales[2 * idx + 1];
+    auto Wt = [&](F3 v) {
+      return f3((vm[0] * v.x + vm[1] * v.y) + vm[2] * v.z, (vm[4] * v.x + vm[5] * v.y) + vm[6] * v.z,
+                (vm[8] * v.x + vm[9] * v.y) + vm[10] * v.z);
+    };
+    // dL_dM = (P^T dL_dT^T): column j = (fx dTu_j, fy dTv_j, (cx dTu_j + cy dTv_j) + dTw_j)
+    const F3 dM0 = f3(fx * dT0.x, fy * dT1.x, (cx * dT0.x + cy * dT1.x) + dT2.x);
+    const F3 dM1 = f3(fx * dT0.y, fy * dT1.y, (cx * dT0.y + cy * dT1.y) + dT2.y);
+    const F3 dM2 = f3(fx * dT0.z, fy * dT1.z, (cx * dT0.z + cy * dT1.z) + dT2.z);
+    const F3 dRS0 = Wt(dM0), dRS1 = Wt(dM1), dpw = Wt(dM2);
+    F3 dL_dtn = Wt(f3(gs[11], gs[12], gs[13]));
+    // DUAL_VISIABLE (:528-533): the sign from tn = W R[2] against p_view
+    const F3 tn = f3((vm[0] * Rc2.x + vm[4] * Rc2.y) + vm[8] * Rc2.z, (vm[1] * Rc2.x + vm[5] * Rc2.y) + vm[9] * Rc2.z,
+                     (vm[2] * Rc2.x + vm[6] * Rc2.y) + vm[10] * Rc2.z);
+    const float pvx = ((vm[0] * m0 + vm[4] * m1) + vm[8] * m2) + vm[12];
+    const float pvy = ((vm[1] * m0 + vm[5] * m1) + vm[9] * m2) + vm[13];
+    const float pvz = ((vm[2] * m0 + vm[6] * m1) + vm[10] * m2) + vm[14];
+    const float cosv = ((-tn.x * pvx) + (-tn.y * pvy)) + (-tn.z * pvz);
+    dL_dtn = (cosv > 0.0f ? 1.0f : -1.0f) * dL_dtn;
+    const F3 dR0 = f3(dRS0.x * sx, dRS0.y * sx, dRS0.z * sx);
+    const F3 dR1 = f3(dRS1.x * sy, dRS1.y * sy, dRS1.z * sy);
+    const float vR[3][3] = {{dR0.x, dR0.y, dR0.z}, {dR1.x, dR1.y, dR1.z}, {dL_dtn.x, dL_dtn.y, dL_dtn.z}};
+    float* gq = a.dL_drots + 4 * idx;
+    gq[0] = 2.0f * ((x * (vR[1][2] - vR[2][1]) + y * (vR[2][0] - vR[0][2])) + zq * (vR[0][1] - vR[1][0]));
+    gq[1] = 2.0f * ((((-2.0f * x) * (vR[1][1] + vR[2][2]) + y * (vR[0][1] + vR[1][0])) + zq * (vR[0][2] + vR[2][0])) +
+                    w * (vR[1][2] - vR[2][1]));
+    gq[2] = 2.0f * (((x * (vR[0][1] + vR[1][0]) - (2.0f * y) * (vR[0][0] + vR[2][2])) + zq * (vR[1][2] + vR[2][1])) +
+                    w * (vR[2][0] - vR[0][2]));
+    gq[3] = 2.0f * (((x * (vR[0][2] + vR[2][0]) + y * (vR[1][2] + vR[2][1])) - (2.0f * zq) * (vR[0][0] + vR[1][1])) +
+                    w * (vR[0][1] - vR[1][0]));
+    a.dL_dscales[2 * idx] = (dRS0.x * Rc0.x + dRS0.y * Rc0.y) + dRS0.z * Rc0.z;
+    a.dL_dscales[2 * idx + 1] = (dRS1.x * Rc1.x + dRS1.y * Rc1.y) + dRS1.z * Rc1.z;
+    dm3 = dpw;
+  } else {
+    // the reference reads scales and rotations it was not given here; with
+    // a precomputed T these gradients are zero
+    a.dL_dscales[2 * idx] = a.dL_dscales[2 * idx + 1] = 0.0f;
+    for (int k = 0; k < 4; ++k) a.dL_drots[4 * idx + k] = 0.0f;
+  }
+  if (a.shs != nullptr) {
+    F3 dRGB = f3(gs[15], gs[16], gs[17]);
+    const uchar4 cl = reinterpret_cast<const uchar4*>(a.clamped)[idx];
+    if (cl.x) dRGB.x = 0.0f;
+    if (cl.y) dRGB.y = 0.0f;
+    if (cl.z) dRGB.z = 0.0f;
+    const F3 dir_orig = f3(m0 - a.campos[0], m1 - a.campos[1], m2 - a.campos[2]);
+    dm3 = dm3 + sh_backward<false, false, false>(a.D, a.M, a.shs + (size_t)idx * a.M * 3, dir_orig, dRGB,
+                                                  a.dL_dsh + (size_t)idx * a.M * 3, nullptr);
+  }
+  a.dL_dmean3D[3 * idx] = dm3.x;
+  a.dL_dmean3D[3 * idx + 1] = dm3.y;
+  a.dL_dmean3D[3 * idx + 2] = dm3.z;
+}
+
+hipError_t launch_part_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s) {
+  if (a.P == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_part_preprocess_bwd, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace sr

@@ -302,6 +302,61 @@ SR_DEV bool degenerate(const Geo<float>& g, float vx, float vy, float vz, float 
   return rect_ambiguous(b.px, b.py, radius_f32(b.h0x, b.h0y), B, gx, gy);
 }
 
+// The _part variant's homography, normal and AABB (submodules/
+// diff-surfel-rasterization_part, "dsp/": cuda_rasterizer/forward.cu:75-163,
+// 235-239): T built in camera space with intrinsics (fx, fy, W/2, H/2), no
+// scale modifier, AABB with t = (1, 1, -1), radius ceil(3 max(extent,
+// FilterSize)) with the reference's double FilterSize.  Returns false for a
+// culled surfel.
+constexpr double kPartFS = 0.7071067811865476;
+SR_DEV bool part_geom(const PreprocessArgs& a, int idx, const float* vm, const ViewInfo& vi, float vx, float vy,
+                      float vz, F3& Tu, F3& Tv, F3& Tw, F3& nrm, float& px, float& py, int& ir) {
+  const float fx = (float)vi.W / (2.0f * vi.tanx), fy = (float)vi.H / (2.0f * vi.tany);
+  const float cx = (float)((double)(float)vi.W / 2.0), cy = (float)((double)(float)vi.H / 2.0);
+  auto Wv = [&](F3 v) {
+    return f3((vm[0] * v.x + vm[4] * v.y) + vm[8] * v.z, (vm[1] * v.x + vm[5] * v.y) + vm[9] * v.z,
+              (vm[2] * v.x + vm[6] * v.y) + vm[10] * v.z);
+  };
+  if (a.tprec == nullptr) {
+    const float* q = a.rots + 4 * idx;
+    const float q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    const float n2 = ((q3 * q3 + q0 * q0) + q1 * q1) + q2 * q2;
+    const float s = 1.0f / sqrtf(n2);
+    const float w = q0 * s, x = q1 * s, y = q2 * s, z = q3 * s;
+    const F3 c0 = f3(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + w * z), 2.0f * (x * z - w * y));
+    const F3 c1 = f3(2.0f * (x * y - w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + w * x));
+    const F3 c2 = f3(2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y));
+    const float sx = a.scales[2 * idx], sy = a.scales[2 * idx + 1];
+    const F3 M0 = Wv(sx * c0), M1 = Wv(sy * c1);
+    F3 tn = Wv(c2);
+    const float cosv = ((-tn.x * vx) + (-tn.y * vy)) + (-tn.z * vz);
+    if (cosv == 0.0f) return false;  // BACKFACE_CULL
+    tn = (cosv > 0.0f ? 1.0f : -1.0f) * tn;
+    Tu = f3(fx * M0.x + cx * M0.z, fx * M1.x + cx * M1.z, fx * vx + cx * vz);
+    Tv = f3(fy * M0.y + cy * M0.z, fy * M1.y + cy * M1.z, fy * vy + cy * vz);
+    Tw = f3(M0.z, M1.z, vz);
+    nrm = tn;
+  } else {
+    const float* t = a.tprec + 9 * idx;
+    Tu = f3(t[0], t[1], t[2]);
+    Tv = f3(t[3], t[4], t[5]);
+    Tw = f3(t[6], t[7], t[8]);
+    nrm = f3(0.0f, 0.0f, 1.0f);  // left unset by the reference
+  }
+  const float d = (1.0f * (Tw.x * Tw.x) + 1.0f * (Tw.y * Tw.y)) + -1.0f * (Tw.z * Tw.z);
+  if (d == 0.0f) return false;
+  const float inv = 1.0f / d;
+  const float f0 = 1.0f * inv, f1 = 1.0f * inv, f2 = -1.0f * inv;
+  px = (f0 * (Tu.x * Tw.x) + f1 * (Tu.y * Tw.y)) + f2 * (Tu.z * Tw.z);
+  py = (f0 * (Tv.x * Tw.x) + f1 * (Tv.y * Tw.y)) + f2 * (Tv.z * Tw.z);
+  const float h0x = px * px - ((f0 * (Tu.x * Tu.x) + f1 * (Tu.y * Tu.y)) + f2 * (Tu.z * Tu.z));
+  const float h0y = py * py - ((f0 * (Tv.x * Tv.x) + f1 * (Tv.y * Tv.y)) + f2 * (Tv.z * Tv.z));
+  const float hx = sqrtf(fmax_(0.0f, h0x)), hy = sqrtf(fmax_(0.0f, h0y));
+  const float radius = (float)ceil(3.0 * fmax((double)fmaxf(hx, hy), kPartFS));
+  ir = f2i(radius);
+  return true;
+}
+
 // Forward preprocess of surfel idx in view v (record r = v * P + idx),
 // dsr/cuda_rasterizer/forward.cu:75-145, 148-251.  Returns false for culled
 // surfels, else its tile rectangle in the view's tile grid.  Near-degenerate
@@ -323,13 +378,17 @@ SR_DEV bool preprocess_one(const PreprocessArgs& a, int v, int idx, size_t r, in
     if (a.prefiltered) atomicOr(&a.counters[1], kErrPrefiltered);
     return false;
   }
+  F3 Tu, Tv, Tw, nrm;
+  float px, py, mult;
+  int ir;
+  if (a.part) {
+    if (!part_geom(a, idx, vm, a.vs.v[v], vx, vy, vz, Tu, Tv, Tw, nrm, px, py, ir)) return false;
+    mult = 1.0f;
+  } else {
   Geo<float> g;
   surfel_geom<float>(a, idx, vm, pm, W, H, g);
   const float cosv = -((vx * g.n[0] + vy * g.n[1]) + vz * g.n[2]);
   Box<float> b = aabb_terms(g);
-  F3 Tu, Tv, Tw, nrm;
-  float px, py, mult;
-  int ir;
   if (!degenerate(g, vx, vy, vz, cosv, b, a.vs.B, gx, gy)) {
     mult = cosv > 0.0f ? 1.0f : -1.0f;
     Tu = f3(g.Tu[0], g.Tu[1], g.Tu[2]);
@@ -358,6 +417,7 @@ SR_DEV bool preprocess_one(const PreprocessArgs& a, int v, int idx, size_t r, in
     py = (float)B.py;
     ir = radius_f64(B.h0x, B.h0y);
     mult *= 2.0f;  // marks the record as evaluated in FP64
+  }
   }
   // the rectangle from the stored (FP32) centre, as the scatter recomputes it
   get_rect(px, py, ir, a.vs.B, gx, gy, x0, y0, x1, y1);

@@ -159,6 +159,7 @@ struct BinLayout {
   size_t inst_rec, inst_at, inst_quads, part, part_nz;
   size_t prec, fW;
   size_t frag_cap;
+  size_t sem_part;
 };
 constexpr int kPartStride = 20;  // 18 partial gradients, padded to five float4
 // Accepted-pair store: per accepted (pixel, entry) pair the pair-write pass
@@ -192,7 +193,7 @@ inline size_t work_capacity(int L, int B, int T) {
   const size_t Q = (size_t)quads_per_tile(B);
   return Q * ((size_t)T + ((size_t)(L > 0 ? L : 1) + 63) / 64);
 }
-inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0, int pair_capacity = 0) {
+inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0, int pair_capacity = 0, int sem = 0) {
   BinLayout b{};
   size_t off = 0;
   const size_t n = (size_t)(L > 0 ? L : 1);
@@ -219,6 +220,7 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0, int pair_ca
   if (b.frag_cap > 0x7fffffffu) b.frag_cap = 0x7fffffffu;
   b.prec = off; off += align_up(b.frag_cap * kPairQuads * 16);
   b.fW = off; off += align_up(b.frag_cap * 16);
+  b.sem_part = off; off += align_up(n * Q * (size_t)sem * 4);  // _part: semantic partials per (instance, quadrant)
   b.total = off;
   return b;
 }
@@ -235,6 +237,7 @@ constexpr uint32_t kErrCapacity = 2u;  // more tile instances than the binning c
 struct PreprocessArgs {
   ViewSet vs;
   int P, D, M, prefiltered;
+  int part;  // the _part variant's homography and AABB
   float mod;
   const float* means3D;
   const float* scales;
@@ -406,9 +409,45 @@ struct PreprocessBwdArgs {
   float* dL_dmean3D;
   float* dL_dscales;
   float* dL_drots;
+  // the _part variant (one view): semantic partials in, their sums out
+  int S;
+  const float* sem_part;
+  float* dL_dsem;
+};
+
+// The _part variant's blend passes (one view): the plain pipeline's
+// records, ranges and quadrant sub-lists, plus semantics.
+struct PartArgs {
+  ViewSet vs;
+  int S;                     // semantic channels, <= 16
+  const uint32_t* ranges;
+  const uint2* sub;
+  const uint32_t* sub_count;
+  const float4* rec;
+  const float* sem;          // P x S
+  const float* bg;
+  // forward outputs
+  float* final_T;
+  uint32_t* n_contrib;
+  float* out_color;
+  float* out_sem;            // S x H x W
+  float* out_others;         // 8 x H x W
+  // backward
+  const float* final_T_in;
+  const uint32_t* n_contrib_in;
+  const float* dL_dpix;
+  const float* dL_dsem;
+  const float* dL_dothers;
+  const uint32_t* inst_at;
+  float* part;
+  float* sem_part;           // S per (instance, quadrant)
+  uint8_t* part_nz;
 };
 
 // Launchers (return hipGetLastError()).
+hipError_t launch_part_blend_fwd(const PartArgs& a, hipStream_t s);
+hipError_t launch_part_blend_bwd(const PartArgs& a, hipStream_t s);
+
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 // Sets the message sr_last_error() returns on the calling thread; returns code.
@@ -428,6 +467,7 @@ hipError_t launch_blend_fwd(const BlendArgs& a, hipStream_t s);
 hipError_t launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s);
 hipError_t launch_blend_bwd_grad(const BlendBwdArgs& a, size_t max_items, hipStream_t s);
 hipError_t launch_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s);
+hipError_t launch_part_preprocess_bwd(const PreprocessBwdArgs& a, hipStream_t s);
 hipError_t launch_mark_visible(int P, const float* means3D, const float* vm, uint8_t* present,
                                hipStream_t s);
 hipError_t launch_copy_status(const uint32_t* counters, uint32_t* status, hipStream_t s);

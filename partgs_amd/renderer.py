@@ -20,6 +20,7 @@ import math
 
 import torch
 
+from . import rasterizer_part
 from .rasterizer import GaussianRasterizationSettings, GaussianRasterizer
 
 
@@ -128,6 +129,58 @@ def render(viewpoint_camera, pc, pipe, bg_color: torch.Tensor, scaling_modifier:
                                                scales=scales, rotations=rotations, cov3D_precomp=cov3D_precomp)
     rets = {"render": rendered_image, "viewspace_points": screenspace_points, "visibility_filter": radii > 0,
             "radii": radii}
+    rets.update(postprocess_allmap(viewpoint_camera, allmap, pipe.depth_ratio))
+    return rets
+
+
+def render_part(viewpoint_camera, pc, pipe, bg_color: torch.Tensor, scaling_modifier: float = 1.0,
+                override_color=None, tile: int = 16) -> dict:
+    """One view of ``pc`` through the _part rasteriser
+    (renderer/gaussian_renderer_2d/__init__.py:10-150).
+
+    ``pc.get_semantic`` (P, S) with S <= 16 is blended into ``render_semantic``
+    (S, H, W); the other keys are those of ``render``.  With
+    ``compute_cov3D_python`` the reference hands ``pc.get_covariance`` to the
+    rasteriser as the precomputed homography unchanged, and so does this.
+    """
+    xyz = pc.get_xyz
+    screenspace_points = torch.zeros_like(xyz, dtype=xyz.dtype, requires_grad=True, device=xyz.device) + 0
+    try:
+        screenspace_points.retain_grad()
+    except RuntimeError:
+        pass
+    settings = rasterizer_part.GaussianRasterizationSettings(
+        image_height=int(viewpoint_camera.image_height),
+        image_width=int(viewpoint_camera.image_width),
+        tanfovx=math.tan(viewpoint_camera.FoVx * 0.5),
+        tanfovy=math.tan(viewpoint_camera.FoVy * 0.5),
+        bg=bg_color,
+        scale_modifier=scaling_modifier,
+        viewmatrix=viewpoint_camera.world_view_transform,
+        projmatrix=viewpoint_camera.full_proj_transform,
+        sh_degree=pc.active_sh_degree,
+        campos=viewpoint_camera.camera_center,
+        prefiltered=False,
+        debug=False,
+        tile=tile,
+    )
+    rasterizer = rasterizer_part.GaussianRasterizer(raster_settings=settings)
+    scales = rotations = cov3D_precomp = None
+    if getattr(pipe, "compute_cov3D_python", False):
+        cov3D_precomp = pc.get_covariance(scaling_modifier)
+    else:
+        scales = pc.get_scaling
+        rotations = pc.get_rotation
+    shs = colors_precomp = None
+    if override_color is None:
+        shs = pc.get_features
+    else:
+        colors_precomp = override_color
+    rendered_image, semantic, radii, allmap = rasterizer(
+        means3D=xyz, means2D=screenspace_points, opacities=pc.get_opacity, semantics=pc.get_semantic, shs=shs,
+        colors_precomp=colors_precomp, scales=scales, rotations=rotations, cov3D_precomp=cov3D_precomp)
+    rets = {"render": rendered_image, "render_semantic": semantic, "viewspace_points": screenspace_points,
+            "visibility_filter": radii > 0, "radii": radii}
     rets.update(postprocess_allmap(viewpoint_camera, allmap, pipe.depth_ratio))
     return rets
 

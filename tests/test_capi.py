@@ -5,6 +5,7 @@ import os
 import re
 
 import pytest
+import torch
 
 from partgs_amd import _lib
 
@@ -69,6 +70,38 @@ def test_invalid_arguments_are_reported_without_gpu():
     rc = lib.sr_forward(ctypes.byref(s), ctypes.byref(g), fn, None, 1, 1, 1, ctypes.byref(nr), None)
     assert rc == _lib.SR_ERR_INVALID
     assert b"SHs or precomputed colors" in lib.sr_last_error()
+
+
+def test_part_arguments_are_reported_without_gpu():
+    """The _part entry points reject more than 16 semantic channels and NULL
+    semantics before any HIP call; the Python layer checks shapes first."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    s = _lib.SrSettings()
+    s.image_height, s.image_width, s.tile = 48, 64, 16
+    g = _lib.SrSurfels()
+    g.P = 10
+    g.means3D = g.colors_precomp = g.opacities = g.scales = g.rotations = 16  # never dereferenced
+    s.bg = s.viewmatrix = s.projmatrix = s.campos = 16
+    nr = ctypes.c_int(0)
+    fn = _lib.ALLOC_FN(lambda c, n, w: None)
+    rc = lib.sr_forward_part(ctypes.byref(s), ctypes.byref(g), 17, 16, fn, None, 1, 1, 1, 1, ctypes.byref(nr), None)
+    assert rc == _lib.SR_ERR_INVALID
+    assert b"semantic channels" in lib.sr_last_error()
+    rc = lib.sr_forward_part(ctypes.byref(s), ctypes.byref(g), 4, None, fn, None, 1, 1, 1, 1, ctypes.byref(nr), None)
+    assert rc == _lib.SR_ERR_INVALID
+    assert b"semantics must not be NULL" in lib.sr_last_error()
+    gr = _lib.SrGrads()
+    rc = lib.sr_backward_part(ctypes.byref(s), ctypes.byref(g), 4, 16, 0, 16, 16, 16, 16, 16, None, 16,
+                              ctypes.byref(gr), 16, None)
+    assert rc == _lib.SR_ERR_INVALID
+    from partgs_amd import rasterizer_part as RP
+    with pytest.raises(RuntimeError, match="semantic types"):
+        RP._semantics(torch.zeros(10, 17), 10, torch.device("cpu"))
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        RP._semantics(torch.zeros(10, 3), 10, torch.device("cpu"))
+    assert RP._semantics(None, 10, torch.device("cpu"))[1] == 0
 
 
 def _views(V=2, tile=16):
