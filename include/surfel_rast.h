@@ -308,6 +308,17 @@ int sb_generate_bwd_noalloc(const sb_blocks* blk, const float* dL_dmeans3D, cons
                             const float* dL_drotations, const float* dL_dopacities, float* dL_dparams,
                             float* dL_doccupancy, sr_stream_t stream);
 
+/* ---- simple-knn ------------------------------------------------------------
+ * distCUDA2 (submodules/simple-knn/spatial.cu:16-27, SimpleKNN::knn at
+ * simple_knn.cu:185-220): mean_dists[i] = ((b0 + b1) + b2) / 3 over the three
+ * smallest FP32 squared distances (dx dx + dy dy) + dz dz from point i to the
+ * other points (FLT_MAX for missing ones when P < 4).  points is P x 3
+ * float32; the result depends on the point set only (exact search).
+ * workspace: sk_dist2_workspace_bytes(P) bytes of device memory. */
+size_t sk_dist2_workspace_bytes(int P);
+int sk_dist2(int P, const float* points, float* mean_dists, void* workspace, size_t workspace_bytes,
+             sr_stream_t stream);
+
 const char* sr_last_error(void);
 const char* sr_version(void);
 

@@ -55,6 +55,8 @@ EXPORTED = (
     "sb_generate_bwd",
     "sb_bwd_workspace_bytes",
     "sb_generate_bwd_noalloc",
+    "sk_dist2",
+    "sk_dist2_workspace_bytes",
     "sr_batch_workspace_bytes",
     "sr_forward_batched",
     "sr_backward_batched",
@@ -207,6 +209,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
                                          vp, vp, C.POINTER(SrGrads), vp, vp]
         lib.sr_backward_part.restype = i
         lib.sr_pair_count.restype = i
+        lib.sk_dist2_workspace_bytes.argtypes = [i]
+        lib.sk_dist2_workspace_bytes.restype = sz
+        lib.sk_dist2.argtypes = [i, vp, vp, vp, sz, vp]
+        lib.sk_dist2.restype = i
         _lib = lib
         return lib
 

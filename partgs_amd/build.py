@@ -26,7 +26,8 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 LIBDIR = os.path.join(ROOT, "lib")
 LIBNAME = "libsurfel_rast.so"
-SOURCES = ["sr_api.hip", "sr_forward.hip", "sr_backward.hip", "sr_part.hip", "sr_block.hip", "sp_mlp.hip"]
+SOURCES = ["sr_api.hip", "sr_forward.hip", "sr_backward.hip", "sr_part.hip", "sr_block.hip", "sp_mlp.hip",
+           "sk_knn.hip"]
 ARCH = os.environ.get("PARTGS_AMD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
