@@ -308,6 +308,22 @@ int sb_generate_bwd_noalloc(const sb_blocks* blk, const float* dL_dmeans3D, cons
                             const float* dL_drotations, const float* dL_dopacities, float* dL_dparams,
                             float* dL_doccupancy, sr_stream_t stream);
 
+/* ---- superquadric signed distance --------------------------------------------
+ * implicit_sq(points_inv, eps1, eps2, as_sdf=2) with safe powers
+ * (games/block_mesh_splatting/utils/superquadric.py:25-46) of N world points
+ * in every block's frame, points_inv = ((p - T) @ R^T) / (S ratio)
+ * (block_gaussian_model.py:529-531,613-614), S, R, T, eps from the B x 12
+ * block parameters as in sb_blocks.  sdf is B x N (block-major).  The
+ * backward returns dL/d(block parameters) (B x 12) for an upstream dL/dsdf,
+ * summed per block in a fixed order (deterministic); the points get no
+ * gradient (the reference samples them without one, :524-528,604-608).
+ * workspace: sb_sdf_bwd_workspace_bytes(B, N) bytes of device memory. */
+int sb_sdf_fwd(const float* params, int B, float ratio, float scale_min, const float* points, int N, float* sdf,
+               sr_stream_t stream);
+size_t sb_sdf_bwd_workspace_bytes(int B, int N);
+int sb_sdf_bwd(const float* params, int B, float ratio, float scale_min, const float* points, int N,
+               const float* dL_dsdf, float* dL_dparams, void* workspace, size_t workspace_bytes, sr_stream_t stream);
+
 /* ---- simple-knn ------------------------------------------------------------
  * distCUDA2 (submodules/simple-knn/spatial.cu:16-27, SimpleKNN::knn at
  * simple_knn.cu:185-220): mean_dists[i] = ((b0 + b1) + b2) / 3 over the three

@@ -128,3 +128,66 @@ def random_blocks(rng, B, scale_range=(0.1, 0.5), eps_range=(0.2, 1.8)):
     sq_eps = np.log((e - 0.1) / 1.8 / (1 - (e - 0.1) / 1.8))
     occ = rng.normal(size=B)
     return np.concatenate([sq_s, q, t, sq_eps], axis=1), occ
+
+
+# ---------------------------------------------------------------------------
+# Superquadric signed distance and the block-stage losses
+# (utils/superquadric.py:25-46, utils/pytorch.py:32-33,
+# block_gaussian_model.py:522-582), FP64.  Pinned by tests/test_block_losses.py:
+# eps1 = eps2 = 1 gives |p_block| - 1 (a sphere); the block layer's own
+# surface vertices lie on the zero set at eps1 != eps2; a point on the
+# block's y axis at height y gives (2^(e2/e1) 1e-6^(1/e1) + y^(2/e1))^(e1/2) - 1
+# (the clamped x and z terms); the clamps' flat regions; the loss reductions
+# on hand-made fields with known answers.
+# ---------------------------------------------------------------------------
+SQRT_EPS = 1e-6
+
+
+def block_frame(params, scale_min=0.2):
+    """S, R (row-major), T, eps1, eps2 of B x 12 block parameters."""
+    p = np.asarray(params, dtype=np.float64).reshape(-1, 12)
+    S = np.exp(p[:, 0:3]) + scale_min
+    R = quaternion_to_rotation_matrix(normalize(p[:, 3:7]))
+    eps = 1.0 / (1.0 + np.exp(-p[:, 10:12])) * 1.8 + 0.1
+    return S, R, p[:, 7:10], eps[:, 0], eps[:, 1]
+
+
+def implicit_sq_sdf(pinv, eps1, eps2):
+    """implicit_sq(points, eps1, eps2, safe=True, as_sdf=2) on (..., 3)
+    points with eps broadcast against the leading dims."""
+    q = np.clip(pinv, -5.0, 5.0)
+    e1, e2 = np.asarray(eps1, np.float64), np.asarray(eps2, np.float64)
+    sp = lambda t, e: np.maximum(t, SQRT_EPS) ** e  # noqa: E731 (safe_pow)
+    x = sp(q[..., 0] ** 2, 1.0 / e2)
+    y = sp(q[..., 1] ** 2, 1.0 / e1)
+    z = sp(q[..., 2] ** 2, 1.0 / e2)
+    res = sp(x + z, e2 / e1) + y
+    return sp(res, e1 / 2.0) - 1.0
+
+
+def superquadric_sdf(params, points, ratio=0.25, scale_min=0.2):
+    """B x N: implicit_sq of ((p - T) @ R^T) / (S ratio) for every block."""
+    S, R, T, e1, e2 = block_frame(params, scale_min)
+    pts = np.asarray(points, dtype=np.float64).reshape(1, -1, 3)
+    pinv = np.einsum("bnj,bkj->bnk", pts - T[:, None, :], R) / (S[:, None, :] * ratio)
+    return implicit_sq_sdf(pinv, e1[:, None], e2[:, None])
+
+
+def overlap_value(sdf, alpha):
+    """(sum_b sigmoid(-sdf / 0.005) alpha_b - 1.95).clamp(0).mean() (block_gaussian_model.py:534-543)."""
+    occ = 1.0 / (1.0 + np.exp(sdf / 0.005)) * np.asarray(alpha, np.float64).reshape(-1, 1)
+    return np.maximum(occ.sum(0) - 1.95, 0.0).mean()
+
+
+def ray_terms(sdf, label_mask, occ_logits):
+    """(ray_in_mask, ray_out_mask, primitive_opacity) of a B x rays x samples
+    field (block_gaussian_model.py:553-582)."""
+    m = np.asarray(label_mask, np.float64)
+    rays = np.transpose(sdf, (1, 0, 2)).reshape(sdf.shape[1], -1)
+    ray_in = np.maximum(rays.min(1)[m > 0.5] - 1e-3, 0.0).mean()
+    out = np.maximum(np.max(-rays, axis=1) * (m < 0.5) - 1e-3, 0.0).sum() / (m < 0.5).sum()
+    a = 1.0 / (1.0 + np.exp(-np.asarray(occ_logits, np.float64).reshape(-1)))
+    alpha = np.where(sdf <= 1e-8, a[:, None, None], 0.0)
+    alpha = np.transpose(alpha, (1, 0, 2)).reshape(sdf.shape[1], -1).max(1).clip(1e-3, 1 - 1e-3)
+    bce = -(m * np.log(alpha) + (1 - m) * np.log(1 - alpha)).mean()
+    return ray_in, out, bce

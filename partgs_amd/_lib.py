@@ -56,6 +56,9 @@ EXPORTED = (
     "sb_bwd_workspace_bytes",
     "sb_generate_bwd_noalloc",
     "sk_dist2",
+    "sb_sdf_fwd",
+    "sb_sdf_bwd",
+    "sb_sdf_bwd_workspace_bytes",
     "sk_dist2_workspace_bytes",
     "sr_batch_workspace_bytes",
     "sr_forward_batched",
@@ -209,6 +212,12 @@ def load(path: str = LIB_PATH) -> C.CDLL:
                                          vp, vp, C.POINTER(SrGrads), vp, vp]
         lib.sr_backward_part.restype = i
         lib.sr_pair_count.restype = i
+        lib.sb_sdf_fwd.argtypes = [vp, i, f, f, vp, i, vp, vp]
+        lib.sb_sdf_fwd.restype = i
+        lib.sb_sdf_bwd_workspace_bytes.argtypes = [i, i]
+        lib.sb_sdf_bwd_workspace_bytes.restype = sz
+        lib.sb_sdf_bwd.argtypes = [vp, i, f, f, vp, i, vp, vp, vp, sz, vp]
+        lib.sb_sdf_bwd.restype = i
         lib.sk_dist2_workspace_bytes.argtypes = [i]
         lib.sk_dist2_workspace_bytes.restype = sz
         lib.sk_dist2.argtypes = [i, vp, vp, vp, sz, vp]

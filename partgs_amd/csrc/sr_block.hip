@@ -377,6 +377,176 @@ int run_bwd(const Args& a, long long P, const float* dmeans, const float* dscale
   return hipGetLastError() == hipSuccess ? SR_OK : SR_ERR_HIP;
 }
 
+// ---------------------------------------------------------------------------
+// Superquadric signed distance of points to every block: implicit_sq with
+// as_sdf=2 and safe=True (games/block_mesh_splatting/utils/superquadric.py:
+// 25-46, safe_pow pytorch.py:32-33) of the points in each block's frame,
+// ((p - T) @ R^T) / (S ratio) (block_gaussian_model.py:529-531,613-614):
+//   q = clamp(p_block, -5, 5)
+//   x = max(q0^2, 1e-6)^(1/e2), y = max(q1^2, 1e-6)^(1/e1), z = max(q2^2, 1e-6)^(1/e2)
+//   sdf = max(max(x + z, 1e-6)^(e2/e1) + y, 1e-6)^(e1/2) - 1
+// with S, R, T, e1, e2 from the 12 block parameters as for the surfels.  The
+// overlap, coverage and occupancy losses (block_gaussian_model.py:522-582)
+// reduce this (B x N) field; its adjoint returns d/d(12 block parameters).
+// ---------------------------------------------------------------------------
+constexpr float kSdfEps = 1e-6f;  // SQRT_EPS, pytorch.py:6
+constexpr int kSdfThreads = 256;
+constexpr int kSdfSums = 17;  // per block: sum g_k d_j (9), g_k (3), g_k pinv_k (3), dL/de1, dL/de2
+
+struct SdfBlock {
+  float R[9], S[3], t[3], e1, e2;
+};
+
+SR_DEV SdfBlock sdf_block(const float* prm, int b, float scale_min) {
+  float p[12];
+  for (int k = 0; k < 12; ++k) p[k] = prm[12 * b + k];
+  const Block<float> blk = make_block<float, float>(p, scale_min);
+  SdfBlock o;
+  for (int k = 0; k < 9; ++k) o.R[k] = blk.R[k];
+  o.S[0] = blk.Sx;
+  o.S[1] = blk.Sy;
+  o.S[2] = blk.Sz;
+  o.t[0] = blk.tx;
+  o.t[1] = blk.ty;
+  o.t[2] = blk.tz;
+  o.e1 = blk.e1;
+  o.e2 = blk.e2;
+  return o;
+}
+
+// Point in the block frame: pinv_k = (sum_j d_j R_kj) / (S_k ratio), d = p - t.
+SR_DEV void to_block(const SdfBlock& k, float ratio, const float* pt, float* d, float* pinv) {
+  for (int j = 0; j < 3; ++j) d[j] = pt[j] - k.t[j];
+  for (int r = 0; r < 3; ++r)
+    pinv[r] = ((d[0] * k.R[3 * r] + d[1] * k.R[3 * r + 1]) + d[2] * k.R[3 * r + 2]) / (k.S[r] * ratio);
+}
+
+// The signed distance, and with grad != nullptr its derivatives: grad[0..2]
+// along pinv, grad[3] along e1, grad[4] along e2 (the clamps pass gradient
+// where their input is inside the range, as torch.clamp does).
+SR_DEV float implicit_sdf(const float* pinv, float e1, float e2, float* grad) {
+  float q[3], sq[3], base[3], pw[3];
+  const float ie1 = 1.0f / e1, ie2 = 1.0f / e2;
+  const float ex[3] = {ie2, ie1, ie2};
+  for (int k = 0; k < 3; ++k) {
+    q[k] = fminf(fmaxf(pinv[k], -5.0f), 5.0f);
+    sq[k] = q[k] * q[k];
+    base[k] = fmaxf(sq[k], kSdfEps);
+    pw[k] = powf(base[k], ex[k]);
+  }
+  const float xz = pw[0] + pw[2];
+  const float B = fmaxf(xz, kSdfEps);
+  const float r21 = e2 / e1;
+  const float Bp = powf(B, r21);
+  const float res = Bp + pw[1];
+  const float A = fmaxf(res, kSdfEps);
+  const float h = e1 / 2.0f;
+  const float Ap = powf(A, h);
+  if (grad) {
+    const float d_res = res >= kSdfEps ? h * powf(A, h - 1.0f) : 0.0f;
+    const float d_B = xz >= kSdfEps ? d_res * (r21 * powf(B, r21 - 1.0f)) : 0.0f;
+    const float d_r21 = d_res * (Bp * logf(B));
+    float d_e1 = 0.5f * (Ap * logf(A)) + d_r21 * (-e2 / (e1 * e1));
+    float d_e2 = d_r21 * ie1;
+    const float d_pw[3] = {d_B, d_res, d_B};
+    for (int k = 0; k < 3; ++k) {
+      const float d_base = sq[k] >= kSdfEps ? d_pw[k] * (ex[k] * powf(base[k], ex[k] - 1.0f)) : 0.0f;
+      const float d_ex = d_pw[k] * (pw[k] * logf(base[k]));
+      if (k == 1) d_e1 += d_ex * (-ie1 * ie1);
+      else d_e2 += d_ex * (-ie2 * ie2);
+      const bool inside = pinv[k] >= -5.0f && pinv[k] <= 5.0f;
+      grad[k] = inside ? d_base * (2.0f * q[k]) : 0.0f;
+    }
+    grad[3] = d_e1;
+    grad[4] = d_e2;
+  }
+  return Ap - 1.0f;
+}
+
+__global__ __launch_bounds__(kSdfThreads) void k_sdf_fwd(const float* prm, float ratio, float scale_min,
+                                                         const float* pts, int N, float* sdf) {
+  const int b = blockIdx.y, n = blockIdx.x * kSdfThreads + threadIdx.x;
+  if (n >= N) return;
+  const SdfBlock k = sdf_block(prm, b, scale_min);
+  float d[3], pinv[3];
+  to_block(k, ratio, pts + 3 * (size_t)n, d, pinv);
+  sdf[(size_t)b * N + n] = implicit_sdf(pinv, k.e1, k.e2, nullptr);
+}
+
+// Per (block, chunk of 256 points): the 17 sums, each over the chunk's
+// points in a fixed tree order.
+__global__ __launch_bounds__(kSdfThreads) void k_sdf_bwd(const float* prm, float ratio, float scale_min,
+                                                         const float* pts, int N, const float* dsdf, float* part) {
+  __shared__ float s_w[kSdfSums][kSdfThreads / 64];
+  const int b = blockIdx.y, n = blockIdx.x * kSdfThreads + threadIdx.x;
+  float c[kSdfSums];
+  for (int i = 0; i < kSdfSums; ++i) c[i] = 0.0f;
+  if (n < N) {
+    const SdfBlock k = sdf_block(prm, b, scale_min);
+    float d[3], pinv[3], gr[5];
+    to_block(k, ratio, pts + 3 * (size_t)n, d, pinv);
+    implicit_sdf(pinv, k.e1, k.e2, gr);
+    const float g = dsdf[(size_t)b * N + n];
+    for (int r = 0; r < 3; ++r) {
+      const float gk = g * gr[r];
+      for (int j = 0; j < 3; ++j) c[3 * r + j] = gk * d[j];
+      c[9 + r] = gk;
+      c[12 + r] = gk * pinv[r];
+    }
+    c[15] = g * gr[3];
+    c[16] = g * gr[4];
+  }
+  for (int i = 0; i < kSdfSums; ++i) {
+    float v = c[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s_w[i][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kSdfSums) {
+    const float v = (s_w[threadIdx.x][0] + s_w[threadIdx.x][1]) + (s_w[threadIdx.x][2] + s_w[threadIdx.x][3]);
+    part[((size_t)b * gridDim.x + blockIdx.x) * kSdfSums + threadIdx.x] = v;
+  }
+}
+
+// Per block: the chunk sums in a fixed order, then the chain to the 12 block
+// parameters through S = exp(s) + scale_min, R(normalised quaternion) and
+// e = 1.8 sigmoid(.) + 0.1 (dual numbers through make_block).
+__global__ __launch_bounds__(64) void k_sdf_reduce(const float* prm, float ratio, float scale_min, int nchunk,
+                                                   const float* part, float* dprm) {
+  __shared__ float s_tot[kSdfSums];
+  const int b = blockIdx.x;
+  if (threadIdx.x < kSdfSums) {
+    float acc = 0.0f;
+    for (int c = 0; c < nchunk; ++c) acc += part[((size_t)b * nchunk + c) * kSdfSums + threadIdx.x];
+    s_tot[threadIdx.x] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x >= 12) return;
+  const SdfBlock k = sdf_block(prm, b, scale_min);
+  // d/dR_kj, d/dS_k, d/dt_j, d/de1, d/de2 of the loss
+  float dR[9], dS[3], dt[3];
+  for (int r = 0; r < 3; ++r) {
+    const float sr = k.S[r] * ratio;
+    for (int j = 0; j < 3; ++j) dR[3 * r + j] = s_tot[3 * r + j] / sr;
+    dS[r] = -s_tot[12 + r] / k.S[r];
+  }
+  for (int j = 0; j < 3; ++j) {
+    float acc = 0.0f;
+    for (int r = 0; r < 3; ++r) acc += s_tot[9 + r] * k.R[3 * r + j] / (k.S[r] * ratio);
+    dt[j] = -acc;
+  }
+  const int pi = threadIdx.x;
+  Dual<float> p[12];
+  for (int t = 0; t < 12; ++t) p[t] = Dual<float>{prm[12 * b + t], t == pi ? 1.0f : 0.0f};
+  const Block<Dual<float>> blk = make_block<Dual<float>, float>(p, scale_min);
+  float acc = 0.0f;
+  for (int t = 0; t < 9; ++t) acc += dR[t] * blk.R[t].d;
+  acc += dS[0] * blk.Sx.d + dS[1] * blk.Sy.d + dS[2] * blk.Sz.d;
+  acc += dt[0] * blk.tx.d + dt[1] * blk.ty.d + dt[2] * blk.tz.d;
+  acc += s_tot[15] * blk.e1.d + s_tot[16] * blk.e2.d;
+  dprm[12 * b + pi] = acc;
+}
+
 // Scratch of sb_generate_bwd_noalloc: the 13 contributions of up to
 // kNoAllocSurfels surfels (13.6 MB), one copy per GPU the library runs on,
 // held until the process ends.  In host code `scratch` names a host-side
@@ -473,6 +643,46 @@ int sb_generate_bwd_noalloc(const sb_blocks* blk, const float* dL_dmeans3D, cons
   // call, so the scratch needs no clearing
   return sb::run_bwd(a, P, dL_dmeans3D, dL_dscales, dL_drotations, dL_dopacities, dL_dparams, dL_doccupancy,
                      contrib, (hipStream_t)stream);
+}
+
+int sb_sdf_fwd(const float* params, int B, float ratio, float scale_min, const float* points, int N, float* sdf,
+               sr_stream_t stream) {
+  if (B < 0 || N < 0) return sr::set_error(SR_ERR_INVALID, "sb_sdf_fwd: B and N must be >= 0");
+  if ((long long)B * N == 0) return SR_OK;
+  if (!params || !points || !sdf) return sr::set_error(SR_ERR_INVALID, "sb_sdf_fwd: NULL argument");
+  if (B > 65535) return sr::set_error(SR_ERR_INVALID, "sb_sdf_fwd: more than 65535 blocks");
+  hipLaunchKernelGGL(sb::k_sdf_fwd, dim3((unsigned)((N + sb::kSdfThreads - 1) / sb::kSdfThreads), (unsigned)B),
+                     dim3(sb::kSdfThreads), 0, (hipStream_t)stream, params, ratio, scale_min, points, N, sdf);
+  return hipGetLastError() == hipSuccess ? SR_OK : sr::set_error(SR_ERR_HIP, "sb_sdf_fwd: launch failed");
+}
+
+size_t sb_sdf_bwd_workspace_bytes(int B, int N) {
+  if (B < 0 || N < 0) return 0;
+  const size_t nchunk = (size_t)(N + sb::kSdfThreads - 1) / sb::kSdfThreads;
+  return (size_t)(B > 0 ? B : 1) * (nchunk > 0 ? nchunk : 1) * sb::kSdfSums * sizeof(float);
+}
+
+int sb_sdf_bwd(const float* params, int B, float ratio, float scale_min, const float* points, int N,
+               const float* dL_dsdf, float* dL_dparams, void* workspace, size_t workspace_bytes, sr_stream_t stream) {
+  if (B < 0 || N < 0) return sr::set_error(SR_ERR_INVALID, "sb_sdf_bwd: B and N must be >= 0");
+  if (B == 0) return SR_OK;
+  if (!params || !dL_dparams || (N > 0 && (!points || !dL_dsdf)))
+    return sr::set_error(SR_ERR_INVALID, "sb_sdf_bwd: NULL argument");
+  if (B > 65535) return sr::set_error(SR_ERR_INVALID, "sb_sdf_bwd: more than 65535 blocks");
+  hipStream_t st = (hipStream_t)stream;
+  if (N == 0) {
+    return sr::launch_fill_f32(dL_dparams, 0.0f, 12 * (size_t)B, st) == hipSuccess
+               ? SR_OK
+               : sr::set_error(SR_ERR_HIP, "sb_sdf_bwd: launch failed");
+  }
+  if (!workspace || workspace_bytes < sb_sdf_bwd_workspace_bytes(B, N))
+    return sr::set_error(SR_ERR_INVALID, "sb_sdf_bwd: workspace smaller than sb_sdf_bwd_workspace_bytes");
+  const int nchunk = (N + sb::kSdfThreads - 1) / sb::kSdfThreads;
+  hipLaunchKernelGGL(sb::k_sdf_bwd, dim3((unsigned)nchunk, (unsigned)B), dim3(sb::kSdfThreads), 0, st, params, ratio,
+                     scale_min, points, N, dL_dsdf, (float*)workspace);
+  hipLaunchKernelGGL(sb::k_sdf_reduce, dim3((unsigned)B), dim3(64), 0, st, params, ratio, scale_min, nchunk,
+                     (const float*)workspace, dL_dparams);
+  return hipGetLastError() == hipSuccess ? SR_OK : sr::set_error(SR_ERR_HIP, "sb_sdf_bwd: launch failed");
 }
 
 }  // extern "C"
