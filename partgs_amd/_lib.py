@@ -14,7 +14,9 @@ import threading
 import torch  # noqa: F401  (loads the HIP runtime before the library)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libsurfel_rast.so")
+# PARTGS_AMD_LIB selects another build of the library (a variant built with
+# other compile-time constants, for timing it against the default)
+LIB_PATH = os.environ.get("PARTGS_AMD_LIB") or os.path.join(_HERE, "lib", "libsurfel_rast.so")
 
 SR_OK = 0
 SR_ERR_INVALID = 1

@@ -423,6 +423,7 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st, bool p
   aa.sub = sa.sub;
   aa.sub_count = sa.sub_count;
   aa.rec = sc.rec;
+  aa.cullbox = (const short4*)(pl.geom + pl.gl.cullbox);
   aa.work = sa.work;
   aa.work_count = counters + 2;
   aa.masks = (uint64_t*)(bin + bl.masks);

@@ -26,7 +26,10 @@ namespace sr {
 // 11..13 dL_dnormal, 14 dL_dopacity, 15..17 dL_dcolors.
 constexpr int kNumG = 18;
 constexpr int kPre = 8;       // pairs a lane fetches ahead in the per-pair pass
-constexpr int kChainPre = 4;  // stored pairs a lane fetches ahead in the chain pass
+#ifndef SR_CHAIN_PREFETCH
+#define SR_CHAIN_PREFETCH 4
+#endif
+constexpr int kChainPre = SR_CHAIN_PREFETCH;  // stored pairs a lane fetches ahead in the chain pass
 // waves per quadrant in the chain pass (64 / kSplitBwd = 8 pixels each,
 // 8 lanes per pixel)
 constexpr int kSplitBwd = 8;

@@ -130,9 +130,15 @@ SR_DEV F3 color_from_sh(int deg, int M, const float* sh, F3 dir, uint8_t* clampe
 // The box is the union of the disk rho2d <= rho_reject and the bounding box
 // of the projected level set rho3d <= rho_reject (the compute_aabb conic with
 // cutoff^2 = rho_reject, forward.cu:119-145), where rho_reject exceeds
-// rho_max by a margin far above FP32 rounding, plus one pixel.  A level set
-// reaching the camera plane (d >= 0) projects unbounded: no box then.
-SR_DEV short4 cull_box(F3 Tu, F3 Tv, F3 Tw, float mx, float my, float opa, int W, int H, float& rho_rej) {
+// rho_max by 0.1% + 0.02, far above FP32 rounding of any pixel's rho; the
+// conic's extent (whose FP32 evaluation can cancel) gets a further 0.05 pixel
+// plus 1e-3 of its coordinates.  The bounds are on pixel-centre coordinates;
+// pixel i has its centre at i + shift (0 here, 0.5 in the _part variant), so
+// the pixel-index box is the real box moved by -shift, rounded outwards.  A
+// level set reaching the camera plane (d >= 0) projects unbounded: no box
+// then.
+SR_DEV short4 cull_box(F3 Tu, F3 Tv, F3 Tw, float mx, float my, float opa, int W, int H, float shift,
+                       float& rho_rej) {
   const short4 none = make_short4(32767, 32767, -32768, -32768);
   const short4 all = make_short4(-32768, -32768, 32767, 32767);
   if (!(opa >= 1.0f / 255.0f)) {
@@ -152,15 +158,15 @@ SR_DEV short4 cull_box(F3 Tu, F3 Tv, F3 Tw, float mx, float my, float opa, int W
   const float cy = fx * (Tv.x * Tw.x + Tv.y * Tw.y) + fz * (Tv.z * Tw.z);
   const float hx = sqrtf(fmax_(0.0f, cx * cx - (fx * (Tu.x * Tu.x + Tu.y * Tu.y) + fz * (Tu.z * Tu.z))));
   const float hy = sqrtf(fmax_(0.0f, cy * cy - (fx * (Tv.x * Tv.x + Tv.y * Tv.y) + fz * (Tv.z * Tv.z))));
-  const float mgx = 1.0f + 1e-3f * (fabsf(cx) + hx), mgy = 1.0f + 1e-3f * (fabsf(cy) + hy);
+  const float mgx = 0.05f + 1e-3f * (fabsf(cx) + hx), mgy = 0.05f + 1e-3f * (fabsf(cy) + hy);
   x0 = fmin_(x0, cx - hx - mgx);
   x1 = fmax_(x1, cx + hx + mgx);
   y0 = fmin_(y0, cy - hy - mgy);
   y1 = fmax_(y1, cy + hy + mgy);
-  x0 = floorf(x0 - 1.0f);
-  y0 = floorf(y0 - 1.0f);
-  x1 = ceilf(x1 + 1.0f);
-  y1 = ceilf(y1 + 1.0f);
+  x0 = floorf(x0 - shift);
+  y0 = floorf(y0 - shift);
+  x1 = ceilf(x1 - shift);
+  y1 = ceilf(y1 - shift);
   if (!(x0 == x0 && x1 == x1 && y0 == y0 && y1 == y1)) return all;  // NaN: no culling
   x0 = fmax_(x0, -1.0f);
   y0 = fmax_(y0, -1.0f);
@@ -441,7 +447,7 @@ SR_DEV bool preprocess_one(const PreprocessArgs& a, int v, int idx, size_t r, in
   }
   const float opa = a.opac[idx];
   float rho_rej;
-  const short4 box = cull_box(Tu, Tv, Tw, px, py, opa, W, H, rho_rej);
+  const short4 box = cull_box(Tu, Tv, Tw, px, py, opa, W, H, a.part ? 0.5f : 0.0f, rho_rej);
   a.depths[r] = vz;
   a.radii[r] = ir;
   float4* rc = a.rec + (size_t)kRecQuads * r;
@@ -1409,15 +1415,31 @@ hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Acceptance pass.  One wave per (quadrant, chunk of 64 sub-list entries),
-// one lane per pixel: bit j of the lane's mask is set when entry j of the
-// chunk reaches alpha >= 1/255 at that pixel.  The chunks of all quadrants
-// are independent, so this, the bulk of the pair evaluations, runs balanced
-// over the whole GPU; the blend kernels then visit only accepted pairs.
+// Acceptance pass.  One wave per (quadrant, chunk of 64 sub-list entries):
+// bit j of pixel p's mask is set when entry j of the chunk reaches
+// alpha >= 1/255 at p.  Only pixels inside an entry's cull box can (the box
+// holds every such pixel), so the wave evaluates exactly the (entry, pixel)
+// pairs of the boxes clipped to the quadrant, 64 pairs per step: an LDS
+// table names the entry of every pair slot, and the slot's offset in the
+// entry's box gives the pixel.  The bits go to per-pixel masks in LDS; each
+// accepted pair adds log2(1 - alpha) to its pixel's sum, from which the
+// chunk's transmittance follows.  That transmittance is only compared
+// against half the saturation threshold (a margin of a factor 2 against its
+// rounding, whatever the summation order), so the order in which the lanes
+// add is free.  The chunks of all quadrants are independent, so this, the
+// bulk of the pair evaluations, runs balanced over the whole GPU; the blend
+// kernels then visit only accepted pairs.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64];
   __shared__ float s_rej[64];
+  __shared__ uint32_t s_start[64];  // running box area before entry j
+  __shared__ uint32_t s_geo[64];    // entry j's clipped box: x0 | y0 << 4 | w << 8
+  __shared__ unsigned long long s_mask[64];
+  __shared__ float s_logT[64];          // per pixel: sum of log2(1 - alpha) over its accepted pairs
+  __shared__ uint8_t s_owner[64 * 64];  // entry of every pair slot
+  // r / w for r < 64 and 1 <= w <= 8 as (r * kDiv[w]) >> 9
+  constexpr uint32_t kDiv[9] = {0u, 512u, 256u, 171u, 128u, 103u, 86u, 74u, 64u};
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const int lane = threadIdx.x;
   if (a.counters[1] & kErrCapacity) return;
@@ -1429,38 +1451,65 @@ __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
     const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
     const int lt = tile - (int)vi.tile_base;
     const int tx = lt % vi.gx, ty = lt / vi.gx;
-    const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (lane & 7));
-    const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (lane >> 3));
-    const bool inside = pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
+    const int qx0 = tx * B + (q % QX) * kQuad, qy0 = ty * B + (q / QX) * kQuad;
+    const int qw = min(kQuad, vi.W - qx0), qh = min(kQuad, vi.H - qy0);  // pixels inside the image
     const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
     const int cnt = (int)a.sub_count[qi];
     const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
     const int e0 = c * 64;
     const int m = min(64, cnt - e0);
-    __syncthreads();
+    __syncthreads();  // the previous item's readers are done
+    uint32_t area = 0, geo = 0;
     if (lane < m) {
-      const float4* r = a.rec + (size_t)kRecQuads * list[e0 + lane].x;
+      const uint32_t rid = list[e0 + lane].x;
+      const float4* r = a.rec + (size_t)kRecQuads * rid;
       s_q0[lane] = r[0];
       s_q1[lane] = r[1];
       s_q2[lane] = r[2];
       s_rej[lane] = r[4].z;
+      const short4 bx = a.cullbox[rid];
+      const int x0 = max((int)bx.x - qx0, 0), x1 = min((int)bx.z - qx0, qw - 1);
+      const int y0 = max((int)bx.y - qy0, 0), y1 = min((int)bx.w - qy0, qh - 1);
+      if (x1 >= x0 && y1 >= y0) {
+        const uint32_t bw = (uint32_t)(x1 - x0 + 1);
+        area = bw * (uint32_t)(y1 - y0 + 1);
+        geo = (uint32_t)x0 | ((uint32_t)y0 << 4) | (bw << 8);
+      }
     }
+    uint32_t incl = area;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t v = (uint32_t)__shfl_up((int)incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);  // at most 64 boxes of 64 pixels
+    const uint32_t first = incl - area;
+    s_start[lane] = first;
+    s_geo[lane] = geo;
+    for (uint32_t i = 0; i < area; ++i) s_owner[first + i] = (uint8_t)lane;
+    s_mask[lane] = 0ull;
+    s_logT[lane] = 0.0f;
     __syncthreads();
-    uint64_t mask = 0;
-    float tc = 1.0f;  // transmittance across the chunk's accepted pairs
-    const float pixx = (float)pxi, pixy = (float)pyi;
-    if (inside) {
-      for (int j = 0; j < m; ++j) {
+    for (uint32_t base = 0; base < total; base += 64) {
+      const uint32_t t = base + (uint32_t)lane;
+      if (t < total) {
+        const int j = s_owner[t];
+        const uint32_t g = s_geo[j];
+        const uint32_t r = t - s_start[j], bw = (g >> 8) & 15u;
+        const uint32_t dy = (r * kDiv[bw]) >> 9, dx = r - dy * bw;
+        const int lx = (int)(g & 15u) + (int)dx, ly = (int)((g >> 4) & 15u) + (int)dy;
         Pair e;
-        if (eval_pair<true>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], s_rej[j], e)) {
-          mask |= 1ull << j;
-          tc *= 1.0f - e.alpha;
+        if (eval_pair<true>((float)(qx0 + lx), (float)(qy0 + ly), s_q0[j], s_q1[j], s_q2[j], s_rej[j], e)) {
+          const int p = ly * kQuad + lx;
+          atomicOr(&s_mask[p], 1ull << j);
+          atomicAdd(&s_logT[p], __log2f(1.0f - e.alpha));
         }
       }
     }
+    __syncthreads();
     const size_t slot = mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane;
-    a.masks[slot] = mask;
-    a.chunk_T[slot] = tc;
+    a.masks[slot] = s_mask[lane];
+    a.chunk_T[slot] = exp2f(s_logT[lane]);
   }
 }
 
@@ -1473,10 +1522,11 @@ hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s) {
 
 // Transmittance below which a pixel is certainly saturated.  The blend stops
 // a pixel at its first pair with T (1 - alpha) < 1e-4, T the running product
-// of (1 - alpha) over its accepted pairs in order.  The product over whole
-// chunks is the same product rounded in another order, off by at most a few
-// thousand ulp (one rounding per pair, 2^-24 each): below half the
-// threshold, the pixel stopped in an earlier chunk.
+// of (1 - alpha) over its accepted pairs in order.  The chunk transmittances
+// give the same product through sums of log2(1 - alpha): with hardware
+// log2 / exp2 (relative error near 2^-22 each) that is off by far less than a
+// factor 2 even over thousands of pairs, so below half the threshold the
+// pixel stopped in an earlier chunk.
 constexpr float kSaturatedT = 0.5e-4f;
 
 // Pair-store offsets: one wave per quadrant, one lane per pixel.  Each
@@ -1706,7 +1756,10 @@ hipError_t launch_pair_write(const AcceptArgs& a, size_t max_items, hipStream_t 
 // skips, so T, the accumulators and the contributor counts (positions in the
 // full tile list) come out bit-identical.
 // ---------------------------------------------------------------------------
-constexpr int kPre = 8;  // stored pairs a lane fetches ahead in the blend kernels
+#ifndef SR_FWD_PREFETCH
+#define SR_FWD_PREFETCH 8
+#endif
+constexpr int kPre = SR_FWD_PREFETCH;  // stored pairs a lane fetches ahead in the blend kernels
 
 __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];

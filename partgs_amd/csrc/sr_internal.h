@@ -33,7 +33,10 @@ __host__ __device__ inline int quads_per_tile(int B) { return (B / kQuad) * (B /
 // The per-pixel blend chains run 64 / kSplit pixels of a quadrant per wave:
 // a lane streams its own pairs, so spreading a heavy quadrant over more
 // waves spreads its memory traffic over more in-flight requests.
-constexpr int kSplit = 4;
+#ifndef SR_FWD_SPLIT
+#define SR_FWD_SPLIT 4
+#endif
+constexpr int kSplit = SR_FWD_SPLIT;
 
 // A batch of views rasterised together.  Every view has its own camera and
 // image size; all share the surfels and the tile edge B.  Surfel i seen from
@@ -312,6 +315,7 @@ struct AcceptArgs {
   const uint2* sub;
   const uint32_t* sub_count;
   const float4* rec;
+  const short4* cullbox;  // per record: inclusive pixel box holding every pixel with alpha >= 1/255
   const uint2* work;
   const uint32_t* work_count;
   uint64_t* masks;
