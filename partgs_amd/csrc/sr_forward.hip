@@ -1681,12 +1681,37 @@ hipError_t launch_pair_offsets(const AcceptArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// The k-th set bit (k < popcount) of a 64-bit mask, by halving.
+SR_DEV int select_bit(uint64_t m, uint32_t k) {
+  int at = 0;
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) {
+    const uint64_t low = m & ((1ull << w) - 1ull);
+    const uint32_t n = (uint32_t)__popcll(low);
+    if (k >= n) {
+      k -= n;
+      m >>= w;
+      at += w;
+    } else {
+      m = low;
+    }
+  }
+  return at;
+}
+
 // Pair-store fill: balanced like the acceptance pass, one wave per
-// (quadrant, chunk); each lane recomputes its accepted pairs of the chunk and
-// writes them at its offsets, with the entry data the blend chains need.
+// (quadrant, chunk).  The chunk's accepted pairs are dealt out 64 per step
+// in pixel order (an LDS table names the pixel of every pair slot; the
+// slot's rank among the pixel's pairs picks the entry from the pixel's mask
+// bits), so consecutive lanes write consecutive records of a pixel's run.
+// Each lane recomputes its pair and stores the entry data the blend chains
+// need.
 __global__ __launch_bounds__(64) void k_pair_write(AcceptArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
   __shared__ uint32_t s_pos[64];
+  __shared__ unsigned long long s_mask[64];
+  __shared__ uint32_t s_start[64], s_foff[64];
+  __shared__ uint8_t s_owner[64 * 64];  // pixel of every pair slot
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const int lane = threadIdx.x;
   if (a.counters[1] & kErrCapacity) return;
@@ -1701,17 +1726,31 @@ __global__ __launch_bounds__(64) void k_pair_write(AcceptArgs a) {
     const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
     const int lt = tile - (int)vi.tile_base;
     const int tx = lt % vi.gx, ty = lt / vi.gx;
-    const float pixx = (float)(tx * B + (q % QX) * kQuad + (lane & 7));
-    const float pixy = (float)(ty * B + (q / QX) * kQuad + (lane >> 3));
+    const int qx0 = tx * B + (q % QX) * kQuad, qy0 = ty * B + (q / QX) * kQuad;
     const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
     const int cnt = (int)a.sub_count[qi];
     const uint2* list = a.sub + (size_t)Q * rbeg + (size_t)q * ntile;
     const int e0 = c * 64;
     const int m = min(64, cnt - e0);
+    // lane as pixel: its accepted pairs of the chunk and where they go
     const size_t slot = mask_base(Q, rbeg, ntile, tile, q) + (size_t)e0 + lane;
-    uint64_t mm = a.masks[slot];
-    uint32_t r = mm ? a.frag_off[slot] : 0u;
-    __syncthreads();
+    const uint64_t mm = a.masks[slot];
+    const uint32_t foff = mm ? a.frag_off[slot] : 0u;
+    const uint32_t n = (uint32_t)__popcll(mm);
+    uint32_t incl = n;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t v = (uint32_t)__shfl_up((int)incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+    if (total == 0u) continue;  // uniform across the wave
+    __syncthreads();  // the previous item's readers are done
+    const uint32_t first = incl - n;
+    s_mask[lane] = mm;
+    s_start[lane] = first;
+    s_foff[lane] = foff;
+    for (uint32_t i = 0; i < n; ++i) s_owner[first + i] = (uint8_t)lane;
     if (lane < m) {
       const uint2 v = list[e0 + lane];
       const float4* rc = a.rec + (size_t)kRecQuads * v.x;
@@ -1723,20 +1762,22 @@ __global__ __launch_bounds__(64) void k_pair_write(AcceptArgs a) {
       s_pos[lane] = v.y;
     }
     __syncthreads();
-    while (mm != 0ull) {
-      const int j = __builtin_ctzll(mm);
-      mm &= mm - 1ull;
+    for (uint32_t base = 0; base < total; base += 64) {
+      const uint32_t t = base + (uint32_t)lane;
+      if (t >= total) continue;
+      const int p = s_owner[t];
+      const uint32_t k = t - s_start[p];
+      const int j = select_bit(s_mask[p], k);
       Pair e;
-      eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e);
+      eval_pair<false>((float)(qx0 + (p & 7)), (float)(qy0 + (p >> 3)), s_q0[j], s_q1[j], s_q2[j], 0.0f, e);
       const float4 q3 = s_q3[j], q4 = s_q4[j];
       // the distortion weight m(depth) and its derivative, as the blends form them
       const float m_d = mcoef * (1.0f - kNear / e.depth);
       const float dmd_dd = dmd_num / ((dmd_den0 * e.depth) * e.depth);
-      float4* pr = a.ps.rec + kPairQuads * (size_t)r;
+      float4* pr = a.ps.rec + kPairQuads * (size_t)(s_foff[p] + k);
       pr[0] = make_float4(e.alpha, e.depth, m_d, dmd_dd);
       pr[1] = q3;
       pr[2] = make_float4(q4.x, q4.y, __uint_as_float(s_pos[j]), 0.0f);
-      ++r;
     }
   }
 }
