@@ -1802,6 +1802,123 @@ hipError_t launch_pair_write(const AcceptArgs& a, size_t max_items, hipStream_t 
 #endif
 constexpr int kPre = SR_FWD_PREFETCH;  // stored pairs a lane fetches ahead in the blend kernels
 
+// Stored path of the forward blend: every pair of the quadrant is in the
+// store and each pixel's pairs form one contiguous run in entry order.  A
+// wave takes 16 pixels with 4 lanes each; the 4 lanes of a pixel walk the
+// same run in step (each runs the transmittance update, so all agree on
+// where the pixel stops) and split the accumulators: role 0 depth, M1, M2
+// and distortion, role 1 the normal and red, role 2 green and blue, role 3
+// the median and contributor counts.  Every lane executes the same
+// instructions (role values are picked by select), so the serial chain of a
+// long pixel costs a quarter of the full update per pair; each accumulator
+// sees exactly the reference's operations in order.
+SR_DEV void blend_fwd_stored(const BlendArgs& a, const ViewInfo& vi, int qi, int tx, int ty, int q, int part) {
+  static_assert(kSplit == 4, "4 waves of 16 pixels x 4 roles per quadrant");
+  const int B = a.vs.B, QX = B / kQuad;
+  const int lane = threadIdx.x, role = lane & 3;
+  const int pix = part * 16 + (lane >> 2);
+  const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (pix & 7));
+  const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (pix >> 3));
+  const bool inside = pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
+  const int cnt = (int)a.sub_count[qi];
+  // lane_range is written by the pair-offset pass, which runs only when the
+  // view has instances: read it for non-empty quadrants only
+  const uint2 rg = (inside && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
+  const int sel = role == 1 ? 1 : 2;  // the record quad with this role's values
+  float T = 1.0f;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, distortion = 0.0f, median_depth = 0.0f;
+  float median_contributor = -1.0f;
+  uint32_t last_contributor = 0;
+  int blended = 0;
+  bool done = !inside;
+  uint32_t f = rg.x;
+  int left = (int)rg.y;
+  float4 f0[kPre], f1[kPre];
+  auto fetch = [&](uint32_t at, int n) {
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+      if (k < n) {
+        const float4* pr = a.ps.rec + kPairQuads * (size_t)(at + k);
+        f0[k] = pr[0];
+        f1[k] = pr[sel];
+      }
+    }
+  };
+  fetch(f, left);
+  while (left > 0 && !done) {
+    float4 c0[kPre], c1[kPre];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+      c0[k] = f0[k];
+      c1[k] = f1[k];
+    }
+    const int n = left;
+    f += kPre;
+    left -= kPre;
+    if (left > 0) fetch(f, left);  // next batch in flight
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+      if (k < n && !done) {
+        // forward.cu:383-415: (alpha, depth, mm) in quad 0; mm = mcoef (1 - near / depth)
+        const float alpha = c0[k].x, depth = c0[k].y, mm = c0[k].z;
+        const float4 v = c1[k];
+        const float test_T = T * (1.0f - alpha);
+        if (test_T < 0.0001f) {
+          done = true;
+        } else {
+          const uint32_t contributor = __float_as_uint(v.z) + 1u;  // quad 2's z: position in the tile list
+          const float w = alpha * T;
+          const float A = 1.0f - T;
+          const float mm2 = mm * mm;
+          distortion += ((mm2 * A + a2) - (2.0f * mm) * a1) * w;  // role 0: a1 = M1, a2 = M2
+          const float x0 = role == 0 ? depth : v.x;
+          const float x1 = role == 0 ? mm : v.y;
+          const float x2 = role == 0 ? mm2 : v.z;
+          a0 += x0 * w;
+          a1 += x1 * w;
+          a2 += x2 * w;
+          a3 += v.w * w;
+          if (T > 0.5f) {
+            median_depth = depth;
+            median_contributor = (float)contributor;
+          }
+          T = test_T;
+          last_contributor = contributor;
+          ++blended;
+        }
+      }
+    }
+  }
+  if (!inside) return;
+  const size_t HW = (size_t)vi.H * vi.W, pid = (size_t)vi.W * pyi + pxi, pb = vi.pix_base;
+  float* final_T = a.final_T + 3 * pb;
+  float* out_color = a.out_color + 3 * pb;
+  float* out_others = a.out_others + 7 * pb;
+  if (role == 0) {
+    out_others[pid] = a0;
+    final_T[pid + HW] = a1;
+    final_T[pid + 2 * HW] = a2;
+    out_others[pid + 6 * HW] = distortion;
+  } else if (role == 1) {
+    out_others[pid + 2 * HW] = a0;
+    out_others[pid + 3 * HW] = a1;
+    out_others[pid + 4 * HW] = a2;
+    out_color[pid] = a3 + T * a.bg[0];
+  } else if (role == 2) {
+    out_color[HW + pid] = a0 + T * a.bg[1];
+    out_color[2 * HW + pid] = a1 + T * a.bg[2];
+  } else {
+    uint32_t* n_contrib = a.n_contrib + 2 * pb;
+    final_T[pid] = T;
+    n_contrib[pid] = last_contributor;
+    n_contrib[pid + HW] = f2u(median_contributor);
+    out_others[pid + HW] = 1.0f - T;
+    out_others[pid + 5 * HW] = median_depth;
+    // the backward passes walk only the blended prefix of the run
+    if (cnt > 0) a.lane_range[(size_t)qi * 64 + pix].y = (uint32_t)blended;
+  }
+}
+
 __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   __shared__ float4 s_q0[64], s_q1[64], s_q2[64], s_q3[64], s_q4[64];
   __shared__ uint32_t s_pos[64];
@@ -1815,9 +1932,13 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   const int lane = threadIdx.x;
   const bool staged = a.q_flag[qi] != 0u;
   if (staged && part != 0) return;  // the staged path uses all 64 lanes of part 0
-  // pixel of this lane: all 64 in the staged path, 64 / kSplit per part otherwise
-  const int pix = staged ? lane : part * (64 / kSplit) + lane;
-  const bool lane_ok = staged || lane < 64 / kSplit;
+  if (!staged) {
+    blend_fwd_stored(a, vi, qi, tx, ty, q, part);
+    return;
+  }
+  // staged path: lane = pixel of the quadrant
+  const int pix = lane;
+  const bool lane_ok = true;
   const uint32_t pxi = (uint32_t)(tx * B + (q % QX) * kQuad + (pix & 7));
   const uint32_t pyi = (uint32_t)(ty * B + (q / QX) * kQuad + (pix >> 3));
   const bool inside = lane_ok && pxi < (uint32_t)vi.W && pyi < (uint32_t)vi.H;
@@ -1861,55 +1982,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
     last_contributor = contributor;
     return true;
   };
-  if (!staged) {
-    // Every pair of the quadrant is in the store, which holds all a step
-    // reads, and each pixel's pairs form one contiguous run in entry order:
-    // every lane streams its own run, the next kPre pairs in flight while
-    // the current ones blend, with no staging and no barriers.
-    // lane_range is written by the pair-offset pass, which runs only when the
-    // view has instances: read it for non-empty quadrants only
-    const uint2 rg = (inside && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
-    uint32_t f = rg.x;
-    int left = (int)rg.y;
-    float4 f0[kPre], f1[kPre], f2[kPre];
-    auto fetch = [&](uint32_t at, int n) {
-#pragma unroll
-      for (int k = 0; k < kPre; ++k) {
-        if (k < n) {
-          const float4* pr = a.ps.rec + kPairQuads * (size_t)(at + k);
-          f0[k] = pr[0];
-          f1[k] = pr[1];
-          f2[k] = pr[2];
-        }
-      }
-    };
-    fetch(f, left);
-    int blended = 0;
-    while (left > 0 && !done) {
-      float4 c0[kPre], c1[kPre], c2[kPre];
-#pragma unroll
-      for (int k = 0; k < kPre; ++k) {
-        c0[k] = f0[k];
-        c1[k] = f1[k];
-        c2[k] = f2[k];
-      }
-      const int n = left;
-      f += kPre;
-      left -= kPre;
-      if (left > 0) fetch(f, left);  // next batch in flight
-#pragma unroll
-      for (int k = 0; k < kPre; ++k) {
-        if (k < n && !done) {
-          if (step(c0[k].x, c0[k].y, c0[k].z, __float_as_uint(c2[k].z), c1[k], c2[k].x, c2[k].y))
-            ++blended;
-          else
-            done = true;
-        }
-      }
-    }
-    // the backward passes walk only the blended prefix of the run
-    if (inside && cnt > 0) a.lane_range[(size_t)qi * 64 + pix].y = (uint32_t)blended;
-  } else {
+  {
     // Some chunk missed the store: stage the records chunk by chunk and
     // recompute the pairs of chunks without stored pairs.
     ChunkRegs nx{};
