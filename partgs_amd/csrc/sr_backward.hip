@@ -318,13 +318,18 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
     if (sg == 0 && nseg > 1) {
       ChainBwd w0;
       w0.T = px.T_final;
+      // segment starts from the back: multiples of seg_len, counted down
+      // (no integer division on the serial walk)
+      int boundary = ((n - 1) / seg_len) * seg_len, slot = boundary / seg_len - 1;
       walk(seg_len, n - 1, [&](int k, const float4& c0, const float4& c1, const float4& c2) {
         w0.advance(px, c0.x, c0.y, c0.z, c1, c2.x, c2.y);
-        if (k % seg_len == 0) {
+        if (k == boundary) {
           const float* src = reinterpret_cast<const float*>(&w0);
-          float* dst = st_out + (k / seg_len - 1) * 18;
+          float* dst = st_out + slot * 18;
 #pragma unroll
           for (int i = 0; i < 18; ++i) dst[i] = src[i];
+          boundary -= seg_len;
+          --slot;
         }
       });
     }
