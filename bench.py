@@ -74,6 +74,8 @@ def cpu_baseline(scene, gc, go, budget_s=20.0):
 
     from oracle import oracle as O
 
+    # the port compiled for this host; the shipped portable build otherwise
+    native = O.use_native_build()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     O.set_threads(threads)
     cam = scene.camera
@@ -92,7 +94,10 @@ def cpu_baseline(scene, gc, go, budget_s=20.0):
         times.append((time.perf_counter() - t0) * 1e3)
         print(f"[bench] cpu oracle fwd+bwd run {len(times)}: {times[-1]:.1f} ms", file=sys.stderr, flush=True)
     return {"value": statistics.median(times), "unit": "ms/view", "cores": O.get_threads(), "kind": "port",
-            "sample": f"{len(times)} full fwd+bwd views of the same 300k-surfel 800x600 workload (median)"}
+            "sample": f"{len(times)} full fwd+bwd views of the same 300k-surfel 800x600 workload (median)",
+            "build": "g++ -O3 -march=native -fopenmp" if native else "g++ -O3 -mavx2 -mfma -fopenmp",
+            "note": "the C++ restatement of the reference's per-pixel logic: OpenMP over tiles, per-thread "
+                    "FP64 gradient buffers, per-tile parallel sort; a reported baseline, not the target"}
 
 
 def run_batched(args):
@@ -144,6 +149,48 @@ def run_batched(args):
     print(json.dumps(out))
 
 
+def run_kernels(args):
+    """Event timings of distCUDA2 (simple-knn) on superquadric point clouds of
+    100k / 300k / 1M points, and of the superquadric signed distance of 16
+    blocks (forward, forward + adjoint) at the overlap loss's 16 x 2048 points
+    and the ray losses' 2048 x 128 samples."""
+    import numpy as np
+    import torch
+
+    from partgs_amd import block_losses as BL
+    from partgs_amd.knn import distCUDA2
+    from partgs_amd.scenes import superquadric_surfels
+
+    dev = torch.device("cuda:0")
+
+    def ev_ms(fn, reps=20):
+        fn()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        return round(s.elapsed_time(e) / reps, 4)
+
+    out = {"metric": "kernel timings (ms per call)", "n_gpus": 1, "data": "synthetic (seeded)"}
+    for P in (100_000, 300_000, 1_000_000):
+        pts = superquadric_surfels(0, P=P).means3D.to(dev)
+        out[f"distCUDA2_{P}"] = ev_ms(lambda: distCUDA2(pts))
+    rng = np.random.default_rng(0)
+    prm = np.concatenate([np.log(rng.uniform(0.1, 0.5, (16, 3))), rng.normal(size=(16, 4)),
+                          rng.normal(size=(16, 3)) * 0.3, rng.normal(size=(16, 2))], 1)
+    params = torch.tensor(prm.astype(np.float32), device=dev)
+    for N in (16 * 2048, 2048 * 128):
+        pts = torch.rand(N, 3, device=dev) * 2 - 1
+        out[f"sdf_fwd_16x{N}"] = ev_ms(lambda: BL.superquadric_sdf(params, pts))
+        P = params.clone().requires_grad_(True)
+        g = torch.randn(16, N, device=dev)
+        out[f"sdf_fwd_bwd_16x{N}"] = ev_ms(lambda: torch.autograd.backward(BL.superquadric_sdf(P, pts), g))
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +201,8 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="32 ragged views of 1,000,000 surfels in one graph-captured batch instead")
     ap.add_argument("--autotune", action="store_true", help="with --batched: time tile / pair-store choices first")
+    ap.add_argument("--kernels", action="store_true",
+                    help="time the simple-knn and superquadric-distance kernels instead (one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 10 copies of a 1 GiB buffer (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -177,6 +226,9 @@ def main():
 
     if args.batched:
         run_batched(args)
+        return
+    if args.kernels:
+        run_kernels(args)
         return
 
     from partgs_amd import _lib

@@ -36,45 +36,72 @@ def build() -> str:
 _lib = None
 
 
+def use_native_build() -> bool:
+    """Compiles the oracle for the host it runs on (-march=native, the
+    Makefile's flags otherwise) into a temporary directory and makes it the
+    library this module calls; False (and the portable build stays) when no
+    compiler is at hand.  For timing the CPU baseline on the machine that
+    runs it; the shipped build targets any AVX2 host."""
+    global _lib
+    import shutil
+    import tempfile
+
+    cxx = shutil.which("g++")
+    if not cxx:
+        return False
+    d = tempfile.mkdtemp(prefix="sr_oracle_native_")
+    out = os.path.join(d, "libsr_oracle_native.so")
+    cmd = [cxx, "-O3", "-std=c++17", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+           "-shared", "-o", out, os.path.join(HERE, "sr_oracle.cpp")]
+    if subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL).returncode != 0:
+        return False
+    _lib = _bind(C.CDLL(out))
+    return True
+
+
 def load():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        lib = C.CDLL(LIB_PATH)
-        lib.sro_create.restype = C.c_void_p
-        lib.sro_destroy.argtypes = [C.c_void_p]
-        lib.sro_forward.argtypes = [C.c_void_p, C.POINTER(SroScene), C.c_int]
-        lib.sro_forward.restype = C.c_int
-        lib.sro_num_rendered.argtypes = [C.c_void_p]
-        lib.sro_num_rendered.restype = C.c_int
-        lib.sro_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
-        lib.sro_backward.restype = C.c_int
-        lib.sro_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]
-        lib.sro_get.restype = C.c_int
-        lib.sro_expf.argtypes = [C.c_float]
-        lib.sro_expf.restype = C.c_float
-        lib.sro_expf_array.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
-        lib.sro_sh_rgb.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
-                                   C.c_void_p, C.c_int]
-        lib.sro_higher_msb.argtypes = [C.c_uint32]
-        lib.sro_higher_msb.restype = C.c_uint32
-        lib.sro_set_threads.argtypes = [C.c_int]
-        lib.sro_get_threads.restype = C.c_int
-        lib.sro_mark_visible.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
-        lib.sro_forward_record.argtypes = [C.c_void_p, C.POINTER(SroScene), C.c_int]
-        lib.sro_forward_record.restype = C.c_int
-        lib.sro_replay.argtypes = [C.c_void_p, C.POINTER(SroScene)]
-        lib.sro_replay.restype = C.c_int
-        lib.sro_set_fp64_degenerate.argtypes = [C.c_int]
-        lib.sro_forward_part.argtypes = [C.c_void_p, C.POINTER(SroScene), C.c_int, C.c_int, C.c_void_p]
-        lib.sro_forward_part.restype = C.c_int
-        lib.sro_backward_part.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-        lib.sro_backward_part.restype = C.c_int
-        lib.sro_alpha_capped.argtypes = [C.c_void_p]
-        lib.sro_alpha_capped.restype = C.c_int
-        _lib = lib
+        _lib = _bind(C.CDLL(LIB_PATH))
     return _lib
+
+
+def _bind(lib):
+    """Argument and result types of the oracle's C functions."""
+    lib.sro_create.restype = C.c_void_p
+    lib.sro_destroy.argtypes = [C.c_void_p]
+    lib.sro_forward.argtypes = [C.c_void_p, C.POINTER(SroScene), C.c_int]
+    lib.sro_forward.restype = C.c_int
+    lib.sro_num_rendered.argtypes = [C.c_void_p]
+    lib.sro_num_rendered.restype = C.c_int
+    lib.sro_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.sro_backward.restype = C.c_int
+    lib.sro_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]
+    lib.sro_get.restype = C.c_int
+    lib.sro_expf.argtypes = [C.c_float]
+    lib.sro_expf.restype = C.c_float
+    lib.sro_expf_array.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    lib.sro_sh_rgb.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                               C.c_void_p, C.c_int]
+    lib.sro_higher_msb.argtypes = [C.c_uint32]
+    lib.sro_higher_msb.restype = C.c_uint32
+    lib.sro_set_threads.argtypes = [C.c_int]
+    lib.sro_get_threads.restype = C.c_int
+    lib.sro_mark_visible.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.sro_forward_record.argtypes = [C.c_void_p, C.POINTER(SroScene), C.c_int]
+    lib.sro_forward_record.restype = C.c_int
+    lib.sro_replay.argtypes = [C.c_void_p, C.POINTER(SroScene)]
+    lib.sro_replay.restype = C.c_int
+    lib.sro_set_fp64_degenerate.argtypes = [C.c_int]
+    lib.sro_forward_part.argtypes = [C.c_void_p, C.POINTER(SroScene), C.c_int, C.c_int, C.c_void_p]
+    lib.sro_forward_part.restype = C.c_int
+    lib.sro_backward_part.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.sro_backward_part.restype = C.c_int
+    lib.sro_alpha_capped.argtypes = [C.c_void_p]
+    lib.sro_alpha_capped.restype = C.c_int
+    return lib
 
 
 def set_threads(n: int) -> None:

@@ -143,7 +143,7 @@ template <typename R> struct Ctx {
   int gx = 0, gy = 0, T = 0, N = 0;
   std::vector<R> means3D, shs, colors, opac, scales, rots, tprec, vm, pm, campos, bg;
   std::vector<R> sem, out_sem, g_sem;  // _part: semantics (P x S), image (S x H x W), gradient
-  std::vector<double> acc_sem;
+  double* acc_sem = nullptr;
   // geometry state (dsr/cuda_rasterizer/rasterizer_impl.h:29-44)
   std::vector<int> radii;
   std::vector<R> depths, means2D, transMat, normal_opacity, rgb;
@@ -159,7 +159,13 @@ template <typename R> struct Ctx {
   std::vector<R> out_color, out_others, final_T;
   std::vector<uint32_t> n_contrib;
   // backward: render-stage sums in FP64, then final grads in R
-  std::vector<double> acc_transMat, acc_mean2D, acc_normal, acc_opacity, acc_colors;
+  // backward accumulators in FP64: one copy per OpenMP thread in acc_pool
+  // (copy t at t * acc_n), the pointers naming copy 0, into which the other
+  // copies are summed in thread order after the parallel pass
+  double *acc_transMat = nullptr, *acc_mean2D = nullptr, *acc_normal = nullptr, *acc_opacity = nullptr,
+         *acc_colors = nullptr;
+  std::vector<double> acc_pool;
+  size_t acc_n = 0;
   std::vector<R> g_mean2D, g_colors, g_opacity, g_means3D, g_transMat, g_sh, g_scales, g_rots, g_normal;
   // Frozen decisions (finite-difference support): a recorded forward keeps
   // every discrete choice it made; a replay re-evaluates the same
@@ -510,10 +516,19 @@ template <typename R> void binning(Ctx<R>& c) {
   }
   const uint32_t bit = higher_msb((uint32_t)(c.gx * c.gy));
   const uint64_t mask = (32 + bit >= 64) ? ~0ull : ((1ull << (32 + bit)) - 1ull);
-  std::vector<uint32_t> order(c.L);
-  for (int i = 0; i < c.L; ++i) order[i] = (uint32_t)i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint32_t a, uint32_t b) { return (ku[a] & mask) < (ku[b] & mask); });
+  // the stable sort by (tile, depth bits): a stable counting pass by tile,
+  // then each tile's run stably by depth bits, tiles in parallel
+  std::vector<uint32_t> order(c.L), tstart((size_t)c.T + 1, 0u);
+  for (int i = 0; i < c.L; ++i) ++tstart[(size_t)((ku[i] & mask) >> 32) + 1];
+  for (int t = 0; t < c.T; ++t) tstart[t + 1] += tstart[t];
+  {
+    std::vector<uint32_t> cur(tstart.begin(), tstart.end() - 1);
+    for (int i = 0; i < c.L; ++i) order[cur[(size_t)((ku[i] & mask) >> 32)]++] = (uint32_t)i;
+  }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int t = 0; t < c.T; ++t)
+    std::stable_sort(order.begin() + tstart[t], order.begin() + tstart[t + 1],
+                     [&](uint32_t a, uint32_t b) { return (ku[a] & mask) < (ku[b] & mask); });
   c.keys.resize(c.L);
   c.point_list.resize(c.L);
   for (int i = 0; i < c.L; ++i) {
@@ -685,9 +700,10 @@ template <typename R> void run_replay(Ctx<R>& c) {
 // FAR_PLANE; dsp/cuda_rasterizer/auxiliary.h:19-36), so the expressions
 // that use them are evaluated in double and rounded, as C++ promotion does.
 // ===========================================================================
-inline void atomic_addd(double* p, double v) {
-#pragma omp atomic
-  *p += v;
+// Adds v to the calling thread's copy of accumulator element p (p points
+// into copy 0; copy t sits t * acc_n doubles further).
+template <typename Cx> inline void acc_add(Cx& c, double* p, double v) {
+  p[(size_t)omp_get_thread_num() * c.acc_n] += v;
 }
 
 const double kP_FS = 0.7071067811865476;  // FilterSize
@@ -913,12 +929,12 @@ void render_bwd_pixel_part(Ctx<R>& c, int tile, int pxi, int pyi, const R* dL_dp
       accum_rec[ch] = last_alpha * last_color[ch] + ((R)1 - last_alpha) * accum_rec[ch];
       last_color[ch] = col;
       dL_dalpha += (col - accum_rec[ch]) * dL_dpixel[ch];
-      atomic_addd(&c.acc_colors[3 * id + ch], (double)(dchannel_dcolor * dL_dpixel[ch]));
+      acc_add(c, &c.acc_colors[3 * id + ch], (double)(dchannel_dcolor * dL_dpixel[ch]));
     }
     // semantics: their gradient only (the alpha term is commented out in
     // the reference, backward.cu:346)
     for (int ch = 0; ch < S; ++ch)
-      atomic_addd(&c.acc_sem[(size_t)id * S + ch], (double)(dchannel_dcolor * dL_dpixel_sem[ch]));
+      acc_add(c, &c.acc_sem[(size_t)id * S + ch], (double)(dchannel_dcolor * dL_dpixel_sem[ch]));
     R dL_dz = 0, dL_dweight = 0;
     const R m_d = part_mapped(c_d);
     const R dmd_dd = (R)((kP_FAR * kP_NEAR) / (((kP_FAR - kP_NEAR) * (double)c_d) * (double)c_d));
@@ -941,7 +957,7 @@ void render_bwd_pixel_part(Ctx<R>& c, int tile, int pxi, int pyi, const R* dL_dp
       accum_normal_rec[ch] = last_alpha * last_normal[ch] + ((R)1 - last_alpha) * accum_normal_rec[ch];
       last_normal[ch] = no[ch];
       dL_dalpha += (no[ch] - accum_normal_rec[ch]) * dL_dnormal2D[ch];
-      atomic_addd(&c.acc_normal[3 * id + ch], (double)((alpha * T) * dL_dnormal2D[ch]));
+      acc_add(c, &c.acc_normal[3 * id + ch], (double)((alpha * T) * dL_dnormal2D[ch]));
     }
     dL_dalpha *= T;
     last_alpha = alpha;
@@ -959,24 +975,24 @@ void render_bwd_pixel_part(Ctx<R>& c, int tile, int pxi, int pyi, const R* dL_dp
       const V3<R> dL_dk = cross(q.l, dL_dp);
       const V3<R> dL_dl = cross(dL_dp, q.k);
       double* g = &c.acc_transMat[9 * id];
-      atomic_addd(&g[0], (double)(-dL_dk.x));
-      atomic_addd(&g[1], (double)(-dL_dk.y));
-      atomic_addd(&g[2], (double)(-dL_dk.z));
-      atomic_addd(&g[3], (double)(-dL_dl.x));
-      atomic_addd(&g[4], (double)(-dL_dl.y));
-      atomic_addd(&g[5], (double)(-dL_dl.z));
-      atomic_addd(&g[6], (double)((pixx * dL_dk.x + pixy * dL_dl.x) + dL_dz * q.sx));
-      atomic_addd(&g[7], (double)((pixx * dL_dk.y + pixy * dL_dl.y) + dL_dz * q.sy));
-      atomic_addd(&g[8], (double)((pixx * dL_dk.z + pixy * dL_dl.z) + dL_dz * (R)1));
+      acc_add(c, &g[0], (double)(-dL_dk.x));
+      acc_add(c, &g[1], (double)(-dL_dk.y));
+      acc_add(c, &g[2], (double)(-dL_dk.z));
+      acc_add(c, &g[3], (double)(-dL_dl.x));
+      acc_add(c, &g[4], (double)(-dL_dl.y));
+      acc_add(c, &g[5], (double)(-dL_dl.z));
+      acc_add(c, &g[6], (double)((pixx * dL_dk.x + pixy * dL_dl.x) + dL_dz * q.sx));
+      acc_add(c, &g[7], (double)((pixx * dL_dk.y + pixy * dL_dl.y) + dL_dz * q.sy));
+      acc_add(c, &g[8], (double)((pixx * dL_dk.z + pixy * dL_dl.z) + dL_dz * (R)1));
     } else {
       // -G * FilterInvSquare * d.x: ((-G * 1) / (FS * FS)) * d.x in double
       const R dG_ddelx = (R)(((double)(-G) / (kP_FS * kP_FS)) * (double)q.dx);
       const R dG_ddely = (R)(((double)(-G) / (kP_FS * kP_FS)) * (double)q.dy);
-      atomic_addd(&c.acc_mean2D[3 * id], (double)(dL_dG * dG_ddelx));
-      atomic_addd(&c.acc_mean2D[3 * id + 1], (double)(dL_dG * dG_ddely));
-      atomic_addd(&c.acc_transMat[9 * id + 8], (double)dL_dz);
+      acc_add(c, &c.acc_mean2D[3 * id], (double)(dL_dG * dG_ddelx));
+      acc_add(c, &c.acc_mean2D[3 * id + 1], (double)(dL_dG * dG_ddely));
+      acc_add(c, &c.acc_transMat[9 * id + 8], (double)dL_dz);
     }
-    atomic_addd(&c.acc_opacity[id], (double)(G * dL_dalpha));
+    acc_add(c, &c.acc_opacity[id], (double)(G * dL_dalpha));
   }
 }
 
@@ -1185,7 +1201,7 @@ void render_bwd_pixel(Ctx<R>& c, int tile, int pxi, int pyi, const R* dL_dpix, c
       accum_rec[ch] = last_alpha * last_color[ch] + ((R)1 - last_alpha) * accum_rec[ch];
       last_color[ch] = col;
       dL_dalpha += (col - accum_rec[ch]) * dL_dpixel[ch];
-      atomic_addd(&c.acc_colors[3 * id + ch], (double)(w * dL_dpixel[ch]));
+      acc_add(c, &c.acc_colors[3 * id + ch], (double)(w * dL_dpixel[ch]));
     }
     R dL_dz = 0, dL_dweight = 0;
     const R m_d = mcoef * ((R)1 - near_n / c_d);
@@ -1205,7 +1221,7 @@ void render_bwd_pixel(Ctx<R>& c, int tile, int pxi, int pyi, const R* dL_dpix, c
       accum_normal_rec[ch] = last_alpha * last_normal[ch] + ((R)1 - last_alpha) * accum_normal_rec[ch];
       last_normal[ch] = no[ch];
       dL_dalpha += (no[ch] - accum_normal_rec[ch]) * dL_dnormal2D[ch];
-      atomic_addd(&c.acc_normal[3 * id + ch], (double)((alpha * T) * dL_dnormal2D[ch]));
+      acc_add(c, &c.acc_normal[3 * id + ch], (double)((alpha * T) * dL_dnormal2D[ch]));
     }
     dL_dalpha *= T;
     last_alpha = alpha;
@@ -1220,23 +1236,23 @@ void render_bwd_pixel(Ctx<R>& c, int tile, int pxi, int pyi, const R* dL_dpix, c
       const V3<R> dL_dk = cross(l, dL_dp);
       const V3<R> dL_dl = cross(dL_dp, k);
       double* g = &c.acc_transMat[9 * id];
-      atomic_addd(&g[0], (double)(-dL_dk.x));
-      atomic_addd(&g[1], (double)(-dL_dk.y));
-      atomic_addd(&g[2], (double)(-dL_dk.z));
-      atomic_addd(&g[3], (double)(-dL_dl.x));
-      atomic_addd(&g[4], (double)(-dL_dl.y));
-      atomic_addd(&g[5], (double)(-dL_dl.z));
-      atomic_addd(&g[6], (double)((pixx * dL_dk.x + pixy * dL_dl.x) + dL_dz * sx));
-      atomic_addd(&g[7], (double)((pixx * dL_dk.y + pixy * dL_dl.y) + dL_dz * sy));
-      atomic_addd(&g[8], (double)((pixx * dL_dk.z + pixy * dL_dl.z) + dL_dz * (R)1));
+      acc_add(c, &g[0], (double)(-dL_dk.x));
+      acc_add(c, &g[1], (double)(-dL_dk.y));
+      acc_add(c, &g[2], (double)(-dL_dk.z));
+      acc_add(c, &g[3], (double)(-dL_dl.x));
+      acc_add(c, &g[4], (double)(-dL_dl.y));
+      acc_add(c, &g[5], (double)(-dL_dl.z));
+      acc_add(c, &g[6], (double)((pixx * dL_dk.x + pixy * dL_dl.x) + dL_dz * sx));
+      acc_add(c, &g[7], (double)((pixx * dL_dk.y + pixy * dL_dl.y) + dL_dz * sy));
+      acc_add(c, &g[8], (double)((pixx * dL_dk.z + pixy * dL_dl.z) + dL_dz * (R)1));
     } else {
       const R dG_ddelx = (-G * (R)2) * dx;
       const R dG_ddely = (-G * (R)2) * dy;
-      atomic_addd(&c.acc_mean2D[3 * id], (double)(dL_dG * dG_ddelx));
-      atomic_addd(&c.acc_mean2D[3 * id + 1], (double)(dL_dG * dG_ddely));
-      atomic_addd(&c.acc_transMat[9 * id + 8], (double)dL_dz);
+      acc_add(c, &c.acc_mean2D[3 * id], (double)(dL_dG * dG_ddelx));
+      acc_add(c, &c.acc_mean2D[3 * id + 1], (double)(dL_dG * dG_ddely));
+      acc_add(c, &c.acc_transMat[9 * id + 8], (double)dL_dz);
     }
-    atomic_addd(&c.acc_opacity[id], (double)(G * dL_dalpha));
+    acc_add(c, &c.acc_opacity[id], (double)(G * dL_dalpha));
   }
 }
 
@@ -1461,12 +1477,18 @@ template <typename R>
 void run_backward(Ctx<R>& c, const R* dL_dpix, const R* dL_dothers, const R* dL_dsem = nullptr) {
   const Scene& s = c.sc;
   const int P = s.P;
-  c.acc_sem.assign((size_t)P * s.S, 0.0);
-  c.acc_transMat.assign(9 * (size_t)P, 0.0);
-  c.acc_mean2D.assign(3 * (size_t)P, 0.0);
-  c.acc_normal.assign(3 * (size_t)P, 0.0);
-  c.acc_opacity.assign((size_t)P, 0.0);
-  c.acc_colors.assign(3 * (size_t)P, 0.0);
+  // per-thread FP64 accumulators, no atomics: transMat 9, mean2D 3, normal 3,
+  // opacity 1, colours 3 and the semantic channels per surfel
+  const int nt = omp_get_max_threads();
+  c.acc_n = (size_t)P * (19 + (size_t)s.S);
+  c.acc_pool.assign(c.acc_n * (size_t)nt, 0.0);
+  double* base = c.acc_pool.data();
+  c.acc_transMat = base;
+  c.acc_mean2D = base + 9 * (size_t)P;
+  c.acc_normal = base + 12 * (size_t)P;
+  c.acc_opacity = base + 15 * (size_t)P;
+  c.acc_colors = base + 16 * (size_t)P;
+  c.acc_sem = base + 19 * (size_t)P;
   const int B = s.B;
 #pragma omp parallel for schedule(dynamic, 1)
   for (int t = 0; t < c.T; ++t) {
@@ -1480,20 +1502,27 @@ void run_backward(Ctx<R>& c, const R* dL_dpix, const R* dL_dothers, const R* dL_
         }
       }
   }
-  auto round_into = [](std::vector<R>& dst, const std::vector<double>& src) {
-    dst.resize(src.size());
-    for (size_t i = 0; i < src.size(); ++i) dst[i] = (R)src[i];
+  // the thread copies into copy 0, in thread order
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)c.acc_n; ++i) {
+    double v = base[i];
+    for (int t = 1; t < nt; ++t) v += base[(size_t)t * c.acc_n + (size_t)i];
+    base[i] = v;
+  }
+  auto round_into = [](std::vector<R>& dst, const double* src, size_t n) {
+    dst.resize(n);
+    for (size_t i = 0; i < n; ++i) dst[i] = (R)src[i];
   };
-  round_into(c.g_transMat, c.acc_transMat);
-  round_into(c.g_mean2D, c.acc_mean2D);
-  round_into(c.g_normal, c.acc_normal);
-  round_into(c.g_opacity, c.acc_opacity);
-  round_into(c.g_colors, c.acc_colors);
+  round_into(c.g_transMat, c.acc_transMat, 9 * (size_t)P);
+  round_into(c.g_mean2D, c.acc_mean2D, 3 * (size_t)P);
+  round_into(c.g_normal, c.acc_normal, 3 * (size_t)P);
+  round_into(c.g_opacity, c.acc_opacity, (size_t)P);
+  round_into(c.g_colors, c.acc_colors, 3 * (size_t)P);
   c.g_means3D.assign(3 * (size_t)P, (R)0);
   c.g_sh.assign((size_t)P * s.M * 3, (R)0);
   c.g_scales.assign(2 * (size_t)P, (R)0);
   c.g_rots.assign(4 * (size_t)P, (R)0);
-  round_into(c.g_sem, c.acc_sem);
+  round_into(c.g_sem, c.acc_sem, (size_t)P * s.S);
   if (s.part) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < P; ++i) preprocess_bwd_one_part(c, i);
@@ -1546,6 +1575,14 @@ template <typename T> int copy_out(const std::vector<T>& v, void* dst, size_t nb
   return 0;
 }
 
+// An accumulator of n doubles (copy 0 of the pool; absent before a backward).
+inline int copy_out_acc(const double* p, size_t n, void* dst, size_t nbytes) {
+  if (!p) n = 0;
+  if (nbytes != n * sizeof(double)) return -(int)(n * sizeof(double)) - 1;
+  if (nbytes) std::memcpy(dst, p, nbytes);
+  return 0;
+}
+
 template <typename R> int get_named(Ctx<R>& c, const std::string& n, void* dst, size_t nbytes) {
   if (n == "radii") return copy_out(c.radii, dst, nbytes);
   if (n == "depths") return copy_out(c.depths, dst, nbytes);
@@ -1566,11 +1603,11 @@ template <typename R> int get_named(Ctx<R>& c, const std::string& n, void* dst, 
   if (n == "dL_dsemantics") return copy_out(c.g_sem, dst, nbytes);
   if (n == "final_T") return copy_out(c.final_T, dst, nbytes);
   if (n == "n_contrib") return copy_out(c.n_contrib, dst, nbytes);
-  if (n == "acc_dL_dtransMat") return copy_out(c.acc_transMat, dst, nbytes);
-  if (n == "acc_dL_dmean2D") return copy_out(c.acc_mean2D, dst, nbytes);
-  if (n == "acc_dL_dnormal") return copy_out(c.acc_normal, dst, nbytes);
-  if (n == "acc_dL_dopacity") return copy_out(c.acc_opacity, dst, nbytes);
-  if (n == "acc_dL_dcolors") return copy_out(c.acc_colors, dst, nbytes);
+  if (n == "acc_dL_dtransMat") return copy_out_acc(c.acc_transMat, 9 * (size_t)c.sc.P, dst, nbytes);
+  if (n == "acc_dL_dmean2D") return copy_out_acc(c.acc_mean2D, 3 * (size_t)c.sc.P, dst, nbytes);
+  if (n == "acc_dL_dnormal") return copy_out_acc(c.acc_normal, 3 * (size_t)c.sc.P, dst, nbytes);
+  if (n == "acc_dL_dopacity") return copy_out_acc(c.acc_opacity, (size_t)c.sc.P, dst, nbytes);
+  if (n == "acc_dL_dcolors") return copy_out_acc(c.acc_colors, 3 * (size_t)c.sc.P, dst, nbytes);
   if (n == "dL_dmeans2D") return copy_out(c.g_mean2D, dst, nbytes);
   if (n == "dL_dcolors") return copy_out(c.g_colors, dst, nbytes);
   if (n == "dL_dopacity") return copy_out(c.g_opacity, dst, nbytes);

@@ -10,9 +10,14 @@ rasterising the same blocks from cameras on the radius-3 sphere.
 fit_blocks optimises block parameters, occupancy and per-block colours
 against the target images with the block stage's Adam groups
 (games/block_mesh_splatting/scene/block_gaussian_model.py:429-442: rotation,
-scale and translation 5e-3, eps and occupancy 1e-3, eps 1e-15) and the image
-term of train.py ((1 - 0.2) L1 + 0.2 (1 - SSIM)), every view rasterised in one
-batched call and the block layer run by the fused HIP kernel.
+scale and translation 5e-3, eps and occupancy 1e-3, eps 1e-15) and the
+objective of summarize_losses (:584-648): the image term of train.py
+((1 - 0.2) L1 + 0.2 (1 - SSIM)) over every view, rasterised in one batched
+call, plus (with block_losses) parsimony, overlap and the three ray terms
+against one view's object mask per iteration (views in turn, the mask being
+the target's alpha > 0.5), the block layer and the superquadric distances run
+by the fused HIP kernels.  The recorded curve is the image term, the loss
+the comparison of initialisations is judged by.
 """
 from __future__ import annotations
 
@@ -25,6 +30,7 @@ import torch
 from .batch import BatchConfig, BatchRasterizer
 from .blocks import BlockGrid, generate_surfels, make_grid
 from .camera import random_sphere_camera
+from .block_losses import BlockLossWeights, block_regularizers
 from .losses import photometric
 
 
@@ -56,6 +62,7 @@ class BlockScene:
     targets: torch.Tensor  # flat batched colour images (the BatchRasterizer layout)
     grid: BlockGrid
     bg: torch.Tensor
+    masks: list = None     # per view: H x W object mask (target alpha > 0.5)
 
 
 def render_blocks(br: BatchRasterizer, params, occ, colors, grid: BlockGrid):
@@ -65,6 +72,22 @@ def render_blocks(br: BatchRasterizer, params, occ, colors, grid: BlockGrid):
     col = colors.repeat_interleave(per, dim=0)
     out_c, _, _ = br(means3D=means, opacities=opac, scales=scales, rotations=rots, colors_precomp=col)
     return torch.cat([c.reshape(-1) for c in out_c])
+
+
+def block_scene_points(seed: int, n_min: int = 4, n_max: int = 16, n_points: int = 20000, device="cuda"):
+    """The surface points of random_block_scene(seed) alone (the same draws,
+    no cameras or target images)."""
+    rng = np.random.default_rng(seed)
+    K = int(rng.integers(n_min, n_max + 1))
+    dev = torch.device(device)
+    params = torch.tensor(random_block_params(rng, K), device=dev)
+    occ = torch.full((K,), logit(0.95), device=dev)
+    rng.uniform(0.15, 0.95, size=(K, 3))  # the colours' draws
+    with torch.no_grad():
+        means, _, _, _ = generate_surfels(params, occ, make_grid(64, 64, device=dev))
+        pick = torch.tensor(rng.integers(0, means.shape[0], size=n_points), device=dev)
+        noise = torch.tensor(rng.normal(0.0, 0.005, size=(n_points, 3)).astype(np.float32), device=dev)
+        return means[pick] + noise
 
 
 def random_block_scene(seed: int, n_min: int = 4, n_max: int = 16, n_points: int = 20000, views: int = 6,
@@ -85,8 +108,12 @@ def random_block_scene(seed: int, n_min: int = 4, n_max: int = 16, n_points: int
     bg = torch.zeros(3, device=dev)
     br = BatchRasterizer(cams, bg, 0, BatchConfig(tile=16))
     with torch.no_grad():
-        targets = render_blocks(br, params, occ, colors, grid).detach()
-    return BlockScene(params, occ, colors, points, cams, targets, grid, bg)
+        means, scales, rots, opac = generate_surfels(params, occ, grid)
+        col = colors.repeat_interleave(grid.surfels_per_block(), dim=0)
+        out_c, out_o, _ = br(means3D=means, opacities=opac, scales=scales, rotations=rots, colors_precomp=col)
+        targets = torch.cat([c.reshape(-1) for c in out_c]).detach()
+        masks = [(o[1] > 0.5).float().contiguous() for o in out_o]
+    return BlockScene(params, occ, colors, points, cams, targets, grid, bg, masks)
 
 
 @dataclass
@@ -98,8 +125,10 @@ class FitResult:
 
 
 def fit_blocks(scene: BlockScene, init_params: torch.Tensor, init_occ: torch.Tensor, iters: int = 200,
-               br: BatchRasterizer | None = None) -> FitResult:
-    """The plain optimiser: Adam on the block parameters through the renderer."""
+               br: BatchRasterizer | None = None, block_losses: BlockLossWeights | None = None,
+               near_far=(1.0, 5.0), seed: int = 0) -> FitResult:
+    """The plain optimiser: Adam on the block parameters through the renderer.
+    FitResult.losses holds the image term per iteration."""
     dev = init_params.device
     B = init_params.shape[0]
     sq_s = init_params[:, 0:3].clone().requires_grad_(True)
@@ -114,24 +143,36 @@ def fit_blocks(scene: BlockScene, init_params: torch.Tensor, init_occ: torch.Ten
         {"params": [col_logit], "lr": 2.5e-2},
     ], lr=0.0, eps=1e-15)
     br = br or BatchRasterizer(scene.cameras, scene.bg, 0, BatchConfig(tile=16))
+    target = _stack_views(br, scene.targets)
+    gen = torch.Generator(device=dev).manual_seed(seed) if block_losses is not None else None
     losses = []
-    for _ in range(iters):
+    for it in range(iters):
         params = torch.cat([sq_s, sq_r, sq_t, sq_eps], dim=1)
-        img = render_blocks(br, params, occ, torch.sigmoid(col_logit), scene.grid)
-        loss = _image_loss(br, img, scene.targets)
+        img = _stack_views(br, render_blocks(br, params, occ, torch.sigmoid(col_logit), scene.grid))
+        image_loss = photometric(img, target)
+        loss = image_loss
+        if block_losses is not None:
+            v = it % br.V
+            mask = scene.masks[v] if scene.masks is not None else None
+            reg, _ = block_regularizers(params, occ, scene.cameras[v], block_losses, mask=mask,
+                                        generator=gen, near_far=near_far)
+            loss = loss + reg
         opt.zero_grad(set_to_none=True)
         loss.backward()
         opt.step()
-        losses.append(float(loss.detach()))
+        losses.append(image_loss.detach())
     params = torch.cat([sq_s, sq_r, sq_t, sq_eps], dim=1).detach()
-    return FitResult(params, occ.detach(), torch.sigmoid(col_logit).detach(), losses)
+    curve = torch.stack(losses).cpu().tolist() if losses else []  # one synchronisation per fit
+    return FitResult(params, occ.detach(), torch.sigmoid(col_logit).detach(), curve)
+
+
+def _stack_views(br: BatchRasterizer, flat: torch.Tensor) -> torch.Tensor:
+    """V x 3 x H x W from the flat batched colour layout (views of one size)."""
+    return torch.stack(list(br.split(flat, 3)))
 
 
 def _image_loss(br: BatchRasterizer, img: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
-    total = 0.0
-    for a, b in zip(br.split(img, 3), br.split(target, 3)):
-        total = total + photometric(a, b)
-    return total / br.V
+    return photometric(_stack_views(br, img), _stack_views(br, target))
 
 
 def iterations_to_reach(losses, level: float) -> int:

@@ -228,16 +228,15 @@ class BlockLossWeights:
 _NEAR_FAR = {"dtu": (0.5, 3.5), "blender": (2.0, 5.0), "blendermvs": (1.0, 3.0)}
 
 
-def summarize_losses(image, gt_image, camera, opt: BlockLossWeights, params, occ_logits, mask=None,
-                     ratio: float = 0.25, scale_min: float = 0.2, use_reg: bool = True, generator=None,
-                     near_far=None):
-    """summarize_losses (block_gaussian_model.py:584-648): the image term,
-    then parsimony, the ray terms (with a mask) and overlap, each weighted as
-    the reference does.  near_far overrides the data type's sampling range.
-    Returns (Ll1, loss, loss_dict)."""
+def block_regularizers(params, occ_logits, camera, opt: BlockLossWeights, mask=None, ratio: float = 0.25,
+                       scale_min: float = 0.2, use_reg: bool = True, generator=None, near_far=None,
+                       device="cuda"):
+    """The terms summarize_losses adds to the image loss
+    (block_gaussian_model.py:598-646): primitive scale (dtu / blendermvs),
+    parsimony, the three ray terms for one view's mask and overlap, each
+    weighted.  Returns (weighted sum, loss_dict)."""
     loss_dict = {}
-    Ll1 = l1_loss(image, gt_image) * opt.rgb_weight
-    loss = (1.0 - opt.lambda_dssim) * Ll1 + opt.lambda_dssim * (1.0 - ssim(image, gt_image))
+    loss = torch.zeros((), device=params.device)
     if opt.primtive_scale_weight and opt.data_type in ("dtu", "blendermvs"):
         d = primitive_scale_regularization(params, scale_min)
         loss_dict.update(d)
@@ -248,7 +247,7 @@ def summarize_losses(image, gt_image, camera, opt: BlockLossWeights, params, occ
         loss = loss + d["parsimony"] * opt.parsimony_weight
     if use_reg and mask is not None:
         near, far = near_far if near_far is not None else _NEAR_FAR[opt.data_type]
-        pts, ray_idx = cast_rays(camera, depth_min=near, depth_max=far, num_samples=128, device=image.device,
+        pts, ray_idx = cast_rays(camera, depth_min=near, depth_max=far, num_samples=128, device=params.device,
                                  generator=generator)
         label_mask = mask.flatten(start_dim=0, end_dim=1)[ray_idx]
         n_rays = label_mask.shape[0]
@@ -266,4 +265,18 @@ def summarize_losses(image, gt_image, camera, opt: BlockLossWeights, params, occ
         d = overlap_loss(params, occ_logits, ratio, scale_min, generator=generator)
         loss_dict.update(d)
         loss = loss + d["overlap"] * opt.overlap_weight
-    return Ll1, loss, loss_dict
+    return loss, loss_dict
+
+
+def summarize_losses(image, gt_image, camera, opt: BlockLossWeights, params, occ_logits, mask=None,
+                     ratio: float = 0.25, scale_min: float = 0.2, use_reg: bool = True, generator=None,
+                     near_far=None):
+    """summarize_losses (block_gaussian_model.py:584-648): the image term,
+    then parsimony, the ray terms (with a mask) and overlap, each weighted as
+    the reference does.  near_far overrides the data type's sampling range.
+    Returns (Ll1, loss, loss_dict)."""
+    Ll1 = l1_loss(image, gt_image) * opt.rgb_weight
+    loss = (1.0 - opt.lambda_dssim) * Ll1 + opt.lambda_dssim * (1.0 - ssim(image, gt_image))
+    reg, loss_dict = block_regularizers(params, occ_logits, camera, opt, mask=mask, ratio=ratio, scale_min=scale_min,
+                                        use_reg=use_reg, generator=generator, near_far=near_far)
+    return Ll1, loss + reg, loss_dict

@@ -37,7 +37,8 @@ import time
 import numpy as np
 import torch
 
-from .block_fit import fit_blocks, iterations_to_reach, logit, random_block_scene
+from .block_fit import block_scene_points, fit_blocks, iterations_to_reach, logit, random_block_scene
+from .block_losses import BlockLossWeights
 from .init import plain_block_init
 from .proposal import N_BLOCKS, ProposalNet
 
@@ -63,10 +64,10 @@ def plain_init(points: torch.Tensor, seed: int):
     return params, occ
 
 
-def label_scene(seed: int, fit_iters: int):
+def label_scene(seed: int, fit_iters: int, block_losses=None):
     scene = random_block_scene(seed)
     p0, o0 = plain_init(scene.points, seed)
-    fit = fit_blocks(scene, p0, o0, iters=fit_iters)
+    fit = fit_blocks(scene, p0, o0, iters=fit_iters, block_losses=block_losses, seed=seed)
     return scene, canonical(fit.params), fit.losses
 
 
@@ -115,17 +116,17 @@ def train(net: ProposalNet, points: torch.Tensor, labels: torch.Tensor, epochs: 
     return hist
 
 
-def evaluate(net: ProposalNet, seeds, fit_iters: int):
+def evaluate(net: ProposalNet, seeds, fit_iters: int, block_losses=None):
     rows = []
     for sd in seeds:
         scene = random_block_scene(sd)
         p0, o0 = plain_init(scene.points, sd)
-        plain = fit_blocks(scene, p0, o0, iters=fit_iters)
+        plain = fit_blocks(scene, p0, o0, iters=fit_iters, block_losses=block_losses, seed=sd)
         with torch.no_grad():
             out, _ = net.forward(scene.points[None])
         pl = canonical(out[0])
         ol = torch.full((N_BLOCKS,), logit(0.7), device=pl.device)
-        learned = fit_blocks(scene, pl, ol, iters=fit_iters)
+        learned = fit_blocks(scene, pl, ol, iters=fit_iters, block_losses=block_losses, seed=sd)
         final_plain = float(np.mean(plain.losses[-5:]))
         level = 1.01 * final_plain
         it_learned = iterations_to_reach(learned.losses, level)
@@ -154,11 +155,15 @@ def main(argv=None):
     ap.add_argument("--labels-out", default=None, help="save the labelled seeds and blocks here")
     ap.add_argument("--labels-in", nargs="*", default=None, help="train on these saved labels")
     ap.add_argument("--label-only", action="store_true", help="label (and save), then stop")
+    ap.add_argument("--no-block-losses", action="store_true",
+                    help="fit with the image term alone (default: the block stage's full objective)")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
     torch.manual_seed(0)
     net = ProposalNet(seed=0)
-    report = {"train_scenes": args.train, "eval_scenes": args.eval, "fit_iters": args.fit_iters}
+    report = {"train_scenes": args.train, "eval_scenes": args.eval, "fit_iters": args.fit_iters,
+              "block_losses": not args.no_block_losses}
+    bl = None if args.no_block_losses else BlockLossWeights()
     t0 = time.time()
     if args.weights:
         sd = np.load(args.weights)
@@ -170,19 +175,22 @@ def main(argv=None):
                 z = np.load(fn)
                 for sd, lb in zip(z["seeds"], z["labels"]):
                     seeds.append(int(sd))
-                    pts.append(random_block_scene(int(sd)).points)
+                    pts.append(block_scene_points(int(sd)))
                     labels.append(torch.tensor(lb, device=pts[-1].device))
             _log(f"[proposal] loaded {len(seeds)} labelled scenes from {len(args.labels_in)} file(s)")
         else:
             for i in range(args.train):
                 sd = args.label_seed + i
-                scene, label, losses = label_scene(sd, args.fit_iters)
+                scene, label, losses = label_scene(sd, args.fit_iters, bl)
                 seeds.append(sd)
                 pts.append(scene.points)
                 labels.append(label)
                 if i % 16 == 0:
                     _log(f"[proposal] labelled {i + 1}/{args.train} scenes, fit loss {losses[0]:.4f} -> "
                          f"{losses[-1]:.4f} ({time.time() - t0:.0f} s)")
+                if args.labels_out and (i + 1) % 64 == 0:  # kept if the run is cut short
+                    np.savez(args.labels_out, seeds=np.array(seeds, np.int64),
+                             labels=torch.stack(labels).cpu().numpy())
         points = torch.stack(pts)
         labels = torch.stack(labels)
         report["train_scenes"] = len(seeds)
@@ -198,7 +206,7 @@ def main(argv=None):
         report["train_loss"] = [hist[0], hist[-1]]
         np.savez(os.path.join(args.out, "proposal_weights.npz"),
                  **{k: v.numpy() for k, v in net.state_dict().items()})
-    rows = evaluate(net, range(90_000, 90_000 + args.eval), args.fit_iters)
+    rows = evaluate(net, range(90_000, 90_000 + args.eval), args.fit_iters, bl)
     ratios = [r["iters_learned"] / r["iters_plain"] for r in rows]
     report["eval"] = rows
     report["median_iters_ratio_vs_plain_total"] = float(np.median(ratios))

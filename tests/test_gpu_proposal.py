@@ -146,3 +146,25 @@ def test_protocol_runs_end_to_end(tmp_path):
 
     rep = json.load(open(tmp_path / "proposal_report.json"))
     assert rep["eval"] and (tmp_path / "proposal_weights.npz").exists()
+
+
+def test_scene_points_match_full_scene():
+    from partgs_amd.block_fit import block_scene_points, random_block_scene
+
+    print("[proposal] scene points", flush=True)
+    for sd in (5, 90001):
+        assert torch.equal(block_scene_points(sd), random_block_scene(sd).points)
+
+
+def test_block_fit_with_block_losses():
+    from partgs_amd.block_fit import fit_blocks, random_block_scene
+    from partgs_amd.block_losses import BlockLossWeights
+    from partgs_amd.init import plain_block_init
+
+    print("[proposal] block fit with the block-stage objective", flush=True)
+    scene = random_block_scene(321, n_points=4000, views=3, width=64, height=48)
+    assert scene.masks is not None and len(scene.masks) == 3 and scene.masks[0].shape == (48, 64)
+    p0, o0 = plain_block_init(scene.points, n_blocks=8, seed=0)
+    fit = fit_blocks(scene, p0, o0, iters=40, block_losses=BlockLossWeights(), near_far=(1.0, 5.0), seed=1)
+    print(f"[proposal] fit loss {fit.losses[0]:.4f} -> {fit.losses[-1]:.4f}", flush=True)
+    assert all(np.isfinite(fit.losses)) and fit.losses[-1] < fit.losses[0]
