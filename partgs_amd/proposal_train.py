@@ -6,7 +6,10 @@ Labelling dominates the cost (one plain fit per training scene), so it can
 be split over several runs: `--label-seed S --labels-out F --label-only`
 labels scenes S, S+1, ... and saves their seeds and fitted blocks (the
 points are regenerated from the seeds); `--labels-in F1 F2 ...` trains on
-saved labels instead of labelling.
+saved labels instead of labelling.  The evaluation splits the same way:
+`--train-only` trains and saves the weights; `--weights W --eval N
+--eval-seed S` evaluates held-out scenes S .. S+N-1; `--merge R1 R2 ...`
+combines the evaluation rows of several reports into one.
 
 1. Dataset: synthetic block scenes (partgs_amd.block_fit.random_block_scene,
    4-16 superquadrics, 20,000 noisy surface points each), each labelled by
@@ -155,9 +158,16 @@ def main(argv=None):
     ap.add_argument("--labels-out", default=None, help="save the labelled seeds and blocks here")
     ap.add_argument("--labels-in", nargs="*", default=None, help="train on these saved labels")
     ap.add_argument("--label-only", action="store_true", help="label (and save), then stop")
+    ap.add_argument("--eval-seed", type=int, default=90_000, help="first held-out scene seed")
+    ap.add_argument("--train-only", action="store_true", help="train and save the weights, then stop")
+    ap.add_argument("--merge", nargs="*", default=None, help="combine the evaluation rows of these reports")
     ap.add_argument("--no-block-losses", action="store_true",
                     help="fit with the image term alone (default: the block stage's full objective)")
     args = ap.parse_args(argv)
+    if args.merge:
+        merged = merge_reports(args.merge, args.out)
+        _log(json.dumps({k: v for k, v in merged.items() if k != "eval"}))
+        return 0
     os.makedirs(args.out, exist_ok=True)
     torch.manual_seed(0)
     net = ProposalNet(seed=0)
@@ -177,6 +187,8 @@ def main(argv=None):
                     seeds.append(int(sd))
                     pts.append(block_scene_points(int(sd)))
                     labels.append(torch.tensor(lb, device=pts[-1].device))
+                    if len(seeds) % 1000 == 0:
+                        _log(f"[proposal] regenerated {len(seeds)} scenes ({time.time() - t0:.0f} s)")
             _log(f"[proposal] loaded {len(seeds)} labelled scenes from {len(args.labels_in)} file(s)")
         else:
             for i in range(args.train):
@@ -206,18 +218,51 @@ def main(argv=None):
         report["train_loss"] = [hist[0], hist[-1]]
         np.savez(os.path.join(args.out, "proposal_weights.npz"),
                  **{k: v.numpy() for k, v in net.state_dict().items()})
-    rows = evaluate(net, range(90_000, 90_000 + args.eval), args.fit_iters, bl)
-    ratios = [r["iters_learned"] / r["iters_plain"] for r in rows]
+    if args.train_only:
+        with open(os.path.join(args.out, "proposal_report.json"), "w") as f:
+            json.dump(report, f, indent=1)
+        _log(json.dumps(report))
+        return 0
+    rows = evaluate(net, range(args.eval_seed, args.eval_seed + args.eval), args.fit_iters, bl)
+    report["eval_seed"] = args.eval_seed
     report["eval"] = rows
-    report["median_iters_ratio_vs_plain_total"] = float(np.median(ratios))
-    report["median_iters_ratio_vs_plain_first_hit"] = float(
-        np.median([r["iters_learned"] / r["iters_plain_first_hit"] for r in rows]))
-    report["reached_within_1pct"] = float(np.mean([r["iters_learned"] <= args.fit_iters for r in rows]))
-    report["fallback_fraction"] = float(np.mean([r["fallback_to_plain"] for r in rows]))
+    report.update(summarize(rows, args.fit_iters))
     with open(os.path.join(args.out, "proposal_report.json"), "w") as f:
         json.dump(report, f, indent=1)
     _log(json.dumps({k: v for k, v in report.items() if k != "eval"}))
+    return 0
 
+
+def summarize(rows, fit_iters: int) -> dict:
+    """Medians over evaluation rows: the proposal's iterations to 1.01 x the
+    plain arm's final loss against the plain arm's total and first-hit
+    iterations; the share of scenes reached within the budget and the share
+    that falls back to the plain result."""
+    return {
+        "eval_scenes": len(rows),
+        "median_iters_ratio_vs_plain_total": float(np.median([r["iters_learned"] / r["iters_plain"] for r in rows])),
+        "median_iters_ratio_vs_plain_first_hit": float(
+            np.median([r["iters_learned"] / r["iters_plain_first_hit"] for r in rows])),
+        "reached_within_1pct": float(np.mean([r["iters_learned"] <= fit_iters for r in rows])),
+        "fallback_fraction": float(np.mean([r["fallback_to_plain"] for r in rows])),
+    }
+
+
+def merge_reports(paths, out: str) -> dict:
+    """One report from the evaluation rows of several (rows ordered by seed,
+    duplicates rejected); the other fields come from the first report."""
+    reports = [json.load(open(p)) for p in paths]
+    rows = sorted((r for rep in reports for r in rep["eval"]), key=lambda r: r["seed"])
+    seeds = [r["seed"] for r in rows]
+    if len(set(seeds)) != len(seeds):
+        raise ValueError("the reports evaluate some scene twice")
+    merged = {k: v for k, v in reports[0].items() if k not in ("eval", "eval_seed")}
+    merged["eval"] = rows
+    merged.update(summarize(rows, merged["fit_iters"]))
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(out, "proposal_report.json"), "w") as f:
+        json.dump(merged, f, indent=1)
+    return merged
 
 if __name__ == "__main__":
     sys.exit(main())

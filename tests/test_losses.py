@@ -55,3 +55,34 @@ def test_canonical_quaternions():
     q = torch.tensor([[0.0] * 3 + [-2.0, 0.0, 0.0, 0.0] + [0.0] * 5])
     c = canonical(q)
     assert np.allclose(c[0, 3:7].numpy(), [1.0, 0.0, 0.0, 0.0])
+
+
+def test_proposal_report_merge(tmp_path):
+    """Evaluation rows split over several runs merge into the report one run
+    over all scenes writes: rows by seed, medians over the union."""
+    import json
+
+    from partgs_amd.proposal_train import merge_reports, summarize
+
+    def row(seed, learned, first):
+        return {"seed": seed, "iters_plain": 200, "iters_plain_first_hit": first, "iters_learned": learned,
+                "fallback_to_plain": learned > 200}
+
+    a = [row(3, 50, 100), row(1, 201, 150)]
+    b = [row(2, 20, 200)]
+    paths = []
+    for i, rows in enumerate((a, b)):
+        fn = tmp_path / f"r{i}.json"
+        fn.write_text(json.dumps({"fit_iters": 200, "block_losses": True, "eval_seed": 0, "eval": rows}))
+        paths.append(str(fn))
+    m = merge_reports(paths, str(tmp_path / "m"))
+    assert [r["seed"] for r in m["eval"]] == [1, 2, 3]
+    assert m["eval_scenes"] == 3
+    assert m["median_iters_ratio_vs_plain_total"] == 50 / 200
+    assert m["median_iters_ratio_vs_plain_first_hit"] == 0.5
+    assert abs(m["reached_within_1pct"] - 2 / 3) < 1e-12 and abs(m["fallback_fraction"] - 1 / 3) < 1e-12
+    assert m == json.loads((tmp_path / "m" / "proposal_report.json").read_text())
+    assert summarize(a + b, 200)["eval_scenes"] == 3
+    import pytest
+    with pytest.raises(ValueError):
+        merge_reports([paths[0], paths[0]], str(tmp_path / "dup"))
