@@ -180,6 +180,79 @@ struct ChainBwd {
 };
 static_assert(sizeof(ChainBwd) == 18 * sizeof(float), "chain state is 18 floats");
 
+// ChainBwd's recurrences over one segment of a run, as an affine map of the
+// state entering the segment.  The seven colour, depth and normal
+// accumulators and accum_alpha_rec follow x <- la l + (1 - la) x (la, l the
+// alpha and value of the pair behind), last_dL_dT follows
+// y <- dw alpha + (1 - alpha) y, and T <- T / (1 - alpha).  Accumulated from
+// the identity over the segment's pairs (back to front), the map takes the
+// entering state to the leaving one: x_out = A x_in + b, y_out = A2 y_in + b2,
+// T_out = T_in / D.  Folding the maps of the later segments onto the chain's
+// initial state gives the state where a segment starts without walking the
+// pairs behind it; every path folds in the same order with the same
+// operations, so a segment's replay starts from the same bits wherever its
+// pairs come from.
+struct SegMap {
+  float A = 1.0f, A2 = 1.0f, b2 = 0.0f, D = 1.0f;
+  float b[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};  // r, g, b, depth, n0, n1, n2, alpha
+  // one pair (alpha, distortion weight m_d) after the pair behind it (la,
+  // values l: r, g, b, depth, n0, n1, n2)
+  SR_DEV void add(const PixelBwd& p, float la, const float* l, float alpha, float m_d) {
+    const float om = 1.0f - la;
+#pragma unroll
+    for (int x = 0; x < 7; ++x) b[x] = la * l[x] + om * b[x];
+    b[7] = la + om * b[7];
+    A = om * A;
+    const float dw = ((p.final_D2 + (m_d * m_d) * p.final_A) - (2.0f * m_d) * p.final_D) * p.dL_dreg;
+    const float oa = 1.0f - alpha;
+    b2 = dw * alpha + oa * b2;
+    A2 = oa * A2;
+    D = D * oa;
+  }
+  // the map applied to the recurrence part of a chain state
+  SR_DEV void apply(ChainBwd& s) const {
+    s.T = s.T / D;
+    s.acc_r0 = A * s.acc_r0 + b[0];
+    s.acc_r1 = A * s.acc_r1 + b[1];
+    s.acc_r2 = A * s.acc_r2 + b[2];
+    s.accum_depth_rec = A * s.accum_depth_rec + b[3];
+    s.acc_n0 = A * s.acc_n0 + b[4];
+    s.acc_n1 = A * s.acc_n1 + b[5];
+    s.acc_n2 = A * s.acc_n2 + b[6];
+    s.accum_alpha_rec = A * s.accum_alpha_rec + b[7];
+    s.last_dL_dT = A2 * s.last_dL_dT + b2;
+  }
+  SR_DEV void save(float* d) const {
+    d[0] = A;
+    d[1] = A2;
+    d[2] = b2;
+    d[3] = D;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) d[4 + x] = b[x];
+  }
+  SR_DEV void load(const float* d) {
+    A = d[0];
+    A2 = d[1];
+    b2 = d[2];
+    D = d[3];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) b[x] = d[4 + x];
+  }
+};
+constexpr int kSegMapFloats = 12;
+
+// The values of a chain state that a pair sets for the next one (the "last"
+// members), in SegMap's order.
+SR_DEV void chain_last(const ChainBwd& s, float* l) {
+  l[0] = s.last_c0;
+  l[1] = s.last_c1;
+  l[2] = s.last_c2;
+  l[3] = s.last_depth;
+  l[4] = s.last_n0;
+  l[5] = s.last_n1;
+  l[6] = s.last_n2;
+}
+
 // The 18 partial gradients of one pair (backward.cu:380-437) from its
 // weight w, dL/dalpha and dL/ddepth and its pixel's dL/dpixel (dLp) and
 // dL/dnormal (dLn), handed to add(slot, value).
@@ -238,7 +311,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   __shared__ uint2 s_e[64];
   __shared__ float s_acc[kNumG][64];
   __shared__ float s_tmp[kNumG][65];
-  __shared__ float s_state[8][7][18];  // chain state where each later segment of a run starts
+  __shared__ float s_map[8][8][kSegMapFloats];  // per pixel of the wave, each segment's SegMap
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   if (a.counters[1] & kErrCapacity) return;
   const int qi = (int)a.q_order[blockIdx.x / kSplitBwd], part = blockIdx.x % kSplitBwd;
@@ -264,12 +337,13 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
     // Each pixel's blended pairs are one contiguous run in entry order (the
     // forward cut the run at the pixel's last contributor), walked from the
     // back with the next kChainPre pairs in flight.  A wave holds 8 pixels
-    // and gives each 8 lanes: a long run is cut into up to 8 segments; the
-    // pixel's first lane walks the run once with the state updates alone
-    // (ChainBwd::advance, about half a step), saving the chain state where
-    // each segment starts, then every segment is replayed with full steps by
-    // its own lane from its saved state.  The replay repeats the sequential
-    // walk's operations exactly, so each pair's outputs are bit-identical.
+    // and gives each 8 lanes: a long run is cut into up to 8 segments, one
+    // per lane.  Each segment's lane first runs its pairs through SegMap (the
+    // state recurrences as an affine map, about a third of a step); the maps
+    // of the segments behind a segment, folded onto the chain's initial
+    // state, give the state where it starts; then every lane replays its
+    // segment with full steps from that state.  The serial chain of a run of
+    // n pairs is two passes over n / 8 pairs instead of one over n.
     const int ps = lane & 7, sg = lane >> 3;
     const int pxl = part * 8 + ps;
     const uint32_t qxi = (uint32_t)(tx * B + (q % QX) * kQuad + (pxl & 7));
@@ -312,41 +386,59 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
           if (at - k >= lo) fn(at - k, c0[k], c1[k], c2[k]);
       }
     };
-    // state walk: segment s starts (backward order) after its pairs from
-    // (s + 1) seg_len on have run
-    float* st_out = &s_state[ps][0][0];
-    if (sg == 0 && nseg > 1) {
-      ChainBwd w0;
-      w0.T = px.T_final;
-      // segment starts from the back: multiples of seg_len, counted down
-      // (no integer division on the serial walk)
-      int boundary = ((n - 1) / seg_len) * seg_len, slot = boundary / seg_len - 1;
-      walk(seg_len, n - 1, [&](int k, const float4& c0, const float4& c1, const float4& c2) {
-        w0.advance(px, c0.x, c0.y, c0.z, c1, c2.x, c2.y);
-        if (k == boundary) {
-          const float* src = reinterpret_cast<const float*>(&w0);
-          float* dst = st_out + slot * 18;
+    const int lo = sg * seg_len, hi = min(n, lo + seg_len) - 1;
+    // alpha and values of the pair behind the segment (none behind the last)
+    float la = 0.0f, l[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (sg < nseg - 1) {
+      const float4* pr = a.ps.rec + kPairQuads * (size_t)(f0 + hi + 1);
+      const float4 e0 = pr[0], e1 = pr[1], e2 = pr[2];
+      la = e0.x;
+      l[0] = e1.w;
+      l[1] = e2.x;
+      l[2] = e2.y;
+      l[3] = e0.y;
+      l[4] = e1.x;
+      l[5] = e1.y;
+      l[6] = e1.z;
+    }
+    float* maps = &s_map[ps][0][0];
+    if (sg >= 1 && sg < nseg) {  // segment 0's map is never needed
+      SegMap m;
+      float pa = la, pl[7];
 #pragma unroll
-          for (int i = 0; i < 18; ++i) dst[i] = src[i];
-          boundary -= seg_len;
-          --slot;
-        }
+      for (int x = 0; x < 7; ++x) pl[x] = l[x];
+      walk(lo, hi, [&](int, const float4& c0, const float4& c1, const float4& c2) {
+        m.add(px, pa, pl, c0.x, c0.z);
+        pa = c0.x;
+        pl[0] = c1.w;
+        pl[1] = c2.x;
+        pl[2] = c2.y;
+        pl[3] = c0.y;
+        pl[4] = c1.x;
+        pl[5] = c1.y;
+        pl[6] = c1.z;
       });
+      m.save(maps + sg * kSegMapFloats);
     }
     __syncthreads();
     if (sg < nseg) {
-      if (sg == nseg - 1) {
-        ch.T = px.T_final;
-      } else {
-        const float* src = st_out + sg * 18;
-        float* dst = reinterpret_cast<float*>(&ch);
-#pragma unroll
-        for (int i = 0; i < 18; ++i) dst[i] = src[i];
+      ch.T = px.T_final;
+      for (int t = nseg - 1; t > sg; --t) {
+        SegMap m;
+        m.load(maps + t * kSegMapFloats);
+        m.apply(ch);
       }
-      walk(sg * seg_len, min(n, (sg + 1) * seg_len) - 1,
-           [&](int k, const float4& c0, const float4& c1, const float4& c2) {
-             a.ps.fW[f0 + k] = ch.step(px, c0.x, c0.y, c0.z, c0.w, __float_as_uint(c2.z), c1, c2.x, c2.y);
-           });
+      ch.last_alpha = la;
+      ch.last_c0 = l[0];
+      ch.last_c1 = l[1];
+      ch.last_c2 = l[2];
+      ch.last_depth = l[3];
+      ch.last_n0 = l[4];
+      ch.last_n1 = l[5];
+      ch.last_n2 = l[6];
+      walk(lo, hi, [&](int k, const float4& c0, const float4& c1, const float4& c2) {
+        a.ps.fW[f0 + k] = ch.step(px, c0.x, c0.y, c0.z, c0.w, __float_as_uint(c2.z), c1, c2.x, c2.y);
+      });
     }
     return;
   }
@@ -359,6 +451,17 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
   const uint32_t* frag_off = a.frag_off + mask_base(Q, rbeg, ntile, tile, q);
   const PixelBwd px = pixel_bwd(a, vi, inside, pxi, pyi);
   ch.T = px.T_final;
+  // The pixel's run is cut into the stored path's segments (the forward
+  // recorded how many pairs the pixel blended): while a segment's pairs go
+  // through full steps, its SegMap is accumulated alongside, and where the
+  // segment ends the state is reset to the maps folded onto the initial
+  // state, exactly as the stored path's segment lanes start, so each pair's
+  // outputs have the stored path's bits.
+  ChainBwd fold = ch;  // the state the current segment started from
+  SegMap seg_map;
+  const int n_run = inside ? (int)a.lane_range[(size_t)qi * 64 + pix].y : 0;
+  const int seg_len = max(kChainSegMin, (n_run + 7) / 8);
+  int seg = (n_run + seg_len - 1) / seg_len - 1, seg_start = seg * seg_len, idx = n_run;
   const int nch = (cnt + 63) / 64;
   const bool active = inside && px.last_contributor > 0;
   const float mcoef = kFar / (kFar - kNear);
@@ -412,7 +515,29 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
             const float m_d = mcoef * (1.0f - kNear / c_d);
             const float dmd_dd = dmd_num / ((dmd_den0 * c_d) * c_d);
             const float4 q3 = s_q3[j], q4 = s_q4[j];
+            --idx;  // this pair's place in the run
+            if (seg >= 1) {
+              float l[7];
+              chain_last(ch, l);
+              seg_map.add(px, ch.last_alpha, l, e.alpha, m_d);
+            }
             const float4 g = ch.step(px, e.alpha, c_d, m_d, dmd_dd, contributor, q3, q4.x, q4.y);
+            if (idx == seg_start && seg >= 1) {  // the next pair starts segment seg - 1
+              seg_map.apply(fold);
+              ch.T = fold.T;
+              ch.acc_r0 = fold.acc_r0;
+              ch.acc_r1 = fold.acc_r1;
+              ch.acc_r2 = fold.acc_r2;
+              ch.accum_depth_rec = fold.accum_depth_rec;
+              ch.acc_n0 = fold.acc_n0;
+              ch.acc_n1 = fold.acc_n1;
+              ch.acc_n2 = fold.acc_n2;
+              ch.accum_alpha_rec = fold.accum_alpha_rec;
+              ch.last_dL_dT = fold.last_dL_dT;
+              seg_map = SegMap();
+              --seg;
+              seg_start -= seg_len;
+            }
             pair_grads(px.dLp0, px.dLp1, px.dLp2, px.dLn0, px.dLn1, px.dLn2, e, s_q1[j], q2.w, g.x, g.y, g.z,
                        pixx, pixy, [&](int k, float v) { gk[k] += v; });
             took = true;

@@ -1953,6 +1953,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
   uint32_t last_contributor = 0;
   float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, N0 = 0.0f, N1 = 0.0f, N2 = 0.0f;
   float D = 0.0f, M1 = 0.0f, M2 = 0.0f, distortion = 0.0f, median_depth = 0.0f, median_contributor = -1.0f;
+  int blended = 0;
   const float mcoef = kFar / (kFar - kNear);
   const int nch = (cnt + 63) / 64;
   // One front-to-back step of the reference's update (forward.cu:383-415);
@@ -1980,6 +1981,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
     C2 += bl * w;
     T = test_T;
     last_contributor = contributor;
+    ++blended;
     return true;
   };
   {
@@ -2056,6 +2058,8 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
     out_others[pid + 4 * HW] = N2;
     out_others[pid + 5 * HW] = median_depth;
     out_others[pid + 6 * HW] = distortion;
+    // the backward cuts the pixel's run into the same segments as a stored run
+    a.lane_range[(size_t)qi * 64 + pix].y = (uint32_t)blended;
   }
 }
 
