@@ -171,7 +171,9 @@ def main(argv=None):
     os.makedirs(args.out, exist_ok=True)
     torch.manual_seed(0)
     net = ProposalNet(seed=0)
-    report = {"train_scenes": args.train, "eval_scenes": args.eval, "fit_iters": args.fit_iters,
+    # a run on saved weights reports no training-set size (the training run's report holds it)
+    report = {"train_scenes": None if args.weights else args.train, "eval_scenes": args.eval,
+              "fit_iters": args.fit_iters,
               "block_losses": not args.no_block_losses}
     bl = None if args.no_block_losses else BlockLossWeights()
     t0 = time.time()
