@@ -530,20 +530,50 @@ hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
 }
 
 
-// Inclusive Hillis-Steele scan of one value per lane of a 1024-lane
-// workgroup (s: 1024 words of LDS); returns the lane's exclusive prefix.
-SR_DEV uint32_t scan1024_excl(uint32_t* s, uint32_t x) {
-  const int tid = threadIdx.x;
-  __syncthreads();
-  s[tid] = x;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint32_t v = tid >= off ? s[tid - off] : 0u;
-    __syncthreads();
-    s[tid] += v;
-    __syncthreads();
+// Exclusive scan of N values per lane over a 1024-lane workgroup: each wave
+// scans its 64 lanes with shuffles, one wave scans the 16 wave totals, so
+// the workgroup meets at three barriers instead of two per doubling step.
+// s: 32 N words of LDS.  Returns the lane's exclusive prefixes in x and the
+// workgroup totals in total.
+template <int N>
+SR_DEV void scan1024_excl(uint32_t* s, uint32_t (&x)[N], uint32_t (&total)[N]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t incl[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) incl[k] = x[k];
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const uint32_t y = (uint32_t)__shfl_up((int)incl[k], off, 64);
+      if (lane >= off) incl[k] += y;
+    }
   }
-  return tid == 0 ? 0u : s[tid - 1];
+  __syncthreads();  // s may still be read by a previous scan
+  if (lane == 63) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) s[k * 32 + wave] = incl[k];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      uint32_t t = lane < 16 ? s[k * 32 + lane] : 0u;
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)t, off, 64);
+        if (lane >= off) t += y;
+      }
+      if (lane < 16) s[k * 32 + 16 + lane] = t;  // inclusive over the waves
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const uint32_t base = wave == 0 ? 0u : s[k * 32 + 16 + wave - 1];
+    total[k] = s[k * 32 + 31];
+    x[k] = base + incl[k] - x[k];
+  }
 }
 
 // Exclusive scan of the tile histogram by one 1024-lane workgroup, with the
@@ -561,23 +591,28 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
                                                               uint32_t capacity, int nb, uint32_t* bsum,
                                                               uint32_t* large, uint32_t* lchunk_off,
                                                               int large_max) {
-  __shared__ uint32_t s_part[kScanThreads];
+  __shared__ uint32_t s_part[3 * 32];
   const int tid = threadIdx.x;
   const int per = (T + kScanThreads - 1) / kScanThreads;
   const int b = tid * per;
   const int e = min(T, b + per);
-  uint32_t local = 0;
-  for (int i = b; i < e; ++i) local += count[i];
-  s_part[tid] = local;
-  __syncthreads();
-  // Hillis-Steele inclusive scan over the 1024 partial sums.
-  for (int off = 1; off < kScanThreads; off <<= 1) {
-    const uint32_t v = tid >= off ? s_part[tid - off] : 0u;
-    __syncthreads();
-    s_part[tid] += v;
-    __syncthreads();
+  auto chunks = [](uint32_t c) { return (c + (uint32_t)kSmallMax - 1u) / (uint32_t)kSmallMax; };
+  auto is_large = [&](uint32_t c) { return c > (uint32_t)kSmallMax && c <= (uint32_t)large_max; };
+  // this lane's tiles: instances, large tiles, their chunks, huge tiles
+  uint32_t v1[1] = {0u}, v3[3] = {0u, 0u, 0u};
+  for (int i = b; i < e; ++i) {
+    const uint32_t c = count[i];
+    v1[0] += c;
+    if (is_large(c)) {
+      v3[0] += 1u;
+      v3[1] += chunks(c);
+    }
+    v3[2] += c > (uint32_t)large_max ? 1u : 0u;
   }
-  uint32_t run = tid == 0 ? 0u : s_part[tid - 1];
+  uint32_t tot1[1], tot3[3];
+  scan1024_excl(s_part, v1, tot1);
+  scan1024_excl(s_part, v3, tot3);
+  uint32_t run = v1[0], at = v3[0], cat = v3[1], hat = v3[2];
   for (int i = b; i < e; ++i) {
     const uint32_t c = count[i];
     start[i] = run;
@@ -585,56 +620,30 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
     ranges[2 * i] = c ? run : 0u;
     ranges[2 * i + 1] = c ? run + c : 0u;
     run += c;
+    if (is_large(c)) {
+      large[at] = (uint32_t)i;
+      lchunk_off[at] = cat;
+      ++at;
+      cat += chunks(c);
+    }
+    if (c > (uint32_t)large_max) large[T - 1 - (int)hat++] = (uint32_t)i;
   }
   if (tid == kScanThreads - 1) {
-    start[T] = s_part[kScanThreads - 1];
-    counters[0] = s_part[kScanThreads - 1];
-    if (s_part[kScanThreads - 1] > capacity) atomicOr(&counters[1], kErrCapacity);
+    start[T] = tot1[0];
+    counters[0] = tot1[0];
+    if (tot1[0] > capacity) atomicOr(&counters[1], kErrCapacity);
+    counters[3] = tot3[0];
+    counters[6] = tot3[1];
+    counters[7] = tot3[2];
+    lchunk_off[tot3[0]] = tot3[1];
   }
-  {
-    auto chunks = [](uint32_t c) { return (c + (uint32_t)kSmallMax - 1u) / (uint32_t)kSmallMax; };
-    auto is_large = [&](uint32_t c) { return c > (uint32_t)kSmallMax && c <= (uint32_t)large_max; };
-    uint32_t nl = 0, nc = 0, nh = 0;
-    for (int i = b; i < e; ++i) {
-      if (is_large(count[i])) {
-        nl += 1u;
-        nc += chunks(count[i]);
-      }
-      nh += count[i] > (uint32_t)large_max ? 1u : 0u;
-    }
-    uint32_t at = scan1024_excl(s_part, nl);
-    uint32_t cat = scan1024_excl(s_part, nc);
-    uint32_t hat = scan1024_excl(s_part, nh);
-    for (int i = b; i < e; ++i) {
-      if (is_large(count[i])) {
-        large[at] = (uint32_t)i;
-        lchunk_off[at] = cat;
-        ++at;
-        cat += chunks(count[i]);
-      }
-      if (count[i] > (uint32_t)large_max) large[T - 1 - (int)hat++] = (uint32_t)i;
-    }
-    if (tid == kScanThreads - 1) {
-      counters[3] = at;
-      counters[6] = cat;
-      counters[7] = hat;
-      lchunk_off[at] = cat;
-    }
-  }
-  __syncthreads();
+  // the per-workgroup instance totals of the preprocess, in place
   const int pb = (nb + kScanThreads - 1) / kScanThreads;
   const int b2 = tid * pb, e2 = min(nb, b2 + pb);
-  uint32_t loc = 0;
-  for (int i = b2; i < e2; ++i) loc += bsum[i];
-  s_part[tid] = loc;
-  __syncthreads();
-  for (int off = 1; off < kScanThreads; off <<= 1) {
-    const uint32_t v = tid >= off ? s_part[tid - off] : 0u;
-    __syncthreads();
-    s_part[tid] += v;
-    __syncthreads();
-  }
-  uint32_t run2 = tid == 0 ? 0u : s_part[tid - 1];
+  uint32_t w1[1] = {0u}, wt[1];
+  for (int i = b2; i < e2; ++i) w1[0] += bsum[i];
+  scan1024_excl(s_part, w1, wt);
+  uint32_t run2 = w1[0];
   for (int i = b2; i < e2; ++i) {
     const uint32_t c = bsum[i];
     bsum[i] = run2;
@@ -1026,7 +1035,10 @@ SR_DEV void sort_run(const SortArgs& a, uint32_t c0, int cn, uint64_t* s_keys) {
 // in sub-list q is likewise its own lpre plus the other chunks' hits of q
 // below it.  Keys are unique, so this is the merge of the sorted chunks,
 // and the lists equal those of sorting the whole bucket at once.
-constexpr uint32_t kChunkBlocks = 256;  // workgroups striding over the chunk items, per pass
+#ifndef SR_CHUNK_BLOCKS
+#define SR_CHUNK_BLOCKS 256
+#endif
+constexpr uint32_t kChunkBlocks = SR_CHUNK_BLOCKS;  // workgroups striding over the chunk items, per pass
 
 // Large tile and chunk of chunk item `it` (< counters[6]).
 SR_DEV void chunk_item(const SortArgs& a, uint32_t it, uint32_t& li, int& c) {
@@ -1547,7 +1559,10 @@ __global__ __launch_bounds__(64) void k_pair_offsets(AcceptArgs a) {
   const int nch = (cnt + 63) / 64;
   const uint32_t rbeg = a.ranges[2 * tile], ntile = a.ranges[2 * tile + 1] - rbeg;
   const size_t base = mask_base(Q, rbeg, ntile, tile, q) + lane;
-  constexpr int kU = 8;  // mask loads in flight
+#ifndef SR_PO_UNROLL
+#define SR_PO_UNROLL 8
+#endif
+  constexpr int kU = SR_PO_UNROLL;  // mask loads in flight
   uint32_t tot = 0;
   int cut = nch;  // first chunk the pixel cannot reach
   float trun = 1.0f;
@@ -1833,7 +1848,10 @@ SR_DEV void blend_fwd_stored(const BlendArgs& a, const ViewInfo& vi, int qi, int
   bool done = !inside;
   uint32_t f = rg.x;
   int left = (int)rg.y;
+  // slots past the run's end keep defined (stale) values; nothing uses them
   float4 f0[kPre], f1[kPre];
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) f0[k] = f1[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   auto fetch = [&](uint32_t at, int n) {
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
@@ -1856,37 +1874,39 @@ SR_DEV void blend_fwd_stored(const BlendArgs& a, const ViewInfo& vi, int qi, int
     f += kPre;
     left -= kPre;
     if (left > 0) fetch(f, left);  // next batch in flight
+    // Straight-line over the batch: every update is computed and kept only
+    // where the pair is live (in the run, the pixel not done) and does not
+    // saturate the pixel, so the body has no per-pair branch.
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
-      if (k < n && !done) {
-        // forward.cu:383-415: (alpha, depth, mm) in quad 0; mm = mcoef (1 - near / depth)
-        const float alpha = c0[k].x, depth = c0[k].y, mm = c0[k].z;
-        const float4 v = c1[k];
-        const float test_T = T * (1.0f - alpha);
-        if (test_T < 0.0001f) {
-          done = true;
-        } else {
-          const uint32_t contributor = __float_as_uint(v.z) + 1u;  // quad 2's z: position in the tile list
-          const float w = alpha * T;
-          const float A = 1.0f - T;
-          const float mm2 = mm * mm;
-          distortion += ((mm2 * A + a2) - (2.0f * mm) * a1) * w;  // role 0: a1 = M1, a2 = M2
-          const float x0 = role == 0 ? depth : v.x;
-          const float x1 = role == 0 ? mm : v.y;
-          const float x2 = role == 0 ? mm2 : v.z;
-          a0 += x0 * w;
-          a1 += x1 * w;
-          a2 += x2 * w;
-          a3 += v.w * w;
-          if (T > 0.5f) {
-            median_depth = depth;
-            median_contributor = (float)contributor;
-          }
-          T = test_T;
-          last_contributor = contributor;
-          ++blended;
-        }
-      }
+      // forward.cu:383-415: (alpha, depth, mm) in quad 0; mm = mcoef (1 - near / depth)
+      const float alpha = c0[k].x, depth = c0[k].y, mm = c0[k].z;
+      const float4 v = c1[k];
+      const float test_T = T * (1.0f - alpha);
+      const bool live = k < n && !done;
+      const bool sat = test_T < 0.0001f;
+      done = done || (live && sat);
+      const bool ok = live && !sat;
+      const uint32_t contributor = __float_as_uint(v.z) + 1u;  // quad 2's z: position in the tile list
+      const float w = alpha * T;
+      const float A = 1.0f - T;
+      const float mm2 = mm * mm;
+      const float dist = distortion + ((mm2 * A + a2) - (2.0f * mm) * a1) * w;  // role 0: a1 = M1, a2 = M2
+      const float x0 = role == 0 ? depth : v.x;
+      const float x1 = role == 0 ? mm : v.y;
+      const float x2 = role == 0 ? mm2 : v.z;
+      const float n0 = a0 + x0 * w, n1 = a1 + x1 * w, n2 = a2 + x2 * w, n3 = a3 + v.w * w;
+      distortion = ok ? dist : distortion;
+      a0 = ok ? n0 : a0;
+      a1 = ok ? n1 : a1;
+      a2 = ok ? n2 : a2;
+      a3 = ok ? n3 : a3;
+      const bool med = ok && T > 0.5f;
+      median_depth = med ? depth : median_depth;
+      median_contributor = med ? (float)contributor : median_contributor;
+      T = ok ? test_T : T;
+      last_contributor = ok ? contributor : last_contributor;
+      blended += ok ? 1 : 0;
     }
   }
   if (!inside) return;
