@@ -34,6 +34,14 @@ constexpr int kChainPre = SR_CHAIN_PREFETCH;  // stored pairs a lane fetches ahe
 // 8 lanes per pixel)
 constexpr int kSplitBwd = 8;
 constexpr int kChainSegMin = 16;  // shortest chain segment a lane replays
+#ifndef SR_GRAD_GROUP
+#define SR_GRAD_GROUP 4
+#endif
+// pixels per group in the per-pair gradient sums: an entry's partials are
+// summed over each run of kGradGroup blended pixels in pixel order, and
+// the runs' sums are added in order
+constexpr int kGradGroup = SR_GRAD_GROUP;
+static_assert(kGradGroup >= 1 && 64 % kGradGroup == 0, "groups tile the 64 pixels");
 
 // Stores the 18 partial sums (g(k) for slot k) of the sub-list entry at
 // tile-list position pos of quadrant q as five float4 in its (instance,
@@ -496,7 +504,8 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
       // entries from last to first, the wave in step (each lane still walks
       // its own pixel's entries back to front); an entry's 18 partial
       // gradients are then summed over the pixels that blended it in pixel
-      // order from 0, exactly as the per-pair gradient pass sums a stored
+      // order from 0, in runs of kGradGroup pixels whose sums are added in
+      // order, exactly as the per-pair gradient pass sums a stored
       // quadrant's, so the partials have the same bits whichever quadrants
       // missed the pair store
       for (int j = 63; j >= 0; --j) {
@@ -547,9 +556,18 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
 #pragma unroll
         for (int k = 0; k < kNumG; ++k) s_tmp[k][lane] = gk[k];
         __syncthreads();
-        if (lane < kNumG) {
-          float acc = 0.0f;
-          for (uint64_t mm = tookm; mm != 0ull; mm &= mm - 1ull) acc += s_tmp[lane][__builtin_ctzll(mm)];
+        if (lane < kNumG) {  // grouped exactly as the per-pair gradient pass sums
+          float acc = 0.0f, grp = 0.0f;
+          int in_grp = 0;
+          for (uint64_t mm = tookm; mm != 0ull; mm &= mm - 1ull) {
+            grp += s_tmp[lane][__builtin_ctzll(mm)];
+            if (++in_grp == kGradGroup) {
+              acc += grp;
+              grp = 0.0f;
+              in_grp = 0;
+            }
+          }
+          if (in_grp > 0) acc += grp;
           s_acc[lane][j] = acc;
         }
         __syncthreads();
@@ -601,18 +619,27 @@ SR_DEV uint64_t wave_transpose64(uint64_t x, int lane) {
 // Backward blend, pass 2 of 2.  The per-pair gradients, balanced over the
 // whole GPU on the acceptance work list: one wave per (quadrant, 64-entry
 // chunk).  A pair's gradients need no pixel recurrence any more (pass 1
-// stored its w, dL/dalpha and dL/ddepth), so the wave works entry-major:
-// lane j owns entry j of the chunk, keeps its record in registers and sums
-// its 18 partial gradients over the pixels that blended it, in registers;
-// the acceptance masks (one word per pixel) are transposed into one word of
-// pixels per entry, trimmed first to the pairs in front of each pixel's last
-// contributor.  Lane j stores its sums to entry j's (position, quadrant)
-// slot of the partial-gradient buffer.
+// stored its w, dL/dalpha and dL/ddepth).  The acceptance masks (one word
+// per pixel), trimmed to the pairs in front of each pixel's last
+// contributor, are transposed into one word of pixels per entry; each
+// entry's pixels are cut into groups of kGradGroup in pixel order, and the
+// groups of all entries are dealt over the lanes, 64 per round, so the
+// lanes stay busy however unevenly the entries cover the quadrant.  A lane
+// recomputes its group's pairs exactly and sums their 18 partial gradients
+// in pixel order; the entry's lane then adds its group sums in order.  Each
+// entry's sums go to its (position, quadrant) slot of the partial-gradient
+// buffer.  The recompute path of pass 1 groups its sums the same way, so
+// the partials have the same bits whichever quadrants missed the pair store.
 __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
   __shared__ uint4 s_pm[64];    // per pixel: trimmed mask (lo, hi), first stored pair of the chunk
   __shared__ float4 s_pg[64];   // per pixel: dL/dpixel (rgb), dL/dnormal.x
   __shared__ float2 s_pn[64];   // per pixel: dL/dnormal.yz
   __shared__ uint32_t s_pos[64];
+  __shared__ float4 s_e0[64], s_e1[64], s_e2[64];  // per entry: record quads 0..2
+  __shared__ unsigned long long s_col[64];         // per entry: the pixels that blended it
+  __shared__ uint32_t s_tbase[64];                 // per entry: its first group
+  __shared__ uint8_t s_towner[64 * (64 / kGradGroup)];  // entry of every group
+  __shared__ float s_grp[kNumG][64];               // group sums of the round
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const int lane = threadIdx.x;
   if (a.counters[1] & kErrCapacity) return;
@@ -660,6 +687,9 @@ __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
     }
     __syncthreads();  // the previous item's readers are done
     s_pos[lane] = v.y;
+    s_e0[lane] = q0;
+    s_e1[lane] = q1;
+    s_e2[lane] = q2;
     __syncthreads();
     if (mask) {
       // entries in front of the last contributor: positions rise along the
@@ -673,42 +703,75 @@ __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
     s_pm[lane] = make_uint4((uint32_t)mask, (uint32_t)(mask >> 32), foff, 0u);
     s_pg[lane] = pg;
     s_pn[lane] = pn;
-    uint64_t col = wave_transpose64(mask, lane);  // bit p: pixel p blended this lane's entry
+    const uint64_t col = wave_transpose64(mask, lane);  // bit p: pixel p blended this lane's entry
+    const uint32_t ng = ((uint32_t)__popcll(col) + (kGradGroup - 1)) / kGradGroup;
+    uint32_t tincl = ng;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)tincl, off, 64);
+      if (lane >= off) tincl += y;
+    }
+    const uint32_t tbase = tincl - ng, ntask = (uint32_t)__shfl((int)tincl, 63, 64);
+    s_col[lane] = col;
+    s_tbase[lane] = tbase;
+    for (uint32_t i = 0; i < ng; ++i) s_towner[tbase + i] = (uint8_t)lane;
     __syncthreads();
     float acc[kNumG];
 #pragma unroll
     for (int k = 0; k < kNumG; ++k) acc[k] = 0.0f;
-    const uint64_t below = (1ull << lane) - 1ull;  // this entry's rank within a pixel's mask
-    while (col != 0ull) {
-      int pp[kPre];
-      float4 fw[kPre];
+    for (uint32_t base = 0; base < ntask; base += 64) {  // uniform
+      const uint32_t t = base + (uint32_t)lane;
+      float g[kNumG];
 #pragma unroll
-      for (int k = 0; k < kPre; ++k) {
-        pp[k] = -1;
-        if (col != 0ull) {
-          const int p = __builtin_ctzll(col);
-          col &= col - 1ull;
-          pp[k] = p;
-          const uint4 pm = s_pm[p];
-          const uint64_t mp = (uint64_t)pm.x | ((uint64_t)pm.y << 32);
-          const uint32_t f = pm.z + (uint32_t)__popcll(mp & below);
-          fw[k] = a.ps.fW[f];  // (w, dL/dalpha, dL/ddepth)
+      for (int k = 0; k < kNumG; ++k) g[k] = 0.0f;
+      if (t < ntask) {
+        const int j = s_towner[t];
+        const uint32_t gi = t - s_tbase[j];
+        const uint64_t cj = s_col[j];
+        // the group's pixels: the set bits of cj from the (gi kGradGroup)-th on
+        const int first = select_bit(cj, gi * kGradGroup);
+        uint64_t mm = cj & ~((1ull << first) - 1ull);
+        const uint64_t below = (1ull << j) - 1ull;  // the entry's rank within a pixel's mask
+        int pp[kGradGroup];
+        float4 fw[kGradGroup];
+#pragma unroll
+        for (int k = 0; k < kGradGroup; ++k) {
+          pp[k] = -1;
+          if (mm != 0ull) {
+            const int p = __builtin_ctzll(mm);
+            mm &= mm - 1ull;
+            pp[k] = p;
+            const uint4 pm = s_pm[p];
+            const uint64_t mp = (uint64_t)pm.x | ((uint64_t)pm.y << 32);
+            fw[k] = a.ps.fW[pm.z + (uint32_t)__popcll(mp & below)];  // (w, dL/dalpha, dL/ddepth)
+          }
+        }
+        const float4 r0 = s_e0[j], r1 = s_e1[j], r2 = s_e2[j];
+#pragma unroll
+        for (int k = 0; k < kGradGroup; ++k) {
+          const int p = pp[k];
+          if (p < 0) continue;
+          const float4 g4 = s_pg[p];
+          const float2 n2 = s_pn[p];
+          const float pixx = (float)(bx + (p & 7)), pixy = (float)(by + (p >> 3));
+          // the pair itself (s, G, planes) recomputed exactly as the
+          // acceptance pass evaluated it: cheaper than fetching it
+          Pair e;
+          (void)eval_pair<false>(pixx, pixy, r0, r1, r2, 0.0f, e);
+          pair_grads(g4.x, g4.y, g4.z, g4.w, n2.x, n2.y, e, r1, r2.w, fw[k].x, fw[k].y, fw[k].z, pixx, pixy,
+                     [&](int s, float val) { g[s] += val; });
         }
       }
 #pragma unroll
-      for (int k = 0; k < kPre; ++k) {
-        const int p = pp[k];
-        if (p < 0) continue;
-        const float4 g4 = s_pg[p];
-        const float2 n2 = s_pn[p];
-        const float pixx = (float)(bx + (p & 7)), pixy = (float)(by + (p >> 3));
-        // the pair itself (s, G, planes) recomputed exactly as the acceptance
-        // pass evaluated it: cheaper than fetching it from the pair store
-        Pair e;
-        (void)eval_pair<false>(pixx, pixy, q0, q1, q2, 0.0f, e);
-        pair_grads(g4.x, g4.y, g4.z, g4.w, n2.x, n2.y, e, q1, q2.w, fw[k].x, fw[k].y, fw[k].z, pixx, pixy,
-                   [&](int s, float val) { acc[s] += val; });
+      for (int k = 0; k < kNumG; ++k) s_grp[k][lane] = g[k];
+      __syncthreads();
+      // the entry's lane adds its groups of this round, in order
+      const uint32_t lo = max(tbase, base), hi = min(tbase + ng, base + 64u);
+      for (uint32_t u = lo; u < hi; ++u) {
+#pragma unroll
+        for (int k = 0; k < kNumG; ++k) acc[k] += s_grp[k][u - base];
       }
+      __syncthreads();
     }
     if (has) store_part(a, rbeg + v.y, q, [&](int k) { return acc[k]; });
   }

@@ -70,6 +70,25 @@ SR_DEV uint32_t f2u(float x) {
 }
 
 SR_DEV float fmin_(float a, float b) { return (b < a) ? b : a; }
+
+// The k-th set bit (k < popcount) of a 64-bit mask, by halving.
+SR_DEV int select_bit(uint64_t m, uint32_t k) {
+  int at = 0;
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) {
+    const uint64_t low = m & ((1ull << w) - 1ull);
+    const uint32_t n = (uint32_t)__popcll(low);
+    if (k >= n) {
+      k -= n;
+      m >>= w;
+      at += w;
+    } else {
+      m = low;
+    }
+  }
+  return at;
+}
+
 SR_DEV float fmax_(float a, float b) { return (a < b) ? b : a; }
 
 // getRect, dsr/cuda_rasterizer/auxiliary.h:67-77, with tile edge B.

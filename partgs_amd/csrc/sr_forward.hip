@@ -1696,24 +1696,6 @@ hipError_t launch_pair_offsets(const AcceptArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// The k-th set bit (k < popcount) of a 64-bit mask, by halving.
-SR_DEV int select_bit(uint64_t m, uint32_t k) {
-  int at = 0;
-#pragma unroll
-  for (int w = 32; w > 0; w >>= 1) {
-    const uint64_t low = m & ((1ull << w) - 1ull);
-    const uint32_t n = (uint32_t)__popcll(low);
-    if (k >= n) {
-      k -= n;
-      m >>= w;
-      at += w;
-    } else {
-      m = low;
-    }
-  }
-  return at;
-}
-
 // Pair-store fill: balanced like the acceptance pass, one wave per
 // (quadrant, chunk).  The chunk's accepted pairs are dealt out 64 per step
 // in pixel order (an LDS table names the pixel of every pair slot; the
