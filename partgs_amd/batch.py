@@ -319,6 +319,127 @@ def capture_step(br: BatchRasterizer, means3D, shs, colors, opacities, scales, r
                        grads, g)
 
 
+# ---- view groups sharing one workspace ---------------------------------------------
+class ViewGroups:
+    """The views of a batch rendered in consecutive groups that share one
+    workspace.  Each group's forward and backward run before the next group's
+    forward, so the workspace holds one group's state at a time (the largest
+    group's size instead of the whole batch's); per-surfel gradients are
+    summed over the groups, screen-space statistics and images stay per view.
+    Per view, every output equals the one-batch result; the per-surfel
+    gradient sums are regrouped (one addition per group)."""
+
+    def __init__(self, cameras, bg: torch.Tensor, sh_degree: int, groups: int = 2,
+                 config: BatchConfig | None = None, scale_modifier: float = 1.0, device="cuda"):
+        V = len(cameras)
+        if not 1 <= groups <= V:
+            raise ValueError(f"1..{V} groups for {V} views (got {groups})")
+        bounds = [V * g // groups for g in range(groups + 1)]
+        self.view_base = bounds[:-1]
+        self.parts = [BatchRasterizer(cameras[a:b], bg, sh_degree, config, scale_modifier, device=device)
+                      for a, b in zip(bounds[:-1], bounds[1:])]
+        self.V = V
+        self.device = self.parts[0].device
+        self.pix_base = []
+        n = 0
+        for br in self.parts:
+            self.pix_base.append(n)
+            n += br.N
+        self.N = n
+        self.workspace = None
+
+    def share_workspace(self, P: int, sizes) -> None:
+        """One workspace for every group: sizes[g] = (capacity, pair capacity)."""
+        need = max(br.workspace_bytes(P, c, pc) for br, (c, pc) in zip(self.parts, sizes))
+        if need == 0:
+            raise RuntimeError("invalid batch description")
+        for br in self.parts:
+            br.workspace = None
+        if self.workspace is None or self.workspace.numel() < need:
+            self.workspace = None
+            self.workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
+        for br, (c, pc) in zip(self.parts, sizes):
+            br.workspace = self.workspace
+            br.capacity = int(c)
+            br.pair_capacity = int(pc)
+            br._P = P
+
+    def ok(self) -> bool:
+        return all(not (br.read_status()[1] & _lib.SR_STATUS_CAPACITY) for br in self.parts)
+
+
+@dataclass
+class GroupedStep:
+    """capture_grouped_step's graph and buffers: outputs hold every view
+    (flat colour and aux planes in view order, radii V x P), grads the
+    per-surfel sums over all views and dL_dmeans2D per view."""
+
+    vg: ViewGroups
+    outputs: tuple
+    grads: dict
+    graph: torch.cuda.CUDAGraph
+
+    def replay(self):
+        self.graph.replay()
+
+    def ok(self) -> bool:
+        return self.vg.ok()
+
+
+def capture_grouped_step(vg: ViewGroups, means3D, shs, colors, opacities, scales, rotations, dL_dcolor,
+                         dL_dothers, headroom: float = 1.25) -> GroupedStep:
+    """capture_step over view groups: every group's forward and backward in
+    one hipGraph, on one shared workspace.  dL_dcolor / dL_dothers are laid
+    out like the whole batch's outputs (each group reads its own slice)."""
+    P = means3D.shape[0]
+    M = shs.shape[1] if shs is not None else 0
+    sizes = []
+    for br in vg.parts:  # size each group once, eagerly, one workspace at a time
+        br.forward_checked(means3D, shs, colors, opacities, scales, rotations)
+        need, _ = br.read_status()
+        cap = max(br.capacity, int(need * headroom) + 1024)
+        sizes.append((cap, br._auto_pairs(cap, int(br.read_pairs() * headroom))))
+        br.workspace = None
+    vg.share_workspace(P, sizes)
+    f32 = dict(dtype=torch.float32, device=vg.device)
+    color = torch.empty(3 * vg.N, **f32)
+    others = torch.empty(7 * vg.N, **f32)
+    radii = torch.empty((vg.V, P), dtype=torch.int32, device=vg.device)
+    total = {"means2D": torch.empty((vg.V, P, 3), **f32), "colors": torch.empty((P, 3), **f32),
+             "opacities": torch.empty((P, 1), **f32), "means3D": torch.empty((P, 3), **f32),
+             "shs": torch.empty((P, M, 3), **f32), "scales": torch.empty((P, 2), **f32),
+             "rotations": torch.empty((P, 4), **f32)}
+    summed = [k for k in total if k != "means2D"]
+    extra = {k: torch.empty_like(total[k]) for k in summed}
+
+    def run():
+        for g, br in enumerate(vg.parts):
+            a, n = vg.pix_base[g], br.N
+            v0 = vg.view_base[g]
+            outs = (color[3 * a: 3 * (a + n)], others[7 * a: 7 * (a + n)], radii[v0: v0 + br.V])
+            br.forward_raw(means3D, shs, colors, opacities, scales, rotations, *outs)
+            grads = dict(total if g == 0 else extra)
+            grads["means2D"] = total["means2D"][v0: v0 + br.V]
+            br.backward_raw(means3D, shs, colors, opacities, scales, rotations, outs[2],
+                            dL_dcolor[3 * a: 3 * (a + n)], dL_dothers[7 * a: 7 * (a + n)], grads)
+            if g > 0:
+                for k in summed:
+                    if total[k].numel():
+                        total[k].add_(extra[k])
+
+    _lib.set_timing(False)  # no event records inside a capture
+    s = torch.cuda.Stream(device=vg.device)
+    s.wait_stream(torch.cuda.current_stream(vg.device))
+    with torch.cuda.stream(s):  # warm-up on the capture stream
+        run()
+    torch.cuda.current_stream(vg.device).wait_stream(s)
+    torch.cuda.synchronize(vg.device)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        run()
+    return GroupedStep(vg, (color, others, radii), total, graph)
+
+
 # ---- autotuner ---------------------------------------------------------------------
 _CACHE: dict = {}
 
