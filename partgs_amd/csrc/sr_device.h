@@ -89,6 +89,12 @@ SR_DEV int select_bit(uint64_t m, uint32_t k) {
   return at;
 }
 
+// The value of lane 4k in lanes 4k .. 4k + 3 (a DPP quad permutation; all
+// four lanes must be active).
+SR_DEV float quad_bcast0(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x00, 0xF, 0xF, false));
+}
+
 SR_DEV float fmax_(float a, float b) { return (a < b) ? b : a; }
 
 // getRect, dsr/cuda_rasterizer/auxiliary.h:67-77, with tile edge B.

@@ -1569,11 +1569,19 @@ __global__ __launch_bounds__(64) void k_pair_offsets(AcceptArgs a) {
   for (int c0 = 0; c0 < nch; c0 += kU) {
     uint64_t mk[kU];
     float tk[kU];
+    // unconditional loads (chunk index clamped into range) so all 2 kU are
+    // in flight together; the values past the end are then dropped
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const size_t cc = (size_t)min(c0 + k, nch - 1) * 64;
+      mk[k] = a.masks[base + cc];
+      tk[k] = a.chunk_T[base + cc];
+    }
 #pragma unroll
     for (int k = 0; k < kU; ++k) {
       const bool in = c0 + k < nch;
-      mk[k] = in ? a.masks[base + (size_t)(c0 + k) * 64] : 0ull;
-      tk[k] = in ? a.chunk_T[base + (size_t)(c0 + k) * 64] : 1.0f;
+      mk[k] = in ? mk[k] : 0ull;
+      tk[k] = in ? tk[k] : 1.0f;
     }
 #pragma unroll
     for (int k = 0; k < kU; ++k) {
@@ -1616,7 +1624,9 @@ __global__ __launch_bounds__(64) void k_pair_offsets(AcceptArgs a) {
   for (int c0 = 0; c0 < nch; c0 += kU) {
     uint64_t mk[kU];
 #pragma unroll
-    for (int k = 0; k < kU; ++k) mk[k] = (c0 + k < cut) ? a.masks[base + (size_t)(c0 + k) * 64] : 0ull;
+    for (int k = 0; k < kU; ++k) mk[k] = a.masks[base + (size_t)min(c0 + k, nch - 1) * 64];
+#pragma unroll
+    for (int k = 0; k < kU; ++k) mk[k] = (c0 + k < cut) ? mk[k] : 0ull;
 #pragma unroll
     for (int k = 0; k < kU; ++k) {
       if (c0 + k < nch) a.frag_off[base + (size_t)(c0 + k) * 64] = run;
@@ -1795,7 +1805,7 @@ hipError_t launch_pair_write(const AcceptArgs& a, size_t max_items, hipStream_t 
 // full tile list) come out bit-identical.
 // ---------------------------------------------------------------------------
 #ifndef SR_FWD_PREFETCH
-#define SR_FWD_PREFETCH 8
+#define SR_FWD_PREFETCH 20
 #endif
 constexpr int kPre = SR_FWD_PREFETCH;  // stored pairs a lane fetches ahead in the blend kernels
 
@@ -1821,7 +1831,11 @@ SR_DEV void blend_fwd_stored(const BlendArgs& a, const ViewInfo& vi, int qi, int
   // lane_range is written by the pair-offset pass, which runs only when the
   // view has instances: read it for non-empty quadrants only
   const uint2 rg = (inside && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
-  const int sel = role == 1 ? 1 : 2;  // the record quad with this role's values
+  // Each lane loads one record quad per pair: role 0 quad 0 (alpha, depth,
+  // mm), role 1 quad 1 (normal, red), roles 2 and 3 quad 2 (green, blue,
+  // tile position); alpha and depth reach the other lanes of the pixel from
+  // role 0 by a quad broadcast.
+  const int qsel = role < 2 ? role : 2;
   float T = 1.0f;
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, distortion = 0.0f, median_depth = 0.0f;
   float median_contributor = -1.0f;
@@ -1831,27 +1845,19 @@ SR_DEV void blend_fwd_stored(const BlendArgs& a, const ViewInfo& vi, int qi, int
   uint32_t f = rg.x;
   int left = (int)rg.y;
   // slots past the run's end keep defined (stale) values; nothing uses them
-  float4 f0[kPre], f1[kPre];
+  float4 fq[kPre];
 #pragma unroll
-  for (int k = 0; k < kPre; ++k) f0[k] = f1[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int k = 0; k < kPre; ++k) fq[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   auto fetch = [&](uint32_t at, int n) {
 #pragma unroll
-    for (int k = 0; k < kPre; ++k) {
-      if (k < n) {
-        const float4* pr = a.ps.rec + kPairQuads * (size_t)(at + k);
-        f0[k] = pr[0];
-        f1[k] = pr[sel];
-      }
-    }
+    for (int k = 0; k < kPre; ++k)
+      if (k < n) fq[k] = a.ps.rec[kPairQuads * (size_t)(at + k) + qsel];
   };
   fetch(f, left);
-  while (left > 0 && !done) {
-    float4 c0[kPre], c1[kPre];
+  while (left > 0 && !done) {  // the four lanes of a pixel share left and done
+    float4 cq[kPre];
 #pragma unroll
-    for (int k = 0; k < kPre; ++k) {
-      c0[k] = f0[k];
-      c1[k] = f1[k];
-    }
+    for (int k = 0; k < kPre; ++k) cq[k] = fq[k];
     const int n = left;
     f += kPre;
     left -= kPre;
@@ -1862,8 +1868,8 @@ SR_DEV void blend_fwd_stored(const BlendArgs& a, const ViewInfo& vi, int qi, int
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
       // forward.cu:383-415: (alpha, depth, mm) in quad 0; mm = mcoef (1 - near / depth)
-      const float alpha = c0[k].x, depth = c0[k].y, mm = c0[k].z;
-      const float4 v = c1[k];
+      const float4 v = cq[k];
+      const float alpha = quad_bcast0(v.x), depth = quad_bcast0(v.y), mm = v.z;
       const float test_T = T * (1.0f - alpha);
       const bool live = k < n && !done;
       const bool sat = test_T < 0.0001f;
