@@ -933,16 +933,26 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
   const int tx = ltile % vi.gx, ty = ltile / vi.gx;
   const int tid = threadIdx.x;
   uint32_t inst[K], rid[K], hm[K];  // instance number: the key's low half
+  // the gathers are unconditional (entries past the run read the run's
+  // first entry) so each round has all K loads in flight
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = tid * K + k;
-    inst[k] = i < cn ? (uint32_t)s_keys[lds_slot<K>(i)] : 0u;
-    rid[k] = i < cn ? a.inst_rec[inst[k]] : 0u;
+    inst[k] = (uint32_t)s_keys[lds_slot<K>(i < cn ? i : 0)];
   }
+#pragma unroll
+  for (int k = 0; k < K; ++k) rid[k] = a.inst_rec[inst[k]];
   short4 bx[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k)
-    bx[k] = tid * K + k < cn ? a.cullbox[rid[k]] : make_short4(32767, 32767, -32768, -32768);
+  for (int k = 0; k < K; ++k) bx[k] = a.cullbox[rid[k]];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (tid * K + k >= cn) {
+      inst[k] = 0u;
+      rid[k] = 0u;
+      bx[k] = make_short4(32767, 32767, -32768, -32768);
+    }
+  }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = tid * K + k;
@@ -1012,10 +1022,15 @@ SR_DEV void sort_run(const SortArgs& a, uint32_t c0, int cn, uint64_t* s_keys) {
   constexpr int K = kSmallMax / kSortThreads;
   const int tid = threadIdx.x;
   const int S = max(next_pow2(cn), K);
+  // all K loads in flight at once: unconditional (index clamped into the
+  // run), the padding applied afterwards
+  uint64_t kv[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {  // all K loads in flight at once
+  for (int k = 0; k < K; ++k) kv[k] = a.keys[c0 + (uint32_t)min(tid + k * kSortThreads, cn - 1)];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
     const int i = tid + k * kSortThreads;
-    if (i < S) s_keys[lds_slot<K>(i)] = i < cn ? a.keys[c0 + i] : ~0ull;
+    if (i < S) s_keys[lds_slot<K>(i)] = i < cn ? kv[k] : ~0ull;
   }
   __syncthreads();
   merge_sort_lds<kSortThreads, K>(s_keys, S);
@@ -1186,8 +1201,15 @@ SR_DEV void sort_huge_tile(const SortArgs& a, int tile, uint64_t* s_keys) {
   for (int c0 = 0; c0 < n; c0 += kChunk) {
     const int cn = min(kChunk, n - c0);
     const int S = max(next_pow2(cn), K);
+    uint64_t kv[K];  // all loads in flight together (index clamped into the chunk)
+#pragma unroll
+    for (int k = 0; k < K; ++k) kv[k] = src[c0 + min(tid + k * NT, cn - 1)];
     __syncthreads();
-    for (int i = tid; i < S; i += NT) s_keys[lds_slot<K>(i)] = i < cn ? src[c0 + i] : ~0ull;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = tid + k * NT;
+      if (i < S) s_keys[lds_slot<K>(i)] = i < cn ? kv[k] : ~0ull;
+    }
     __syncthreads();
     merge_sort_lds<NT, K>(s_keys, S);
     for (int i = tid; i < cn; i += NT) src[c0 + i] = s_keys[lds_slot<K>(i)];
