@@ -587,6 +587,7 @@ size_t sr_state_offset(const char* name, int P, int W, int H, int tile, int L) {
   if (n == "sub_count") return im.sub_count;
   if (n == "cullbox") return g.cullbox;
   if (n == "masks") return b.masks;
+  if (n == "lane_range") return im.lane_range;
   return (size_t)-1;
 }
 
