@@ -80,6 +80,18 @@ typedef struct {
                               forward asked for, so a caller can size the next
                               one from it.  The backward must be given the
                               same value. */
+  int slot_capacity;       /* with capacity > 0: the partial-gradient slots
+                              the binning buffer holds (one per tile instance
+                              and 8 x 8 quadrant its cull box reaches); 0:
+                              every (instance, quadrant) of the capacity,
+                              which always fits.  More slots than this return
+                              SR_ERR_CAPACITY with *num_rendered within the
+                              capacity (call again with a larger value or 0).
+                              With capacity 0 the slots are sized from the
+                              exact count and this is not used.
+                              sr_slot_count reports what a forward needed;
+                              the backward does not need the value.  No
+                              counterpart in the reference. */
 } sr_settings;
 
 /* Per-surfel inputs (device).  Exactly one of shs / colors_precomp and exactly
@@ -169,7 +181,8 @@ int sr_mark_visible(int P, const float* means3D, const float* viewmatrix,
 /* Byte sizes of the three state buffers (what sr_forward asks alloc for). */
 size_t sr_geom_bytes(int P);
 size_t sr_image_bytes(int W, int H, int tile);
-size_t sr_binning_bytes(int num_rendered, int W, int H, int tile, int pair_store, int pair_capacity);
+size_t sr_binning_bytes(int num_rendered, int W, int H, int tile, int pair_store, int pair_capacity,
+                        int slot_capacity);
 
 /* Accepted pairs (*pairs) and tile instances (*instances) of the calling
  * thread's latest completed sr_forward on the current device: the pairs are
@@ -180,6 +193,12 @@ size_t sr_binning_bytes(int num_rendered, int W, int H, int tile, int pair_store
  * the next forward's pair_capacity from them.  No counterpart in the
  * reference. */
 int sr_pair_count(unsigned long long* pairs, unsigned int* instances);
+
+/* Partial-gradient slots (*slots) and tile instances (*instances) of the
+ * same latest completed forward, under the same rules as sr_pair_count; a
+ * caller sizes the next forward's slot_capacity from them.  No counterpart
+ * in the reference. */
+int sr_slot_count(unsigned int* slots, unsigned int* instances);
 
 /* Byte offsets of the named arrays inside the state buffers, so callers and
  * tests can read internal state (e.g. "depths", "rec", "tiles_touched",
@@ -231,16 +250,23 @@ typedef struct {
   int tile;                   /* 8, 16 or 32 */
   int pair_store;             /* as in sr_settings */
   int pair_capacity;          /* as in sr_settings (pairs over all views) */
+  int slot_capacity;          /* partial-gradient slots over all views; 0: every
+                                 (instance, quadrant) of the capacity.  More
+                                 slots set SR_STATUS_CAPACITY with status[0]
+                                 within the capacity; status[8] holds the
+                                 slots needed.  Forward and backward must be
+                                 given the same value. */
 } sr_views;
 
 size_t sr_batch_workspace_bytes(const sr_views* views, int P, int capacity);
 
 /* Outputs: out_color / out_others hold each view's 3 x H x W / 7 x H x W
  * planes back to back in view order (view v starts at 3 / 7 times the pixel
- * count of views 0..v-1); radii is V x P; status (device, 8 words) receives
+ * count of views 0..v-1); radii is V x P; status (device, 16 words) receives
  * [0] instances, [1] flags (SR_STATUS_CAPACITY), [2] acceptance work items,
  * [4..5] the accepted pairs asked of the pair store (64-bit, low word first;
- * see sr_pair_count); the other words are internal.
+ * see sr_pair_count), [8] the partial-gradient slots; the other words are
+ * internal.
  * The surfels need scales and rotations (no precomputed T, which is per
  * view); prefiltered is not supported. */
 int sr_forward_batched(const sr_views* views, const sr_surfels* g, void* workspace, size_t workspace_bytes,

@@ -66,6 +66,7 @@ EXPORTED = (
     "sr_forward_batched",
     "sr_backward_batched",
     "sr_pair_count",
+    "sr_slot_count",
     "sr_forward_part",
     "sr_backward_part",
 )
@@ -92,6 +93,7 @@ class SrSettings(C.Structure):
         ("pair_store", C.c_int),
         ("capacity", C.c_int),
         ("pair_capacity", C.c_int),
+        ("slot_capacity", C.c_int),
     ]
 
 
@@ -138,6 +140,7 @@ class SrViews(C.Structure):
         ("tile", C.c_int),
         ("pair_store", C.c_int),
         ("pair_capacity", C.c_int),
+        ("slot_capacity", C.c_int),
     ]
 
 
@@ -175,7 +178,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_geom_bytes.restype = sz
         lib.sr_image_bytes.argtypes = [i, i, i]
         lib.sr_image_bytes.restype = sz
-        lib.sr_binning_bytes.argtypes = [i, i, i, i, i, i]
+        lib.sr_binning_bytes.argtypes = [i, i, i, i, i, i, i]
         lib.sr_binning_bytes.restype = sz
         lib.sr_state_offset.argtypes = [C.c_char_p, i, i, i, i, i]
         lib.sr_state_offset.restype = sz
@@ -207,6 +210,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         lib.sr_version.argtypes = []
         lib.sr_version.restype = C.c_char_p
         lib.sr_pair_count.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint)]
+        lib.sr_slot_count.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_uint)]
+        lib.sr_slot_count.restype = i
         lib.sr_forward_part.argtypes = [C.POINTER(SrSettings), C.POINTER(SrSurfels), i, vp, ALLOC_FN, vp, vp, vp,
                                         vp, vp, C.POINTER(C.c_int), vp]
         lib.sr_forward_part.restype = i
@@ -229,12 +234,18 @@ def load(path: str = LIB_PATH) -> C.CDLL:
 
 
 class CapacityError(RuntimeError):
-    """More tile instances than the binning capacity the caller gave."""
+    """More tile instances than the binning capacity the caller gave, or,
+    with the instances within it (slots True), more partial-gradient slots
+    than its slot capacity."""
 
-    def __init__(self, needed: int, capacity: int):
-        super().__init__(f"{needed} tile instances exceed the binning capacity {capacity}")
+    def __init__(self, needed: int, capacity: int, slots: bool = False):
+        if slots:
+            super().__init__(f"the partial-gradient slots exceed the slot capacity ({needed} tile instances)")
+        else:
+            super().__init__(f"{needed} tile instances exceed the binning capacity {capacity}")
         self.needed = needed
         self.capacity = capacity
+        self.slots = slots
 
 
 def check(rc: int) -> None:
@@ -269,6 +280,16 @@ def pair_count():
     n = C.c_ulonglong(0)
     L = C.c_uint(0)
     if load().sr_pair_count(C.byref(n), C.byref(L)) != SR_OK:
+        return None
+    return int(n.value), int(L.value)
+
+
+def slot_count():
+    """(partial-gradient slots, tile instances) of the same forward as
+    pair_count, or None before one completed (never waits)."""
+    n = C.c_uint(0)
+    L = C.c_uint(0)
+    if load().sr_slot_count(C.byref(n), C.byref(L)) != SR_OK:
         return None
     return int(n.value), int(L.value)
 

@@ -71,8 +71,9 @@ class BatchRasterizer:
         self.N = sum(self.pixels)
         self.capacity = int(capacity) if capacity else 0
         self.pair_capacity = 0
+        self.slot_capacity = 0
         self.workspace = None
-        self.status = torch.zeros(8, dtype=torch.int32, device=self.device)
+        self.status = torch.zeros(16, dtype=torch.int32, device=self.device)
         self._P = None
         self._host = dict(w=(C.c_int * self.V)(*self.widths), h=(C.c_int * self.V)(*self.heights),
                           tx=(C.c_float * self.V)(*self.tanx), ty=(C.c_float * self.V)(*self.tany))
@@ -92,6 +93,7 @@ class BatchRasterizer:
         v.tile = self.config.tile
         v.pair_store = self.config.pair_store
         v.pair_capacity = self.pair_capacity
+        v.slot_capacity = self.slot_capacity
         return v
 
     @staticmethod
@@ -108,14 +110,17 @@ class BatchRasterizer:
         g.transMat_precomp = None
         return g
 
-    def workspace_bytes(self, P: int, capacity: int, pair_capacity: int | None = None) -> int:
-        saved = self.pair_capacity
+    def workspace_bytes(self, P: int, capacity: int, pair_capacity: int | None = None,
+                        slot_capacity: int | None = None) -> int:
+        saved = self.pair_capacity, self.slot_capacity
         if pair_capacity is not None:
             self.pair_capacity = int(pair_capacity)
+        if slot_capacity is not None:
+            self.slot_capacity = int(slot_capacity)
         try:
             return int(self.lib.sr_batch_workspace_bytes(C.byref(self._views()), P, capacity))
         finally:
-            self.pair_capacity = saved
+            self.pair_capacity, self.slot_capacity = saved
 
     def _auto_pairs(self, capacity: int, pairs: int | None = None) -> int:
         """Pair-store size for a capacity: 0 (pair_store per instance) when
@@ -126,11 +131,13 @@ class BatchRasterizer:
         n = pairs if pairs is not None else 8 * capacity
         return min(int(n * 1.125) + 4096, 0x7FFFFFFF)
 
-    def ensure_workspace(self, P: int, capacity: int | None = None, pair_capacity: int | None = None) -> None:
+    def ensure_workspace(self, P: int, capacity: int | None = None, pair_capacity: int | None = None,
+                         slot_capacity: int | None = None) -> None:
         """Sizes the workspace for P surfels, a capacity of (surfel, tile)
-        instances and an accepted-pair store.  With no capacity known yet it
-        starts at 1: the first forward then stops after the tile histogram
-        and reports the exact count, which the next call allocates for."""
+        instances, an accepted-pair store and partial-gradient slots (0: one
+        per instance and quadrant).  With no capacity known yet it starts at
+        1: the first forward then stops after the tile histogram and reports
+        the exact count, which the next call allocates for."""
         cap = int(capacity or self.capacity or 1)
         if pair_capacity is not None:
             pcap = int(pair_capacity)
@@ -138,14 +145,24 @@ class BatchRasterizer:
             pcap = self.pair_capacity  # keep a store sized from a measured count
         else:
             pcap = self._auto_pairs(cap)
-        need = self.workspace_bytes(P, cap, pcap)
+        if slot_capacity is not None:
+            scap = int(slot_capacity)
+        elif cap == self.capacity:
+            scap = self.slot_capacity
+        else:
+            scap = 0
+        need = self.workspace_bytes(P, cap, pcap, scap)
         if need == 0:
             raise RuntimeError("invalid batch description")
-        if self.workspace is None or self.workspace.numel() < need:
+        # reallocated when too small, and when over a quarter too large (the
+        # sizes measured after a first forward are usually far below its
+        # guesses); a captured graph keeps the workspace it was recorded on
+        if self.workspace is None or not need <= self.workspace.numel() <= need + need // 4:
             self.workspace = None
             self.workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
         self.capacity = cap
         self.pair_capacity = pcap
+        self.slot_capacity = scap
         self._P = P
 
     # -- raw passes (graph-capturable) ---------------------------------------------
@@ -195,6 +212,10 @@ class BatchRasterizer:
         st = self.status.cpu()
         return (int(st[4]) & 0xFFFFFFFF) | ((int(st[5]) & 0xFFFFFFFF) << 32)
 
+    def read_slots(self) -> int:
+        """Partial-gradient slots the last forward needed."""
+        return int(self.status.cpu()[8]) & 0xFFFFFFFF
+
     def forward_checked(self, means3D, shs, colors, opacities, scales, rotations):
         """Forward that grows the workspace until the batch fits (host sync)."""
         P = means3D.shape[0]
@@ -205,7 +226,10 @@ class BatchRasterizer:
             need, flags = self.read_status()
             if not flags & _lib.SR_STATUS_CAPACITY:
                 return out
-            self.ensure_workspace(P, int(need * 1.1) + 1024)
+            if need <= self.capacity:  # the instances fit, the slots did not
+                self.ensure_workspace(P, self.capacity, self.pair_capacity, int(self.read_slots() * 1.1) + 1024)
+            else:  # the scan counted the slots too
+                self.ensure_workspace(P, int(need * 1.1) + 1024, None, int(self.read_slots() * 1.1) + 1024)
         raise RuntimeError("batched rasteriser: capacity did not converge")
 
     # -- views of the flat outputs ---------------------------------------------------
@@ -299,7 +323,7 @@ def capture_step(br: BatchRasterizer, means3D, shs, colors, opacities, scales, r
     need, _ = br.read_status()
     cap = max(br.capacity, int(need * headroom) + 1024)
     pairs = br.read_pairs()
-    br.ensure_workspace(P, cap, br._auto_pairs(cap, int(pairs * headroom)))
+    br.ensure_workspace(P, cap, br._auto_pairs(cap, int(pairs * headroom)), int(br.read_slots() * headroom) + 1024)
     outs = br.new_outputs(P)
     M = shs.shape[1] if shs is not None else 0
     grads = br.new_grads(P, M)
@@ -349,8 +373,9 @@ class ViewGroups:
         self.workspace = None
 
     def share_workspace(self, P: int, sizes) -> None:
-        """One workspace for every group: sizes[g] = (capacity, pair capacity)."""
-        need = max(br.workspace_bytes(P, c, pc) for br, (c, pc) in zip(self.parts, sizes))
+        """One workspace for every group: sizes[g] = (capacity, pair capacity,
+        slot capacity)."""
+        need = max(br.workspace_bytes(P, c, pc, sc) for br, (c, pc, sc) in zip(self.parts, sizes))
         if need == 0:
             raise RuntimeError("invalid batch description")
         for br in self.parts:
@@ -358,10 +383,11 @@ class ViewGroups:
         if self.workspace is None or self.workspace.numel() < need:
             self.workspace = None
             self.workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
-        for br, (c, pc) in zip(self.parts, sizes):
+        for br, (c, pc, sc) in zip(self.parts, sizes):
             br.workspace = self.workspace
             br.capacity = int(c)
             br.pair_capacity = int(pc)
+            br.slot_capacity = int(sc)
             br._P = P
 
     def ok(self) -> bool:
@@ -398,7 +424,7 @@ def capture_grouped_step(vg: ViewGroups, means3D, shs, colors, opacities, scales
         br.forward_checked(means3D, shs, colors, opacities, scales, rotations)
         need, _ = br.read_status()
         cap = max(br.capacity, int(need * headroom) + 1024)
-        sizes.append((cap, br._auto_pairs(cap, int(br.read_pairs() * headroom))))
+        sizes.append((cap, br._auto_pairs(cap, int(br.read_pairs() * headroom)), int(br.read_slots() * headroom) + 1024))
         br.workspace = None
     vg.share_workspace(P, sizes)
     f32 = dict(dtype=torch.float32, device=vg.device)

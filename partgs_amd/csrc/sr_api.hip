@@ -130,12 +130,12 @@ hipEvent_t counts_event() {
   if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
   return e;
 }
-
+// Pinned 64-byte landing zone for the forward counters (instances, error flags, slots).
 // Pinned 8-byte landing zone for the instance count and error flags.
 uint32_t* pinned_counters() {
   thread_local uint32_t* p = nullptr;
   if (!p) {
-    if (hipHostMalloc((void**)&p, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) p = nullptr;
+    if (hipHostMalloc((void**)&p, 16 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) p = nullptr;
   }
   return p;
 }
@@ -147,6 +147,7 @@ int validate(const sr_settings* s, const sr_surfels* g) {
     return fail(SR_ERR_INVALID, "image size must be in [1, 32000] (got %d x %d)", s->image_width, s->image_height);
   if (s->pair_store < 0) return fail(SR_ERR_INVALID, "pair_store must be >= 0");
   if (s->pair_capacity < 0) return fail(SR_ERR_INVALID, "pair_capacity must be >= 0");
+  if (s->slot_capacity < 0) return fail(SR_ERR_INVALID, "slot_capacity must be >= 0");
   if (s->tile != 8 && s->tile != 16 && s->tile != 32)
     return fail(SR_ERR_INVALID, "tile must be 8, 16 or 32 (got %d)", s->tile);
   if (g->P == 0) return SR_OK;  // the reference glue returns before looking at the inputs
@@ -203,8 +204,9 @@ PairStore pair_store(char* bin, const BinLayout& bl) {
 // forward only ever waits for the copy of the forward kPairSlots before it
 // (long finished), never for the one still running.
 constexpr int kPairSlots = 4;
+constexpr int kCountWords = 16;
 struct PairCount {
-  uint32_t* host = nullptr;  // kPairSlots x 8 words: counters[0..5] of a forward
+  uint32_t* host = nullptr;  // kPairSlots x kCountWords words: counters[0..8] of a forward
   hipEvent_t ev[kPairSlots] = {};
   unsigned long long seq[kPairSlots] = {};  // 0 = slot never used
   unsigned long long next_seq = 1;
@@ -217,12 +219,13 @@ PairCount& pair_count_slot() {
   return slots[dev];
 }
 
-// Enqueues the copy of the instance and pair counts (counters[0..5]) of the
-// forward just enqueued into the next slot of the ring.
+// Enqueues the copy of the instance, pair and slot counts (counters[0..8])
+// of the forward just enqueued into the next slot of the ring.
 int post_pair_count(Pipeline& pl, hipStream_t st) {
   PairCount& pc = pair_count_slot();
   if (!pc.host) {
-    if (hipHostMalloc((void**)&pc.host, kPairSlots * 8 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&pc.host, kPairSlots * kCountWords * sizeof(uint32_t), hipHostMallocDefault) !=
+        hipSuccess) {
       pc.host = nullptr;
       return fail(SR_ERR_HIP, "hipHostMalloc failed");
     }
@@ -232,7 +235,7 @@ int post_pair_count(Pipeline& pl, hipStream_t st) {
   }
   const int k = pc.next;
   if (pc.seq[k] != 0) SR_HIP(hipEventSynchronize(pc.ev[k]));  // the copy of kPairSlots forwards ago
-  SR_HIP(hipMemcpyAsync(pc.host + 8 * k, pl.counters(), 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SR_HIP(hipMemcpyAsync(pc.host + kCountWords * k, pl.counters(), 9 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   SR_HIP(hipEventRecord(pc.ev[k], st));
   pc.seq[k] = pc.next_seq++;
   pc.next = (k + 1) % kPairSlots;
@@ -256,12 +259,13 @@ struct BatchLayout {
   ImageLayout il;
   BinLayout bl;
 };
-BatchLayout batch_layout(const ViewSet& vs, int P, int capacity, int pair_store, int pair_capacity) {
+BatchLayout batch_layout(const ViewSet& vs, int P, int capacity, int pair_store, int pair_capacity,
+                         int slot_capacity) {
   BatchLayout b{};
   const size_t R = (size_t)vs.V * (size_t)(P > 0 ? P : 0);
   b.gl = geom_layout(R);
   b.il = image_layout(vs);
-  b.bl = bin_layout(capacity, vs.B, (int)vs.T, pair_store, pair_capacity);
+  b.bl = bin_layout(capacity, vs.B, (int)vs.T, pair_store, pair_capacity, 0, slot_capacity);
   size_t off = 0;
   b.geom = off; off += align_up(b.gl.total);
   b.img = off; off += align_up(b.il.total);
@@ -284,6 +288,7 @@ int validate_views(const sr_views* v, const sr_surfels* g, int capacity) {
     return fail(SR_ERR_INVALID, "tile must be 8, 16 or 32 (got %d)", v->tile);
   if (v->pair_store < 0) return fail(SR_ERR_INVALID, "pair_store must be >= 0");
   if (v->pair_capacity < 0) return fail(SR_ERR_INVALID, "pair_capacity must be >= 0");
+  if (v->slot_capacity < 0) return fail(SR_ERR_INVALID, "slot_capacity must be >= 0");
   if (capacity < 1) return fail(SR_ERR_INVALID, "capacity must be >= 1");
   size_t pixels = 0;
   for (int i = 0; i < v->V; ++i) {
@@ -330,7 +335,8 @@ Pipeline batch_pipeline(const sr_views* v, const sr_surfels* g, const ViewSet& v
 
 // Clear counters and histogram, preprocess every (surfel, view), scan the
 // tile histogram; the instance count lands in counters[0].
-int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, bool dbg, hipStream_t st) {
+int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, uint32_t slot_capacity, bool dbg,
+              hipStream_t st) {
   uint32_t* counters = pl.counters();
   uint32_t* tile_count = (uint32_t*)(pl.img + pl.il.tile_count);
   // the counters and the tile histogram are adjacent: one clear for both
@@ -362,6 +368,7 @@ int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, 
   pa.clamped = (uint8_t*)(pl.geom + pl.gl.clamped);
   pa.cullbox = (short4*)(pl.geom + pl.gl.cullbox);
   pa.bsum = (uint32_t*)(pl.geom + pl.gl.bsum);
+  pa.ssum = (uint32_t*)(pl.geom + pl.gl.ssum);
   pa.tile_count = tile_count;
   pa.counters = counters;
   SR_STAGE(0, dbg, st, "preprocess", launch_preprocess(pa, st));
@@ -369,7 +376,7 @@ int run_front(Pipeline& pl, const sr_surfels* g, int* radii, uint32_t capacity, 
            launch_scan_tiles(pl.il.T, tile_count, (uint32_t*)(pl.img + pl.il.tile_start),
                              (uint32_t*)(pl.img + pl.il.tile_fill), (uint32_t*)(pl.img + pl.il.ranges), counters,
                              capacity, pl.vs.V * ((pl.P + 255) / 256), (uint32_t*)(pl.geom + pl.gl.bsum),
-                             (uint32_t*)(pl.img + pl.il.large), (uint32_t*)(pl.img + pl.il.lchunk_off),
+                             (uint32_t*)(pl.geom + pl.gl.ssum), slot_capacity, (uint32_t*)(pl.img + pl.il.large), (uint32_t*)(pl.img + pl.il.lchunk_off),
                              pl.side_huge ? kSmallMax : kLargeMax, st));
   return SR_OK;
 }
@@ -388,7 +395,11 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st, bool p
   sc.radii = radii;
   sc.counters = counters;
   sc.bsum = (const uint32_t*)(pl.geom + pl.gl.bsum);
+  sc.ssum = (const uint32_t*)(pl.geom + pl.gl.ssum);
+  sc.cullbox = (const short4*)(pl.geom + pl.gl.cullbox);
   sc.roff = (uint32_t*)(pl.geom + pl.gl.roff);
+  sc.rslot = (uint2*)(pl.geom + pl.gl.rslot);
+  sc.islot = (uint32_t*)(bin + bl.islot);
   sc.inst_rec = (uint32_t*)(bin + bl.inst_rec);
   sc.tile_fill = (uint32_t*)(pl.img + pl.il.tile_fill);
   sc.keys = (uint64_t*)(bin + bl.keys);
@@ -408,7 +419,6 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st, bool p
   sa.inst_rec = (const uint32_t*)(bin + bl.inst_rec);
   sa.inst_at = (uint32_t*)(bin + bl.inst_at);
   sa.inst_quads = (uint16_t*)(bin + bl.inst_quads);
-  sa.part_nz = (uint8_t*)(bin + bl.part_nz);
   sa.sub = (uint2*)(bin + bl.sub);
   sa.sub_count = (uint32_t*)(pl.img + pl.il.sub_count);
   sa.work = (uint2*)(bin + bl.work);
@@ -492,6 +502,8 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
     ba.dL_dpix = dL_dcolor;
     ba.dL_dothers = dL_dothers;
     ba.inst_at = (const uint32_t*)(pl.bin + pl.bl.inst_at);
+    ba.islot = (const uint32_t*)(pl.bin + pl.bl.islot);
+    ba.inst_quads = (const uint16_t*)(pl.bin + pl.bl.inst_quads);
     ba.part = (float*)(pl.bin + pl.bl.part);
     ba.part_nz = (uint8_t*)(pl.bin + pl.bl.part_nz);
     SR_STAGE(6, dbg, st, "blend_bwd", launch_blend_bwd(ba, st));
@@ -506,6 +518,7 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
   pb.counters = pl.counters();
   pb.tiles = (const uint32_t*)(pl.geom + pl.gl.tiles);
   pb.roff = (const uint32_t*)(pl.geom + pl.gl.roff);
+  pb.rslot = (const uint2*)(pl.geom + pl.gl.rslot);
   pb.inst_quads = blend ? (const uint16_t*)(pl.bin + pl.bl.inst_quads) : nullptr;
   pb.part = blend ? (const float*)(pl.bin + pl.bl.part) : nullptr;
   pb.part_nz = blend ? (const uint8_t*)(pl.bin + pl.bl.part_nz) : nullptr;
@@ -543,26 +556,51 @@ const char* sr_version(void) { return "surfel_rast gfx950 1.0"; }
 
 size_t sr_geom_bytes(int P) { return geom_layout((size_t)(P < 0 ? 0 : P)).total; }
 size_t sr_image_bytes(int W, int H, int tile) { return image_layout(W, H, tile > 0 ? tile : 16).total; }
-size_t sr_binning_bytes(int L, int W, int H, int tile, int pair_store, int pair_capacity) {
+size_t sr_binning_bytes(int L, int W, int H, int tile, int pair_store, int pair_capacity, int slot_capacity) {
   const int B = tile > 0 ? tile : 16;
-  return bin_layout(L, B, image_layout(W, H, B).T, pair_store, pair_capacity).total;
+  return bin_layout(L, B, image_layout(W, H, B).T, pair_store, pair_capacity, 0, slot_capacity).total;
 }
 
-int sr_pair_count(unsigned long long* pairs, unsigned int* instances) {
-  if (!pairs || !instances) return fail(SR_ERR_INVALID, "NULL argument");
+namespace {
+// The newest landed count record of the calling thread and device, or NULL.
+const uint32_t* latest_counts(int& rc) {
+  rc = SR_OK;
   PairCount& pc = pair_count_slot();
-  // newest slot whose copy has landed
   int best = -1;
   for (int k = 0; k < kPairSlots; ++k) {
     if (pc.seq[k] == 0 || (best >= 0 && pc.seq[k] < pc.seq[best])) continue;
     const hipError_t q = hipEventQuery(pc.ev[k]);
     if (q == hipSuccess) best = k;
-    else if (q != hipErrorNotReady) return fail(SR_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+    else if (q != hipErrorNotReady) {
+      rc = fail(SR_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+      return nullptr;
+    }
   }
-  if (best < 0) return fail(SR_ERR_INVALID, "no forward has completed on this thread and device");
-  const uint32_t* c = pc.host + 8 * best;
+  if (best < 0) {
+    rc = fail(SR_ERR_INVALID, "no forward has completed on this thread and device");
+    return nullptr;
+  }
+  return pc.host + kCountWords * best;
+}
+}  // namespace
+
+int sr_pair_count(unsigned long long* pairs, unsigned int* instances) {
+  if (!pairs || !instances) return fail(SR_ERR_INVALID, "NULL argument");
+  int rc;
+  const uint32_t* c = latest_counts(rc);
+  if (!c) return rc;
   *instances = c[0];
   *pairs = (unsigned long long)c[4] | ((unsigned long long)c[5] << 32);
+  return SR_OK;
+}
+
+int sr_slot_count(unsigned int* slots, unsigned int* instances) {
+  if (!slots || !instances) return fail(SR_ERR_INVALID, "NULL argument");
+  int rc;
+  const uint32_t* c = latest_counts(rc);
+  if (!c) return rc;
+  *instances = c[0];
+  *slots = c[8];
   return SR_OK;
 }
 
@@ -640,13 +678,13 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     // exceed it.  The instance count is copied back right after the scan
     // and waited for only once the rest is enqueued, by when it has long
     // arrived, so the stream never drains mid-forward.
-    pl.bl = bin_layout(cap, B, il.T, s->pair_store, s->pair_capacity);
+    pl.bl = bin_layout(cap, B, il.T, s->pair_store, s->pair_capacity, 0, s->slot_capacity);
     pl.bin = (char*)alloc(ctx, pl.bl.total, 1);
     if (!pl.bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
     pl.capacity = cap;
-    rc = run_front(pl, g, radii, (uint32_t)cap, dbg, st);
+    rc = run_front(pl, g, radii, (uint32_t)cap, (uint32_t)pl.bl.nslot, dbg, st);
     if (rc != SR_OK) return rc;
-    SR_HIP(hipMemcpyAsync(host, pl.counters(), 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    SR_HIP(hipMemcpyAsync(host, pl.counters(), 9 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     hipEvent_t counted = counts_event();
     if (!counted) return fail(SR_ERR_HIP, "hipEventCreate failed");
     SR_HIP(hipEventRecord(counted, st));
@@ -658,19 +696,24 @@ int sr_forward(const sr_settings* s, const sr_surfels* g, sr_alloc_fn alloc, voi
     if (host[1] & kErrPrefiltered)
       return fail(SR_ERR_PREFILTERED, "Point is filtered although prefiltered is set. This shouldn't happen!");
     *num_rendered = (int)host[0];
-    if (host[1] & kErrCapacity)
+    if ((host[1] & kErrCapacity) && (uint32_t)cap < host[0])
       return fail(SR_ERR_CAPACITY, "%u tile instances exceed the capacity %d", host[0], cap);
+    if (host[1] & kErrCapacity)
+      return fail(SR_ERR_CAPACITY, "%u partial-gradient slots exceed the slot capacity %zu", host[8], pl.bl.nslot);
     return SR_OK;
   }
-  rc = run_front(pl, g, radii, 0xFFFFFFFFu, dbg, st);
+  rc = run_front(pl, g, radii, 0xFFFFFFFFu, 0xFFFFFFFFu, dbg, st);
   if (rc != SR_OK) return rc;
-  SR_HIP(hipMemcpyAsync(host, pl.counters(), 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SR_HIP(hipMemcpyAsync(host, pl.counters(), 9 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   SR_HIP(hipStreamSynchronize(st));
   if (host[1] & kErrPrefiltered)
     return fail(SR_ERR_PREFILTERED, "Point is filtered although prefiltered is set. This shouldn't happen!");
   const int L = (int)host[0];
   *num_rendered = L;
-  pl.bl = bin_layout(L, B, il.T, s->pair_store, s->pair_capacity);
+  // the slots the forward needs, exactly (the slot region is the layout's
+  // last, so the backward locates everything without this count)
+  const int S = (int)(host[8] > 0u ? host[8] : 1u);
+  pl.bl = bin_layout(L, B, il.T, s->pair_store, s->pair_capacity, 0, S);
   pl.bin = (char*)alloc(ctx, pl.bl.total, 1);
   if (!pl.bin) return fail(SR_ERR_ALLOC, "allocation callback failed");
   pl.capacity = L;
@@ -792,7 +835,7 @@ int sr_forward_part(const sr_settings* s, const sr_surfels* g, int S, const floa
   if (!pl.geom || !pl.img) return fail(SR_ERR_ALLOC, "allocation callback failed");
   uint32_t* host = pinned_counters();
   if (!host) return fail(SR_ERR_HIP, "hipHostMalloc failed");
-  rc = run_front(pl, g, radii, 0xFFFFFFFFu, dbg, st);
+  rc = run_front(pl, g, radii, 0xFFFFFFFFu, 0xFFFFFFFFu, dbg, st);
   if (rc != SR_OK) return rc;
   SR_HIP(hipMemcpyAsync(host, pl.counters(), 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   SR_HIP(hipStreamSynchronize(st));
@@ -864,6 +907,8 @@ int sr_backward_part(const sr_settings* s, const sr_surfels* g, int S, const flo
     pa.dL_dsem = dL_dsemantic;
     pa.dL_dothers = dL_dothers;
     pa.inst_at = (const uint32_t*)(pl.bin + pl.bl.inst_at);
+    pa.islot = (const uint32_t*)(pl.bin + pl.bl.islot);
+    pa.inst_quads = (const uint16_t*)(pl.bin + pl.bl.inst_quads);
     pa.part = (float*)(pl.bin + pl.bl.part);
     pa.sem_part = (float*)(pl.bin + pl.bl.sem_part);
     pa.part_nz = (uint8_t*)(pl.bin + pl.bl.part_nz);
@@ -877,6 +922,7 @@ int sr_backward_part(const sr_settings* s, const sr_surfels* g, int S, const flo
   pb.counters = pl.counters();
   pb.tiles = (const uint32_t*)(pl.geom + pl.gl.tiles);
   pb.roff = (const uint32_t*)(pl.geom + pl.gl.roff);
+  pb.rslot = (const uint2*)(pl.geom + pl.gl.rslot);
   pb.inst_quads = blend ? (const uint16_t*)(pl.bin + pl.bl.inst_quads) : nullptr;
   pb.part = blend ? (const float*)(pl.bin + pl.bl.part) : nullptr;
   pb.part_nz = blend ? (const uint8_t*)(pl.bin + pl.bl.part_nz) : nullptr;
@@ -909,8 +955,9 @@ int sr_backward_part(const sr_settings* s, const sr_surfels* g, int S, const flo
 size_t sr_batch_workspace_bytes(const sr_views* v, int P, int capacity) {
   if (!v || v->V < 1 || v->V > SR_MAX_VIEWS || P < 0 || capacity < 0 || !v->widths || !v->heights) return 0;
   const int B = v->tile > 0 ? v->tile : 16;
+  if (v->slot_capacity < 0) return 0;
   return batch_layout(make_viewset(v->V, v->widths, v->heights, v->tanfovx, v->tanfovy, B), P, capacity,
-                      v->pair_store, v->pair_capacity).total;
+                      v->pair_store, v->pair_capacity, v->slot_capacity).total;
 }
 
 int sr_forward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t ws_bytes, int capacity,
@@ -920,7 +967,7 @@ int sr_forward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t 
   if (rc != SR_OK) return rc;
   if (!ws || !out_color || !out_others || !radii || !status) return fail(SR_ERR_INVALID, "NULL argument");
   const ViewSet vs = make_viewset(v->V, v->widths, v->heights, v->tanfovx, v->tanfovy, v->tile);
-  const BatchLayout bl = batch_layout(vs, g->P, capacity, v->pair_store, v->pair_capacity);
+  const BatchLayout bl = batch_layout(vs, g->P, capacity, v->pair_store, v->pair_capacity, v->slot_capacity);
   if (ws_bytes < bl.total)
     return fail(SR_ERR_INVALID, "workspace holds %zu bytes, needs %zu", ws_bytes, bl.total);
   Pipeline pl = batch_pipeline(v, g, vs, bl, (char*)ws, capacity);
@@ -931,7 +978,7 @@ int sr_forward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t 
   // kSmallMax takes the one-workgroup sort, beside the small-tile sort on a
   // second stream (a captured graph makes the fork cheap).
   pl.side_huge = true;
-  rc = run_front(pl, g, radii, (uint32_t)capacity, false, st);
+  rc = run_front(pl, g, radii, (uint32_t)capacity, (uint32_t)bl.bl.nslot, false, st);
   if (rc == SR_OK) rc = run_binning(pl, radii, false, st);
   if (rc == SR_OK) rc = run_blend_fwd(pl, out_color, out_others, false, st);
   if (rc != SR_OK) return rc;
@@ -951,7 +998,7 @@ int sr_backward_batched(const sr_views* v, const sr_surfels* g, void* ws, size_t
       !gr->dL_drotations || (g->shs && !gr->dL_dsh))
     return fail(SR_ERR_INVALID, "gradient output missing");
   const ViewSet vs = make_viewset(v->V, v->widths, v->heights, v->tanfovx, v->tanfovy, v->tile);
-  const BatchLayout bl = batch_layout(vs, g->P, capacity, v->pair_store, v->pair_capacity);
+  const BatchLayout bl = batch_layout(vs, g->P, capacity, v->pair_store, v->pair_capacity, v->slot_capacity);
   if (ws_bytes < bl.total)
     return fail(SR_ERR_INVALID, "workspace holds %zu bytes, needs %zu", ws_bytes, bl.total);
   Pipeline pl = batch_pipeline(v, g, vs, bl, (char*)ws, capacity);

@@ -44,14 +44,12 @@ constexpr int kGradGroup = SR_GRAD_GROUP;
 static_assert(kGradGroup >= 1 && 64 % kGradGroup == 0, "groups tile the 64 pixels");
 
 // Stores the 18 partial sums (g(k) for slot k) of the sub-list entry at
-// tile-list position pos of quadrant q as five float4 in its (instance,
-// quadrant) slot.  Every entry of every sub-list sets its flag exactly once
-// per backward and stores its sums when any is non-zero, so neither buffer
-// needs clearing.
+// tile-list position pos of quadrant q as five float4 in its slot.  Every
+// entry of every sub-list sets its flag exactly once per backward and
+// stores its sums when any is non-zero, so neither buffer needs clearing.
 template <class G>
 SR_DEV void store_part(const BlendBwdArgs& a, uint32_t pos, int q, G g) {
-  const int Q = quads_per_tile(a.vs.B);
-  const size_t slot = (size_t)a.inst_at[pos] * Q + q;
+  const size_t slot = slot_of(a.islot, a.inst_quads, a.inst_at[pos], q);
   bool nz = false;
 #pragma unroll
   for (int k = 0; k < 18; ++k) nz |= g(k) != 0.0f;
@@ -957,48 +955,24 @@ struct SurfelGrad {
   float m[3], sc[2], q[4];
 };
 
-// Record r's 18 partial gradients (the slots of the backward blend) summed
-// over the quadrants of each of its instances (a contiguous run), in
-// instance order: a fixed order, so the sums are the same on every run.
+// Record r's 18 partial gradients summed over its slots (one per instance
+// and quadrant holding it, a contiguous run in instance order, quadrants in
+// order): a fixed order, so the sums are the same on every run.
 SR_DEV void reduce_record(const PreprocessBwdArgs& a, size_t r, float (&gs)[18]) {
 #pragma unroll
   for (int k = 0; k < 18; ++k) gs[k] = 0.0f;
-  const uint32_t n = a.tiles[r];
-  if (n == 0 || a.part == nullptr || (a.counters[1] & kErrCapacity)) return;
-  const int Q = quads_per_tile(a.vs.B);
-  const uint32_t i0 = a.roff[r];
-  auto add_slot = [&](size_t slot) {
-    const float4* p = reinterpret_cast<const float4*>(a.part + slot * kPartStride);
-    const float4 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4];
-    gs[0] += p0.x; gs[1] += p0.y; gs[2] += p0.z; gs[3] += p0.w;
-    gs[4] += p1.x; gs[5] += p1.y; gs[6] += p1.z; gs[7] += p1.w;
-    gs[8] += p2.x; gs[9] += p2.y; gs[10] += p2.z; gs[11] += p2.w;
-    gs[12] += p3.x; gs[13] += p3.y; gs[14] += p3.z; gs[15] += p3.w;
-    gs[16] += p4.x; gs[17] += p4.y;
-  };
-  if (Q == 4) {
-    // the four flags of an instance are one word (zero for quadrants that do
-    // not hold it); four instances' flags in flight at a time
-    const uint32_t* nz = reinterpret_cast<const uint32_t*>(a.part_nz);
-    for (uint32_t i = i0; i < i0 + n; i += 4) {
-      uint32_t fl[4];
+  if (a.tiles[r] == 0 || a.part == nullptr || (a.counters[1] & kErrCapacity)) return;
+  const uint2 rs = a.rslot[r];  // the record's slots: its instances in order, each one's quadrants in order
+  const uint32_t s0 = rs.x, s1 = rs.x + rs.y;
+  // four flags in flight at a time
+  for (uint32_t s = s0; s < s1; s += 4) {
+    uint8_t fl[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) fl[t] = i + t < i0 + n ? nz[i + t] : 0u;
+    for (int t = 0; t < 4; ++t) fl[t] = a.part_nz[min(s + (uint32_t)t, s1 - 1u)];
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if ((fl[t] >> (8 * q)) & 0xffu) add_slot((size_t)(i + t) * 4 + q);
-    }
-    return;
-  }
-  for (uint32_t i = i0; i < i0 + n; ++i) {
-    uint32_t m = a.inst_quads[i];
-    while (m != 0u) {
-      const int q = __builtin_ctz(m);
-      m &= m - 1u;
-      if (a.part_nz[(size_t)i * Q + q] == 0) continue;
-      const float4* p = reinterpret_cast<const float4*>(a.part + ((size_t)i * Q + q) * kPartStride);
+    for (int t = 0; t < 4; ++t) {
+      if (s + (uint32_t)t >= s1 || fl[t] == 0) continue;
+      const float4* p = reinterpret_cast<const float4*>(a.part + (size_t)(s + (uint32_t)t) * kPartStride);
       const float4 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4];
       gs[0] += p0.x; gs[1] += p0.y; gs[2] += p0.z; gs[3] += p0.w;
       gs[4] += p1.x; gs[5] += p1.y; gs[6] += p1.z; gs[7] += p1.w;
@@ -1290,20 +1264,13 @@ __global__ __launch_bounds__(256) void k_part_preprocess_bwd(PreprocessBwdArgs a
   {
     float sm[16];
     for (int ch = 0; ch < 16; ++ch) sm[ch] = 0.0f;
-    const uint32_t n = a.tiles[idx];
-    const int Q = quads_per_tile(a.vs.B);
-    const uint32_t i0 = a.roff[idx];
-    if (S > 0 && a.part != nullptr)
-      for (uint32_t i = i0; i < i0 + n; ++i) {
-        uint32_t qm = a.inst_quads[i];
-        while (qm != 0u) {
-          const int q = __builtin_ctz(qm);
-          qm &= qm - 1u;
-          const size_t slot = (size_t)i * Q + q;
-          if (a.part_nz[slot] == 0) continue;
-          for (int ch = 0; ch < S; ++ch) sm[ch] += a.sem_part[slot * S + ch];
-        }
+    if (S > 0 && a.part != nullptr && a.tiles[idx] > 0 && !(a.counters[1] & kErrCapacity)) {
+      const uint2 rs = a.rslot[idx];  // the record's slots, in the order of reduce_record
+      for (uint32_t sl = rs.x; sl < rs.x + rs.y; ++sl) {
+        if (a.part_nz[sl] == 0) continue;
+        for (int ch = 0; ch < S; ++ch) sm[ch] += a.sem_part[(size_t)sl * S + ch];
       }
+    }
     for (int ch = 0; ch < S; ++ch) a.dL_dsem[(size_t)idx * S + ch] = sm[ch];
   }
   const ViewInfo& vi = a.vs.v[0];

@@ -489,9 +489,17 @@ SR_DEV uint32_t block_scan256(uint32_t x, uint32_t* s_w, uint32_t& total) {
 // blockIdx.y, so its tile histogram is that view's.  Each workgroup also
 // stores its records' instance total in bsum (view-major), which the tile
 // scan turns into the first instance index of every workgroup.
+// Partial-gradient slots of a record: one per (instance, quadrant holding
+// it), the quadrants found as the sort's sub-lists find them.
+SR_DEV uint32_t record_slots(short4 bx, int x0, int y0, int x1, int y1, int B, const ViewInfo& vi) {
+  const int QX = B / kQuad;
+  return (uint32_t)(quad_span(bx.x, bx.z, x0 * QX, x1 * QX, vi.W) * quad_span(bx.y, bx.w, y0 * QX, y1 * QX, vi.H));
+}
+
 __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
   __shared__ uint32_t s_hist[kHistTiles];
   __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_ssum[4];
   const int v = blockIdx.y;
   const int gx = a.vs.v[v].gx;
   const int T = gx * a.vs.v[v].gy;
@@ -506,6 +514,10 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
     uint32_t total = 0;
     (void)block_scan256(ok ? (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0) : 0u, s_wsum, total);
     if (threadIdx.x == 0) a.bsum[(size_t)v * gridDim.x + blockIdx.x] = total;
+    // the cull box was stored by preprocess_one (this thread)
+    const uint32_t ns = ok ? record_slots(a.cullbox[(size_t)v * a.P + idx], x0, y0, x1, y1, a.vs.B, a.vs.v[v]) : 0u;
+    (void)block_scan256(ns, s_ssum, total);
+    if (threadIdx.x == 0) a.ssum[(size_t)v * gridDim.x + blockIdx.x] = total;
   }
   if (ok) {
     for (int ty = y0; ty < y1; ++ty)
@@ -589,6 +601,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
                                                               uint32_t* start, uint32_t* fill,
                                                               uint32_t* ranges, uint32_t* counters,
                                                               uint32_t capacity, int nb, uint32_t* bsum,
+                                                              uint32_t* ssum, uint32_t slot_capacity,
                                                               uint32_t* large, uint32_t* lchunk_off,
                                                               int large_max) {
   __shared__ uint32_t s_part[3 * 32];
@@ -637,25 +650,35 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int T, const uint32
     counters[7] = tot3[2];
     lchunk_off[tot3[0]] = tot3[1];
   }
-  // the per-workgroup instance totals of the preprocess, in place
+  // the per-workgroup instance and slot totals of the preprocess, in place
   const int pb = (nb + kScanThreads - 1) / kScanThreads;
   const int b2 = tid * pb, e2 = min(nb, b2 + pb);
-  uint32_t w1[1] = {0u}, wt[1];
-  for (int i = b2; i < e2; ++i) w1[0] += bsum[i];
-  scan1024_excl(s_part, w1, wt);
-  uint32_t run2 = w1[0];
+  uint32_t w2[2] = {0u, 0u}, wt[2];
   for (int i = b2; i < e2; ++i) {
-    const uint32_t c = bsum[i];
+    w2[0] += bsum[i];
+    w2[1] += ssum[i];
+  }
+  scan1024_excl(s_part, w2, wt);
+  uint32_t run2 = w2[0], run3 = w2[1];
+  for (int i = b2; i < e2; ++i) {
+    const uint32_t c = bsum[i], d = ssum[i];
     bsum[i] = run2;
+    ssum[i] = run3;
     run2 += c;
+    run3 += d;
+  }
+  if (tid == 0) {
+    counters[8] = wt[1];
+    if (wt[1] > slot_capacity) atomicOr(&counters[1], kErrCapacity | kErrSlots);
   }
 }
 
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
                              uint32_t* ranges, uint32_t* counters, uint32_t capacity, int nb, uint32_t* bsum,
-                             uint32_t* large, uint32_t* lchunk_off, int large_max, hipStream_t s) {
+                             uint32_t* ssum, uint32_t slot_capacity, uint32_t* large, uint32_t* lchunk_off,
+                             int large_max, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, T, count, start, fill, ranges,
-                     counters, capacity, nb, bsum, large, lchunk_off, large_max);
+                     counters, capacity, nb, bsum, ssum, slot_capacity, large, lchunk_off, large_max);
   return hipGetLastError();
 }
 
@@ -671,6 +694,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   __shared__ uint32_t s_cnt[kHistTiles];
   __shared__ uint32_t s_base[kHistTiles];
   __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_ssum[4];
   if (a.counters[1] & kErrCapacity) return;
   const int v = blockIdx.y;
   const int B = a.vs.B, gx = a.vs.v[v].gx, gy = a.vs.v[v].gy;
@@ -692,10 +716,24 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
       a.bsum[(size_t)v * gridDim.x + blockIdx.x] +
       block_scan256(ok ? (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0) : 0u, s_wsum, total);
   if (idx < a.P) a.roff[r] = inst0;
+  // partial-gradient slots: the record's run, and each instance's first
+  // slot (its quadrants' slots follow in quadrant order)
+  const ViewInfo& vi = a.vs.v[v];
+  const short4 bx = ok ? a.cullbox[r] : make_short4(32767, 32767, -32768, -32768);
+  const uint32_t ns = ok ? record_slots(bx, x0, y0, x1, y1, B, vi) : 0u;
+  const uint32_t slot0 = a.ssum[(size_t)v * gridDim.x + blockIdx.x] + block_scan256(ns, s_ssum, total);
+  if (idx < a.P) a.rslot[r] = make_uint2(slot0, ns);
   if (ok) {
-    uint32_t i = inst0;
-    for (int ty = y0; ty < y1; ++ty)
-      for (int tx = x0; tx < x1; ++tx) a.inst_rec[i++] = (uint32_t)r;
+    const int QX = B / kQuad;
+    uint32_t i = inst0, sl = slot0;
+    for (int ty = y0; ty < y1; ++ty) {
+      const uint32_t rows = (uint32_t)quad_span(bx.y, bx.w, ty * QX, ty * QX + QX, vi.H);
+      for (int tx = x0; tx < x1; ++tx) {
+        a.islot[i] = sl;
+        sl += rows * (uint32_t)quad_span(bx.x, bx.z, tx * QX, tx * QX + QX, vi.W);
+        a.inst_rec[i++] = (uint32_t)r;
+      }
+    }
   }
   uint64_t* keys = a.keys;
   if (!lds) {
@@ -836,23 +874,6 @@ SR_DEV int next_pow2(int n) {
 // the sub-list gives the reference's result); it starts at Q * beg + q * n
 // and holds (surfel id, position in the tile list) pairs.  Each non-empty
 // sub-list appends its 64-entry chunks to the acceptance work list.
-constexpr int kMaxQuads = 16;
-
-// Bit q set when cull box bx meets quadrant q (of QX x QX) of tile (tx, ty)
-// inside the image; the empty box (32767, 32767, -32768, -32768) meets none.
-SR_DEV uint32_t quad_hits(short4 bx, int tx, int ty, int B, int QX, int Q, const ViewInfo& vi) {
-  uint32_t m = 0;
-#pragma unroll
-  for (int q = 0; q < kMaxQuads; ++q) {
-    if (q < Q) {
-      const int qx0 = tx * B + (q % QX) * kQuad, qy0 = ty * B + (q / QX) * kQuad;
-      const bool hit = qx0 < vi.W && qy0 < vi.H && bx.x <= qx0 + kQuad - 1 && bx.z >= qx0 &&
-                       bx.y <= qy0 + kQuad - 1 && bx.w >= qy0;
-      m |= (uint32_t)hit << q;
-    }
-  }
-  return m;
-}
 
 // The hit bits of quadrants 4g..4g+3 in the four 16-bit fields of a packed
 // counter (at most kSmallMax < 2^16 entries are counted in one scan, so
@@ -970,14 +991,8 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
     const int i = tid * K + k;
     hm[k] = quad_hits(bx[k], tx, ty, B, QX, Q, vi);
     if (i < cn) {
-      // for the gradient reduction: the instance's quadrants, their
-      // stored-sum flags cleared
+      // for the gradient slots: the instance's quadrants
       a.inst_quads[inst[k]] = (uint16_t)hm[k];
-      if (Q == 4) {
-        reinterpret_cast<uint32_t*>(a.part_nz)[inst[k]] = 0u;
-      } else {
-        for (int q = 0; q < Q; ++q) a.part_nz[(size_t)inst[k] * Q + q] = 0;
-      }
       if constexpr (!kWhole) a.lmeta[c0 + i] = make_uint2(rid[k], hm[k]);
     }
   }
@@ -1274,10 +1289,7 @@ SR_DEV void sort_huge_tile(const SortArgs& a, int tile, uint64_t* s_keys) {
         if (lane == 0) s_wcnt[q][wave] = (uint32_t)__popcll(m);
       }
     }
-    if (valid) {
-      a.inst_quads[inst] = (uint16_t)hm;
-      for (int q = 0; q < Q; ++q) a.part_nz[(size_t)inst * Q + q] = 0;
-    }
+    if (valid) a.inst_quads[inst] = (uint16_t)hm;
     __syncthreads();
     if (tid < Q * kWaves) {  // per quadrant, the waves' first slots
       const int q = tid / kWaves, w = tid % kWaves;
@@ -2120,7 +2132,7 @@ hipError_t launch_mark_visible(int P, const float* means3D, const float* vm, uin
 // Copies the instance count and the error flags to a caller-visible status
 // word, so batched callers can check capacity without reading internal state.
 __global__ void k_copy_status(const uint32_t* counters, uint32_t* status) {
-  if (threadIdx.x < 8) status[threadIdx.x] = counters[threadIdx.x];
+  if (threadIdx.x < 9) status[threadIdx.x] = counters[threadIdx.x];
 }
 
 hipError_t launch_copy_status(const uint32_t* counters, uint32_t* status, hipStream_t s) {

@@ -95,7 +95,7 @@ __device__ inline int view_of_tile(const ViewSet& vs, uint32_t t) {
 
 // Geometry state: one entry per record (surfel, view).
 struct GeomLayout {
-  size_t rec, depths, tiles, clamped, roff, cullbox, bsum, total;
+  size_t rec, depths, tiles, clamped, roff, cullbox, bsum, ssum, rslot, total;
 };
 // Workgroups of 256 records, per view, that preprocess and scatter use:
 // at most R / 256 + one partial workgroup per view.
@@ -110,6 +110,8 @@ inline GeomLayout geom_layout(size_t R) {
   g.roff = off; off += align_up(R * 4);     // first instance number of the record
   g.cullbox = off; off += align_up(R * 8);   // short4 inclusive pixel box
   g.bsum = off; off += align_up(record_groups(R) * 4);  // instances per workgroup, then their scan
+  g.ssum = off; off += align_up(record_groups(R) * 4);  // partial-gradient slots per workgroup, then their scan
+  g.rslot = off; off += align_up(R * 8);  // uint2 per record: first slot, slots
   g.total = off;
   return g;
 }
@@ -127,7 +129,8 @@ inline ImageLayout image_layout(const ViewSet& vs) {
   size_t off = 0;
   l.counters = off; off += kAlign;  // [0] instances, [1] error flags, [2] work items,
                                     // [3] large tiles, [4..5] pair-store fill,
-                                    // [6] chunks of the large tiles, [7] huge tiles
+                                    // [6] chunks of the large tiles, [7] huge tiles,
+                                    // [8] partial-gradient slots
   l.tile_count = off; off += align_up((size_t)l.T * 4);
   l.tile_start = off; off += align_up(((size_t)l.T + 1) * 4);
   l.tile_fill = off; off += align_up((size_t)l.T * 4);
@@ -152,14 +155,20 @@ inline ImageLayout image_layout(int W, int H, int B) { return image_layout(singl
 //   inst_rec   instance -> record
 //   inst_at    position in the concatenated sorted tile lists -> instance
 //   inst_quads per instance, the quadrants whose sub-lists hold it
-//   part       per (instance, quadrant): the 18 per-surfel partial
-//              gradients the backward blend summed over that quadrant's
-//              pixels (kPartStride floats); the backward preprocess sums a
-//              record's contiguous run of them, so no gradient needs a
-//              global atomic
+//   islot      per instance, its first partial-gradient slot
+//   part       one slot per sub-list entry, i.e. per (instance, quadrant
+//              holding it): the 18 per-surfel partial gradients the
+//              backward blend summed over that quadrant's pixels
+//              (kPartStride floats).  Slots are numbered in record order,
+//              then instance order, then quadrant order, so a record's
+//              slots are one contiguous run (rslot) that the backward
+//              preprocess sums in order: no gradient needs a global
+//              atomic.  The slot array is the layout's last region, so its
+//              size (slot capacity) does not move any other offset.
 struct BinLayout {
   size_t keys, lmeta, lpre, point_list, sub, masks, chunk_T, work, frag_off, total;
-  size_t inst_rec, inst_at, inst_quads, part, part_nz;
+  size_t inst_rec, inst_at, inst_quads, islot, part, part_nz;
+  size_t nslot;  // slots the part region holds
   size_t prec, fW;
   size_t frag_cap;
   size_t sem_part;
@@ -196,7 +205,10 @@ inline size_t work_capacity(int L, int B, int T) {
   const size_t Q = (size_t)quads_per_tile(B);
   return Q * ((size_t)T + ((size_t)(L > 0 ? L : 1) + 63) / 64);
 }
-inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0, int pair_capacity = 0, int sem = 0) {
+// slot_capacity: slots the part region is sized for; 0 sizes it for every
+// (instance, quadrant), which no batch can exceed.
+inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0, int pair_capacity = 0, int sem = 0,
+                            int slot_capacity = 0) {
   BinLayout b{};
   size_t off = 0;
   const size_t n = (size_t)(L > 0 ? L : 1);
@@ -216,14 +228,16 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0, int pair_ca
   b.inst_rec = off; off += align_up(n * 4);
   b.inst_at = off; off += align_up(n * 4);
   b.inst_quads = off; off += align_up(n * 2);
-  b.part = off; off += align_up(n * Q * kPartStride * 4);
-  b.part_nz = off; off += align_up(n * Q);
+  b.islot = off; off += align_up(n * 4);
+  b.nslot = slot_capacity > 0 ? (size_t)slot_capacity : n * Q;
+  b.part_nz = off; off += align_up(n * Q);  // per slot: 1 when the slot holds sums
   b.frag_cap = pair_capacity > 0 ? (size_t)pair_capacity
                                   : n * (pair_store > 0 ? (size_t)pair_store : kFragPerInstance);
   if (b.frag_cap > 0x7fffffffu) b.frag_cap = 0x7fffffffu;
   b.prec = off; off += align_up(b.frag_cap * kPairQuads * 16);
   b.fW = off; off += align_up(b.frag_cap * 16);
-  b.sem_part = off; off += align_up(n * Q * (size_t)sem * 4);  // _part: semantic partials per (instance, quadrant)
+  b.sem_part = off; off += align_up(n * Q * (size_t)sem * 4);  // _part: semantic partials per slot
+  b.part = off; off += align_up(b.nslot * kPartStride * 4);         // last: see islot
   b.total = off;
   return b;
 }
@@ -235,7 +249,8 @@ struct PairStore {
 };
 
 constexpr uint32_t kErrPrefiltered = 1u;
-constexpr uint32_t kErrCapacity = 2u;  // more tile instances than the binning capacity
+constexpr uint32_t kErrCapacity = 2u;  // more tile instances (or slots) than the binning capacity
+constexpr uint32_t kErrSlots = 4u;     // with kErrCapacity: the partial-gradient slots did not fit
 
 struct PreprocessArgs {
   ViewSet vs;
@@ -259,6 +274,7 @@ struct PreprocessArgs {
   uint8_t* clamped;
   short4* cullbox;
   uint32_t* bsum;
+  uint32_t* ssum;  // partial-gradient slots per workgroup
   uint32_t* tile_count;
   uint32_t* counters;
 };
@@ -271,7 +287,11 @@ struct ScatterArgs {
   const int* radii;
   const uint32_t* counters;
   const uint32_t* bsum;
+  const uint32_t* ssum;  // first slot of each workgroup
+  const short4* cullbox;
   uint32_t* roff;
+  uint2* rslot;    // per record: first slot, slots
+  uint32_t* islot;  // per instance: first slot
   uint32_t* inst_rec;
   uint32_t* tile_fill;
   uint64_t* keys;
@@ -300,7 +320,6 @@ struct SortArgs {
   const uint32_t* inst_rec;
   uint32_t* inst_at;
   uint16_t* inst_quads;
-  uint8_t* part_nz;  // cleared here for every (instance, quadrant)
   uint2* sub;
   uint32_t* sub_count;
   uint2* work;          // (quadrant index, chunk) items for the acceptance pass
@@ -374,7 +393,9 @@ struct BlendBwdArgs {
   const float* dL_dpix;
   const float* dL_dothers;
   const uint32_t* inst_at;
-  float* part;        // partial gradients per (instance, quadrant)
+  const uint32_t* islot;
+  const uint16_t* inst_quads;
+  float* part;        // partial gradients per slot (slot_of)
   uint8_t* part_nz;   // 1 where that slot was stored (any non-zero sum)
 };
 
@@ -390,6 +411,7 @@ struct PreprocessBwdArgs {
   const uint32_t* tiles;
   const uint32_t* roff;
   const uint16_t* inst_quads;
+  const uint2* rslot;  // per record: first slot, slots
   const float* part;
   const uint8_t* part_nz;
   const float* means3D;
@@ -443,10 +465,53 @@ struct PartArgs {
   const float* dL_dsem;
   const float* dL_dothers;
   const uint32_t* inst_at;
+  const uint32_t* islot;
+  const uint16_t* inst_quads;
   float* part;
-  float* sem_part;           // S per (instance, quadrant)
+  float* sem_part;           // S per slot
   uint8_t* part_nz;
 };
+
+constexpr int kMaxQuads = 16;
+
+// Bit q set when cull box bx meets quadrant q (of QX x QX) of tile (tx, ty)
+// inside the image; the empty box (32767, 32767, -32768, -32768) meets none.
+__device__ inline uint32_t quad_hits(short4 bx, int tx, int ty, int B, int QX, int Q, const ViewInfo& vi) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxQuads; ++q) {
+    if (q < Q) {
+      const int qx0 = tx * B + (q % QX) * kQuad, qy0 = ty * B + (q / QX) * kQuad;
+      const bool hit = qx0 < vi.W && qy0 < vi.H && bx.x <= qx0 + kQuad - 1 && bx.z >= qx0 &&
+                       bx.y <= qy0 + kQuad - 1 && bx.w >= qy0;
+      m |= (uint32_t)hit << q;
+    }
+  }
+  return m;
+}
+
+// quad_hits factorises: quadrant (qx, qy) of tile (tx, ty) is hit when its
+// column c = tx * QX + qx meets the box's x span and its row meets the y
+// span.  Number of kQuad-wide columns c in [c0, c1) inside the image
+// (c * kQuad < W) that meet the pixel span [b0, b1]; with the y span, rows
+// and H it counts rows.  popc(quad_hits) over any rectangle of tiles is the
+// product of its column and row counts.
+__host__ __device__ inline int quad_span(int b0, int b1, int c0, int c1, int W) {
+  const int lo0 = b0 > 0 ? b0 / kQuad : 0;
+  const int lo = lo0 > c0 ? lo0 : c0;
+  const int hi0 = b1 >= 0 ? b1 / kQuad : -1;
+  const int hi1 = (W - 1) / kQuad;
+  int hi = c1 - 1;
+  hi = hi0 < hi ? hi0 : hi;
+  hi = hi1 < hi ? hi1 : hi;
+  return hi >= lo ? hi - lo + 1 : 0;
+}
+
+// Partial-gradient slot of an instance's entry in sub-list q of its tile:
+// the instance's first slot plus the quadrants before q that hold it.
+__device__ inline size_t slot_of(const uint32_t* islot, const uint16_t* inst_quads, uint32_t inst, int q) {
+  return (size_t)islot[inst] + (size_t)__popc((uint32_t)inst_quads[inst] & ((1u << q) - 1u));
+}
 
 // Launchers (return hipGetLastError()).
 hipError_t launch_part_blend_fwd(const PartArgs& a, hipStream_t s);
@@ -459,7 +524,8 @@ int set_error(int code, const char* msg);
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s);
 hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint32_t* fill,
                              uint32_t* ranges, uint32_t* counters, uint32_t capacity, int nb, uint32_t* bsum,
-                             uint32_t* large, uint32_t* lchunk_off, int large_max, hipStream_t s);
+                             uint32_t* ssum, uint32_t slot_capacity, uint32_t* large, uint32_t* lchunk_off,
+                             int large_max, hipStream_t s);
 hipError_t launch_scatter(const ScatterArgs& a, hipStream_t s);
 hipError_t launch_sort_tiles(const SortArgs& a, hipStream_t s);
 hipError_t launch_accept(const AcceptArgs& a, size_t max_items, hipStream_t s);

@@ -64,7 +64,7 @@ class _Arena:
 
 
 def _settings(bg, scale_modifier, viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height, image_width,
-              degree, campos, prefiltered, debug, tile, pair_store=0, capacity=0, pair_capacity=0):
+              degree, campos, prefiltered, debug, tile, pair_store=0, capacity=0, pair_capacity=0, slot_capacity=0):
     s = _lib.SrSettings()
     s.image_height = int(image_height)
     s.image_width = int(image_width)
@@ -82,6 +82,7 @@ def _settings(bg, scale_modifier, viewmatrix, projmatrix, tan_fovx, tan_fovy, im
     s.pair_store = int(pair_store)
     s.capacity = int(capacity)
     s.pair_capacity = int(pair_capacity)
+    s.slot_capacity = int(slot_capacity)
     return s
 
 
@@ -102,7 +103,7 @@ def _surfels(P, means3D, sh, colors, opacity, scales, rotations, transMat_precom
 def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, scale_modifier, transMat_precomp,
                             viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height, image_width, sh, degree,
                             campos, prefiltered, debug, tile: int = 16, pair_store: int = 0, capacity: int = 0,
-                            pair_capacity: int = 0):
+                            pair_capacity: int = 0, slot_capacity: int = 0):
     """RasterizeGaussiansCUDA (rasterize_points.cu:40-134).
 
     Returns (num_rendered, color (3,H,W), others (7,H,W), radii (P,) int32,
@@ -111,8 +112,10 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
     synchronised mid-forward; more instances raise CapacityError (whose
     `needed` is the count).  pair_capacity > 0 sizes the accepted-pair store
     in pairs (0: pair_store per instance); pairs beyond it are recomputed,
-    with the same results.  The backward must be given the same capacity and
-    pair_capacity.
+    with the same results.  slot_capacity > 0 (with capacity > 0) sizes the
+    partial-gradient slots (0: one per instance and quadrant); more raise
+    CapacityError with slots set.  The backward must be given the same
+    capacity and pair_capacity.
     """
     if means3D.ndimension() != 2 or means3D.size(1) != 3:
         raise RuntimeError("means3D must have dimensions (num_points, 3)")
@@ -131,7 +134,7 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
     others = torch.empty((7, H, W), dtype=torch.float32, device=dev)
     radii = torch.empty((P,), dtype=torch.int32, device=dev)  # the preprocess writes every entry
     s = _settings(a["bg"], scale_modifier, a["viewmatrix"], a["projmatrix"], tan_fovx, tan_fovy, H, W, degree,
-                  a["campos"], prefiltered, debug, tile, pair_store, capacity, pair_capacity)
+                  a["campos"], prefiltered, debug, tile, pair_store, capacity, pair_capacity, slot_capacity)
     g = _surfels(P, means3D, a["sh"], a["colors"], a["opacity"], a["scales"], a["rotations"], a["transMat_precomp"])
     arena = _Arena(dev)
     nr = C.c_int(0)
@@ -141,7 +144,7 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
     if rc == _lib.SR_ERR_ALLOC and arena.error is not None:
         raise arena.error
     if rc == _lib.SR_ERR_CAPACITY:
-        raise _lib.CapacityError(nr.value, capacity)
+        raise _lib.CapacityError(nr.value, capacity, slots=nr.value <= capacity)
     _lib.check(rc)
     return nr.value, color, others, radii, arena.get(0), arena.get(1), arena.get(2)
 
@@ -149,7 +152,7 @@ def rasterize_gaussians_fwd(bg, means3D, colors, opacity, scales, rotations, sca
 def rasterize_gaussians_bwd(bg, means3D, radii, colors, scales, rotations, scale_modifier, transMat_precomp,
                             viewmatrix, projmatrix, tan_fovx, tan_fovy, dL_dout_color, dL_dout_others, sh, degree,
                             campos, geomBuffer, R, binningBuffer, imageBuffer, debug, tile: int = 16,
-                            pair_store: int = 0, capacity: int = 0, pair_capacity: int = 0):
+                            pair_store: int = 0, capacity: int = 0, pair_capacity: int = 0, slot_capacity: int = 0):
     """RasterizeGaussiansBackwardCUDA (rasterize_points.cu:137-233).
 
     Returns (dL_dmeans2D, dL_dcolors, dL_dopacity, dL_dmeans3D, dL_dtransMat,
@@ -179,7 +182,7 @@ def rasterize_gaussians_bwd(bg, means3D, radii, colors, scales, rotations, scale
     dL_drotations = torch.empty((P, 4), **opts)
     if P != 0:
         s = _settings(a["bg"], scale_modifier, a["viewmatrix"], a["projmatrix"], tan_fovx, tan_fovy, H, W, degree,
-                      a["campos"], False, debug, tile, pair_store, capacity, pair_capacity)
+                      a["campos"], False, debug, tile, pair_store, capacity, pair_capacity, slot_capacity)
         g = _surfels(P, means3D, a["sh"], a["colors"], torch.empty(0, **opts), a["scales"], a["rotations"],
                      a["transMat_precomp"])
         gr = _lib.SrGrads()
@@ -227,6 +230,7 @@ def mark_visible(means3D, viewmatrix, projmatrix):
 # recomputed with the same results.
 _capacity_hint = {}
 _pair_ratio = {}
+_slot_ratio = {}  # partial-gradient slots per instance, likewise
 _PAIRS_PER_INSTANCE = 8.0  # before any count is known
 
 
@@ -247,11 +251,16 @@ class _RasterizeGaussians(torch.autograd.Function):
         rkey = (means3D.device, tile)
         with torch.cuda.device(means3D.device):
             seen = _lib.pair_count()
+            seen_slots = _lib.slot_count()
         if seen is not None and seen[1] > 0 and rkey in _pair_ratio:
             _pair_ratio[rkey] = seen[0] / seen[1]
+        if seen_slots is not None and seen_slots[1] > 0 and rkey in _slot_ratio:
+            _slot_ratio[rkey] = seen_slots[0] / seen_slots[1]
         while True:
             ratio = _pair_ratio.get(rkey) or _PAIRS_PER_INSTANCE
             pcap = min(int(ratio * cap * 1.125) + 4096, 0x7FFFFFFF) if cap > 0 else 0
+            sratio = _slot_ratio.get(rkey)
+            scap = min(int(sratio * cap * 1.125) + 1024, 0x7FFFFFFF) if cap > 0 and sratio else 0
             try:
                 num_rendered, color, depth, radii, geomBuffer, binningBuffer, imgBuffer = rasterize_gaussians_fwd(
                     raster_settings.bg, means3D, colors_precomp, opacities, scales, rotations,
@@ -259,17 +268,22 @@ class _RasterizeGaussians(torch.autograd.Function):
                     raster_settings.projmatrix, raster_settings.tanfovx, raster_settings.tanfovy,
                     raster_settings.image_height, raster_settings.image_width, sh, raster_settings.sh_degree,
                     raster_settings.campos, raster_settings.prefiltered, raster_settings.debug, tile,
-                    capacity=cap, pair_capacity=pcap)
+                    capacity=cap, pair_capacity=pcap, slot_capacity=scap)
                 break
             except _lib.CapacityError as e:
-                cap = _grown(e.needed)
+                if e.slots:
+                    _slot_ratio[rkey] = 0.0  # this forward: every (instance, quadrant)
+                else:
+                    cap = _grown(e.needed)
         # keep the hint tracking the count: set by an exact or a retried
         # forward, shrunk when the count fell far below it
         if cap != hint or cap == 0 or _grown(num_rendered) * 4 < cap:
             _capacity_hint[key] = _grown(num_rendered)
-        # from now on the pair store follows the measured pairs per instance
+        # from now on the pair store and the slots follow the measured counts
         _pair_ratio.setdefault(rkey, 0.0)
+        _slot_ratio.setdefault(rkey, 0.0)
         ctx.pair_capacity = pcap
+        ctx.slot_capacity = scap
         ctx.raster_settings = raster_settings
         ctx.capacity = cap
         ctx.num_rendered = num_rendered
@@ -296,7 +310,7 @@ class _RasterizeGaussians(torch.autograd.Function):
             rs.bg, means3D, radii, colors_precomp, scales, rotations, rs.scale_modifier, cov3Ds_precomp,
             rs.viewmatrix, rs.projmatrix, rs.tanfovx, rs.tanfovy, grad_out_color, grad_depth, sh, rs.sh_degree,
             rs.campos, geomBuffer, num_rendered, binningBuffer, imgBuffer, rs.debug, getattr(rs, "tile", 16),
-            capacity=ctx.capacity, pair_capacity=ctx.pair_capacity)
+            capacity=ctx.capacity, pair_capacity=ctx.pair_capacity, slot_capacity=ctx.slot_capacity)
         return (grad_means3D, grad_means2D, grad_sh, grad_colors_precomp, grad_opacities, grad_scales,
                 grad_rotations, grad_cov3Ds_precomp, None)
 

@@ -14,7 +14,7 @@ REC_QUADS = 5
 
 
 def gpu_forward(scene, *, tile=16, scale_modifier=1.0, colors=None, transmat=None, prefiltered=False, debug=False,
-                pair_store=0, capacity=0, pair_capacity=0):
+                pair_store=0, capacity=0, pair_capacity=0, slot_capacity=0):
     """Runs the HIP forward on a SurfelScene; returns a dict of outputs and state."""
     dev = torch.device("cuda")
     cam = scene.camera
@@ -29,10 +29,11 @@ def gpu_forward(scene, *, tile=16, scale_modifier=1.0, colors=None, transmat=Non
             math.tan(cam.FoVy * 0.5), cam.image_height, cam.image_width, shs, scene.sh_degree,
             cam.camera_center.to(dev), prefiltered, debug, tile)
     L, color, others, radii, geom, binb, img = rasterize_gaussians_fwd(*args, pair_store=pair_store,
-                                                                       capacity=capacity, pair_capacity=pair_capacity)
+                                                                       capacity=capacity, pair_capacity=pair_capacity,
+                                                                       slot_capacity=slot_capacity)
     torch.cuda.synchronize()
     out = dict(L=L, color=color, others=others, radii=radii, geom=geom, bin=binb, img=img, args=args, tile=tile,
-               pair_store=pair_store, capacity=capacity, pair_capacity=pair_capacity)
+               pair_store=pair_store, capacity=capacity, pair_capacity=pair_capacity, slot_capacity=slot_capacity)
     return out
 
 

@@ -42,9 +42,12 @@ def test_layout_queries_without_gpu():
     lib = _lib.load()
     assert lib.sr_geom_bytes(1000) > 1000 * 80
     assert lib.sr_image_bytes(800, 600, 16) > 800 * 600 * 20
-    assert lib.sr_binning_bytes(5000, 800, 600, 16, 0, 0) > 5000 * 20
+    assert lib.sr_binning_bytes(5000, 800, 600, 16, 0, 0, 0) > 5000 * 20
     # the pair store follows pair_capacity when one is given
-    assert lib.sr_binning_bytes(5000, 800, 600, 16, 0, 1000) < lib.sr_binning_bytes(5000, 800, 600, 16, 0, 0)
+    assert lib.sr_binning_bytes(5000, 800, 600, 16, 0, 1000, 0) < lib.sr_binning_bytes(5000, 800, 600, 16, 0, 0, 0)
+    # and the partial-gradient slots follow slot_capacity (80 bytes each)
+    full = lib.sr_binning_bytes(5000, 800, 600, 16, 0, 1000, 0)
+    assert full - lib.sr_binning_bytes(5000, 800, 600, 16, 0, 1000, 6000) >= (4 * 5000 - 6000) * 80 - 256
     off = _lib.state_offset("point_list", 1000, 800, 600, 16, 5000)
     assert off % 256 == 0
     with pytest.raises(KeyError):

@@ -128,6 +128,28 @@ def test_capacity_overflow_is_reported_and_recovered():
     assert torch.equal(br.split(color, 3)[0], ref["color"])
 
 
+def test_slot_overflow_is_reported_and_recovered():
+    """A batch whose partial-gradient slots outgrow the slot capacity sets
+    the capacity flag with the instances within the capacity and reports the
+    slots needed; forward_checked then re-sizes the slots and renders the
+    same images as a batch sized from the start."""
+    s, cams = _setup(2)
+    print("[batch] slot overflow", flush=True)
+    br = BatchRasterizer(cams, s.bg, s.sh_degree)
+    ref_color, _, _ = br.forward_checked(s.means3D, s.shs, None, s.opacities, s.scales, s.rotations)
+    need, _ = br.read_status()
+    slots = br.read_slots()
+    assert need <= slots <= 4 * need
+    br.ensure_workspace(s.P, br.capacity, br.pair_capacity, slots // 2)
+    out = br.new_outputs(s.P)
+    br.forward_raw(s.means3D, s.shs, None, s.opacities, s.scales, s.rotations, *out)
+    n2, flags = br.read_status()
+    assert flags & _lib.SR_STATUS_CAPACITY and n2 == need and br.read_slots() == slots
+    color, _, _ = br.forward_checked(s.means3D, s.shs, None, s.opacities, s.scales, s.rotations)
+    assert br.slot_capacity >= slots
+    assert torch.equal(color, ref_color)
+
+
 def test_graph_replay_equals_eager():
     """Forward + backward captured once, replayed three times with eager work
     and input changes between replays; every replay is checked against the

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 import torch
 
+from partgs_amd import _lib
 from partgs_amd.scenes import superquadric_surfels, surfel_cloud, upstream_grads
 from tests.helpers import (assert_close_scaled, gpu_backward, gpu_forward, oracle_run, read_state, rel_err)
 
@@ -112,6 +113,67 @@ def test_recompute_path_gives_stored_path_bits():
         g = gpu_backward(res, gc, go)
         for k in g:
             np.testing.assert_array_equal(g[k], full[k], err_msg=f"{k} pair_capacity={cap}")
+
+
+def test_compact_slots_give_the_same_bits():
+    """The partial-gradient slots sized from the measured count (and from
+    the exact count when no capacity is given) hold the same sums in the
+    same order as one slot per (instance, quadrant): bitwise the same
+    gradients.  A slot capacity below the count is reported as a capacity
+    error with the instances within the capacity."""
+    scene = superquadric_surfels(4, P=20000, width=160, height=120)
+    H, W = scene.camera.image_height, scene.camera.image_width
+    gc, go = upstream_grads(4, H, W)
+    exact = gpu_forward(scene)
+    g_exact = gpu_backward(exact, gc, go)
+    L = exact["L"]
+    counts = _lib.slot_count()
+    assert counts is not None and counts[1] == L
+    S = counts[0]
+    assert L <= S <= 4 * L
+    print(f"[parity] slots {S} for {L} instances", flush=True)
+    full_f = gpu_forward(scene, capacity=L + 100)
+    full = gpu_backward(full_f, gc, go)
+    tight = gpu_forward(scene, capacity=L + 100, slot_capacity=S)
+    assert tight["bin"].numel() < full_f["bin"].numel()
+    g_tight = gpu_backward(tight, gc, go)
+    for k in g_exact:
+        np.testing.assert_array_equal(g_exact[k], full[k], err_msg=k)
+        np.testing.assert_array_equal(g_exact[k], g_tight[k], err_msg=k)
+    with pytest.raises(_lib.CapacityError) as e:
+        gpu_forward(scene, capacity=L + 100, slot_capacity=S - 1)
+    assert e.value.slots and e.value.needed == L
+
+
+def test_autograd_recovers_from_a_slot_overflow(monkeypatch):
+    """The autograd wrapper sizes the slots from the last forward's ratio;
+    when a view needs more it renders again with one slot per (instance,
+    quadrant), with the same results."""
+    from partgs_amd import rasterizer as R
+
+    scene = superquadric_surfels(5, P=20000, width=160, height=120)
+    cam = scene.camera
+    s = scene.to("cuda")
+    rs = R.GaussianRasterizationSettings(image_height=cam.image_height, image_width=cam.image_width,
+                                         tanfovx=math.tan(cam.FoVx * 0.5), tanfovy=math.tan(cam.FoVy * 0.5),
+                                         bg=s.bg, scale_modifier=1.0, viewmatrix=s.camera.world_view_transform,
+                                         projmatrix=s.camera.full_proj_transform, sh_degree=s.sh_degree,
+                                         campos=s.camera.camera_center, prefiltered=False, debug=False)
+    rast = R.GaussianRasterizer(rs)
+    outs = []
+    for step in range(3):
+        if step == 2:  # report a slot count far below what this view needs
+            monkeypatch.setattr(_lib, "slot_count", lambda: (1, 1000))
+        leaves = [t.clone().requires_grad_(True) for t in (s.means3D, s.shs, s.opacities, s.scales, s.rotations)]
+        m2 = torch.zeros_like(s.means3D, requires_grad=True)
+        color, radii, allmap = rast(means3D=leaves[0], means2D=m2, shs=leaves[1], opacities=leaves[2],
+                                    scales=leaves[3], rotations=leaves[4])
+        (color.sum() + allmap.sum()).backward()
+        torch.cuda.synchronize()
+        outs.append((color.detach().cpu(), leaves[0].grad.cpu()))
+    for c, g in outs[1:]:
+        assert torch.equal(c, outs[0][0])
+        assert torch.equal(g, outs[0][1])
 
 
 @pytest.mark.parametrize("tile", [8, 32])
