@@ -124,12 +124,13 @@ class BatchRasterizer:
 
     def _auto_pairs(self, capacity: int, pairs: int | None = None) -> int:
         """Pair-store size for a capacity: 0 (pair_store per instance) when
-        the config fixes pair_store, else the measured pair count with an
-        eighth of headroom, or 8 pairs per instance before any count."""
+        the config fixes pair_store, else the given pair count (the caller
+        scales a measured count by its headroom), or 9 pairs per instance
+        before any count."""
         if self.config.pair_store > 0:
             return 0
-        n = pairs if pairs is not None else 8 * capacity
-        return min(int(n * 1.125) + 4096, 0x7FFFFFFF)
+        n = pairs if pairs is not None else 9 * capacity
+        return min(int(n) + 4096, 0x7FFFFFFF)
 
     def ensure_workspace(self, P: int, capacity: int | None = None, pair_capacity: int | None = None,
                          slot_capacity: int | None = None) -> None:

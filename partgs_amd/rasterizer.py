@@ -226,8 +226,8 @@ def mark_visible(means3D, viewmatrix, projmatrix):
 # once and only fall back to a larger capacity when the count outgrows it.
 # The accepted-pair store follows the pairs per instance of the latest
 # completed forward on the device with the same tile (read back without
-# waiting), with an eighth of headroom; pairs beyond the store are
-# recomputed with the same results.
+# waiting) times the instance capacity, so it shares the capacity's eighth
+# of headroom; pairs beyond the store are recomputed with the same results.
 _capacity_hint = {}
 _pair_ratio = {}
 _slot_ratio = {}  # partial-gradient slots per instance, likewise
@@ -258,9 +258,11 @@ class _RasterizeGaussians(torch.autograd.Function):
             _slot_ratio[rkey] = seen_slots[0] / seen_slots[1]
         while True:
             ratio = _pair_ratio.get(rkey) or _PAIRS_PER_INSTANCE
-            pcap = min(int(ratio * cap * 1.125) + 4096, 0x7FFFFFFF) if cap > 0 else 0
+            # the capacity already holds an eighth more instances than the
+            # last count: pairs and slots per instance scale with it
+            pcap = min(int(ratio * cap) + 4096, 0x7FFFFFFF) if cap > 0 else 0
             sratio = _slot_ratio.get(rkey)
-            scap = min(int(sratio * cap * 1.125) + 1024, 0x7FFFFFFF) if cap > 0 and sratio else 0
+            scap = min(int(sratio * cap) + 1024, 0x7FFFFFFF) if cap > 0 and sratio else 0
             try:
                 num_rendered, color, depth, radii, geomBuffer, binningBuffer, imgBuffer = rasterize_gaussians_fwd(
                     raster_settings.bg, means3D, colors_precomp, opacities, scales, rotations,
