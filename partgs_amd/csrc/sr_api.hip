@@ -399,8 +399,7 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st, bool p
   sc.cullbox = (const short4*)(pl.geom + pl.gl.cullbox);
   sc.roff = (uint32_t*)(pl.geom + pl.gl.roff);
   sc.rslot = (uint2*)(pl.geom + pl.gl.rslot);
-  sc.islot = (uint32_t*)(bin + bl.islot);
-  sc.inst_rec = (uint32_t*)(bin + bl.inst_rec);
+  sc.inst_rs = (uint2*)(bin + bl.inst_rs);
   sc.tile_fill = (uint32_t*)(pl.img + pl.il.tile_fill);
   sc.keys = (uint64_t*)(bin + bl.keys);
   SR_STAGE(2, dbg, st, "scatter", launch_scatter(sc, st));
@@ -416,9 +415,10 @@ int run_binning(Pipeline& pl, const int* radii, bool dbg, hipStream_t st, bool p
   sa.side_huge = pl.side_huge ? 1 : 0;
   sa.point_list = (uint32_t*)(bin + bl.point_list);
   sa.cullbox = (const short4*)(pl.geom + pl.gl.cullbox);
-  sa.inst_rec = (const uint32_t*)(bin + bl.inst_rec);
-  sa.inst_at = (uint32_t*)(bin + bl.inst_at);
-  sa.inst_quads = (uint16_t*)(bin + bl.inst_quads);
+  sa.inst_rs = (const uint2*)(bin + bl.inst_rs);
+  sa.pos_slot = (uint32_t*)(bin + bl.pos_slot);
+  sa.pos_quads = (uint16_t*)(bin + bl.pos_quads);
+  sa.lslot = (uint32_t*)(bin + bl.lslot);
   sa.sub = (uint2*)(bin + bl.sub);
   sa.sub_count = (uint32_t*)(pl.img + pl.il.sub_count);
   sa.work = (uint2*)(bin + bl.work);
@@ -501,9 +501,8 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
     ba.n_contrib = (const uint32_t*)(pl.img + pl.il.n_contrib);
     ba.dL_dpix = dL_dcolor;
     ba.dL_dothers = dL_dothers;
-    ba.inst_at = (const uint32_t*)(pl.bin + pl.bl.inst_at);
-    ba.islot = (const uint32_t*)(pl.bin + pl.bl.islot);
-    ba.inst_quads = (const uint16_t*)(pl.bin + pl.bl.inst_quads);
+    ba.pos_slot = (const uint32_t*)(pl.bin + pl.bl.pos_slot);
+    ba.pos_quads = (const uint16_t*)(pl.bin + pl.bl.pos_quads);
     ba.part = (float*)(pl.bin + pl.bl.part);
     ba.part_nz = (uint8_t*)(pl.bin + pl.bl.part_nz);
     SR_STAGE(6, dbg, st, "blend_bwd", launch_blend_bwd(ba, st));
@@ -519,7 +518,6 @@ int run_backward(Pipeline& pl, const sr_surfels* g, const int* radii, const floa
   pb.tiles = (const uint32_t*)(pl.geom + pl.gl.tiles);
   pb.roff = (const uint32_t*)(pl.geom + pl.gl.roff);
   pb.rslot = (const uint2*)(pl.geom + pl.gl.rslot);
-  pb.inst_quads = blend ? (const uint16_t*)(pl.bin + pl.bl.inst_quads) : nullptr;
   pb.part = blend ? (const float*)(pl.bin + pl.bl.part) : nullptr;
   pb.part_nz = blend ? (const uint8_t*)(pl.bin + pl.bl.part_nz) : nullptr;
   pb.means3D = g->means3D;
@@ -906,9 +904,8 @@ int sr_backward_part(const sr_settings* s, const sr_surfels* g, int S, const flo
     pa.dL_dpix = dL_dcolor;
     pa.dL_dsem = dL_dsemantic;
     pa.dL_dothers = dL_dothers;
-    pa.inst_at = (const uint32_t*)(pl.bin + pl.bl.inst_at);
-    pa.islot = (const uint32_t*)(pl.bin + pl.bl.islot);
-    pa.inst_quads = (const uint16_t*)(pl.bin + pl.bl.inst_quads);
+    pa.pos_slot = (const uint32_t*)(pl.bin + pl.bl.pos_slot);
+    pa.pos_quads = (const uint16_t*)(pl.bin + pl.bl.pos_quads);
     pa.part = (float*)(pl.bin + pl.bl.part);
     pa.sem_part = (float*)(pl.bin + pl.bl.sem_part);
     pa.part_nz = (uint8_t*)(pl.bin + pl.bl.part_nz);
@@ -923,7 +920,6 @@ int sr_backward_part(const sr_settings* s, const sr_surfels* g, int S, const flo
   pb.tiles = (const uint32_t*)(pl.geom + pl.gl.tiles);
   pb.roff = (const uint32_t*)(pl.geom + pl.gl.roff);
   pb.rslot = (const uint2*)(pl.geom + pl.gl.rslot);
-  pb.inst_quads = blend ? (const uint16_t*)(pl.bin + pl.bl.inst_quads) : nullptr;
   pb.part = blend ? (const float*)(pl.bin + pl.bl.part) : nullptr;
   pb.part_nz = blend ? (const uint8_t*)(pl.bin + pl.bl.part_nz) : nullptr;
   pb.sem_part = blend ? (const float*)(pl.bin + pl.bl.sem_part) : nullptr;

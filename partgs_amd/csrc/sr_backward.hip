@@ -49,7 +49,7 @@ static_assert(kGradGroup >= 1 && 64 % kGradGroup == 0, "groups tile the 64 pixel
 // stores its sums when any is non-zero, so neither buffer needs clearing.
 template <class G>
 SR_DEV void store_part(const BlendBwdArgs& a, uint32_t pos, int q, G g) {
-  const size_t slot = slot_of(a.islot, a.inst_quads, a.inst_at[pos], q);
+  const size_t slot = slot_of(a.pos_slot, a.pos_quads, pos, q);
   bool nz = false;
 #pragma unroll
   for (int k = 0; k < 18; ++k) nz |= g(k) != 0.0f;

@@ -689,7 +689,8 @@ hipError_t launch_scan_tiles(int T, const uint32_t* count, uint32_t* start, uint
 // Instances are numbered in record order (record r's k-th tile, row-major
 // in its rectangle, is instance roff[r] + k) and the key's low word is the
 // instance number, which orders equal depths by record exactly as the
-// reference's (tile | depth, record) sort does; inst_rec maps it back.
+// reference's (tile | depth, record) sort does; inst_rs maps it back to the
+// record (and to the instance's first partial-gradient slot).
 __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   __shared__ uint32_t s_cnt[kHistTiles];
   __shared__ uint32_t s_base[kHistTiles];
@@ -729,9 +730,8 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     for (int ty = y0; ty < y1; ++ty) {
       const uint32_t rows = (uint32_t)quad_span(bx.y, bx.w, ty * QX, ty * QX + QX, vi.H);
       for (int tx = x0; tx < x1; ++tx) {
-        a.islot[i] = sl;
+        a.inst_rs[i++] = make_uint2((uint32_t)r, sl);
         sl += rows * (uint32_t)quad_span(bx.x, bx.z, tx * QX, tx * QX + QX, vi.W);
-        a.inst_rec[i++] = (uint32_t)r;
       }
     }
   }
@@ -910,11 +910,13 @@ SR_DEV uint64_t scan256_excl_u64(uint64_t v, uint64_t* s_w, uint64_t& total) {
 // lane tid takes the K consecutive sorted entries tid K .. tid K + K - 1,
 // looks up their records and cull boxes (all in flight at once), and one
 // packed scan per four quadrants numbers every entry's hits.
-// kWhole (c0 == beg, cn == n): writes point_list, inst_at and the quadrant
-// sub-lists, their lengths and the acceptance work items.  Otherwise (one
-// chunk of a larger tile): writes the sorted keys back in place, and per
-// entry (record, quadrant mask) to lmeta and, per quadrant, the number of
-// the chunk's earlier entries that hit it to lpre, for k_merge_chunks.
+// kWhole (c0 == beg, cn == n): writes point_list, each position's first
+// slot and quadrant mask (pos_slot, pos_quads) and the quadrant sub-lists,
+// their lengths and the acceptance work items.  Otherwise (one chunk of a
+// larger tile): writes the sorted keys back in place, and per entry
+// (record, quadrant mask) to lmeta, its first slot to lslot and, per
+// quadrant, the number of the chunk's earlier entries that hit it to lpre,
+// for k_merge_chunks.
 constexpr int kSortThreads = 256;
 
 // Called by one whole wave; lane q < Q holds the length of the tile's
@@ -953,7 +955,7 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
   const int ltile = tile - (int)vi.tile_base;
   const int tx = ltile % vi.gx, ty = ltile / vi.gx;
   const int tid = threadIdx.x;
-  uint32_t inst[K], rid[K], hm[K];  // instance number: the key's low half
+  uint32_t inst[K], rid[K], sl[K], hm[K];  // instance number: the key's low half
   // the gathers are unconditional (entries past the run read the run's
   // first entry) so each round has all K loads in flight
 #pragma unroll
@@ -962,7 +964,11 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
     inst[k] = (uint32_t)s_keys[lds_slot<K>(i < cn ? i : 0)];
   }
 #pragma unroll
-  for (int k = 0; k < K; ++k) rid[k] = a.inst_rec[inst[k]];
+  for (int k = 0; k < K; ++k) {
+    const uint2 rs = a.inst_rs[inst[k]];
+    rid[k] = rs.x;
+    sl[k] = rs.y;
+  }
   short4 bx[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) bx[k] = a.cullbox[rid[k]];
@@ -980,9 +986,10 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
     if (i < cn) {
       if constexpr (kWhole) {
         a.point_list[c0 + i] = rid[k];
-        a.inst_at[c0 + i] = inst[k];
+        a.pos_slot[c0 + i] = sl[k];
       } else {
         a.keys[c0 + i] = s_keys[lds_slot<K>(i)];
+        a.lslot[c0 + i] = sl[k];
       }
     }
   }
@@ -991,9 +998,10 @@ SR_DEV void place_sorted(const SortArgs& a, int tile, uint32_t beg, int n, uint3
     const int i = tid * K + k;
     hm[k] = quad_hits(bx[k], tx, ty, B, QX, Q, vi);
     if (i < cn) {
-      // for the gradient slots: the instance's quadrants
-      a.inst_quads[inst[k]] = (uint16_t)hm[k];
-      if constexpr (!kWhole) a.lmeta[c0 + i] = make_uint2(rid[k], hm[k]);
+      // for the gradient slots: the instance's quadrants (a chunk's go to
+      // their final positions with the merge)
+      if constexpr (kWhole) a.pos_quads[c0 + i] = (uint16_t)hm[k];
+      else a.lmeta[c0 + i] = make_uint2(rid[k], hm[k]);
     }
   }
   uint2* base = a.sub + (size_t)Q * beg;
@@ -1177,7 +1185,8 @@ SR_DEV void merge_chunk(const SortArgs& a, uint32_t beg, int n, int c, const uin
     if (!ok[r]) continue;
     const uint2 m = a.lmeta[c0 + tid + r * kMergeThreads];
     a.point_list[beg + e[r]] = m.x;
-    a.inst_at[beg + e[r]] = (uint32_t)x[r];
+    a.pos_slot[beg + e[r]] = a.lslot[c0 + tid + r * kMergeThreads];
+    a.pos_quads[beg + e[r]] = (uint16_t)m.y;
 #pragma unroll
     for (int q = 0; q < kMaxQuads; ++q)
       if (q < Q && ((m.y >> q) & 1u)) base[(size_t)q * n + rq[r][q]] = make_uint2(m.x, e[r]);
@@ -1255,9 +1264,9 @@ SR_DEV void sort_huge_tile(const SortArgs& a, int tile, uint64_t* s_keys) {
     dst = t;
   }
   for (int i = tid; i < n; i += NT) {
-    const uint32_t inst = (uint32_t)src[i];
-    a.inst_at[beg + i] = inst;
-    a.point_list[beg + i] = a.inst_rec[inst];
+    const uint2 rs = a.inst_rs[(uint32_t)src[i]];
+    a.point_list[beg + i] = rs.x;
+    a.pos_slot[beg + i] = rs.y;
   }
   const int B = a.vs.B, QX = B / kQuad, Q = QX * QX;
   const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
@@ -1275,7 +1284,6 @@ SR_DEV void sort_huge_tile(const SortArgs& a, int tile, uint64_t* s_keys) {
     const int e = c + tid;
     const bool valid = e < n;
     const uint32_t id = id_next;
-    const uint32_t inst = valid ? a.inst_at[beg + e] : 0u;
     const short4 bx = bx_next;
     if (e + NT < n) {
       id_next = a.point_list[beg + e + NT];
@@ -1289,7 +1297,7 @@ SR_DEV void sort_huge_tile(const SortArgs& a, int tile, uint64_t* s_keys) {
         if (lane == 0) s_wcnt[q][wave] = (uint32_t)__popcll(m);
       }
     }
-    if (valid) a.inst_quads[inst] = (uint16_t)hm;
+    if (valid) a.pos_quads[beg + e] = (uint16_t)hm;
     __syncthreads();
     if (tid < Q * kWaves) {  // per quadrant, the waves' first slots
       const int q = tid / kWaves, w = tid % kWaves;

@@ -152,10 +152,12 @@ inline ImageLayout image_layout(int W, int H, int B) { return image_layout(singl
 // Binning state: one entry per (surfel, tile) instance.  Instances are
 // numbered in record order: record r's k-th tile (row-major in its tile
 // rectangle) is instance roff[r] + k.
-//   inst_rec   instance -> record
-//   inst_at    position in the concatenated sorted tile lists -> instance
-//   inst_quads per instance, the quadrants whose sub-lists hold it
-//   islot      per instance, its first partial-gradient slot
+//   inst_rs    instance -> (record, its first partial-gradient slot)
+//   pos_slot   position in the concatenated sorted tile lists -> the first
+//              slot of the instance there
+//   pos_quads  position -> the quadrants whose sub-lists hold the instance
+//   lslot      large-tile sort: per entry of a sorted chunk, its first slot
+//              (the merge moves it to pos_slot)
 //   part       one slot per sub-list entry, i.e. per (instance, quadrant
 //              holding it): the 18 per-surfel partial gradients the
 //              backward blend summed over that quadrant's pixels
@@ -167,7 +169,7 @@ inline ImageLayout image_layout(int W, int H, int B) { return image_layout(singl
 //              size (slot capacity) does not move any other offset.
 struct BinLayout {
   size_t keys, lmeta, lpre, point_list, sub, masks, chunk_T, work, frag_off, total;
-  size_t inst_rec, inst_at, inst_quads, islot, part, part_nz;
+  size_t inst_rs, pos_slot, pos_quads, lslot, part, part_nz;
   size_t nslot;  // slots the part region holds
   size_t prec, fW;
   size_t frag_cap;
@@ -225,10 +227,10 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0, int pair_ca
   b.chunk_T = off; off += align_up(nslots * 4);  // per mask slot: product of (1 - alpha) over its pairs
   b.work = off; off += align_up(work_capacity(L, B, T) * 8);
   b.frag_off = off; off += align_up(nslots * 4);
-  b.inst_rec = off; off += align_up(n * 4);
-  b.inst_at = off; off += align_up(n * 4);
-  b.inst_quads = off; off += align_up(n * 2);
-  b.islot = off; off += align_up(n * 4);
+  b.inst_rs = off; off += align_up(n * 8);
+  b.pos_slot = off; off += align_up(n * 4);
+  b.pos_quads = off; off += align_up(n * 2);
+  b.lslot = off; off += align_up(n * 4);
   b.nslot = slot_capacity > 0 ? (size_t)slot_capacity : n * Q;
   b.part_nz = off; off += align_up(n * Q);  // per slot: 1 when the slot holds sums
   b.frag_cap = pair_capacity > 0 ? (size_t)pair_capacity
@@ -237,7 +239,7 @@ inline BinLayout bin_layout(int L, int B, int T, int pair_store = 0, int pair_ca
   b.prec = off; off += align_up(b.frag_cap * kPairQuads * 16);
   b.fW = off; off += align_up(b.frag_cap * 16);
   b.sem_part = off; off += align_up(n * Q * (size_t)sem * 4);  // _part: semantic partials per slot
-  b.part = off; off += align_up(b.nslot * kPartStride * 4);         // last: see islot
+  b.part = off; off += align_up(b.nslot * kPartStride * 4);         // last: see part
   b.total = off;
   return b;
 }
@@ -291,8 +293,7 @@ struct ScatterArgs {
   const short4* cullbox;
   uint32_t* roff;
   uint2* rslot;    // per record: first slot, slots
-  uint32_t* islot;  // per instance: first slot
-  uint32_t* inst_rec;
+  uint2* inst_rs;  // per instance: record, first slot
   uint32_t* tile_fill;
   uint64_t* keys;
 };
@@ -317,9 +318,10 @@ struct SortArgs {
   uint16_t* lpre;
   uint32_t* point_list;
   const short4* cullbox;
-  const uint32_t* inst_rec;
-  uint32_t* inst_at;
-  uint16_t* inst_quads;
+  const uint2* inst_rs;
+  uint32_t* pos_slot;
+  uint16_t* pos_quads;
+  uint32_t* lslot;
   uint2* sub;
   uint32_t* sub_count;
   uint2* work;          // (quadrant index, chunk) items for the acceptance pass
@@ -392,9 +394,8 @@ struct BlendBwdArgs {
   const uint32_t* n_contrib;
   const float* dL_dpix;
   const float* dL_dothers;
-  const uint32_t* inst_at;
-  const uint32_t* islot;
-  const uint16_t* inst_quads;
+  const uint32_t* pos_slot;    // slot_of
+  const uint16_t* pos_quads;
   float* part;        // partial gradients per slot (slot_of)
   uint8_t* part_nz;   // 1 where that slot was stored (any non-zero sum)
 };
@@ -410,7 +411,6 @@ struct PreprocessBwdArgs {
   const uint32_t* counters;
   const uint32_t* tiles;
   const uint32_t* roff;
-  const uint16_t* inst_quads;
   const uint2* rslot;  // per record: first slot, slots
   const float* part;
   const uint8_t* part_nz;
@@ -464,9 +464,8 @@ struct PartArgs {
   const float* dL_dpix;
   const float* dL_dsem;
   const float* dL_dothers;
-  const uint32_t* inst_at;
-  const uint32_t* islot;
-  const uint16_t* inst_quads;
+  const uint32_t* pos_slot;    // slot_of
+  const uint16_t* pos_quads;
   float* part;
   float* sem_part;           // S per slot
   uint8_t* part_nz;
@@ -509,8 +508,8 @@ __host__ __device__ inline int quad_span(int b0, int b1, int c0, int c1, int W) 
 
 // Partial-gradient slot of an instance's entry in sub-list q of its tile:
 // the instance's first slot plus the quadrants before q that hold it.
-__device__ inline size_t slot_of(const uint32_t* islot, const uint16_t* inst_quads, uint32_t inst, int q) {
-  return (size_t)islot[inst] + (size_t)__popc((uint32_t)inst_quads[inst] & ((1u << q) - 1u));
+__device__ inline size_t slot_of(const uint32_t* pos_slot, const uint16_t* pos_quads, uint32_t pos, int q) {
+  return (size_t)pos_slot[pos] + (size_t)__popc((uint32_t)pos_quads[pos] & ((1u << q) - 1u));
 }
 
 // Launchers (return hipGetLastError()).

@@ -336,7 +336,7 @@ __global__ __launch_bounds__(64) void k_part_blend_bwd(PartArgs a) {
     }
     __syncthreads();
     if (lane < m) {
-      const size_t slot = slot_of(a.islot, a.inst_quads, a.inst_at[d.rbeg + s_pos[lane]], d.q);
+      const size_t slot = slot_of(a.pos_slot, a.pos_quads, d.rbeg + s_pos[lane], d.q);
       bool nz = false;
       for (int k = 0; k < NG; ++k) nz |= s_acc[k][lane] != 0.0f;
       a.part_nz[slot] = nz ? 1 : 0;
