@@ -17,6 +17,43 @@ def _c2w_rot(cam):
     return cam.world_view_transform.double()[:3, :3].numpy()  # = (W2C rot)^T = C2W rot
 
 
+# An FP64 numpy restatement of the post-processing, written from the pinhole
+# model rather than from the projection matrices the torch code inverts; the
+# plane and reprojection tests below pin it alongside the torch version.
+def _np_points(cam, depth):
+    """World point of every pixel (x, y) at view depth d (H, W): the camera
+    centre plus d R_c2w ((x - W/2) / fx, (y - H/2) / fy, 1)
+    (utils/point_utils.py:4-20)."""
+    W, H = cam.image_width, cam.image_height
+    fx = W / (2 * math.tan(cam.FoVx / 2))
+    fy = H / (2 * math.tan(cam.FoVy / 2))
+    gy, gx = np.mgrid[0:H, 0:W].astype(np.float64)
+    rays = np.stack([(gx - W / 2) / fx, (gy - H / 2) / fy, np.ones_like(gx)], -1) @ _c2w_rot(cam).T
+    return depth[..., None] * rays + cam.camera_center.double().cpu().numpy()
+
+
+def _np_depth_to_normal(cam, depth):
+    """utils/point_utils.py:22-33: normalize(d_rows x d_cols) of the points, border 0."""
+    p = _np_points(cam, depth)
+    out = np.zeros_like(p)
+    n = np.cross(p[2:, 1:-1] - p[:-2, 1:-1], p[1:-1, 2:] - p[1:-1, :-2])
+    out[1:-1, 1:-1] = n / np.maximum(np.linalg.norm(n, axis=-1, keepdims=True), 1e-12)
+    return out
+
+
+def _np_postprocess(cam, allmap, depth_ratio):
+    """renderer/gaussian_renderer/__init__.py:104-149 on a (7, H, W) FP64 allmap."""
+    alpha = allmap[1]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        expected = allmap[0] / alpha
+    expected = np.nan_to_num(expected, nan=0.0, posinf=0.0)
+    median = np.nan_to_num(allmap[5], nan=0.0, posinf=0.0)
+    surf = expected * (1 - depth_ratio) + median * depth_ratio
+    return {"rend_alpha": alpha, "rend_normal": np.einsum("ij,jhw->ihw", _c2w_rot(cam), allmap[2:5]),
+            "rend_dist": allmap[6], "surf_depth": surf,
+            "surf_normal": _np_depth_to_normal(cam, surf).transpose(2, 0, 1) * alpha}
+
+
 def test_depths_to_points_reproject_to_pixels():
     cam = random_sphere_camera(4, 3.0, 40, 30, fov_deg=50.0)
     rng = np.random.default_rng(0)
@@ -34,6 +71,14 @@ def test_depths_to_points_reproject_to_pixels():
     assert np.allclose(py.numpy(), gy.reshape(-1), atol=2e-3)
     # the depth is the view-space z of the point
     assert np.allclose(view[:, 2].numpy(), depth.reshape(-1).double().numpy(), rtol=1e-5)
+    # the FP64 restatement lands on the same pixels
+    npts = _np_points(cam, depth[0].double().numpy()).reshape(-1, 3)
+    assert np.abs(npts - pts.numpy()).max() < 1e-4
+    clip = np.concatenate([npts, np.ones((npts.shape[0], 1))], 1) @ cam.full_proj_transform.double().numpy()
+    # (the projection matrix is stored in FP32: a wrong principal point or
+    # focal length would be off by tenths of a pixel)
+    assert np.allclose(clip[:, 0] / clip[:, 3] * W / 2 + W / 2, gx.reshape(-1), atol=1e-4)
+    assert np.allclose(clip[:, 1] / clip[:, 3] * H / 2 + H / 2, gy.reshape(-1), atol=1e-4)
 
 
 @pytest.mark.parametrize("normal_cam", [(0.0, 0.0, -1.0), (0.3, -0.2, -0.9), (-0.5, 0.4, -0.6)])
@@ -58,6 +103,9 @@ def test_depth_to_normal_of_a_plane(normal_cam):
     want = _c2w_rot(cam) @ n  # view -> world of a column vector
     inner = out[1:-1, 1:-1].reshape(-1, 3)
     assert np.allclose(inner, want[None], atol=2e-3), np.abs(inner - want).max()
+    np_out = _np_depth_to_normal(cam, d)
+    assert np.abs(np_out[1:-1, 1:-1].reshape(-1, 3) - want[None]).max() < 1e-6  # R from FP32 entries
+    assert (np_out[0] == 0).all() and (np_out[:, -1] == 0).all()
     assert (out[0] == 0).all() and (out[-1] == 0).all() and (out[:, 0] == 0).all() and (out[:, -1] == 0).all()
 
 
@@ -133,3 +181,38 @@ def test_render_matches_direct_rasteriser_call():
     for t in (pc.get_xyz, pc.get_opacity, pc.get_scaling, pc.get_rotation, pc.get_features):
         assert t.grad is not None and torch.isfinite(t.grad).all()
     assert out["viewspace_points"].grad is not None
+
+
+@pytest.mark.gpu
+def test_render_maps_match_the_oracle():
+    """render() on the GPU against the oracle: the image, alpha and distortion
+    maps equal the oracle's rasteriser outputs bit for bit, and the world
+    normal, surface depth and surface normal maps match the FP64 numpy
+    restatement of the post-processing on the oracle's allmap (pinned by the
+    plane and reprojection tests) to FP32 rounding."""
+    from partgs_amd.renderer import render
+    from partgs_amd.scenes import superquadric_surfels
+    from tests.helpers import oracle_run
+
+    scene = superquadric_surfels(6, P=20000, n_sq=4, width=160, height=120, sh_degree=2)
+    dev = torch.device("cuda")
+    cam = scene.camera.to(dev)
+    pc = _Model(scene, dev)
+    ratio = 0.3
+    out = render(cam, pc, SimpleNamespace(compute_cov3D_python=False, depth_ratio=ratio), scene.bg.to(dev))
+    o, rc = oracle_run(scene)
+    assert rc == 0
+    color, others = o.get("out_color"), o.get("out_others")
+    print(f"[render] oracle P={scene.P} L={o.num_rendered} covered={(others[1] > 0).mean():.3f}", flush=True)
+    got = {k: v.detach().cpu().numpy() for k, v in out.items() if torch.is_tensor(v)}
+    np.testing.assert_array_equal(got["render"], color)
+    np.testing.assert_array_equal(got["rend_alpha"][0], others[1])
+    np.testing.assert_array_equal(got["rend_dist"][0], others[6])
+    np.testing.assert_array_equal(got["radii"], o.get("radii"))
+    want = _np_postprocess(scene.camera, others.astype(np.float64), ratio)
+    assert np.abs(got["rend_normal"] - want["rend_normal"]).max() < 1e-5
+    sd = want["surf_depth"]
+    assert np.abs(got["surf_depth"][0] - sd).max() <= 1e-5 * max(np.abs(sd).max(), 1.0)
+    err = np.abs(got["surf_normal"] - want["surf_normal"]).max(axis=0)
+    print(f"[render] surf_normal max err {err.max():.2e}, p99 {np.percentile(err, 99):.2e}", flush=True)
+    assert np.percentile(err, 99) < 1e-3 and err.max() < 2e-2
