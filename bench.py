@@ -85,6 +85,7 @@ def cpu_baseline(scene, gc, go, budget_s=20.0):
               shs=scene.shs, sh_degree=scene.sh_degree, scales=scene.scales, rotations=scene.rotations)
     gcn, gon = gc.numpy(), go.numpy()
     times = []
+    o = None
     t_start = time.perf_counter()
     while len(times) < 5 and (time.perf_counter() - t_start) < budget_s:
         o = O.Oracle()
@@ -93,11 +94,34 @@ def cpu_baseline(scene, gc, go, budget_s=20.0):
         o.backward(gcn, gon)
         times.append((time.perf_counter() - t0) * 1e3)
         print(f"[bench] cpu oracle fwd+bwd run {len(times)}: {times[-1]:.1f} ms", file=sys.stderr, flush=True)
-    return {"value": statistics.median(times), "unit": "ms/view", "cores": O.get_threads(), "kind": "port",
+    base = {"value": statistics.median(times), "unit": "ms/view", "cores": O.get_threads(), "kind": "port",
             "sample": f"{len(times)} full fwd+bwd views of the same 300k-surfel 800x600 workload (median)",
             "build": "g++ -O3 -march=native -fopenmp" if native else "g++ -O3 -mavx2 -mfma -fopenmp",
             "note": "the C++ restatement of the reference's per-pixel logic: OpenMP over tiles, per-thread "
                     "FP64 gradient buffers, per-tile parallel sort; a reported baseline, not the target"}
+    return base, o
+
+
+GRAD_TOL = 1e-5
+
+
+def oracle_parity(o, color, allmap, radii, grads):
+    """The GPU step against the oracle run of the same workload: images,
+    auxiliary maps and radii bit for bit, every gradient within GRAD_TOL of
+    the oracle's FP64-accumulated sums per element (|a - b| <= tol (|b| +
+    max|b|)).  grads maps the oracle's names to the GPU's arrays."""
+    import numpy as np
+
+    worst = 0.0
+    for name, g in grads.items():
+        ref = np.asarray(o.get(name), dtype=np.float64)
+        a = np.asarray(g, dtype=np.float64).reshape(ref.shape)
+        if ref.size:
+            worst = max(worst, float((np.abs(a - ref) / (np.abs(ref) + np.abs(ref).max() + 1e-300)).max()))
+    images = bool(np.array_equal(color, o.get("out_color")) and np.array_equal(allmap, o.get("out_others")))
+    radii_ok = bool(np.array_equal(radii, o.get("radii")))
+    return {"images_bitwise": images, "radii_bitwise": radii_ok, "grad_max_scaled_err": worst,
+            "grad_tol": GRAD_TOL, "ok": images and radii_ok and worst <= GRAD_TOL}
 
 
 def run_batched_groups(args, scene, cams, cfg):
@@ -304,6 +328,7 @@ def main():
                                     scales=leaves[3], rotations=leaves[4])
         torch.autograd.backward((color, allmap), (gc, go))
         info["radii"] = radii
+        info["color"], info["allmap"] = color, allmap
 
     for _ in range(args.warmup):
         step()
@@ -357,7 +382,16 @@ def main():
     b_pipe = pipeline_bytes(P, M, L, N, T)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene_cpu, gc_cpu, go_cpu)
+        # the last timed step's outputs and gradients, checked against the
+        # oracle run of the same workload that the baseline times
+        last = {"color": info["color"].detach().cpu().numpy(), "allmap": info["allmap"].detach().cpu().numpy(),
+                "radii": info["radii"].cpu().numpy(),
+                "grads": {name: t.grad.cpu().numpy() for t, name in
+                          zip(leaves + [means2D], ("dL_dmeans3D", "dL_dsh", "dL_dopacity", "dL_dscales",
+                                                   "dL_drotations", "dL_dmeans2D"))}}
+        cpu, o = cpu_baseline(scene_cpu, gc_cpu, go_cpu)
+        cpu["parity"] = oracle_parity(o, last["color"], last["allmap"], last["radii"], last["grads"])
+        print(f"[bench] parity against the oracle: {cpu['parity']}", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,
         "value": round(ms_per_step, 4),
