@@ -117,9 +117,10 @@ struct ChainBwd {
   float last_n0 = 0.0f, last_n1 = 0.0f, last_n2 = 0.0f;
   float acc_n0 = 0.0f, acc_n1 = 0.0f, acc_n2 = 0.0f;
   float last_dL_dT = 0.0f, last_alpha = 0.0f;
-  // nc = (normal.xyz, r); m_d, dmd_dd = distortion weight and its derivative
-  SR_DEV float4 step(const PixelBwd& p, float alpha, float c_d, float m_d, float dmd_dd, uint32_t contributor,
-                     const float4& nc, float g, float bl) {
+  // nc = (normal.xyz, r); m_d, m2 = m_d * m_d, dmd_dd = distortion weight,
+  // its square and its derivative
+  SR_DEV float4 step(const PixelBwd& p, float alpha, float c_d, float m_d, float m2, float dmd_dd,
+                     uint32_t contributor, const float4& nc, float g, float bl) {
     T = T / (1.0f - alpha);
     const float w = alpha * T;
     float dL_dalpha = 0.0f;
@@ -134,7 +135,7 @@ struct ChainBwd {
     dL_dalpha += (bl - acc_r2) * p.dLp2;
     float dL_dz = 0.0f, dL_dweight = 0.0f;
     if ((int)contributor == p.median_contributor - 1) dL_dz += p.dL_dmedian;
-    dL_dweight += ((p.final_D2 + (m_d * m_d) * p.final_A) - (2.0f * m_d) * p.final_D) * p.dL_dreg;
+    dL_dweight += ((p.final_D2 + m2 * p.final_A) - (2.0f * m_d) * p.final_D) * p.dL_dreg;
     dL_dalpha += dL_dweight - last_dL_dT;
     last_dL_dT = dL_dweight * alpha + (1.0f - alpha) * last_dL_dT;
     const float dL_dmd = ((2.0f * (T * alpha)) * (m_d * p.final_A - p.final_D)) * p.dL_dreg;
@@ -161,7 +162,8 @@ struct ChainBwd {
   }
   // The state updates of step() alone, with the same operations in the same
   // order (so a chain resumed from a saved state continues bit for bit).
-  SR_DEV void advance(const PixelBwd& p, float alpha, float c_d, float m_d, const float4& nc, float g, float bl) {
+  SR_DEV void advance(const PixelBwd& p, float alpha, float c_d, float m_d, float m2, const float4& nc, float g,
+                      float bl) {
     T = T / (1.0f - alpha);
     acc_r0 = last_alpha * last_c0 + (1.0f - last_alpha) * acc_r0;
     last_c0 = nc.w;
@@ -170,7 +172,7 @@ struct ChainBwd {
     acc_r2 = last_alpha * last_c2 + (1.0f - last_alpha) * acc_r2;
     last_c2 = bl;
     float dL_dweight = 0.0f;
-    dL_dweight += ((p.final_D2 + (m_d * m_d) * p.final_A) - (2.0f * m_d) * p.final_D) * p.dL_dreg;
+    dL_dweight += ((p.final_D2 + m2 * p.final_A) - (2.0f * m_d) * p.final_D) * p.dL_dreg;
     last_dL_dT = dL_dweight * alpha + (1.0f - alpha) * last_dL_dT;
     accum_depth_rec = last_alpha * last_depth + (1.0f - last_alpha) * accum_depth_rec;
     last_depth = c_d;
@@ -203,13 +205,13 @@ struct SegMap {
   float b[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};  // r, g, b, depth, n0, n1, n2, alpha
   // one pair (alpha, distortion weight m_d) after the pair behind it (la,
   // values l: r, g, b, depth, n0, n1, n2)
-  SR_DEV void add(const PixelBwd& p, float la, const float* l, float alpha, float m_d) {
+  SR_DEV void add(const PixelBwd& p, float la, const float* l, float alpha, float m_d, float m2) {
     const float om = 1.0f - la;
 #pragma unroll
     for (int x = 0; x < 7; ++x) b[x] = la * l[x] + om * b[x];
     b[7] = la + om * b[7];
     A = om * A;
-    const float dw = ((p.final_D2 + (m_d * m_d) * p.final_A) - (2.0f * m_d) * p.final_D) * p.dL_dreg;
+    const float dw = ((p.final_D2 + m2 * p.final_A) - (2.0f * m_d) * p.final_D) * p.dL_dreg;
     const float oa = 1.0f - alpha;
     b2 = dw * alpha + oa * b2;
     A2 = oa * A2;
@@ -398,11 +400,11 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
     if (sg < nseg - 1) {
       const float4* pr = a.ps.rec + kPairQuads * (size_t)(f0 + hi + 1);
       const float4 e0 = pr[0], e1 = pr[1], e2 = pr[2];
-      la = e0.x;
+      la = e0.w;
       l[0] = e1.w;
       l[1] = e2.x;
       l[2] = e2.y;
-      l[3] = e0.y;
+      l[3] = e0.x;
       l[4] = e1.x;
       l[5] = e1.y;
       l[6] = e1.z;
@@ -414,12 +416,12 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
 #pragma unroll
       for (int x = 0; x < 7; ++x) pl[x] = l[x];
       walk(lo, hi, [&](int, const float4& c0, const float4& c1, const float4& c2) {
-        m.add(px, pa, pl, c0.x, c0.z);
-        pa = c0.x;
+        m.add(px, pa, pl, c0.w, c0.y, c0.z);
+        pa = c0.w;
         pl[0] = c1.w;
         pl[1] = c2.x;
         pl[2] = c2.y;
-        pl[3] = c0.y;
+        pl[3] = c0.x;
         pl[4] = c1.x;
         pl[5] = c1.y;
         pl[6] = c1.z;
@@ -443,7 +445,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
       ch.last_n1 = l[5];
       ch.last_n2 = l[6];
       walk(lo, hi, [&](int k, const float4& c0, const float4& c1, const float4& c2) {
-        a.ps.fW[f0 + k] = ch.step(px, c0.x, c0.y, c0.z, c0.w, __float_as_uint(c2.z), c1, c2.x, c2.y);
+        a.ps.fW[f0 + k] = ch.step(px, c0.w, c0.x, c0.y, c0.z, c2.w, __float_as_uint(c2.z), c1, c2.x, c2.y);
       });
     }
     return;
@@ -526,9 +528,9 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendBwdArgs a) {
             if (seg >= 1) {
               float l[7];
               chain_last(ch, l);
-              seg_map.add(px, ch.last_alpha, l, e.alpha, m_d);
+              seg_map.add(px, ch.last_alpha, l, e.alpha, m_d, m_d * m_d);
             }
-            const float4 g = ch.step(px, e.alpha, c_d, m_d, dmd_dd, contributor, q3, q4.x, q4.y);
+            const float4 g = ch.step(px, e.alpha, c_d, m_d, m_d * m_d, dmd_dd, contributor, q3, q4.x, q4.y);
             if (idx == seg_start && seg >= 1) {  // the next pair starts segment seg - 1
               seg_map.apply(fold);
               ch.T = fold.T;

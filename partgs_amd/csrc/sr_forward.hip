@@ -1824,9 +1824,9 @@ __global__ __launch_bounds__(64) void k_pair_write(AcceptArgs a) {
       const float m_d = mcoef * (1.0f - kNear / e.depth);
       const float dmd_dd = dmd_num / ((dmd_den0 * e.depth) * e.depth);
       float4* pr = a.ps.rec + kPairQuads * (size_t)(s_foff[p] + k);
-      pr[0] = make_float4(e.alpha, e.depth, m_d, dmd_dd);
+      pr[0] = make_float4(e.depth, m_d, m_d * m_d, e.alpha);
       pr[1] = q3;
-      pr[2] = make_float4(q4.x, q4.y, __uint_as_float(s_pos[j]), 0.0f);
+      pr[2] = make_float4(q4.x, q4.y, __uint_as_float(s_pos[j]), dmd_dd);
     }
   }
 }
@@ -1873,13 +1873,14 @@ SR_DEV void blend_fwd_stored(const BlendArgs& a, const ViewInfo& vi, int qi, int
   // lane_range is written by the pair-offset pass, which runs only when the
   // view has instances: read it for non-empty quadrants only
   const uint2 rg = (inside && cnt > 0) ? a.lane_range[(size_t)qi * 64 + pix] : make_uint2(0u, 0u);
-  // Each lane loads one record quad per pair: role 0 quad 0 (alpha, depth,
-  // mm), role 1 quad 1 (normal, red), roles 2 and 3 quad 2 (green, blue,
-  // tile position); alpha and depth reach the other lanes of the pixel from
-  // role 0 by a quad broadcast.
+  // Each lane loads one record quad per pair: role 0 quad 0 (depth, mm,
+  // mm^2, alpha), role 1 quad 1 (normal, red), roles 2 and 3 quad 2 (green,
+  // blue, tile position); alpha and depth reach the other lanes of the pixel
+  // from role 0 by a quad broadcast.  A lane accumulates x, y and z of its
+  // quad in a0..a2 and w in a3, except that role 0's a3 is the distortion.
   const int qsel = role < 2 ? role : 2;
   float T = 1.0f;
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, distortion = 0.0f, median_depth = 0.0f;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, median_depth = 0.0f;
   float median_contributor = -1.0f;
   uint32_t last_contributor = 0;
   int blended = 0;
@@ -1909,9 +1910,9 @@ SR_DEV void blend_fwd_stored(const BlendArgs& a, const ViewInfo& vi, int qi, int
     // saturate the pixel, so the body has no per-pair branch.
 #pragma unroll
     for (int k = 0; k < kPre; ++k) {
-      // forward.cu:383-415: (alpha, depth, mm) in quad 0; mm = mcoef (1 - near / depth)
+      // forward.cu:383-415; role 0's quad: (depth, mm, mm^2, alpha), mm = mcoef (1 - near / depth)
       const float4 v = cq[k];
-      const float alpha = quad_bcast0(v.x), depth = quad_bcast0(v.y), mm = v.z;
+      const float alpha = quad_bcast0(v.w), depth = quad_bcast0(v.x);
       const float test_T = T * (1.0f - alpha);
       const bool live = k < n && !done;
       const bool sat = test_T < 0.0001f;
@@ -1920,13 +1921,9 @@ SR_DEV void blend_fwd_stored(const BlendArgs& a, const ViewInfo& vi, int qi, int
       const uint32_t contributor = __float_as_uint(v.z) + 1u;  // quad 2's z: position in the tile list
       const float w = alpha * T;
       const float A = 1.0f - T;
-      const float mm2 = mm * mm;
-      const float dist = distortion + ((mm2 * A + a2) - (2.0f * mm) * a1) * w;  // role 0: a1 = M1, a2 = M2
-      const float x0 = role == 0 ? depth : v.x;
-      const float x1 = role == 0 ? mm : v.y;
-      const float x2 = role == 0 ? mm2 : v.z;
-      const float n0 = a0 + x0 * w, n1 = a1 + x1 * w, n2 = a2 + x2 * w, n3 = a3 + v.w * w;
-      distortion = ok ? dist : distortion;
+      // role 0: the distortion term (mm^2 A + M2 - 2 mm M1), a1 = M1, a2 = M2
+      const float x3 = role == 0 ? (v.z * A + a2) - (2.0f * v.y) * a1 : v.w;
+      const float n0 = a0 + v.x * w, n1 = a1 + v.y * w, n2 = a2 + v.z * w, n3 = a3 + x3 * w;
       a0 = ok ? n0 : a0;
       a1 = ok ? n1 : a1;
       a2 = ok ? n2 : a2;
@@ -1948,7 +1945,7 @@ SR_DEV void blend_fwd_stored(const BlendArgs& a, const ViewInfo& vi, int qi, int
     out_others[pid] = a0;
     final_T[pid + HW] = a1;
     final_T[pid + 2 * HW] = a2;
-    out_others[pid + 6 * HW] = distortion;
+    out_others[pid + 6 * HW] = a3;  // the distortion
   } else if (role == 1) {
     out_others[pid + 2 * HW] = a0;
     out_others[pid + 3 * HW] = a1;
@@ -2070,8 +2067,8 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
         float alpha, depth;
         if (foff != kNoFrag) {
           const float4 p0 = a.ps.rec[kPairQuads * (size_t)(f++)];
-          alpha = p0.x;
-          depth = p0.y;
+          alpha = p0.w;
+          depth = p0.x;
         } else {
           Pair e;
           if (!eval_pair<false>(pixx, pixy, s_q0[j], s_q1[j], s_q2[j], 0.0f, e)) continue;

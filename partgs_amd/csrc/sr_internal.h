@@ -179,9 +179,11 @@ constexpr int kPartStride = 20;  // 18 partial gradients, padded to five float4
 // Accepted-pair store: per accepted (pixel, entry) pair the pair-write pass
 // keeps one 48-byte record with everything the per-pixel blend chains read,
 // so they stream it without staging records:
-//   p0 = (alpha, depth, m, dm/ddepth)   m = distortion weight of the depth
+//   p0 = (depth, m, m^2, alpha)   m = distortion weight of the depth
 //   p1 = (normal.xyz, r)
-//   p2 = (g, b, position of the entry in the tile list (bits), 0)
+//   p2 = (g, b, position of the entry in the tile list (bits), dm/ddepth)
+// (p0, p1 and p2 are what the forward blend's lanes of roles 0, 1 and 2
+// accumulate, in their accumulators' order)
 // A lane's consecutive pairs are consecutive records, so the fill writes
 // whole lines.  (The per-pair gradient pass recomputes s and G.)  fW = (w, dL/dalpha, dL/ddepth, 0) is written by the backward chain
 // pass for the per-pair gradient pass, which recomputes the rest of the pair.  All pairs of one pixel of a quadrant are
