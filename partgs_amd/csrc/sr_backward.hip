@@ -645,7 +645,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd_grad(BlendBwdArgs a) {
   if (a.counters[1] & kErrCapacity) return;
   const uint32_t nwork = *a.work_count;
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const uint2 item = a.work[w];
+    const uint2 item = a.work[nwork - 1u - w];  // from the end: see work_capacity
     const int qi = (int)item.x, c = (int)item.y;
     if (a.q_flag[qi] != 0u) continue;  // done by the staged path of pass 1
     const int tile = qi / Q, q = qi % Q;

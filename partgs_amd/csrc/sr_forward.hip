@@ -1499,7 +1499,7 @@ __global__ __launch_bounds__(64) void k_accept(AcceptArgs a) {
   if (a.counters[1] & kErrCapacity) return;
   const uint32_t nwork = *a.work_count;
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const uint2 item = a.work[w];
+    const uint2 item = a.work[nwork - 1u - w];  // from the end: see work_capacity
     const int qi = (int)item.x, c = (int)item.y;
     const int tile = qi / Q, q = qi % Q;
     const ViewInfo& vi = a.vs.v[view_of_tile(a.vs, (uint32_t)tile)];
@@ -1768,7 +1768,7 @@ __global__ __launch_bounds__(64) void k_pair_write(AcceptArgs a) {
   const float dmd_num = kFar * kNear, dmd_den0 = kFar - kNear;
   const uint32_t nwork = *a.work_count;
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const uint2 item = a.work[w];
+    const uint2 item = a.work[nwork - 1u - w];  // from the end: see work_capacity
     const int qi = (int)item.x, c = (int)item.y;
     if (a.q_flag[qi] != 0u) continue;
     const int tile = qi / Q, q = qi % Q;

@@ -205,6 +205,11 @@ constexpr uint32_t kNoFrag = 0xFFFFFFFFu;
 __host__ __device__ inline size_t mask_base(int Q, uint32_t beg, uint32_t n, int tile, int q) {
   return (size_t)Q * beg + (size_t)q * n + (size_t)kQuad * kQuad * ((size_t)tile * Q + q);
 }
+// (quadrant, chunk) work items, appended by the tile sorts as they finish;
+// the kernels that walk the list take it from the end, so the items of the
+// large tiles (appended last, by the chunk merge and the huge-tile sort)
+// start first (measured: 4 us less per 300k-surfel 800x600 step, in the
+// passes that follow).
 inline size_t work_capacity(int L, int B, int T) {
   const size_t Q = (size_t)quads_per_tile(B);
   return Q * ((size_t)T + ((size_t)(L > 0 ? L : 1) + 63) / 64);
